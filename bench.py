@@ -1,0 +1,179 @@
+"""Benchmark of the convective-flux RHS on one MI355X.
+
+Workload (one "step" = one RHS evaluation of a whole AMR level): 2D
+single-species Euler (gamma 1.4), FP64, a 2048^2 periodic domain split into
+256 patches of 128^2 with 4-cell halos, four-quadrant Riemann data (seeded)
+with 1e-3 white noise, WCNS5-JS + hybrid HLLC/HLLC-HLL.  Inputs are resident
+in HBM before the timed region.  The batched path (one work-list launch per
+pipeline stage for the whole level) is the headline; the per-patch path (one
+launch sequence per patch, the plain drop-in) is timed beside it, and the
+CPU oracle is timed on a bounded sample of patches on the host cores.
+
+Prints one JSON line (rank 0).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--N", type=int, default=2048)
+    ap.add_argument("--patch", type=int, default=128)
+    ap.add_argument("--scheme", default="WCNS5_JS_HLLC_HLL")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-per-patch", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design only "
+                          "(see the sharding section of the design notes)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from hamers_amd import ConvectiveFluxReconstructor, PatchData, capi
+    from hamers_amd import workloads as W
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = "cuda:0"
+    N, P, g = args.N, args.patch, 4
+    Qi, n_glob, dx = W.four_quadrant_riemann(N=N, seed=1234)
+    patches = W.split_patches(Qi, (P, P), g)
+    neq = 4
+    rec = ConvectiveFluxReconstructor("bench", 2, "SINGLE_SPECIES", args.scheme, species_gamma=(1.4,))
+    pds = []
+    for _, Qp in patches:
+        Qd = torch.from_numpy(Qp).to(dev)
+        F = [torch.empty((neq, P, P + 1), dtype=torch.float64, device=dev),
+             torch.empty((neq, P + 1, P), dtype=torch.float64, device=dev)]
+        S = torch.zeros((neq, P, P), dtype=torch.float64, device=dev)
+        pds.append(PatchData((P, P), (g, g), dx, Qd, F, S))
+    cells = N * N
+    dt = 1.0e-4
+    stream = torch.cuda.current_stream()
+
+    def step_batched():
+        rec.computeConvectiveFluxAndSourceOnPatches(pds, 0.0, dt, 0, stream)
+
+    # Per-patch path: views prebuilt so the loop is one C call per patch.
+    views = [pd.view(neq) for pd in pds]
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    fn = capi.lib.hm_compute_convective_flux_and_source_on_patch
+
+    def step_per_patch():
+        for v in views:
+            st = fn(rec._ctx, ctypes.byref(v), 0.0, dt, 0, sp)
+            if st:
+                capi.check(rec._ctx, st)
+
+    def timed(fn_step, steps, warmup):
+        for _ in range(warmup):
+            fn_step()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn_step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return e0.elapsed_time(e1) / steps, wall / steps * 1e3
+
+    ms_batched, wall_batched = timed(step_batched, args.steps, args.warmup)
+
+    # Stage timing of the batched path (events around each stage launch).
+    rec.enable_profiling(True)
+    for _ in range(args.steps):
+        step_batched()
+    prof = rec.read_profile()
+    rec.enable_profiling(False)
+    stage_avg = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in prof.items()}
+    dominant = max(stage_avg, key=stage_avg.get)
+
+    ms_pp = None
+    if not args.no_per_patch:
+        ms_pp, _ = timed(step_per_patch, max(3, args.steps // 4), 1)
+
+    # Algorithmic (compulsory) HBM bytes per cell per RHS: read Q (neq), write
+    # F on d axes (d*neq): 8*(4 + 2*4) = 96 B for 2D single-species.
+    bytes_per_cell = 8 * (neq + 2 * neq)
+    achieved_gbs = bytes_per_cell * cells / (stage_avg[dominant] * 1e-3) / 1e9
+    rhs_gbs = bytes_per_cell * cells / (ms_batched * 1e-3) / 1e9
+    value = cells / (ms_batched * 1e-3)
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        nthreads = min(16, os.cpu_count() or 1)
+        prm = O.Params(dim=2, scheme=1 if args.scheme.startswith("WCNS6") else 0)
+
+        plist = [{"Q": Qp, "n": (P, P), "g": g, "dx": dx,
+                  "F": [np.empty((neq, P, P + 1)), np.empty((neq, P + 1, P))],
+                  "S": np.zeros((neq, P, P))} for _, Qp in patches]
+        t0 = time.perf_counter()
+        O.rhs_batch(prm, plist, dt, nthreads)        # one pass over the whole level, to size the sample
+        t1 = time.perf_counter() - t0
+        reps = max(1, int(round(args.cpu_seconds / max(t1, 1e-3))))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            O.rhs_batch(prm, plist, dt, nthreads)
+        tk = time.perf_counter() - t0
+        cpu = {"value": reps * cells / tk, "unit": "cell-RHS-evals/s", "cores": nthreads, "kind": "port",
+               "sample": f"the whole 256-patch level, {reps} RHS evaluations ({tk:.1f} s), oracle built -O3 "
+                         f"-fopenmp -ffp-contract=off, {nthreads} OpenMP threads over patches"}
+
+    out = {
+        "metric": "FP64 cell-RHS-evals/s (batched convective RHS, WCNS5-JS + hybrid HLLC/HLLC-HLL)",
+        "value": value,
+        "unit": "cell-RHS-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_batched,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "2D single-species Euler four-quadrant Riemann, 2048^2 in 256 patches of "
+                               "128^2, 4-cell halos, WCNS5_JS_HLLC_HLL, batched RHS", "cells": cells,
+                   "patches": len(pds), "patch": [P, P], "scheme": args.scheme},
+        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_{dominant}", "kernel_ms": stage_avg[dominant],
+                     "bytes_per_cell": bytes_per_cell},
+        "rhs_hbm_frac": rhs_gbs / HBM_PEAK_GBS,
+        "stage_ms": stage_avg,
+        "per_patch": {"ms_per_step": ms_pp, "value": (cells / (ms_pp * 1e-3)) if ms_pp else None,
+                      "batched_speedup": (ms_pp / ms_batched) if ms_pp else None},
+        "host_wall_ms_per_step": wall_batched,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

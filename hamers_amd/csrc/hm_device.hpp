@@ -1,0 +1,123 @@
+// hm_device.hpp -- device-side patch descriptors and work-list addressing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/hamers_convective.h"
+
+namespace hm {
+
+constexpr int kGc = 4;     // convective ghost width (WCNS56-HLLC-HLL.cpp:22)
+constexpr int kGs = 2;     // ghost width of dilatation / vorticity (WCNS56-HLLC-HLL.cpp:116-123)
+constexpr int kBlock = 256;
+
+// Per-patch descriptor as the kernels see it: caller arrays plus this
+// library's scratch for the patch.
+struct DevPatch {
+    int n[3];                     // interior extents (1 on unused axes)
+    int gq[3];                    // ghost width of Q (0 on unused axes)
+    double dx[3];
+    const double* Q[HM_MAX_EQN];  // conservative variables, per equation, over the Q ghost box
+    double* F[3];                 // output face flux per axis (component-major)
+    double* S;                    // output source (component-major interior)
+    double* prim;                 // scratch: V[0..neq) and c over the 4-ghost box
+    double* sens;                 // scratch: theta, Omega over the 2-ghost box
+    double* fmid[3];              // scratch: midpoint flux per axis, neq comps over (n_a+3) x n_b box
+    double* ustar[3];             // scratch: HLLC normal velocity per axis (advective models)
+};
+
+// Extents of the boxes the kernels walk, for a patch of dimension D.
+template <int D>
+struct Boxes {
+    int n[3];
+    __device__ __forceinline__ explicit Boxes(const DevPatch& P) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) n[a] = (a < D) ? P.n[a] : 1;
+    }
+    // Box grown by g on the used axes.
+    __device__ __forceinline__ int ext(int a, int g) const { return (a < D) ? n[a] + 2 * g : 1; }
+    __device__ __forceinline__ int64_t vol(int g) const {
+        return int64_t(ext(0, g)) * ext(1, g) * ext(2, g);
+    }
+    // Linear index of cell (i,j,k) in the box grown by g.
+    __device__ __forceinline__ int64_t idx(int g, int i, int j, int k) const {
+        const int gx = g, gy = (D > 1) ? g : 0, gz = (D > 2) ? g : 0;
+        return (i + gx) + int64_t(j + gy) * ext(0, g) + int64_t(k + gz) * ext(0, g) * ext(1, g);
+    }
+    // Midpoint faces of axis A: i_A in [-1, n_A+1], others interior.
+    __device__ __forceinline__ int mid_ext(int A, int a) const { return (a == A) ? n[a] + 3 : n[a]; }
+    __device__ __forceinline__ int64_t mid_vol(int A) const {
+        return int64_t(mid_ext(A, 0)) * mid_ext(A, 1) * mid_ext(A, 2);
+    }
+    __device__ __forceinline__ int64_t mid_idx(int A, int i, int j, int k) const {
+        const int ci = i + (A == 0), cj = j + (A == 1), ck = k + (A == 2);
+        return ci + int64_t(cj) * mid_ext(A, 0) + int64_t(ck) * mid_ext(A, 0) * mid_ext(A, 1);
+    }
+    // Output faces of axis A: (n_A+1) x n_b.
+    __device__ __forceinline__ int face_ext(int A, int a) const { return (a == A) ? n[a] + 1 : n[a]; }
+    __device__ __forceinline__ int64_t face_vol(int A) const {
+        return int64_t(face_ext(A, 0)) * face_ext(A, 1) * face_ext(A, 2);
+    }
+};
+
+// Q ghost box addressing (caller's ghost width).
+template <int D>
+__device__ __forceinline__ int64_t q_idx(const DevPatch& P, int i, int j, int k) {
+    const int e0 = P.n[0] + 2 * P.gq[0];
+    const int e1 = (D > 1) ? P.n[1] + 2 * P.gq[1] : 1;
+    const int gy = (D > 1) ? P.gq[1] : 0, gz = (D > 2) ? P.gq[2] : 0;
+    return (i + P.gq[0]) + int64_t(j + gy) * e0 + int64_t(k + gz) * e0 * e1;
+}
+
+// A work segment is a contiguous run of items (cells or faces) of one patch
+// and one axis.  Items of all segments are numbered consecutively and every
+// segment starts on a multiple of the wavefront size (kSegAlign), so one
+// wavefront never spans two segments: the per-axis dispatch inside a kernel
+// is uniform per wavefront, while one block may still span several small
+// patches.  Numbers in [start+cnt, next start) are holes.
+constexpr int kSegAlign = 64;
+
+struct SegSingle {  // one patch, up to 3 segments (one per axis), all by value
+    DevPatch p;
+    int64_t start[4];
+    int64_t cnt[3];
+    int nseg;
+    __device__ __forceinline__ bool find(int64_t gid, const DevPatch*& P, int& seg_axis, int64_t& loc) const {
+        if (gid >= start[nseg]) return false;
+        int s = 0;
+        while (s + 1 < nseg && gid >= start[s + 1]) s++;
+        loc = gid - start[s];
+        if (loc >= cnt[s]) return false;
+        P = &p;
+        seg_axis = s;
+        return true;
+    }
+};
+
+struct SegTable {  // many patches: device arrays
+    const DevPatch* patches;
+    const int64_t* start;  // nseg + 1 entries
+    const int64_t* cnt;    // nseg entries
+    const int* seg;        // per segment: patch * 4 + axis
+    int nseg;
+    __device__ __forceinline__ bool find(int64_t gid, const DevPatch*& P, int& seg_axis, int64_t& loc) const {
+        if (gid >= start[nseg]) return false;
+        int lo = 0, hi = nseg - 1;  // last segment with start <= gid
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (start[mid] <= gid)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        loc = gid - start[lo];
+        if (loc >= cnt[lo]) return false;
+        const int pd = seg[lo];
+        P = patches + (pd >> 2);
+        seg_axis = pd & 3;
+        return true;
+    }
+};
+
+}  // namespace hm

@@ -1,0 +1,32 @@
+// hm_kernels.hpp -- launch interface shared by the host layer and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "hm_device.hpp"
+
+namespace hm {
+
+// Scheme constants and per-call values, passed by value to every kernel.
+struct KParams {
+    int p, q;                       // WCNS exponents
+    double C, alpha_tau;            // WCNS6-LD constants
+    double gamma[HM_MAX_SPECIES];   // species ratios of specific heats
+    double R[HM_MAX_SPECIES];       // species gas constants
+    int num_species;
+    double dt;
+};
+
+// Work of one RHS evaluation: one work source per pipeline stage
+// (0 derived cells, 1 sensor cells, 2 midpoint faces, 3 output faces) and the
+// number of work items (padded) each stage launches over.
+template <class W>
+struct Plan {
+    W work[4];
+    int64_t items[4];
+};
+
+template <class W>
+hipError_t launch_rhs_single(int dim, int scheme, const Plan<W>& plan, const KParams& k, hipStream_t s);
+
+}  // namespace hm

@@ -1,0 +1,472 @@
+// hm_kernels_single.hip -- convective-flux RHS kernels, single-species model.
+//
+// Pipeline per RHS evaluation (each stage is one launch over all work items
+// of all patches handed to it):
+//   k_derived   cells of the 4-ghost box: V = (rho, u.., p) and c
+//               (FlowModelSingleSpecies.cpp:2649-3179)
+//   k_sensor    cells of the 2-ghost box: dilatation and vorticity magnitude
+//               (WCNS56-HLLC-HLL.cpp:626-731 (2D), 1470-1656 (3D))
+//   k_midflux   every midpoint face of every axis: characteristic projection,
+//               WCNS interpolation, back-projection, positivity fallback,
+//               HLLC / HLLC-HLL and sensor select, fused in registers
+//               (WCNS56-HLLC-HLL.cpp:879-1194, 1746-2320)
+//   k_faceflux  every face: sixth-order midpoint-to-face flux times dt
+//               (WCNS56-HLLC-HLL.cpp:1197-1282, 2327-2490)
+#include "hm_kernels.hpp"
+#include "hm_numerics.hpp"
+
+namespace hm {
+
+// ----------------------------------------------------------------------------
+// Derived cell data.
+// ----------------------------------------------------------------------------
+template <int D, class W>
+__global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const DevPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    constexpr int NEQ = D + 2;
+    const Boxes<D> B(*P);
+    const int e0 = B.ext(0, kGc), e1 = B.ext(1, kGc);
+    const int i = int(loc % e0) - kGc;
+    const int64_t r = loc / e0;
+    const int j = (D > 1) ? int(r % e1) - kGc : 0;
+    const int kk = (D > 2) ? int(r / e1) - kGc : 0;
+    const int64_t iq = q_idx<D>(*P, i, j, kk);
+    const double gam = k.gamma[0];
+
+    const double rho = P->Q[0][iq];
+    double u[3];
+#pragma unroll
+    for (int b = 0; b < D; b++) u[b] = P->Q[1 + b][iq] / rho;  // FlowModelSingleSpecies.cpp:2719
+    double ke;
+    if constexpr (D == 1)
+        ke = u[0] * u[0];
+    else if constexpr (D == 2)
+        ke = u[0] * u[0] + u[1] * u[1];
+    else
+        ke = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const double E = P->Q[D + 1][iq];
+    const double eps_int = E / rho - double(1) / double(2) * ke;  // :2934
+    const double p = (gam - double(1)) * rho * eps_int;           // EquationOfStateIdealGas.cpp:5477
+    const double c = sqrt(gam * p / rho);                         // EquationOfStateIdealGas.cpp:5806
+
+    const int64_t vol = B.vol(kGc);
+    double* pr = P->prim;
+    pr[loc] = rho;
+#pragma unroll
+    for (int b = 0; b < D; b++) pr[(1 + b) * vol + loc] = u[b];
+    pr[(D + 1) * vol + loc] = p;
+    pr[NEQ * vol + loc] = c;
+}
+
+// ----------------------------------------------------------------------------
+// Dilatation and vorticity magnitude (second-order central derivatives,
+// DerivativeFirstOrder.cpp:216-228).
+// ----------------------------------------------------------------------------
+template <int D, class W>
+__global__ __launch_bounds__(kBlock) void k_sensor(W w, KParams k) {
+    (void)k;
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const DevPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    const Boxes<D> B(*P);
+    const int e0 = B.ext(0, kGs), e1 = B.ext(1, kGs);
+    const int i = int(loc % e0) - kGs;
+    const int64_t r = loc / e0;
+    const int j = int(r % e1) - kGs;
+    const int kk = (D > 2) ? int(r / e1) - kGs : 0;
+    const int64_t v4 = B.vol(kGc);
+    const int64_t c = B.idx(kGc, i, j, kk);
+    const int64_t st[3] = {1, B.ext(0, kGc), int64_t(B.ext(0, kGc)) * B.ext(1, kGc)};
+    // d u_b / d x_a
+    double g[3][3];
+#pragma unroll
+    for (int b = 0; b < D; b++) {
+        const double* ub = P->prim + (1 + b) * v4;
+#pragma unroll
+        for (int a = 0; a < D; a++)
+            g[b][a] = (double(1) / double(2) * (ub[c + st[a]] - ub[c - st[a]])) / P->dx[a];
+    }
+    double theta, Omega;
+    if constexpr (D == 2) {
+        theta = g[0][0] + g[1][1];
+        Omega = fabs(g[1][0] - g[0][1]);
+    } else {
+        theta = g[0][0] + g[1][1] + g[2][2];
+        const double ox = g[2][1] - g[1][2];
+        const double oy = g[0][2] - g[2][0];
+        const double oz = g[1][0] - g[0][1];
+        Omega = sqrt(ox * ox + oy * oy + oz * oz);
+    }
+    const int64_t vs = B.vol(kGs);
+    P->sens[loc] = theta;
+    P->sens[vs + loc] = Omega;
+}
+
+// ----------------------------------------------------------------------------
+// Fused midpoint flux of one face (single-species).
+// ----------------------------------------------------------------------------
+template <int D, int A, int SCHEME, int PP, int QQ>
+__device__ __forceinline__ void midflux_single_face(const DevPatch& P, int64_t loc, const KParams& k) {
+    constexpr int NEQ = D + 2;
+    constexpr int IP = D + 1;  // pressure slot of V
+    const Boxes<D> B(P);
+    const int m0 = B.mid_ext(A, 0), m1 = B.mid_ext(A, 1);
+    int c0 = int(loc % m0);
+    const int64_t r = loc / m0;
+    int c1 = (D > 1) ? int(r % m1) : 0;
+    int c2 = (D > 2) ? int(r / m1) : 0;
+    if constexpr (A == 0) c0 -= 1;
+    if constexpr (A == 1) c1 -= 1;
+    if constexpr (A == 2) c2 -= 1;
+
+    const int64_t v4 = B.vol(kGc);
+    const int64_t st = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGc)) : int64_t(B.ext(0, kGc)) * B.ext(1, kGc);
+    const int64_t iR = B.idx(kGc, c0, c1, c2);  // cell i (right of the face); cell i-1 is iR - st
+    const double* pr = P.prim;
+
+    // Stencil of primitive variables, cells i-3 .. i+2.
+    double V[6][NEQ];
+#pragma unroll
+    for (int m = 0; m < 6; m++)
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) V[m][e] = pr[e * v4 + iR + (m - 3) * st];
+    const double cL = pr[NEQ * v4 + iR - st];
+    const double cR = pr[NEQ * v4 + iR];
+
+    // Projection variables: simple averages (FlowModelBasicUtilitiesSingleSpecies.cpp:4638-5134).
+    const double ra = double(1) / double(2) * (V[2][0] + V[3][0]);
+    const double ca = double(1) / double(2) * (cL + cR);
+
+    // Characteristic fields (FlowModelBasicUtilitiesSingleSpecies.cpp:6194-6242 and 3D analogues),
+    // reconstructed one at a time.
+    double Wm[NEQ], Wp[NEQ];
+    const LDConst ld{k.p, k.q, k.C, k.alpha_tau};
+    {
+        Six s;
+#pragma unroll
+        for (int m = 0; m < 6; m++)
+            s.v[m] = -double(1) / double(2) * ra * ca * V[m][1 + A] + double(1) / double(2) * V[m][IP];
+        reconstruct_pair<SCHEME, PP, QQ>(s, ld, Wm[0], Wp[0]);
+    }
+    {
+        Six s;
+#pragma unroll
+        for (int m = 0; m < 6; m++) s.v[m] = V[m][0] - double(1) / (ca * ca) * V[m][IP];
+        reconstruct_pair<SCHEME, PP, QQ>(s, ld, Wm[1], Wp[1]);
+    }
+    {
+        int t = 2;
+#pragma unroll
+        for (int b = 0; b < D; b++) {
+            if (b == A) continue;
+            Six s;
+#pragma unroll
+            for (int m = 0; m < 6; m++) s.v[m] = V[m][1 + b];
+            reconstruct_pair<SCHEME, PP, QQ>(s, ld, Wm[t], Wp[t]);
+            t++;
+        }
+    }
+    {
+        Six s;
+#pragma unroll
+        for (int m = 0; m < 6; m++)
+            s.v[m] = double(1) / double(2) * ra * ca * V[m][1 + A] + double(1) / double(2) * V[m][IP];
+        reconstruct_pair<SCHEME, PP, QQ>(s, ld, Wm[IP], Wp[IP]);
+    }
+
+    // Back to primitive variables (FlowModelBasicUtilitiesSingleSpecies.cpp:7279-7284 and analogues).
+    double VL[NEQ], VR[NEQ];
+    auto back = [&](const double* W, double* Vo) {
+        Vo[0] = double(1) / (ca * ca) * W[0] + W[1] + double(1) / (ca * ca) * W[IP];
+        Vo[1 + A] = -double(1) / (ra * ca) * W[0] + double(1) / (ra * ca) * W[IP];
+        int t = 2;
+#pragma unroll
+        for (int b = 0; b < D; b++) {
+            if (b == A) continue;
+            Vo[1 + b] = W[t++];
+        }
+        Vo[IP] = W[0] + W[IP];
+    };
+    back(Wm, VL);
+    back(Wp, VR);
+
+    // Positivity of rho and p on both sides, else first-order values
+    // (FlowModelBasicUtilitiesSingleSpecies.cpp:3205-3222, WCNS56-HLLC-HLL.cpp:947-1032).
+    const bool ok = (VL[0] > double(0)) && (VL[IP] > double(0)) && (VR[0] > double(0)) && (VR[IP] > double(0));
+    if (!ok) {
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) {
+            VL[e] = V[2][e];
+            VR[e] = V[3][e];
+        }
+    }
+
+    // Riemann problem at the midpoint.
+    const double gam = k.gamma[0];
+    const double rhoL = VL[0], rhoR = VR[0];
+    const double pL = VL[IP], pR = VR[IP];
+    const double uL = VL[1 + A], uR = VR[1 + A];
+    const double csL = sqrt(gam * pL / rhoL);                // EquationOfStateIdealGas.cpp:5806
+    const double csR = sqrt(gam * pR / rhoR);
+    const double epsL = pL / ((gam - double(1)) * rhoL);    // EquationOfStateIdealGas.cpp:6135
+    const double epsR = pR / ((gam - double(1)) * rhoR);
+
+    // Wave speeds and contact speed (FlowModelRiemannSolverSingleSpeciesHLLC.cpp:619-631).
+    const double u_avg = double(1) / double(2) * (uL + uR);
+    const double c_avg = double(1) / double(2) * (csL + csR);
+    const double sL = fmin(u_avg - c_avg, uL - csL);
+    const double sR = fmax(u_avg + c_avg, uR + csR);
+    const double s_minus = fmin(double(0), sL);
+    const double s_plus = fmax(double(0), sR);
+    const double s_star =
+        (pR - pL + rhoL * uL * (sL - uL) - rhoR * uR * (sR - uR)) / (rhoL * (sL - uL) - rhoR * (sR - uR));
+
+    // Conservative state and physical flux of one side (:637-648).
+    auto cons_flux = [&](const double* Vs, double eps_int, double* Qs, double* Fs) {
+        Qs[0] = Vs[0];
+#pragma unroll
+        for (int b = 0; b < D; b++) Qs[1 + b] = Vs[0] * Vs[1 + b];
+        double ke;
+        if constexpr (D == 1)
+            ke = Vs[1] * Vs[1];
+        else if constexpr (D == 2)
+            ke = Vs[1] * Vs[1] + Vs[2] * Vs[2];
+        else
+            ke = Vs[1] * Vs[1] + Vs[2] * Vs[2] + Vs[3] * Vs[3];
+        Qs[IP] = Vs[0] * (eps_int + double(1) / double(2) * ke);
+        Fs[0] = Qs[1 + A];
+#pragma unroll
+        for (int b = 0; b < D; b++) Fs[1 + b] = (b == A) ? Qs[1 + A] * Vs[1 + b] + Vs[IP] : Qs[1 + A] * Vs[1 + b];
+        Fs[IP] = Vs[1 + A] * (Qs[IP] + Vs[IP]);
+    };
+
+    // Sensor (WCNS56-HLLC-HLL.cpp:1066-1113): only when it exceeds 0.65 is the
+    // HLLC-HLL flux used, so HLLC-HLL is evaluated only on those faces.
+    bool use_hll = false;
+    if constexpr (D > 1) {
+        const int64_t vs = B.vol(kGs);
+        const int64_t tR = B.idx(kGs, c0, c1, c2);
+        const int64_t sts = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGs)) : int64_t(B.ext(0, kGs)) * B.ext(1, kGs);
+        const int64_t tL = tR - sts;
+        const double th_avg = 0.5 * (P.sens[tL] + P.sens[tR]);
+        const double om_avg = 0.5 * (P.sens[vs + tL] + P.sens[vs + tR]);
+        const double sensor = -th_avg / (fabs(th_avg) + om_avg + kEps);
+        use_hll = sensor > 0.65;
+    }
+
+    double Fo[NEQ];
+    double QL[NEQ], QR[NEQ], FL[NEQ], FR[NEQ];
+    const bool left = s_star > double(0);
+    if (use_hll) {
+        cons_flux(VL, epsL, QL, FL);
+        cons_flux(VR, epsR, QR, FR);
+    } else if (left) {
+        cons_flux(VL, epsL, QL, FL);
+    } else {
+        cons_flux(VR, epsR, QR, FR);
+    }
+    {
+        // HLLC (:650-688): star state of the upwind side of the contact.
+        const double* Vs = left ? VL : VR;
+        const double* Qs = left ? QL : QR;
+        const double* Fs = left ? FL : FR;
+        const double sK = left ? sL : sR;
+        const double spm = left ? s_minus : s_plus;
+        const double chi = (sK - Vs[1 + A]) / (sK - s_star);
+        double Qst[NEQ];
+        Qst[0] = chi * Vs[0];
+#pragma unroll
+        for (int b = 0; b < D; b++) Qst[1 + b] = (b == A) ? chi * Vs[0] * s_star : chi * Qs[1 + b];
+        Qst[IP] = chi * (Qs[IP] + (s_star - Vs[1 + A]) * (Vs[0] * s_star + Vs[IP] / (sK - Vs[1 + A])));
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) Fo[e] = Fs[e] + spm * (Qst[e] - Qs[e]);
+    }
+    if constexpr (D > 1) {
+        if (use_hll) {
+            // HLL on mass and transverse momentum, blended by the velocity-jump
+            // angle (FlowModelRiemannSolverSingleSpeciesHLLC-HLL.cpp:961-1022).
+            auto hll = [&](int e) {
+                double h = (sR * FL[e] - sL * FR[e] + sR * sL * (QR[e] - QL[e])) / (sR - sL);
+                if (sL > double(0)) h = FL[e];
+                if (sR < double(0)) h = FR[e];
+                return h;
+            };
+            double dsq;
+            {
+                const double du = VR[1] - VL[1];
+                const double dv = VR[2] - VL[2];
+                if constexpr (D == 2) {
+                    dsq = du * du + dv * dv;
+                } else {
+                    const double dw = VR[3] - VL[3];
+                    dsq = du * du + dv * dv + dw * dw;
+                }
+            }
+            const double vel_mag = sqrt(dsq);
+            double al1, al2;
+            if (vel_mag < kEps) {
+                al1 = double(1);
+                al2 = double(0);
+            } else {
+                al1 = fabs(VR[1 + A] - VL[1 + A]) / vel_mag;
+                al2 = sqrt(double(1) - al1 * al1);
+            }
+            const double be1 = double(1) / double(2) * (double(1) + al1 / (al1 + al2));
+            const double be2 = double(1) - be1;
+            Fo[0] = be1 * Fo[0] + be2 * hll(0);
+#pragma unroll
+            for (int b = 0; b < D; b++)
+                if (b != A) Fo[1 + b] = be1 * Fo[1 + b] + be2 * hll(1 + b);
+        }
+    }
+
+    const int64_t mv = B.mid_vol(A);
+    double* out = P.fmid[A];
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) out[e * mv + loc] = Fo[e];
+}
+
+template <int D, int SCHEME, int PP, int QQ, class W>
+__global__ __launch_bounds__(kBlock) void k_midflux_single(W w, KParams k) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const DevPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    if (ax == 0) {
+        midflux_single_face<D, 0, SCHEME, PP, QQ>(*P, loc, k);
+    } else if constexpr (D > 1) {
+        if (ax == 1) {
+            midflux_single_face<D, 1, SCHEME, PP, QQ>(*P, loc, k);
+        } else if constexpr (D > 2) {
+            midflux_single_face<D, 2, SCHEME, PP, QQ>(*P, loc, k);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Midpoint-to-face flux (sixth order), times dt.
+// ----------------------------------------------------------------------------
+template <int D, int A>
+__device__ __forceinline__ void faceflux_single_face(const DevPatch& P, int64_t loc, const KParams& k) {
+    constexpr int NEQ = D + 2;
+    constexpr int IP = D + 1;
+    const Boxes<D> B(P);
+    const int f0 = B.face_ext(A, 0), f1 = B.face_ext(A, 1);
+    const int c0 = int(loc % f0);
+    const int64_t r = loc / f0;
+    const int c1 = (D > 1) ? int(r % f1) : 0;
+    const int c2 = (D > 2) ? int(r / f1) : 0;
+
+    const int64_t mv = B.mid_vol(A);
+    const int64_t mC = B.mid_idx(A, c0, c1, c2);
+    const int64_t mst = (A == 0) ? 1 : (A == 1) ? int64_t(B.mid_ext(A, 0)) : int64_t(B.mid_ext(A, 0)) * B.mid_ext(A, 1);
+
+    // Node flux at cells i-1 and i (FlowModelSingleSpecies.cpp:3329-3331, 3388-3391, 3470-3474).
+    const int64_t v4 = B.vol(kGc);
+    const int64_t st4 = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGc)) : int64_t(B.ext(0, kGc)) * B.ext(1, kGc);
+    const int64_t pR = B.idx(kGc, c0, c1, c2), pL = pR - st4;
+    const int qst = (A == 0) ? 1 : (A == 1) ? (P.n[0] + 2 * P.gq[0]) : (P.n[0] + 2 * P.gq[0]) * (P.n[1] + 2 * P.gq[1]);
+    const int64_t qR = q_idx<D>(P, c0, c1, c2), qL = qR - qst;
+    const double uaL = P.prim[(1 + A) * v4 + pL], uaR = P.prim[(1 + A) * v4 + pR];
+    const double prL = P.prim[IP * v4 + pL], prR = P.prim[IP * v4 + pR];
+
+    const int64_t fv = B.face_vol(A);
+    const double* fm = P.fmid[A];
+    double* out = P.F[A];
+    const double dt = k.dt;
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        double nL, nR;
+        if (e == 0) {
+            nL = P.Q[1 + A][qL];
+            nR = P.Q[1 + A][qR];
+        } else if (e == IP) {
+            nL = uaL * (P.Q[IP][qL] + prL);
+            nR = uaR * (P.Q[IP][qR] + prR);
+        } else if (e == 1 + A) {
+            nL = uaL * P.Q[e][qL] + prL;
+            nR = uaR * P.Q[e][qR] + prR;
+        } else {
+            nL = uaL * P.Q[e][qL];
+            nR = uaR * P.Q[e][qR];
+        }
+        const double* f = fm + e * mv;
+        const double mR = f[mC + mst], mL = f[mC - mst], mM = f[mC];
+        out[e * fv + loc] = dt * (double(1) / double(30) * (mR + mL) - double(3) / double(10) * (nR + nL) +
+                                  double(23) / double(15) * mM);
+    }
+}
+
+template <int D, class W>
+__global__ __launch_bounds__(kBlock) void k_faceflux_single(W w, KParams k) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const DevPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    if (ax == 0) {
+        faceflux_single_face<D, 0>(*P, loc, k);
+    } else if constexpr (D > 1) {
+        if (ax == 1) {
+            faceflux_single_face<D, 1>(*P, loc, k);
+        } else if constexpr (D > 2) {
+            faceflux_single_face<D, 2>(*P, loc, k);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Host-side launch of the whole single-species pipeline.
+// ----------------------------------------------------------------------------
+namespace {
+
+template <int D, int SCHEME, int PP, int QQ, class W>
+hipError_t launch_single_dsq(const Plan<W>& plan, const KParams& k, hipStream_t s) {
+    if (plan.items[0] > 0)
+        hipLaunchKernelGGL((k_derived_single<D, W>), dim3(unsigned((plan.items[0] + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, s, plan.work[0], k);
+    if constexpr (D > 1) {
+        if (plan.items[1] > 0)
+            hipLaunchKernelGGL((k_sensor<D, W>), dim3(unsigned((plan.items[1] + kBlock - 1) / kBlock)),
+                               dim3(kBlock), 0, s, plan.work[1], k);
+    }
+    if (plan.items[2] > 0)
+        hipLaunchKernelGGL((k_midflux_single<D, SCHEME, PP, QQ, W>),
+                           dim3(unsigned((plan.items[2] + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, plan.work[2], k);
+    if (plan.items[3] > 0)
+        hipLaunchKernelGGL((k_faceflux_single<D, W>), dim3(unsigned((plan.items[3] + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, s, plan.work[3], k);
+    return hipGetLastError();
+}
+
+template <int D, class W>
+hipError_t launch_single_d(const Plan<W>& plan, const KParams& k, int scheme, hipStream_t s) {
+    const bool dflt = (k.p == 2 && k.q == 4);
+    if (scheme == kWCNS6_LD)
+        return dflt ? launch_single_dsq<D, kWCNS6_LD, 2, 4>(plan, k, s) : launch_single_dsq<D, kWCNS6_LD, -1, -1>(plan, k, s);
+    return (k.p == 2) ? launch_single_dsq<D, kWCNS5_JS, 2, 4>(plan, k, s) : launch_single_dsq<D, kWCNS5_JS, -1, -1>(plan, k, s);
+}
+
+}  // namespace
+
+template <class W>
+hipError_t launch_rhs_single(int dim, int scheme, const Plan<W>& plan, const KParams& k, hipStream_t s) {
+    switch (dim) {
+        case 1: return launch_single_d<1>(plan, k, scheme, s);
+        case 2: return launch_single_d<2>(plan, k, scheme, s);
+        case 3: return launch_single_d<3>(plan, k, scheme, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template hipError_t launch_rhs_single<SegSingle>(int, int, const Plan<SegSingle>&, const KParams&, hipStream_t);
+template hipError_t launch_rhs_single<SegTable>(int, int, const Plan<SegTable>&, const KParams&, hipStream_t);
+
+}  // namespace hm

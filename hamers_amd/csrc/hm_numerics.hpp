@@ -1,0 +1,161 @@
+// hm_numerics.hpp -- pointwise numerics of the convective-flux RHS on the GPU.
+//
+// Every expression keeps the association order and reciprocal forms of the
+// reference source lines it cites, so that a build without multiply-add
+// contraction (-ffp-contract=off) rounds exactly as the reference's
+// uncontracted x86-64 build does.  The knife-edge predicates of the scheme
+// (R_tau > alpha_tau, s* > 0, sensor > 0.65, positivity) then take the same
+// branch as the reference on identical inputs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace hm {
+
+constexpr double kEps = 1.0e-15;  // HAMERS_EPSILON, include/HAMeRS_config.hpp.in:16
+
+// ipow of ConvectiveFluxReconstructorWCNS6-LD-HLLC-HLL.cpp:9-18: repeated
+// multiplication, so base^4 is ((b*b)*b)*b.  E < 0 means "exponent known only
+// at run time".
+template <int E>
+__device__ __forceinline__ double rep_pow(double b, int e_rt) {
+    if constexpr (E > 0) {
+        double r = b;
+#pragma unroll
+        for (int i = 1; i < E; i++) r *= b;
+        return r;
+    } else {
+        double r = b;
+        for (int i = 1; i < e_rt; i++) r *= b;
+        return r;
+    }
+}
+
+// The six stencil values of one characteristic field at one face, cells
+// i-3 .. i+2 of the face between cells i-1 and i.
+struct Six {
+    double v[6];
+};
+
+__device__ __forceinline__ Six flip(const Six& s) { return Six{{s.v[5], s.v[4], s.v[3], s.v[2], s.v[1], s.v[0]}}; }
+
+// Smoothness indicators of the three 3-point sub-stencils
+// (WCNS6-LD-HLLC-HLL.cpp:55-68, identical in WCNS5-JS-HLLC-HLL.cpp:32-44).
+__device__ __forceinline__ void beta3(const Six& s, double& b0, double& b1, double& b2) {
+    b0 = double(1) / double(3) *
+         (s.v[0] * (double(4) * s.v[0] - double(19) * s.v[1] + double(11) * s.v[2]) +
+          s.v[1] * (double(25) * s.v[1] - double(31) * s.v[2]) + double(10) * s.v[2] * s.v[2]);
+    b1 = double(1) / double(3) *
+         (s.v[1] * (double(4) * s.v[1] - double(13) * s.v[2] + double(5) * s.v[3]) +
+          double(13) * s.v[2] * (s.v[2] - s.v[3]) + double(4) * s.v[3] * s.v[3]);
+    b2 = double(1) / double(3) *
+         (s.v[2] * (double(10) * s.v[2] - double(31) * s.v[3] + double(11) * s.v[4]) +
+          s.v[3] * (double(25) * s.v[3] - double(19) * s.v[4]) + double(4) * s.v[4] * s.v[4]);
+}
+
+// Six-point smoothness indicator (WCNS6-LD-HLLC-HLL.cpp:70-83).
+__device__ __forceinline__ double beta6(const Six& s) {
+    return double(1) / double(232243200) *
+           (s.v[0] * (double(525910327) * s.v[0] - double(4562164630) * s.v[1] + double(7799501420) * s.v[2] -
+                    double(6610694540) * s.v[3] + double(2794296070) * s.v[4] - double(472758974) * s.v[5]) +
+            double(5) * s.v[1] *
+                (double(2146987907) * s.v[1] - double(7722406988) * s.v[2] + double(6763559276) * s.v[3] -
+                 double(2926461814) * s.v[4] + double(503766638) * s.v[5]) +
+            double(20) * s.v[2] *
+                (double(1833221603) * s.v[2] - double(3358664662) * s.v[3] + double(1495974539) * s.v[4] -
+                 double(263126407) * s.v[5]) +
+            double(20) * s.v[3] * (double(1607794163) * s.v[3] - double(1486026707) * s.v[4] + double(268747951) * s.v[5]) +
+            double(5) * s.v[4] * (double(1432381427) * s.v[4] - double(536951582) * s.v[5]) +
+            double(263126407) * s.v[5] * s.v[5]);
+}
+
+// Shock measure sigma (WCNS6-LD-HLLC-HLL.cpp:24-41); both sides use the
+// unflipped stencil.
+__device__ __forceinline__ double ld_sigma(const Six& s) {
+    const double a1 = s.v[2] - s.v[1];
+    const double a2 = s.v[3] - s.v[2];
+    const double a3 = s.v[4] - s.v[3];
+    const double t1 = fabs(a1 - a2) / (fabs(a1) + fabs(a2) + kEps);
+    const double t2 = fabs(a2 - a3) / (fabs(a2) + fabs(a3) + kEps);
+    return fmax(t1, t2);
+}
+
+struct LDConst {
+    int p, q;
+    double C, alpha_tau;
+};
+
+// One side of the WCNS6-LD nonlinear midpoint interpolation
+// (WCNS6-LD-HLLC-HLL.cpp:133-223; the plus side 229-319 is this on flip(s)).
+template <int P, int Q>
+__device__ __forceinline__ double ld_side(const Six& s, double sigma, const LDConst& k) {
+    double b0, b1, b2;
+    beta3(s, b0, b1, b2);
+    const double b3 = beta6(s);
+
+    const double tau5 = fabs(b0 - b2);
+    double u0 = double(1) / double(16) * (double(1) + rep_pow<P>(tau5 / (b0 + kEps), k.p));
+    double u1 = double(5) / double(8) * (double(1) + rep_pow<P>(tau5 / (b1 + kEps), k.p));
+    double u2 = double(5) / double(16) * (double(1) + rep_pow<P>(tau5 / (b2 + kEps), k.p));
+    const double us = u0 + u1 + u2;
+    u0 = u0 / us;
+    u1 = u1 / us;
+    u2 = u2 / us;
+
+    const double bavg = double(1) / double(8) * (b0 + b2 + double(6) * b1);
+    const double tau6 = fabs(b3 - bavg);
+    double w0 = double(1) / double(32) * (k.C + rep_pow<Q>(tau6 / (b0 + kEps), k.q));
+    double w1 = double(15) / double(32) * (k.C + rep_pow<Q>(tau6 / (b1 + kEps), k.q));
+    double w2 = double(15) / double(32) * (k.C + rep_pow<Q>(tau6 / (b2 + kEps), k.q));
+    double w3 = double(1) / double(32) * (k.C + rep_pow<Q>(tau6 / (b3 + kEps), k.q));
+    const double ws = w0 + w1 + w2 + w3;
+    w0 = w0 / ws;
+    w1 = w1 / ws;
+    w2 = w2 / ws;
+    w3 = w3 / ws;
+
+    if (tau6 / (bavg + kEps) > k.alpha_tau) {
+        w0 = sigma * u0 + (double(1) - sigma) * w0;
+        w1 = sigma * u1 + (double(1) - sigma) * w1;
+        w2 = sigma * u2 + (double(1) - sigma) * w2;
+        w3 = (double(1) - sigma) * w3;
+    }
+    return double(3) / double(8) * w0 * s.v[0] + (-double(10) / double(8) * w0 - double(1) / double(8) * w1) * s.v[1] +
+           (double(15) / double(8) * w0 + double(6) / double(8) * w1 + double(3) / double(8) * w2) * s.v[2] +
+           (double(3) / double(8) * w1 + double(6) / double(8) * w2 + double(15) / double(8) * w3) * s.v[3] +
+           (-double(1) / double(8) * w2 - double(10) / double(8) * w3) * s.v[4] + double(3) / double(8) * w3 * s.v[5];
+}
+
+// One side of WCNS5-JS (WCNS5-JS-HLLC-HLL.cpp:78-118; plus side 124-164 on flip(s)).
+template <int P>
+__device__ __forceinline__ double js_side(const Six& s, int p_rt) {
+    double b0, b1, b2;
+    beta3(s, b0, b1, b2);
+    double w0 = double(1.0) / double(16) / rep_pow<P>((b0 + kEps), p_rt);
+    double w1 = double(5.0) / double(8) / rep_pow<P>((b1 + kEps), p_rt);
+    double w2 = double(5.0) / double(16) / rep_pow<P>((b2 + kEps), p_rt);
+    const double ws = w0 + w1 + w2;
+    w0 = w0 / ws;
+    w1 = w1 / ws;
+    w2 = w2 / ws;
+    return double(3) / double(8) * w0 * s.v[0] + (-double(10) / double(8) * w0 - double(1) / double(8) * w1) * s.v[1] +
+           (double(15) / double(8) * w0 + double(6) / double(8) * w1 + double(3) / double(8) * w2) * s.v[2] +
+           (double(3) / double(8) * w1 + double(6) / double(8) * w2) * s.v[3] - double(1) / double(8) * w2 * s.v[4];
+}
+
+// Scheme selector: both sides of one characteristic field.
+enum { kWCNS5_JS = 0, kWCNS6_LD = 1 };
+
+template <int SCHEME, int P, int Q>
+__device__ __forceinline__ void reconstruct_pair(const Six& s, const LDConst& k, double& um, double& up) {
+    if constexpr (SCHEME == kWCNS6_LD) {
+        const double sig = ld_sigma(s);
+        um = ld_side<P, Q>(s, sig, k);
+        up = ld_side<P, Q>(flip(s), sig, k);
+    } else {
+        um = js_side<P>(s, k.p);
+        up = js_side<P>(flip(s), k.p);
+    }
+}
+
+}  // namespace hm

@@ -1,0 +1,180 @@
+"""Python mirror of the reference's convective-flux reconstructor interface.
+
+Mirrors ``ConvectiveFluxReconstructor`` (include/flow/convective_flux_reconstructors/
+ConvectiveFluxReconstructor.hpp:23-124) and ``ConvectiveFluxReconstructorManager``
+(src/flow/convective_flux_reconstructors/ConvectiveFluxReconstructorManager.cpp:14-153):
+the same deck keys, the same method name and argument meaning, and the same
+output semantics (flux overwritten, source accumulated, both scaled by dt).
+Patch data are torch tensors on the GPU; torch only provides device memory
+and streams, every computation runs in libhamers_amd.so.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+from . import capi
+
+_SCHEMES = {
+    "WCNS5_JS_HLLC_HLL": capi.HM_SCHEME_WCNS5_JS_HLLC_HLL,  # ConvectiveFluxReconstructorManager.cpp:40
+    "WCNS6_LD_HLLC_HLL": capi.HM_SCHEME_WCNS6_LD_HLLC_HLL,  # ConvectiveFluxReconstructorManager.cpp:79
+}
+_MODELS = {
+    "SINGLE_SPECIES": capi.HM_MODEL_SINGLE_SPECIES,
+    "FOUR_EQN_CONSERVATIVE": capi.HM_MODEL_FOUR_EQN_CONSERVATIVE,
+    "FIVE_EQN_ALLAIRE": capi.HM_MODEL_FIVE_EQN_ALLAIRE,
+}
+
+
+def num_equations(model: str, dim: int, num_species: int) -> int:
+    if model == "SINGLE_SPECIES":
+        return dim + 2
+    if model == "FIVE_EQN_ALLAIRE":
+        return dim + 2 * num_species
+    if model == "FOUR_EQN_CONSERVATIVE":
+        return num_species + dim + 1
+    raise ValueError(f"unknown flow model {model!r}")
+
+
+@dataclass
+class PatchData:
+    """The patch data one reconstructor call reads and writes.
+
+    ``Q``: list of per-equation tensors over the ghost box, or one tensor of
+    shape (neq, *ghost box reversed); ``F``: per-axis flux tensors of shape
+    (neq, *face box reversed); ``S``: source tensor (neq, *interior reversed).
+    ``n``, ``ghost`` and ``dx`` are in (x, y, z) order.
+    """
+    n: tuple
+    ghost: tuple
+    dx: tuple
+    Q: object
+    F: list
+    S: object = None
+    _keep: list = field(default_factory=list, repr=False)
+
+    def view(self, neq: int) -> capi.hm_patch_view:
+        v = capi.hm_patch_view()
+        d = len(self.n)
+        for a in range(3):
+            v.n[a] = int(self.n[a]) if a < d else 1
+            v.ghost[a] = int(self.ghost[a]) if a < d else 0
+            v.dx[a] = float(self.dx[a]) if a < d else 1.0
+        qs = list(self.Q) if isinstance(self.Q, (list, tuple)) else [self.Q[e] for e in range(self.Q.shape[0])]
+        if len(qs) < neq:
+            raise ValueError(f"patch has {len(qs)} conservative variables, the model needs {neq}")
+        for e in range(neq):
+            t = qs[e]
+            _check_tensor(t)
+            v.Q[e] = t.data_ptr()
+        for a in range(d):
+            _check_tensor(self.F[a])
+            v.F[a] = self.F[a].data_ptr()
+        if self.S is not None:
+            _check_tensor(self.S)
+            v.S = self.S.data_ptr()
+        return v
+
+
+def _check_tensor(t):
+    import torch
+
+    if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise ValueError("patch arrays must be contiguous float64 tensors on the GPU")
+
+
+class ConvectiveFluxReconstructor:
+    """A WCNS56 hybrid HLLC / HLLC-HLL convective flux reconstructor on the GPU."""
+
+    def __init__(self, object_name: str, dim: int, flow_model: str, scheme: str, num_species: int = 1,
+                 species_gamma=(1.4,), species_R=(1.0,), constant_p: int = 2, constant_q: int = 4,
+                 constant_C: float = 1.0e9, constant_alpha_tau: float = 35.0, device: int = 0):
+        self.object_name = object_name
+        self.dim = dim
+        self.flow_model = flow_model
+        self.scheme = scheme
+        self.num_species = 1 if flow_model == "SINGLE_SPECIES" else num_species
+        self.num_eqn = num_equations(flow_model, dim, self.num_species)
+        cfg = capi.hm_config()
+        cfg.dim = dim
+        cfg.model = _MODELS[flow_model]
+        cfg.num_species = self.num_species
+        cfg.scheme = _SCHEMES[scheme]
+        cfg.constant_p = constant_p
+        cfg.constant_q = constant_q
+        cfg.constant_C = constant_C
+        cfg.constant_alpha_tau = constant_alpha_tau
+        for i, gv in enumerate(species_gamma):
+            cfg.species_gamma[i] = gv
+        for i, rv in enumerate(species_R):
+            cfg.species_R[i] = rv
+        cfg.device = device
+        self._ctx = ctypes.c_void_p()
+        st = capi.lib.hm_ctx_create(ctypes.byref(cfg), ctypes.byref(self._ctx))
+        if st != capi.HM_OK:
+            msg = capi.lib.hm_last_error(self._ctx)
+            capi.lib.hm_ctx_destroy(self._ctx)
+            self._ctx = None
+            raise capi.HamersError(st, msg.decode() if msg else "")
+
+    def __del__(self):
+        if getattr(self, "_ctx", None):
+            capi.lib.hm_ctx_destroy(self._ctx)
+            self._ctx = None
+
+    def getConvectiveFluxNumberOfGhostCells(self) -> int:
+        g = ctypes.c_int()
+        capi.check(self._ctx, capi.lib.hm_get_num_ghosts(self._ctx, ctypes.byref(g)))
+        return g.value
+
+    @staticmethod
+    def _stream_ptr(stream):
+        if stream is None:
+            import torch
+
+            stream = torch.cuda.current_stream()
+        return ctypes.c_void_p(stream.cuda_stream)
+
+    def computeConvectiveFluxAndSourceOnPatch(self, patch: PatchData, time: float, dt: float,
+                                              RK_step_number: int = 0, stream=None):
+        v = patch.view(self.num_eqn)
+        st = capi.lib.hm_compute_convective_flux_and_source_on_patch(
+            self._ctx, ctypes.byref(v), time, dt, RK_step_number, self._stream_ptr(stream))
+        capi.check(self._ctx, st)
+
+    STAGES = ("derived", "sensor", "midflux", "faceflux")
+
+    def enable_profiling(self, on: bool = True):
+        """Time each pipeline stage with HIP events on the launch stream."""
+        capi.check(self._ctx, capi.lib.hm_profile_enable(self._ctx, int(on)))
+
+    def read_profile(self) -> dict:
+        ms = (ctypes.c_double * 4)()
+        n = (ctypes.c_int64 * 4)()
+        capi.check(self._ctx, capi.lib.hm_profile_read(self._ctx, ms, n))
+        return {name: (ms[i], n[i]) for i, name in enumerate(self.STAGES)}
+
+    def computeConvectiveFluxAndSourceOnPatches(self, patches, time: float, dt: float, RK_step_number: int = 0,
+                                                stream=None):
+        arr = (capi.hm_patch_view * len(patches))()
+        for i, p in enumerate(patches):
+            arr[i] = p.view(self.num_eqn)
+        st = capi.lib.hm_compute_convective_flux_and_source_on_patches(
+            self._ctx, arr, len(patches), time, dt, RK_step_number, self._stream_ptr(stream))
+        capi.check(self._ctx, st)
+
+
+class ConvectiveFluxReconstructorManager:
+    """Builds the reconstructor named by the deck key, as
+    ConvectiveFluxReconstructorManager.cpp:14-153 does; unknown keys raise."""
+
+    def __init__(self, object_name: str, dim: int, flow_model: str, convective_flux_reconstructor: str, **db):
+        if convective_flux_reconstructor not in _SCHEMES:
+            raise ValueError(f"Unknown convective_flux_reconstructor string {convective_flux_reconstructor!r}")
+        if flow_model not in _MODELS:
+            raise ValueError(f"Unknown flow_model string {flow_model!r}")
+        self.reconstructor = ConvectiveFluxReconstructor(object_name, dim, flow_model,
+                                                         convective_flux_reconstructor, **db)
+
+    def getConvectiveFluxReconstructor(self) -> ConvectiveFluxReconstructor:
+        return self.reconstructor

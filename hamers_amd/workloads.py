@@ -1,0 +1,137 @@
+"""Synthetic workloads for the convective-flux RHS (numpy, host side).
+
+Arrays are component-major SAMRAI layouts in numpy C order: a d-dimensional
+field with ``ncomp`` components is ``(ncomp, n_{d-1}, ..., n_0)`` (x last).
+
+Initial conditions follow the reference's problem files where one exists
+(``problems/Euler/initial_conditions/ConvergenceSingleSpecies.cpp:77-101,
+150-177``) and otherwise the workload descriptions the benchmark is quoted on.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def ghost_shape(ncomp: int, n: tuple, g: int) -> tuple:
+    return (ncomp,) + tuple(reversed([v + 2 * g for v in n]))
+
+
+def fill_periodic(Qi: np.ndarray, g: int) -> np.ndarray:
+    """Interior array (ncomp, *rev(n)) -> ghosted array with periodic halo."""
+    d = Qi.ndim - 1
+    pad = [(0, 0)] + [(g, g)] * d
+    return np.pad(Qi, pad, mode="wrap")
+
+
+def conservative_from_primitive_single(rho, vel, p, gamma):
+    """(rho, [u, v, w], p) -> (rho, rho*u.., E) for the single-species model
+    (E = p/(gamma-1) + 1/2 rho |u|^2)."""
+    d = len(vel)
+    Q = np.empty((d + 2,) + np.shape(rho))
+    Q[0] = rho
+    ke = np.zeros_like(rho)
+    for b in range(d):
+        Q[1 + b] = rho * vel[b]
+        ke = ke + vel[b] * vel[b]
+    Q[d + 1] = p / (gamma - 1.0) + 0.5 * rho * ke
+    return Q
+
+
+def cell_centres(n: tuple, xlo: tuple, dx: tuple):
+    """Meshgrid of cell-centre coordinates, returned in reversed (numpy) order
+    as a list [x, y, (z)] of arrays of shape rev(n)."""
+    axes = [xlo[a] + (np.arange(n[a]) + 0.5) * dx[a] for a in range(len(n))]
+    grids = np.meshgrid(*reversed(axes), indexing="ij")
+    return list(reversed(grids))
+
+
+def density_wave_convergence(N: int, dim: int = 2, gamma: float = 1.4):
+    """Periodic density wave of the reference's convergence test on [-1,1]^d:
+    rho = 1 + 1/2 sin(pi sum x), u = 1, p = 1
+    (problems/Euler/initial_conditions/ConvergenceSingleSpecies.cpp:77-101)."""
+    n = (N,) * dim
+    dx = (2.0 / N,) * dim
+    X = cell_centres(n, (-1.0,) * dim, dx)
+    s = sum(X)
+    rho = 1.0 + 0.5 * np.sin(np.pi * s)
+    vel = [np.ones_like(rho) for _ in range(dim)]
+    p = np.ones_like(rho)
+    return conservative_from_primitive_single(rho, vel, p, gamma), n, dx
+
+
+def density_wave_unit(N: int = 128, dim: int = 2, gamma: float = 1.4):
+    """Smooth advected density wave on [0,1]^d: rho = 1 + 0.2 sin(2 pi sum x),
+    u = 1, p = 1 (single 128^2 patch workload)."""
+    n = (N,) * dim
+    dx = (1.0 / N,) * dim
+    X = cell_centres(n, (0.0,) * dim, dx)
+    rho = 1.0 + 0.2 * np.sin(2.0 * np.pi * sum(X))
+    vel = [np.ones_like(rho) for _ in range(dim)]
+    p = np.ones_like(rho)
+    return conservative_from_primitive_single(rho, vel, p, gamma), n, dx
+
+
+def four_quadrant_riemann(N: int = 2048, seed: int = 1234, gamma: float = 1.4, noise: float = 1.0e-3):
+    """2D four-quadrant Riemann data on [0,1]^2 split at (1/2,1/2): per quadrant
+    (rho,u,v,p) with rho,p ~ U[0.1,1.5], u,v ~ U[-0.8,0.8]; then additive white
+    noise noise*U[-1,1] on every primitive variable."""
+    rng = np.random.default_rng(seed)
+    quad = {}
+    for q in range(4):
+        rho = rng.uniform(0.1, 1.5)
+        u = rng.uniform(-0.8, 0.8)
+        v = rng.uniform(-0.8, 0.8)
+        p = rng.uniform(0.1, 1.5)
+        quad[q] = (rho, u, v, p)
+    n = (N, N)
+    dx = (1.0 / N, 1.0 / N)
+    X, Y = cell_centres(n, (0.0, 0.0), dx)
+    qi = (X >= 0.5).astype(int) + 2 * (Y >= 0.5).astype(int)
+    prim = np.empty((4, N, N))
+    for q in range(4):
+        for c in range(4):
+            prim[c][qi == q] = quad[q][c]
+    prim += noise * rng.uniform(-1.0, 1.0, size=prim.shape)
+    return conservative_from_primitive_single(prim[0], [prim[1], prim[2]], prim[3], gamma), n, dx
+
+
+def kelvin_helmholtz(N: int = 512, amplitudes=None, seed: int = 7, gamma: float = 1.4):
+    """Periodic Kelvin-Helmholtz double shear layer on [0,1]^2 with 8 sinusoidal
+    transverse-velocity perturbation modes whose amplitudes are the parameters.
+    u = U0 tanh((y-1/4)/delta) - U0 tanh((y-3/4)/delta) - U0 (periodic pair),
+    v = sum_k a_k sin(2 pi k x + phi_k) * envelope(y), rho = 1 (+ 1 inside), p = 2.5."""
+    rng = np.random.default_rng(seed)
+    if amplitudes is None:
+        amplitudes = 1.0e-2 * rng.uniform(0.5, 1.0, size=8)
+    phases = rng.uniform(0.0, 2.0 * np.pi, size=8)
+    n = (N, N)
+    dx = (1.0 / N, 1.0 / N)
+    X, Y = cell_centres(n, (0.0, 0.0), dx)
+    delta = 0.02
+    U0 = 0.5
+    band = 0.5 * (np.tanh((Y - 0.25) / delta) - np.tanh((Y - 0.75) / delta))
+    u = U0 * (2.0 * band - 1.0)
+    rho = 1.0 + band
+    env = np.exp(-((Y - 0.25) / 0.05) ** 2) + np.exp(-((Y - 0.75) / 0.05) ** 2)
+    v = np.zeros_like(X)
+    for k in range(8):
+        v = v + amplitudes[k] * np.sin(2.0 * np.pi * (k + 1) * X + phases[k]) * env
+    p = 2.5 * np.ones_like(X)
+    return conservative_from_primitive_single(rho, [u, v], p, gamma), n, dx, np.asarray(amplitudes), phases
+
+
+def split_patches(Q_global_interior: np.ndarray, patch: tuple, g: int):
+    """Split a periodic global interior field into patches of extent ``patch``
+    (same rank), each returned with its ghost halo filled from its neighbours.
+
+    Returns a list of (offset, Q_patch) with offset in (x, y, z) order."""
+    d = Q_global_interior.ndim - 1
+    N = tuple(reversed(Q_global_interior.shape[1:]))
+    Qg = fill_periodic(Q_global_interior, g)
+    out = []
+    counts = [N[a] // patch[a] for a in range(d)]
+    for idx in np.ndindex(*reversed(counts)):
+        off = tuple(reversed([i * p for i, p in zip(idx, reversed(patch))]))
+        sl = [slice(None)] + [slice(off[a], off[a] + patch[a] + 2 * g) for a in reversed(range(d))]
+        out.append((off, np.ascontiguousarray(Qg[tuple(sl)])))
+    return out

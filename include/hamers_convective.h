@@ -1,0 +1,147 @@
+/*
+ * hamers_convective.h -- C-ABI of the MI355X convective-flux RHS library
+ * (libhamers_amd.so).
+ *
+ * Drop-in boundary for HAMeRS' convective-flux hot path.  Every entry point
+ * takes plain pointers and sizes; device pointers are HIP device addresses.
+ * All calls return an int status (HM_OK = 0) and never abort; the message of
+ * the last failure on a context is available from hm_last_error().
+ *
+ * Array layouts are SAMRAI's (x fastest, component-major):
+ *   conservative variables: one array per equation over the ghost box
+ *       idx = (i+g0) + (j+g1)*(n0+2g0) + (k+g2)*(n0+2g0)*(n1+2g1)
+ *   convective flux (SideData, 0 ghosts), one array per axis a holding num_eqn
+ *       components of (n_a+1) * prod_{b!=a} n_b faces each
+ *   source (CellData, 0 ghosts): num_eqn components of prod n_b cells each.
+ */
+#ifndef HAMERS_CONVECTIVE_H
+#define HAMERS_CONVECTIVE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HM_MAX_EQN 12
+#define HM_MAX_SPECIES 8
+
+/* Status codes. */
+enum {
+    HM_OK = 0,
+    HM_ERR_INVALID_ARGUMENT = 1,
+    HM_ERR_UNSUPPORTED = 2,
+    HM_ERR_DEVICE = 3,
+    HM_ERR_OUT_OF_MEMORY = 4
+};
+
+/* Flow models: include/flow/flow_models/FlowModels.hpp (FLOW_MODEL::TYPE). */
+enum {
+    HM_MODEL_SINGLE_SPECIES = 0,
+    HM_MODEL_FOUR_EQN_CONSERVATIVE = 1,
+    HM_MODEL_FIVE_EQN_ALLAIRE = 2
+};
+
+/* Convective flux schemes: the deck strings "WCNS5_JS_HLLC_HLL" and
+ * "WCNS6_LD_HLLC_HLL" of ConvectiveFluxReconstructorManager.cpp:40,79. */
+enum {
+    HM_SCHEME_WCNS5_JS_HLLC_HLL = 0,
+    HM_SCHEME_WCNS6_LD_HLLC_HLL = 1
+};
+
+/*
+ * Construction parameters of a reconstructor context.  Replaces the
+ * constructor arguments of ConvectiveFluxReconstructorWCNS6_LD_HLLC_HLL /
+ * ..._WCNS5_JS_HLLC_HLL (ConvectiveFluxReconstructorManager.cpp:14-153,
+ * ConvectiveFluxReconstructorWCNS6-LD-HLLC-HLL.cpp:324-361) and of the flow
+ * model's equation-of-state mixing rules (species_gamma, species_R).
+ */
+typedef struct hm_config {
+    int dim;                 /* 1, 2 or 3 */
+    int model;               /* HM_MODEL_* */
+    int num_species;         /* 1 for HM_MODEL_SINGLE_SPECIES */
+    int scheme;              /* HM_SCHEME_* */
+    int constant_p;          /* WCNS exponent p (deck default 2) */
+    int constant_q;          /* WCNS6-LD exponent q (deck default 4) */
+    double constant_C;       /* WCNS6-LD C (deck default 1e9) */
+    double constant_alpha_tau; /* WCNS6-LD alpha_tau (deck default 35) */
+    double species_gamma[HM_MAX_SPECIES];
+    double species_R[HM_MAX_SPECIES];
+    int device;              /* HIP device ordinal the context runs on */
+} hm_config;
+
+/*
+ * One patch as the reconstructor sees it: the SAMRAI hier::Patch box, the
+ * CartesianPatchGeometry dx and the patch-data pointers that
+ * computeConvectiveFluxAndSourceOnPatch reads and writes.
+ */
+typedef struct hm_patch_view {
+    int n[3];                     /* interior cells per axis (1 on unused axes) */
+    int ghost[3];                 /* ghost width of Q per axis (>= 4 on used axes) */
+    double dx[3];                 /* grid spacing */
+    const double* Q[HM_MAX_EQN];  /* device: one array per equation over the ghost box */
+    double* F[3];                 /* device: flux SideData array per axis (overwritten) */
+    double* S;                    /* device: source CellData (accumulated, ADVECTIVE eqs) */
+} hm_patch_view;
+
+typedef struct hm_ctx hm_ctx;
+
+/* Create / destroy a reconstructor context (owns device scratch and the
+ * launch plan cache).  Replaces ConvectiveFluxReconstructorManager
+ * construction (ConvectiveFluxReconstructorManager.cpp:14-153). */
+int hm_ctx_create(const hm_config* cfg, hm_ctx** out);
+int hm_ctx_destroy(hm_ctx* ctx);
+
+/* ConvectiveFluxReconstructor::getConvectiveFluxNumberOfGhostCells
+ * (ConvectiveFluxReconstructor.hpp:50-54; 4 for WCNS56, WCNS56-HLLC-HLL.cpp:22). */
+int hm_get_num_ghosts(const hm_ctx* ctx, int* num_ghosts);
+
+/* Number of equations of the configured flow model (FlowModel::getNumberOfEquations). */
+int hm_get_num_eqn(const hm_ctx* ctx, int* num_eqn);
+
+/* Message describing the last failure on this context ("" if none). */
+const char* hm_last_error(const hm_ctx* ctx);
+
+/*
+ * ConvectiveFluxReconstructor::computeConvectiveFluxAndSourceOnPatch
+ * (ConvectiveFluxReconstructor.hpp:73-81, implemented for WCNS56 at
+ * ConvectiveFluxReconstructorWCNS56-HLLC-HLL.cpp:39-2656): per-patch
+ * ("plain") path.  F is overwritten with dt-scaled face fluxes; S is
+ * accumulated (+=) for ADVECTIVE equations only.  `time` and `rk_step` are
+ * accepted for signature parity and unused, as in the reference.
+ * Asynchronous on `stream` (a hipStream_t, NULL = default stream).
+ */
+int hm_compute_convective_flux_and_source_on_patch(hm_ctx* ctx, const hm_patch_view* patch, double time,
+                                                    double dt, int rk_step, void* stream);
+
+/*
+ * Batched form of the call above for all patches of a level (uneven sizes
+ * allowed): one work-list driven launch per kernel stage instead of one
+ * launch sequence per patch.  Results are identical to calling the per-patch
+ * entry point on every patch.  Replaces the serial patch loop of
+ * RungeKuttaLevelIntegrator::advanceLevel (RungeKuttaLevelIntegrator.cpp:1715-1742)
+ * around the reconstructor call.
+ */
+int hm_compute_convective_flux_and_source_on_patches(hm_ctx* ctx, const hm_patch_view* patches, int num_patches,
+                                                      double time, double dt, int rk_step, void* stream);
+
+/*
+ * Stage timing (instrumentation; no counterpart in the reference).  While
+ * enabled, every RHS call records HIP events on its stream around each of the
+ * four pipeline stages: 0 derived cell data, 1 dilatation/vorticity, 2 fused
+ * midpoint flux, 3 midpoint-to-face flux.  hm_profile_read waits for the
+ * recorded events and returns the summed milliseconds and launch counts per
+ * stage since the last hm_profile_enable.
+ */
+int hm_profile_enable(hm_ctx* ctx, int enable);
+int hm_profile_read(hm_ctx* ctx, double* stage_ms /* [4] */, int64_t* stage_launches /* [4] */);
+
+/* Block the host until all work this context queued on `stream` is done. */
+int hm_synchronize(hm_ctx* ctx, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HAMERS_CONVECTIVE_H */

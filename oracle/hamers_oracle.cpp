@@ -1,0 +1,800 @@
+// hamers_oracle.cpp -- CPU restatement of HAMeRS' convective-flux RHS hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: only tests/,
+// __graft_entry__.smoke() and bench.py's CPU-baseline leg may load it, and
+// only as the checker (or as the timed CPU baseline).  The product path
+// (hamers_amd/) never calls it.
+//
+// It restates, in operation order, the reference loops of
+//   src/flow/convective_flux_reconstructors/WCNS56/ConvectiveFluxReconstructorWCNS56-HLLC-HLL.cpp:39-2656
+//   src/flow/convective_flux_reconstructors/WCNS56/ConvectiveFluxReconstructorWCNS6-LD-HLLC-HLL.cpp:9-319
+//   src/flow/convective_flux_reconstructors/WCNS56/ConvectiveFluxReconstructorWCNS5-JS-HLLC-HLL.cpp:9-164
+//   src/flow/flow_models/single-species/* (derived data, projection, bounds, Riemann solvers)
+//   src/util/derivatives/DerivativeFirstOrder.cpp:216-228 (2nd-order central first derivative)
+//   src/apps/Euler/Euler.cpp:1003-1680 (Runge-Kutta stage update)
+//   src/algs/integrator/RungeKuttaLevelIntegrator.cpp:1672-1745, 3907-3924 (SSP-RK3 stage loop, weights)
+// pass by pass over whole-patch arrays, as the reference does, so that it is
+// structurally independent of the fused GPU kernels it checks.  Each
+// arithmetic expression keeps the reference's expression tree (association
+// order, reciprocal-then-multiply forms) so that an uncontracted build
+// (-ffp-contract=off, no fast-math) reproduces the reference's rounding.
+//
+// Layouts (SAMRAI CellData / SideData, x fastest, component-major):
+//   Q  : neq components over the ghost box (n+2g)^d
+//   F_a: neq components over the axis-a face box (n_a+1) x prod_{b!=a} n_b, 0 ghosts
+//   S  : neq components over the interior box, accumulated (+=)
+//
+// Pinned by tests/test_oracle.py (closed forms, invariants, convergence orders
+// of tests/2D_convergence_test_single_species/convergence_test.py:13-16).
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define HMO_EPS 1.0e-15  // include/HAMeRS_config.hpp.in:16 (HAMERS_EPSILON)
+
+extern "C" {
+
+typedef struct hmo_params {
+    int dim;           // 1, 2 or 3
+    int model;         // 0 = SINGLE_SPECIES, 1 = FOUR_EQN_CONSERVATIVE, 2 = FIVE_EQN_ALLAIRE
+    int num_species;   // 1 for SINGLE_SPECIES
+    int scheme;        // 0 = WCNS5_JS_HLLC_HLL, 1 = WCNS6_LD_HLLC_HLL
+    int p, q;          // WCNS exponents (defaults 2, 4)
+    double C;          // WCNS6-LD constant C (default 1e9)
+    double alpha_tau;  // WCNS6-LD constant alpha_tau (default 35)
+    double gamma[8];   // species ratios of specific heats
+    double R[8];       // species gas constants (four-eqn mixing)
+    int mutation;      // test-only: plants a known mistake (0 = none)
+} hmo_params;
+
+}  // extern "C"
+
+namespace {
+
+constexpr double EPS = HMO_EPS;
+
+// ---------------------------------------------------------------------------
+// Pointwise reconstruction (WCNS6-LD: ...WCNS6-LD-HLLC-HLL.cpp:9-319,
+// WCNS5-JS: ...WCNS5-JS-HLLC-HLL.cpp:9-164).  U[0..5] are the six stencil
+// values of one characteristic field for one face.
+// ---------------------------------------------------------------------------
+
+inline double ipow_ref(double base, int e) {  // repeated multiplication, WCNS6-LD:9-18
+    double r = base;
+    for (int i = 1; i < e; i++) r *= base;
+    return r;
+}
+
+// Jiang-Shu style smoothness indicators on sub-stencils (0,1,2), (1,2,3),
+// (2,3,4) and the six-point indicator (WCNS6-LD:47-84).  `a` maps the
+// stencil: a(k) = U[k] (minus side) or U[5-k] (plus side, WCNS6-LD:90-127).
+struct Stencil6 {
+    double u0, u1, u2, u3, u4, u5;
+};
+
+inline Stencil6 mirror(const Stencil6& s) { return {s.u5, s.u4, s.u3, s.u2, s.u1, s.u0}; }
+
+inline void smoothness3(const Stencil6& s, double& b0, double& b1, double& b2) {
+    b0 = double(1) / double(3) *
+         (s.u0 * (double(4) * s.u0 - double(19) * s.u1 + double(11) * s.u2) +
+          s.u1 * (double(25) * s.u1 - double(31) * s.u2) + double(10) * s.u2 * s.u2);
+    b1 = double(1) / double(3) *
+         (s.u1 * (double(4) * s.u1 - double(13) * s.u2 + double(5) * s.u3) +
+          double(13) * s.u2 * (s.u2 - s.u3) + double(4) * s.u3 * s.u3);
+    b2 = double(1) / double(3) *
+         (s.u2 * (double(10) * s.u2 - double(31) * s.u3 + double(11) * s.u4) +
+          s.u3 * (double(25) * s.u3 - double(19) * s.u4) + double(4) * s.u4 * s.u4);
+}
+
+inline double smoothness6(const Stencil6& s) {
+    return double(1) / double(232243200) *
+           (s.u0 * (double(525910327) * s.u0 - double(4562164630) * s.u1 + double(7799501420) * s.u2 -
+                    double(6610694540) * s.u3 + double(2794296070) * s.u4 - double(472758974) * s.u5) +
+            double(5) * s.u1 *
+                (double(2146987907) * s.u1 - double(7722406988) * s.u2 + double(6763559276) * s.u3 -
+                 double(2926461814) * s.u4 + double(503766638) * s.u5) +
+            double(20) * s.u2 *
+                (double(1833221603) * s.u2 - double(3358664662) * s.u3 + double(1495974539) * s.u4 -
+                 double(263126407) * s.u5) +
+            double(20) * s.u3 * (double(1607794163) * s.u3 - double(1486026707) * s.u4 + double(268747951) * s.u5) +
+            double(5) * s.u4 * (double(1432381427) * s.u4 - double(536951582) * s.u5) +
+            double(263126407) * s.u5 * s.u5);
+}
+
+// WCNS6-LD one side.  `s` is already oriented (mirrored for the plus side);
+// `sig` is the shock measure computed on the unmirrored stencil
+// (WCNS6-LD:24-41 is called with U_array unchanged on both sides).
+inline double wcns6ld_side(const Stencil6& s, double sig, int p, int q, double C, double alpha_tau, int mut) {
+    double b0, b1, b2;
+    smoothness3(s, b0, b1, b2);
+    const double b3 = smoothness6(s);
+
+    const double tau5 = std::fabs(b0 - b2);
+    double wu0 = double(1) / double(16) * (double(1) + ipow_ref(tau5 / (b0 + EPS), p));
+    double wu1 = double(5) / double(8) * (double(1) + ipow_ref(tau5 / (b1 + EPS), p));
+    double wu2 = double(5) / double(16) * (double(1) + ipow_ref(tau5 / (b2 + EPS), p));
+    const double wus = wu0 + wu1 + wu2;
+    wu0 = wu0 / wus;
+    wu1 = wu1 / wus;
+    wu2 = wu2 / wus;
+
+    const double bavg = double(1) / double(8) * (b0 + b2 + double(6) * b1);
+    const double tau6 = std::fabs(b3 - bavg);
+    double w0 = double(1) / double(32) * (C + ipow_ref(tau6 / (b0 + EPS), q));
+    double w1 = double(15) / double(32) * (C + ipow_ref(tau6 / (b1 + EPS), q));
+    double w2 = double(15) / double(32) * (C + ipow_ref(tau6 / (b2 + EPS), q));
+    double w3 = double(1) / double(32) * (C + ipow_ref(tau6 / (b3 + EPS), q));
+    const double ws = w0 + w1 + w2 + w3;
+    w0 = w0 / ws;
+    w1 = w1 / ws;
+    w2 = w2 / ws;
+    w3 = w3 / ws;
+
+    const double Rtau = tau6 / (bavg + EPS);
+    if (Rtau > alpha_tau) {
+        w0 = sig * wu0 + (double(1) - sig) * w0;
+        w1 = sig * wu1 + (double(1) - sig) * w1;
+        w2 = sig * wu2 + (double(1) - sig) * w2;
+        w3 = (double(1) - sig) * w3;
+    }
+    if (mut == 1) {  // planted mistake: drop the sixth-point contribution
+        return double(3) / double(8) * w0 * s.u0 + (-double(10) / double(8) * w0 - double(1) / double(8) * w1) * s.u1 +
+               (double(15) / double(8) * w0 + double(6) / double(8) * w1 + double(3) / double(8) * w2) * s.u2 +
+               (double(3) / double(8) * w1 + double(6) / double(8) * w2 + double(15) / double(8) * w3) * s.u3 +
+               (-double(1) / double(8) * w2 - double(10) / double(8) * w3) * s.u4;
+    }
+    return double(3) / double(8) * w0 * s.u0 + (-double(10) / double(8) * w0 - double(1) / double(8) * w1) * s.u1 +
+           (double(15) / double(8) * w0 + double(6) / double(8) * w1 + double(3) / double(8) * w2) * s.u2 +
+           (double(3) / double(8) * w1 + double(6) / double(8) * w2 + double(15) / double(8) * w3) * s.u3 +
+           (-double(1) / double(8) * w2 - double(10) / double(8) * w3) * s.u4 + double(3) / double(8) * w3 * s.u5;
+}
+
+inline double wcns6ld_sigma(const Stencil6& s) {  // WCNS6-LD:24-41
+    const double a1 = s.u2 - s.u1;
+    const double a2 = s.u3 - s.u2;
+    const double a3 = s.u4 - s.u3;
+    const double t1 = std::fabs(a1 - a2) / (std::fabs(a1) + std::fabs(a2) + EPS);
+    const double t2 = std::fabs(a2 - a3) / (std::fabs(a2) + std::fabs(a3) + EPS);
+    return std::fmax(t1, t2);
+}
+
+// WCNS5-JS one side (WCNS5-JS:78-118); the plus side is the mirrored stencil.
+inline double wcns5js_side(const Stencil6& s, int p, int mut) {
+    double b0, b1, b2;
+    smoothness3(s, b0, b1, b2);
+    double w0 = double(1.0) / double(16) / ipow_ref((b0 + EPS), p);
+    double w1 = double(5.0) / double(8) / ipow_ref((b1 + EPS), p);
+    double w2 = double(5.0) / double(16) / ipow_ref((b2 + EPS), p);
+    if (mut == 2) w2 = double(5.0) / double(16) / ipow_ref((b1 + EPS), p);  // planted mistake: wrong beta index
+    const double ws = w0 + w1 + w2;
+    w0 = w0 / ws;
+    w1 = w1 / ws;
+    w2 = w2 / ws;
+    return double(3) / double(8) * w0 * s.u0 + (-double(10) / double(8) * w0 - double(1) / double(8) * w1) * s.u1 +
+           (double(15) / double(8) * w0 + double(6) / double(8) * w1 + double(3) / double(8) * w2) * s.u2 +
+           (double(3) / double(8) * w1 + double(6) / double(8) * w2) * s.u3 - double(1) / double(8) * w2 * s.u4;
+}
+
+// ---------------------------------------------------------------------------
+// Patch geometry helpers.
+// ---------------------------------------------------------------------------
+
+struct Box {
+    int lo[3], hi[3];  // inclusive-exclusive per axis
+    int64_t size() const {
+        return int64_t(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+    }
+};
+
+struct CellArray {  // a component-major array over a box with given ghost width
+    int n[3];
+    int g[3];
+    int ext[3];
+    int nc;
+    std::vector<double> v;
+    void init(const int* nn, int gw, int dim, int ncomp) {
+        for (int a = 0; a < 3; a++) {
+            n[a] = nn[a];
+            g[a] = (a < dim) ? gw : 0;
+            ext[a] = n[a] + 2 * g[a];
+        }
+        nc = ncomp;
+        v.assign(size_t(nc) * ext[0] * ext[1] * ext[2], 0.0);
+    }
+    int64_t vol() const { return int64_t(ext[0]) * ext[1] * ext[2]; }
+    int64_t idx(int i, int j, int k) const {
+        return (i + g[0]) + int64_t(j + g[1]) * ext[0] + int64_t(k + g[2]) * ext[0] * ext[1];
+    }
+    double* c(int comp) { return v.data() + comp * vol(); }
+    const double* c(int comp) const { return v.data() + comp * vol(); }
+};
+
+// Midpoint face array for axis a: faces i_a in [-1, n_a+1], transverse
+// indices in [0, n_b).  (The reference allocates these with one ghost on
+// every axis, but only this sub-box is ever read, WCNS6-LD:486-496.)
+struct FaceArray {
+    int a;
+    int lo[3], ext[3];
+    int nc;
+    std::vector<double> v;
+    void init(const int* n, int axis, int ncomp) {
+        a = axis;
+        for (int b = 0; b < 3; b++) {
+            lo[b] = (b == axis) ? -1 : 0;
+            ext[b] = (b == axis) ? n[b] + 3 : n[b];
+        }
+        nc = ncomp;
+        v.assign(size_t(nc) * vol(), 0.0);
+    }
+    int64_t vol() const { return int64_t(ext[0]) * ext[1] * ext[2]; }
+    int64_t idx(int i, int j, int k) const {
+        return (i - lo[0]) + int64_t(j - lo[1]) * ext[0] + int64_t(k - lo[2]) * ext[0] * ext[1];
+    }
+    double* c(int comp) { return v.data() + comp * vol(); }
+    const double* c(int comp) const { return v.data() + comp * vol(); }
+};
+
+struct Model {
+    int dim, model, ns, neq;
+    int nproj;            // number of projection variables
+    double gamma[8], R[8];
+    bool advective[16];
+    bool has_advective;
+    // Equation/primitive-variable layout (single-species: rho, u.., p).
+    int iv_vel0;  // index of first velocity component in V
+    int iv_p;     // index of pressure in V
+};
+
+Model make_model(const hmo_params& prm) {
+    Model m{};
+    m.dim = prm.dim;
+    m.model = prm.model;
+    m.ns = prm.num_species;
+    for (int i = 0; i < 8; i++) {
+        m.gamma[i] = prm.gamma[i];
+        m.R[i] = prm.R[i];
+    }
+    if (m.model == 0) {
+        m.ns = 1;
+        m.neq = m.dim + 2;  // FlowModelSingleSpecies: rho, rho*u, E
+        m.nproj = 2;        // FlowModelBasicUtilitiesSingleSpecies.cpp:3526-3529
+        m.iv_vel0 = 1;
+        m.iv_p = m.dim + 1;
+    } else {
+        m.neq = -1;  // other models are restated separately
+    }
+    m.has_advective = false;
+    for (int e = 0; e < 16; e++) m.advective[e] = false;
+    return m;
+}
+
+// Derived cell data on the 4-ghost box (FlowModelSingleSpecies.cpp:2649-3183,
+// EquationOfStateIdealGas.cpp:5477, 5806): V = (rho, u.., p), c, and the node
+// convective flux along every axis (FlowModelSingleSpecies.cpp:3233-3883).
+struct Derived {
+    CellArray V;     // neq components
+    CellArray c;     // sound speed
+    CellArray Fn[3]; // node convective flux per axis, neq comps
+};
+
+void compute_derived_single(const Model& m, const int* n, int g, const double* Q, Derived& D) {
+    const int d = m.dim;
+    const double gam = m.gamma[0];
+    CellArray Qv;  // view description only
+    Qv.init(n, g, d, 0);
+    D.V.init(n, 4, d, m.neq);
+    D.c.init(n, 4, d, 1);
+    for (int a = 0; a < d; a++) D.Fn[a].init(n, 4, d, m.neq);
+    const int64_t qvol = Qv.vol();
+    const int lo2 = (d > 2) ? -4 : 0, hi2 = (d > 2) ? n[2] + 4 : 1;
+    const int lo1 = (d > 1) ? -4 : 0, hi1 = (d > 1) ? n[1] + 4 : 1;
+    // velocity (FlowModelSingleSpecies.cpp:2649-2848)
+    for (int k = lo2; k < hi2; k++)
+        for (int j = lo1; j < hi1; j++)
+            for (int i = -4; i < n[0] + 4; i++) {
+                const int64_t iq = Qv.idx(i, j, k), iv = D.V.idx(i, j, k);
+                const double rho = Q[iq];
+                D.V.c(0)[iv] = rho;
+                for (int b = 0; b < d; b++) D.V.c(1 + b)[iv] = Q[(1 + b) * qvol + iq] / rho;
+            }
+    // internal energy, pressure, sound speed (FlowModelSingleSpecies.cpp:2849-3179,
+    // EquationOfStateIdealGas.cpp:5477, 5806)
+    for (int k = lo2; k < hi2; k++)
+        for (int j = lo1; j < hi1; j++)
+            for (int i = -4; i < n[0] + 4; i++) {
+                const int64_t iq = Qv.idx(i, j, k), iv = D.V.idx(i, j, k);
+                const double rho = Q[iq];
+                const double E = Q[(d + 1) * qvol + iq];
+                double ke;
+                const double u = D.V.c(1)[iv];
+                if (d == 1) {
+                    ke = u * u;
+                } else if (d == 2) {
+                    const double v = D.V.c(2)[iv];
+                    ke = u * u + v * v;
+                } else {
+                    const double v = D.V.c(2)[iv], w = D.V.c(3)[iv];
+                    ke = u * u + v * v + w * w;
+                }
+                const double eps_int = E / rho - double(1) / double(2) * ke;
+                const double p = (gam - double(1)) * rho * eps_int;
+                D.V.c(d + 1)[iv] = p;
+                D.c.c(0)[iv] = std::sqrt(gam * p / rho);
+            }
+    // node convective flux per axis (FlowModelSingleSpecies.cpp:3329-3331, 3388-3391, 3470-3474, ...)
+    for (int a = 0; a < d; a++)
+        for (int k = lo2; k < hi2; k++)
+            for (int j = lo1; j < hi1; j++)
+                for (int i = -4; i < n[0] + 4; i++) {
+                    const int64_t iq = Qv.idx(i, j, k), iv = D.V.idx(i, j, k);
+                    const double ua = D.V.c(1 + a)[iv];
+                    const double p = D.V.c(d + 1)[iv];
+                    const double E = Q[(d + 1) * qvol + iq];
+                    D.Fn[a].c(0)[iv] = Q[(1 + a) * qvol + iq];
+                    for (int b = 0; b < d; b++) {
+                        if (b == a)
+                            D.Fn[a].c(1 + b)[iv] = ua * Q[(1 + b) * qvol + iq] + p;
+                        else
+                            D.Fn[a].c(1 + b)[iv] = ua * Q[(1 + b) * qvol + iq];
+                    }
+                    D.Fn[a].c(d + 1)[iv] = ua * (E + p);
+                }
+}
+
+// Dilatation and vorticity magnitude on the 2-ghost box from second-order
+// central velocity derivatives (WCNS56-HLLC-HLL.cpp:626-731 (2D),
+// 1470-1656 (3D); DerivativeFirstOrder.cpp:216-228).
+void compute_theta_omega(const Model& m, const int* n, const Derived& D, const double* dx, CellArray& theta,
+                         CellArray& Omega) {
+    const int d = m.dim;
+    theta.init(n, 2, d, 1);
+    Omega.init(n, 2, d, 1);
+    if (d == 1) return;
+    // derivative array du_b/dx_a, layout [b*d + a] as in the reference (dudx, dudy, ..).
+    CellArray dv;
+    dv.init(n, 2, d, d * d);
+    const int lo2 = (d > 2) ? -2 : 0, hi2 = (d > 2) ? n[2] + 2 : 1;
+    for (int b = 0; b < d; b++)
+        for (int a = 0; a < d; a++) {
+            const double* u = D.V.c(1 + b);
+            double* out = dv.c(b * d + a);
+            const int off[3] = {a == 0, a == 1, a == 2};
+            for (int k = lo2; k < hi2; k++)
+                for (int j = -2; j < n[1] + 2; j++)
+                    for (int i = -2; i < n[0] + 2; i++) {
+                        const double uR = u[D.V.idx(i + off[0], j + off[1], k + off[2])];
+                        const double uL = u[D.V.idx(i - off[0], j - off[1], k - off[2])];
+                        out[dv.idx(i, j, k)] = (double(1) / double(2) * (uR - uL)) / dx[a];
+                    }
+        }
+    for (int k = lo2; k < hi2; k++)
+        for (int j = -2; j < n[1] + 2; j++)
+            for (int i = -2; i < n[0] + 2; i++) {
+                const int64_t id = dv.idx(i, j, k);
+                if (d == 2) {
+                    const double dudx = dv.c(0)[id], dudy = dv.c(1)[id], dvdx = dv.c(2)[id], dvdy = dv.c(3)[id];
+                    theta.c(0)[id] = dudx + dvdy;
+                    Omega.c(0)[id] = std::fabs(dvdx - dudy);
+                } else {
+                    const double dudx = dv.c(0)[id], dudy = dv.c(1)[id], dudz = dv.c(2)[id];
+                    const double dvdx = dv.c(3)[id], dvdy = dv.c(4)[id], dvdz = dv.c(5)[id];
+                    const double dwdx = dv.c(6)[id], dwdy = dv.c(7)[id], dwdz = dv.c(8)[id];
+                    theta.c(0)[id] = dudx + dvdy + dwdz;
+                    const double ox = dwdy - dvdz;
+                    const double oy = dudz - dwdx;
+                    const double oz = dvdx - dudy;
+                    Omega.c(0)[id] = std::sqrt(ox * ox + oy * oy + oz * oz);
+                }
+            }
+}
+
+// Midpoint flux of one face for the single-species model: HLLC
+// (FlowModelRiemannSolverSingleSpeciesHLLC.cpp:519-1081), replaced in 2D/3D
+// by the HLLC-HLL flux (FlowModelRiemannSolverSingleSpeciesHLLC-HLL.cpp:889-1629)
+// where the Ducros-type sensor selects it (WCNS56-HLLC-HLL.cpp:1090-1113).
+// Sound speed and internal energy on faces come from the ideal-gas EOS
+// (EquationOfStateIdealGas.cpp:5806, 6135).  When `ustar` is non-null it
+// receives the HLLC normal velocity (FlowModelRiemannSolverSingleSpeciesHLLC.cpp:3213-3220).
+void riemann_face_single(int d, int a, double gam, bool use_hll, const double* VL, const double* VR, double* Fout,
+                         double* ustar) {
+    const int neq = d + 2;
+    const double rhoL = VL[0], rhoR = VR[0];
+    const double pL = VL[neq - 1], pR = VR[neq - 1];
+    const double cL = std::sqrt(gam * pL / rhoL);
+    const double cR = std::sqrt(gam * pR / rhoR);
+    const double epsL = pL / ((gam - double(1)) * rhoL);
+    const double epsR = pR / ((gam - double(1)) * rhoR);
+    const double uL = VL[1 + a], uR = VR[1 + a];
+
+    const double u_avg = double(1) / double(2) * (uL + uR);
+    const double c_av = double(1) / double(2) * (cL + cR);
+    const double sL = std::fmin(u_avg - c_av, uL - cL);
+    const double sR = std::fmax(u_avg + c_av, uR + cR);
+    const double sm = std::fmin(double(0), sL);
+    const double sp = std::fmax(double(0), sR);
+    const double sstar =
+        (pR - pL + rhoL * uL * (sL - uL) - rhoR * uR * (sR - uR)) / (rhoL * (sL - uL) - rhoR * (sR - uR));
+
+    // Conservative states and physical fluxes on both sides.
+    auto cons = [&](const double* V, double eps_int, double* Qs) {
+        Qs[0] = V[0];
+        for (int b = 0; b < d; b++) Qs[1 + b] = V[0] * V[1 + b];
+        double ke;
+        if (d == 1)
+            ke = V[1] * V[1];
+        else if (d == 2)
+            ke = V[1] * V[1] + V[2] * V[2];
+        else
+            ke = V[1] * V[1] + V[2] * V[2] + V[3] * V[3];
+        Qs[d + 1] = V[0] * (eps_int + double(1) / double(2) * ke);
+    };
+    auto flux = [&](const double* V, const double* Qs, double* Fs) {
+        Fs[0] = Qs[1 + a];
+        for (int b = 0; b < d; b++) {
+            if (b == a)
+                Fs[1 + b] = Qs[1 + a] * V[1 + b] + V[d + 1];
+            else
+                Fs[1 + b] = Qs[1 + a] * V[1 + b];
+        }
+        Fs[d + 1] = V[1 + a] * (Qs[d + 1] + V[d + 1]);
+    };
+    double QL[5], QR[5], FL[5], FR[5];
+    cons(VL, epsL, QL);
+    cons(VR, epsR, QR);
+    flux(VL, QL, FL);
+    flux(VR, QR, FR);
+
+    double Fhllc[5];
+    const bool left = (sstar > double(0));
+    {
+        const double* V = left ? VL : VR;
+        const double* Qs = left ? QL : QR;
+        const double* Fs = left ? FL : FR;
+        const double sK = left ? sL : sR;
+        const double spm = left ? sm : sp;
+        const double chi = (sK - V[1 + a]) / (sK - sstar);
+        double Qst[5];
+        Qst[0] = chi * V[0];
+        for (int b = 0; b < d; b++) {
+            if (b == a)
+                Qst[1 + b] = chi * V[0] * sstar;
+            else
+                Qst[1 + b] = chi * Qs[1 + b];
+        }
+        Qst[d + 1] = chi * (Qs[d + 1] + (sstar - V[1 + a]) * (V[0] * sstar + V[d + 1] / (sK - V[1 + a])));
+        for (int e = 0; e < neq; e++) Fhllc[e] = Fs[e] + spm * (Qst[e] - Qs[e]);
+        if (ustar) *ustar = V[1 + a] + spm * (chi - double(1));
+    }
+    for (int e = 0; e < neq; e++) Fout[e] = Fhllc[e];
+    if (d > 1 && use_hll) {
+        // HLL flux on mass and transverse momentum.
+        double Fhll[5];
+        for (int e = 0; e < neq; e++) Fhll[e] = 0.0;
+        auto hll = [&](int e) {
+            double r = (sR * FL[e] - sL * FR[e] + sR * sL * (QR[e] - QL[e])) / (sR - sL);
+            if (sL > double(0)) r = FL[e];
+            if (sR < double(0)) r = FR[e];
+            return r;
+        };
+        Fhll[0] = hll(0);
+        for (int b = 0; b < d; b++)
+            if (b != a) Fhll[1 + b] = hll(1 + b);
+        double dsq;
+        {
+            const double du = VR[1] - VL[1];
+            if (d == 2) {
+                const double dv = VR[2] - VL[2];
+                dsq = du * du + dv * dv;
+            } else {
+                const double dv = VR[2] - VL[2], dw = VR[3] - VL[3];
+                dsq = du * du + dv * dv + dw * dw;
+            }
+        }
+        const double vel_mag = std::sqrt(dsq);
+        double al1, al2;
+        if (vel_mag < EPS) {
+            al1 = double(1);
+            al2 = double(0);
+        } else {
+            al1 = std::fabs(VR[1 + a] - VL[1 + a]) / vel_mag;
+            al2 = std::sqrt(double(1) - al1 * al1);
+        }
+        const double be1 = double(1) / double(2) * (double(1) + al1 / (al1 + al2));
+        const double be2 = double(1) - be1;
+        Fout[0] = be1 * Fhllc[0] + be2 * Fhll[0];
+        for (int b = 0; b < d; b++) {
+            if (b == a)
+                Fout[1 + b] = Fhllc[1 + b];
+            else
+                Fout[1 + b] = be1 * Fhllc[1 + b] + be2 * Fhll[1 + b];
+        }
+        Fout[d + 1] = Fhllc[d + 1];
+    }
+}
+
+// Characteristic variables of one stencil cell for axis a
+// (FlowModelBasicUtilitiesSingleSpecies.cpp:5882-6451).
+inline void char_from_prim_single(int d, int a, double ra, double ca, const double* V, double* W) {
+    const double un = V[1 + a];
+    const double p = V[d + 1];
+    W[0] = -double(1) / double(2) * ra * ca * un + double(1) / double(2) * p;
+    W[1] = V[0] - double(1) / (ca * ca) * p;
+    int t = 2;
+    for (int b = 0; b < d; b++)
+        if (b != a) W[t++] = V[1 + b];
+    W[d + 1] = double(1) / double(2) * ra * ca * un + double(1) / double(2) * p;
+}
+
+// Primitive variables back from characteristic ones
+// (FlowModelBasicUtilitiesSingleSpecies.cpp:7023-7473).
+inline void prim_from_char_single(int d, int a, double ra, double ca, const double* W, double* V) {
+    const double w0 = W[0], wl = W[d + 1];
+    V[0] = double(1) / (ca * ca) * w0 + W[1] + double(1) / (ca * ca) * wl;
+    V[1 + a] = -double(1) / (ra * ca) * w0 + double(1) / (ra * ca) * wl;
+    int t = 2;
+    for (int b = 0; b < d; b++)
+        if (b != a) V[1 + b] = W[t++];
+    V[d + 1] = w0 + wl;
+}
+
+// One axis of the WCNS56 driver for the single-species model
+// (WCNS56-HLLC-HLL.cpp:879-1282 in 2D, 1746-2490 in 3D), pass by pass.
+void direction_pass_single(const hmo_params& prm, const Model& m, const int* n, int a, const Derived& D,
+                           const CellArray& theta, const CellArray& Omega, double dt, double* F_out) {
+    const int d = m.dim, neq = m.neq;
+    const double gam = m.gamma[0];
+    const int sh[3] = {a == 0, a == 1, a == 2};
+
+    FaceArray rho_avg, c_avg;
+    rho_avg.init(n, a, 1);
+    c_avg.init(n, a, 1);
+    auto for_faces = [&](auto&& fn) {
+        for (int k = rho_avg.lo[2]; k < rho_avg.lo[2] + rho_avg.ext[2]; k++)
+            for (int j = rho_avg.lo[1]; j < rho_avg.lo[1] + rho_avg.ext[1]; j++)
+                for (int i = rho_avg.lo[0]; i < rho_avg.lo[0] + rho_avg.ext[0]; i++) fn(i, j, k);
+    };
+
+    // Projection variables: simple face averages (FlowModelBasicUtilitiesSingleSpecies.cpp:4638-5134).
+    for_faces([&](int i, int j, int k) {
+        const int64_t iL = D.V.idx(i - sh[0], j - sh[1], k - sh[2]);
+        const int64_t iR = D.V.idx(i, j, k);
+        const int64_t f = rho_avg.idx(i, j, k);
+        rho_avg.v[f] = double(1) / double(2) * (D.V.c(0)[iL] + D.V.c(0)[iR]);
+        c_avg.v[f] = double(1) / double(2) * (D.c.c(0)[iL] + D.c.c(0)[iR]);
+    });
+
+    // Characteristic variables for the six stencil offsets m-3, m = 0..5 (driver WCNS56:886-893).
+    std::vector<FaceArray> W(6 * neq);
+    for (int mm = 0; mm < 6; mm++) {
+        for (int e = 0; e < neq; e++) W[mm * neq + e].init(n, a, 1);
+        const int o = mm - 3;
+        for_faces([&](int i, int j, int k) {
+            const int64_t ic = D.V.idx(i + o * sh[0], j + o * sh[1], k + o * sh[2]);
+            const int64_t f = rho_avg.idx(i, j, k);
+            double V[5], Wc[5];
+            for (int e = 0; e < neq; e++) V[e] = D.V.c(e)[ic];
+            char_from_prim_single(d, a, rho_avg.v[f], c_avg.v[f], V, Wc);
+            for (int e = 0; e < neq; e++) W[mm * neq + e].v[f] = Wc[e];
+        });
+    }
+
+    // WENO interpolation of each characteristic field (WCNS6-LD:459-954 / WCNS5-JS:234-737).
+    std::vector<FaceArray> Wm(neq), Wp(neq);
+    for (int e = 0; e < neq; e++) {
+        Wm[e].init(n, a, 1);
+        Wp[e].init(n, a, 1);
+        for_faces([&](int i, int j, int k) {
+            const int64_t f = rho_avg.idx(i, j, k);
+            Stencil6 s{W[0 * neq + e].v[f], W[1 * neq + e].v[f], W[2 * neq + e].v[f],
+                       W[3 * neq + e].v[f], W[4 * neq + e].v[f], W[5 * neq + e].v[f]};
+            if (prm.scheme == 1) {
+                const double sig = wcns6ld_sigma(s);
+                Wm[e].v[f] = wcns6ld_side(s, sig, prm.p, prm.q, prm.C, prm.alpha_tau, prm.mutation);
+                Wp[e].v[f] = wcns6ld_side(mirror(s), sig, prm.p, prm.q, prm.C, prm.alpha_tau, prm.mutation);
+            } else {
+                Wm[e].v[f] = wcns5js_side(s, prm.p, prm.mutation);
+                Wp[e].v[f] = wcns5js_side(mirror(s), prm.p, prm.mutation);
+            }
+        });
+    }
+
+    // Back to primitive variables.
+    std::vector<FaceArray> Vm(neq), Vp(neq);
+    for (int e = 0; e < neq; e++) {
+        Vm[e].init(n, a, 1);
+        Vp[e].init(n, a, 1);
+    }
+    for_faces([&](int i, int j, int k) {
+        const int64_t f = rho_avg.idx(i, j, k);
+        double Wc[5], V[5];
+        for (int e = 0; e < neq; e++) Wc[e] = Wm[e].v[f];
+        prim_from_char_single(d, a, rho_avg.v[f], c_avg.v[f], Wc, V);
+        for (int e = 0; e < neq; e++) Vm[e].v[f] = V[e];
+        for (int e = 0; e < neq; e++) Wc[e] = Wp[e].v[f];
+        prim_from_char_single(d, a, rho_avg.v[f], c_avg.v[f], Wc, V);
+        for (int e = 0; e < neq; e++) Vp[e].v[f] = V[e];
+    });
+
+    // Bounds check and first-order fallback (FlowModelBasicUtilitiesSingleSpecies.cpp:3013-3439,
+    // driver WCNS56:947-1032): rho > 0 and p > 0 on both sides.
+    for_faces([&](int i, int j, int k) {
+        const int64_t f = rho_avg.idx(i, j, k);
+        int fm = 1, fp = 1;
+        fm &= (Vm[0].v[f] > double(0)) ? 1 : 0;
+        fm &= (Vm[neq - 1].v[f] > double(0)) ? 1 : 0;
+        fp &= (Vp[0].v[f] > double(0)) ? 1 : 0;
+        fp &= (Vp[neq - 1].v[f] > double(0)) ? 1 : 0;
+        if (prm.mutation == 3) fm = fp = 1;  // planted mistake: bounds check dropped
+        if (fm == 0 || fp == 0) {
+            const int64_t iL = D.V.idx(i - sh[0], j - sh[1], k - sh[2]);
+            const int64_t iR = D.V.idx(i, j, k);
+            for (int e = 0; e < neq; e++) {
+                Vm[e].v[f] = D.V.c(e)[iL];
+                Vp[e].v[f] = D.V.c(e)[iR];
+            }
+        }
+    });
+
+    // Riemann solvers and sensor select.
+    FaceArray Fmid;
+    Fmid.init(n, a, neq);
+    for_faces([&](int i, int j, int k) {
+        const int64_t f = rho_avg.idx(i, j, k);
+        double VL[5], VR[5];
+        for (int e = 0; e < neq; e++) {
+            VL[e] = Vm[e].v[f];
+            VR[e] = Vp[e].v[f];
+        }
+        bool use_hll = false;
+        if (d > 1) {
+            // Ducros-type sensor from the cells either side (WCNS56:1066-1088).
+            const int64_t tL = theta.idx(i - sh[0], j - sh[1], k - sh[2]);
+            const int64_t tR = theta.idx(i, j, k);
+            const double th_avg = 0.5 * (theta.c(0)[tL] + theta.c(0)[tR]);
+            const double om_avg = 0.5 * (Omega.c(0)[tL] + Omega.c(0)[tR]);
+            const double sensor = -th_avg / (std::fabs(th_avg) + om_avg + EPS);
+            use_hll = sensor > 0.65;
+        }
+        double Fo[5];
+        riemann_face_single(d, a, gam, use_hll, VL, VR, Fo, nullptr);
+        for (int e = 0; e < neq; e++) Fmid.c(e)[f] = Fo[e];
+    });
+
+    // Midpoint-to-face sixth-order flux, scaled by dt (WCNS56:1197-1282, 2327-2490).
+    int fext[3];
+    for (int b = 0; b < 3; b++) fext[b] = (b == a) ? n[b] + 1 : n[b];
+    const int64_t fvol = int64_t(fext[0]) * fext[1] * fext[2];
+    for (int e = 0; e < neq; e++) {
+        const double* Fm = Fmid.c(e);
+        const double* Fn = D.Fn[a].c(e);
+        double* out = F_out + e * fvol;
+        for (int k = 0; k < fext[2]; k++)
+            for (int j = 0; j < fext[1]; j++)
+                for (int i = 0; i < fext[0]; i++) {
+                    const int64_t of = i + int64_t(j) * fext[0] + int64_t(k) * fext[0] * fext[1];
+                    const double mR = Fm[Fmid.idx(i + sh[0], j + sh[1], k + sh[2])];
+                    const double mL = Fm[Fmid.idx(i - sh[0], j - sh[1], k - sh[2])];
+                    const double mC = Fm[Fmid.idx(i, j, k)];
+                    const double nR = Fn[D.V.idx(i, j, k)];
+                    const double nL = Fn[D.V.idx(i - sh[0], j - sh[1], k - sh[2])];
+                    out[of] = dt * (double(1) / double(30) * (mR + mL) - double(3) / double(10) * (nR + nL) +
+                                    double(23) / double(15) * mC);
+                }
+    }
+}
+
+int rhs_patch_impl(const hmo_params* prm, const int* n_in, int g, const double* dx, double dt, const double* Q,
+                   double* const* F, double* S) {
+    (void)S;
+    if (!prm || !n_in || !dx || !Q || !F) return 1;
+    Model m = make_model(*prm);
+    if (m.model != 0) return 2;  // models other than single-species: not in this restatement yet
+    if (g < 4) return 3;
+    int n[3] = {1, 1, 1};
+    for (int a = 0; a < m.dim; a++) {
+        n[a] = n_in[a];
+        if (n[a] < 1) return 4;
+    }
+    Derived D;
+    compute_derived_single(m, n, g, Q, D);
+    CellArray theta, Omega;
+    compute_theta_omega(m, n, D, dx, theta, Omega);
+    for (int a = 0; a < m.dim; a++) direction_pass_single(*prm, m, n, a, D, theta, Omega, dt, F[a]);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hmo_num_eqn(const hmo_params* prm) {
+    Model m = make_model(*prm);
+    return m.neq;
+}
+
+// ConvectiveFluxReconstructorWCNS56::computeConvectiveFluxAndSourceOnPatch
+// (WCNS56-HLLC-HLL.cpp:39-2656) for one patch with host arrays.
+// F0/F1/F2 are overwritten; S is accumulated for ADVECTIVE equations only.
+int hmo_rhs_patch(const hmo_params* prm, const int* n, int g, const double* dx, double dt, const double* Q,
+                  double* F0, double* F1, double* F2, double* S) {
+    double* F[3] = {F0, F1, F2};
+    return rhs_patch_impl(prm, n, g, dx, dt, Q, F, S);
+}
+
+typedef struct hmo_patch {
+    int n[3];
+    int g;
+    double dx[3];
+    const double* Q;
+    double* F[3];
+    double* S;
+} hmo_patch;
+
+// Many independent patches, parallel over patches with OpenMP (the analogue of
+// one MPI rank per core, halo exchange excluded).  Returns the first nonzero
+// status, or 0.
+int hmo_rhs_batch(const hmo_params* prm, int npatch, const hmo_patch* patches, double dt, int nthreads) {
+    int status = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : status)
+#endif
+    for (int ip = 0; ip < npatch; ip++) {
+        const hmo_patch& P = patches[ip];
+        double* F[3] = {P.F[0], P.F[1], P.F[2]};
+        const int st = rhs_patch_impl(prm, P.n, P.g, P.dx, dt, P.Q, F, P.S);
+        if (st > status) status = st;
+    }
+    return status;
+}
+
+// ---- pointwise entry points (used only to pin the oracle in tests) ----
+
+// Both sides of one characteristic field: WCNS6-LD (scheme 1) or WCNS5-JS (0).
+void hmo_weno_pair(const hmo_params* prm, const double* U, double* um, double* up) {
+    Stencil6 s{U[0], U[1], U[2], U[3], U[4], U[5]};
+    if (prm->scheme == 1) {
+        const double sig = wcns6ld_sigma(s);
+        *um = wcns6ld_side(s, sig, prm->p, prm->q, prm->C, prm->alpha_tau, prm->mutation);
+        *up = wcns6ld_side(mirror(s), sig, prm->p, prm->q, prm->C, prm->alpha_tau, prm->mutation);
+    } else {
+        *um = wcns5js_side(s, prm->p, prm->mutation);
+        *up = wcns5js_side(mirror(s), prm->p, prm->mutation);
+    }
+}
+
+// Smoothness indicators beta_0..beta_3 of the minus side.
+void hmo_smoothness(const double* U, double* beta) {
+    Stencil6 s{U[0], U[1], U[2], U[3], U[4], U[5]};
+    smoothness3(s, beta[0], beta[1], beta[2]);
+    beta[3] = smoothness6(s);
+}
+
+// Characteristic round trip of one primitive state (single species).
+void hmo_char_roundtrip_single(int dim, int axis, double ra, double ca, const double* V, double* W, double* Vback) {
+    char_from_prim_single(dim, axis, ra, ca, V, W);
+    prim_from_char_single(dim, axis, ra, ca, W, Vback);
+}
+
+// Midpoint flux of one face from left/right primitive states (single species).
+void hmo_riemann_single(int dim, int axis, double gamma, int use_hll, const double* VL, const double* VR, double* F,
+                        double* ustar) {
+    riemann_face_single(dim, axis, gamma, use_hll != 0, VL, VR, F, ustar);
+}
+
+int hmo_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

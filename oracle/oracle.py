@@ -1,0 +1,232 @@
+"""ctypes wrapper around the CPU oracle (``libhamers_oracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the CPU-baseline leg of ``bench.py``, never by the product package.
+
+Array convention (numpy, C order): a d-dimensional component-major SAMRAI
+array is a numpy array of shape ``(ncomp, n_{d-1}, ..., n_0)`` -- the last axis
+is x, so x is the fastest-varying index as in SAMRAI CellData/SideData.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libhamers_oracle.so")
+
+SINGLE_SPECIES, FOUR_EQN_CONSERVATIVE, FIVE_EQN_ALLAIRE = 0, 1, 2
+WCNS5_JS_HLLC_HLL, WCNS6_LD_HLLC_HLL = 0, 1
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int),
+        ("model", ctypes.c_int),
+        ("num_species", ctypes.c_int),
+        ("scheme", ctypes.c_int),
+        ("p", ctypes.c_int),
+        ("q", ctypes.c_int),
+        ("C", ctypes.c_double),
+        ("alpha_tau", ctypes.c_double),
+        ("gamma", ctypes.c_double * 8),
+        ("R", ctypes.c_double * 8),
+        ("mutation", ctypes.c_int),
+    ]
+
+
+class _Patch(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int * 3),
+        ("g", ctypes.c_int),
+        ("dx", ctypes.c_double * 3),
+        ("Q", ctypes.c_void_p),
+        ("F", ctypes.c_void_p * 3),
+        ("S", ctypes.c_void_p),
+    ]
+
+
+@dataclass
+class Params:
+    dim: int = 2
+    model: int = SINGLE_SPECIES
+    num_species: int = 1
+    scheme: int = WCNS6_LD_HLLC_HLL
+    p: int = 2
+    q: int = 4
+    C: float = 1.0e9
+    alpha_tau: float = 35.0
+    gamma: list = field(default_factory=lambda: [1.4])
+    R: list = field(default_factory=lambda: [1.0])
+    mutation: int = 0
+
+    def to_c(self) -> _Params:
+        c = _Params()
+        c.dim, c.model, c.num_species, c.scheme = self.dim, self.model, self.num_species, self.scheme
+        c.p, c.q, c.C, c.alpha_tau = self.p, self.q, self.C, self.alpha_tau
+        for i, gv in enumerate(self.gamma[:8]):
+            c.gamma[i] = gv
+        for i, rv in enumerate(self.R[:8]):
+            c.R[i] = rv
+        c.mutation = self.mutation
+        return c
+
+    @property
+    def neq(self) -> int:
+        if self.model == SINGLE_SPECIES:
+            return self.dim + 2
+        if self.model == FIVE_EQN_ALLAIRE:
+            return self.dim + 2 * self.num_species
+        return self.num_species + self.dim + 1
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (idempotent)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.hmo_rhs_patch.restype = ctypes.c_int
+        L.hmo_rhs_patch.argtypes = [
+            ctypes.POINTER(_Params), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+            ctypes.POINTER(ctypes.c_double), ctypes.c_double, ctypes.c_void_p,
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+        ]
+        L.hmo_rhs_batch.restype = ctypes.c_int
+        L.hmo_rhs_batch.argtypes = [
+            ctypes.POINTER(_Params), ctypes.c_int, ctypes.POINTER(_Patch), ctypes.c_double, ctypes.c_int,
+        ]
+        L.hmo_max_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def face_shape(neq: int, n: tuple, axis: int) -> tuple:
+    """numpy shape of the axis-``axis`` flux SideData (0 ghosts)."""
+    ext = [n[a] + (1 if a == axis else 0) for a in range(len(n))]
+    return (neq,) + tuple(reversed(ext))
+
+
+def rhs_patch(prm: Params, Q: np.ndarray, n: tuple, g: int, dx: tuple, dt: float, S: np.ndarray | None = None):
+    """computeConvectiveFluxAndSourceOnPatch on one patch.
+
+    ``Q`` has shape ``(neq, *(n[a]+2g reversed))``.  Returns ``(F, S)`` where F
+    is a list of per-axis face arrays (overwritten) and S the source (accumulated).
+    """
+    d = prm.dim
+    neq = prm.neq
+    n = tuple(int(v) for v in n)
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    assert Q.shape == (neq,) + tuple(reversed([v + 2 * g for v in n])), Q.shape
+    F = [np.zeros(face_shape(neq, n, a)) for a in range(d)]
+    if S is None:
+        S = np.zeros((neq,) + tuple(reversed(n)))
+    nn = (ctypes.c_int * 3)(*(list(n) + [1] * (3 - d)))
+    dd = (ctypes.c_double * 3)(*(list(dx) + [1.0] * (3 - d)))
+    cp = prm.to_c()
+    st = lib().hmo_rhs_patch(ctypes.byref(cp), nn, g, dd, dt, _ptr(Q),
+                             _ptr(F[0]), _ptr(F[1]) if d > 1 else None, _ptr(F[2]) if d > 2 else None, _ptr(S))
+    if st != 0:
+        raise RuntimeError(f"oracle hmo_rhs_patch failed with status {st}")
+    return F, S
+
+
+def rhs_batch(prm: Params, patches: list, dt: float, nthreads: int = 0):
+    """Many patches in parallel (OpenMP).  ``patches`` is a list of dicts with
+    keys Q, n, g, dx, F (list), S -- arrays are written in place."""
+    arr = (_Patch * len(patches))()
+    keep = []
+    d = prm.dim
+    for i, P in enumerate(patches):
+        c = arr[i]
+        n = list(P["n"]) + [1] * (3 - d)
+        for a in range(3):
+            c.n[a] = n[a]
+        c.g = P["g"]
+        dx = list(P["dx"]) + [1.0] * (3 - d)
+        for a in range(3):
+            c.dx[a] = dx[a]
+        c.Q = _ptr(P["Q"])
+        for a in range(3):
+            c.F[a] = _ptr(P["F"][a]) if a < d else None
+        c.S = _ptr(P["S"])
+        keep.append(P)
+    cp = prm.to_c()
+    st = lib().hmo_rhs_batch(ctypes.byref(cp), len(patches), arr, dt, nthreads)
+    if st != 0:
+        raise RuntimeError(f"oracle hmo_rhs_batch failed with status {st}")
+
+
+def max_threads() -> int:
+    return lib().hmo_max_threads()
+
+
+# ---- pointwise entry points (pinning tests only) ----
+
+def _arr(x, n=None):
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    if n is not None:
+        assert a.size == n
+    return a
+
+
+def weno_pair(prm: Params, U) -> tuple:
+    L = lib()
+    L.hmo_weno_pair.argtypes = [ctypes.POINTER(_Params), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    U = _arr(U, 6)
+    um, up = ctypes.c_double(), ctypes.c_double()
+    cp = prm.to_c()
+    L.hmo_weno_pair(ctypes.byref(cp), U.ctypes.data, ctypes.byref(um), ctypes.byref(up))
+    return um.value, up.value
+
+
+def smoothness(U) -> np.ndarray:
+    L = lib()
+    L.hmo_smoothness.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    U = _arr(U, 6)
+    b = np.zeros(4)
+    L.hmo_smoothness(U.ctypes.data, b.ctypes.data)
+    return b
+
+
+def char_roundtrip_single(dim: int, axis: int, rho_avg: float, c_avg: float, V):
+    L = lib()
+    L.hmo_char_roundtrip_single.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    V = _arr(V, dim + 2)
+    Wc = np.zeros(dim + 2)
+    Vb = np.zeros(dim + 2)
+    L.hmo_char_roundtrip_single(dim, axis, rho_avg, c_avg, V.ctypes.data, Wc.ctypes.data, Vb.ctypes.data)
+    return Wc, Vb
+
+
+def riemann_single(dim: int, axis: int, gamma: float, use_hll: bool, VL, VR):
+    L = lib()
+    L.hmo_riemann_single.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    VL = _arr(VL, dim + 2)
+    VR = _arr(VR, dim + 2)
+    F = np.zeros(dim + 2)
+    us = ctypes.c_double()
+    L.hmo_riemann_single(dim, axis, gamma, int(use_hll), VL.ctypes.data, VR.ctypes.data, F.ctypes.data,
+                         ctypes.byref(us))
+    return F, us.value

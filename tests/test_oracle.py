@@ -1,0 +1,306 @@
+"""Pins the CPU oracle against what the reference and the mathematics fix.
+
+No golden vectors exist in the reference for this path (its only numerical
+assertions are the convergence rates of tests/*/convergence_test.py), so each
+piece of the oracle is checked against an independent closed form, an
+invariant, or the reference's own convergence assertions.  Planted mistakes
+(``Params.mutation``) show that the checks catch plausible slips.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from hamers_amd import workloads as W
+from util import oracle_rk3, oracle_rhs_periodic
+
+RNG = np.random.default_rng(20240601)
+
+
+# ---------------------------------------------------------------------------
+# Smoothness indicators: the expanded forms (WCNS6-LD-HLLC-HLL.cpp:47-84)
+# equal the Jiang-Shu definition sum_l int_cell (d^l p / dx^l)^2 dx of the
+# Lagrange interpolant of each sub-stencil over the cell of U2.
+# ---------------------------------------------------------------------------
+
+def _beta_definition(xs, ys, a=1.5, b=2.5):
+    c = np.polyfit(xs, ys, len(xs) - 1)
+    tot = 0.0
+    for _ in range(1, len(xs)):
+        c = np.polyder(c)
+        I = np.polyint(np.polymul(c, c))
+        tot += np.polyval(I, b) - np.polyval(I, a)
+    return tot
+
+
+@pytest.mark.parametrize("trial", range(8))
+def test_smoothness_indicators_match_definition(trial):
+    U = RNG.normal(size=6) * 10.0 ** RNG.uniform(-3, 3)
+    b = O.smoothness(U)
+    ref = [
+        _beta_definition([0, 1, 2], U[0:3]),
+        _beta_definition([1, 2, 3], U[1:4]),
+        _beta_definition([2, 3, 4], U[2:5]),
+        _beta_definition([0, 1, 2, 3, 4, 5], U),
+    ]
+    for k in range(4):
+        assert b[k] == pytest.approx(ref[k], rel=1e-9, abs=1e-12 * max(1.0, np.max(np.abs(U)) ** 2))
+
+
+def test_smoothness_jiang_shu_closed_form():
+    U = RNG.normal(size=6)
+    b = O.smoothness(U)
+    js0 = 13 / 12 * (U[0] - 2 * U[1] + U[2]) ** 2 + 1 / 4 * (U[0] - 4 * U[1] + 3 * U[2]) ** 2
+    js1 = 13 / 12 * (U[1] - 2 * U[2] + U[3]) ** 2 + 1 / 4 * (U[1] - U[3]) ** 2
+    js2 = 13 / 12 * (U[2] - 2 * U[3] + U[4]) ** 2 + 1 / 4 * (3 * U[2] - 4 * U[3] + U[4]) ** 2
+    assert np.allclose(b[:3], [js0, js1, js2], rtol=1e-12, atol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# WENO interpolation.
+# ---------------------------------------------------------------------------
+
+LD = O.Params(dim=1, scheme=O.WCNS6_LD_HLLC_HLL)
+JS = O.Params(dim=1, scheme=O.WCNS5_JS_HLLC_HLL)
+
+
+@pytest.mark.parametrize("prm", [LD, JS], ids=["WCNS6_LD", "WCNS5_JS"])
+def test_constant_data_is_reproduced(prm):
+    for c in (1.0, -3.25, 1e-8, 7.5e5):
+        um, up = O.weno_pair(prm, [c] * 6)
+        assert um == pytest.approx(c, rel=1e-15)
+        assert up == pytest.approx(c, rel=1e-15)
+
+
+@pytest.mark.parametrize("prm", [LD, JS], ids=["WCNS6_LD", "WCNS5_JS"])
+def test_quadratic_data_interpolated_exactly(prm):
+    # Every candidate stencil is exact for quadratics and the weights sum to
+    # one, so both sides give p(2.5) for any nonlinear weights.
+    for _ in range(5):
+        a, b, c = RNG.normal(size=3)
+        U = [a + b * k + c * k * k for k in range(6)]
+        exact = a + b * 2.5 + c * 2.5 ** 2
+        um, up = O.weno_pair(prm, U)
+        assert um == pytest.approx(exact, rel=1e-12, abs=1e-13)
+        assert up == pytest.approx(exact, rel=1e-12, abs=1e-13)
+
+
+def test_wcns6ld_central_limit_is_sixth_order_interpolant():
+    # With C -> infinity the weights are the linear ones (1,15,15,1)/32 and the
+    # interpolant is (3,-25,150,150,-25,3)/256: exact for quintics.
+    prm = O.Params(dim=1, scheme=O.WCNS6_LD_HLLC_HLL, C=1e300, alpha_tau=1e300)
+    for _ in range(5):
+        c = RNG.normal(size=6)
+        U = [np.polyval(c, k) for k in range(6)]
+        um, up = O.weno_pair(prm, U)
+        exact = np.polyval(c, 2.5)
+        lin = np.dot([3, -25, 150, 150, -25, 3], U) / 256.0
+        assert lin == pytest.approx(exact, rel=1e-10, abs=1e-11)
+        assert um == pytest.approx(exact, rel=1e-10, abs=1e-11)
+        assert up == pytest.approx(exact, rel=1e-10, abs=1e-11)
+
+
+def test_wcns6ld_planted_mistake_is_caught():
+    prm = O.Params(dim=1, scheme=O.WCNS6_LD_HLLC_HLL, C=1e300, alpha_tau=1e300, mutation=1)
+    c = np.array([0.3, -0.2, 0.5, 1.0, -0.7, 2.0])
+    U = [np.polyval(c, k) for k in range(6)]
+    um, _ = O.weno_pair(prm, U)
+    assert abs(um - np.polyval(c, 2.5)) > 1e-3
+
+
+def test_wcns5js_linear_data_gives_linear_weights():
+    # Linear data: all three indicators are equal, the weights are the linear
+    # ones (1/16, 5/8, 5/16) and the interpolant is (3,-20,90,60,-5)/128.
+    for _ in range(5):
+        a, b = RNG.normal(size=2)
+        U = np.array([a + b * k for k in range(6)])
+        um, up = O.weno_pair(JS, U)
+        assert um == pytest.approx(np.dot([3, -20, 90, 60, -5], U[:5]) / 128.0, rel=1e-12, abs=1e-13)
+        assert up == pytest.approx(np.dot([3, -20, 90, 60, -5], U[::-1][:5]) / 128.0, rel=1e-12, abs=1e-13)
+
+
+def test_wcns5js_planted_mistake_is_caught():
+    c = RNG.normal(size=4)
+    U = np.array([np.polyval(c, k) for k in range(6)])
+    good, _ = O.weno_pair(JS, U)
+    bad, _ = O.weno_pair(O.Params(dim=1, scheme=O.WCNS5_JS_HLLC_HLL, mutation=2), U)
+    assert abs(good - bad) > 1e-6 * max(1.0, abs(good))
+
+
+@pytest.mark.parametrize("prm", [LD, JS], ids=["WCNS6_LD", "WCNS5_JS"])
+def test_plus_side_is_mirror_of_minus_side(prm):
+    for _ in range(20):
+        U = RNG.normal(size=6)
+        um, up = O.weno_pair(prm, U)
+        vm, vp = O.weno_pair(prm, U[::-1].copy())
+        assert up == vm
+        assert um == vp
+
+
+def test_wcns6ld_shock_uses_upwind_weights():
+    # A step: tau6/beta_avg is large, sigma = 1, so the weights are the
+    # upwind ones and the minus side stays bounded by the data (no overshoot).
+    U = [1.0, 1.0, 1.0, 0.0, 0.0, 0.0]
+    um, up = O.weno_pair(LD, U)
+    assert 0.0 <= um <= 1.0 and 0.0 <= up <= 1.0
+    assert um > 0.9  # left-biased value near the left state
+    assert up < 0.1
+
+
+# ---------------------------------------------------------------------------
+# Characteristic projection and Riemann solvers.
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_characteristic_round_trip(dim):
+    for axis in range(dim):
+        for _ in range(5):
+            V = np.concatenate([[RNG.uniform(0.1, 2)], RNG.normal(size=dim), [RNG.uniform(0.1, 2)]])
+            ra, ca = RNG.uniform(0.5, 2.0), RNG.uniform(0.5, 2.0)
+            Wc, Vb = O.char_roundtrip_single(dim, axis, ra, ca, V)
+            assert np.allclose(Vb, V, rtol=1e-14, atol=1e-14)
+            # the acoustic fields are the left eigenvectors of the primitive Jacobian
+            un, p = V[1 + axis], V[dim + 1]
+            assert Wc[0] == pytest.approx(-0.5 * ra * ca * un + 0.5 * p, rel=1e-15)
+            assert Wc[dim + 1] == pytest.approx(0.5 * ra * ca * un + 0.5 * p, rel=1e-15)
+
+
+def _phys_flux(dim, axis, V, gamma):
+    rho, u, p = V[0], V[1:1 + dim], V[dim + 1]
+    E = p / (gamma - 1) + 0.5 * rho * np.dot(u, u)
+    F = np.zeros(dim + 2)
+    F[0] = rho * u[axis]
+    F[1:1 + dim] = rho * u[axis] * u
+    F[1 + axis] += p
+    F[dim + 1] = u[axis] * (E + p)
+    return F
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("use_hll", [False, True])
+def test_riemann_consistency(dim, use_hll):
+    for axis in range(dim):
+        V = np.concatenate([[1.3], RNG.normal(size=dim) * 0.4, [0.8]])
+        F, us = O.riemann_single(dim, axis, 1.4, use_hll, V, V)
+        assert np.allclose(F, _phys_flux(dim, axis, V, 1.4), rtol=1e-13, atol=1e-14)
+        assert us == pytest.approx(V[1 + axis], rel=1e-13, abs=1e-14)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_riemann_supersonic_upwinding(dim):
+    VL = np.concatenate([[1.0], [3.0] + [0.1] * (dim - 1), [1.0]])
+    VR = np.concatenate([[0.5], [2.8] + [-0.2] * (dim - 1), [0.7]])
+    for use_hll in (False, True):
+        F, _ = O.riemann_single(dim, 0, 1.4, use_hll, VL, VR)
+        assert np.allclose(F, _phys_flux(dim, 0, VL, 1.4), rtol=1e-13)
+        F, _ = O.riemann_single(dim, 0, 1.4, use_hll, VR[::1] * np.array([1, -1] + [1] * dim), VL * np.array([1, -1] + [1] * dim))
+        assert np.allclose(F, _phys_flux(dim, 0, VL * np.array([1, -1] + [1] * dim), 1.4), rtol=1e-13)
+
+
+def test_hllc_resolves_stationary_contact_exactly():
+    VL = np.array([1.0, 0.0, 0.3, 1.0])
+    VR = np.array([0.125, 0.0, -0.2, 1.0])
+    F, us = O.riemann_single(2, 0, 1.4, False, VL, VR)
+    assert np.allclose(F, [0.0, 1.0, 0.0, 0.0], atol=1e-15)
+    assert us == 0.0
+
+
+def test_hllc_hll_pure_normal_jump_is_hllc():
+    # Velocity jump purely normal to the face: alpha_1 = 1, beta_1 = 1.
+    VL = np.array([1.0, 0.5, 0.2, 1.0])
+    VR = np.array([0.6, -0.3, 0.2, 0.5])
+    F1, _ = O.riemann_single(2, 0, 1.4, False, VL, VR)
+    F2, _ = O.riemann_single(2, 0, 1.4, True, VL, VR)
+    assert np.allclose(F1, F2, rtol=1e-15, atol=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# Whole-patch invariants.
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_uniform_state_gives_dt_times_physical_flux(dim, scheme):
+    prm = O.Params(dim=dim, scheme=scheme)
+    n = (6,) * dim
+    g = 4
+    vel = [0.3, -0.2, 0.1][:dim]
+    V = np.array([1.2] + vel + [0.9])
+    Qc = W.conservative_from_primitive_single(np.array(V[0]), [np.array(v) for v in vel], np.array(V[-1]), 1.4)
+    Q = np.empty((dim + 2,) + tuple([6 + 2 * g] * dim))
+    for e in range(dim + 2):
+        Q[e] = Qc[e]
+    dt = 0.37
+    F, S = O.rhs_patch(prm, Q, n, g, (0.1,) * dim, dt)
+    for a in range(dim):
+        Fp = dt * _phys_flux(dim, a, V, 1.4)
+        for e in range(dim + 2):
+            assert np.allclose(F[a][e], Fp[e], rtol=1e-13, atol=1e-15)
+    assert np.all(S == 0.0)
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_periodic_rhs_conserves(scheme):
+    Q, n, dx = W.four_quadrant_riemann(N=32, seed=5)
+    prm = O.Params(dim=2, scheme=scheme)
+    R, F, S = oracle_rhs_periodic(prm, Q, n, dx, 1e-3)
+    assert np.all(np.isfinite(R))
+    for e in range(4):
+        assert abs(R[e].sum()) <= 1e-12 * np.abs(R[e]).sum()
+
+
+def positivity_case(N=16, seed=3):
+    """Density and pressure log-uniform over six decades: high-order face
+    states go non-positive on some faces and must fall back to first order."""
+    rng = np.random.default_rng(seed)
+    rho = 10 ** rng.uniform(-6, 0, (N, N))
+    p = 10 ** rng.uniform(-6, 0, (N, N))
+    u = rng.normal(size=(N, N))
+    return W.conservative_from_primitive_single(rho, [u, np.zeros_like(rho)], p, 1.4)
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_positivity_fallback_keeps_rhs_finite(scheme):
+    N = 16
+    Q = positivity_case(N)
+    R, _, _ = oracle_rhs_periodic(O.Params(dim=2, scheme=scheme), Q, (N, N), (1 / N, 1 / N), 1e-3)
+    assert np.all(np.isfinite(R))
+    with np.errstate(invalid="ignore"):
+        Rb, _, _ = oracle_rhs_periodic(O.Params(dim=2, scheme=scheme, mutation=3), Q, (N, N), (1 / N, 1 / N), 1e-3)
+    assert not np.array_equal(R, Rb)  # the fallback was exercised
+
+
+# ---------------------------------------------------------------------------
+# The reference's own assertions: L2 convergence rate of the periodic density
+# wave (tests/2D_convergence_test_single_species/convergence_test.py:13-23, 281;
+# problems/Euler/initial_conditions/ConvergenceSingleSpecies.cpp:77-101;
+# problems/Euler/error_statistics/ConvergenceSingleSpecies.cpp:84-138).
+# ---------------------------------------------------------------------------
+
+def _l2_rates(prm, levels=4):
+    errs = []
+    for lev in range(levels):
+        N = 8 * 2 ** lev
+        Q0, n, dx = W.density_wave_convergence(N, dim=2)
+        dt = 0.001 * dx[0]
+        nsteps = 8 * 2 ** lev
+        Q = oracle_rk3(prm, Q0, n, dx, dt, nsteps)
+        X = W.cell_centres(n, (-1.0, -1.0), dx)
+        exact = 1.0 + 0.5 * np.sin(np.pi * (X[0] + X[1] - 2.0 * dt * nsteps))
+        e = np.abs(exact - Q[0])
+        errs.append(np.sqrt(np.sum(dx[0] * dx[1] * e * e) / (N * N * dx[0] * dx[1])))
+    return [np.log2(errs[i - 1] / errs[i]) for i in range(1, len(errs))]
+
+
+def test_convergence_rate_wcns6ld_2d():
+    rates = _l2_rates(O.Params(dim=2, scheme=O.WCNS6_LD_HLLC_HLL))
+    assert rates[-1] > 5.8
+
+
+def test_convergence_rate_wcns5js_2d():
+    rates = _l2_rates(O.Params(dim=2, scheme=O.WCNS5_JS_HLLC_HLL))
+    assert rates[-1] > 4.8
+
+
+def test_convergence_catches_dropped_stencil_point():
+    rates = _l2_rates(O.Params(dim=2, scheme=O.WCNS6_LD_HLLC_HLL, mutation=1), levels=3)
+    assert rates[-1] < 5.8

@@ -1,0 +1,61 @@
+"""Shared helpers of the test suite (oracle-side reference computations)."""
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import oracle as O
+from hamers_amd import workloads as W
+
+# SSP-RK3 weights of RungeKuttaLevelIntegrator.cpp:3907-3924.
+RK_ALPHA = [[1.0], [0.75, 0.25], [1.0 / 3.0, 0.0, 2.0 / 3.0]]
+RK_BETA = [[1.0], [0.0, 0.25], [0.0, 0.0, 2.0 / 3.0]]
+
+
+def flux_divergence(F, S, dx):
+    """-(sum_a (F_a,R - F_a,L)/dx_a) + S, with the reference's association
+    (Euler.cpp:1212, 1344-1347, 1544-1548)."""
+    d = len(F)
+    acc = None
+    for a in range(d):
+        Fa = F[a]
+        ax = Fa.ndim - 1 - a
+        R = [slice(None)] * Fa.ndim
+        Lf = [slice(None)] * Fa.ndim
+        R[ax] = slice(1, None)
+        Lf[ax] = slice(0, -1)
+        t = (Fa[tuple(R)] - Fa[tuple(Lf)]) / dx[a]
+        acc = -t if acc is None else acc - t
+    return acc + S
+
+
+def oracle_rhs_periodic(prm, Qi, n, dx, dt, g=4):
+    """Oracle RHS (flux divergence, dt-scaled) on a periodic domain treated as one patch."""
+    Qg = W.fill_periodic(Qi, g)
+    F, S = O.rhs_patch(prm, Qg, n, g, dx, dt)
+    return flux_divergence(F, S, dx), F, S
+
+
+def oracle_rk3(prm, Qi, n, dx, dt, nsteps):
+    """SSP-RK3 steps on a periodic domain with the oracle RHS; the stage
+    update follows Euler::advanceSingleStepOnPatch (Euler.cpp:1003-1680)."""
+    Q = Qi.copy()
+    for _ in range(nsteps):
+        Qint, R = [], []
+        for sn in range(3):
+            Qint.append(Q.copy())
+            R.append(oracle_rhs_periodic(prm, Q, n, dx, dt)[0])
+            Qn = np.zeros_like(Q)
+            for m in range(sn + 1):
+                if RK_ALPHA[sn][m] != 0.0:
+                    Qn = Qn + RK_ALPHA[sn][m] * Qint[m]
+                if RK_BETA[sn][m] != 0.0:
+                    Qn = Qn + RK_BETA[sn][m] * R[m]
+            Q = Qn
+    return Q
+
+
+def max_rel_err(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b)) / scale)
