@@ -51,7 +51,7 @@ def main():
     import numpy as np
     import torch
 
-    from hamers_amd import ConvectiveFluxReconstructor, PatchData, capi
+    from hamers_amd import ConvectiveFluxReconstructor, PatchData, PatchList, capi
     from hamers_amd import workloads as W
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -72,8 +72,10 @@ def main():
     dt = 1.0e-4
     stream = torch.cuda.current_stream()
 
+    plist = PatchList(pds, neq)  # the level's patch list, built once as a C++ caller would hold it
+
     def step_batched():
-        rec.computeConvectiveFluxAndSourceOnPatches(pds, 0.0, dt, 0, stream)
+        rec.computeConvectiveFluxAndSourceOnPatches(plist, 0.0, dt, 0, stream)
 
     # Per-patch path: views prebuilt so the loop is one C call per patch.
     views = [pd.view(neq) for pd in pds]
@@ -130,16 +132,16 @@ def main():
         nthreads = min(16, os.cpu_count() or 1)
         prm = O.Params(dim=2, scheme=1 if args.scheme.startswith("WCNS6") else 0)
 
-        plist = [{"Q": Qp, "n": (P, P), "g": g, "dx": dx,
+        cpu_list = [{"Q": Qp, "n": (P, P), "g": g, "dx": dx,
                   "F": [np.empty((neq, P, P + 1)), np.empty((neq, P + 1, P))],
                   "S": np.zeros((neq, P, P))} for _, Qp in patches]
         t0 = time.perf_counter()
-        O.rhs_batch(prm, plist, dt, nthreads)        # one pass over the whole level, to size the sample
+        O.rhs_batch(prm, cpu_list, dt, nthreads)        # one pass over the whole level, to size the sample
         t1 = time.perf_counter() - t0
         reps = max(1, int(round(args.cpu_seconds / max(t1, 1e-3))))
         t0 = time.perf_counter()
         for _ in range(reps):
-            O.rhs_batch(prm, plist, dt, nthreads)
+            O.rhs_batch(prm, cpu_list, dt, nthreads)
         tk = time.perf_counter() - t0
         cpu = {"value": reps * cells / tk, "unit": "cell-RHS-evals/s", "cores": nthreads, "kind": "port",
                "sample": f"the whole 256-patch level, {reps} RHS evaluations ({tk:.1f} s), oracle built -O3 "

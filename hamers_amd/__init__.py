@@ -17,4 +17,5 @@ from .reconstructor import (  # noqa: E402,F401
     ConvectiveFluxReconstructor,
     ConvectiveFluxReconstructorManager,
     PatchData,
+    PatchList,
 )

@@ -55,6 +55,15 @@ class LibraryMissing(RuntimeError):
 
 
 def _load():
+    # torch ships its own HIP runtime (soname libamdhip64.so.7) and links it by
+    # the unversioned name.  Importing torch first lets the dynamic linker bind
+    # this library's libamdhip64.so.7 dependency to that same runtime, so that
+    # device pointers and streams handed over from torch belong to the runtime
+    # the kernels are launched on (one HIP runtime per process).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise LibraryMissing(
             f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "

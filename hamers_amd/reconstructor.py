@@ -83,6 +83,20 @@ def _check_tensor(t):
         raise ValueError("patch arrays must be contiguous float64 tensors on the GPU")
 
 
+class PatchList:
+    """A level's patch list with its C views built once (the arrays' device
+    addresses are captured, so the tensors must stay alive and in place)."""
+
+    def __init__(self, patches, neq: int):
+        self.patches = list(patches)
+        self.views = (capi.hm_patch_view * len(self.patches))()
+        for i, p in enumerate(self.patches):
+            self.views[i] = p.view(neq)
+
+    def __len__(self):
+        return len(self.patches)
+
+
 class ConvectiveFluxReconstructor:
     """A WCNS56 hybrid HLLC / HLLC-HLL convective flux reconstructor on the GPU."""
 
@@ -156,11 +170,9 @@ class ConvectiveFluxReconstructor:
 
     def computeConvectiveFluxAndSourceOnPatches(self, patches, time: float, dt: float, RK_step_number: int = 0,
                                                 stream=None):
-        arr = (capi.hm_patch_view * len(patches))()
-        for i, p in enumerate(patches):
-            arr[i] = p.view(self.num_eqn)
+        plist = patches if isinstance(patches, PatchList) else PatchList(patches, self.num_eqn)
         st = capi.lib.hm_compute_convective_flux_and_source_on_patches(
-            self._ctx, arr, len(patches), time, dt, RK_step_number, self._stream_ptr(stream))
+            self._ctx, plist.views, len(plist), time, dt, RK_step_number, self._stream_ptr(stream))
         capi.check(self._ctx, st)
 
 
