@@ -8,6 +8,7 @@ from . import LIB_PATH
 
 HM_MAX_EQN = 12
 HM_MAX_SPECIES = 8
+HM_NUM_STAGES = 5
 
 HM_OK = 0
 HM_ERR_INVALID_ARGUMENT = 1
@@ -86,6 +87,8 @@ def _load():
     L.hm_compute_convective_flux_and_source_on_patches.argtypes = [
         P, ctypes.POINTER(hm_patch_view), ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, P]
     L.hm_compute_convective_flux_and_source_on_patches.restype = ctypes.c_int
+    L.hm_get_num_conservative_arrays.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
+    L.hm_get_num_conservative_arrays.restype = ctypes.c_int
     L.hm_profile_enable.argtypes = [P, ctypes.c_int]
     L.hm_profile_enable.restype = ctypes.c_int
     L.hm_profile_read.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]

@@ -83,6 +83,14 @@ class ConvectiveFluxReconstructorGPU {
 
     int numEqn() const { return neq_; }
 
+    // Conservative arrays a patch view must provide: the five-equation model
+    // also hands over the last species' volume fraction (stored, not evolved).
+    int numConservativeArrays() const { return cfg_.model == HM_MODEL_FIVE_EQN_ALLAIRE ? neq_ + 1 : neq_; }
+
+    // Scratch components per cell of the 4-ghost box: V, c (and rho for the
+    // five-equation model).
+    int prim_comps() const { return cfg_.model == HM_MODEL_FIVE_EQN_ALLAIRE ? neq_ + 2 : neq_ + 1; }
+
     // ConvectiveFluxReconstructor::getConvectiveFluxNumberOfGhostCells (4 for WCNS56).
     int getConvectiveFluxNumberOfGhostCells() const { return kGc; }
 
@@ -126,6 +134,13 @@ class ConvectiveFluxReconstructorGPU {
             w.start[d] = acc;
             plan.items[stage] = acc;
         }
+        // advective source cells
+        plan.work[4].p = dp;
+        plan.work[4].nseg = 1;
+        plan.work[4].start[0] = 0;
+        plan.work[4].cnt[0] = has_advective_ ? e.vol0 : 0;
+        plan.work[4].start[1] = align_items(plan.work[4].cnt[0]);
+        plan.items[4] = plan.work[4].start[1];
         KParams k = k_;
         k.dt = dt;
         return run(plan, k, s);
@@ -153,7 +168,7 @@ class ConvectiveFluxReconstructorGPU {
     const std::string& lastError() const { return err_; }
 
     // Stage timing with HIP events recorded on the launch stream around every
-    // pipeline stage (0 derived, 1 sensor, 2 midpoint flux, 3 face flux).
+    // pipeline stage (0 derived, 1 sensor, 2 midpoint flux, 3 face flux, 4 source).
     int setProfiling(int on) {
         if (int st = select_device()) return st;
         if (on && prof_events_.empty()) {
@@ -165,7 +180,7 @@ class ConvectiveFluxReconstructorGPU {
         }
         profiling_ = on != 0;
         prof_used_ = 0;
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < kStages; i++) {
             prof_ms_[i] = 0.0;
             prof_n_[i] = 0;
         }
@@ -175,7 +190,7 @@ class ConvectiveFluxReconstructorGPU {
     // Fold recorded event pairs into per-stage totals (waits for them).
     int readProfile(double* ms, int64_t* launches) {
         if (int st = collect_profile()) return st;
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < kStages; i++) {
             ms[i] = prof_ms_[i];
             launches[i] = prof_n_[i];
         }
@@ -189,7 +204,7 @@ class ConvectiveFluxReconstructorGPU {
 
    private:
     struct Extents {
-        int64_t vol4, vol2, mid[3], face[3];
+        int64_t vol4, vol2, vol0, mid[3], face[3];
     };
 
     struct TablePlan {
@@ -214,6 +229,7 @@ class ConvectiveFluxReconstructorGPU {
         };
         e.vol4 = grown(kGc);
         e.vol2 = grown(kGs);
+        e.vol0 = grown(0);
         for (int a = 0; a < 3; a++) {
             e.mid[a] = 0;
             e.face[a] = 0;
@@ -232,7 +248,7 @@ class ConvectiveFluxReconstructorGPU {
     ScratchLayout layout(const hm_patch_view& v) const {
         const Extents e = extents(v);
         ScratchLayout L{};
-        L.prim = align_up(size_t(neq_ + 1) * e.vol4, kAlignDoubles);
+        L.prim = align_up(size_t(prim_comps()) * e.vol4, kAlignDoubles);
         L.sens = (cfg_.dim > 1) ? align_up(size_t(2) * e.vol2, kAlignDoubles) : 0;
         L.total = L.prim + L.sens;
         for (int a = 0; a < 3; a++) {
@@ -251,7 +267,8 @@ class ConvectiveFluxReconstructorGPU {
             p.gq[a] = (a < d) ? v.ghost[a] : 0;
             p.dx[a] = (a < d) ? v.dx[a] : 1.0;
         }
-        for (int e = 0; e < HM_MAX_EQN; e++) p.Q[e] = (e < neq_) ? v.Q[e] : nullptr;
+        const int nq = numConservativeArrays();
+        for (int e = 0; e < HM_MAX_EQN; e++) p.Q[e] = (e < nq) ? v.Q[e] : nullptr;
         for (int a = 0; a < 3; a++) p.F[a] = (a < d) ? v.F[a] : nullptr;
         p.S = v.S;
         double* q = base;
@@ -290,7 +307,7 @@ class ConvectiveFluxReconstructorGPU {
                 return fail(HM_ERR_INVALID_ARGUMENT, buf);
             }
         }
-        for (int e = 0; e < neq_; e++)
+        for (int e = 0; e < numConservativeArrays(); e++)
             if (!v.Q[e]) {
                 snprintf(buf, sizeof buf, "patch %d: conservative variable %d is NULL", which, e);
                 return fail(HM_ERR_INVALID_ARGUMENT, buf);
@@ -352,15 +369,15 @@ class ConvectiveFluxReconstructorGPU {
         // Host image of the tables: DevPatch[np], then per stage start/cnt/seg.
         const int d = cfg_.dim;
         std::vector<DevPatch> patches(np);
-        std::vector<int64_t> start[4], cnt[4];
-        std::vector<int> seg[4];
+        std::vector<int64_t> start[kStages], cnt[kStages];
+        std::vector<int> seg[kStages];
         double* base = arena_;
         for (int i = 0; i < np; i++) {
             const ScratchLayout L = layout(views[i]);
             patches[i] = make_dev_patch(views[i], base, L);
             base += L.total;
         }
-        int64_t acc[4] = {0, 0, 0, 0};
+        int64_t acc[kStages] = {0, 0, 0, 0, 0};
         for (int i = 0; i < np; i++) {
             const Extents e = extents(views[i]);
             auto add = [&](int stage, int axis, int64_t c) {
@@ -374,12 +391,13 @@ class ConvectiveFluxReconstructorGPU {
             if (d > 1) add(1, 0, e.vol2);
             for (int a = 0; a < d; a++) add(2, a, e.mid[a]);
             for (int a = 0; a < d; a++) add(3, a, e.face[a]);
+            if (has_advective_) add(4, 0, e.vol0);
         }
-        for (int st = 0; st < 4; st++) start[st].push_back(acc[st]);
+        for (int st = 0; st < kStages; st++) start[st].push_back(acc[st]);
         // Pack into one buffer.
         size_t bytes = align_up(sizeof(DevPatch) * size_t(np), 256);
-        size_t off_start[4], off_cnt[4], off_seg[4];
-        for (int st = 0; st < 4; st++) {
+        size_t off_start[kStages], off_cnt[kStages], off_seg[kStages];
+        for (int st = 0; st < kStages; st++) {
             off_start[st] = bytes;
             bytes = align_up(bytes + sizeof(int64_t) * start[st].size(), 256);
             off_cnt[st] = bytes;
@@ -389,7 +407,7 @@ class ConvectiveFluxReconstructorGPU {
         }
         std::vector<char> img(bytes, 0);
         std::memcpy(img.data(), patches.data(), sizeof(DevPatch) * size_t(np));
-        for (int st = 0; st < 4; st++) {
+        for (int st = 0; st < kStages; st++) {
             if (!start[st].empty()) std::memcpy(img.data() + off_start[st], start[st].data(), sizeof(int64_t) * start[st].size());
             if (!cnt[st].empty()) std::memcpy(img.data() + off_cnt[st], cnt[st].data(), sizeof(int64_t) * cnt[st].size());
             if (!seg[st].empty()) std::memcpy(img.data() + off_seg[st], seg[st].data(), sizeof(int) * seg[st].size());
@@ -420,7 +438,7 @@ class ConvectiveFluxReconstructorGPU {
         pl->arena_at_build = arena_;
         pl->use = use_clock_;
         char* dv = static_cast<char*>(pl->dev);
-        for (int st = 0; st < 4; st++) {
+        for (int st = 0; st < kStages; st++) {
             SegTable& w = pl->plan.work[st];
             w.patches = reinterpret_cast<const DevPatch*>(dv);
             w.start = reinterpret_cast<const int64_t*>(dv + off_start[st]);
@@ -435,9 +453,9 @@ class ConvectiveFluxReconstructorGPU {
 
     template <class W>
     int run(const Plan<W>& plan, const KParams& k, hipStream_t s) {
-        if (!profiling_) return run_stages(plan, k, s, 0, 4);
+        if (!profiling_) return run_stages(plan, k, s, 0, kStages);
         // One launch per stage, bracketed by events.
-        for (int stage = 0; stage < 4; stage++) {
+        for (int stage = 0; stage < kStages; stage++) {
             if (plan.items[stage] == 0) continue;
             if (prof_used_ + 2 > int(prof_events_.size()))
                 if (int st = collect_profile()) return st;
@@ -454,11 +472,12 @@ class ConvectiveFluxReconstructorGPU {
     template <class W>
     int run_stages(const Plan<W>& plan, const KParams& k, hipStream_t s, int first, int last) {
         Plan<W> sub = plan;
-        for (int st = 0; st < 4; st++)
+        for (int st = 0; st < kStages; st++)
             if (st < first || st >= last) sub.items[st] = 0;
         hipError_t e = hipSuccess;
         switch (cfg_.model) {
             case HM_MODEL_SINGLE_SPECIES: e = launch_rhs_single<W>(cfg_.dim, cfg_.scheme, sub, k, s); break;
+            case HM_MODEL_FIVE_EQN_ALLAIRE: e = launch_rhs_five<W>(cfg_.dim, cfg_.scheme, sub, k, s); break;
             default: return fail(HM_ERR_UNSUPPORTED, "flow model not available in this build");
         }
         if (e != hipSuccess) return fail(HM_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
@@ -485,8 +504,8 @@ class ConvectiveFluxReconstructorGPU {
     std::vector<hipEvent_t> prof_events_;
     int prof_used_ = 0;
     int prof_stage_[kProfSlots / 2] = {};
-    double prof_ms_[4] = {};
-    int64_t prof_n_[4] = {};
+    double prof_ms_[kStages] = {};
+    int64_t prof_n_[kStages] = {};
     hm_config cfg_;
     int neq_ = 0;
     bool has_advective_ = false;
@@ -509,14 +528,18 @@ int make_reconstructor(const hm_config& c, std::unique_ptr<ConvectiveFluxReconst
         err = "unknown convective flux reconstructor";  // ConvectiveFluxReconstructorManager.cpp:146
         return HM_ERR_UNSUPPORTED;
     }
-    if (c.model != HM_MODEL_SINGLE_SPECIES) {
+    if (c.model != HM_MODEL_SINGLE_SPECIES && c.model != HM_MODEL_FIVE_EQN_ALLAIRE) {
         err = "flow model not available in this build";
         return HM_ERR_UNSUPPORTED;
     }
     const int ns = (c.model == HM_MODEL_SINGLE_SPECIES) ? 1 : c.num_species;
-    if (ns < 1 || ns > HM_MAX_SPECIES || num_equations(c.model, c.dim, ns) > HM_MAX_EQN) {
+    if (ns < 1 || ns > HM_MAX_SPECIES || num_equations(c.model, c.dim, ns) + 1 > HM_MAX_EQN) {
         err = "number of species out of range";
         return HM_ERR_INVALID_ARGUMENT;
+    }
+    if (c.model == HM_MODEL_FIVE_EQN_ALLAIRE && (ns < 2 || ns > 3)) {
+        err = "the five-equation model is built for 2 or 3 species";
+        return HM_ERR_UNSUPPORTED;
     }
     if (c.constant_p < 1 || (c.scheme == HM_SCHEME_WCNS6_LD_HLLC_HLL && c.constant_q < 1)) {
         err = "WCNS exponents must be positive";
@@ -614,6 +637,12 @@ int hm_compute_convective_flux_and_source_on_patches(hm_ctx* ctx, const hm_patch
 int hm_profile_enable(hm_ctx* ctx, int enable) {
     if (!ctx || !ctx->rec) return HM_ERR_INVALID_ARGUMENT;
     return ctx->rec->setProfiling(enable);
+}
+
+int hm_get_num_conservative_arrays(const hm_ctx* ctx, int* num_arrays) {
+    if (!ctx || !ctx->rec || !num_arrays) return HM_ERR_INVALID_ARGUMENT;
+    *num_arrays = ctx->rec->numConservativeArrays();
+    return HM_OK;
 }
 
 int hm_profile_read(hm_ctx* ctx, double* stage_ms, int64_t* stage_launches) {

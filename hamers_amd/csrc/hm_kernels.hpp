@@ -18,15 +18,20 @@ struct KParams {
 };
 
 // Work of one RHS evaluation: one work source per pipeline stage
-// (0 derived cells, 1 sensor cells, 2 midpoint faces, 3 output faces) and the
-// number of work items (padded) each stage launches over.
+// (0 derived cells, 1 sensor cells, 2 midpoint faces, 3 output faces,
+// 4 interior cells for the advective source) and the number of work items
+// (padded) each stage launches over.
+constexpr int kStages = 5;
+
 template <class W>
 struct Plan {
-    W work[4];
-    int64_t items[4];
+    W work[kStages];
+    int64_t items[kStages];
 };
 
 template <class W>
 hipError_t launch_rhs_single(int dim, int scheme, const Plan<W>& plan, const KParams& k, hipStream_t s);
+template <class W>
+hipError_t launch_rhs_five(int dim, int scheme, const Plan<W>& plan, const KParams& k, hipStream_t s);
 
 }  // namespace hm

@@ -14,6 +14,7 @@
 //               (WCNS56-HLLC-HLL.cpp:1197-1282, 2327-2490)
 #include "hm_kernels.hpp"
 #include "hm_numerics.hpp"
+#include "hm_sensor.hpp"
 
 namespace hm {
 
@@ -60,52 +61,6 @@ __global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
     for (int b = 0; b < D; b++) pr[(1 + b) * vol + loc] = u[b];
     pr[(D + 1) * vol + loc] = p;
     pr[NEQ * vol + loc] = c;
-}
-
-// ----------------------------------------------------------------------------
-// Dilatation and vorticity magnitude (second-order central derivatives,
-// DerivativeFirstOrder.cpp:216-228).
-// ----------------------------------------------------------------------------
-template <int D, class W>
-__global__ __launch_bounds__(kBlock) void k_sensor(W w, KParams k) {
-    (void)k;
-    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    const DevPatch* P;
-    int ax;
-    int64_t loc;
-    if (!w.find(gid, P, ax, loc)) return;
-    const Boxes<D> B(*P);
-    const int e0 = B.ext(0, kGs), e1 = B.ext(1, kGs);
-    const int i = int(loc % e0) - kGs;
-    const int64_t r = loc / e0;
-    const int j = int(r % e1) - kGs;
-    const int kk = (D > 2) ? int(r / e1) - kGs : 0;
-    const int64_t v4 = B.vol(kGc);
-    const int64_t c = B.idx(kGc, i, j, kk);
-    const int64_t st[3] = {1, B.ext(0, kGc), int64_t(B.ext(0, kGc)) * B.ext(1, kGc)};
-    // d u_b / d x_a
-    double g[3][3];
-#pragma unroll
-    for (int b = 0; b < D; b++) {
-        const double* ub = P->prim + (1 + b) * v4;
-#pragma unroll
-        for (int a = 0; a < D; a++)
-            g[b][a] = (double(1) / double(2) * (ub[c + st[a]] - ub[c - st[a]])) / P->dx[a];
-    }
-    double theta, Omega;
-    if constexpr (D == 2) {
-        theta = g[0][0] + g[1][1];
-        Omega = fabs(g[1][0] - g[0][1]);
-    } else {
-        theta = g[0][0] + g[1][1] + g[2][2];
-        const double ox = g[2][1] - g[1][2];
-        const double oy = g[0][2] - g[2][0];
-        const double oz = g[1][0] - g[0][1];
-        Omega = sqrt(ox * ox + oy * oy + oz * oz);
-    }
-    const int64_t vs = B.vol(kGs);
-    P->sens[loc] = theta;
-    P->sens[vs + loc] = Omega;
 }
 
 // ----------------------------------------------------------------------------
@@ -434,7 +389,7 @@ hipError_t launch_single_dsq(const Plan<W>& plan, const KParams& k, hipStream_t 
                            dim3(kBlock), 0, s, plan.work[0], k);
     if constexpr (D > 1) {
         if (plan.items[1] > 0)
-            hipLaunchKernelGGL((k_sensor<D, W>), dim3(unsigned((plan.items[1] + kBlock - 1) / kBlock)),
+            hipLaunchKernelGGL((k_sensor<D, 1, W>), dim3(unsigned((plan.items[1] + kBlock - 1) / kBlock)),
                                dim3(kBlock), 0, s, plan.work[1], k);
     }
     if (plan.items[2] > 0)

@@ -62,7 +62,7 @@ class PatchData:
             v.dx[a] = float(self.dx[a]) if a < d else 1.0
         qs = list(self.Q) if isinstance(self.Q, (list, tuple)) else [self.Q[e] for e in range(self.Q.shape[0])]
         if len(qs) < neq:
-            raise ValueError(f"patch has {len(qs)} conservative variables, the model needs {neq}")
+            raise ValueError(f"patch has {len(qs)} conservative arrays, the model needs {neq}")
         for e in range(neq):
             t = qs[e]
             _check_tensor(t)
@@ -109,6 +109,8 @@ class ConvectiveFluxReconstructor:
         self.scheme = scheme
         self.num_species = 1 if flow_model == "SINGLE_SPECIES" else num_species
         self.num_eqn = num_equations(flow_model, dim, self.num_species)
+        # arrays a patch hands over: the five-equation model adds the last volume fraction
+        self.num_arrays = self.num_eqn + (1 if flow_model == "FIVE_EQN_ALLAIRE" else 0)
         cfg = capi.hm_config()
         cfg.dim = dim
         cfg.model = _MODELS[flow_model]
@@ -151,26 +153,26 @@ class ConvectiveFluxReconstructor:
 
     def computeConvectiveFluxAndSourceOnPatch(self, patch: PatchData, time: float, dt: float,
                                               RK_step_number: int = 0, stream=None):
-        v = patch.view(self.num_eqn)
+        v = patch.view(self.num_arrays)
         st = capi.lib.hm_compute_convective_flux_and_source_on_patch(
             self._ctx, ctypes.byref(v), time, dt, RK_step_number, self._stream_ptr(stream))
         capi.check(self._ctx, st)
 
-    STAGES = ("derived", "sensor", "midflux", "faceflux")
+    STAGES = ("derived", "sensor", "midflux", "faceflux", "source")
 
     def enable_profiling(self, on: bool = True):
         """Time each pipeline stage with HIP events on the launch stream."""
         capi.check(self._ctx, capi.lib.hm_profile_enable(self._ctx, int(on)))
 
     def read_profile(self) -> dict:
-        ms = (ctypes.c_double * 4)()
-        n = (ctypes.c_int64 * 4)()
+        ms = (ctypes.c_double * capi.HM_NUM_STAGES)()
+        n = (ctypes.c_int64 * capi.HM_NUM_STAGES)()
         capi.check(self._ctx, capi.lib.hm_profile_read(self._ctx, ms, n))
         return {name: (ms[i], n[i]) for i, name in enumerate(self.STAGES)}
 
     def computeConvectiveFluxAndSourceOnPatches(self, patches, time: float, dt: float, RK_step_number: int = 0,
                                                 stream=None):
-        plist = patches if isinstance(patches, PatchList) else PatchList(patches, self.num_eqn)
+        plist = patches if isinstance(patches, PatchList) else PatchList(patches, self.num_arrays)
         st = capi.lib.hm_compute_convective_flux_and_source_on_patches(
             self._ctx, plist.views, len(plist), time, dt, RK_step_number, self._stream_ptr(stream))
         capi.check(self._ctx, st)

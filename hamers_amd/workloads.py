@@ -135,3 +135,39 @@ def split_patches(Q_global_interior: np.ndarray, patch: tuple, g: int):
         sl = [slice(None)] + [slice(off[a], off[a] + patch[a] + 2 * g) for a in reversed(range(d))]
         out.append((off, np.ascontiguousarray(Qg[tuple(sl)])))
     return out
+
+
+def conservative_from_primitive_five(Z_rho, vel, p, Z, gamma):
+    """Five-equation Allaire model: (Z_i rho_i, u, p, Z_i (all ns)) ->
+    (Z_i rho_i, rho u, E, Z_i (all ns)); E = p/(gamma_mix - 1) + 1/2 rho |u|^2
+    with 1/(gamma_mix - 1) = sum Z_i / (gamma_i - 1) (isobaric mixing)."""
+    ns = len(Z_rho)
+    d = len(vel)
+    rho = sum(Z_rho)
+    xi = sum(Z[i] / (gamma[i] - 1.0) for i in range(ns))
+    ke = sum(v * v for v in vel)
+    Q = np.empty((2 * ns + d + 1,) + np.shape(rho))
+    for i in range(ns):
+        Q[i] = Z_rho[i]
+    for b in range(d):
+        Q[ns + b] = rho * vel[b]
+    gamma_m = 1.0 / xi + 1.0
+    Q[ns + d] = p / (gamma_m - 1.0) + 0.5 * rho * ke
+    for i in range(ns):
+        Q[ns + d + 1 + i] = Z[i]
+    return Q
+
+
+def interface_advection_convergence(N: int, dim: int = 2):
+    """Periodic material-interface advection of the reference's five-equation
+    convergence test on [-1,1]^d: Z_1 = 1/2 + 1/4 sin(pi sum x), rho_1 = 2,
+    rho_2 = 1, gamma = (1.6, 1.4), u = 1, p = 1
+    (problems/Euler/initial_conditions/ConvergenceFiveEqnAllaire.cpp:77-130)."""
+    n = (N,) * dim
+    dx = (2.0 / N,) * dim
+    X = cell_centres(n, (-1.0,) * dim, dx)
+    Z1 = 0.5 + 0.25 * np.sin(np.pi * sum(X))
+    Z2 = 1.0 - Z1
+    Q = conservative_from_primitive_five([Z1 * 2.0, Z2 * 1.0], [np.ones_like(Z1)] * dim, np.ones_like(Z1),
+                                         [Z1, Z2], (1.6, 1.4))
+    return Q, n, dx

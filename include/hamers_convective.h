@@ -80,7 +80,9 @@ typedef struct hm_patch_view {
     int n[3];                     /* interior cells per axis (1 on unused axes) */
     int ghost[3];                 /* ghost width of Q per axis (>= 4 on used axes) */
     double dx[3];                 /* grid spacing */
-    const double* Q[HM_MAX_EQN];  /* device: one array per equation over the ghost box */
+    const double* Q[HM_MAX_EQN];  /* device: one array per conservative variable over the ghost box;
+                                     FIVE_EQN_ALLAIRE also passes the last species' volume fraction
+                                     (SAMRAI stores all ns, the system evolves ns-1), i.e. num_eqn+1 */
     double* F[3];                 /* device: flux SideData array per axis (overwritten) */
     double* S;                    /* device: source CellData (accumulated, ADVECTIVE eqs) */
 } hm_patch_view;
@@ -99,6 +101,11 @@ int hm_get_num_ghosts(const hm_ctx* ctx, int* num_ghosts);
 
 /* Number of equations of the configured flow model (FlowModel::getNumberOfEquations). */
 int hm_get_num_eqn(const hm_ctx* ctx, int* num_eqn);
+
+/* Number of conservative arrays a patch view must provide: num_eqn, plus one
+ * (the last volume fraction) for HM_MODEL_FIVE_EQN_ALLAIRE
+ * (FlowModelFiveEqnAllaire::getCellDataOfConservativeVariables). */
+int hm_get_num_conservative_arrays(const hm_ctx* ctx, int* num_arrays);
 
 /* Message describing the last failure on this context ("" if none). */
 const char* hm_last_error(const hm_ctx* ctx);
@@ -129,13 +136,15 @@ int hm_compute_convective_flux_and_source_on_patches(hm_ctx* ctx, const hm_patch
 /*
  * Stage timing (instrumentation; no counterpart in the reference).  While
  * enabled, every RHS call records HIP events on its stream around each of the
- * four pipeline stages: 0 derived cell data, 1 dilatation/vorticity, 2 fused
- * midpoint flux, 3 midpoint-to-face flux.  hm_profile_read waits for the
- * recorded events and returns the summed milliseconds and launch counts per
- * stage since the last hm_profile_enable.
+ * pipeline stages: 0 derived cell data, 1 dilatation/vorticity, 2 fused
+ * midpoint flux, 3 midpoint-to-face flux, 4 advective source.  hm_profile_read
+ * waits for the recorded events and returns the summed milliseconds and launch
+ * counts per stage since the last hm_profile_enable.
  */
+#define HM_NUM_STAGES 5
 int hm_profile_enable(hm_ctx* ctx, int enable);
-int hm_profile_read(hm_ctx* ctx, double* stage_ms /* [4] */, int64_t* stage_launches /* [4] */);
+int hm_profile_read(hm_ctx* ctx, double* stage_ms /* [HM_NUM_STAGES] */,
+                    int64_t* stage_launches /* [HM_NUM_STAGES] */);
 
 /* Block the host until all work this context queued on `stream` is done. */
 int hm_synchronize(hm_ctx* ctx, void* stream);

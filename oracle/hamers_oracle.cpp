@@ -261,7 +261,10 @@ Model make_model(const hmo_params& prm) {
         m.gamma[i] = prm.gamma[i];
         m.R[i] = prm.R[i];
     }
-    if (m.model == 0) {
+    if (m.model == 2) {
+        m.neq = m.dim + 2 * m.ns;  // FlowModelFiveEqnAllaire.cpp:30
+        m.nproj = m.ns + 2;
+    } else if (m.model == 0) {
         m.ns = 1;
         m.neq = m.dim + 2;  // FlowModelSingleSpecies: rho, rho*u, E
         m.nproj = 2;        // FlowModelBasicUtilitiesSingleSpecies.cpp:3526-3529
@@ -690,18 +693,528 @@ void direction_pass_single(const hmo_params& prm, const Model& m, const int* n, 
     }
 }
 
+// ===========================================================================
+// Five-equation Allaire model (FlowModelFiveEqnAllaire*, ISOBARIC ideal-gas
+// mixing).  num_eqn = d + 2 ns (FlowModelFiveEqnAllaire.cpp:30); the last
+// ns-1 equations (volume fractions) are ADVECTIVE (:94-113).
+//   Q (input, neq+1 comps): Z_i rho_i (ns), rho u (d), E, Z_i (all ns, the
+//       last one stored but not evolved, WCNS56:223-239)
+//   V (primitive, neq comps): Z_i rho_i (ns), u (d), p, Z_i (ns-1)
+// ===========================================================================
+
+struct Five {
+    int d, ns, neq;
+    double gam[8];
+    int iu() const { return ns; }          // first velocity slot of V
+    int ip() const { return ns + d; }      // pressure slot of V
+    int iz() const { return ns + d + 1; }  // first volume-fraction slot of V
+};
+
+// Mixture gamma from all ns volume fractions (EquationOfStateMixingRulesIdealGas.cpp:7404-7598):
+// xi = sum Z_i * (1/(gamma_i - 1)), gamma = 1/xi + 1.
+inline double mix_gamma_all(const Five& f, const double* Z) {
+    double xi = double(0);
+    for (int si = 0; si < f.ns; si++) {
+        const double one_over_denominator = double(1) / (f.gam[si] - double(1));
+        xi += Z[si] * one_over_denominator;
+    }
+    return double(1) / xi + double(1);
+}
+
+// Mixture gamma from ns-1 volume fractions, the last one being 1 - sum
+// (EquationOfStateMixingRulesIdealGas.cpp:7599-7800).
+inline double mix_gamma_last(const Five& f, const double* Z) {
+    double xi = double(0);
+    double Z_last = double(1);
+    for (int si = 0; si < f.ns - 1; si++) {
+        const double one_over_denominator = double(1) / (f.gam[si] - double(1));
+        xi += Z[si] * one_over_denominator;
+        Z_last -= Z[si];
+    }
+    xi += Z_last / (f.gam[f.ns - 1] - double(1));
+    return double(1) / xi + double(1);
+}
+
+struct DerivedFive {
+    CellArray V;      // neq comps
+    CellArray c;      // sound speed
+    CellArray rho;    // mixture density
+    CellArray Fn[3];  // node convective flux per axis
+};
+
+// Derived cell data (FlowModelFiveEqnAllaire.cpp:3734-4876): density
+// (EquationOfStateMixingRules.cpp:735-860), mass fractions, velocity, internal
+// energy, pressure (ISOBARIC, EquationOfStateIdealGas.cpp:5633), sound speed
+// c^2 = Gamma p / rho + sum Y_i Psi_i with Psi_i = p / rho (:4512-4876,
+// EquationOfStateMixingRulesIdealGas.cpp:6564-6610), node flux (:4877-5901).
+void compute_derived_five(const Five& f, const int* n, int g, const double* Q, DerivedFive& D) {
+    const int d = f.d, ns = f.ns, neq = f.neq;
+    CellArray Qv;
+    Qv.init(n, g, d, 0);
+    const int64_t qvol = Qv.vol();
+    D.V.init(n, 4, d, neq);
+    D.c.init(n, 4, d, 1);
+    D.rho.init(n, 4, d, 1);
+    for (int a = 0; a < d; a++) D.Fn[a].init(n, 4, d, neq);
+    const int lo2 = (d > 2) ? -4 : 0, hi2 = (d > 2) ? n[2] + 4 : 1;
+    const int lo1 = (d > 1) ? -4 : 0, hi1 = (d > 1) ? n[1] + 4 : 1;
+    for (int k = lo2; k < hi2; k++)
+        for (int j = lo1; j < hi1; j++)
+            for (int i = -4; i < n[0] + 4; i++) {
+                const int64_t iq = Qv.idx(i, j, k), iv = D.V.idx(i, j, k);
+                double Zr[8], Z[8];
+                for (int si = 0; si < ns; si++) Zr[si] = Q[si * qvol + iq];
+                for (int si = 0; si < ns; si++) Z[si] = Q[(ns + d + 1 + si) * qvol + iq];
+                double rho = double(0);
+                for (int si = 0; si < ns; si++) rho += Zr[si];
+                double Y[8];
+                for (int si = 0; si < ns; si++) Y[si] = Zr[si] / rho;
+                double u[3] = {0.0, 0.0, 0.0};
+                for (int b = 0; b < d; b++) u[b] = Q[(ns + b) * qvol + iq] / rho;
+                double ke;
+                if (d == 1)
+                    ke = u[0] * u[0];
+                else if (d == 2)
+                    ke = u[0] * u[0] + u[1] * u[1];
+                else
+                    ke = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+                const double E = Q[(ns + d) * qvol + iq];
+                const double eps_int = E / rho - double(1) / double(2) * ke;
+                const double gmix = mix_gamma_all(f, Z);
+                const double p = (gmix - double(1)) * rho * eps_int;
+                const double Gamma = gmix - double(1);
+                double c2 = Gamma * p / rho;
+                for (int si = 0; si < ns; si++) {
+                    const double Psi = p / rho;
+                    c2 += Y[si] * Psi;
+                }
+                for (int si = 0; si < ns; si++) D.V.c(si)[iv] = Zr[si];
+                for (int b = 0; b < d; b++) D.V.c(ns + b)[iv] = u[b];
+                D.V.c(ns + d)[iv] = p;
+                for (int si = 0; si < ns - 1; si++) D.V.c(ns + d + 1 + si)[iv] = Z[si];
+                D.c.c(0)[iv] = std::sqrt(c2);
+                D.rho.c(0)[iv] = rho;
+                for (int a = 0; a < d; a++) {
+                    const double ua = u[a];
+                    for (int si = 0; si < ns; si++) D.Fn[a].c(si)[iv] = ua * Zr[si];
+                    for (int b = 0; b < d; b++) {
+                        if (b == a)
+                            D.Fn[a].c(ns + b)[iv] = ua * Q[(ns + b) * qvol + iq] + p;
+                        else
+                            D.Fn[a].c(ns + b)[iv] = ua * Q[(ns + b) * qvol + iq];
+                    }
+                    D.Fn[a].c(ns + d)[iv] = ua * (E + p);
+                    for (int si = 0; si < ns - 1; si++) D.Fn[a].c(ns + d + 1 + si)[iv] = ua * Z[si];
+                }
+            }
+}
+
+// Characteristic variables (FlowModelBasicUtilitiesFiveEqnAllaire.cpp:8232-9178).
+inline void char_from_prim_five(const Five& f, int a, const double* Zra, double ra, double ca, const double* V,
+                                double* W) {
+    const int ns = f.ns, d = f.d;
+    const double un = V[ns + a], p = V[ns + d];
+    W[0] = un - double(1) / (ra * ca) * p;
+    for (int si = 0; si < ns; si++) W[1 + si] = V[si] - Zra[si] / (ra * ca * ca) * p;
+    int t = ns + 1;
+    for (int b = 0; b < d; b++)
+        if (b != a) W[t++] = V[ns + b];
+    for (int si = 0; si < ns - 1; si++) W[ns + d + si] = V[ns + d + 1 + si];
+    W[f.neq - 1] = un + double(1) / (ra * ca) * p;
+}
+
+// Back-projection (FlowModelBasicUtilitiesFiveEqnAllaire.cpp:9201-9974).
+inline void prim_from_char_five(const Five& f, int a, const double* Zra, double ra, double ca, const double* W,
+                                double* V) {
+    const int ns = f.ns, d = f.d, neq = f.neq;
+    for (int si = 0; si < ns; si++)
+        V[si] = -double(1) / double(2) * Zra[si] / ca * W[0] + W[si + 1] +
+                double(1) / double(2) * Zra[si] / ca * W[neq - 1];
+    V[ns + a] = double(1) / double(2) * W[0] + double(1) / double(2) * W[neq - 1];
+    int t = ns + 1;
+    for (int b = 0; b < d; b++)
+        if (b != a) V[ns + b] = W[t++];
+    V[ns + d] = -double(1) / double(2) * ra * ca * W[0] + double(1) / double(2) * ra * ca * W[neq - 1];
+    for (int si = 0; si < ns - 1; si++) V[ns + d + 1 + si] = W[ns + d + si];
+}
+
+// Bounds of one side (FlowModelBasicUtilitiesFiveEqnAllaire.cpp:5446-7347,
+// bounds FlowModelBasicUtilitiesFiveEqnAllaire.hpp:24-27).
+inline int bounded_five(const Five& f, const double* V) {
+    const int ns = f.ns, d = f.d;
+    int ok = 1;
+    double Z[8];
+    Z[ns - 1] = double(1);
+    for (int si = 0; si < ns - 1; si++) {
+        Z[si] = V[ns + d + 1 + si];
+        Z[ns - 1] -= Z[si];
+        ok &= (Z[si] > double(-1000.0) && Z[si] < double(1000.0)) ? 1 : 0;
+    }
+    ok &= (Z[ns - 1] > double(-1000.0) && Z[ns - 1] < double(1000.0)) ? 1 : 0;
+    double rho = double(0);
+    for (int si = 0; si < ns; si++) rho += V[si];
+    double Y[8];
+    for (int si = 0; si < ns; si++) {
+        Y[si] = V[si] / rho;
+        ok &= (Y[si] > double(-0.001) && Y[si] < double(1.001)) ? 1 : 0;
+    }
+    for (int si = 0; si < ns; si++) ok &= (V[si] > double(0)) ? 1 : 0;
+    const double p = V[ns + d];
+    const double Gamma = mix_gamma_all(f, Z) - double(1);
+    double c2 = Gamma * p / rho;
+    for (int si = 0; si < ns; si++) {
+        const double Psi = p / rho;
+        c2 += Y[si] * Psi;
+    }
+    ok &= (c2 > double(0)) ? 1 : 0;
+    return ok;
+}
+
+// HLLC (FlowModelRiemannSolverFiveEqnAllaireHLLC.cpp:711-1591; face EOS in the
+// drivers :4824-7716) and HLLC-HLL (...HLLC-HLL.cpp:1146-2439) of one face.
+void riemann_face_five(const Five& f, int a, bool use_hll, const double* VL, const double* VR, double* Fout,
+                       double* ustar) {
+    const int ns = f.ns, d = f.d, neq = f.neq;
+    const int iu = ns + a, ip = ns + d;
+    auto face_eos = [&](const double* V, double& rho, double& c, double& eps_int) {
+        rho = double(0);
+        for (int si = 0; si < ns; si++) rho += V[si];
+        double Y[8];
+        for (int si = 0; si < ns; si++) Y[si] = V[si] / rho;
+        const double gmix = mix_gamma_last(f, V + ns + d + 1);
+        const double p = V[ip];
+        const double Gamma = gmix - double(1);
+        double c2 = Gamma * p / rho;
+        for (int si = 0; si < ns; si++) {
+            const double Psi = p / rho;
+            c2 += Y[si] * Psi;
+        }
+        c = std::sqrt(c2);
+        eps_int = p / ((gmix - double(1)) * rho);
+    };
+    double rhoL, rhoR, cL, cR, epsL, epsR;
+    face_eos(VL, rhoL, cL, epsL);
+    face_eos(VR, rhoR, cR, epsR);
+    const double uL = VL[iu], uR = VR[iu], pL = VL[ip], pR = VR[ip];
+    const double u_avg = double(1) / double(2) * (uL + uR);
+    const double c_av = double(1) / double(2) * (cL + cR);
+    const double sL = std::fmin(u_avg - c_av, uL - cL);
+    const double sR = std::fmax(u_avg + c_av, uR + cR);
+    const double sm = std::fmin(double(0), sL);
+    const double sp = std::fmax(double(0), sR);
+    const double sstar =
+        (pR - pL + rhoL * uL * (sL - uL) - rhoR * uR * (sR - uR)) / (rhoL * (sL - uL) - rhoR * (sR - uR));
+
+    auto cons_flux = [&](const double* V, double rho, double eps_int, double* Qs, double* Fs) {
+        for (int si = 0; si < ns; si++) Qs[si] = V[si];
+        for (int b = 0; b < d; b++) Qs[ns + b] = rho * V[ns + b];
+        double ke;
+        if (d == 1)
+            ke = V[ns] * V[ns];
+        else if (d == 2)
+            ke = V[ns] * V[ns] + V[ns + 1] * V[ns + 1];
+        else
+            ke = V[ns] * V[ns] + V[ns + 1] * V[ns + 1] + V[ns + 2] * V[ns + 2];
+        Qs[ns + d] = rho * (eps_int + double(1) / double(2) * ke);
+        for (int si = 0; si < ns - 1; si++) Qs[ns + d + 1 + si] = V[ns + d + 1 + si];
+        const double un = V[iu];
+        for (int si = 0; si < ns; si++) Fs[si] = un * V[si];
+        for (int b = 0; b < d; b++) {
+            if (b == a)
+                Fs[ns + b] = un * Qs[ns + b] + V[ip];
+            else
+                Fs[ns + b] = un * Qs[ns + b];
+        }
+        Fs[ns + d] = un * (Qs[ns + d] + V[ip]);
+        for (int si = 0; si < ns - 1; si++) Fs[ns + d + 1 + si] = un * V[ns + d + 1 + si];
+    };
+    double QL[12], QR[12], FL[12], FR[12];
+    cons_flux(VL, rhoL, epsL, QL, FL);
+    cons_flux(VR, rhoR, epsR, QR, FR);
+
+    double Fhllc[12];
+    {
+        const bool left = (sstar > double(0));
+        const double* V = left ? VL : VR;
+        const double* Qs = left ? QL : QR;
+        const double* Fs = left ? FL : FR;
+        const double rho = left ? rhoL : rhoR;
+        const double sK = left ? sL : sR;
+        const double spm = left ? sm : sp;
+        const double chi = (sK - V[iu]) / (sK - sstar);
+        double Qst[12];
+        for (int si = 0; si < ns; si++) Qst[si] = chi * V[si];
+        for (int b = 0; b < d; b++) {
+            if (b == a)
+                Qst[ns + b] = chi * rho * sstar;
+            else
+                Qst[ns + b] = chi * Qs[ns + b];
+        }
+        Qst[ns + d] = chi * (Qs[ns + d] + (sstar - V[iu]) * (rho * sstar + V[ip] / (sK - V[iu])));
+        for (int si = 0; si < ns - 1; si++) Qst[ns + d + 1 + si] = chi * V[ns + d + 1 + si];
+        for (int e = 0; e < neq; e++) Fhllc[e] = Fs[e] + spm * (Qst[e] - Qs[e]);
+        if (ustar) *ustar = V[iu] + spm * (chi - double(1));
+    }
+    for (int e = 0; e < neq; e++) Fout[e] = Fhllc[e];
+    if (d > 1 && use_hll) {
+        auto hll = [&](int e) {
+            double r = (sR * FL[e] - sL * FR[e] + sR * sL * (QR[e] - QL[e])) / (sR - sL);
+            if (sL > double(0)) r = FL[e];
+            if (sR < double(0)) r = FR[e];
+            return r;
+        };
+        double dsq;
+        {
+            const double du = VR[ns] - VL[ns];
+            const double dv = VR[ns + 1] - VL[ns + 1];
+            if (d == 2) {
+                dsq = du * du + dv * dv;
+            } else {
+                const double dw = VR[ns + 2] - VL[ns + 2];
+                dsq = du * du + dv * dv + dw * dw;
+            }
+        }
+        const double vel_mag = std::sqrt(dsq);
+        double al1, al2;
+        if (vel_mag < EPS) {
+            al1 = double(1);
+            al2 = double(0);
+        } else {
+            al1 = std::fabs(VR[iu] - VL[iu]) / vel_mag;
+            al2 = std::sqrt(double(1) - al1 * al1);
+        }
+        const double be1 = double(1) / double(2) * (double(1) + al1 / (al1 + al2));
+        const double be2 = double(1) - be1;
+        for (int si = 0; si < ns; si++) Fout[si] = be1 * Fhllc[si] + be2 * hll(si);
+        for (int b = 0; b < d; b++)
+            if (b != a) Fout[ns + b] = be1 * Fhllc[ns + b] + be2 * hll(ns + b);
+        for (int si = 0; si < ns - 1; si++) {
+            const int e = ns + d + 1 + si;
+            Fout[e] = be1 * Fhllc[e] + be2 * hll(e);
+        }
+    }
+}
+
+// One axis of the WCNS56 driver for the five-equation model, pass by pass.
+// Also returns the HLLC normal velocity on the midpoint faces (for the source).
+void direction_pass_five(const hmo_params& prm, const Five& f, const int* n, int a, const DerivedFive& D,
+                         const CellArray& theta, const CellArray& Omega, double dt, double* F_out, FaceArray& ustar) {
+    const int d = f.d, ns = f.ns, neq = f.neq;
+    const int sh[3] = {a == 0, a == 1, a == 2};
+    FaceArray avg;  // ns + 2 projection variables: Z_i rho_i, rho, c averages
+    avg.init(n, a, ns + 2);
+    auto for_faces = [&](auto&& fn) {
+        for (int k = avg.lo[2]; k < avg.lo[2] + avg.ext[2]; k++)
+            for (int j = avg.lo[1]; j < avg.lo[1] + avg.ext[1]; j++)
+                for (int i = avg.lo[0]; i < avg.lo[0] + avg.ext[0]; i++) fn(i, j, k);
+    };
+    // Projection variables (FlowModelBasicUtilitiesFiveEqnAllaire.cpp:7474-8207).
+    for_faces([&](int i, int j, int k) {
+        const int64_t iL = D.V.idx(i - sh[0], j - sh[1], k - sh[2]);
+        const int64_t iR = D.V.idx(i, j, k);
+        const int64_t fc = avg.idx(i, j, k);
+        for (int si = 0; si < ns; si++)
+            avg.c(si)[fc] = double(1) / double(2) * (D.V.c(si)[iL] + D.V.c(si)[iR]);
+        avg.c(ns)[fc] = double(1) / double(2) * (D.rho.c(0)[iL] + D.rho.c(0)[iR]);
+        avg.c(ns + 1)[fc] = double(1) / double(2) * (D.c.c(0)[iL] + D.c.c(0)[iR]);
+    });
+    auto proj = [&](int64_t fc, double* Zra, double& ra, double& ca) {
+        for (int si = 0; si < ns; si++) Zra[si] = avg.c(si)[fc];
+        ra = avg.c(ns)[fc];
+        ca = avg.c(ns + 1)[fc];
+    };
+    std::vector<FaceArray> W(6 * neq);
+    for (int mm = 0; mm < 6; mm++) {
+        for (int e = 0; e < neq; e++) W[mm * neq + e].init(n, a, 1);
+        const int o = mm - 3;
+        for_faces([&](int i, int j, int k) {
+            const int64_t ic = D.V.idx(i + o * sh[0], j + o * sh[1], k + o * sh[2]);
+            const int64_t fc = avg.idx(i, j, k);
+            double V[12], Wc[12], Zra[8], ra, ca;
+            for (int e = 0; e < neq; e++) V[e] = D.V.c(e)[ic];
+            proj(fc, Zra, ra, ca);
+            char_from_prim_five(f, a, Zra, ra, ca, V, Wc);
+            for (int e = 0; e < neq; e++) W[mm * neq + e].v[fc] = Wc[e];
+        });
+    }
+    std::vector<FaceArray> Wm(neq), Wp(neq);
+    for (int e = 0; e < neq; e++) {
+        Wm[e].init(n, a, 1);
+        Wp[e].init(n, a, 1);
+        for_faces([&](int i, int j, int k) {
+            const int64_t fc = avg.idx(i, j, k);
+            Stencil6 s{W[0 * neq + e].v[fc], W[1 * neq + e].v[fc], W[2 * neq + e].v[fc],
+                       W[3 * neq + e].v[fc], W[4 * neq + e].v[fc], W[5 * neq + e].v[fc]};
+            if (prm.scheme == 1) {
+                const double sig = wcns6ld_sigma(s);
+                Wm[e].v[fc] = wcns6ld_side(s, sig, prm.p, prm.q, prm.C, prm.alpha_tau, prm.mutation);
+                Wp[e].v[fc] = wcns6ld_side(mirror(s), sig, prm.p, prm.q, prm.C, prm.alpha_tau, prm.mutation);
+            } else {
+                Wm[e].v[fc] = wcns5js_side(s, prm.p, prm.mutation);
+                Wp[e].v[fc] = wcns5js_side(mirror(s), prm.p, prm.mutation);
+            }
+        });
+    }
+    std::vector<FaceArray> Vm(neq), Vp(neq);
+    for (int e = 0; e < neq; e++) {
+        Vm[e].init(n, a, 1);
+        Vp[e].init(n, a, 1);
+    }
+    for_faces([&](int i, int j, int k) {
+        const int64_t fc = avg.idx(i, j, k);
+        double Wc[12], V[12], Zra[8], ra, ca;
+        proj(fc, Zra, ra, ca);
+        for (int e = 0; e < neq; e++) Wc[e] = Wm[e].v[fc];
+        prim_from_char_five(f, a, Zra, ra, ca, Wc, V);
+        for (int e = 0; e < neq; e++) Vm[e].v[fc] = V[e];
+        for (int e = 0; e < neq; e++) Wc[e] = Wp[e].v[fc];
+        prim_from_char_five(f, a, Zra, ra, ca, Wc, V);
+        for (int e = 0; e < neq; e++) Vp[e].v[fc] = V[e];
+    });
+    for_faces([&](int i, int j, int k) {
+        const int64_t fc = avg.idx(i, j, k);
+        double VL[12], VR[12];
+        for (int e = 0; e < neq; e++) {
+            VL[e] = Vm[e].v[fc];
+            VR[e] = Vp[e].v[fc];
+        }
+        int fm = bounded_five(f, VL), fp = bounded_five(f, VR);
+        if (prm.mutation == 3) fm = fp = 1;
+        if (fm == 0 || fp == 0) {
+            const int64_t iL = D.V.idx(i - sh[0], j - sh[1], k - sh[2]);
+            const int64_t iR = D.V.idx(i, j, k);
+            for (int e = 0; e < neq; e++) {
+                Vm[e].v[fc] = D.V.c(e)[iL];
+                Vp[e].v[fc] = D.V.c(e)[iR];
+            }
+        }
+    });
+    FaceArray Fmid;
+    Fmid.init(n, a, neq);
+    ustar.init(n, a, 1);
+    for_faces([&](int i, int j, int k) {
+        const int64_t fc = avg.idx(i, j, k);
+        double VL[12], VR[12];
+        for (int e = 0; e < neq; e++) {
+            VL[e] = Vm[e].v[fc];
+            VR[e] = Vp[e].v[fc];
+        }
+        bool use_hll = false;
+        if (d > 1) {
+            const int64_t tL = theta.idx(i - sh[0], j - sh[1], k - sh[2]);
+            const int64_t tR = theta.idx(i, j, k);
+            const double th_avg = 0.5 * (theta.c(0)[tL] + theta.c(0)[tR]);
+            const double om_avg = 0.5 * (Omega.c(0)[tL] + Omega.c(0)[tR]);
+            const double sensor = -th_avg / (std::fabs(th_avg) + om_avg + EPS);
+            use_hll = sensor > 0.65;
+        }
+        double Fo[12], us;
+        riemann_face_five(f, a, use_hll, VL, VR, Fo, &us);
+        for (int e = 0; e < neq; e++) Fmid.c(e)[fc] = Fo[e];
+        ustar.v[fc] = us;
+    });
+    int fext[3];
+    for (int b = 0; b < 3; b++) fext[b] = (b == a) ? n[b] + 1 : n[b];
+    const int64_t fvol = int64_t(fext[0]) * fext[1] * fext[2];
+    for (int e = 0; e < neq; e++) {
+        const double* Fm = Fmid.c(e);
+        const double* Fn = D.Fn[a].c(e);
+        double* out = F_out + e * fvol;
+        for (int k = 0; k < fext[2]; k++)
+            for (int j = 0; j < fext[1]; j++)
+                for (int i = 0; i < fext[0]; i++) {
+                    const int64_t of = i + int64_t(j) * fext[0] + int64_t(k) * fext[0] * fext[1];
+                    const double mR = Fm[Fmid.idx(i + sh[0], j + sh[1], k + sh[2])];
+                    const double mL = Fm[Fmid.idx(i - sh[0], j - sh[1], k - sh[2])];
+                    const double mC = Fm[Fmid.idx(i, j, k)];
+                    const double nR = Fn[D.V.idx(i, j, k)];
+                    const double nL = Fn[D.V.idx(i - sh[0], j - sh[1], k - sh[2])];
+                    out[of] = dt * (double(1) / double(30) * (mR + mL) - double(3) / double(10) * (nR + nL) +
+                                    double(23) / double(15) * mC);
+                }
+    }
+}
+
+// Advective source of the volume-fraction equations (WCNS56:491-535, 1288-1369, 2492-2650):
+// S_i += dt Q_i sum_a [3/2 (u*_{i+1/2} - u*_{i-1/2}) - 3/10 (u_{i+1} - u_{i-1})
+//                      + 1/30 (u*_{i+3/2} - u*_{i-3/2})] / dx_a
+void source_five(const Five& f, const int* n, int g, const double* dx, double dt, const double* Q,
+                 const DerivedFive& D, const FaceArray* ustar, double* S) {
+    const int d = f.d, ns = f.ns;
+    CellArray Qv;
+    Qv.init(n, g, d, 0);
+    const int64_t qvol = Qv.vol();
+    const int64_t svol = int64_t(n[0]) * n[1] * n[2];
+    for (int si = 0; si < ns - 1; si++) {
+        const int ei = ns + d + 1 + si;
+        for (int k = 0; k < n[2]; k++)
+            for (int j = 0; j < n[1]; j++)
+                for (int i = 0; i < n[0]; i++) {
+                    double acc = 0.0;
+                    for (int a = 0; a < d; a++) {
+                        const int s0 = a == 0, s1 = a == 1, s2 = a == 2;
+                        const FaceArray& us = ustar[a];
+                        const double* u = D.V.c(ns + a);
+                        const double uLL = us.v[us.idx(i - s0, j - s1, k - s2)];
+                        const double uL = us.v[us.idx(i, j, k)];
+                        const double uR = us.v[us.idx(i + s0, j + s1, k + s2)];
+                        const double uRR = us.v[us.idx(i + 2 * s0, j + 2 * s1, k + 2 * s2)];
+                        const double cL = u[D.V.idx(i - s0, j - s1, k - s2)];
+                        const double cR = u[D.V.idx(i + s0, j + s1, k + s2)];
+                        const double term = (double(3) / double(2) * (uR - uL) - double(3) / double(10) * (cR - cL) +
+                                             double(1) / double(30) * (uRR - uLL)) /
+                                            dx[a];
+                        acc = (a == 0) ? term : acc + term;
+                    }
+                    const int64_t is = i + int64_t(j) * n[0] + int64_t(k) * n[0] * n[1];
+                    S[ei * svol + is] += dt * Q[ei * qvol + Qv.idx(i, j, k)] * acc;
+                }
+    }
+}
+
+// theta / Omega for the five-equation model (velocity in V slots ns..ns+d-1).
+void compute_theta_omega_five(const Five& f, const int* n, const DerivedFive& D, const double* dx, CellArray& theta,
+                              CellArray& Omega) {
+    // Build a single-species-like view of the velocity so the same derivative code runs.
+    Model m{};
+    m.dim = f.d;
+    Derived tmp;
+    tmp.V.init(n, 4, f.d, f.d + 2);
+    for (int b = 0; b < f.d; b++) std::memcpy(tmp.V.c(1 + b), D.V.c(f.ns + b), sizeof(double) * D.V.vol());
+    compute_theta_omega(m, n, tmp, dx, theta, Omega);
+}
+
+int rhs_patch_five(const hmo_params* prm, const int* n, int g, const double* dx, double dt, const double* Q,
+                   double* const* F, double* S) {
+    Five f{};
+    f.d = prm->dim;
+    f.ns = prm->num_species;
+    f.neq = f.d + 2 * f.ns;
+    if (f.ns < 2 || f.ns > 8) return 5;
+    for (int i = 0; i < 8; i++) f.gam[i] = prm->gamma[i];
+    DerivedFive D;
+    compute_derived_five(f, n, g, Q, D);
+    CellArray theta, Omega;
+    compute_theta_omega_five(f, n, D, dx, theta, Omega);
+    FaceArray ustar[3];
+    for (int a = 0; a < f.d; a++) direction_pass_five(*prm, f, n, a, D, theta, Omega, dt, F[a], ustar[a]);
+    if (S) source_five(f, n, g, dx, dt, Q, D, ustar, S);
+    return 0;
+}
+
 int rhs_patch_impl(const hmo_params* prm, const int* n_in, int g, const double* dx, double dt, const double* Q,
                    double* const* F, double* S) {
     (void)S;
     if (!prm || !n_in || !dx || !Q || !F) return 1;
     Model m = make_model(*prm);
-    if (m.model != 0) return 2;  // models other than single-species: not in this restatement yet
     if (g < 4) return 3;
     int n[3] = {1, 1, 1};
     for (int a = 0; a < m.dim; a++) {
         n[a] = n_in[a];
         if (n[a] < 1) return 4;
     }
+    if (prm->model == 2) return rhs_patch_five(prm, n, g, dx, dt, Q, F, S);
+    if (m.model != 0) return 2;  // four-equation model: not in this restatement yet
     Derived D;
     compute_derived_single(m, n, g, Q, D);
     CellArray theta, Omega;

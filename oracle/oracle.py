@@ -76,6 +76,12 @@ class Params:
         return c
 
     @property
+    def nq(self) -> int:
+        """Number of conservative arrays handed over: the five-equation model
+        also passes the last species' volume fraction (stored, not evolved)."""
+        return self.neq + 1 if self.model == FIVE_EQN_ALLAIRE else self.neq
+
+    @property
     def neq(self) -> int:
         if self.model == SINGLE_SPECIES:
             return self.dim + 2
@@ -136,7 +142,7 @@ def rhs_patch(prm: Params, Q: np.ndarray, n: tuple, g: int, dx: tuple, dt: float
     neq = prm.neq
     n = tuple(int(v) for v in n)
     Q = np.ascontiguousarray(Q, dtype=np.float64)
-    assert Q.shape == (neq,) + tuple(reversed([v + 2 * g for v in n])), Q.shape
+    assert Q.shape == (prm.nq,) + tuple(reversed([v + 2 * g for v in n])), Q.shape
     F = [np.zeros(face_shape(neq, n, a)) for a in range(d)]
     if S is None:
         S = np.zeros((neq,) + tuple(reversed(n)))

@@ -29,10 +29,15 @@ def flux_divergence(F, S, dx):
 
 
 def oracle_rhs_periodic(prm, Qi, n, dx, dt, g=4):
-    """Oracle RHS (flux divergence, dt-scaled) on a periodic domain treated as one patch."""
+    """Oracle RHS (flux divergence, dt-scaled) on a periodic domain treated as
+    one patch; for the five-equation model the source is part of the RHS and
+    the last volume fraction's row is zero."""
     Qg = W.fill_periodic(Qi, g)
     F, S = O.rhs_patch(prm, Qg, n, g, dx, dt)
-    return flux_divergence(F, S, dx), F, S
+    R = flux_divergence(F, S, dx)
+    if prm.nq > prm.neq:
+        R = np.concatenate([R, np.zeros_like(R[:1])], axis=0)
+    return R, F, S
 
 
 def oracle_rk3(prm, Qi, n, dx, dt, nsteps):
@@ -41,6 +46,7 @@ def oracle_rk3(prm, Qi, n, dx, dt, nsteps):
     Q = Qi.copy()
     for _ in range(nsteps):
         Qint, R = [], []
+        # (the zero-fill of Euler::advanceSingleStepOnPatch covers all stored arrays)
         for sn in range(3):
             Qint.append(Q.copy())
             R.append(oracle_rhs_periodic(prm, Q, n, dx, dt)[0])
@@ -50,6 +56,12 @@ def oracle_rk3(prm, Qi, n, dx, dt, nsteps):
                     Qn = Qn + RK_ALPHA[sn][m] * Qint[m]
                 if RK_BETA[sn][m] != 0.0:
                     Qn = Qn + RK_BETA[sn][m] * R[m]
+            if prm.nq > prm.neq:
+                # last volume fraction = 1 - sum of the others (FlowModelFiveEqnAllaire.cpp:1739-1890)
+                ns = prm.num_species
+                Qn[-1] = 1.0
+                for si in range(ns - 1):
+                    Qn[-1] -= Qn[prm.neq - (ns - 1) + si]
             Q = Qn
     return Q
 
