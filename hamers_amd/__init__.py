@@ -19,3 +19,11 @@ from .reconstructor import (  # noqa: E402,F401
     PatchData,
     PatchList,
 )
+from .integrator import (  # noqa: E402,F401
+    SSP_RK3_ALPHA,
+    SSP_RK3_BETA,
+    SSP_RK3_GAMMA,
+    Level,
+    RKPatchData,
+    advanceSingleStepOnPatch,
+)

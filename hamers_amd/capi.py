@@ -9,6 +9,7 @@ from . import LIB_PATH
 HM_MAX_EQN = 12
 HM_MAX_SPECIES = 8
 HM_NUM_STAGES = 5
+HM_MAX_RK_COEFFS = 5
 
 HM_OK = 0
 HM_ERR_INVALID_ARGUMENT = 1
@@ -48,6 +49,21 @@ class hm_patch_view(ctypes.Structure):
         ("Q", ctypes.c_void_p * HM_MAX_EQN),
         ("F", ctypes.c_void_p * 3),
         ("S", ctypes.c_void_p),
+    ]
+
+
+class hm_rk_patch_view(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int * 3),
+        ("ghost", ctypes.c_int * 3),
+        ("ghost_intermediate", ctypes.c_int * 3),
+        ("dx", ctypes.c_double * 3),
+        ("Q", ctypes.c_void_p * HM_MAX_EQN),
+        ("Q_intermediate", (ctypes.c_void_p * HM_MAX_EQN) * HM_MAX_RK_COEFFS),
+        ("F_intermediate", (ctypes.c_void_p * 3) * HM_MAX_RK_COEFFS),
+        ("S_intermediate", ctypes.c_void_p * HM_MAX_RK_COEFFS),
+        ("F_accumulated", ctypes.c_void_p * 3),
+        ("S_accumulated", ctypes.c_void_p),
     ]
 
 
@@ -95,6 +111,24 @@ def _load():
     L.hm_profile_read.restype = ctypes.c_int
     L.hm_synchronize.argtypes = [P, P]
     L.hm_synchronize.restype = ctypes.c_int
+    D = ctypes.POINTER(ctypes.c_double)
+    I = ctypes.POINTER(ctypes.c_int)
+    for name in ("hm_advance_single_step_on_patch",):
+        getattr(L, name).argtypes = [P, ctypes.POINTER(hm_rk_patch_view), ctypes.c_int, D, D, D, P]
+    L.hm_advance_single_step_on_patches.argtypes = [P, ctypes.POINTER(hm_rk_patch_view), ctypes.c_int,
+                                                    ctypes.c_int, D, D, D, P]
+    L.hm_level_create.argtypes = [P, ctypes.c_int, I, I, I, I, I, D, ctypes.POINTER(P)]
+    L.hm_level_destroy.argtypes = [P]
+    L.hm_level_set_rk_coefficients.argtypes = [P, ctypes.c_int, D, D, D]
+    L.hm_level_write_patch.argtypes = [P, ctypes.c_int, D]
+    L.hm_level_read_patch.argtypes = [P, ctypes.c_int, D]
+    L.hm_level_patch_array.argtypes = [P, ctypes.c_int, ctypes.POINTER(P), I]
+    L.hm_level_fill_ghosts.argtypes = [P, P]
+    L.hm_level_advance.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.c_int, P]
+    for name in ("hm_advance_single_step_on_patch", "hm_advance_single_step_on_patches", "hm_level_create",
+                 "hm_level_destroy", "hm_level_set_rk_coefficients", "hm_level_write_patch",
+                 "hm_level_read_patch", "hm_level_patch_array", "hm_level_fill_ghosts", "hm_level_advance"):
+        getattr(L, name).restype = ctypes.c_int
     return L
 
 

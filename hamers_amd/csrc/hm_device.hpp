@@ -78,12 +78,13 @@ __device__ __forceinline__ int64_t q_idx(const DevPatch& P, int i, int j, int k)
 // patches.  Numbers in [start+cnt, next start) are holes.
 constexpr int kSegAlign = 64;
 
-struct SegSingle {  // one patch, up to 3 segments (one per axis), all by value
-    DevPatch p;
-    int64_t start[4];
-    int64_t cnt[3];
+template <class T>
+struct SegSingleT {  // one patch, up to 4 segments (one per axis, one extra), all by value
+    T p;
+    int64_t start[5];
+    int64_t cnt[4];
     int nseg;
-    __device__ __forceinline__ bool find(int64_t gid, const DevPatch*& P, int& seg_axis, int64_t& loc) const {
+    __device__ __forceinline__ bool find(int64_t gid, const T*& P, int& seg_axis, int64_t& loc) const {
         if (gid >= start[nseg]) return false;
         int s = 0;
         while (s + 1 < nseg && gid >= start[s + 1]) s++;
@@ -95,13 +96,14 @@ struct SegSingle {  // one patch, up to 3 segments (one per axis), all by value
     }
 };
 
-struct SegTable {  // many patches: device arrays
-    const DevPatch* patches;
+template <class T>
+struct SegTableT {  // many patches: device arrays
+    const T* patches;
     const int64_t* start;  // nseg + 1 entries
     const int64_t* cnt;    // nseg entries
     const int* seg;        // per segment: patch * 4 + axis
     int nseg;
-    __device__ __forceinline__ bool find(int64_t gid, const DevPatch*& P, int& seg_axis, int64_t& loc) const {
+    __device__ __forceinline__ bool find(int64_t gid, const T*& P, int& seg_axis, int64_t& loc) const {
         if (gid >= start[nseg]) return false;
         int lo = 0, hi = nseg - 1;  // last segment with start <= gid
         while (lo < hi) {
@@ -119,5 +121,8 @@ struct SegTable {  // many patches: device arrays
         return true;
     }
 };
+
+using SegSingle = SegSingleT<DevPatch>;
+using SegTable = SegTableT<DevPatch>;
 
 }  // namespace hm

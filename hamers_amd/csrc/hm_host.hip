@@ -1,581 +1,6 @@
-// hm_host.hip -- host layer: the GPU convective flux reconstructor and its C-ABI.
-//
-// ConvectiveFluxReconstructorGPU mirrors the reference's
-// ConvectiveFluxReconstructor interface (ConvectiveFluxReconstructor.hpp:23-124)
-// for the WCNS56 hybrid HLLC / HLLC-HLL schemes, and the manager function
-// mirrors ConvectiveFluxReconstructorManager (ConvectiveFluxReconstructorManager.cpp:14-153).
-// The extern "C" functions at the end are the library's ABI
+// hm_host.hip -- C-ABI of the GPU convective flux reconstructor
 // (include/hamers_convective.h).
-#include <hip/hip_runtime.h>
-
-#include <cstdint>
-#include <cstdio>
-#include <cstring>
-#include <memory>
-#include <new>
-#include <string>
-#include <vector>
-
-#include "hm_kernels.hpp"
-
-namespace hm {
-
-namespace {
-
-constexpr size_t kAlignDoubles = 32;  // 256-byte alignment of every scratch array
-
-size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-int64_t align_items(int64_t v) { return (v + kSegAlign - 1) / kSegAlign * kSegAlign; }
-
-uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull) {
-    const unsigned char* p = static_cast<const unsigned char*>(data);
-    for (size_t i = 0; i < n; i++) {
-        h ^= p[i];
-        h *= 1099511628211ull;
-    }
-    return h;
-}
-
-}  // namespace
-
-int num_equations(int model, int dim, int ns) {
-    switch (model) {
-        case HM_MODEL_SINGLE_SPECIES: return dim + 2;                 // FlowModelSingleSpecies
-        case HM_MODEL_FOUR_EQN_CONSERVATIVE: return ns + dim + 1;     // FlowModelFourEqnConservative
-        case HM_MODEL_FIVE_EQN_ALLAIRE: return dim + 2 * ns;          // FlowModelFiveEqnAllaire.cpp:30
-        default: return -1;
-    }
-}
-
-// Sizes (in doubles) of one patch's scratch arrays.
-struct ScratchLayout {
-    size_t prim, sens, fmid[3], ustar[3], total;
-};
-
-class ConvectiveFluxReconstructorGPU {
-   public:
-    explicit ConvectiveFluxReconstructorGPU(const hm_config& cfg) : cfg_(cfg) {
-        neq_ = num_equations(cfg.model, cfg.dim, cfg.num_species);
-        k_.p = cfg.constant_p;
-        k_.q = cfg.constant_q;
-        k_.C = cfg.constant_C;
-        k_.alpha_tau = cfg.constant_alpha_tau;
-        for (int i = 0; i < HM_MAX_SPECIES; i++) {
-            k_.gamma[i] = cfg.species_gamma[i];
-            k_.R[i] = cfg.species_R[i];
-        }
-        k_.num_species = cfg.num_species;
-        k_.dt = 0.0;
-        has_advective_ = (cfg.model == HM_MODEL_FIVE_EQN_ALLAIRE && cfg.num_species > 1);
-        plans_.reserve(kMaxPlans);  // plan addresses stay valid while cached
-    }
-
-    ~ConvectiveFluxReconstructorGPU() {
-        int prev = -1;
-        (void)hipGetDevice(&prev);
-        (void)hipSetDevice(cfg_.device);
-        (void)hipDeviceSynchronize();
-        for (auto& pl : plans_) free_plan(pl);
-        for (auto e : prof_events_) (void)hipEventDestroy(e);
-        if (arena_) (void)hipFree(arena_);
-        if (prev >= 0) (void)hipSetDevice(prev);
-    }
-
-    int numEqn() const { return neq_; }
-
-    // Conservative arrays a patch view must provide: the five-equation model
-    // also hands over the last species' volume fraction (stored, not evolved).
-    int numConservativeArrays() const { return cfg_.model == HM_MODEL_FIVE_EQN_ALLAIRE ? neq_ + 1 : neq_; }
-
-    // Scratch components per cell of the 4-ghost box: V, c (and rho for the
-    // five-equation model).
-    int prim_comps() const { return cfg_.model == HM_MODEL_FIVE_EQN_ALLAIRE ? neq_ + 2 : neq_ + 1; }
-
-    // ConvectiveFluxReconstructor::getConvectiveFluxNumberOfGhostCells (4 for WCNS56).
-    int getConvectiveFluxNumberOfGhostCells() const { return kGc; }
-
-    int computeConvectiveFluxAndSourceOnPatch(const hm_patch_view& v, double time, double dt, int rk_step,
-                                              hipStream_t s) {
-        (void)time;
-        (void)rk_step;
-        if (int st = check_view(v, 0)) return st;
-        if (int st = select_device()) return st;
-        const ScratchLayout L = layout(v);
-        if (int st = ensure_arena(L.total, s)) return st;
-        DevPatch dp = make_dev_patch(v, arena_, L);
-        Plan<SegSingle> plan{};
-        const int d = cfg_.dim;
-        const Extents e = extents(v);
-        // derived cells
-        plan.work[0].p = dp;
-        plan.work[0].nseg = 1;
-        plan.work[0].start[0] = 0;
-        plan.work[0].cnt[0] = e.vol4;
-        plan.work[0].start[1] = align_items(e.vol4);
-        plan.items[0] = plan.work[0].start[1];
-        // sensor cells
-        plan.work[1].p = dp;
-        plan.work[1].nseg = 1;
-        plan.work[1].start[0] = 0;
-        plan.work[1].cnt[0] = (d > 1) ? e.vol2 : 0;
-        plan.work[1].start[1] = align_items(plan.work[1].cnt[0]);
-        plan.items[1] = plan.work[1].start[1];
-        // midpoint faces and output faces, one segment per axis
-        for (int stage = 2; stage < 4; stage++) {
-            SegSingle& w = plan.work[stage];
-            w.p = dp;
-            w.nseg = d;
-            int64_t acc = 0;
-            for (int a = 0; a < d; a++) {
-                w.start[a] = acc;
-                w.cnt[a] = (stage == 2) ? e.mid[a] : e.face[a];
-                acc += align_items(w.cnt[a]);
-            }
-            w.start[d] = acc;
-            plan.items[stage] = acc;
-        }
-        // advective source cells
-        plan.work[4].p = dp;
-        plan.work[4].nseg = 1;
-        plan.work[4].start[0] = 0;
-        plan.work[4].cnt[0] = has_advective_ ? e.vol0 : 0;
-        plan.work[4].start[1] = align_items(plan.work[4].cnt[0]);
-        plan.items[4] = plan.work[4].start[1];
-        KParams k = k_;
-        k.dt = dt;
-        return run(plan, k, s);
-    }
-
-    int computeConvectiveFluxAndSourceOnPatches(const hm_patch_view* views, int np, double time, double dt,
-                                                int rk_step, hipStream_t s) {
-        (void)time;
-        (void)rk_step;
-        if (np < 0 || (np > 0 && !views)) return fail(HM_ERR_INVALID_ARGUMENT, "invalid patch list");
-        if (np == 0) return HM_OK;
-        for (int i = 0; i < np; i++)
-            if (int st = check_view(views[i], i)) return st;
-        if (int st = select_device()) return st;
-        size_t total = 0;
-        for (int i = 0; i < np; i++) total += layout(views[i]).total;
-        if (int st = ensure_arena(total, s)) return st;
-        TablePlan* tp = nullptr;
-        if (int st = get_table_plan(views, np, s, tp)) return st;
-        KParams k = k_;
-        k.dt = dt;
-        return run(tp->plan, k, s);
-    }
-
-    const std::string& lastError() const { return err_; }
-
-    // Stage timing with HIP events recorded on the launch stream around every
-    // pipeline stage (0 derived, 1 sensor, 2 midpoint flux, 3 face flux, 4 source).
-    int setProfiling(int on) {
-        if (int st = select_device()) return st;
-        if (on && prof_events_.empty()) {
-            for (int i = 0; i < kProfSlots; i++) {
-                hipEvent_t e;
-                if (hipEventCreate(&e) != hipSuccess) return fail(HM_ERR_DEVICE, "hipEventCreate failed");
-                prof_events_.push_back(e);
-            }
-        }
-        profiling_ = on != 0;
-        prof_used_ = 0;
-        for (int i = 0; i < kStages; i++) {
-            prof_ms_[i] = 0.0;
-            prof_n_[i] = 0;
-        }
-        return HM_OK;
-    }
-
-    // Fold recorded event pairs into per-stage totals (waits for them).
-    int readProfile(double* ms, int64_t* launches) {
-        if (int st = collect_profile()) return st;
-        for (int i = 0; i < kStages; i++) {
-            ms[i] = prof_ms_[i];
-            launches[i] = prof_n_[i];
-        }
-        return HM_OK;
-    }
-
-    int fail(int code, const std::string& msg) {
-        err_ = msg;
-        return code;
-    }
-
-   private:
-    struct Extents {
-        int64_t vol4, vol2, vol0, mid[3], face[3];
-    };
-
-    struct TablePlan {
-        uint64_t key = 0;
-        std::vector<hm_patch_view> views;
-        void* dev = nullptr;   // device copy of the tables
-        size_t dev_bytes = 0;
-        Plan<SegTable> plan{};
-        void* arena_at_build = nullptr;
-        uint64_t use = 0;
-    };
-
-    Extents extents(const hm_patch_view& v) const {
-        const int d = cfg_.dim;
-        Extents e{};
-        int64_t n[3] = {1, 1, 1};
-        for (int a = 0; a < d; a++) n[a] = v.n[a];
-        auto grown = [&](int g) {
-            int64_t r = 1;
-            for (int a = 0; a < 3; a++) r *= (a < d) ? n[a] + 2 * g : 1;
-            return r;
-        };
-        e.vol4 = grown(kGc);
-        e.vol2 = grown(kGs);
-        e.vol0 = grown(0);
-        for (int a = 0; a < 3; a++) {
-            e.mid[a] = 0;
-            e.face[a] = 0;
-            if (a >= d) continue;
-            int64_t m = 1, f = 1;
-            for (int b = 0; b < 3; b++) {
-                m *= (b == a) ? n[b] + 3 : n[b];
-                f *= (b == a) ? n[b] + 1 : n[b];
-            }
-            e.mid[a] = m;
-            e.face[a] = f;
-        }
-        return e;
-    }
-
-    ScratchLayout layout(const hm_patch_view& v) const {
-        const Extents e = extents(v);
-        ScratchLayout L{};
-        L.prim = align_up(size_t(prim_comps()) * e.vol4, kAlignDoubles);
-        L.sens = (cfg_.dim > 1) ? align_up(size_t(2) * e.vol2, kAlignDoubles) : 0;
-        L.total = L.prim + L.sens;
-        for (int a = 0; a < 3; a++) {
-            L.fmid[a] = align_up(size_t(neq_) * e.mid[a], kAlignDoubles);
-            L.ustar[a] = has_advective_ ? align_up(size_t(e.mid[a]), kAlignDoubles) : 0;
-            L.total += L.fmid[a] + L.ustar[a];
-        }
-        return L;
-    }
-
-    DevPatch make_dev_patch(const hm_patch_view& v, double* base, const ScratchLayout& L) const {
-        DevPatch p{};
-        const int d = cfg_.dim;
-        for (int a = 0; a < 3; a++) {
-            p.n[a] = (a < d) ? v.n[a] : 1;
-            p.gq[a] = (a < d) ? v.ghost[a] : 0;
-            p.dx[a] = (a < d) ? v.dx[a] : 1.0;
-        }
-        const int nq = numConservativeArrays();
-        for (int e = 0; e < HM_MAX_EQN; e++) p.Q[e] = (e < nq) ? v.Q[e] : nullptr;
-        for (int a = 0; a < 3; a++) p.F[a] = (a < d) ? v.F[a] : nullptr;
-        p.S = v.S;
-        double* q = base;
-        p.prim = q;
-        q += L.prim;
-        p.sens = L.sens ? q : nullptr;
-        q += L.sens;
-        for (int a = 0; a < 3; a++) {
-            p.fmid[a] = L.fmid[a] ? q : nullptr;
-            q += L.fmid[a];
-            p.ustar[a] = L.ustar[a] ? q : nullptr;
-            q += L.ustar[a];
-        }
-        return p;
-    }
-
-    int check_view(const hm_patch_view& v, int which) {
-        const int d = cfg_.dim;
-        char buf[160];
-        for (int a = 0; a < d; a++) {
-            if (v.n[a] < 1) {
-                snprintf(buf, sizeof buf, "patch %d: interior extent n[%d]=%d < 1", which, a, v.n[a]);
-                return fail(HM_ERR_INVALID_ARGUMENT, buf);
-            }
-            if (v.ghost[a] < kGc) {
-                snprintf(buf, sizeof buf, "patch %d: ghost width %d on axis %d is below the 4 cells WCNS56 needs",
-                         which, v.ghost[a], a);
-                return fail(HM_ERR_INVALID_ARGUMENT, buf);
-            }
-            if (!(v.dx[a] > 0.0)) {
-                snprintf(buf, sizeof buf, "patch %d: dx[%d] must be positive", which, a);
-                return fail(HM_ERR_INVALID_ARGUMENT, buf);
-            }
-            if (!v.F[a]) {
-                snprintf(buf, sizeof buf, "patch %d: flux array for axis %d is NULL", which, a);
-                return fail(HM_ERR_INVALID_ARGUMENT, buf);
-            }
-        }
-        for (int e = 0; e < numConservativeArrays(); e++)
-            if (!v.Q[e]) {
-                snprintf(buf, sizeof buf, "patch %d: conservative variable %d is NULL", which, e);
-                return fail(HM_ERR_INVALID_ARGUMENT, buf);
-            }
-        if (has_advective_ && !v.S) {
-            snprintf(buf, sizeof buf, "patch %d: source array is NULL but the model has advective equations", which);
-            return fail(HM_ERR_INVALID_ARGUMENT, buf);
-        }
-        return HM_OK;
-    }
-
-    int select_device() {
-        int cur = -1;
-        if (hipGetDevice(&cur) != hipSuccess) return fail(HM_ERR_DEVICE, "hipGetDevice failed");
-        if (cur != cfg_.device && hipSetDevice(cfg_.device) != hipSuccess)
-            return fail(HM_ERR_DEVICE, "hipSetDevice failed");
-        return HM_OK;
-    }
-
-    // Grow the scratch arena to at least `doubles` entries.  Growth waits for
-    // work already queued on `s` (which may still read the old arena) and
-    // invalidates cached batch plans, whose tables hold arena addresses.
-    int ensure_arena(size_t doubles, hipStream_t s) {
-        if (doubles <= arena_doubles_) return HM_OK;
-        size_t want = std::max(doubles, arena_doubles_ + arena_doubles_ / 2);
-        if (arena_) {
-            if (hipStreamSynchronize(s) != hipSuccess) return fail(HM_ERR_DEVICE, "stream synchronize failed");
-            (void)hipFree(arena_);
-            arena_ = nullptr;
-            arena_doubles_ = 0;
-        }
-        for (auto& pl : plans_) free_plan(pl);
-        plans_.clear();
-        if (hipMalloc(reinterpret_cast<void**>(&arena_), want * sizeof(double)) != hipSuccess) {
-            arena_ = nullptr;
-            return fail(HM_ERR_OUT_OF_MEMORY, "hipMalloc of the scratch arena failed");
-        }
-        arena_doubles_ = want;
-        return HM_OK;
-    }
-
-    void free_plan(TablePlan& pl) {
-        if (pl.dev) (void)hipFree(pl.dev);
-        pl.dev = nullptr;
-    }
-
-    // Build (or reuse) the device work tables of a patch list.
-    int get_table_plan(const hm_patch_view* views, int np, hipStream_t s, TablePlan*& out) {
-        const uint64_t key = fnv1a(views, sizeof(hm_patch_view) * size_t(np), fnv1a(&np, sizeof np));
-        use_clock_++;
-        for (auto& pl : plans_) {
-            if (pl.key == key && int(pl.views.size()) == np && pl.arena_at_build == arena_ &&
-                std::memcmp(pl.views.data(), views, sizeof(hm_patch_view) * size_t(np)) == 0) {
-                pl.use = use_clock_;
-                out = &pl;
-                return HM_OK;
-            }
-        }
-        // Host image of the tables: DevPatch[np], then per stage start/cnt/seg.
-        const int d = cfg_.dim;
-        std::vector<DevPatch> patches(np);
-        std::vector<int64_t> start[kStages], cnt[kStages];
-        std::vector<int> seg[kStages];
-        double* base = arena_;
-        for (int i = 0; i < np; i++) {
-            const ScratchLayout L = layout(views[i]);
-            patches[i] = make_dev_patch(views[i], base, L);
-            base += L.total;
-        }
-        int64_t acc[kStages] = {0, 0, 0, 0, 0};
-        for (int i = 0; i < np; i++) {
-            const Extents e = extents(views[i]);
-            auto add = [&](int stage, int axis, int64_t c) {
-                if (c <= 0) return;
-                start[stage].push_back(acc[stage]);
-                cnt[stage].push_back(c);
-                seg[stage].push_back(i * 4 + axis);
-                acc[stage] += align_items(c);
-            };
-            add(0, 0, e.vol4);
-            if (d > 1) add(1, 0, e.vol2);
-            for (int a = 0; a < d; a++) add(2, a, e.mid[a]);
-            for (int a = 0; a < d; a++) add(3, a, e.face[a]);
-            if (has_advective_) add(4, 0, e.vol0);
-        }
-        for (int st = 0; st < kStages; st++) start[st].push_back(acc[st]);
-        // Pack into one buffer.
-        size_t bytes = align_up(sizeof(DevPatch) * size_t(np), 256);
-        size_t off_start[kStages], off_cnt[kStages], off_seg[kStages];
-        for (int st = 0; st < kStages; st++) {
-            off_start[st] = bytes;
-            bytes = align_up(bytes + sizeof(int64_t) * start[st].size(), 256);
-            off_cnt[st] = bytes;
-            bytes = align_up(bytes + sizeof(int64_t) * cnt[st].size(), 256);
-            off_seg[st] = bytes;
-            bytes = align_up(bytes + sizeof(int) * seg[st].size(), 256);
-        }
-        std::vector<char> img(bytes, 0);
-        std::memcpy(img.data(), patches.data(), sizeof(DevPatch) * size_t(np));
-        for (int st = 0; st < kStages; st++) {
-            if (!start[st].empty()) std::memcpy(img.data() + off_start[st], start[st].data(), sizeof(int64_t) * start[st].size());
-            if (!cnt[st].empty()) std::memcpy(img.data() + off_cnt[st], cnt[st].data(), sizeof(int64_t) * cnt[st].size());
-            if (!seg[st].empty()) std::memcpy(img.data() + off_seg[st], seg[st].data(), sizeof(int) * seg[st].size());
-        }
-        // Evict the least recently used plan when the cache is full.
-        TablePlan* pl = nullptr;
-        if (plans_.size() < kMaxPlans) {
-            plans_.emplace_back();
-            pl = &plans_.back();
-        } else {
-            pl = &plans_[0];
-            for (auto& c : plans_)
-                if (c.use < pl->use) pl = &c;
-            if (hipStreamSynchronize(s) != hipSuccess) return fail(HM_ERR_DEVICE, "stream synchronize failed");
-            free_plan(*pl);
-        }
-        if (hipMalloc(&pl->dev, bytes) != hipSuccess) {
-            pl->dev = nullptr;
-            pl->key = 0;
-            return fail(HM_ERR_OUT_OF_MEMORY, "hipMalloc of the work tables failed");
-        }
-        // The tables are built once per distinct patch list, so a blocking copy is fine here.
-        if (hipMemcpy(pl->dev, img.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
-            return fail(HM_ERR_DEVICE, "upload of the work tables failed");
-        pl->dev_bytes = bytes;
-        pl->key = key;
-        pl->views.assign(views, views + np);
-        pl->arena_at_build = arena_;
-        pl->use = use_clock_;
-        char* dv = static_cast<char*>(pl->dev);
-        for (int st = 0; st < kStages; st++) {
-            SegTable& w = pl->plan.work[st];
-            w.patches = reinterpret_cast<const DevPatch*>(dv);
-            w.start = reinterpret_cast<const int64_t*>(dv + off_start[st]);
-            w.cnt = reinterpret_cast<const int64_t*>(dv + off_cnt[st]);
-            w.seg = reinterpret_cast<const int*>(dv + off_seg[st]);
-            w.nseg = int(cnt[st].size());
-            pl->plan.items[st] = acc[st];
-        }
-        out = pl;
-        return HM_OK;
-    }
-
-    template <class W>
-    int run(const Plan<W>& plan, const KParams& k, hipStream_t s) {
-        if (!profiling_) return run_stages(plan, k, s, 0, kStages);
-        // One launch per stage, bracketed by events.
-        for (int stage = 0; stage < kStages; stage++) {
-            if (plan.items[stage] == 0) continue;
-            if (prof_used_ + 2 > int(prof_events_.size()))
-                if (int st = collect_profile()) return st;
-            hipEvent_t e0 = prof_events_[prof_used_], e1 = prof_events_[prof_used_ + 1];
-            if (hipEventRecord(e0, s) != hipSuccess) return fail(HM_ERR_DEVICE, "hipEventRecord failed");
-            if (int st = run_stages(plan, k, s, stage, stage + 1)) return st;
-            if (hipEventRecord(e1, s) != hipSuccess) return fail(HM_ERR_DEVICE, "hipEventRecord failed");
-            prof_stage_[prof_used_ / 2] = stage;
-            prof_used_ += 2;
-        }
-        return HM_OK;
-    }
-
-    template <class W>
-    int run_stages(const Plan<W>& plan, const KParams& k, hipStream_t s, int first, int last) {
-        Plan<W> sub = plan;
-        for (int st = 0; st < kStages; st++)
-            if (st < first || st >= last) sub.items[st] = 0;
-        hipError_t e = hipSuccess;
-        switch (cfg_.model) {
-            case HM_MODEL_SINGLE_SPECIES: e = launch_rhs_single<W>(cfg_.dim, cfg_.scheme, sub, k, s); break;
-            case HM_MODEL_FIVE_EQN_ALLAIRE: e = launch_rhs_five<W>(cfg_.dim, cfg_.scheme, sub, k, s); break;
-            default: return fail(HM_ERR_UNSUPPORTED, "flow model not available in this build");
-        }
-        if (e != hipSuccess) return fail(HM_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
-        return HM_OK;
-    }
-
-    int collect_profile() {
-        for (int i = 0; i + 1 < prof_used_; i += 2) {
-            if (hipEventSynchronize(prof_events_[i + 1]) != hipSuccess)
-                return fail(HM_ERR_DEVICE, "hipEventSynchronize failed");
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, prof_events_[i], prof_events_[i + 1]) != hipSuccess)
-                return fail(HM_ERR_DEVICE, "hipEventElapsedTime failed");
-            prof_ms_[prof_stage_[i / 2]] += ms;
-            prof_n_[prof_stage_[i / 2]] += 1;
-        }
-        prof_used_ = 0;
-        return HM_OK;
-    }
-
-    static constexpr size_t kMaxPlans = 8;
-    static constexpr int kProfSlots = 512;
-    bool profiling_ = false;
-    std::vector<hipEvent_t> prof_events_;
-    int prof_used_ = 0;
-    int prof_stage_[kProfSlots / 2] = {};
-    double prof_ms_[kStages] = {};
-    int64_t prof_n_[kStages] = {};
-    hm_config cfg_;
-    int neq_ = 0;
-    bool has_advective_ = false;
-    KParams k_{};
-    double* arena_ = nullptr;
-    size_t arena_doubles_ = 0;
-    std::vector<TablePlan> plans_;
-    uint64_t use_clock_ = 0;
-    std::string err_;
-};
-
-// ConvectiveFluxReconstructorManager (ConvectiveFluxReconstructorManager.cpp:14-153):
-// validates the configuration the deck would give and builds the reconstructor.
-int make_reconstructor(const hm_config& c, std::unique_ptr<ConvectiveFluxReconstructorGPU>& out, std::string& err) {
-    if (c.dim < 1 || c.dim > 3) {
-        err = "dim must be 1, 2 or 3";
-        return HM_ERR_INVALID_ARGUMENT;
-    }
-    if (c.scheme != HM_SCHEME_WCNS5_JS_HLLC_HLL && c.scheme != HM_SCHEME_WCNS6_LD_HLLC_HLL) {
-        err = "unknown convective flux reconstructor";  // ConvectiveFluxReconstructorManager.cpp:146
-        return HM_ERR_UNSUPPORTED;
-    }
-    if (c.model != HM_MODEL_SINGLE_SPECIES && c.model != HM_MODEL_FIVE_EQN_ALLAIRE) {
-        err = "flow model not available in this build";
-        return HM_ERR_UNSUPPORTED;
-    }
-    const int ns = (c.model == HM_MODEL_SINGLE_SPECIES) ? 1 : c.num_species;
-    if (ns < 1 || ns > HM_MAX_SPECIES || num_equations(c.model, c.dim, ns) + 1 > HM_MAX_EQN) {
-        err = "number of species out of range";
-        return HM_ERR_INVALID_ARGUMENT;
-    }
-    if (c.model == HM_MODEL_FIVE_EQN_ALLAIRE && (ns < 2 || ns > 3)) {
-        err = "the five-equation model is built for 2 or 3 species";
-        return HM_ERR_UNSUPPORTED;
-    }
-    if (c.constant_p < 1 || (c.scheme == HM_SCHEME_WCNS6_LD_HLLC_HLL && c.constant_q < 1)) {
-        err = "WCNS exponents must be positive";
-        return HM_ERR_INVALID_ARGUMENT;
-    }
-    for (int i = 0; i < ns; i++)
-        if (!(c.species_gamma[i] > 1.0)) {
-            err = "species_gamma must exceed 1";
-            return HM_ERR_INVALID_ARGUMENT;
-        }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-        err = "no HIP device available";
-        return HM_ERR_DEVICE;
-    }
-    if (c.device < 0 || c.device >= ndev) {
-        err = "device ordinal out of range";
-        return HM_ERR_INVALID_ARGUMENT;
-    }
-    hm_config cc = c;
-    if (c.model == HM_MODEL_SINGLE_SPECIES) cc.num_species = 1;
-    try {
-        out.reset(new ConvectiveFluxReconstructorGPU(cc));
-    } catch (const std::bad_alloc&) {
-        err = "out of host memory";
-        return HM_ERR_OUT_OF_MEMORY;
-    }
-    return HM_OK;
-}
-
-}  // namespace hm
-
-struct hm_ctx {
-    std::unique_ptr<hm::ConvectiveFluxReconstructorGPU> rec;
-    std::string err;
-};
+#include "hm_ctx.hpp"
 
 extern "C" {
 
@@ -597,6 +22,10 @@ int hm_ctx_create(const hm_config* cfg, hm_ctx** out) {
 }
 
 int hm_ctx_destroy(hm_ctx* ctx) {
+    if (ctx && ctx->levels > 0) {
+        if (ctx->rec) ctx->rec->fail(HM_ERR_INVALID_ARGUMENT, "destroy the levels of this context first");
+        return HM_ERR_INVALID_ARGUMENT;
+    }
     delete ctx;
     return HM_OK;
 }

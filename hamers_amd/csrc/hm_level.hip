@@ -1,0 +1,812 @@
+// hm_level.hip -- one AMR level advanced by the Runge-Kutta stage loop of
+// RungeKuttaLevelIntegrator::advanceLevel (RungeKuttaLevelIntegrator.cpp:1457-1939,
+// stage loop 1672-1745, default SSP-RK3 weights 3907-3924), entirely on the GPU:
+//
+//   fill ghosts of the scratch state (same level + periodic images)      :1568
+//   for each stage sn:
+//     intermediate[sn] <- scratch (whole ghost box)                       :1677
+//     sn > 0: fill ghosts of intermediate[sn] (same level only)           :1686-1711
+//     S[sn] <- 0, F[sn], S[sn] <- batched convective RHS of intermediate  Euler.cpp:904-1001
+//     scratch interior <- stage update from intermediate[0..sn]           Euler.cpp:1003-1680
+//
+// One step is a fixed sequence of launches on one stream, so it is captured
+// once into a hipGraph and replayed.  Ghost cells that no same-level patch
+// covers (coarse-fine boundaries) keep the values written by the caller, as
+// the reference's intermediate fills do not use the coarser level.
+//
+// Also here: the stage-update entry points for callers that keep their own
+// patch data (per patch and batched).
+#include <algorithm>
+#include <cstring>
+#include <map>
+
+#include "hm_ctx.hpp"
+#include "hm_rk.hpp"
+
+namespace hm {
+
+constexpr int kMaxBuf = 1 + kMaxRK;  // scratch + intermediates
+
+struct LevelPatch {
+    int n[3];
+    int64_t gvol;                // ghost-box cells (component stride)
+    double* Q[kMaxBuf];          // buffer b: nq components over the ghost box
+    double* S[kMaxRK];           // source of stage n: neq components over the interior
+};
+
+struct HaloItem {
+    int dst, src;
+    int lo[3];   // first destination cell, in the destination's ghost-box coordinates (0-based)
+    int ext[3];  // extent of the copied box
+    int off[3];  // source ghost-box coordinates = destination + off
+};
+
+// Ghost-cell copy from same-level neighbours (and periodic images).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_halo(SegTableT<HaloItem> w, const LevelPatch* patches, int buf, int nq,
+                                                 int g) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const HaloItem* it;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, it, ax, loc)) return;
+    const int i = int(loc % it->ext[0]);
+    const int64_t r = loc / it->ext[0];
+    const int j = (D > 1) ? int(r % it->ext[1]) : 0;
+    const int k = (D > 2) ? int(r / it->ext[1]) : 0;
+    const LevelPatch& Pd = patches[it->dst];
+    const LevelPatch& Ps = patches[it->src];
+    auto gidx = [&](const LevelPatch& P, int a0, int a1, int a2) -> int64_t {
+        const int e0 = P.n[0] + 2 * g;
+        const int e1 = (D > 1) ? P.n[1] + 2 * g : 1;
+        return a0 + int64_t(a1) * e0 + int64_t(a2) * e0 * e1;
+    };
+    const int d0 = it->lo[0] + i, d1 = it->lo[1] + j, d2 = it->lo[2] + k;
+    const int64_t id = gidx(Pd, d0, d1, d2);
+    const int64_t is = gidx(Ps, d0 + it->off[0], d1 + it->off[1], d2 + it->off[2]);
+    for (int c = 0; c < nq; c++) Pd.Q[buf][c * Pd.gvol + id] = Ps.Q[buf][c * Ps.gvol + is];
+}
+
+// Whole-ghost-box copy between two state buffers of every patch.
+__global__ __launch_bounds__(kBlock) void k_copy_state(SegTableT<LevelPatch> w, int from, int to, int nq) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const LevelPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    for (int c = 0; c < nq; c++) P->Q[to][c * P->gvol + loc] = P->Q[from][c * P->gvol + loc];
+}
+
+// Zero the source of one stage (Euler.cpp:913-926); a kernel, not a memset, so
+// that the step can be captured in a graph.
+__global__ __launch_bounds__(kBlock) void k_zero_source(SegTableT<LevelPatch> w, int stage) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const LevelPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    P->S[stage][loc] = 0.0;
+}
+
+namespace {
+
+size_t up256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Segment tables packed into one host image (see the batched RHS plans).
+template <class T>
+struct TableImage {
+    std::vector<T> items;
+    std::vector<int64_t> start, cnt;
+    std::vector<int> seg;
+    int64_t total = 0;
+    void add(int item, int axis, int64_t count) {
+        if (count <= 0) return;
+        start.push_back(total);
+        cnt.push_back(count);
+        seg.push_back(item * 4 + axis);
+        total += (count + kSegAlign - 1) / kSegAlign * kSegAlign;
+    }
+    size_t bytes() const {
+        return up256(sizeof(T) * items.size()) + up256(sizeof(int64_t) * (start.size() + 1)) +
+               up256(sizeof(int64_t) * cnt.size()) + up256(sizeof(int) * seg.size());
+    }
+    // Writes the image at host pointer h, returns the device view for device base d.
+    SegTableT<T> write(char* h, char* d) {
+        std::vector<int64_t> st = start;
+        st.push_back(total);
+        size_t o = 0;
+        SegTableT<T> t{};
+        std::memcpy(h + o, items.data(), sizeof(T) * items.size());
+        t.patches = reinterpret_cast<const T*>(d + o);
+        o += up256(sizeof(T) * items.size());
+        std::memcpy(h + o, st.data(), sizeof(int64_t) * st.size());
+        t.start = reinterpret_cast<const int64_t*>(d + o);
+        o += up256(sizeof(int64_t) * st.size());
+        if (!cnt.empty()) std::memcpy(h + o, cnt.data(), sizeof(int64_t) * cnt.size());
+        t.cnt = reinterpret_cast<const int64_t*>(d + o);
+        o += up256(sizeof(int64_t) * cnt.size());
+        if (!seg.empty()) std::memcpy(h + o, seg.data(), sizeof(int) * seg.size());
+        t.seg = reinterpret_cast<const int*>(d + o);
+        t.nseg = int(cnt.size());
+        return t;
+    }
+};
+
+unsigned blocks_for(int64_t items) { return unsigned((items + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+class Level {
+   public:
+    Level(ConvectiveFluxReconstructorGPU* rec) : rec_(rec) {}
+
+    ~Level() {
+        (void)rec_->selectDevice();
+        (void)hipDeviceSynchronize();
+        for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+        for (auto& p : rhs_plan_) rec_->releaseOwnedPlan(p);
+        if (rhs_scratch_) (void)hipFree(rhs_scratch_);
+        if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
+        if (ev_a_) (void)hipEventDestroy(ev_a_);
+        if (ev_b_) (void)hipEventDestroy(ev_b_);
+        if (tables_) (void)hipFree(tables_);
+        if (storage_) (void)hipFree(storage_);
+    }
+
+    int init(int np, const int* lo, const int* n, const int* dlo, const int* dn, const int* periodic, const double* dx) {
+        const hm_config& c = rec_->config();
+        d_ = c.dim;
+        neq_ = rec_->numEqn();
+        nq_ = rec_->numConservativeArrays();
+        g_ = kGc;
+        np_ = np;
+        if (np < 1) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "a level needs at least one patch");
+        for (int a = 0; a < 3; a++) {
+            dlo_[a] = a < d_ ? dlo[a] : 0;
+            dn_[a] = a < d_ ? dn[a] : 1;
+            periodic_[a] = a < d_ ? periodic[a] != 0 : false;
+            dx_[a] = a < d_ ? dx[a] : 1.0;
+        }
+        lo_.assign(size_t(np) * 3, 0);
+        n_.assign(size_t(np) * 3, 1);
+        for (int p = 0; p < np; p++)
+            for (int a = 0; a < d_; a++) {
+                lo_[3 * p + a] = lo[3 * p + a];
+                n_[3 * p + a] = n[3 * p + a];
+                if (n_[3 * p + a] < 1) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "patch extent below 1");
+                if (lo_[3 * p + a] < dlo_[a] || lo_[3 * p + a] + n_[3 * p + a] > dlo_[a] + dn_[a])
+                    return rec_->fail(HM_ERR_INVALID_ARGUMENT, "patch outside the domain");
+            }
+        for (int a = 0; a < d_; a++)
+            if (periodic_[a] && dn_[a] < g_)
+                return rec_->fail(HM_ERR_INVALID_ARGUMENT, "periodic domain narrower than the ghost width");
+        // default SSP-RK3 (RungeKuttaLevelIntegrator.cpp:3907-3924)
+        const double a3[3][3] = {{1.0, 0, 0}, {3.0 / 4.0, 1.0 / 4.0, 0}, {1.0 / 3.0, 0.0, 2.0 / 3.0}};
+        const double b3[3][3] = {{1.0, 0, 0}, {0.0, 1.0 / 4.0, 0}, {0.0, 0.0, 2.0 / 3.0}};
+        const double c3[3][3] = {{1.0 / 6.0, 0, 0}, {0.0, 1.0 / 6.0, 0}, {0.0, 0.0, 2.0 / 3.0}};
+        nst_ = 3;
+        for (int s = 0; s < kMaxRK; s++)
+            for (int m = 0; m < kMaxRK; m++) {
+                alpha_[s][m] = (s < 3 && m < 3) ? a3[s][m] : 0.0;
+                beta_[s][m] = (s < 3 && m < 3) ? b3[s][m] : 0.0;
+                gamma_[s][m] = (s < 3 && m < 3) ? c3[s][m] : 0.0;
+            }
+        return allocate();
+    }
+
+    int setCoefficients(int nst, const double* a, const double* b, const double* c) {
+        if (nst < 1 || nst > kMaxRK) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "1..5 Runge-Kutta stages");
+        nst_ = nst;
+        for (int s = 0; s < kMaxRK; s++)
+            for (int m = 0; m < kMaxRK; m++) {
+                const bool in = s < nst && m <= s;
+                alpha_[s][m] = in ? a[s * nst + m] : 0.0;
+                beta_[s][m] = in ? b[s * nst + m] : 0.0;
+                gamma_[s][m] = in && c ? c[s * nst + m] : 0.0;
+            }
+        // Launches already queued read the tables rewritten below.
+        if (hipDeviceSynchronize() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "device synchronize failed");
+        drop_graphs();
+        return build_rk_tables();
+    }
+
+    // Interior cells only: ghost cells keep their values.
+    int writePatch(int p, const double* Q) {
+        if (p < 0 || p >= np_ || !Q) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "bad patch index or array");
+        std::vector<double> h(size_t(nq_) * gvol(p));
+        if (hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(h.data(), patch_[p].Q[0], h.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            return rec_->fail(HM_ERR_DEVICE, "hipMemcpy from the level failed");
+        const double* src = Q;
+        const int64_t vol = ivol(p);
+        for (int c = 0; c < nq_; c++)
+            for (int64_t l = 0; l < vol; l++) h[size_t(c) * gvol(p) + gidx_of_interior(p, l)] = src[c * vol + l];
+        if (hipMemcpy(patch_[p].Q[0], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            return rec_->fail(HM_ERR_DEVICE, "hipMemcpy to the level failed");
+        return HM_OK;
+    }
+
+    int readPatch(int p, double* Q) {
+        if (p < 0 || p >= np_ || !Q) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "bad patch index or array");
+        std::vector<double> h(size_t(nq_) * gvol(p));
+        if (hipDeviceSynchronize() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "device synchronize failed");
+        if (hipMemcpy(h.data(), patch_[p].Q[0], h.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            return rec_->fail(HM_ERR_DEVICE, "hipMemcpy from the level failed");
+        const int64_t vol = ivol(p);
+        for (int c = 0; c < nq_; c++)
+            for (int64_t l = 0; l < vol; l++) Q[c * vol + l] = h[size_t(c) * gvol(p) + gidx_of_interior(p, l)];
+        return HM_OK;
+    }
+
+    int patchArray(int p, double** Q, int* ghost) {
+        if (p < 0 || p >= np_ || !Q) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "bad patch index");
+        *Q = patch_[p].Q[0];
+        if (ghost) *ghost = g_;
+        return HM_OK;
+    }
+
+    int fillGhosts(hipStream_t s) { return launch_halo(0, s); }
+
+    // num_steps steps of the configured Runge-Kutta scheme with fixed dt.
+    int advance(double dt, int nsteps, bool use_graph, hipStream_t s) {
+        if (nsteps < 0 || !(dt > 0.0)) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "bad dt or step count");
+        if (int st = rec_->selectDevice()) return st;
+        if (!use_graph) {
+            for (int it = 0; it < nsteps; it++)
+                if (int st = step(dt, s)) return st;
+            return HM_OK;
+        }
+        if (nsteps == 0) return HM_OK;
+        auto itg = graphs_.find(dt);
+        if (itg == graphs_.end()) {
+            // Every table and scratch array exists since init, so the captured
+            // step allocates and uploads nothing.
+            if (int st = capture(dt)) return st;
+            itg = graphs_.find(dt);
+        }
+        if (hipEventRecord(ev_a_, s) != hipSuccess || hipStreamWaitEvent(cap_stream_, ev_a_, 0) != hipSuccess)
+            return rec_->fail(HM_ERR_DEVICE, "stream ordering failed");
+        for (int it = 0; it < nsteps; it++)
+            if (hipGraphLaunch(itg->second, cap_stream_) != hipSuccess)
+                return rec_->fail(HM_ERR_DEVICE, "hipGraphLaunch failed");
+        if (hipEventRecord(ev_b_, cap_stream_) != hipSuccess || hipStreamWaitEvent(s, ev_b_, 0) != hipSuccess)
+            return rec_->fail(HM_ERR_DEVICE, "stream ordering failed");
+        return HM_OK;
+    }
+
+    int numPatches() const { return np_; }
+    int ghost() const { return g_; }
+    int numArrays() const { return nq_; }
+
+   private:
+    int64_t gvol(int p) const {
+        int64_t v = 1;
+        for (int a = 0; a < d_; a++) v *= n_[3 * p + a] + 2 * g_;
+        return v;
+    }
+    int64_t ivol(int p) const {
+        int64_t v = 1;
+        for (int a = 0; a < d_; a++) v *= n_[3 * p + a];
+        return v;
+    }
+    int64_t fvol(int p, int ax) const {
+        int64_t v = 1;
+        for (int a = 0; a < d_; a++) v *= n_[3 * p + a] + (a == ax);
+        return v;
+    }
+    int64_t gidx_of_interior(int p, int64_t l) const {
+        const int* n = &n_[3 * p];
+        int c[3] = {0, 0, 0};
+        c[0] = int(l % n[0]);
+        if (d_ > 1) c[1] = int((l / n[0]) % n[1]);
+        if (d_ > 2) c[2] = int(l / (int64_t(n[0]) * n[1]));
+        int64_t idx = 0, st = 1;
+        for (int a = 0; a < d_; a++) {
+            idx += (c[a] + g_) * st;
+            st *= n[a] + 2 * g_;
+        }
+        return idx;
+    }
+
+    int allocate() {
+        // Storage: per patch, kMaxBuf state buffers, kMaxRK flux and source sets.
+        size_t total = 0;
+        std::vector<size_t> off(np_);
+        for (int p = 0; p < np_; p++) {
+            off[p] = total;
+            total += up256(sizeof(double) * size_t(nq_) * gvol(p)) * kMaxBuf;
+            for (int s = 0; s < kMaxRK; s++) {
+                for (int a = 0; a < d_; a++) total += up256(sizeof(double) * size_t(neq_) * fvol(p, a));
+                total += up256(sizeof(double) * size_t(neq_) * ivol(p));
+            }
+        }
+        if (hipMalloc(&storage_, total) != hipSuccess) return rec_->fail(HM_ERR_OUT_OF_MEMORY, "level storage");
+        if (hipMemset(storage_, 0, total) != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "level storage clear");
+        patch_.resize(np_);
+        fl_.assign(size_t(np_) * kMaxRK * 3, nullptr);
+        for (int p = 0; p < np_; p++) {
+            char* b = static_cast<char*>(storage_) + off[p];
+            LevelPatch& L = patch_[p];
+            for (int a = 0; a < 3; a++) L.n[a] = a < d_ ? n_[3 * p + a] : 1;
+            L.gvol = gvol(p);
+            for (int q = 0; q < kMaxBuf; q++) {
+                L.Q[q] = reinterpret_cast<double*>(b);
+                b += up256(sizeof(double) * size_t(nq_) * gvol(p));
+            }
+            for (int s = 0; s < kMaxRK; s++) {
+                for (int a = 0; a < d_; a++) {
+                    fl_[(size_t(p) * kMaxRK + s) * 3 + a] = reinterpret_cast<double*>(b);
+                    b += up256(sizeof(double) * size_t(neq_) * fvol(p, a));
+                }
+                L.S[s] = reinterpret_cast<double*>(b);
+                b += up256(sizeof(double) * size_t(neq_) * ivol(p));
+            }
+        }
+        // RHS views of every intermediate buffer.
+        views_.assign(kMaxRK, std::vector<hm_patch_view>(np_));
+        for (int s = 0; s < kMaxRK; s++)
+            for (int p = 0; p < np_; p++) {
+                hm_patch_view v{};
+                for (int a = 0; a < 3; a++) {
+                    v.n[a] = a < d_ ? n_[3 * p + a] : 1;
+                    v.ghost[a] = a < d_ ? g_ : 0;
+                    v.dx[a] = dx_[a];
+                }
+                for (int e = 0; e < nq_; e++) v.Q[e] = patch_[p].Q[1 + s] + size_t(e) * gvol(p);
+                for (int a = 0; a < d_; a++) v.F[a] = fl_[(size_t(p) * kMaxRK + s) * 3 + a];
+                v.S = patch_[p].S[s];
+                views_[s][p] = v;
+            }
+        // RHS scratch: the stages run one after another, so they share it.
+        const size_t sd = rec_->scratchDoubles(views_[0].data(), np_);
+        if (hipMalloc(&rhs_scratch_, sd * sizeof(double)) != hipSuccess)
+            return rec_->fail(HM_ERR_OUT_OF_MEMORY, "level RHS scratch");
+        for (int s = 0; s < kMaxRK; s++)
+            if (int st = rec_->buildOwnedPlan(views_[s].data(), np_, static_cast<double*>(rhs_scratch_), rhs_plan_[s]))
+                return st;
+        if (hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&ev_a_, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ev_b_, hipEventDisableTiming) != hipSuccess)
+            return rec_->fail(HM_ERR_DEVICE, "stream or event creation failed");
+        return build_tables();
+    }
+
+    // Same-level ghost copies, periodic images included.
+    void build_halo(TableImage<HaloItem>& img) {
+        int shifts[3][3];
+        int nsh[3];
+        for (int a = 0; a < 3; a++) {
+            nsh[a] = 1;
+            shifts[a][0] = 0;
+            if (a < d_ && periodic_[a]) {
+                shifts[a][1] = -dn_[a];
+                shifts[a][2] = dn_[a];
+                nsh[a] = 3;
+            }
+        }
+        for (int p = 0; p < np_; p++) {
+            int glo[3], ghi[3];  // ghost box of p (inclusive, global index space)
+            for (int a = 0; a < 3; a++) {
+                const bool used = a < d_;
+                glo[a] = used ? lo_[3 * p + a] - g_ : 0;
+                ghi[a] = used ? lo_[3 * p + a] + n_[3 * p + a] - 1 + g_ : 0;
+            }
+            for (int q = 0; q < np_; q++)
+                for (int s0 = 0; s0 < nsh[0]; s0++)
+                    for (int s1 = 0; s1 < nsh[1]; s1++)
+                        for (int s2 = 0; s2 < nsh[2]; s2++) {
+                            const int sh[3] = {shifts[0][s0], shifts[1][s1], shifts[2][s2]};
+                            if (q == p && sh[0] == 0 && sh[1] == 0 && sh[2] == 0) continue;
+                            int blo[3], bhi[3];
+                            bool empty = false;
+                            for (int a = 0; a < 3; a++) {
+                                const bool used = a < d_;
+                                const int qlo = used ? lo_[3 * q + a] + sh[a] : 0;
+                                const int qhi = used ? lo_[3 * q + a] + n_[3 * q + a] - 1 + sh[a] : 0;
+                                blo[a] = std::max(glo[a], qlo);
+                                bhi[a] = std::min(ghi[a], qhi);
+                                if (blo[a] > bhi[a]) empty = true;
+                            }
+                            if (empty) continue;
+                            HaloItem it{};
+                            it.dst = p;
+                            it.src = q;
+                            int64_t cells = 1;
+                            for (int a = 0; a < 3; a++) {
+                                const bool used = a < d_;
+                                it.lo[a] = used ? blo[a] - (lo_[3 * p + a] - g_) : 0;
+                                it.ext[a] = used ? bhi[a] - blo[a] + 1 : 1;
+                                // source ghost coordinate of global cell x: x - sh - (lo_q - g)
+                                it.off[a] = used ? (lo_[3 * p + a] - g_) - sh[a] - (lo_[3 * q + a] - g_) : 0;
+                                cells *= it.ext[a];
+                            }
+                            img.items.push_back(it);
+                            img.add(int(img.items.size()) - 1, 0, cells);
+                        }
+        }
+    }
+
+    int build_tables() {
+        TableImage<HaloItem> halo;
+        build_halo(halo);
+        TableImage<LevelPatch> copy, zero;
+        copy.items = patch_;
+        zero.items = patch_;
+        for (int p = 0; p < np_; p++) {
+            copy.add(p, 0, gvol(p));
+            zero.add(p, 0, int64_t(neq_) * ivol(p));
+        }
+        halo_items_ = halo.total;
+        copy_items_ = copy.total;
+        zero_items_ = zero.total;
+        const size_t bytes = halo.bytes() + copy.bytes() + zero.bytes() + rk_bytes();
+        if (tables_) (void)hipFree(tables_);
+        tables_ = nullptr;
+        if (hipMalloc(&tables_, bytes) != hipSuccess) return rec_->fail(HM_ERR_OUT_OF_MEMORY, "level tables");
+        std::vector<char> h(bytes, 0);
+        char* d = static_cast<char*>(tables_);
+        size_t o = 0;
+        halo_ = halo.write(h.data() + o, d + o);
+        o += halo.bytes();
+        copy_ = copy.write(h.data() + o, d + o);
+        o += copy.bytes();
+        zero_ = zero.write(h.data() + o, d + o);
+        o += zero.bytes();
+        rk_off_ = o;
+        if (hipMemcpy(tables_, h.data(), o, hipMemcpyHostToDevice) != hipSuccess)
+            return rec_->fail(HM_ERR_DEVICE, "level table upload");
+        return build_rk_tables();
+    }
+
+    size_t rk_bytes() const {
+        TableImage<RKPatch> t;
+        t.items.resize(np_);
+        for (int p = 0; p < np_; p++) t.add(p, 0, ivol(p));
+        return t.bytes() * kMaxRK;
+    }
+
+    // One stage-update table per stage (the intermediate pointers differ).
+    int build_rk_tables() {
+        const hm_config& c = rec_->config();
+        const int ns = c.num_species;
+        std::vector<char> h(rk_bytes(), 0);
+        char* d = static_cast<char*>(tables_) + rk_off_;
+        size_t o = 0;
+        for (int s = 0; s < kMaxRK; s++) {
+            TableImage<RKPatch> t;
+            t.items.resize(np_);
+            for (int p = 0; p < np_; p++) {
+                RKPatch& R = t.items[p];
+                std::memset(&R, 0, sizeof R);
+                for (int a = 0; a < 3; a++) {
+                    R.n[a] = a < d_ ? n_[3 * p + a] : 1;
+                    R.gq[a] = R.gi[a] = a < d_ ? g_ : 0;
+                    R.dx[a] = dx_[a];
+                }
+                for (int e = 0; e < nq_; e++) R.Q[e] = patch_[p].Q[0] + size_t(e) * gvol(p);
+                for (int m = 0; m <= s && m < kMaxRK; m++) {
+                    for (int e = 0; e < nq_; e++) R.Qi[m][e] = patch_[p].Q[1 + m] + size_t(e) * gvol(p);
+                    for (int a = 0; a < d_; a++) R.Fi[m][a] = fl_[(size_t(p) * kMaxRK + m) * 3 + a];
+                    R.Si[m] = patch_[p].S[m];
+                }
+                t.add(p, 0, ivol(p));
+            }
+            rk_[s] = t.write(h.data() + o, d + o);
+            rk_items_ = t.total;
+            o += t.bytes();
+            RKParams& r = rkp_[s];
+            r.nc = s + 1;
+            r.neq = neq_;
+            const bool five = c.model == HM_MODEL_FIVE_EQN_ALLAIRE;
+            r.nz = five ? ns - 1 : 0;
+            r.zfirst = five ? ns + d_ + 1 : 0;
+            r.last_beta = -1;
+            for (int m = 0; m < kMaxRK; m++) {
+                r.alpha[m] = alpha_[s][m];
+                r.beta[m] = beta_[s][m];
+                r.gamma[m] = gamma_[s][m];
+                if (m <= s && beta_[s][m] != 0.0) r.last_beta = m;
+            }
+        }
+        if (hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return rec_->fail(HM_ERR_DEVICE, "level table upload");
+        return HM_OK;
+    }
+
+    int launch_halo(int buf, hipStream_t s) {
+        if (halo_items_ == 0) return HM_OK;
+        switch (d_) {
+            case 1: hipLaunchKernelGGL(k_halo<1>, dim3(blocks_for(halo_items_)), dim3(kBlock), 0, s, halo_, patch_dev(), buf, nq_, g_); break;
+            case 2: hipLaunchKernelGGL(k_halo<2>, dim3(blocks_for(halo_items_)), dim3(kBlock), 0, s, halo_, patch_dev(), buf, nq_, g_); break;
+            default: hipLaunchKernelGGL(k_halo<3>, dim3(blocks_for(halo_items_)), dim3(kBlock), 0, s, halo_, patch_dev(), buf, nq_, g_); break;
+        }
+        return hipGetLastError() == hipSuccess ? HM_OK : rec_->fail(HM_ERR_DEVICE, "halo launch failed");
+    }
+
+    const LevelPatch* patch_dev() const { return copy_.patches; }
+
+    // One Runge-Kutta step (the sequence the graph captures).
+    int step(double dt, hipStream_t s) {
+        if (int st = launch_halo(0, s)) return st;
+        for (int sn = 0; sn < nst_; sn++) {
+            hipLaunchKernelGGL(k_copy_state, dim3(blocks_for(copy_items_)), dim3(kBlock), 0, s, copy_, 0, 1 + sn, nq_);
+            if (sn > 0)
+                if (int st = launch_halo(1 + sn, s)) return st;
+            hipLaunchKernelGGL(k_zero_source, dim3(blocks_for(zero_items_)), dim3(kBlock), 0, s, zero_, sn);
+            if (hipGetLastError() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "level launch failed");
+            if (int st = rec_->runOwnedPlan(rhs_plan_[sn], dt, s)) return st;
+            if (launch_rk_update<SegTableT<RKPatch>>(d_, rk_[sn], rk_items_, nullptr, 0, rkp_[sn], s) != hipSuccess)
+                return rec_->fail(HM_ERR_DEVICE, "stage update launch failed");
+        }
+        return HM_OK;
+    }
+
+    int capture(double dt) {
+        hipGraph_t graph = nullptr;
+        if (hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return rec_->fail(HM_ERR_DEVICE, "hipStreamBeginCapture failed");
+        const int st = step(dt, cap_stream_);
+        const hipError_t e = hipStreamEndCapture(cap_stream_, &graph);
+        if (st != HM_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st;
+        }
+        if (e != hipSuccess || !graph) return rec_->fail(HM_ERR_DEVICE, "hipStreamEndCapture failed");
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "hipGraphInstantiate failed");
+        if (graphs_.size() >= 4) drop_graphs();
+        graphs_[dt] = exec;
+        return HM_OK;
+    }
+
+    void drop_graphs() {
+        if (graphs_.empty()) return;
+        (void)hipDeviceSynchronize();
+        for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+        graphs_.clear();
+    }
+
+    ConvectiveFluxReconstructorGPU* rec_;
+    int d_ = 0, neq_ = 0, nq_ = 0, g_ = kGc, np_ = 0, nst_ = 3;
+    int dlo_[3] = {}, dn_[3] = {};
+    bool periodic_[3] = {};
+    double dx_[3] = {};
+    double alpha_[kMaxRK][kMaxRK] = {}, beta_[kMaxRK][kMaxRK] = {}, gamma_[kMaxRK][kMaxRK] = {};
+    std::vector<int> lo_, n_;
+    void* storage_ = nullptr;
+    void* tables_ = nullptr;
+    std::vector<LevelPatch> patch_;
+    std::vector<double*> fl_;
+    std::vector<std::vector<hm_patch_view>> views_;
+    void* rhs_scratch_ = nullptr;
+    ConvectiveFluxReconstructorGPU::OwnedPlan rhs_plan_[kMaxRK];
+    SegTableT<HaloItem> halo_{};
+    SegTableT<LevelPatch> copy_{}, zero_{};
+    SegTableT<RKPatch> rk_[kMaxRK] = {};
+    RKParams rkp_[kMaxRK] = {};
+    int64_t halo_items_ = 0, copy_items_ = 0, zero_items_ = 0, rk_items_ = 0;
+    size_t rk_off_ = 0;
+    hipStream_t cap_stream_ = nullptr;
+    hipEvent_t ev_a_ = nullptr, ev_b_ = nullptr;
+    std::map<double, hipGraphExec_t> graphs_;
+};
+
+// Stage update for caller-owned patch data (Euler::advanceSingleStepOnPatch).
+int advance_single_step(ConvectiveFluxReconstructorGPU* rec, const hm_rk_patch_view* views, int np, int nc,
+                        const double* alpha, const double* beta, const double* gamma, hipStream_t s) {
+    if (np < 0 || (np > 0 && !views) || nc < 1 || nc > kMaxRK || !alpha || !beta)
+        return rec->fail(HM_ERR_INVALID_ARGUMENT, "bad stage-update arguments");
+    if (np == 0) return HM_OK;
+    if (int st = rec->selectDevice()) return st;
+    const hm_config& c = rec->config();
+    const int d = c.dim, neq = rec->numEqn(), nq = rec->numConservativeArrays();
+    RKParams r{};
+    r.nc = nc;
+    r.neq = neq;
+    const bool five = c.model == HM_MODEL_FIVE_EQN_ALLAIRE;
+    r.nz = five ? c.num_species - 1 : 0;
+    r.zfirst = five ? c.num_species + d + 1 : 0;
+    r.last_beta = -1;
+    for (int m = 0; m < kMaxRK; m++) {
+        r.alpha[m] = m < nc ? alpha[m] : 0.0;
+        r.beta[m] = m < nc ? beta[m] : 0.0;
+        r.gamma[m] = (m < nc && gamma) ? gamma[m] : 0.0;
+        if (m < nc && beta[m] != 0.0) r.last_beta = m;
+    }
+    TableImage<RKPatch> cells, acc;
+    cells.items.resize(np);
+    acc.items.resize(np);
+    bool any_acc = false;
+    char msg[160];
+    for (int p = 0; p < np; p++) {
+        const hm_rk_patch_view& v = views[p];
+        RKPatch& R = cells.items[p];
+        std::memset(&R, 0, sizeof R);
+        int64_t ivol = 1;
+        for (int a = 0; a < 3; a++) {
+            const bool used = a < d;
+            R.n[a] = used ? v.n[a] : 1;
+            R.gq[a] = used ? v.ghost[a] : 0;
+            R.gi[a] = used ? v.ghost_intermediate[a] : 0;
+            R.dx[a] = used ? v.dx[a] : 1.0;
+            if (used && (v.n[a] < 1 || !(v.dx[a] > 0.0) || v.ghost[a] < 0 || v.ghost_intermediate[a] < 0)) {
+                std::snprintf(msg, sizeof msg, "patch %d: bad extent, ghost width or dx on axis %d", p, a);
+                return rec->fail(HM_ERR_INVALID_ARGUMENT, msg);
+            }
+            ivol *= R.n[a];
+        }
+        for (int e = 0; e < nq; e++) {
+            if (!v.Q[e]) {
+                std::snprintf(msg, sizeof msg, "patch %d: conservative array %d is NULL", p, e);
+                return rec->fail(HM_ERR_INVALID_ARGUMENT, msg);
+            }
+            R.Q[e] = v.Q[e];
+        }
+        for (int m = 0; m < nc; m++) {
+            const bool needs_flux = beta[m] != 0.0 || (gamma && gamma[m] != 0.0);
+            for (int e = 0; e < neq; e++) {
+                if (alpha[m] != 0.0 && !v.Q_intermediate[m][e]) {
+                    std::snprintf(msg, sizeof msg, "patch %d: intermediate %d array %d is NULL", p, m, e);
+                    return rec->fail(HM_ERR_INVALID_ARGUMENT, msg);
+                }
+                R.Qi[m][e] = v.Q_intermediate[m][e];
+            }
+            for (int a = 0; a < d; a++) {
+                if (needs_flux && !v.F_intermediate[m][a]) {
+                    std::snprintf(msg, sizeof msg, "patch %d: intermediate %d flux on axis %d is NULL", p, m, a);
+                    return rec->fail(HM_ERR_INVALID_ARGUMENT, msg);
+                }
+                R.Fi[m][a] = v.F_intermediate[m][a];
+            }
+            if (needs_flux && !v.S_intermediate[m]) {
+                std::snprintf(msg, sizeof msg, "patch %d: intermediate %d source is NULL", p, m);
+                return rec->fail(HM_ERR_INVALID_ARGUMENT, msg);
+            }
+            R.Si[m] = v.S_intermediate[m];
+        }
+        for (int a = 0; a < 3; a++) R.Facc[a] = a < d ? v.F_accumulated[a] : nullptr;
+        R.Sacc = v.S_accumulated;
+        cells.add(p, 0, ivol);
+        acc.items[p] = R;
+        if (gamma) {
+            for (int a = 0; a < d; a++)
+                if (R.Facc[a]) {
+                    acc.add(p, a, int64_t(R.n[0] + (a == 0)) * (R.n[1] + (a == 1)) * (R.n[2] + (a == 2)));
+                    any_acc = true;
+                }
+            if (R.Sacc) {
+                acc.add(p, d, ivol);
+                any_acc = true;
+            }
+        }
+    }
+    // One patch: descriptors by value, no table upload.  A single-patch
+    // segment's axis is its position, so this needs every accumulated axis.
+    if (np == 1 && (!any_acc || int(acc.cnt.size()) == d + 1)) {
+        SegSingleT<RKPatch> wc{}, wa{};
+        wc.p = cells.items[0];
+        wc.nseg = int(cells.cnt.size());
+        for (int i = 0; i < wc.nseg; i++) {
+            wc.start[i] = cells.start[i];
+            wc.cnt[i] = cells.cnt[i];
+        }
+        wc.start[wc.nseg] = cells.total;
+        wa.p = acc.items[0];
+        wa.nseg = int(acc.cnt.size());
+        for (int i = 0; i < wa.nseg; i++) {
+            wa.start[i] = acc.start[i];
+            wa.cnt[i] = acc.cnt[i];
+        }
+        wa.start[wa.nseg] = acc.total;
+        if (launch_rk_update<SegSingleT<RKPatch>>(d, wc, cells.total, any_acc ? &wa : nullptr, acc.total, r, s) !=
+            hipSuccess)
+            return rec->fail(HM_ERR_DEVICE, "stage update launch failed");
+        return HM_OK;
+    }
+    const size_t bytes = cells.bytes() + (any_acc ? acc.bytes() : 0);
+    void* dev = rec->stageTables(bytes);
+    if (!dev) return rec->fail(HM_ERR_OUT_OF_MEMORY, "stage-update tables");
+    std::vector<char> h(bytes, 0);
+    SegTableT<RKPatch> tc = cells.write(h.data(), static_cast<char*>(dev));
+    SegTableT<RKPatch> ta{};
+    if (any_acc) ta = acc.write(h.data() + cells.bytes(), static_cast<char*>(dev) + cells.bytes());
+    // Synchronous upload: the host image dies with this call.
+    if (hipMemcpy(dev, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+        return rec->fail(HM_ERR_DEVICE, "stage-update table upload");
+    if (launch_rk_update<SegTableT<RKPatch>>(d, tc, cells.total, any_acc ? &ta : nullptr, acc.total, r, s) !=
+        hipSuccess)
+        return rec->fail(HM_ERR_DEVICE, "stage update launch failed");
+    return rec->stageTablesInUse(s);
+}
+
+}  // namespace hm
+
+struct hm_level {
+    std::unique_ptr<hm::Level> lvl;
+    hm_ctx* ctx;
+};
+
+extern "C" {
+
+int hm_level_create(hm_ctx* ctx, int num_patches, const int* patch_lo, const int* patch_n, const int* domain_lo,
+                    const int* domain_n, const int* periodic, const double* dx, hm_level** out) {
+    if (!ctx || !ctx->rec || !out || !patch_lo || !patch_n || !domain_lo || !domain_n || !periodic || !dx)
+        return HM_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (int st = ctx->rec->selectDevice()) return st;
+    hm_level* l = new (std::nothrow) hm_level();
+    if (!l) return HM_ERR_OUT_OF_MEMORY;
+    l->ctx = ctx;
+    ctx->levels++;
+    l->lvl.reset(new (std::nothrow) hm::Level(ctx->rec.get()));
+    if (!l->lvl) {
+        ctx->levels--;
+        delete l;
+        return HM_ERR_OUT_OF_MEMORY;
+    }
+    const int st = l->lvl->init(num_patches, patch_lo, patch_n, domain_lo, domain_n, periodic, dx);
+    if (st != HM_OK) {
+        l->lvl.reset();
+        ctx->levels--;
+        delete l;
+        return st;
+    }
+    *out = l;
+    return HM_OK;
+}
+
+int hm_level_destroy(hm_level* level) {
+    if (!level) return HM_OK;
+    level->lvl.reset();
+    level->ctx->levels--;
+    delete level;
+    return HM_OK;
+}
+
+int hm_level_set_rk_coefficients(hm_level* level, int num_stages, const double* alpha, const double* beta,
+                                 const double* gamma) {
+    if (!level || !alpha || !beta) return HM_ERR_INVALID_ARGUMENT;
+    return level->lvl->setCoefficients(num_stages, alpha, beta, gamma);
+}
+
+int hm_level_write_patch(hm_level* level, int patch, const double* Q) {
+    if (!level) return HM_ERR_INVALID_ARGUMENT;
+    return level->lvl->writePatch(patch, Q);
+}
+
+int hm_level_read_patch(hm_level* level, int patch, double* Q) {
+    if (!level) return HM_ERR_INVALID_ARGUMENT;
+    return level->lvl->readPatch(patch, Q);
+}
+
+int hm_level_patch_array(hm_level* level, int patch, double** Q, int* ghost) {
+    if (!level) return HM_ERR_INVALID_ARGUMENT;
+    return level->lvl->patchArray(patch, Q, ghost);
+}
+
+int hm_level_fill_ghosts(hm_level* level, void* stream) {
+    if (!level) return HM_ERR_INVALID_ARGUMENT;
+    return level->lvl->fillGhosts(static_cast<hipStream_t>(stream));
+}
+
+int hm_level_advance(hm_level* level, double dt, int num_steps, int use_graph, void* stream) {
+    if (!level) return HM_ERR_INVALID_ARGUMENT;
+    return level->lvl->advance(dt, num_steps, use_graph != 0, static_cast<hipStream_t>(stream));
+}
+
+int hm_advance_single_step_on_patches(hm_ctx* ctx, const hm_rk_patch_view* patches, int num_patches,
+                                      int num_coeffs, const double* alpha, const double* beta, const double* gamma,
+                                      void* stream) {
+    if (!ctx || !ctx->rec) return HM_ERR_INVALID_ARGUMENT;
+    return hm::advance_single_step(ctx->rec.get(), patches, num_patches, num_coeffs, alpha, beta, gamma,
+                                   static_cast<hipStream_t>(stream));
+}
+
+int hm_advance_single_step_on_patch(hm_ctx* ctx, const hm_rk_patch_view* patch, int num_coeffs,
+                                    const double* alpha, const double* beta, const double* gamma, void* stream) {
+    return hm_advance_single_step_on_patches(ctx, patch, 1, num_coeffs, alpha, beta, gamma, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,146 @@
+// hm_rk.hip -- Runge-Kutta stage update of Euler::advanceSingleStepOnPatch
+// (src/apps/Euler/Euler.cpp:1003-1680) for one or many patches.
+//
+// Per interior cell and equation, with the scratch value zero-filled first
+// (FlowModel*::fillCellDataOfConservativeVariablesWithZero):
+//   for n < num_coeffs:
+//     if alpha[n] != 0:  Q += alpha[n] * Q_int[n]
+//     if beta[n]  != 0:  Q += beta[n] * (-(F_x,R - F_x,L)/dx_0 - (F_y,T - F_y,B)/dx_1
+//                                        - (F_z,F - F_z,B)/dx_2 + S_int[n])
+//     if beta[n] != 0 (five-equation model): last volume fraction = 1 - sum of the others
+//                        (FlowModelFiveEqnAllaire.cpp:1739-1890)
+// and, where accumulation arrays are given, F_acc += gamma[n] F_int[n],
+// S_acc += gamma[n] S_int[n] (Euler.cpp:1230-1270, 1360-1420, 1560-1660).
+#include "hm_rk.hpp"
+
+namespace hm {
+
+template <int D, class W>
+__global__ __launch_bounds__(kBlock) void k_rk_update(W w, RKParams r) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const RKPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    const int n0 = P->n[0], n1 = (D > 1) ? P->n[1] : 1;
+    const int i = int(loc % n0);
+    const int64_t rr = loc / n0;
+    const int j = (D > 1) ? int(rr % n1) : 0;
+    const int k = (D > 2) ? int(rr / n1) : 0;
+    // Scratch (output) and intermediate (input) ghost boxes.
+    auto box_idx = [&](const int* g) -> int64_t {
+        const int e0 = n0 + 2 * g[0];
+        const int e1 = (D > 1) ? n1 + 2 * g[1] : 1;
+        return (i + g[0]) + int64_t(j + ((D > 1) ? g[1] : 0)) * e0 + int64_t(k + ((D > 2) ? g[2] : 0)) * e0 * e1;
+    };
+    const int64_t iq = box_idx(P->gq);
+    const int64_t ii = box_idx(P->gi);
+    // Face indices of the two faces of this cell on each axis.
+    int64_t fL[3], fR[3];
+    {
+        // x faces: i + j (n0+1) + k (n0+1) n1
+        fL[0] = i + int64_t(j) * (n0 + 1) + int64_t(k) * (n0 + 1) * n1;
+        fR[0] = fL[0] + 1;
+        if (D > 1) {
+            fL[1] = i + int64_t(j) * n0 + int64_t(k) * n0 * (n1 + 1);
+            fR[1] = fL[1] + n0;
+        }
+        if (D > 2) {
+            fL[2] = i + int64_t(j) * n0 + int64_t(k) * n0 * n1;
+            fR[2] = fL[2] + int64_t(n0) * n1;
+        }
+    }
+    int64_t fvol[3];
+    fvol[0] = int64_t(n0 + 1) * n1 * ((D > 2) ? P->n[2] : 1);
+    if (D > 1) fvol[1] = int64_t(n0) * (n1 + 1) * ((D > 2) ? P->n[2] : 1);
+    if (D > 2) fvol[2] = int64_t(n0) * n1 * (P->n[2] + 1);
+    const int64_t svol = int64_t(n0) * n1 * ((D > 2) ? P->n[2] : 1);
+
+    const int nz = r.nz;
+    double zl = double(1);  // last volume fraction: 1 - sum of the others, in species order
+    for (int e = 0; e < r.neq; e++) {
+        double q = double(0);
+        double snap = double(0);
+        for (int n = 0; n < r.nc; n++) {
+            if (r.alpha[n] != 0.0) q += r.alpha[n] * P->Qi[n][e][ii];
+            if (r.beta[n] != 0.0) {
+                const double* Fx = P->Fi[n][0] + e * fvol[0];
+                double t = -(Fx[fR[0]] - Fx[fL[0]]) / P->dx[0];
+                if constexpr (D > 1) {
+                    const double* Fy = P->Fi[n][1] + e * fvol[1];
+                    t = t - (Fy[fR[1]] - Fy[fL[1]]) / P->dx[1];
+                }
+                if constexpr (D > 2) {
+                    const double* Fz = P->Fi[n][2] + e * fvol[2];
+                    t = t - (Fz[fR[2]] - Fz[fL[2]]) / P->dx[2];
+                }
+                q += r.beta[n] * (t + P->Si[n][e * svol + loc]);
+                snap = q;
+            }
+        }
+        P->Q[e][iq] = q;
+        if (nz > 0 && e >= r.zfirst && e < r.zfirst + nz) zl -= snap;
+    }
+    // The zero-filled value stays if no coefficient beta is nonzero.
+    if (nz > 0) P->Q[r.neq][iq] = (r.last_beta >= 0) ? zl : double(0);
+}
+
+// gamma-weighted accumulation of the stage fluxes and sources.
+template <int D, class W>
+__global__ __launch_bounds__(kBlock) void k_rk_accumulate(W w, RKParams r) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const RKPatch* P;
+    int ax;  // 0..D-1: faces of that axis, D: cells (source)
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    const int n2 = (D > 2) ? P->n[2] : 1, n1 = (D > 1) ? P->n[1] : 1;
+    int64_t vol;
+    if (ax == D)
+        vol = int64_t(P->n[0]) * n1 * n2;
+    else
+        vol = int64_t(P->n[0] + (ax == 0)) * (n1 + (ax == 1)) * (n2 + (ax == 2));
+    for (int e = 0; e < r.neq; e++) {
+        double* acc = (ax == D) ? P->Sacc : P->Facc[ax];
+        if (!acc) return;
+        double v = acc[e * vol + loc];
+        for (int n = 0; n < r.nc; n++) {
+            if (r.gamma[n] == 0.0) continue;
+            const double* src = (ax == D) ? P->Si[n] : P->Fi[n][ax];
+            v += r.gamma[n] * src[e * vol + loc];
+        }
+        acc[e * vol + loc] = v;
+    }
+}
+
+template <class W>
+hipError_t launch_rk_update(int dim, const W& cells, int64_t cell_items, const W* acc, int64_t acc_items,
+                            const RKParams& r, hipStream_t s) {
+    const unsigned nb = unsigned((cell_items + kBlock - 1) / kBlock);
+    if (cell_items > 0) {
+        switch (dim) {
+            case 1: hipLaunchKernelGGL((k_rk_update<1, W>), dim3(nb), dim3(kBlock), 0, s, cells, r); break;
+            case 2: hipLaunchKernelGGL((k_rk_update<2, W>), dim3(nb), dim3(kBlock), 0, s, cells, r); break;
+            case 3: hipLaunchKernelGGL((k_rk_update<3, W>), dim3(nb), dim3(kBlock), 0, s, cells, r); break;
+            default: return hipErrorInvalidValue;
+        }
+    }
+    if (acc && acc_items > 0) {
+        const unsigned na = unsigned((acc_items + kBlock - 1) / kBlock);
+        switch (dim) {
+            case 1: hipLaunchKernelGGL((k_rk_accumulate<1, W>), dim3(na), dim3(kBlock), 0, s, *acc, r); break;
+            case 2: hipLaunchKernelGGL((k_rk_accumulate<2, W>), dim3(na), dim3(kBlock), 0, s, *acc, r); break;
+            case 3: hipLaunchKernelGGL((k_rk_accumulate<3, W>), dim3(na), dim3(kBlock), 0, s, *acc, r); break;
+            default: return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
+}
+
+template hipError_t launch_rk_update<SegSingleT<RKPatch>>(int, const SegSingleT<RKPatch>&, int64_t,
+                                                          const SegSingleT<RKPatch>*, int64_t, const RKParams&,
+                                                          hipStream_t);
+template hipError_t launch_rk_update<SegTableT<RKPatch>>(int, const SegTableT<RKPatch>&, int64_t,
+                                                         const SegTableT<RKPatch>*, int64_t, const RKParams&,
+                                                         hipStream_t);
+
+}  // namespace hm

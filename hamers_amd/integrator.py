@@ -1,0 +1,208 @@
+"""Python mirror of the Runge-Kutta stage update and of one level's stage loop.
+
+``advanceSingleStepOnPatch`` mirrors ``Euler::advanceSingleStepOnPatch``
+(src/apps/Euler/Euler.cpp:1003-1680, the zero fill of
+``RungeKuttaLevelIntegrator::advanceLevel`` included) and ``Level`` runs the
+stage loop of ``RungeKuttaLevelIntegrator::advanceLevel``
+(src/algs/integrator/RungeKuttaLevelIntegrator.cpp:1457-1939) for one level of
+patches held on the GPU.  Every computation runs in libhamers_amd.so.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+from .reconstructor import ConvectiveFluxReconstructor, _check_tensor
+
+# SSP-RK3 (RungeKuttaLevelIntegrator.cpp:3907-3924)
+SSP_RK3_ALPHA = ((1.0,), (0.75, 0.25), (1.0 / 3.0, 0.0, 2.0 / 3.0))
+SSP_RK3_BETA = ((1.0,), (0.0, 0.25), (0.0, 0.0, 2.0 / 3.0))
+SSP_RK3_GAMMA = ((1.0 / 6.0,), (0.0, 1.0 / 6.0), (0.0, 0.0, 2.0 / 3.0))
+
+
+def _arrays(Q, count):
+    """A list of tensors, or one (count, ...) tensor -> list of per-component tensors."""
+    if Q is None:
+        return [None] * count
+    if isinstance(Q, (list, tuple)):
+        return list(Q)
+    return [Q[c] for c in range(Q.shape[0])]
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    _check_tensor(t)
+    return t.data_ptr()
+
+
+@dataclass
+class RKPatchData:
+    """One patch of a stage update: scratch arrays ``Q`` (updated), and per
+    coefficient n the intermediate ``Q_intermediate[n]``, ``F_intermediate[n]``
+    (per axis) and ``S_intermediate[n]``; optional ``F_accumulated`` /
+    ``S_accumulated`` receive the gamma-weighted sums."""
+    n: tuple
+    ghost: tuple
+    ghost_intermediate: tuple
+    dx: tuple
+    Q: object
+    Q_intermediate: list
+    F_intermediate: list
+    S_intermediate: list
+    F_accumulated: list = None
+    S_accumulated: object = None
+    _keep: list = field(default_factory=list, repr=False)
+
+    def view(self, num_arrays: int, num_eqn: int) -> capi.hm_rk_patch_view:
+        v = capi.hm_rk_patch_view()
+        d = len(self.n)
+        for a in range(3):
+            v.n[a] = int(self.n[a]) if a < d else 1
+            v.ghost[a] = int(self.ghost[a]) if a < d else 0
+            v.ghost_intermediate[a] = int(self.ghost_intermediate[a]) if a < d else 0
+            v.dx[a] = float(self.dx[a]) if a < d else 1.0
+        Qs = _arrays(self.Q, num_arrays)
+        if len(Qs) != num_arrays:
+            raise ValueError(f"need {num_arrays} conservative arrays, got {len(Qs)}")
+        self._keep = list(Qs)
+        for e, t in enumerate(Qs):
+            v.Q[e] = _ptr(t)
+        for m, Qm in enumerate(self.Q_intermediate):
+            for e, t in enumerate(_arrays(Qm, num_eqn)[:num_eqn]):
+                v.Q_intermediate[m][e] = _ptr(t)
+                self._keep.append(t)
+        for m, Fm in enumerate(self.F_intermediate):
+            for a, t in enumerate(Fm or []):
+                v.F_intermediate[m][a] = _ptr(t)
+        for m, t in enumerate(self.S_intermediate):
+            v.S_intermediate[m] = _ptr(t)
+        for a, t in enumerate(self.F_accumulated or []):
+            v.F_accumulated[a] = _ptr(t)
+        v.S_accumulated = _ptr(self.S_accumulated)
+        return v
+
+
+def _coeffs(values):
+    arr = (ctypes.c_double * capi.HM_MAX_RK_COEFFS)()
+    for i, x in enumerate(values):
+        arr[i] = float(x)
+    return arr
+
+
+def advanceSingleStepOnPatch(rec: ConvectiveFluxReconstructor, patch, alpha, beta, gamma=None, stream=None):
+    """Stage update of one patch (``RKPatchData``) or of a list of patches (one launch)."""
+    if not (1 <= len(alpha) <= capi.HM_MAX_RK_COEFFS) or len(beta) != len(alpha):
+        raise ValueError("alpha and beta need the same length, 1..5")
+    if gamma is not None and len(gamma) != len(alpha):
+        raise ValueError("gamma needs the length of alpha")
+    a, b = _coeffs(alpha), _coeffs(beta)
+    c = _coeffs(gamma) if gamma is not None else None
+    s = rec._stream_ptr(stream)
+    if isinstance(patch, RKPatchData):
+        v = patch.view(rec.num_arrays, rec.num_eqn)
+        st = capi.lib.hm_advance_single_step_on_patch(rec._ctx, ctypes.byref(v), len(alpha), a, b, c, s)
+    else:
+        patches = list(patch)
+        views = (capi.hm_rk_patch_view * len(patches))()
+        for i, p in enumerate(patches):
+            views[i] = p.view(rec.num_arrays, rec.num_eqn)
+        st = capi.lib.hm_advance_single_step_on_patches(rec._ctx, views, len(patches), len(alpha), a, b, c, s)
+    capi.check(rec._ctx, st)
+
+
+class Level:
+    """Patches of one level on the GPU, advanced by the Runge-Kutta stage loop.
+
+    ``boxes`` is a list of (lo, n) pairs in level index space, (x, y, z)
+    ordered; ``domain`` is (lo, n) of the level domain; ``periodic`` per axis.
+    """
+
+    def __init__(self, rec: ConvectiveFluxReconstructor, boxes, domain, periodic, dx):
+        self.rec = rec  # the context must outlive the level
+        d = rec.dim
+        self.dim = d
+        self.boxes = [(tuple(lo), tuple(n)) for lo, n in boxes]
+        np_ = len(self.boxes)
+        lo = (ctypes.c_int * (3 * np_))()
+        nn = (ctypes.c_int * (3 * np_))()
+        for i, (bl, bn) in enumerate(self.boxes):
+            for a in range(d):
+                lo[3 * i + a] = int(bl[a])
+                nn[3 * i + a] = int(bn[a])
+        dlo = (ctypes.c_int * 3)(*([int(x) for x in domain[0]] + [0] * (3 - d)))
+        dn = (ctypes.c_int * 3)(*([int(x) for x in domain[1]] + [1] * (3 - d)))
+        per = (ctypes.c_int * 3)(*([int(bool(x)) for x in periodic] + [0] * (3 - d)))
+        ddx = (ctypes.c_double * 3)(*([float(x) for x in dx] + [1.0] * (3 - d)))
+        self._lvl = ctypes.c_void_p()
+        capi.check(rec._ctx, capi.lib.hm_level_create(rec._ctx, np_, lo, nn, dlo, dn, per, ddx,
+                                                      ctypes.byref(self._lvl)))
+
+    def __del__(self):
+        if getattr(self, "_lvl", None):
+            capi.lib.hm_level_destroy(self._lvl)
+            self._lvl = None
+
+    def __len__(self):
+        return len(self.boxes)
+
+    def set_rk_coefficients(self, alpha, beta, gamma=None):
+        """Rows of per-stage coefficients, e.g. SSP_RK3_ALPHA."""
+        ns = len(alpha)
+
+        def table(rows):
+            t = (ctypes.c_double * (ns * ns))()
+            for s_, row in enumerate(rows):
+                for m, x in enumerate(row):
+                    t[s_ * ns + m] = float(x)
+            return t
+
+        c = table(gamma) if gamma is not None else None
+        capi.check(self.rec._ctx, capi.lib.hm_level_set_rk_coefficients(self._lvl, ns, table(alpha), table(beta), c))
+
+    def write_patch(self, i: int, Q: np.ndarray):
+        """Interior values (num_arrays, *reversed(n)) of patch i (float64, host)."""
+        bn = self.boxes[i][1]
+        shape = (self.rec.num_arrays,) + tuple(reversed(bn))
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        if Q.shape != shape:
+            raise ValueError(f"patch {i}: expected shape {shape}, got {Q.shape}")
+        capi.check(self.rec._ctx, capi.lib.hm_level_write_patch(
+            self._lvl, i, Q.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def read_patch(self, i: int) -> np.ndarray:
+        bn = self.boxes[i][1]
+        Q = np.empty((self.rec.num_arrays,) + tuple(reversed(bn)))
+        capi.check(self.rec._ctx, capi.lib.hm_level_read_patch(
+            self._lvl, i, Q.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return Q
+
+    def write_global(self, Q: np.ndarray):
+        """Scatter a whole-domain interior array over the patches."""
+        for i, (lo, n) in enumerate(self.boxes):
+            sl = tuple(slice(lo[a], lo[a] + n[a]) for a in reversed(range(self.dim)))
+            self.write_patch(i, Q[(slice(None),) + sl])
+
+    def read_global(self, shape) -> np.ndarray:
+        Q = np.full(shape, np.nan)
+        for i, (lo, n) in enumerate(self.boxes):
+            sl = tuple(slice(lo[a], lo[a] + n[a]) for a in reversed(range(self.dim)))
+            Q[(slice(None),) + sl] = self.read_patch(i)
+        return Q
+
+    def patch_array(self, i: int):
+        """(device address, ghost width) of patch i's scratch arrays."""
+        ptr = ctypes.c_void_p()
+        g = ctypes.c_int()
+        capi.check(self.rec._ctx, capi.lib.hm_level_patch_array(self._lvl, i, ctypes.byref(ptr), ctypes.byref(g)))
+        return ptr.value, g.value
+
+    def fill_ghosts(self, stream=None):
+        capi.check(self.rec._ctx, capi.lib.hm_level_fill_ghosts(self._lvl, self.rec._stream_ptr(stream)))
+
+    def advance(self, dt: float, num_steps: int = 1, use_graph: bool = False, stream=None):
+        capi.check(self.rec._ctx, capi.lib.hm_level_advance(self._lvl, float(dt), int(num_steps), int(use_graph),
+                                                            self.rec._stream_ptr(stream)))

@@ -149,6 +149,90 @@ int hm_profile_read(hm_ctx* ctx, double* stage_ms /* [HM_NUM_STAGES] */,
 /* Block the host until all work this context queued on `stream` is done. */
 int hm_synchronize(hm_ctx* ctx, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Runge-Kutta stage update                                                  */
+/* ------------------------------------------------------------------------ */
+
+#define HM_MAX_RK_COEFFS 5
+
+/*
+ * One patch of a stage update, as Euler::advanceSingleStepOnPatch sees it:
+ * the scratch-context conservative arrays (interior overwritten) and, for
+ * each coefficient n, the intermediate-context arrays.  Flux and source
+ * layouts are those of hm_patch_view; F_accumulated / S_accumulated are the
+ * scratch-context flux and source receiving gamma-weighted sums (NULL: skip).
+ */
+typedef struct hm_rk_patch_view {
+    int n[3];
+    int ghost[3];                 /* ghost width of Q */
+    int ghost_intermediate[3];    /* ghost width of Q_intermediate */
+    double dx[3];
+    double* Q[HM_MAX_EQN];        /* num_eqn (+1 for FIVE_EQN_ALLAIRE: the last volume fraction,
+                                     written as 1 - sum of the others) */
+    const double* Q_intermediate[HM_MAX_RK_COEFFS][HM_MAX_EQN];  /* num_eqn arrays each */
+    const double* F_intermediate[HM_MAX_RK_COEFFS][3];
+    const double* S_intermediate[HM_MAX_RK_COEFFS];
+    double* F_accumulated[3];
+    double* S_accumulated;
+} hm_rk_patch_view;
+
+/*
+ * Euler::advanceSingleStepOnPatch (Euler.cpp:1003-1680) with the zero fill of
+ * RungeKuttaLevelIntegrator::advanceLevel before it: per interior cell
+ *   Q = sum_n alpha[n] Q_int[n] + beta[n] (-sum_a (F_a,R - F_a,L)/dx_a + S_int[n])
+ * (terms in the reference's order), the five-equation model's last volume
+ * fraction = 1 - sum of the others after every nonzero beta
+ * (FlowModelFiveEqnAllaire.cpp:1739-1890), and F_acc += gamma[n] F_int[n],
+ * S_acc += gamma[n] S_int[n].  num_coeffs <= HM_MAX_RK_COEFFS; gamma may be NULL.
+ */
+int hm_advance_single_step_on_patch(hm_ctx* ctx, const hm_rk_patch_view* patch, int num_coeffs,
+                                    const double* alpha, const double* beta, const double* gamma, void* stream);
+int hm_advance_single_step_on_patches(hm_ctx* ctx, const hm_rk_patch_view* patches, int num_patches,
+                                      int num_coeffs, const double* alpha, const double* beta,
+                                      const double* gamma, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Level integrator                                                          */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * One AMR level held on the device: the scratch conservative arrays of every
+ * patch (ghost width 4), the intermediate copies, fluxes and sources of each
+ * Runge-Kutta stage.  hm_level_advance runs the stage loop of
+ * RungeKuttaLevelIntegrator::advanceLevel (RungeKuttaLevelIntegrator.cpp:1457-1939,
+ * stages 1672-1745): same-level ghost fill (periodic images included), batched
+ * convective RHS (hm_compute_convective_flux_and_source_on_patches) and the
+ * stage update, with SSP-RK3 weights (RungeKuttaLevelIntegrator.cpp:3907-3924)
+ * unless set otherwise.  Ghost cells no same-level patch covers keep what the
+ * caller wrote there.  Patch boxes are (lo, n) in level index space; all
+ * (x, y, z)-ordered arrays of 3 per patch.
+ */
+typedef struct hm_level hm_level;
+int hm_level_create(hm_ctx* ctx, int num_patches, const int* patch_lo, const int* patch_n,
+                    const int* domain_lo, const int* domain_n, const int* periodic, const double* dx,
+                    hm_level** out);
+int hm_level_destroy(hm_level* level);
+
+/* Butcher-like tables of the level's scheme, row-major [num_stages][num_stages]
+ * (entries above the diagonal ignored); gamma may be NULL. */
+int hm_level_set_rk_coefficients(hm_level* level, int num_stages, const double* alpha, const double* beta,
+                                 const double* gamma);
+
+/* Host copies of one patch's interior (num_conservative_arrays components). */
+int hm_level_write_patch(hm_level* level, int patch, const double* Q);
+int hm_level_read_patch(hm_level* level, int patch, double* Q);
+
+/* Device address of a patch's scratch arrays (components over the ghost box,
+ * component stride = ghost-box volume) and the ghost width. */
+int hm_level_patch_array(hm_level* level, int patch, double** Q, int* ghost);
+
+/* Same-level ghost fill of the scratch arrays. */
+int hm_level_fill_ghosts(hm_level* level, void* stream);
+
+/* num_steps steps of size dt.  use_graph != 0 replays one step captured in a
+ * hipGraph (captured once per dt) instead of issuing its launches. */
+int hm_level_advance(hm_level* level, double dt, int num_steps, int use_graph, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

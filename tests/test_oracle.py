@@ -304,3 +304,38 @@ def test_convergence_rate_wcns5js_2d():
 def test_convergence_catches_dropped_stencil_point():
     rates = _l2_rates(O.Params(dim=2, scheme=O.WCNS6_LD_HLLC_HLL, mutation=1), levels=3)
     assert rates[-1] < 5.8
+
+
+# ---------------------------------------------------------------------------
+# Time integration: the SSP-RK3 weights (RungeKuttaLevelIntegrator.cpp:3907-3924)
+# give third order in time; a wrong weight drops the order.
+# ---------------------------------------------------------------------------
+
+def _temporal_rates(alpha, beta):
+    from util import RK_ALPHA, RK_BETA
+
+    prm = O.Params(dim=1, scheme=O.WCNS6_LD_HLLC_HLL)
+    Q0, n, dx = W.density_wave_convergence(32, dim=1)
+    T = 0.25
+    ref = oracle_rk3(prm, Q0, n, dx, T / 160, 160, RK_ALPHA, RK_BETA)
+    errs = []
+    for steps in (10, 20, 40):
+        Q = oracle_rk3(prm, Q0, n, dx, T / steps, steps, alpha, beta)
+        errs.append(float(np.max(np.abs(Q - ref))))
+    return [np.log2(errs[i] / errs[i + 1]) for i in range(2)]
+
+
+def test_ssp_rk3_is_third_order_in_time():
+    from util import RK_ALPHA, RK_BETA
+
+    rates = _temporal_rates(RK_ALPHA, RK_BETA)
+    assert min(rates) > 2.8, rates
+
+
+def test_ssp_rk3_planted_weight_mistake_is_caught():
+    from util import RK_ALPHA, RK_BETA
+
+    beta = [list(r) for r in RK_BETA]
+    beta[2][2] = 0.6  # instead of 2/3
+    rates = _temporal_rates(RK_ALPHA, beta)
+    assert min(rates) < 1.5, rates

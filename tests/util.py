@@ -40,22 +40,22 @@ def oracle_rhs_periodic(prm, Qi, n, dx, dt, g=4):
     return R, F, S
 
 
-def oracle_rk3(prm, Qi, n, dx, dt, nsteps):
+def oracle_rk3(prm, Qi, n, dx, dt, nsteps, alpha=RK_ALPHA, beta=RK_BETA):
     """SSP-RK3 steps on a periodic domain with the oracle RHS; the stage
     update follows Euler::advanceSingleStepOnPatch (Euler.cpp:1003-1680)."""
     Q = Qi.copy()
     for _ in range(nsteps):
         Qint, R = [], []
         # (the zero-fill of Euler::advanceSingleStepOnPatch covers all stored arrays)
-        for sn in range(3):
+        for sn in range(len(alpha)):
             Qint.append(Q.copy())
             R.append(oracle_rhs_periodic(prm, Q, n, dx, dt)[0])
             Qn = np.zeros_like(Q)
             for m in range(sn + 1):
-                if RK_ALPHA[sn][m] != 0.0:
-                    Qn = Qn + RK_ALPHA[sn][m] * Qint[m]
-                if RK_BETA[sn][m] != 0.0:
-                    Qn = Qn + RK_BETA[sn][m] * R[m]
+                if alpha[sn][m] != 0.0:
+                    Qn = Qn + alpha[sn][m] * Qint[m]
+                if beta[sn][m] != 0.0:
+                    Qn = Qn + beta[sn][m] * R[m]
             if prm.nq > prm.neq:
                 # last volume fraction = 1 - sum of the others (FlowModelFiveEqnAllaire.cpp:1739-1890)
                 ns = prm.num_species
