@@ -13,6 +13,8 @@ struct KParams {
     double C, alpha_tau;            // WCNS6-LD constants
     double gamma[HM_MAX_SPECIES];   // species ratios of specific heats
     double R[HM_MAX_SPECIES];       // species gas constants
+    double cp[HM_MAX_SPECIES];      // species c_p = gamma/(gamma-1) R (four-equation mixing)
+    double cv[HM_MAX_SPECIES];      // species c_v = 1/(gamma-1) R
     int num_species;
     double dt;
 };
@@ -31,6 +33,8 @@ struct Plan {
 
 template <class W>
 hipError_t launch_rhs_single(int dim, int scheme, const Plan<W>& plan, const KParams& k, hipStream_t s);
+template <class W>
+hipError_t launch_rhs_four(int dim, int scheme, const Plan<W>& plan, const KParams& k, hipStream_t s);
 template <class W>
 hipError_t launch_rhs_five(int dim, int scheme, const Plan<W>& plan, const KParams& k, hipStream_t s);
 

@@ -1,575 +1,16 @@
-// hm_kernels_five.hip -- convective-flux RHS kernels, five-equation (Allaire) model.
-//
-// Equations (FlowModelFiveEqnAllaire.cpp:16-113): partial densities Z_i rho_i
-// (NS), momentum (D), total energy, volume fractions Z_i (NS-1, ADVECTIVE).
-// The caller hands over NEQ+1 conservative arrays: the last one is the last
-// species' volume fraction, stored but not evolved (WCNS56-HLLC-HLL.cpp:223-239),
-// which the cell EOS reads.  Scratch prim holds V = (Z_i rho_i, u.., p, Z_i (NS-1)),
-// then c, then the mixture density rho, over the 4-ghost box.
-//
-// Pipeline: k_derived_five, k_sensor (shared), k_midflux_five (also the HLLC
-// normal velocity u* of every midpoint face), k_faceflux_five, k_source_five.
-#include "hm_kernels.hpp"
-#include "hm_numerics.hpp"
-#include "hm_sensor.hpp"
+// hm_kernels_five.hip -- five-equation (Allaire) instantiation of the
+// multi-species RHS pipeline (kernels in hm_multi.hpp).
+#include "hm_multi.hpp"
 
 namespace hm {
 
 namespace {
 
-// Mixture ratio of specific heats from all NS volume fractions
-// (EquationOfStateMixingRulesIdealGas.cpp:7404-7598): xi = sum Z_i * (1/(gamma_i-1)).
-template <int NS>
-__device__ __forceinline__ double mix_gamma_all(const KParams& k, const double* Z) {
-    double xi = double(0);
-#pragma unroll
-    for (int si = 0; si < NS; si++) {
-        const double one_over_denominator = double(1) / (k.gamma[si] - double(1));
-        xi += Z[si] * one_over_denominator;
-    }
-    return double(1) / xi + double(1);
-}
-
-// Mixture ratio from NS-1 volume fractions, the last one being 1 - sum
-// (EquationOfStateMixingRulesIdealGas.cpp:7599-7800).
-template <int NS>
-__device__ __forceinline__ double mix_gamma_last(const KParams& k, const double* Z) {
-    double xi = double(0);
-    double Z_last = double(1);
-#pragma unroll
-    for (int si = 0; si < NS - 1; si++) {
-        const double one_over_denominator = double(1) / (k.gamma[si] - double(1));
-        xi += Z[si] * one_over_denominator;
-        Z_last -= Z[si];
-    }
-    xi += Z_last / (k.gamma[NS - 1] - double(1));
-    return double(1) / xi + double(1);
-}
-
-}  // namespace
-
-// ----------------------------------------------------------------------------
-// Derived cell data (FlowModelFiveEqnAllaire.cpp:3734-4876).
-// ----------------------------------------------------------------------------
-template <int D, int NS, class W>
-__global__ __launch_bounds__(kBlock) void k_derived_five(W w, KParams k) {
-    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    const DevPatch* P;
-    int ax;
-    int64_t loc;
-    if (!w.find(gid, P, ax, loc)) return;
-    constexpr int NEQ = D + 2 * NS;
-    const Boxes<D> B(*P);
-    const int e0 = B.ext(0, kGc), e1 = B.ext(1, kGc);
-    const int i = int(loc % e0) - kGc;
-    const int64_t r = loc / e0;
-    const int j = (D > 1) ? int(r % e1) - kGc : 0;
-    const int kk = (D > 2) ? int(r / e1) - kGc : 0;
-    const int64_t iq = q_idx<D>(*P, i, j, kk);
-
-    double Zr[NS], Z[NS];
-#pragma unroll
-    for (int si = 0; si < NS; si++) Zr[si] = P->Q[si][iq];
-#pragma unroll
-    for (int si = 0; si < NS; si++) Z[si] = P->Q[NS + D + 1 + si][iq];
-    double rho = double(0);  // EquationOfStateMixingRules.cpp:735-860
-#pragma unroll
-    for (int si = 0; si < NS; si++) rho += Zr[si];
-    double u[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int b = 0; b < D; b++) u[b] = P->Q[NS + b][iq] / rho;
-    double ke;
-    if constexpr (D == 1)
-        ke = u[0] * u[0];
-    else if constexpr (D == 2)
-        ke = u[0] * u[0] + u[1] * u[1];
-    else
-        ke = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    const double E = P->Q[NS + D][iq];
-    const double eps_int = E / rho - double(1) / double(2) * ke;
-    const double gmix = mix_gamma_all<NS>(k, Z);
-    const double p = (gmix - double(1)) * rho * eps_int;  // EquationOfStateIdealGas.cpp:5633
-    const double Gamma = gmix - double(1);
-    double c2 = Gamma * p / rho;  // FlowModelFiveEqnAllaire.cpp:4600-4640
-#pragma unroll
-    for (int si = 0; si < NS; si++) {
-        const double Psi = p / rho;  // EquationOfStateMixingRulesIdealGas.cpp:6608
-        c2 += (Zr[si] / rho) * Psi;
-    }
-
-    const int64_t vol = B.vol(kGc);
-    double* pr = P->prim;
-#pragma unroll
-    for (int si = 0; si < NS; si++) pr[si * vol + loc] = Zr[si];
-#pragma unroll
-    for (int b = 0; b < D; b++) pr[(NS + b) * vol + loc] = u[b];
-    pr[(NS + D) * vol + loc] = p;
-#pragma unroll
-    for (int si = 0; si < NS - 1; si++) pr[(NS + D + 1 + si) * vol + loc] = Z[si];
-    pr[NEQ * vol + loc] = sqrt(c2);
-    pr[(NEQ + 1) * vol + loc] = rho;
-}
-
-// ----------------------------------------------------------------------------
-// Fused midpoint flux of one face (five-equation model).
-// ----------------------------------------------------------------------------
-template <int D, int NS>
-__device__ __forceinline__ bool bounded_five(const double* V, const KParams& k) {
-    // FlowModelBasicUtilitiesFiveEqnAllaire.cpp:5446-7347 (bounds in the .hpp:24-27).
-    bool ok = true;
-    double Z[NS];
-    Z[NS - 1] = double(1);
-#pragma unroll
-    for (int si = 0; si < NS - 1; si++) {
-        Z[si] = V[NS + D + 1 + si];
-        Z[NS - 1] -= Z[si];
-        ok = ok && (Z[si] > double(-1000.0) && Z[si] < double(1000.0));
-    }
-    ok = ok && (Z[NS - 1] > double(-1000.0) && Z[NS - 1] < double(1000.0));
-    double rho = double(0);
-#pragma unroll
-    for (int si = 0; si < NS; si++) rho += V[si];
-    double Y[NS];
-#pragma unroll
-    for (int si = 0; si < NS; si++) {
-        Y[si] = V[si] / rho;
-        ok = ok && (Y[si] > double(-0.001) && Y[si] < double(1.001));
-    }
-#pragma unroll
-    for (int si = 0; si < NS; si++) ok = ok && (V[si] > double(0));
-    const double p = V[NS + D];
-    const double Gamma = mix_gamma_all<NS>(k, Z) - double(1);
-    double c2 = Gamma * p / rho;
-#pragma unroll
-    for (int si = 0; si < NS; si++) {
-        const double Psi = p / rho;
-        c2 += Y[si] * Psi;
-    }
-    return ok && (c2 > double(0));
-}
-
-template <int D, int NS, int A, int SCHEME, int PP, int QQ>
-__device__ __forceinline__ void midflux_five_face(const DevPatch& P, int64_t loc, const KParams& k) {
-    constexpr int NEQ = D + 2 * NS;
-    constexpr int IU = NS + A;   // normal velocity slot
-    constexpr int IP = NS + D;   // pressure slot
-    constexpr int IZ = NS + D + 1;
-    const Boxes<D> B(P);
-    const int m0 = B.mid_ext(A, 0), m1 = B.mid_ext(A, 1);
-    int c0 = int(loc % m0);
-    const int64_t r = loc / m0;
-    int c1 = (D > 1) ? int(r % m1) : 0;
-    int c2 = (D > 2) ? int(r / m1) : 0;
-    if constexpr (A == 0) c0 -= 1;
-    if constexpr (A == 1) c1 -= 1;
-    if constexpr (A == 2) c2 -= 1;
-
-    const int64_t v4 = B.vol(kGc);
-    const int64_t st = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGc)) : int64_t(B.ext(0, kGc)) * B.ext(1, kGc);
-    const int64_t iR = B.idx(kGc, c0, c1, c2);
-    const double* pr = P.prim;
-    auto ld = [&](int comp, int m) { return pr[comp * v4 + iR + (m - 3) * st]; };
-
-    // Projection variables (FlowModelBasicUtilitiesFiveEqnAllaire.cpp:7474-8207).
-    double Zra[NS];
-#pragma unroll
-    for (int si = 0; si < NS; si++) Zra[si] = double(1) / double(2) * (ld(si, 2) + ld(si, 3));
-    const double ra = double(1) / double(2) * (ld(NEQ + 1, 2) + ld(NEQ + 1, 3));
-    const double ca = double(1) / double(2) * (ld(NEQ, 2) + ld(NEQ, 3));
-
-    // Characteristic fields (FlowModelBasicUtilitiesFiveEqnAllaire.cpp:8232-9178), one at a time.
-    const LDConst ldk{k.p, k.q, k.C, k.alpha_tau};
-    double Wm[NEQ], Wp[NEQ];
-    {
-        Six s;
-#pragma unroll
-        for (int m = 0; m < 6; m++) s.v[m] = ld(IU, m) - double(1) / (ra * ca) * ld(IP, m);
-        reconstruct_pair<SCHEME, PP, QQ>(s, ldk, Wm[0], Wp[0]);
-    }
-#pragma unroll
-    for (int si = 0; si < NS; si++) {
-        Six s;
-#pragma unroll
-        for (int m = 0; m < 6; m++) s.v[m] = ld(si, m) - Zra[si] / (ra * ca * ca) * ld(IP, m);
-        reconstruct_pair<SCHEME, PP, QQ>(s, ldk, Wm[1 + si], Wp[1 + si]);
-    }
-    {
-        int t = NS + 1;
-#pragma unroll
-        for (int b = 0; b < D; b++) {
-            if (b == A) continue;
-            Six s;
-#pragma unroll
-            for (int m = 0; m < 6; m++) s.v[m] = ld(NS + b, m);
-            reconstruct_pair<SCHEME, PP, QQ>(s, ldk, Wm[t], Wp[t]);
-            t++;
-        }
-    }
-#pragma unroll
-    for (int si = 0; si < NS - 1; si++) {
-        Six s;
-#pragma unroll
-        for (int m = 0; m < 6; m++) s.v[m] = ld(IZ + si, m);
-        reconstruct_pair<SCHEME, PP, QQ>(s, ldk, Wm[NS + D + si], Wp[NS + D + si]);
-    }
-    {
-        Six s;
-#pragma unroll
-        for (int m = 0; m < 6; m++) s.v[m] = ld(IU, m) + double(1) / (ra * ca) * ld(IP, m);
-        reconstruct_pair<SCHEME, PP, QQ>(s, ldk, Wm[NEQ - 1], Wp[NEQ - 1]);
-    }
-
-    // Back-projection (FlowModelBasicUtilitiesFiveEqnAllaire.cpp:9201-9974).
-    double VL[NEQ], VR[NEQ];
-    auto back = [&](const double* Wc, double* Vo) {
-#pragma unroll
-        for (int si = 0; si < NS; si++)
-            Vo[si] = -double(1) / double(2) * Zra[si] / ca * Wc[0] + Wc[si + 1] +
-                     double(1) / double(2) * Zra[si] / ca * Wc[NEQ - 1];
-        Vo[IU] = double(1) / double(2) * Wc[0] + double(1) / double(2) * Wc[NEQ - 1];
-        int t = NS + 1;
-#pragma unroll
-        for (int b = 0; b < D; b++) {
-            if (b == A) continue;
-            Vo[NS + b] = Wc[t++];
-        }
-        Vo[IP] = -double(1) / double(2) * ra * ca * Wc[0] + double(1) / double(2) * ra * ca * Wc[NEQ - 1];
-#pragma unroll
-        for (int si = 0; si < NS - 1; si++) Vo[IZ + si] = Wc[NS + D + si];
-    };
-    back(Wm, VL);
-    back(Wp, VR);
-
-    // Bounds on both sides, else first-order values (WCNS56-HLLC-HLL.cpp:947-1032).
-    if (!(bounded_five<D, NS>(VL, k) && bounded_five<D, NS>(VR, k))) {
-#pragma unroll
-        for (int e = 0; e < NEQ; e++) {
-            VL[e] = ld(e, 2);
-            VR[e] = ld(e, 3);
-        }
-    }
-
-    // Face EOS (HLLC drivers, FlowModelRiemannSolverFiveEqnAllaireHLLC.cpp:4824-7716).
-    auto face_eos = [&](const double* V, double& rho, double& c, double& eps_int) {
-        rho = double(0);
-#pragma unroll
-        for (int si = 0; si < NS; si++) rho += V[si];
-        const double gmix = mix_gamma_last<NS>(k, V + IZ);
-        const double p = V[IP];
-        const double Gamma = gmix - double(1);
-        double c2 = Gamma * p / rho;
-#pragma unroll
-        for (int si = 0; si < NS; si++) {
-            const double Psi = p / rho;
-            c2 += (V[si] / rho) * Psi;
-        }
-        c = sqrt(c2);
-        eps_int = p / ((gmix - double(1)) * rho);
-    };
-    double rhoL, rhoR, cL, cR, epsL, epsR;
-    face_eos(VL, rhoL, cL, epsL);
-    face_eos(VR, rhoR, cR, epsR);
-    const double uL = VL[IU], uR = VR[IU], pL = VL[IP], pR = VR[IP];
-    const double u_avg = double(1) / double(2) * (uL + uR);
-    const double c_avg = double(1) / double(2) * (cL + cR);
-    const double sL = fmin(u_avg - c_avg, uL - cL);
-    const double sR = fmax(u_avg + c_avg, uR + cR);
-    const double s_minus = fmin(double(0), sL);
-    const double s_plus = fmax(double(0), sR);
-    const double s_star =
-        (pR - pL + rhoL * uL * (sL - uL) - rhoR * uR * (sR - uR)) / (rhoL * (sL - uL) - rhoR * (sR - uR));
-
-    // Sensor (WCNS56-HLLC-HLL.cpp:1066-1113).
-    bool use_hll = false;
-    if constexpr (D > 1) {
-        const int64_t vs = B.vol(kGs);
-        const int64_t tR = B.idx(kGs, c0, c1, c2);
-        const int64_t sts = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGs)) : int64_t(B.ext(0, kGs)) * B.ext(1, kGs);
-        const int64_t tL = tR - sts;
-        const double th_avg = 0.5 * (P.sens[tL] + P.sens[tR]);
-        const double om_avg = 0.5 * (P.sens[vs + tL] + P.sens[vs + tR]);
-        const double sensor = -th_avg / (fabs(th_avg) + om_avg + kEps);
-        use_hll = sensor > 0.65;
-    }
-
-    // Conservative state and physical flux of one side
-    // (FlowModelRiemannSolverFiveEqnAllaireHLLC.cpp:883-921).
-    auto cons_flux = [&](const double* V, double rho, double eps_int, double* Qs, double* Fs) {
-#pragma unroll
-        for (int si = 0; si < NS; si++) Qs[si] = V[si];
-#pragma unroll
-        for (int b = 0; b < D; b++) Qs[NS + b] = rho * V[NS + b];
-        double ke;
-        if constexpr (D == 1)
-            ke = V[NS] * V[NS];
-        else if constexpr (D == 2)
-            ke = V[NS] * V[NS] + V[NS + 1] * V[NS + 1];
-        else
-            ke = V[NS] * V[NS] + V[NS + 1] * V[NS + 1] + V[NS + 2] * V[NS + 2];
-        Qs[IP] = rho * (eps_int + double(1) / double(2) * ke);
-#pragma unroll
-        for (int si = 0; si < NS - 1; si++) Qs[IZ + si] = V[IZ + si];
-        const double un = V[IU];
-#pragma unroll
-        for (int si = 0; si < NS; si++) Fs[si] = un * V[si];
-#pragma unroll
-        for (int b = 0; b < D; b++) Fs[NS + b] = (b == A) ? un * Qs[NS + b] + V[IP] : un * Qs[NS + b];
-        Fs[IP] = un * (Qs[IP] + V[IP]);
-#pragma unroll
-        for (int si = 0; si < NS - 1; si++) Fs[IZ + si] = un * V[IZ + si];
-    };
-
-    const bool left = s_star > double(0);
-    double QL[NEQ], QR[NEQ], FL[NEQ], FR[NEQ];
-    if (use_hll) {
-        cons_flux(VL, rhoL, epsL, QL, FL);
-        cons_flux(VR, rhoR, epsR, QR, FR);
-    } else if (left) {
-        cons_flux(VL, rhoL, epsL, QL, FL);
-    } else {
-        cons_flux(VR, rhoR, epsR, QR, FR);
-    }
-    double Fo[NEQ];
-    double ustar;
-    {
-        const double* Vs = left ? VL : VR;
-        const double* Qs = left ? QL : QR;
-        const double* Fs = left ? FL : FR;
-        const double rho = left ? rhoL : rhoR;
-        const double sK = left ? sL : sR;
-        const double spm = left ? s_minus : s_plus;
-        const double chi = (sK - Vs[IU]) / (sK - s_star);
-        double Qst[NEQ];
-#pragma unroll
-        for (int si = 0; si < NS; si++) Qst[si] = chi * Vs[si];
-#pragma unroll
-        for (int b = 0; b < D; b++) Qst[NS + b] = (b == A) ? chi * rho * s_star : chi * Qs[NS + b];
-        Qst[IP] = chi * (Qs[IP] + (s_star - Vs[IU]) * (rho * s_star + Vs[IP] / (sK - Vs[IU])));
-#pragma unroll
-        for (int si = 0; si < NS - 1; si++) Qst[IZ + si] = chi * Vs[IZ + si];
-#pragma unroll
-        for (int e = 0; e < NEQ; e++) Fo[e] = Fs[e] + spm * (Qst[e] - Qs[e]);
-        ustar = Vs[IU] + spm * (chi - double(1));  // FlowModelRiemannSolverFiveEqnAllaireHLLC.cpp:5300-5310
-    }
-    if constexpr (D > 1) {
-        if (use_hll) {
-            // HLL on partial densities, transverse momentum and volume fractions
-            // (FlowModelRiemannSolverFiveEqnAllaireHLLC-HLL.cpp:1283-1504).
-            auto hll = [&](int e) {
-                double h = (sR * FL[e] - sL * FR[e] + sR * sL * (QR[e] - QL[e])) / (sR - sL);
-                if (sL > double(0)) h = FL[e];
-                if (sR < double(0)) h = FR[e];
-                return h;
-            };
-            double dsq;
-            {
-                const double du = VR[NS] - VL[NS];
-                const double dv = VR[NS + 1] - VL[NS + 1];
-                if constexpr (D == 2) {
-                    dsq = du * du + dv * dv;
-                } else {
-                    const double dw = VR[NS + 2] - VL[NS + 2];
-                    dsq = du * du + dv * dv + dw * dw;
-                }
-            }
-            const double vel_mag = sqrt(dsq);
-            double al1, al2;
-            if (vel_mag < kEps) {
-                al1 = double(1);
-                al2 = double(0);
-            } else {
-                al1 = fabs(VR[IU] - VL[IU]) / vel_mag;
-                al2 = sqrt(double(1) - al1 * al1);
-            }
-            const double be1 = double(1) / double(2) * (double(1) + al1 / (al1 + al2));
-            const double be2 = double(1) - be1;
-#pragma unroll
-            for (int si = 0; si < NS; si++) Fo[si] = be1 * Fo[si] + be2 * hll(si);
-#pragma unroll
-            for (int b = 0; b < D; b++)
-                if (b != A) Fo[NS + b] = be1 * Fo[NS + b] + be2 * hll(NS + b);
-#pragma unroll
-            for (int si = 0; si < NS - 1; si++) Fo[IZ + si] = be1 * Fo[IZ + si] + be2 * hll(IZ + si);
-        }
-    }
-
-    const int64_t mv = B.mid_vol(A);
-    double* out = P.fmid[A];
-#pragma unroll
-    for (int e = 0; e < NEQ; e++) out[e * mv + loc] = Fo[e];
-    P.ustar[A][loc] = ustar;
-}
-
-template <int D, int NS, int SCHEME, int PP, int QQ, class W>
-__global__ __launch_bounds__(kBlock) void k_midflux_five(W w, KParams k) {
-    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    const DevPatch* P;
-    int ax;
-    int64_t loc;
-    if (!w.find(gid, P, ax, loc)) return;
-    if (ax == 0) {
-        midflux_five_face<D, NS, 0, SCHEME, PP, QQ>(*P, loc, k);
-    } else if constexpr (D > 1) {
-        if (ax == 1) {
-            midflux_five_face<D, NS, 1, SCHEME, PP, QQ>(*P, loc, k);
-        } else if constexpr (D > 2) {
-            midflux_five_face<D, NS, 2, SCHEME, PP, QQ>(*P, loc, k);
-        }
-    }
-}
-
-// ----------------------------------------------------------------------------
-// Midpoint-to-face flux with the five-equation node flux
-// (FlowModelFiveEqnAllaire.cpp:4877-5901).
-// ----------------------------------------------------------------------------
-template <int D, int NS, int A>
-__device__ __forceinline__ void faceflux_five_face(const DevPatch& P, int64_t loc, const KParams& k) {
-    constexpr int NEQ = D + 2 * NS;
-    constexpr int IP = NS + D;
-    const Boxes<D> B(P);
-    const int f0 = B.face_ext(A, 0), f1 = B.face_ext(A, 1);
-    const int c0 = int(loc % f0);
-    const int64_t r = loc / f0;
-    const int c1 = (D > 1) ? int(r % f1) : 0;
-    const int c2 = (D > 2) ? int(r / f1) : 0;
-    const int64_t mv = B.mid_vol(A);
-    const int64_t mC = B.mid_idx(A, c0, c1, c2);
-    const int64_t mst = (A == 0) ? 1 : (A == 1) ? int64_t(B.mid_ext(A, 0)) : int64_t(B.mid_ext(A, 0)) * B.mid_ext(A, 1);
-    const int64_t v4 = B.vol(kGc);
-    const int64_t st4 = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGc)) : int64_t(B.ext(0, kGc)) * B.ext(1, kGc);
-    const int64_t pR = B.idx(kGc, c0, c1, c2), pL = pR - st4;
-    const int qst = (A == 0) ? 1 : (A == 1) ? (P.n[0] + 2 * P.gq[0]) : (P.n[0] + 2 * P.gq[0]) * (P.n[1] + 2 * P.gq[1]);
-    const int64_t qR = q_idx<D>(P, c0, c1, c2), qL = qR - qst;
-    const double uaL = P.prim[(NS + A) * v4 + pL], uaR = P.prim[(NS + A) * v4 + pR];
-    const double prL = P.prim[IP * v4 + pL], prR = P.prim[IP * v4 + pR];
-    const int64_t fv = B.face_vol(A);
-    const double* fm = P.fmid[A];
-    double* out = P.F[A];
-    const double dt = k.dt;
-#pragma unroll
-    for (int e = 0; e < NEQ; e++) {
-        double nL, nR;
-        if (e == IP) {
-            nL = uaL * (P.Q[IP][qL] + prL);
-            nR = uaR * (P.Q[IP][qR] + prR);
-        } else if (e == NS + A) {
-            nL = uaL * P.Q[e][qL] + prL;
-            nR = uaR * P.Q[e][qR] + prR;
-        } else {
-            // partial densities, transverse momentum and volume fractions: u_a * Q
-            nL = uaL * P.Q[e][qL];
-            nR = uaR * P.Q[e][qR];
-        }
-        const double* f = fm + e * mv;
-        const double mR = f[mC + mst], mL = f[mC - mst], mM = f[mC];
-        out[e * fv + loc] = dt * (double(1) / double(30) * (mR + mL) - double(3) / double(10) * (nR + nL) +
-                                  double(23) / double(15) * mM);
-    }
-}
-
-template <int D, int NS, class W>
-__global__ __launch_bounds__(kBlock) void k_faceflux_five(W w, KParams k) {
-    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    const DevPatch* P;
-    int ax;
-    int64_t loc;
-    if (!w.find(gid, P, ax, loc)) return;
-    if (ax == 0) {
-        faceflux_five_face<D, NS, 0>(*P, loc, k);
-    } else if constexpr (D > 1) {
-        if (ax == 1) {
-            faceflux_five_face<D, NS, 1>(*P, loc, k);
-        } else if constexpr (D > 2) {
-            faceflux_five_face<D, NS, 2>(*P, loc, k);
-        }
-    }
-}
-
-// ----------------------------------------------------------------------------
-// Advective source of the volume-fraction equations (WCNS56-HLLC-HLL.cpp:491-535,
-// 1288-1369, 2492-2650).
-// ----------------------------------------------------------------------------
-template <int D, int NS, class W>
-__global__ __launch_bounds__(kBlock) void k_source_five(W w, KParams k) {
-    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    const DevPatch* P;
-    int ax;
-    int64_t loc;
-    if (!w.find(gid, P, ax, loc)) return;
-    const Boxes<D> B(*P);
-    const int i = int(loc % B.n[0]);
-    const int64_t r = loc / B.n[0];
-    const int j = (D > 1) ? int(r % B.n[1]) : 0;
-    const int kk = (D > 2) ? int(r / B.n[1]) : 0;
-    const int64_t v4 = B.vol(kGc);
-    const int64_t pc = B.idx(kGc, i, j, kk);
-    double acc = 0.0;
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-        const int64_t st4 = (a == 0) ? 1 : (a == 1) ? int64_t(B.ext(0, kGc)) : int64_t(B.ext(0, kGc)) * B.ext(1, kGc);
-        const int64_t mst = (a == 0) ? 1 : (a == 1) ? int64_t(B.mid_ext(a, 0)) : int64_t(B.mid_ext(a, 0)) * B.mid_ext(a, 1);
-        const int64_t m = B.mid_idx(a, i, j, kk);  // face between cells i-1 and i along a
-        const double* us = P->ustar[a];
-        const double uLL = us[m - mst], uL = us[m], uR = us[m + mst], uRR = us[m + 2 * mst];
-        const double* u = P->prim + (NS + a) * v4;
-        const double cL = u[pc - st4], cR = u[pc + st4];
-        const double term = (double(3) / double(2) * (uR - uL) - double(3) / double(10) * (cR - cL) +
-                             double(1) / double(30) * (uRR - uLL)) /
-                            P->dx[a];
-        acc = (a == 0) ? term : acc + term;
-    }
-    const int64_t svol = int64_t(B.n[0]) * B.n[1] * B.n[2];
-    const int64_t iq = q_idx<D>(*P, i, j, kk);
-#pragma unroll
-    for (int si = 0; si < NS - 1; si++) {
-        const int e = NS + D + 1 + si;
-        P->S[e * svol + loc] += k.dt * P->Q[e][iq] * acc;
-    }
-}
-
-// ----------------------------------------------------------------------------
-// Host-side launch of the five-equation pipeline.
-// ----------------------------------------------------------------------------
-namespace {
-
-template <class W>
-unsigned nblocks(const Plan<W>& plan, int stage) {
-    return unsigned((plan.items[stage] + kBlock - 1) / kBlock);
-}
-
-template <int D, int NS, int SCHEME, int PP, int QQ, class W>
-hipError_t launch_five_dsq(const Plan<W>& plan, const KParams& k, hipStream_t s) {
-    if (plan.items[0] > 0)
-        hipLaunchKernelGGL((k_derived_five<D, NS, W>), dim3(nblocks(plan, 0)), dim3(kBlock), 0, s, plan.work[0], k);
-    if constexpr (D > 1) {
-        if (plan.items[1] > 0)
-            hipLaunchKernelGGL((k_sensor<D, NS, W>), dim3(nblocks(plan, 1)), dim3(kBlock), 0, s, plan.work[1], k);
-    }
-    if (plan.items[2] > 0)
-        hipLaunchKernelGGL((k_midflux_five<D, NS, SCHEME, PP, QQ, W>), dim3(nblocks(plan, 2)), dim3(kBlock), 0, s,
-                           plan.work[2], k);
-    if (plan.items[3] > 0)
-        hipLaunchKernelGGL((k_faceflux_five<D, NS, W>), dim3(nblocks(plan, 3)), dim3(kBlock), 0, s, plan.work[3], k);
-    if (plan.items[4] > 0)
-        hipLaunchKernelGGL((k_source_five<D, NS, W>), dim3(nblocks(plan, 4)), dim3(kBlock), 0, s, plan.work[4], k);
-    return hipGetLastError();
-}
-
-template <int D, int NS, class W>
-hipError_t launch_five_dn(const Plan<W>& plan, const KParams& k, int scheme, hipStream_t s) {
-    const bool dflt = (k.p == 2 && k.q == 4);
-    if (scheme == kWCNS6_LD)
-        return dflt ? launch_five_dsq<D, NS, kWCNS6_LD, 2, 4>(plan, k, s)
-                    : launch_five_dsq<D, NS, kWCNS6_LD, -1, -1>(plan, k, s);
-    return (k.p == 2) ? launch_five_dsq<D, NS, kWCNS5_JS, 2, 4>(plan, k, s)
-                      : launch_five_dsq<D, NS, kWCNS5_JS, -1, -1>(plan, k, s);
-}
-
 template <int D, class W>
 hipError_t launch_five_d(const Plan<W>& plan, const KParams& k, int scheme, hipStream_t s) {
     switch (k.num_species) {
-        case 2: return launch_five_dn<D, 2>(plan, k, scheme, s);
-        case 3: return launch_five_dn<D, 3>(plan, k, scheme, s);
+        case 2: return launch_multi_dn<D, 2, false>(plan, k, scheme, s);
+        case 3: return launch_multi_dn<D, 3, false>(plan, k, scheme, s);
         default: return hipErrorInvalidValue;
     }
 }

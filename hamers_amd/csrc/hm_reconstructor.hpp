@@ -63,6 +63,10 @@ class ConvectiveFluxReconstructorGPU {
         for (int i = 0; i < HM_MAX_SPECIES; i++) {
             k_.gamma[i] = cfg.species_gamma[i];
             k_.R[i] = cfg.species_R[i];
+            // EquationOfStateMixingRulesIdealGas.cpp:108-120
+            const bool used = i < cfg.num_species;
+            k_.cp[i] = used ? cfg.species_gamma[i] / (cfg.species_gamma[i] - double(1)) * cfg.species_R[i] : 0.0;
+            k_.cv[i] = used ? double(1) / (cfg.species_gamma[i] - double(1)) * cfg.species_R[i] : 0.0;
         }
         k_.num_species = cfg.num_species;
         k_.dt = 0.0;
@@ -90,8 +94,8 @@ class ConvectiveFluxReconstructorGPU {
     int numConservativeArrays() const { return cfg_.model == HM_MODEL_FIVE_EQN_ALLAIRE ? neq_ + 1 : neq_; }
 
     // Scratch components per cell of the 4-ghost box: V, c (and rho for the
-    // five-equation model).
-    int prim_comps() const { return cfg_.model == HM_MODEL_FIVE_EQN_ALLAIRE ? neq_ + 2 : neq_ + 1; }
+    // multi-species models).
+    int prim_comps() const { return cfg_.model == HM_MODEL_SINGLE_SPECIES ? neq_ + 1 : neq_ + 2; }
 
     // ConvectiveFluxReconstructor::getConvectiveFluxNumberOfGhostCells (4 for WCNS56).
     int getConvectiveFluxNumberOfGhostCells() const { return kGc; }
@@ -543,6 +547,7 @@ class ConvectiveFluxReconstructorGPU {
         hipError_t e = hipSuccess;
         switch (cfg_.model) {
             case HM_MODEL_SINGLE_SPECIES: e = launch_rhs_single<W>(cfg_.dim, cfg_.scheme, sub, k, s); break;
+            case HM_MODEL_FOUR_EQN_CONSERVATIVE: e = launch_rhs_four<W>(cfg_.dim, cfg_.scheme, sub, k, s); break;
             case HM_MODEL_FIVE_EQN_ALLAIRE: e = launch_rhs_five<W>(cfg_.dim, cfg_.scheme, sub, k, s); break;
             default: return fail(HM_ERR_UNSUPPORTED, "flow model not available in this build");
         }
@@ -598,8 +603,9 @@ inline int make_reconstructor(const hm_config& c, std::unique_ptr<ConvectiveFlux
         err = "unknown convective flux reconstructor";  // ConvectiveFluxReconstructorManager.cpp:146
         return HM_ERR_UNSUPPORTED;
     }
-    if (c.model != HM_MODEL_SINGLE_SPECIES && c.model != HM_MODEL_FIVE_EQN_ALLAIRE) {
-        err = "flow model not available in this build";
+    if (c.model != HM_MODEL_SINGLE_SPECIES && c.model != HM_MODEL_FOUR_EQN_CONSERVATIVE &&
+        c.model != HM_MODEL_FIVE_EQN_ALLAIRE) {
+        err = "unknown flow model";
         return HM_ERR_UNSUPPORTED;
     }
     const int ns = (c.model == HM_MODEL_SINGLE_SPECIES) ? 1 : c.num_species;
@@ -611,15 +617,24 @@ inline int make_reconstructor(const hm_config& c, std::unique_ptr<ConvectiveFlux
         err = "the five-equation model is built for 2 or 3 species";
         return HM_ERR_UNSUPPORTED;
     }
+    if (c.model == HM_MODEL_FOUR_EQN_CONSERVATIVE && ns > 3) {
+        err = "the four-equation model is built for 1 to 3 species";
+        return HM_ERR_UNSUPPORTED;
+    }
     if (c.constant_p < 1 || (c.scheme == HM_SCHEME_WCNS6_LD_HLLC_HLL && c.constant_q < 1)) {
         err = "WCNS exponents must be positive";
         return HM_ERR_INVALID_ARGUMENT;
     }
-    for (int i = 0; i < ns; i++)
+    for (int i = 0; i < ns; i++) {
         if (!(c.species_gamma[i] > 1.0)) {
             err = "species_gamma must exceed 1";
             return HM_ERR_INVALID_ARGUMENT;
         }
+        if (c.model == HM_MODEL_FOUR_EQN_CONSERVATIVE && !(c.species_R[i] > 0.0)) {
+            err = "species_R must be positive";
+            return HM_ERR_INVALID_ARGUMENT;
+        }
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         err = "no HIP device available";

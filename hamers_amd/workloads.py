@@ -171,3 +171,57 @@ def interface_advection_convergence(N: int, dim: int = 2):
     Q = conservative_from_primitive_five([Z1 * 2.0, Z2 * 1.0], [np.ones_like(Z1)] * dim, np.ones_like(Z1),
                                          [Z1, Z2], (1.6, 1.4))
     return Q, n, dx
+
+
+def species_heat_capacities(gamma, R):
+    """c_p,i = gamma_i/(gamma_i-1) R_i and c_v,i = 1/(gamma_i-1) R_i
+    (EquationOfStateMixingRulesIdealGas.cpp:108-120)."""
+    cp = [g / (g - 1.0) * r for g, r in zip(gamma, R)]
+    cv = [1.0 / (g - 1.0) * r for g, r in zip(gamma, R)]
+    return cp, cv
+
+
+def mixture_gamma_mass(Y, gamma, R):
+    """Mixture ratio of specific heats with mass fractions (ISOTHERMAL_AND_ISOBARIC
+    closure): sum Y_i c_p,i / sum Y_i c_v,i."""
+    cp, cv = species_heat_capacities(gamma, R)
+    c_p = sum(Y[i] * cp[i] for i in range(len(gamma)))
+    c_v = sum(Y[i] * cv[i] for i in range(len(gamma)))
+    return c_p / c_v
+
+
+def conservative_from_primitive_four(rho_Y, vel, p, gamma, R):
+    """(rho Y_i, [u, v, w], p) -> (rho Y_i, rho u.., E) for the four-equation
+    conservative model, E = p/(gamma_mix-1) + 1/2 rho |u|^2."""
+    ns, d = len(rho_Y), len(vel)
+    rho = sum(rho_Y)
+    Y = [r / rho for r in rho_Y]
+    gm = mixture_gamma_mass(Y, gamma, R)
+    Q = np.empty((ns + d + 1,) + np.shape(rho))
+    for i in range(ns):
+        Q[i] = rho_Y[i]
+    ke = np.zeros_like(rho)
+    for b in range(d):
+        Q[ns + b] = rho * vel[b]
+        ke = ke + vel[b] * vel[b]
+    Q[ns + d] = p / (gm - 1.0) + 0.5 * rho * ke
+    return Q
+
+
+FOUR_EQN_GAMMA = (1.6, 1.4)
+FOUR_EQN_R = (1.0, 2.0)
+
+
+def species_advection_convergence(N: int, dim: int = 2, gamma=FOUR_EQN_GAMMA, R=FOUR_EQN_R):
+    """Two ideal gases of different gamma and R advected on [-1,1]^d at u = 1,
+    p = 1: rho = 1 + 1/2 sin(pi sum x), Y_1 = 1/2 + 1/4 cos(pi sum x).  With
+    uniform velocity and pressure the exact solution is a pure translation."""
+    n = (N,) * dim
+    dx = (2.0 / N,) * dim
+    X = cell_centres(n, (-1.0,) * dim, dx)
+    s = sum(X)
+    rho = 1.0 + 0.5 * np.sin(np.pi * s)
+    Y1 = 0.5 + 0.25 * np.cos(np.pi * s)
+    Q = conservative_from_primitive_four([rho * Y1, rho * (1.0 - Y1)], [np.ones_like(rho)] * dim, np.ones_like(rho),
+                                         gamma, R)
+    return Q, n, dx

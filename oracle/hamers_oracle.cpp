@@ -10,6 +10,8 @@
 //   src/flow/convective_flux_reconstructors/WCNS56/ConvectiveFluxReconstructorWCNS6-LD-HLLC-HLL.cpp:9-319
 //   src/flow/convective_flux_reconstructors/WCNS56/ConvectiveFluxReconstructorWCNS5-JS-HLLC-HLL.cpp:9-164
 //   src/flow/flow_models/single-species/* (derived data, projection, bounds, Riemann solvers)
+//   src/flow/flow_models/four-eqn_conservative/*, five-eqn_Allaire/* (the same, multi-species)
+//   src/util/mixing_rules/equations_of_state/ideal_gas/* (mixture EOS)
 //   src/util/derivatives/DerivativeFirstOrder.cpp:216-228 (2nd-order central first derivative)
 //   src/apps/Euler/Euler.cpp:1003-1680 (Runge-Kutta stage update)
 //   src/algs/integrator/RungeKuttaLevelIntegrator.cpp:1672-1745, 3907-3924 (SSP-RK3 stage loop, weights)
@@ -263,6 +265,9 @@ Model make_model(const hmo_params& prm) {
     }
     if (m.model == 2) {
         m.neq = m.dim + 2 * m.ns;  // FlowModelFiveEqnAllaire.cpp:30
+        m.nproj = m.ns + 2;
+    } else if (m.model == 1) {
+        m.neq = m.ns + m.dim + 1;  // FlowModelBasicUtilitiesFourEqnConservative.hpp:20
         m.nproj = m.ns + 2;
     } else if (m.model == 0) {
         m.ns = 1;
@@ -702,13 +707,56 @@ void direction_pass_single(const hmo_params& prm, const Model& m, const int* n, 
 //   V (primitive, neq comps): Z_i rho_i (ns), u (d), p, Z_i (ns-1)
 // ===========================================================================
 
+// The same structure serves the four-equation conservative model
+// (FlowModelFourEqnConservative.cpp:16-113): partial densities rho Y_i (ns),
+// momentum (d), total energy; no volume fractions (nz = 0), the mixture EOS
+// mixes c_p and c_v with mass fractions, and no equation is ADVECTIVE.
 struct Five {
     int d, ns, neq;
+    int nz;     // evolved volume fractions: ns-1 (five-equation), 0 (four-equation)
+    bool four;  // four-equation conservative model
+    int mutation;
     double gam[8];
+    double cp[8], cv[8];  // species heat capacities (four-equation mixing)
     int iu() const { return ns; }          // first velocity slot of V
     int ip() const { return ns + d; }      // pressure slot of V
     int iz() const { return ns + d + 1; }  // first volume-fraction slot of V
 };
+
+// Mixture ratio of specific heats and c_v from mass fractions, ISOTHERMAL_AND_ISOBARIC
+// closure (EquationOfStateMixingRulesIdealGas.cpp:4677-4737, 6946-7150):
+// c_p = sum Y_i c_p,i, c_v = sum Y_i c_v,i, gamma = c_p / c_v.
+inline void mix_mass(const Five& f, const double* Y, double& gamma, double& c_v) {
+    double c_p = double(0);
+    c_v = double(0);
+    for (int si = 0; si < f.ns; si++) {
+        c_p += Y[si] * f.cp[si];
+        c_v += Y[si] * f.cv[si];
+    }
+    gamma = c_p / c_v;
+    if (f.mutation == 4) {  // planted mistake: mass-fraction average of the species gammas
+        gamma = double(0);
+        for (int si = 0; si < f.ns; si++) gamma += Y[si] * f.gam[si];
+    }
+}
+
+// Squared sound speed of the four-equation model from density, pressure and
+// mass fractions: Gamma p / rho + sum Y_i Psi_i with Gamma = gamma - 1 and
+// Psi_i = ((c_p,i - gamma c_v,i) / c_v + gamma - 1) eps, eps = p / ((gamma - 1) rho)
+// (FlowModelFourEqnConservative.cpp:5087-5440; EquationOfStateIdealGas.cpp:8059,
+// 6291; EquationOfStateMixingRulesIdealGas.cpp:6367-6564).  Also returns eps.
+inline double sound_speed_sq_four(const Five& f, double rho, double p, const double* Y, double& eps) {
+    double gamma, c_v;
+    mix_mass(f, Y, gamma, c_v);
+    const double Gamma = gamma - double(1);
+    eps = p / ((gamma - double(1)) * rho);
+    double c2 = Gamma * p / rho;
+    for (int si = 0; si < f.ns; si++) {
+        const double Psi = ((f.cp[si] - gamma * f.cv[si]) / c_v + gamma - double(1)) * eps;
+        c2 += Y[si] * Psi;
+    }
+    return c2;
+}
 
 // Mixture gamma from all ns volume fractions (EquationOfStateMixingRulesIdealGas.cpp:7404-7598):
 // xi = sum Z_i * (1/(gamma_i - 1)), gamma = 1/xi + 1.
@@ -764,7 +812,8 @@ void compute_derived_five(const Five& f, const int* n, int g, const double* Q, D
                 const int64_t iq = Qv.idx(i, j, k), iv = D.V.idx(i, j, k);
                 double Zr[8], Z[8];
                 for (int si = 0; si < ns; si++) Zr[si] = Q[si * qvol + iq];
-                for (int si = 0; si < ns; si++) Z[si] = Q[(ns + d + 1 + si) * qvol + iq];
+                if (!f.four)
+                    for (int si = 0; si < ns; si++) Z[si] = Q[(ns + d + 1 + si) * qvol + iq];
                 double rho = double(0);
                 for (int si = 0; si < ns; si++) rho += Zr[si];
                 double Y[8];
@@ -780,18 +829,27 @@ void compute_derived_five(const Five& f, const int* n, int g, const double* Q, D
                     ke = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
                 const double E = Q[(ns + d) * qvol + iq];
                 const double eps_int = E / rho - double(1) / double(2) * ke;
-                const double gmix = mix_gamma_all(f, Z);
-                const double p = (gmix - double(1)) * rho * eps_int;
-                const double Gamma = gmix - double(1);
-                double c2 = Gamma * p / rho;
-                for (int si = 0; si < ns; si++) {
-                    const double Psi = p / rho;
-                    c2 += Y[si] * Psi;
+                double p, c2;
+                if (f.four) {
+                    // FlowModelFourEqnConservative.cpp:5034-5086 (pressure), 5087-5440 (sound speed)
+                    double gmix, c_v, eps_psi;
+                    mix_mass(f, Y, gmix, c_v);
+                    p = (gmix - double(1)) * rho * eps_int;
+                    c2 = sound_speed_sq_four(f, rho, p, Y, eps_psi);
+                } else {
+                    const double gmix = mix_gamma_all(f, Z);
+                    p = (gmix - double(1)) * rho * eps_int;
+                    const double Gamma = gmix - double(1);
+                    c2 = Gamma * p / rho;
+                    for (int si = 0; si < ns; si++) {
+                        const double Psi = p / rho;
+                        c2 += Y[si] * Psi;
+                    }
                 }
                 for (int si = 0; si < ns; si++) D.V.c(si)[iv] = Zr[si];
                 for (int b = 0; b < d; b++) D.V.c(ns + b)[iv] = u[b];
                 D.V.c(ns + d)[iv] = p;
-                for (int si = 0; si < ns - 1; si++) D.V.c(ns + d + 1 + si)[iv] = Z[si];
+                for (int si = 0; si < f.nz; si++) D.V.c(ns + d + 1 + si)[iv] = Z[si];
                 D.c.c(0)[iv] = std::sqrt(c2);
                 D.rho.c(0)[iv] = rho;
                 for (int a = 0; a < d; a++) {
@@ -804,7 +862,7 @@ void compute_derived_five(const Five& f, const int* n, int g, const double* Q, D
                             D.Fn[a].c(ns + b)[iv] = ua * Q[(ns + b) * qvol + iq];
                     }
                     D.Fn[a].c(ns + d)[iv] = ua * (E + p);
-                    for (int si = 0; si < ns - 1; si++) D.Fn[a].c(ns + d + 1 + si)[iv] = ua * Z[si];
+                    for (int si = 0; si < f.nz; si++) D.Fn[a].c(ns + d + 1 + si)[iv] = ua * Z[si];
                 }
             }
 }
@@ -819,7 +877,7 @@ inline void char_from_prim_five(const Five& f, int a, const double* Zra, double 
     int t = ns + 1;
     for (int b = 0; b < d; b++)
         if (b != a) W[t++] = V[ns + b];
-    for (int si = 0; si < ns - 1; si++) W[ns + d + si] = V[ns + d + 1 + si];
+    for (int si = 0; si < f.nz; si++) W[ns + d + si] = V[ns + d + 1 + si];
     W[f.neq - 1] = un + double(1) / (ra * ca) * p;
 }
 
@@ -835,7 +893,7 @@ inline void prim_from_char_five(const Five& f, int a, const double* Zra, double 
     for (int b = 0; b < d; b++)
         if (b != a) V[ns + b] = W[t++];
     V[ns + d] = -double(1) / double(2) * ra * ca * W[0] + double(1) / double(2) * ra * ca * W[neq - 1];
-    for (int si = 0; si < ns - 1; si++) V[ns + d + 1 + si] = W[ns + d + si];
+    for (int si = 0; si < f.nz; si++) V[ns + d + 1 + si] = W[ns + d + si];
 }
 
 // Bounds of one side (FlowModelBasicUtilitiesFiveEqnAllaire.cpp:5446-7347,
@@ -843,6 +901,19 @@ inline void prim_from_char_five(const Five& f, int a, const double* Zra, double 
 inline int bounded_five(const Five& f, const double* V) {
     const int ns = f.ns, d = f.d;
     int ok = 1;
+    if (f.four) {
+        // FlowModelBasicUtilitiesFourEqnConservative.cpp:3737-4511 (bounds in the .hpp:24-25):
+        // mass fractions in (-0.001, 1.001), rho > 0, p > 0.
+        double rho = double(0);
+        for (int si = 0; si < ns; si++) rho += V[si];
+        for (int si = 0; si < ns; si++) {
+            const double Y = V[si] / rho;
+            ok &= (Y > double(-0.001) && Y < double(1.001)) ? 1 : 0;
+        }
+        ok &= (rho > double(0)) ? 1 : 0;
+        ok &= (V[ns + d] > double(0)) ? 1 : 0;
+        return ok;
+    }
     double Z[8];
     Z[ns - 1] = double(1);
     for (int si = 0; si < ns - 1; si++) {
@@ -881,6 +952,11 @@ void riemann_face_five(const Five& f, int a, bool use_hll, const double* VL, con
         for (int si = 0; si < ns; si++) rho += V[si];
         double Y[8];
         for (int si = 0; si < ns; si++) Y[si] = V[si] / rho;
+        if (f.four) {
+            // FlowModelRiemannSolverFourEqnConservativeHLLC.cpp:4345-4700 (c, eps on faces)
+            c = std::sqrt(sound_speed_sq_four(f, rho, V[ip], Y, eps_int));
+            return;
+        }
         const double gmix = mix_gamma_last(f, V + ns + d + 1);
         const double p = V[ip];
         const double Gamma = gmix - double(1);
@@ -916,7 +992,7 @@ void riemann_face_five(const Five& f, int a, bool use_hll, const double* VL, con
         else
             ke = V[ns] * V[ns] + V[ns + 1] * V[ns + 1] + V[ns + 2] * V[ns + 2];
         Qs[ns + d] = rho * (eps_int + double(1) / double(2) * ke);
-        for (int si = 0; si < ns - 1; si++) Qs[ns + d + 1 + si] = V[ns + d + 1 + si];
+        for (int si = 0; si < f.nz; si++) Qs[ns + d + 1 + si] = V[ns + d + 1 + si];
         const double un = V[iu];
         for (int si = 0; si < ns; si++) Fs[si] = un * V[si];
         for (int b = 0; b < d; b++) {
@@ -926,7 +1002,7 @@ void riemann_face_five(const Five& f, int a, bool use_hll, const double* VL, con
                 Fs[ns + b] = un * Qs[ns + b];
         }
         Fs[ns + d] = un * (Qs[ns + d] + V[ip]);
-        for (int si = 0; si < ns - 1; si++) Fs[ns + d + 1 + si] = un * V[ns + d + 1 + si];
+        for (int si = 0; si < f.nz; si++) Fs[ns + d + 1 + si] = un * V[ns + d + 1 + si];
     };
     double QL[12], QR[12], FL[12], FR[12];
     cons_flux(VL, rhoL, epsL, QL, FL);
@@ -951,7 +1027,7 @@ void riemann_face_five(const Five& f, int a, bool use_hll, const double* VL, con
                 Qst[ns + b] = chi * Qs[ns + b];
         }
         Qst[ns + d] = chi * (Qs[ns + d] + (sstar - V[iu]) * (rho * sstar + V[ip] / (sK - V[iu])));
-        for (int si = 0; si < ns - 1; si++) Qst[ns + d + 1 + si] = chi * V[ns + d + 1 + si];
+        for (int si = 0; si < f.nz; si++) Qst[ns + d + 1 + si] = chi * V[ns + d + 1 + si];
         for (int e = 0; e < neq; e++) Fhllc[e] = Fs[e] + spm * (Qst[e] - Qs[e]);
         if (ustar) *ustar = V[iu] + spm * (chi - double(1));
     }
@@ -988,7 +1064,7 @@ void riemann_face_five(const Five& f, int a, bool use_hll, const double* VL, con
         for (int si = 0; si < ns; si++) Fout[si] = be1 * Fhllc[si] + be2 * hll(si);
         for (int b = 0; b < d; b++)
             if (b != a) Fout[ns + b] = be1 * Fhllc[ns + b] + be2 * hll(ns + b);
-        for (int si = 0; si < ns - 1; si++) {
+        for (int si = 0; si < f.nz; si++) {
             const int e = ns + d + 1 + si;
             Fout[e] = be1 * Fhllc[e] + be2 * hll(e);
         }
@@ -1184,21 +1260,36 @@ void compute_theta_omega_five(const Five& f, const int* n, const DerivedFive& D,
     compute_theta_omega(m, n, tmp, dx, theta, Omega);
 }
 
-int rhs_patch_five(const hmo_params* prm, const int* n, int g, const double* dx, double dt, const double* Q,
-                   double* const* F, double* S) {
-    Five f{};
+// Model constants of the multi-species restatement; returns false for bad species counts.
+bool make_multi(const hmo_params* prm, Five& f) {
+    f = Five{};
     f.d = prm->dim;
     f.ns = prm->num_species;
-    f.neq = f.d + 2 * f.ns;
-    if (f.ns < 2 || f.ns > 8) return 5;
-    for (int i = 0; i < 8; i++) f.gam[i] = prm->gamma[i];
+    f.four = prm->model == 1;
+    f.neq = f.four ? f.ns + f.d + 1 : f.d + 2 * f.ns;
+    f.nz = f.four ? 0 : f.ns - 1;
+    f.mutation = prm->mutation;
+    if (f.ns < (f.four ? 1 : 2) || f.ns > 8) return false;
+    for (int i = 0; i < 8; i++) {
+        f.gam[i] = prm->gamma[i];
+        // EquationOfStateMixingRulesIdealGas.cpp:108-120
+        f.cp[i] = (i < f.ns) ? prm->gamma[i] / (prm->gamma[i] - double(1)) * prm->R[i] : 0.0;
+        f.cv[i] = (i < f.ns) ? double(1) / (prm->gamma[i] - double(1)) * prm->R[i] : 0.0;
+    }
+    return true;
+}
+
+int rhs_patch_five(const hmo_params* prm, const int* n, int g, const double* dx, double dt, const double* Q,
+                   double* const* F, double* S) {
+    Five f;
+    if (!make_multi(prm, f)) return 5;
     DerivedFive D;
     compute_derived_five(f, n, g, Q, D);
     CellArray theta, Omega;
     compute_theta_omega_five(f, n, D, dx, theta, Omega);
     FaceArray ustar[3];
     for (int a = 0; a < f.d; a++) direction_pass_five(*prm, f, n, a, D, theta, Omega, dt, F[a], ustar[a]);
-    if (S) source_five(f, n, g, dx, dt, Q, D, ustar, S);
+    if (S && !f.four) source_five(f, n, g, dx, dt, Q, D, ustar, S);
     return 0;
 }
 
@@ -1213,8 +1304,7 @@ int rhs_patch_impl(const hmo_params* prm, const int* n_in, int g, const double* 
         n[a] = n_in[a];
         if (n[a] < 1) return 4;
     }
-    if (prm->model == 2) return rhs_patch_five(prm, n, g, dx, dt, Q, F, S);
-    if (m.model != 0) return 2;  // four-equation model: not in this restatement yet
+    if (prm->model == 1 || prm->model == 2) return rhs_patch_five(prm, n, g, dx, dt, Q, F, S);
     Derived D;
     compute_derived_single(m, n, g, Q, D);
     CellArray theta, Omega;
@@ -1300,6 +1390,16 @@ void hmo_char_roundtrip_single(int dim, int axis, double ra, double ca, const do
 void hmo_riemann_single(int dim, int axis, double gamma, int use_hll, const double* VL, const double* VR, double* F,
                         double* ustar) {
     riemann_face_single(dim, axis, gamma, use_hll != 0, VL, VR, F, ustar);
+}
+
+// Midpoint flux of one face from left/right primitive states, four- or
+// five-equation model (V = partial densities, velocity, pressure[, volume fractions]).
+int hmo_riemann_multi(const hmo_params* prm, int axis, int use_hll, const double* VL, const double* VR, double* F,
+                      double* ustar) {
+    Five f;
+    if (!prm || (prm->model != 1 && prm->model != 2) || !make_multi(prm, f)) return 1;
+    riemann_face_five(f, axis, use_hll != 0, VL, VR, F, ustar);
+    return 0;
 }
 
 int hmo_max_threads(void) {

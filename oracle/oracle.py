@@ -70,8 +70,8 @@ class Params:
         c.p, c.q, c.C, c.alpha_tau = self.p, self.q, self.C, self.alpha_tau
         for i, gv in enumerate(self.gamma[:8]):
             c.gamma[i] = gv
-        for i, rv in enumerate(self.R[:8]):
-            c.R[i] = rv
+        for i in range(8):  # gas constants not given default to 1
+            c.R[i] = self.R[i] if i < len(self.R) else 1.0
         c.mutation = self.mutation
         return c
 
@@ -223,6 +223,23 @@ def char_roundtrip_single(dim: int, axis: int, rho_avg: float, c_avg: float, V):
     Vb = np.zeros(dim + 2)
     L.hmo_char_roundtrip_single(dim, axis, rho_avg, c_avg, V.ctypes.data, Wc.ctypes.data, Vb.ctypes.data)
     return Wc, Vb
+
+
+def riemann_multi(prm: Params, axis: int, use_hll: bool, VL, VR):
+    """Midpoint flux of one face, four- or five-equation model."""
+    L = lib()
+    L.hmo_riemann_multi.argtypes = [ctypes.POINTER(_Params), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.hmo_riemann_multi.restype = ctypes.c_int
+    VL = _arr(VL, prm.neq)
+    VR = _arr(VR, prm.neq)
+    F = np.zeros(prm.neq)
+    us = ctypes.c_double()
+    st = L.hmo_riemann_multi(ctypes.byref(prm.to_c()), axis, int(use_hll), VL.ctypes.data, VR.ctypes.data,
+                             F.ctypes.data, ctypes.byref(us))
+    if st != 0:
+        raise ValueError(f"oracle riemann_multi status {st}")
+    return F, us.value
 
 
 def riemann_single(dim: int, axis: int, gamma: float, use_hll: bool, VL, VR):

@@ -1,0 +1,189 @@
+"""GPU parity of the four-equation conservative model against the CPU oracle:
+flux per patch and batched, 1D-3D, 1-3 species, and whole RK3 steps of a
+level.  Tolerance: max-norm relative error <= 1e-12 per array (FP64)."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from hamers_amd import workloads as W
+from tests.util import max_rel_err, oracle_rk3
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+GAM3 = (1.6, 1.4, 1.3)
+R3 = (1.0, 2.0, 0.7)
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a visible GPU"
+    return torch
+
+
+def _prm(dim, scheme, ns=2):
+    return O.Params(dim=dim, model=O.FOUR_EQN_CONSERVATIVE, num_species=ns, scheme=scheme, gamma=list(GAM3[:ns]),
+                    R=list(R3[:ns]))
+
+
+def _rec(dim, scheme, ns=2):
+    from hamers_amd import ConvectiveFluxReconstructor
+
+    key = "WCNS6_LD_HLLC_HLL" if scheme == 1 else "WCNS5_JS_HLLC_HLL"
+    return ConvectiveFluxReconstructor("four", dim, "FOUR_EQN_CONSERVATIVE", key, num_species=ns,
+                                       species_gamma=GAM3[:ns], species_R=R3[:ns])
+
+
+def _patchdata(torch, Q, n, g, dx, neq):
+    from hamers_amd import PatchData
+
+    d = len(n)
+    Qd = torch.from_numpy(np.ascontiguousarray(Q)).to("cuda:0")
+    F = [torch.full(O.face_shape(neq, n, a), np.nan, dtype=torch.float64, device="cuda:0") for a in range(d)]
+    S = torch.zeros((neq,) + tuple(reversed(n)), dtype=torch.float64, device="cuda:0")
+    return PatchData(tuple(n), (g,) * d, tuple(dx), Qd, F, S)
+
+
+def compare(tag, Fg, Fr, Sg):
+    worst = 0.0
+    same = total = 0
+    for fg, fr in zip(Fg, Fr):
+        assert np.all(np.isfinite(fg)), f"{tag}: non-finite GPU flux"
+        worst = max(worst, float(np.max(np.abs(fg - fr)) / max(np.max(np.abs(fr)), 1e-300)))
+        same += int(np.sum(fg == fr))
+        total += fg.size
+    _say(f"{tag}: flux max rel err {worst:.3e}, bit-identical {same}/{total}")
+    assert worst <= TOL, f"{tag}: {worst:.3e}"
+    assert np.all(Sg == 0.0), f"{tag}: the four-equation model has no source"
+
+
+def gpu_vs_oracle(torch, tag, Q, n, g, dx, dt, dim, scheme, ns=2):
+    rec = _rec(dim, scheme, ns)
+    pd = _patchdata(torch, Q, n, g, dx, rec.num_eqn)
+    _say(f"case {tag}")
+    rec.computeConvectiveFluxAndSourceOnPatch(pd, 0.0, dt)
+    torch.cuda.synchronize()
+    Fr, _ = O.rhs_patch(_prm(dim, scheme, ns), Q, n, g, dx, dt)
+    compare(tag, [f.cpu().numpy() for f in pd.F], Fr, pd.S.cpu().numpy())
+
+
+def shock_interface_four(n, g, seed, ns=2):
+    """A Mach-1.5 shock in gas 1 approaching a perturbed interface with gas 2
+    (and 3), as partial densities."""
+    rng = np.random.default_rng(seed)
+    d = len(n)
+    shp = tuple(reversed([v + 2 * g for v in n]))
+    axes = [(np.arange(n[a] + 2 * g) - g + 0.5) / n[a] for a in range(d)]
+    X = list(reversed(np.meshgrid(*reversed(axes), indexing="ij")))
+    x = X[0]
+    pert = sum(0.05 * np.cos(2 * np.pi * (k + 2) * X[1 + (k % (d - 1))] + rng.uniform(0, 6)) for k in range(3)) \
+        if d > 1 else 0.0
+    phi = 0.5 * (1.0 + np.tanh((x - (0.6 + pert)) / 0.03))
+    Zs = [np.clip(1.0 - phi, 1e-6, 1 - 1e-6)]
+    if ns == 3:
+        Zs = [Zs[0] * 0.7, Zs[0] * 0.3]
+    if ns > 1:
+        Zs.append(1.0 - sum(Zs))
+    rho_s = [1.0, 0.2, 3.0][:ns]
+    shock = x < 0.3
+    M, g1 = 1.5, GAM3[0]
+    p2 = 1.0 + 2 * g1 / (g1 + 1) * (M * M - 1)
+    r2 = (g1 + 1) * M * M / ((g1 - 1) * M * M + 2)
+    u2 = M * np.sqrt(g1) * (1 - 1 / r2)
+    rY = [Zs[i] * rho_s[i] * np.where(shock, r2, 1.0) for i in range(ns)]
+    vel = [np.where(shock, u2, 0.0) + 0.01 * rng.uniform(-1, 1, shp)] + \
+          [0.01 * rng.uniform(-1, 1, shp) for _ in range(d - 1)]
+    p = np.where(shock, p2, 1.0)
+    return W.conservative_from_primitive_four(rY, vel, p, GAM3[:ns], R3[:ns])
+
+
+@pytest.mark.parametrize("scheme", [1, 0], ids=["WCNS6_LD", "WCNS5_JS"])
+def test_four_eqn_2d_species_advection(torch_dev, scheme):
+    Qi, n, dx = W.species_advection_convergence(48, dim=2, gamma=GAM3[:2], R=R3[:2])
+    Q = W.fill_periodic(Qi, 4)
+    gpu_vs_oracle(torch_dev, f"four-eqn 2D species advection scheme={scheme}", Q, n, 4, dx, 1e-3, 2, scheme)
+
+
+@pytest.mark.parametrize("scheme", [1, 0], ids=["WCNS6_LD", "WCNS5_JS"])
+def test_four_eqn_3d_shock_interface(torch_dev, scheme):
+    n = (16, 12, 10)
+    Q = shock_interface_four(n, 4, seed=3)
+    gpu_vs_oracle(torch_dev, f"four-eqn 3D shock-interface scheme={scheme}", Q, n, 4, tuple(1.0 / v for v in n),
+                  1e-3, 3, scheme)
+
+
+@pytest.mark.parametrize("ns", [1, 3])
+def test_four_eqn_2d_species_count(torch_dev, ns):
+    n = (24, 20)
+    Q = shock_interface_four(n, 4, seed=5, ns=ns)
+    gpu_vs_oracle(torch_dev, f"four-eqn 2D {ns} species", Q, n, 4, (1 / 24, 1 / 20), 1e-3, 2, 1, ns=ns)
+
+
+@pytest.mark.parametrize("scheme", [1, 0], ids=["WCNS6_LD", "WCNS5_JS"])
+def test_four_eqn_1d(torch_dev, scheme):
+    n = (40,)
+    Q = shock_interface_four(n, 4, seed=7)
+    gpu_vs_oracle(torch_dev, f"four-eqn 1D scheme={scheme}", Q, n, 4, (1 / 40,), 1e-3, 1, scheme)
+
+
+def test_four_eqn_positivity_fallback(torch_dev):
+    N, g = 20, 4
+    rng = np.random.default_rng(4)
+    Y1 = rng.uniform(0.0, 1.0, (N, N))
+    rho = 10 ** rng.uniform(-4, 0, (N, N))
+    Qi = W.conservative_from_primitive_four([rho * Y1 + 1e-12, rho * (1 - Y1) + 1e-12],
+                                            [rng.normal(size=(N, N)), rng.normal(size=(N, N))],
+                                            10 ** rng.uniform(-4, 0, (N, N)), GAM3[:2], R3[:2])
+    Q = W.fill_periodic(Qi, g)
+    gpu_vs_oracle(torch_dev, "four-eqn positivity", Q, (N, N), g, (1 / N, 1 / N), 1e-3, 2, 1)
+
+
+def test_four_eqn_batched_uneven_3d(torch_dev):
+    torch = torch_dev
+    sizes = [(8, 8, 8), (16, 8, 24), (24, 16, 8), (8, 32, 16), (12, 12, 12)]
+    rec = _rec(3, 1)
+    pds, refs = [], []
+    for i, n in enumerate(sizes):
+        Q = shock_interface_four(n, 4, seed=20 + i)
+        dx = tuple(1.0 / v for v in n)
+        pds.append(_patchdata(torch, Q, n, 4, dx, rec.num_eqn))
+        refs.append(O.rhs_patch(_prm(3, 1), Q, n, 4, dx, 1e-3))
+    _say("case four-eqn batched uneven 3D")
+    rec.computeConvectiveFluxAndSourceOnPatches(pds, 0.0, 1e-3)
+    torch.cuda.synchronize()
+    batched = [[f.cpu().numpy().copy() for f in pd.F] for pd in pds]
+    for pd in pds:
+        for f in pd.F:
+            f.fill_(np.nan)
+        rec.computeConvectiveFluxAndSourceOnPatch(pd, 0.0, 1e-3)
+    torch.cuda.synchronize()
+    for i, pd in enumerate(pds):
+        for a in range(3):
+            assert np.array_equal(batched[i][a], pd.F[a].cpu().numpy()), f"patch {i} axis {a}"
+        compare(f"four-eqn batched patch {i} {sizes[i]}", batched[i], refs[i][0], pd.S.cpu().numpy())
+
+
+def test_four_eqn_level_rk3(torch_dev):
+    from hamers_amd import Level
+
+    Qi, n, dx = W.species_advection_convergence(24, dim=2, gamma=GAM3[:2], R=R3[:2])
+    boxes = [((0, 0), (24, 10)), ((0, 10), (13, 14)), ((13, 10), (11, 14))]
+    rec = _rec(2, 1)
+    lvl = Level(rec, boxes, ((0, 0), n), (1, 1), dx)
+    lvl.write_global(Qi)
+    _say("case four-eqn level RK3")
+    lvl.advance(0.3 * dx[0], 2, use_graph=True)
+    torch_dev.cuda.synchronize()
+    got = lvl.read_global(Qi.shape)
+    want = oracle_rk3(_prm(2, 1), Qi, n, dx, 0.3 * dx[0], 2)
+    err = max_rel_err(got, want)
+    _say(f"four-eqn level RK3: max rel err {err:.3e}, bit-identical {int(np.sum(got == want))}/{got.size}")
+    assert err <= TOL
