@@ -9,6 +9,14 @@ pipeline stage for the whole level) is the headline; the per-patch path (one
 launch sequence per patch, the plain drop-in) is timed beside it, and the
 CPU oracle is timed on a bounded sample of patches on the host cores.
 
+A second workload is timed beside it and reported under "uneven_3d": 3D
+five-equation (Allaire) two-gas Euler, a Mach-1.5 shock in gas 1 (gamma 1.4)
+approaching a perturbed interface with gas 2 (gamma 1.67), WCNS6-LD + hybrid
+HLLC/HLLC-HLL, on a level tiled by uneven patches whose extents per axis are
+drawn (seeded) from {8, 16, 24, 32, 48, 64}; batched RHS against the
+per-patch path.  Its fields are generated on the GPU with torch (synthetic
+input only; every RHS runs in libhamers_amd.so).
+
 Prints one JSON line (rank 0).
 """
 from __future__ import annotations
@@ -38,7 +46,122 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-patch", action="store_true")
+    ap.add_argument("--uneven-seg", type=int, default=14, help="patches per axis of the uneven 3D level (0: skip)")
+    ap.add_argument("--uneven-steps", type=int, default=5)
     return ap.parse_args()
+
+
+UNEVEN_CHOICES = (8, 16, 24, 32, 48, 64)
+
+
+def shock_interface_patch_torch(torch, lo, n, g, dx, modes, gamma=(1.4, 1.67), rho_gas=(1.0, 0.2)):
+    """Conservative five-equation arrays (8, n2+2g, n1+2g, n0+2g) of one patch of the
+    shock / perturbed-interface field, halos from the same analytic field."""
+    dev = "cuda:0"
+    ax = [(torch.arange(n[a] + 2 * g, dtype=torch.float64, device=dev) + (lo[a] - g) + 0.5) * dx[a]
+          for a in range(3)]
+    z, y, x = torch.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    xi = torch.full_like(y, 0.6)
+    for k, (amp, ph_y, ph_z) in zip(range(2, 9), modes):
+        xi = xi + 0.05 * amp * 0.5 * (torch.cos(2 * torch.pi * k * y + ph_y) + torch.cos(2 * torch.pi * k * z + ph_z))
+    phi = 0.5 * (1.0 + torch.tanh((x - xi) / (2.0 * dx[0])))
+    Z1 = torch.clamp(1.0 - phi, 1e-6, 1.0 - 1e-6)
+    Z2 = 1.0 - Z1
+    M, g1 = 1.5, gamma[0]
+    p2 = 1.0 + 2 * g1 / (g1 + 1) * (M * M - 1)
+    r2 = (g1 + 1) * M * M / ((g1 - 1) * M * M + 2)
+    u2 = M * (g1 ** 0.5) * (1 - 1 / r2)
+    shock = x < 0.3
+    comp = torch.where(shock, torch.full_like(x, r2), torch.ones_like(x))
+    Zr1 = Z1 * rho_gas[0] * comp
+    Zr2 = Z2 * rho_gas[1] * comp
+    rho = Zr1 + Zr2
+    u = torch.where(shock, torch.full_like(x, u2), torch.zeros_like(x))
+    p = torch.where(shock, torch.full_like(x, p2), torch.ones_like(x))
+    E = p * (Z1 / (gamma[0] - 1.0) + Z2 / (gamma[1] - 1.0)) + 0.5 * rho * u * u
+    zero = torch.zeros_like(x)
+    return torch.stack([Zr1, Zr2, rho * u, zero, zero, E, Z1, Z2]).contiguous()
+
+
+def bench_uneven_3d(args, torch, np, time):
+    """Batched vs per-patch RHS of the uneven-patch 3D five-equation level."""
+    import ctypes
+
+    from hamers_amd import ConvectiveFluxReconstructor, PatchData, PatchList, capi
+
+    rng = np.random.default_rng(2024)
+    segs = [rng.choice(UNEVEN_CHOICES, size=args.uneven_seg) for _ in range(3)]
+    starts = [np.concatenate([[0], np.cumsum(s)[:-1]]) for s in segs]
+    Ng = [int(s.sum()) for s in segs]
+    dx = tuple(1.0 / v for v in Ng)
+    modes = [(rng.uniform(0.0, 1.0), rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi)) for _ in range(7)]
+    tot = sum(m[0] for m in modes)
+    modes = [(m[0] / tot, m[1], m[2]) for m in modes]
+    g, neq = 4, 7
+    rec = ConvectiveFluxReconstructor("uneven", 3, "FIVE_EQN_ALLAIRE", "WCNS6_LD_HLLC_HLL", num_species=2,
+                                      species_gamma=(1.4, 1.67), species_R=(1.0, 1.0))
+    dev = "cuda:0"
+    pds, boxes = [], []
+    for k in range(args.uneven_seg):
+        for j in range(args.uneven_seg):
+            for i in range(args.uneven_seg):
+                lo = (int(starts[0][i]), int(starts[1][j]), int(starts[2][k]))
+                n = (int(segs[0][i]), int(segs[1][j]), int(segs[2][k]))
+                Q = shock_interface_patch_torch(torch, lo, n, g, dx, modes)
+                F = [torch.empty((neq, n[2] + (a == 2), n[1] + (a == 1), n[0] + (a == 0)), dtype=torch.float64,
+                                 device=dev) for a in range(3)]
+                S = torch.zeros((neq, n[2], n[1], n[0]), dtype=torch.float64, device=dev)
+                pds.append(PatchData(n, (g, g, g), dx, Q, F, S))
+                boxes.append((lo, n))
+    cells = sum(b[1][0] * b[1][1] * b[1][2] for b in boxes)
+    plist = PatchList(pds, rec.num_arrays)
+    dt = 1.0e-4
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    fn = capi.lib.hm_compute_convective_flux_and_source_on_patch
+    views = [pd.view(rec.num_arrays) for pd in pds]
+
+    def batched():
+        rec.computeConvectiveFluxAndSourceOnPatches(plist, 0.0, dt, 0, stream)
+
+    def per_patch():
+        for v in views:
+            st = fn(rec._ctx, ctypes.byref(v), 0.0, dt, 0, sp)
+            if st:
+                capi.check(rec._ctx, st)
+
+    def timed(f, steps):
+        f()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            f()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    ms_b = timed(batched, args.uneven_steps)
+    rec.enable_profiling(True)
+    for _ in range(args.uneven_steps):
+        batched()
+    prof = rec.read_profile()
+    rec.enable_profiling(False)
+    stage = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in prof.items()}
+    ms_p = timed(per_patch, max(2, args.uneven_steps // 2)) if not args.no_per_patch else None
+    bytes_per_cell = 8 * (neq + 3 * neq + 2 * 1)  # Q read, F on 3 axes, S read-modify-write (1 advective eq)
+    ext = [np.bincount(np.concatenate(segs), minlength=65)[list(UNEVEN_CHOICES)].tolist()]
+    return {
+        "workload": "3D five-equation Allaire two-gas Euler, Mach-1.5 shock and perturbed interface, "
+                    "WCNS6_LD_HLLC_HLL, uneven patches (extents per axis from {8,16,24,32,48,64})",
+        "patches": len(pds), "cells": int(cells), "domain": Ng, "extent_counts": ext[0],
+        "ms_per_rhs_batched": ms_b, "value_batched": cells / (ms_b * 1e-3),
+        "ms_per_rhs_per_patch": ms_p, "value_per_patch": (cells / (ms_p * 1e-3)) if ms_p else None,
+        "batched_speedup": (ms_p / ms_b) if ms_p else None,
+        "stage_ms": stage, "bytes_per_cell": bytes_per_cell,
+        "rhs_hbm_frac": bytes_per_cell * cells / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
 
 
 def main():
@@ -147,6 +270,14 @@ def main():
                "sample": f"the whole 256-patch level, {reps} RHS evaluations ({tk:.1f} s), oracle built -O3 "
                          f"-fopenmp -ffp-contract=off, {nthreads} OpenMP threads over patches"}
 
+    uneven = None
+    npatches = len(pds)
+    if args.uneven_seg > 0:
+        # free the 2D level first
+        del plist, pds, views
+        torch.cuda.empty_cache()
+        uneven = bench_uneven_3d(args, torch, np, time)
+
     out = {
         "metric": "FP64 cell-RHS-evals/s (batched convective RHS, WCNS5-JS + hybrid HLLC/HLLC-HLL)",
         "value": value,
@@ -162,7 +293,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "2D single-species Euler four-quadrant Riemann, 2048^2 in 256 patches of "
                                "128^2, 4-cell halos, WCNS5_JS_HLLC_HLL, batched RHS", "cells": cells,
-                   "patches": len(pds), "patch": [P, P], "scheme": args.scheme},
+                   "patches": npatches, "patch": [P, P], "scheme": args.scheme},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
                      "kernel": f"k_{dominant}", "kernel_ms": stage_avg[dominant],
@@ -173,6 +304,7 @@ def main():
                       "batched_speedup": (ms_pp / ms_batched) if ms_pp else None},
         "host_wall_ms_per_step": wall_batched,
         "cpu_baseline": cpu,
+        "uneven_3d": uneven,
     }
     print(json.dumps(out), flush=True)
 
