@@ -52,6 +52,18 @@ class hm_patch_view(ctypes.Structure):
     ]
 
 
+class hm_vjp_patch_view(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int * 3),
+        ("ghost", ctypes.c_int * 3),
+        ("dx", ctypes.c_double * 3),
+        ("Q", ctypes.c_void_p * HM_MAX_EQN),
+        ("F_bar", ctypes.c_void_p * 3),
+        ("S_bar", ctypes.c_void_p),
+        ("Q_bar", ctypes.c_void_p * HM_MAX_EQN),
+    ]
+
+
 class hm_rk_patch_view(ctypes.Structure):
     _fields_ = [
         ("n", ctypes.c_int * 3),
@@ -125,9 +137,19 @@ def _load():
     L.hm_level_patch_array.argtypes = [P, ctypes.c_int, ctypes.POINTER(P), I]
     L.hm_level_fill_ghosts.argtypes = [P, P]
     L.hm_level_advance.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.c_int, P]
+    L.hm_convective_flux_vjp_on_patch.argtypes = [P, ctypes.POINTER(hm_vjp_patch_view), ctypes.c_double,
+                                                  ctypes.c_double, ctypes.c_int, P]
+    L.hm_convective_flux_vjp_on_patches.argtypes = [P, ctypes.POINTER(hm_vjp_patch_view), ctypes.c_int,
+                                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, P]
+    L.hm_level_advance_recorded.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
+    L.hm_level_adjoint.argtypes = [P, P]
+    L.hm_level_write_adjoint.argtypes = [P, ctypes.c_int, D]
+    L.hm_level_read_adjoint.argtypes = [P, ctypes.c_int, D]
     for name in ("hm_advance_single_step_on_patch", "hm_advance_single_step_on_patches", "hm_level_create",
                  "hm_level_destroy", "hm_level_set_rk_coefficients", "hm_level_write_patch",
-                 "hm_level_read_patch", "hm_level_patch_array", "hm_level_fill_ghosts", "hm_level_advance"):
+                 "hm_level_read_patch", "hm_level_patch_array", "hm_level_fill_ghosts", "hm_level_advance",
+                 "hm_convective_flux_vjp_on_patch", "hm_convective_flux_vjp_on_patches", "hm_level_advance_recorded",
+                 "hm_level_adjoint", "hm_level_write_adjoint", "hm_level_read_adjoint"):
         getattr(L, name).restype = ctypes.c_int
     return L
 

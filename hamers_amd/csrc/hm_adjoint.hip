@@ -1,0 +1,420 @@
+// hm_adjoint.hip -- VJP kernels of the single-species convective-flux RHS and
+// their host driver (see hm_adjoint.hpp for the structure).
+#include <algorithm>
+#include <cstring>
+
+#include "hm_adjoint.hpp"
+#include "hm_adjoint_face.hpp"
+#include "hm_ctx.hpp"
+#include "hm_tables.hpp"
+
+namespace hm {
+
+// ----------------------------------------------------------------------------
+// Kernels.
+// ----------------------------------------------------------------------------
+
+// Per midpoint face of axis A: adjoint of the midpoint-to-face combination
+// F_i = dt (1/30 (m_{i+1} + m_{i-1}) - 3/10 (n_i + n_{i-1}) + 23/15 m_i)
+// (WCNS56-HLLC-HLL.cpp:1200-1282) with respect to the midpoint flux m_j, then
+// the reverse sweep of the face computation; writes the stencil adjoints to G.
+template <int D, int A, int SCHEME, int PP, int QQ>
+__device__ __forceinline__ void adj_face(const AdjPatch& P, int64_t loc, const KParams& k) {
+    constexpr int NEQ = D + 2;
+    constexpr int NG = 6 * NEQ + 2;
+    const DevPatch& F = P.f;
+    const Boxes<D> B(F);
+    const int m0 = B.mid_ext(A, 0), m1 = B.mid_ext(A, 1);
+    int c0 = int(loc % m0);
+    const int64_t r = loc / m0;
+    int c1 = (D > 1) ? int(r % m1) : 0;
+    int c2 = (D > 2) ? int(r / m1) : 0;
+    if constexpr (A == 0) c0 -= 1;
+    if constexpr (A == 1) c1 -= 1;
+    if constexpr (A == 2) c2 -= 1;
+    const int j = (A == 0) ? c0 : (A == 1) ? c1 : c2;
+    const int nA = B.n[A];
+    const int64_t fv = B.face_vol(A);
+    auto fidx = [&](int jj) -> int64_t {
+        return B.face_idx(A, (A == 0) ? jj : c0, (A == 1) ? jj : c1, (A == 2) ? jj : c2);
+    };
+    const double* Fb = P.Fb[A];
+    const double dt = k.dt;
+    double mb[NEQ];
+    bool any = false;
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        const double fl = (j - 1 >= 0 && j - 1 <= nA) ? Fb[e * fv + fidx(j - 1)] : 0.0;
+        const double fr = (j + 1 >= 0 && j + 1 <= nA) ? Fb[e * fv + fidx(j + 1)] : 0.0;
+        const double fc = (j >= 0 && j <= nA) ? Fb[e * fv + fidx(j)] : 0.0;
+        mb[e] = dt * (1.0 / 30.0 * (fl + fr) + 23.0 / 15.0 * fc);
+        any = any || (mb[e] != 0.0);
+    }
+    const int64_t mv = B.mid_vol(A);
+    double* G = P.G[A];
+    if (!any) {
+#pragma unroll
+        for (int c = 0; c < NG; c++) G[c * mv + loc] = 0.0;
+        return;
+    }
+    const int64_t v4 = B.vol(kGc);
+    const int64_t st = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGc)) : int64_t(B.ext(0, kGc)) * B.ext(1, kGc);
+    const int64_t iR = B.idx(kGc, c0, c1, c2);
+    const double* pr = F.prim;
+    double V[6][NEQ];
+#pragma unroll
+    for (int m = 0; m < 6; m++)
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) V[m][e] = pr[e * v4 + iR + (m - 3) * st];
+    const double cL = pr[NEQ * v4 + iR - st];
+    const double cR = pr[NEQ * v4 + iR];
+    bool use_hll = false;
+    if constexpr (D > 1) {
+        const int64_t vs = B.vol(kGs);
+        const int64_t tR = B.idx(kGs, c0, c1, c2);
+        const int64_t sts = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGs)) : int64_t(B.ext(0, kGs)) * B.ext(1, kGs);
+        const int64_t tL = tR - sts;
+        const double th_avg = 0.5 * (F.sens[tL] + F.sens[tR]);
+        const double om_avg = 0.5 * (F.sens[vs + tL] + F.sens[vs + tR]);
+        const double sensor = -th_avg / (fabs(th_avg) + om_avg + kEps);
+        use_hll = sensor > 0.65;
+    }
+    double Vb[6][NEQ];
+#pragma unroll
+    for (int m = 0; m < 6; m++)
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) Vb[m][e] = 0.0;
+    double cLb = 0.0, cRb = 0.0;
+    face_adj_single<D, A, SCHEME, PP, QQ>(V, cL, cR, k.gamma[0], LDConst{k.p, k.q, k.C, k.alpha_tau}, use_hll, mb,
+                                          Vb, cLb, cRb);
+#pragma unroll
+    for (int m = 0; m < 6; m++)
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) G[(m * NEQ + e) * mv + loc] = Vb[m][e];
+    G[(6 * NEQ) * mv + loc] = cLb;
+    G[(6 * NEQ + 1) * mv + loc] = cRb;
+}
+
+template <int D, int SCHEME, int PP, int QQ, class W>
+__global__ __launch_bounds__(kBlock) void k_adj_face(W w, KParams k) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const AdjPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    if (ax == 0) {
+        adj_face<D, 0, SCHEME, PP, QQ>(*P, loc, k);
+    } else if constexpr (D > 1) {
+        if (ax == 1) {
+            adj_face<D, 1, SCHEME, PP, QQ>(*P, loc, k);
+        } else if constexpr (D > 2) {
+            adj_face<D, 2, SCHEME, PP, QQ>(*P, loc, k);
+        }
+    }
+}
+
+// Per cell of the 4-ghost box: gather the stencil adjoints of every midpoint
+// face whose stencil holds the cell, add the node-flux adjoint, and map V_bar,
+// c_bar through the derived-data Jacobian (FlowModelSingleSpecies.cpp:2649-3179:
+// u = m/rho, p = (gamma-1) rho (E/rho - |u|^2/2), c = sqrt(gamma p / rho)).
+template <int D, class W>
+__global__ __launch_bounds__(kBlock) void k_adj_cells(W w, KParams k) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const AdjPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    constexpr int NEQ = D + 2;
+    constexpr int IP = D + 1;
+    const DevPatch& F = P->f;
+    const Boxes<D> B(F);
+    const int e0 = B.ext(0, kGc), e1 = B.ext(1, kGc);
+    int c[3];
+    c[0] = int(loc % e0) - kGc;
+    const int64_t r = loc / e0;
+    c[1] = (D > 1) ? int(r % e1) - kGc : 0;
+    c[2] = (D > 2) ? int(r / e1) - kGc : 0;
+    const double gam = k.gamma[0];
+    const double dt = k.dt;
+    const int64_t v4 = B.vol(kGc);
+    const double rho = F.prim[loc];
+    double u[D];
+#pragma unroll
+    for (int b = 0; b < D; b++) u[b] = F.prim[(1 + b) * v4 + loc];
+    const double p = F.prim[IP * v4 + loc];
+    const double cs = F.prim[NEQ * v4 + loc];
+    const int64_t iq = q_idx<D>(F, c[0], c[1], c[2]);
+
+    double Vb[NEQ], Qbd[NEQ];
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        Vb[e] = 0.0;
+        Qbd[e] = 0.0;
+    }
+    double cb = 0.0;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        bool trans_ok = true;
+#pragma unroll
+        for (int b = 0; b < D; b++)
+            if (b != a && (c[b] < 0 || c[b] >= B.n[b])) trans_ok = false;
+        if (!trans_ok) continue;
+        const int na = B.n[a];
+        const int64_t mv = B.mid_vol(a);
+        const double* G = P->G[a];
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            const int jf = c[a] + 3 - m;
+            if (jf < -1 || jf > na + 1) continue;
+            const int64_t fi = B.mid_idx(a, (a == 0) ? jf : c[0], (a == 1) ? jf : c[1], (a == 2) ? jf : c[2]);
+#pragma unroll
+            for (int e = 0; e < NEQ; e++) Vb[e] += G[(m * NEQ + e) * mv + fi];
+            if (m == 2) cb += G[(6 * NEQ) * mv + fi];
+            if (m == 3) cb += G[(6 * NEQ + 1) * mv + fi];
+        }
+        // Node flux of this cell enters the faces c_a and c_a + 1 with weight -3/10 dt.
+        if (c[a] >= -1 && c[a] <= na) {
+            const int64_t fv = B.face_vol(a);
+            const double* Fb = P->Fb[a];
+            auto fidx = [&](int jj) -> int64_t {
+                return B.face_idx(a, (a == 0) ? jj : c[0], (a == 1) ? jj : c[1], (a == 2) ? jj : c[2]);
+            };
+            const bool hasL = c[a] >= 0, hasR = c[a] + 1 <= na;
+            double nb[NEQ];
+#pragma unroll
+            for (int e = 0; e < NEQ; e++) {
+                const double fL = hasL ? Fb[e * fv + fidx(c[a])] : 0.0;
+                const double fR = hasR ? Fb[e * fv + fidx(c[a] + 1)] : 0.0;
+                nb[e] = -dt * (3.0 / 10.0) * (fL + fR);
+            }
+            // n[0] = Q[1+a]; n[1+b] = u_a Q[1+b] (+ p if b == a); n[IP] = u_a (E + p)
+            Qbd[1 + a] += nb[0];
+#pragma unroll
+            for (int b = 0; b < D; b++) {
+                const double Qv = F.Q[1 + b][iq];
+                Qbd[1 + b] += u[a] * nb[1 + b];
+                Vb[1 + a] += Qv * nb[1 + b];
+                if (b == a) Vb[IP] += nb[1 + b];
+            }
+            const double E = F.Q[IP][iq];
+            Vb[1 + a] += (E + p) * nb[IP];
+            Qbd[IP] += u[a] * nb[IP];
+            Vb[IP] += u[a] * nb[IP];
+        }
+    }
+    // Derived-data Jacobian.
+    const double pb = Vb[IP] + cb * gam / (2.0 * cs * rho);
+    double rhob = Vb[0] - cb * gam * p / (2.0 * cs * rho * rho);
+    double ke = 0.0;
+#pragma unroll
+    for (int b = 0; b < D; b++) ke += u[b] * u[b];
+    rhob += pb * (gam - 1.0) * 0.5 * ke;
+    double* const* Qb = P->Qb;
+#pragma unroll
+    for (int b = 0; b < D; b++) {
+        Qb[1 + b][iq] = -pb * (gam - 1.0) * u[b] + Vb[1 + b] / rho + Qbd[1 + b];
+        rhob -= Vb[1 + b] * u[b] / rho;
+    }
+    Qb[0][iq] = rhob + Qbd[0];
+    Qb[IP][iq] = pb * (gam - 1.0) + Qbd[IP];
+}
+
+// Zero the whole Q_bar ghost box (cells beyond the 4-ghost box stay zero).
+template <int D, class W>
+__global__ __launch_bounds__(kBlock) void k_adj_zero(W w) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const AdjPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+#pragma unroll
+    for (int e = 0; e < D + 2; e++) P->Qb[e][loc] = 0.0;
+}
+
+namespace {
+
+unsigned nblk(int64_t items) { return unsigned((items + kBlock - 1) / kBlock); }
+
+template <int D, int SCHEME, int PP, int QQ>
+hipError_t launch_adj_dsq(const AdjPlanDev& pl, const KParams& k, hipStream_t s) {
+    using W = SegTableT<AdjPatch>;
+    if (pl.n_qbox > 0) hipLaunchKernelGGL((k_adj_zero<D, W>), dim3(nblk(pl.n_qbox)), dim3(kBlock), 0, s, pl.qbox);
+    if (pl.n_face > 0)
+        hipLaunchKernelGGL((k_adj_face<D, SCHEME, PP, QQ, W>), dim3(nblk(pl.n_face)), dim3(kBlock), 0, s, pl.face, k);
+    if (pl.n_cell > 0)
+        hipLaunchKernelGGL((k_adj_cells<D, W>), dim3(nblk(pl.n_cell)), dim3(kBlock), 0, s, pl.cell, k);
+    return hipGetLastError();
+}
+
+template <int D>
+hipError_t launch_adj_d(int scheme, const AdjPlanDev& pl, const KParams& k, hipStream_t s) {
+    const bool dflt = (k.p == 2 && k.q == 4);
+    if (scheme == kWCNS6_LD)
+        return dflt ? launch_adj_dsq<D, kWCNS6_LD, 2, 4>(pl, k, s) : launch_adj_dsq<D, kWCNS6_LD, -1, -1>(pl, k, s);
+    return (k.p == 2) ? launch_adj_dsq<D, kWCNS5_JS, 2, 4>(pl, k, s) : launch_adj_dsq<D, kWCNS5_JS, -1, -1>(pl, k, s);
+}
+
+}  // namespace
+
+hipError_t launch_adjoint_single(int dim, int scheme, const AdjPlanDev& plan, const KParams& k, hipStream_t s) {
+    switch (dim) {
+        case 1: return launch_adj_d<1>(scheme, plan, k, s);
+        case 2: return launch_adj_d<2>(scheme, plan, k, s);
+        case 3: return launch_adj_d<3>(scheme, plan, k, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Host driver.
+// ----------------------------------------------------------------------------
+
+AdjointSingle::~AdjointSingle() { release(cached_); }
+
+void AdjointSingle::release(Owned& o) {
+    if (o.scratch || o.tables) (void)hipDeviceSynchronize();
+    rec_->releaseOwnedPlan(o.fwd);
+    if (o.scratch) (void)hipFree(o.scratch);
+    if (o.tables) (void)hipFree(o.tables);
+    o.scratch = nullptr;
+    o.tables = nullptr;
+    o.plan = AdjPlanDev{};
+}
+
+int AdjointSingle::build(const hm_vjp_patch_view* views, int np, Owned& out) {
+    const hm_config& c = rec_->config();
+    if (c.model != HM_MODEL_SINGLE_SPECIES)
+        return rec_->fail(HM_ERR_UNSUPPORTED, "the adjoint is available for the single-species model");
+    if (np < 1 || !views) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "empty patch list");
+    const int d = c.dim, neq = rec_->numEqn();
+    std::vector<hm_patch_view> fv(np);
+    for (int i = 0; i < np; i++) {
+        const hm_vjp_patch_view& v = views[i];
+        hm_patch_view w{};
+        for (int a = 0; a < 3; a++) {
+            w.n[a] = v.n[a];
+            w.ghost[a] = v.ghost[a];
+            w.dx[a] = v.dx[a];
+        }
+        for (int e = 0; e < HM_MAX_EQN; e++) w.Q[e] = v.Q[e];
+        for (int a = 0; a < d; a++) {
+            if (!v.F_bar[a]) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "NULL flux adjoint");
+            // the forward stages used here (derived data, sensor) never write F
+            w.F[a] = const_cast<double*>(v.F_bar[a]);
+        }
+        for (int e = 0; e < neq; e++)
+            if (!v.Q_bar[e]) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "NULL Q adjoint array");
+        fv[i] = w;
+    }
+    release(out);
+    // Scratch: forward scratch of every patch, then the G buffers.
+    const size_t fwd_doubles = rec_->scratchDoubles(fv.data(), np);
+    std::vector<size_t> goff(size_t(np) * 3, 0);
+    size_t gtot = 0;
+    for (int i = 0; i < np; i++)
+        for (int a = 0; a < d; a++) {
+            int64_t mv = 1;
+            for (int b = 0; b < d; b++) mv *= fv[i].n[b] + ((b == a) ? 3 : 0);
+            goff[size_t(i) * 3 + a] = gtot;
+            gtot += (size_t(6 * neq + 2) * size_t(mv) + 31) / 32 * 32;
+        }
+    if (hipMalloc(&out.scratch, (fwd_doubles + gtot) * sizeof(double)) != hipSuccess) {
+        out.scratch = nullptr;
+        return rec_->fail(HM_ERR_OUT_OF_MEMORY, "adjoint scratch");
+    }
+    double* base = static_cast<double*>(out.scratch);
+    if (int st = rec_->buildOwnedPlan(fv.data(), np, base, out.fwd)) return st;
+    TableImage<AdjPatch> face, cell, qbox;
+    face.items.resize(np);
+    double* pb = base;
+    for (int i = 0; i < np; i++) {
+        AdjPatch& A = face.items[i];
+        std::memset(&A, 0, sizeof A);
+        A.f = rec_->devPatch(fv[i], pb);
+        pb += rec_->patchScratchDoubles(fv[i]);
+        for (int a = 0; a < 3; a++) {
+            A.Fb[a] = (a < d) ? views[i].F_bar[a] : nullptr;
+            A.G[a] = (a < d) ? base + fwd_doubles + goff[size_t(i) * 3 + a] : nullptr;
+        }
+        for (int e = 0; e < HM_MAX_EQN; e++) A.Qb[e] = (e < neq) ? views[i].Q_bar[e] : nullptr;
+        int64_t v4 = 1, vq = 1;
+        for (int a = 0; a < d; a++) {
+            v4 *= fv[i].n[a] + 2 * kGc;
+            vq *= fv[i].n[a] + 2 * fv[i].ghost[a];
+            int64_t mv = 1;
+            for (int b = 0; b < d; b++) mv *= fv[i].n[b] + ((b == a) ? 3 : 0);
+            face.add(i, a, mv);
+        }
+        cell.add(i, 0, v4);
+        qbox.add(i, 0, vq);
+    }
+    cell.items = face.items;
+    qbox.items = face.items;
+    const size_t bytes = face.bytes() + cell.bytes() + qbox.bytes();
+    if (hipMalloc(&out.tables, bytes) != hipSuccess) {
+        out.tables = nullptr;
+        return rec_->fail(HM_ERR_OUT_OF_MEMORY, "adjoint tables");
+    }
+    std::vector<char> h(bytes, 0);
+    char* dv = static_cast<char*>(out.tables);
+    size_t o = 0;
+    out.plan.face = face.write(h.data() + o, dv + o);
+    out.plan.n_face = face.total;
+    o += face.bytes();
+    out.plan.cell = cell.write(h.data() + o, dv + o);
+    out.plan.n_cell = cell.total;
+    o += cell.bytes();
+    out.plan.qbox = qbox.write(h.data() + o, dv + o);
+    out.plan.n_qbox = qbox.total;
+    if (hipMemcpy(out.tables, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+        return rec_->fail(HM_ERR_DEVICE, "adjoint table upload");
+    return HM_OK;
+}
+
+int AdjointSingle::run(const Owned& o, double dt, hipStream_t s) {
+    if (int st = rec_->runOwnedPlanStages(o.fwd, dt, 0, 2, s)) return st;  // derived data and sensor
+    KParams k = rec_->kernelParams();
+    k.dt = dt;
+    const hm_config& c = rec_->config();
+    const hipError_t e = launch_adjoint_single(c.dim, c.scheme, o.plan, k, s);
+    if (e != hipSuccess) return rec_->fail(HM_ERR_DEVICE, std::string("adjoint launch failed: ") + hipGetErrorString(e));
+    return HM_OK;
+}
+
+int AdjointSingle::vjp(const hm_vjp_patch_view* views, int np, double dt, hipStream_t s) {
+    if (np < 0 || (np > 0 && !views)) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "invalid patch list");
+    if (np == 0) return HM_OK;
+    if (int st = rec_->selectDevice()) return st;
+    const bool same = int(cached_views_.size()) == np && cached_.tables &&
+                      std::memcmp(cached_views_.data(), views, sizeof(hm_vjp_patch_view) * size_t(np)) == 0;
+    if (!same) {
+        if (hipStreamSynchronize(s) != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "stream synchronize failed");
+        cached_views_.clear();
+        if (int st = build(views, np, cached_)) return st;
+        cached_views_.assign(views, views + np);
+    }
+    return run(cached_, dt, s);
+}
+
+}  // namespace hm
+
+extern "C" {
+
+int hm_convective_flux_vjp_on_patches(hm_ctx* ctx, const hm_vjp_patch_view* patches, int num_patches, double time,
+                                      double dt, int rk_step, void* stream) {
+    (void)time;
+    (void)rk_step;
+    if (!ctx || !ctx->rec) return HM_ERR_INVALID_ARGUMENT;
+    if (!ctx->adj) {
+        ctx->adj.reset(new (std::nothrow) hm::AdjointSingle(ctx->rec.get()));
+        if (!ctx->adj) return HM_ERR_OUT_OF_MEMORY;
+    }
+    return ctx->adj->vjp(patches, num_patches, dt, static_cast<hipStream_t>(stream));
+}
+
+int hm_convective_flux_vjp_on_patch(hm_ctx* ctx, const hm_vjp_patch_view* patch, double time, double dt, int rk_step,
+                                    void* stream) {
+    return hm_convective_flux_vjp_on_patches(ctx, patch, 1, time, dt, rk_step, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// hm_adjoint.hpp -- discrete adjoint (vector-Jacobian product) of the
+// convective-flux RHS, single-species model.
+//
+// Forward map of one patch: Q (ghost box) -> F_a (faces, times dt).  Given
+// the adjoints F_bar_a of the face fluxes, the VJP returns Q_bar = (dF/dQ)^T
+// F_bar over the ghost box.  It runs as
+//   1. the forward derived-data and sensor stages (prim V, c; theta, Omega),
+//   2. k_adj_face: per midpoint face, the adjoint of the sixth-order
+//      midpoint-to-face combination, then the reverse sweep of the face's
+//      projection / reconstruction / Riemann flux (hm_adjoint_face.hpp); the
+//      six stencil cells' adjoints go to a per-face buffer G,
+//   3. k_adj_cells: per cell of the 4-ghost box, a deterministic gather of the
+//      G entries of every face whose stencil holds the cell, plus the node
+//      flux adjoint, mapped through the derived-data Jacobian to Q_bar.
+// No atomics: the result is reproducible bit for bit.
+#pragma once
+
+#include <vector>
+
+#include "hm_reconstructor.hpp"
+
+namespace hm {
+
+struct AdjPatch {
+    DevPatch f;                // forward view: n, gq, dx, Q, prim, sens
+    const double* Fb[3];       // face-flux adjoints per axis (layout of F)
+    double* G[3];              // per midpoint face: 6*NEQ + 2 stencil adjoints
+    double* Qb[HM_MAX_EQN];    // output over the Q ghost box
+};
+
+struct AdjPlanDev {
+    SegTableT<AdjPatch> face, cell, qbox;
+    int64_t n_face = 0, n_cell = 0, n_qbox = 0;
+};
+
+hipError_t launch_adjoint_single(int dim, int scheme, const AdjPlanDev& plan, const KParams& k, hipStream_t s);
+
+// Owner of the tables and scratch of VJPs over fixed patch lists.
+class AdjointSingle {
+   public:
+    explicit AdjointSingle(ConvectiveFluxReconstructorGPU* rec) : rec_(rec) {}
+    ~AdjointSingle();
+
+    struct Owned {
+        ConvectiveFluxReconstructorGPU::OwnedPlan fwd;
+        void* scratch = nullptr;  // forward scratch (prim, sens, ...) and the G buffers
+        void* tables = nullptr;   // AdjPatch array and segment tables
+        AdjPlanDev plan{};
+    };
+
+    int build(const hm_vjp_patch_view* views, int np, Owned& out);
+    void release(Owned& o);
+    int run(const Owned& o, double dt, hipStream_t s);
+
+    // One-shot VJP of a patch list (the C-ABI path); keeps the last list's plan.
+    int vjp(const hm_vjp_patch_view* views, int np, double dt, hipStream_t s);
+
+   private:
+    ConvectiveFluxReconstructorGPU* rec_;
+    Owned cached_{};
+    std::vector<hm_vjp_patch_view> cached_views_;
+};
+
+}  // namespace hm

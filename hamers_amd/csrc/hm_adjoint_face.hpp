@@ -1,0 +1,692 @@
+// hm_adjoint_face.hpp -- reverse (adjoint) sweep of the pointwise numerics of
+// one midpoint face, single-species model: characteristic projection, WCNS6-LD
+// or WCNS5-JS reconstruction of both sides, back-projection, positivity
+// fallback and the HLLC / HLLC-HLL flux.
+//
+// Each *_adj routine recomputes its forward intermediates with the forward
+// code's expression trees (so every branch -- R_tau > alpha_tau, s* > 0, the
+// sensor, the fallback, fmin/fmax selections -- is taken as in the forward
+// pass) and then accumulates the transposed local Jacobian times the incoming
+// adjoint.  Branch predicates are treated as constants, the usual discrete
+// adjoint of a piecewise-smooth map.
+//
+// Forward references: WCNS6-LD-HLLC-HLL.cpp:9-319, WCNS5-JS-HLLC-HLL.cpp:9-164,
+// FlowModelBasicUtilitiesSingleSpecies.cpp:3205-3222 (bounds), 4638-5134
+// (projection), 6194-6242 (characteristic), 7279-7323 (back-projection),
+// FlowModelRiemannSolverSingleSpeciesHLLC.cpp:519-1081, ...HLLC-HLL.cpp:804-1629.
+#pragma once
+
+#include "hm_numerics.hpp"
+
+namespace hm {
+
+HM_HD double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
+
+// d/dx of x^E (E >= 1) times g.
+template <int E>
+HM_HD double dpow_times(double x, int e_rt, double g) {
+    const int e = (E > 0) ? E : e_rt;
+    double r = 1.0;
+    for (int i = 1; i < e; i++) r *= x;
+    return double(e) * r * g;
+}
+
+// Gradient of the three sub-stencil indicators, weighted by bb0..bb2, added to sb.
+HM_HD void beta3_adj(const Six& s, double bb0, double bb1, double bb2, double* sb) {
+    const double t = double(1) / double(3);
+    const double* v = s.v;
+    sb[0] += t * bb0 * (8.0 * v[0] - 19.0 * v[1] + 11.0 * v[2]);
+    sb[1] += t * (bb0 * (-19.0 * v[0] + 50.0 * v[1] - 31.0 * v[2]) + bb1 * (8.0 * v[1] - 13.0 * v[2] + 5.0 * v[3]));
+    sb[2] += t * (bb0 * (11.0 * v[0] - 31.0 * v[1] + 20.0 * v[2]) + bb1 * (-13.0 * v[1] + 26.0 * v[2] - 13.0 * v[3]) +
+                  bb2 * (20.0 * v[2] - 31.0 * v[3] + 11.0 * v[4]));
+    sb[3] += t * (bb1 * (5.0 * v[1] - 13.0 * v[2] + 8.0 * v[3]) + bb2 * (-31.0 * v[2] + 50.0 * v[3] - 19.0 * v[4]));
+    sb[4] += t * bb2 * (11.0 * v[2] - 19.0 * v[3] + 8.0 * v[4]);
+}
+
+// Gradient of the six-point indicator (a symmetric quadratic form), times bb3.
+HM_HD void beta6_adj(const Six& s, double bb3, double* sb) {
+    // Upper-triangle coefficients c_ij of sum_{i<=j} c_ij s_i s_j.
+    const double c00 = 525910327.0, c01 = -4562164630.0, c02 = 7799501420.0, c03 = -6610694540.0,
+                 c04 = 2794296070.0, c05 = -472758974.0;
+    const double c11 = 5.0 * 2146987907.0, c12 = -5.0 * 7722406988.0, c13 = 5.0 * 6763559276.0,
+                 c14 = -5.0 * 2926461814.0, c15 = 5.0 * 503766638.0;
+    const double c22 = 20.0 * 1833221603.0, c23 = -20.0 * 3358664662.0, c24 = 20.0 * 1495974539.0,
+                 c25 = -20.0 * 263126407.0;
+    const double c33 = 20.0 * 1607794163.0, c34 = -20.0 * 1486026707.0, c35 = 20.0 * 268747951.0;
+    const double c44 = 5.0 * 1432381427.0, c45 = -5.0 * 536951582.0;
+    const double c55 = 263126407.0;
+    const double g = bb3 / 232243200.0;
+    const double* v = s.v;
+    sb[0] += g * (2.0 * c00 * v[0] + c01 * v[1] + c02 * v[2] + c03 * v[3] + c04 * v[4] + c05 * v[5]);
+    sb[1] += g * (c01 * v[0] + 2.0 * c11 * v[1] + c12 * v[2] + c13 * v[3] + c14 * v[4] + c15 * v[5]);
+    sb[2] += g * (c02 * v[0] + c12 * v[1] + 2.0 * c22 * v[2] + c23 * v[3] + c24 * v[4] + c25 * v[5]);
+    sb[3] += g * (c03 * v[0] + c13 * v[1] + c23 * v[2] + 2.0 * c33 * v[3] + c34 * v[4] + c35 * v[5]);
+    sb[4] += g * (c04 * v[0] + c14 * v[1] + c24 * v[2] + c34 * v[3] + 2.0 * c44 * v[4] + c45 * v[5]);
+    sb[5] += g * (c05 * v[0] + c15 * v[1] + c25 * v[2] + c35 * v[3] + c45 * v[4] + 2.0 * c55 * v[5]);
+}
+
+// Shock measure sigma of ld_sigma: adjoint sigb into sb.
+HM_HD void ld_sigma_adj(const Six& s, double sigb, double* sb) {
+    if (sigb == 0.0) return;
+    const double a1 = s.v[2] - s.v[1];
+    const double a2 = s.v[3] - s.v[2];
+    const double a3 = s.v[4] - s.v[3];
+    const double x1 = a1 - a2, x2 = a2 - a3;
+    const double den1 = fabs(a1) + fabs(a2) + kEps;
+    const double den2 = fabs(a2) + fabs(a3) + kEps;
+    const double t1 = fabs(x1) / den1;
+    const double t2 = fabs(x2) / den2;
+    double a1b = 0.0, a2b = 0.0, a3b = 0.0;
+    if (t1 >= t2) {
+        const double xb = sigb * sgn(x1) / den1;
+        const double db = -sigb * t1 / den1;
+        a1b += xb + db * sgn(a1);
+        a2b += -xb + db * sgn(a2);
+    } else {
+        const double xb = sigb * sgn(x2) / den2;
+        const double db = -sigb * t2 / den2;
+        a2b += xb + db * sgn(a2);
+        a3b += -xb + db * sgn(a3);
+    }
+    sb[1] -= a1b;
+    sb[2] += a1b - a2b;
+    sb[3] += a2b - a3b;
+    sb[4] += a3b;
+}
+
+// Adjoint of ld_side<P,Q>(s, sigma, k): adds d(out)/ds * ob to sb and
+// d(out)/dsigma * ob to sigb.
+template <int P, int Q>
+HM_HD void ld_side_adj(const Six& s, double sigma, const LDConst& k, double ob, double* sb, double& sigb) {
+    if (ob == 0.0) return;
+    double b0, b1, b2;
+    beta3(s, b0, b1, b2);
+    const double b3 = beta6(s);
+    const double d02 = b0 - b2;
+    const double tau5 = fabs(d02);
+    const double e0 = b0 + kEps, e1 = b1 + kEps, e2 = b2 + kEps, e3 = b3 + kEps;
+    const double r0 = tau5 / e0, r1 = tau5 / e1, r2 = tau5 / e2;
+    const double u0 = double(1) / double(16) * (double(1) + rep_pow<P>(r0, k.p));
+    const double u1 = double(5) / double(8) * (double(1) + rep_pow<P>(r1, k.p));
+    const double u2 = double(5) / double(16) * (double(1) + rep_pow<P>(r2, k.p));
+    const double us = u0 + u1 + u2;
+    const double uh0 = u0 / us, uh1 = u1 / us, uh2 = u2 / us;
+
+    const double bavg = double(1) / double(8) * (b0 + b2 + double(6) * b1);
+    const double d3 = b3 - bavg;
+    const double tau6 = fabs(d3);
+    const double q0 = tau6 / e0, q1 = tau6 / e1, q2 = tau6 / e2, q3 = tau6 / e3;
+    const double w0 = double(1) / double(32) * (k.C + rep_pow<Q>(q0, k.q));
+    const double w1 = double(15) / double(32) * (k.C + rep_pow<Q>(q1, k.q));
+    const double w2 = double(15) / double(32) * (k.C + rep_pow<Q>(q2, k.q));
+    const double w3 = double(1) / double(32) * (k.C + rep_pow<Q>(q3, k.q));
+    const double ws = w0 + w1 + w2 + w3;
+    const double wh0 = w0 / ws, wh1 = w1 / ws, wh2 = w2 / ws, wh3 = w3 / ws;
+    const bool blend = tau6 / (bavg + kEps) > k.alpha_tau;
+    double W0 = wh0, W1 = wh1, W2 = wh2, W3 = wh3;
+    if (blend) {
+        W0 = sigma * uh0 + (double(1) - sigma) * wh0;
+        W1 = sigma * uh1 + (double(1) - sigma) * wh1;
+        W2 = sigma * uh2 + (double(1) - sigma) * wh2;
+        W3 = (double(1) - sigma) * wh3;
+    }
+    const double* v = s.v;
+    // out = sum of linear combinations of the stencil values with weights W.
+    sb[0] += ob * (3.0 / 8.0 * W0);
+    sb[1] += ob * (-10.0 / 8.0 * W0 - 1.0 / 8.0 * W1);
+    sb[2] += ob * (15.0 / 8.0 * W0 + 6.0 / 8.0 * W1 + 3.0 / 8.0 * W2);
+    sb[3] += ob * (3.0 / 8.0 * W1 + 6.0 / 8.0 * W2 + 15.0 / 8.0 * W3);
+    sb[4] += ob * (-1.0 / 8.0 * W2 - 10.0 / 8.0 * W3);
+    sb[5] += ob * (3.0 / 8.0 * W3);
+    const double Wb0 = ob * (3.0 / 8.0 * v[0] - 10.0 / 8.0 * v[1] + 15.0 / 8.0 * v[2]);
+    const double Wb1 = ob * (-1.0 / 8.0 * v[1] + 6.0 / 8.0 * v[2] + 3.0 / 8.0 * v[3]);
+    const double Wb2 = ob * (3.0 / 8.0 * v[2] + 6.0 / 8.0 * v[3] - 1.0 / 8.0 * v[4]);
+    const double Wb3 = ob * (15.0 / 8.0 * v[3] - 10.0 / 8.0 * v[4] + 3.0 / 8.0 * v[5]);
+    double whb0 = Wb0, whb1 = Wb1, whb2 = Wb2, whb3 = Wb3;
+    double bb0 = 0.0, bb1 = 0.0, bb2 = 0.0, bb3 = 0.0;
+    if (blend) {
+        const double om = double(1) - sigma;
+        whb0 = om * Wb0;
+        whb1 = om * Wb1;
+        whb2 = om * Wb2;
+        whb3 = om * Wb3;
+        sigb += (uh0 - wh0) * Wb0 + (uh1 - wh1) * Wb1 + (uh2 - wh2) * Wb2 - wh3 * Wb3;
+        // upwind weights: normalization, then u_k = d_k (1 + r_k^P), r_k = tau5 / e_k
+        const double uhb0 = sigma * Wb0, uhb1 = sigma * Wb1, uhb2 = sigma * Wb2;
+        const double dotu = uhb0 * uh0 + uhb1 * uh1 + uhb2 * uh2;
+        const double ub0 = (uhb0 - dotu) / us, ub1 = (uhb1 - dotu) / us, ub2 = (uhb2 - dotu) / us;
+        const double rb0 = dpow_times<P>(r0, k.p, ub0 * (1.0 / 16.0));
+        const double rb1 = dpow_times<P>(r1, k.p, ub1 * (5.0 / 8.0));
+        const double rb2 = dpow_times<P>(r2, k.p, ub2 * (5.0 / 16.0));
+        const double tau5b = rb0 / e0 + rb1 / e1 + rb2 / e2;
+        bb0 -= rb0 * r0 / e0;
+        bb1 -= rb1 * r1 / e1;
+        bb2 -= rb2 * r2 / e2;
+        const double sg = sgn(d02);
+        bb0 += tau5b * sg;
+        bb2 -= tau5b * sg;
+    }
+    // central weights: normalization, then w_k = d_k (C + q_k^Q), q_k = tau6 / e_k
+    const double dotw = whb0 * wh0 + whb1 * wh1 + whb2 * wh2 + whb3 * wh3;
+    const double wb0 = (whb0 - dotw) / ws, wb1 = (whb1 - dotw) / ws, wb2 = (whb2 - dotw) / ws,
+                 wb3 = (whb3 - dotw) / ws;
+    const double qb0 = dpow_times<Q>(q0, k.q, wb0 * (1.0 / 32.0));
+    const double qb1 = dpow_times<Q>(q1, k.q, wb1 * (15.0 / 32.0));
+    const double qb2 = dpow_times<Q>(q2, k.q, wb2 * (15.0 / 32.0));
+    const double qb3 = dpow_times<Q>(q3, k.q, wb3 * (1.0 / 32.0));
+    const double tau6b = qb0 / e0 + qb1 / e1 + qb2 / e2 + qb3 / e3;
+    bb0 -= qb0 * q0 / e0;
+    bb1 -= qb1 * q1 / e1;
+    bb2 -= qb2 * q2 / e2;
+    bb3 -= qb3 * q3 / e3;
+    const double sg3 = sgn(d3);
+    bb3 += tau6b * sg3;
+    const double bavgb = -tau6b * sg3;
+    bb0 += bavgb * (1.0 / 8.0);
+    bb2 += bavgb * (1.0 / 8.0);
+    bb1 += bavgb * (6.0 / 8.0);
+    beta3_adj(s, bb0, bb1, bb2, sb);
+    beta6_adj(s, bb3, sb);
+}
+
+// Adjoint of js_side<P>(s, p).
+template <int P>
+HM_HD void js_side_adj(const Six& s, int p_rt, double ob, double* sb) {
+    if (ob == 0.0) return;
+    double b0, b1, b2;
+    beta3(s, b0, b1, b2);
+    const double e0 = b0 + kEps, e1 = b1 + kEps, e2 = b2 + kEps;
+    const double w0 = double(1.0) / double(16) / rep_pow<P>(e0, p_rt);
+    const double w1 = double(5.0) / double(8) / rep_pow<P>(e1, p_rt);
+    const double w2 = double(5.0) / double(16) / rep_pow<P>(e2, p_rt);
+    const double ws = w0 + w1 + w2;
+    const double W0 = w0 / ws, W1 = w1 / ws, W2 = w2 / ws;
+    const double* v = s.v;
+    sb[0] += ob * (3.0 / 8.0 * W0);
+    sb[1] += ob * (-10.0 / 8.0 * W0 - 1.0 / 8.0 * W1);
+    sb[2] += ob * (15.0 / 8.0 * W0 + 6.0 / 8.0 * W1 + 3.0 / 8.0 * W2);
+    sb[3] += ob * (3.0 / 8.0 * W1 + 6.0 / 8.0 * W2);
+    sb[4] += ob * (-1.0 / 8.0 * W2);
+    const double Wb0 = ob * (3.0 / 8.0 * v[0] - 10.0 / 8.0 * v[1] + 15.0 / 8.0 * v[2]);
+    const double Wb1 = ob * (-1.0 / 8.0 * v[1] + 6.0 / 8.0 * v[2] + 3.0 / 8.0 * v[3]);
+    const double Wb2 = ob * (3.0 / 8.0 * v[2] + 6.0 / 8.0 * v[3] - 1.0 / 8.0 * v[4]);
+    const double dot = Wb0 * W0 + Wb1 * W1 + Wb2 * W2;
+    const double wb0 = (Wb0 - dot) / ws, wb1 = (Wb1 - dot) / ws, wb2 = (Wb2 - dot) / ws;
+    const int p = (P > 0) ? P : p_rt;
+    // w_k = d_k e_k^-p  ->  dw_k/de_k = -p w_k / e_k
+    const double bb0 = -wb0 * double(p) * w0 / e0;
+    const double bb1 = -wb1 * double(p) * w1 / e1;
+    const double bb2 = -wb2 * double(p) * w2 / e2;
+    beta3_adj(s, bb0, bb1, bb2, sb);
+}
+
+// Adjoint of reconstruct_pair: (umb, upb) -> sb (six stencil values).
+template <int SCHEME, int P, int Q>
+HM_HD void reconstruct_pair_adj(const Six& s, const LDConst& k, double umb, double upb, double* sb) {
+    if (umb == 0.0 && upb == 0.0) return;
+    const Six f = flip(s);
+    double fb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if constexpr (SCHEME == kWCNS6_LD) {
+        const double sig = ld_sigma(s);
+        double sigb = 0.0;
+        ld_side_adj<P, Q>(s, sig, k, umb, sb, sigb);
+        ld_side_adj<P, Q>(f, sig, k, upb, fb, sigb);
+        ld_sigma_adj(s, sigb, sb);
+    } else {
+        js_side_adj<P>(s, k.p, umb, sb);
+        js_side_adj<P>(f, k.p, upb, fb);
+    }
+#pragma unroll
+    for (int m = 0; m < 6; m++) sb[m] += fb[5 - m];
+}
+
+// Adjoint of the single-species HLLC / HLLC-HLL midpoint flux of one face
+// along axis A (forward: midflux_single_face).  Fb -> VLb, VRb (accumulated).
+template <int D, int A>
+HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bool use_hll, const double* Fb,
+                              double* VLb, double* VRb) {
+    constexpr int NEQ = D + 2;
+    constexpr int IP = D + 1;
+    const double rhoL = VL[0], rhoR = VR[0];
+    const double pL = VL[IP], pR = VR[IP];
+    const double uL = VL[1 + A], uR = VR[1 + A];
+    const double csL = sqrt(gam * pL / rhoL);
+    const double csR = sqrt(gam * pR / rhoR);
+    const double epsL = pL / ((gam - double(1)) * rhoL);
+    const double epsR = pR / ((gam - double(1)) * rhoR);
+    const double u_avg = double(1) / double(2) * (uL + uR);
+    const double c_avg = double(1) / double(2) * (csL + csR);
+    const double A1 = u_avg - c_avg, A2 = uL - csL;
+    const double B1 = u_avg + c_avg, B2 = uR + csR;
+    const double sL = fmin(A1, A2);
+    const double sR = fmax(B1, B2);
+    const double s_minus = fmin(double(0), sL);
+    const double s_plus = fmax(double(0), sR);
+    const double num = pR - pL + rhoL * uL * (sL - uL) - rhoR * uR * (sR - uR);
+    const double den = rhoL * (sL - uL) - rhoR * (sR - uR);
+    const double s_star = num / den;
+
+    // Conservative states and fluxes of both sides.
+    double Q[2][NEQ], F[2][NEQ];
+    const double* Vs[2] = {VL, VR};
+    const double epss[2] = {epsL, epsR};
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+        const double* V = Vs[x];
+        Q[x][0] = V[0];
+#pragma unroll
+        for (int b = 0; b < D; b++) Q[x][1 + b] = V[0] * V[1 + b];
+        double ke;
+        if constexpr (D == 1)
+            ke = V[1] * V[1];
+        else if constexpr (D == 2)
+            ke = V[1] * V[1] + V[2] * V[2];
+        else
+            ke = V[1] * V[1] + V[2] * V[2] + V[3] * V[3];
+        Q[x][IP] = V[0] * (epss[x] + double(1) / double(2) * ke);
+        F[x][0] = Q[x][1 + A];
+#pragma unroll
+        for (int b = 0; b < D; b++) F[x][1 + b] = (b == A) ? Q[x][1 + A] * V[1 + b] + V[IP] : Q[x][1 + A] * V[1 + b];
+        F[x][IP] = V[1 + A] * (Q[x][IP] + V[IP]);
+    }
+    // Upwind side K of the contact, selected element by element (no
+    // run-time indexing of private arrays).
+    const bool left = s_star > double(0);
+    const double sK = left ? sL : sR;
+    const double spm = left ? s_minus : s_plus;
+    const double uK = left ? uL : uR, rhoK = left ? rhoL : rhoR, pK = left ? pL : pR;
+    double QK[NEQ], FK[NEQ];
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        QK[e] = left ? Q[0][e] : Q[1][e];
+        FK[e] = left ? F[0][e] : F[1][e];
+    }
+    const double chi = (sK - uK) / (sK - s_star);
+    const double g = sK - uK;
+    const double h = rhoK * s_star + pK / g;
+    const double inner = QK[IP] + (s_star - uK) * h;
+    double Qst[NEQ], Fh[NEQ];
+    Qst[0] = chi * rhoK;
+#pragma unroll
+    for (int b = 0; b < D; b++) Qst[1 + b] = (b == A) ? chi * rhoK * s_star : chi * QK[1 + b];
+    Qst[IP] = chi * inner;
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) Fh[e] = FK[e] + spm * (Qst[e] - QK[e]);
+
+    double Qb[2][NEQ], Fxb[2][NEQ];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) {
+            Qb[x][e] = 0.0;
+            Fxb[x][e] = 0.0;
+        }
+    double sLb = 0.0, sRb = 0.0, sstb = 0.0;
+    double Vb[2][NEQ];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) Vb[x][e] = 0.0;
+
+    double Fhb[NEQ];
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) Fhb[e] = Fb[e];
+    if constexpr (D > 1) {
+        if (use_hll) {
+            // Forward hybrid: mass and transverse momentum blend HLLC with HLL.
+            double du[D];
+            double dsq = 0.0;
+            {
+                const double d0 = VR[1] - VL[1];
+                const double d1 = VR[2] - VL[2];
+                du[0] = d0;
+                du[1] = d1;
+                if constexpr (D == 2) {
+                    dsq = d0 * d0 + d1 * d1;
+                } else {
+                    const double d2 = VR[3] - VL[3];
+                    du[2] = d2;
+                    dsq = d0 * d0 + d1 * d1 + d2 * d2;
+                }
+            }
+            const double vm = sqrt(dsq);
+            double al1, al2;
+            const bool tiny = vm < kEps;
+            if (tiny) {
+                al1 = double(1);
+                al2 = double(0);
+            } else {
+                al1 = fabs(VR[1 + A] - VL[1 + A]) / vm;
+                al2 = sqrt(double(1) - al1 * al1);
+            }
+            const double Ssum = al1 + al2;
+            const double be1 = double(1) / double(2) * (double(1) + al1 / Ssum);
+            const double be2 = double(1) - be1;
+            double be1b = 0.0, be2b = 0.0;
+            const double dsl = sR - sL;
+#pragma unroll
+            for (int e = 0; e < NEQ; e++) {
+                const bool hyb = (e == 0) || (e >= 1 && e <= D && e != 1 + A);
+                if (!hyb) continue;
+                // recompute the HLL flux of component e
+                double hv;
+                const double numh = sR * F[0][e] - sL * F[1][e] + sR * sL * (Q[1][e] - Q[0][e]);
+                hv = numh / dsl;
+                if (sL > double(0)) hv = F[0][e];
+                if (sR < double(0)) hv = F[1][e];
+                const double fb = Fb[e];
+                Fhb[e] = be1 * fb;
+                const double hb = be2 * fb;
+                be1b += fb * Fh[e];
+                be2b += fb * hv;
+                if (sR < double(0)) {
+                    Fxb[1][e] += hb;
+                } else if (sL > double(0)) {
+                    Fxb[0][e] += hb;
+                } else {
+                    const double nb = hb / dsl;
+                    const double db = -hb * hv / dsl;
+                    sRb += nb * (F[0][e] + sL * (Q[1][e] - Q[0][e])) + db;
+                    sLb += nb * (-F[1][e] + sR * (Q[1][e] - Q[0][e])) - db;
+                    Fxb[0][e] += nb * sR;
+                    Fxb[1][e] -= nb * sL;
+                    Qb[1][e] += nb * sR * sL;
+                    Qb[0][e] -= nb * sR * sL;
+                }
+            }
+            be1b -= be2b;
+            if (!tiny) {
+                const double al1b = be1b * 0.5 * al2 / (Ssum * Ssum);
+                double al2b = -be1b * 0.5 * al1 / (Ssum * Ssum);
+                double a1b = al1b;
+                if (al2 > 0.0) a1b += al2b * (-al1 / al2);
+                const double dua = du[A];
+                const double duab = a1b * sgn(dua) / vm;
+                const double vmb = -a1b * al1 / vm;
+                const double dsqb = (vm > 0.0) ? vmb / (2.0 * vm) : 0.0;
+#pragma unroll
+                for (int b = 0; b < D; b++) {
+                    double dub = dsqb * 2.0 * du[b];
+                    if (b == A) dub += duab;
+                    Vb[1][1 + b] += dub;
+                    Vb[0][1 + b] -= dub;
+                }
+            }
+        }
+    }
+    // HLLC on the side K.
+    double spmb = 0.0, chib = 0.0;
+    double Qstb[NEQ], QKb[NEQ], FKb[NEQ];
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        FKb[e] = Fhb[e];
+        spmb += Fhb[e] * (Qst[e] - QK[e]);
+        Qstb[e] = spm * Fhb[e];
+        QKb[e] = -spm * Fhb[e];
+    }
+    double rhoKb = 0.0, uKb = 0.0, pKb = 0.0, sKb = 0.0;
+    chib += Qstb[0] * rhoK;
+    rhoKb += Qstb[0] * chi;
+#pragma unroll
+    for (int b = 0; b < D; b++) {
+        if (b == A) {
+            chib += Qstb[1 + b] * rhoK * s_star;
+            rhoKb += Qstb[1 + b] * chi * s_star;
+            sstb += Qstb[1 + b] * chi * rhoK;
+        } else {
+            chib += Qstb[1 + b] * QK[1 + b];
+            QKb[1 + b] += Qstb[1 + b] * chi;
+        }
+    }
+    {
+        chib += Qstb[IP] * inner;
+        const double innerb = Qstb[IP] * chi;
+        QKb[IP] += innerb;
+        sstb += innerb * h;
+        uKb -= innerb * h;
+        const double hb = innerb * (s_star - uK);
+        rhoKb += hb * s_star;
+        sstb += hb * rhoK;
+        pKb += hb / g;
+        const double gb = -hb * pK / (g * g);
+        sKb += gb;
+        uKb -= gb;
+    }
+    {
+        const double b2 = sK - s_star;
+        const double ab = chib / b2;
+        const double b2b = -chib * chi / b2;
+        sKb += ab + b2b;
+        uKb -= ab;
+        sstb -= b2b;
+    }
+    if (left) {
+        if (sL < double(0)) sLb += spmb;
+        sLb += sKb;
+    } else {
+        if (sR > double(0)) sRb += spmb;
+        sRb += sKb;
+    }
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        Qb[0][e] += left ? QKb[e] : 0.0;
+        Qb[1][e] += left ? 0.0 : QKb[e];
+        Fxb[0][e] += left ? FKb[e] : 0.0;
+        Fxb[1][e] += left ? 0.0 : FKb[e];
+    }
+    Vb[0][0] += left ? rhoKb : 0.0;
+    Vb[1][0] += left ? 0.0 : rhoKb;
+    Vb[0][1 + A] += left ? uKb : 0.0;
+    Vb[1][1 + A] += left ? 0.0 : uKb;
+    Vb[0][IP] += left ? pKb : 0.0;
+    Vb[1][IP] += left ? 0.0 : pKb;
+
+    // Conservative states and fluxes of both sides.
+    double epsb[2] = {0.0, 0.0};
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+        const double* V = Vs[x];
+        double* vb = Vb[x];
+        const double* fb = Fxb[x];
+        double* qb = Qb[x];
+        // F[0] = Q[1+A]
+        qb[1 + A] += fb[0];
+#pragma unroll
+        for (int b = 0; b < D; b++) {
+            qb[1 + A] += fb[1 + b] * V[1 + b];
+            vb[1 + b] += fb[1 + b] * Q[x][1 + A];
+            if (b == A) vb[IP] += fb[1 + b];
+        }
+        vb[1 + A] += fb[IP] * (Q[x][IP] + V[IP]);
+        qb[IP] += fb[IP] * V[1 + A];
+        vb[IP] += fb[IP] * V[1 + A];
+        // Q[0] = rho, Q[1+b] = rho u_b, Q[IP] = rho (eps + ke/2)
+        vb[0] += qb[0];
+#pragma unroll
+        for (int b = 0; b < D; b++) {
+            vb[0] += qb[1 + b] * V[1 + b];
+            vb[1 + b] += qb[1 + b] * V[0];
+        }
+        double ke = 0.0;
+#pragma unroll
+        for (int b = 0; b < D; b++) ke += V[1 + b] * V[1 + b];
+        vb[0] += qb[IP] * (epss[x] + 0.5 * ke);
+        epsb[x] += qb[IP] * V[0];
+#pragma unroll
+        for (int b = 0; b < D; b++) vb[1 + b] += qb[IP] * V[0] * V[1 + b];
+    }
+    // s* = num / den
+    {
+        const double numb = sstb / den;
+        const double denb = -sstb * s_star / den;
+        Vb[1][IP] += numb;
+        Vb[0][IP] -= numb;
+        Vb[0][0] += numb * uL * (sL - uL) + denb * (sL - uL);
+        Vb[0][1 + A] += numb * (rhoL * (sL - uL) - rhoL * uL) - denb * rhoL;
+        sLb += numb * rhoL * uL + denb * rhoL;
+        Vb[1][0] -= numb * uR * (sR - uR) + denb * (sR - uR);
+        Vb[1][1 + A] -= numb * (rhoR * (sR - uR) - rhoR * uR) - denb * rhoR;
+        sRb -= numb * rhoR * uR + denb * rhoR;
+    }
+    // sL = fmin(A1, A2), sR = fmax(B1, B2)
+    double uavgb = 0.0, cavgb = 0.0, csb[2] = {0.0, 0.0};
+    if (A1 <= A2) {
+        uavgb += sLb;
+        cavgb -= sLb;
+    } else {
+        Vb[0][1 + A] += sLb;
+        csb[0] -= sLb;
+    }
+    if (B1 >= B2) {
+        uavgb += sRb;
+        cavgb += sRb;
+    } else {
+        Vb[1][1 + A] += sRb;
+        csb[1] += sRb;
+    }
+    Vb[0][1 + A] += 0.5 * uavgb;
+    Vb[1][1 + A] += 0.5 * uavgb;
+    csb[0] += 0.5 * cavgb;
+    csb[1] += 0.5 * cavgb;
+    // c = sqrt(gam p / rho), eps = p / ((gam - 1) rho)
+    const double css[2] = {csL, csR};
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+        const double* V = Vs[x];
+        const double ab = csb[x] / (2.0 * css[x]);
+        Vb[x][IP] += ab * gam / V[0] + epsb[x] / ((gam - 1.0) * V[0]);
+        Vb[x][0] -= ab * gam * V[IP] / (V[0] * V[0]) + epsb[x] * epss[x] / V[0];
+    }
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        VLb[e] += Vb[0][e];
+        VRb[e] += Vb[1][e];
+    }
+}
+
+// Adjoint of the whole midpoint-flux computation of one face along axis A:
+// V[m][e] are the primitive variables of stencil cells i-3+m, cL/cR the sound
+// speeds of cells i-1 and i, mb the adjoint of the midpoint flux.  Adds
+// d(flux)/dV * mb to Vb and d(flux)/dc * mb to cLb, cRb.
+template <int D, int A, int SCHEME, int P, int Q>
+HM_HD void face_adj_single(const double (&V)[6][D + 2], double cL, double cR, double gam, const LDConst& ld,
+                           bool use_hll, const double* mb, double (&Vb)[6][D + 2], double& cLb, double& cRb) {
+    constexpr int NEQ = D + 2;
+    constexpr int IP = D + 1;
+    const double ra = double(1) / double(2) * (V[2][0] + V[3][0]);
+    const double ca = double(1) / double(2) * (cL + cR);
+    // Characteristic field f at stencil point m (forward expression trees).
+    auto field = [&](int f, int m) -> double {
+        if (f == 0) return -double(1) / double(2) * ra * ca * V[m][1 + A] + double(1) / double(2) * V[m][IP];
+        if (f == 1) return V[m][0] - double(1) / (ca * ca) * V[m][IP];
+        if (f == IP) return double(1) / double(2) * ra * ca * V[m][1 + A] + double(1) / double(2) * V[m][IP];
+        int t = 2;
+        for (int b = 0; b < D; b++) {
+            if (b == A) continue;
+            if (t == f) return V[m][1 + b];
+            t++;
+        }
+        return 0.0;
+    };
+    double Wm[NEQ], Wp[NEQ];
+#pragma unroll
+    for (int f = 0; f < NEQ; f++) {
+        Six s;
+#pragma unroll
+        for (int m = 0; m < 6; m++) s.v[m] = field(f, m);
+        reconstruct_pair<SCHEME, P, Q>(s, ld, Wm[f], Wp[f]);
+    }
+    double VL[NEQ], VR[NEQ];
+    auto back = [&](const double* W, double* Vo) {
+        Vo[0] = double(1) / (ca * ca) * W[0] + W[1] + double(1) / (ca * ca) * W[IP];
+        Vo[1 + A] = -double(1) / (ra * ca) * W[0] + double(1) / (ra * ca) * W[IP];
+        int t = 2;
+        for (int b = 0; b < D; b++) {
+            if (b == A) continue;
+            Vo[1 + b] = W[t++];
+        }
+        Vo[IP] = W[0] + W[IP];
+    };
+    back(Wm, VL);
+    back(Wp, VR);
+    const bool ok = (VL[0] > double(0)) && (VL[IP] > double(0)) && (VR[0] > double(0)) && (VR[IP] > double(0));
+    if (!ok) {
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) {
+            VL[e] = V[2][e];
+            VR[e] = V[3][e];
+        }
+    }
+    double VLb[NEQ], VRb[NEQ];
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        VLb[e] = 0.0;
+        VRb[e] = 0.0;
+    }
+    riemann_single_adj<D, A>(VL, VR, gam, use_hll, mb, VLb, VRb);
+    if (!ok) {
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) {
+            Vb[2][e] += VLb[e];
+            Vb[3][e] += VRb[e];
+        }
+        return;
+    }
+    // Back-projection adjoint for both sides.
+    double Wmb[NEQ], Wpb[NEQ];
+    double rab = 0.0, cab = 0.0;
+    auto back_adj = [&](const double* W, const double* Vob, double* Wb) {
+        const double ic2 = double(1) / (ca * ca);
+        const double irc = double(1) / (ra * ca);
+        Wb[0] = Vob[0] * ic2 - Vob[1 + A] * irc + Vob[IP];
+        Wb[1] = Vob[0];
+        Wb[IP] = Vob[0] * ic2 + Vob[1 + A] * irc + Vob[IP];
+        int t = 2;
+        for (int b = 0; b < D; b++) {
+            if (b == A) continue;
+            Wb[t++] = Vob[1 + b];
+        }
+        cab += Vob[0] * (W[0] + W[IP]) * (-2.0 * ic2 / ca);
+        cab += Vob[1 + A] * (W[IP] - W[0]) * (-irc / ca);
+        rab += Vob[1 + A] * (W[IP] - W[0]) * (-irc / ra);
+    };
+    back_adj(Wm, VLb, Wmb);
+    back_adj(Wp, VRb, Wpb);
+    // Reconstruction adjoint, field by field, then the characteristic projection.
+#pragma unroll
+    for (int f = 0; f < NEQ; f++) {
+        Six s;
+#pragma unroll
+        for (int m = 0; m < 6; m++) s.v[m] = field(f, m);
+        double sb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        reconstruct_pair_adj<SCHEME, P, Q>(s, ld, Wmb[f], Wpb[f], sb);
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            const double g = sb[m];
+            if (f == 0 || f == IP) {
+                const double sgn_ = (f == 0) ? -0.5 : 0.5;
+                Vb[m][1 + A] += g * sgn_ * ra * ca;
+                Vb[m][IP] += 0.5 * g;
+                rab += g * sgn_ * ca * V[m][1 + A];
+                cab += g * sgn_ * ra * V[m][1 + A];
+            } else if (f == 1) {
+                Vb[m][0] += g;
+                Vb[m][IP] -= g / (ca * ca);
+                cab += g * V[m][IP] * 2.0 / (ca * ca * ca);
+            } else {
+                int t = 2;
+                for (int b = 0; b < D; b++) {
+                    if (b == A) continue;
+                    if (t == f) Vb[m][1 + b] += g;
+                    t++;
+                }
+            }
+        }
+    }
+    Vb[2][0] += 0.5 * rab;
+    Vb[3][0] += 0.5 * rab;
+    cLb += 0.5 * cab;
+    cRb += 0.5 * cab;
+}
+
+}  // namespace hm

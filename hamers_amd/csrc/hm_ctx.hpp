@@ -4,10 +4,12 @@
 #include <memory>
 #include <string>
 
+#include "hm_adjoint.hpp"
 #include "hm_reconstructor.hpp"
 
 struct hm_ctx {
     std::unique_ptr<hm::ConvectiveFluxReconstructorGPU> rec;
+    std::unique_ptr<hm::AdjointSingle> adj;  // created on the first VJP call; destroyed before rec
     std::string err;
     int levels = 0;  // live hm_level objects built on this context
 };

@@ -59,6 +59,9 @@ struct Boxes {
     __device__ __forceinline__ int64_t face_vol(int A) const {
         return int64_t(face_ext(A, 0)) * face_ext(A, 1) * face_ext(A, 2);
     }
+    __device__ __forceinline__ int64_t face_idx(int A, int i, int j, int k) const {
+        return i + int64_t(j) * face_ext(A, 0) + int64_t(k) * face_ext(A, 0) * face_ext(A, 1);
+    }
 };
 
 // Q ghost box addressing (caller's ghost width).

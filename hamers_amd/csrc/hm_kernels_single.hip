@@ -13,6 +13,7 @@
 //   k_faceflux  every face: sixth-order midpoint-to-face flux times dt
 //               (WCNS56-HLLC-HLL.cpp:1197-1282, 2327-2490)
 #include "hm_kernels.hpp"
+#include "hm_face_single.hpp"
 #include "hm_numerics.hpp"
 #include "hm_sensor.hpp"
 
@@ -69,7 +70,6 @@ __global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
 template <int D, int A, int SCHEME, int PP, int QQ>
 __device__ __forceinline__ void midflux_single_face(const DevPatch& P, int64_t loc, const KParams& k) {
     constexpr int NEQ = D + 2;
-    constexpr int IP = D + 1;  // pressure slot of V
     const Boxes<D> B(P);
     const int m0 = B.mid_ext(A, 0), m1 = B.mid_ext(A, 1);
     int c0 = int(loc % m0);
@@ -94,113 +94,6 @@ __device__ __forceinline__ void midflux_single_face(const DevPatch& P, int64_t l
     const double cL = pr[NEQ * v4 + iR - st];
     const double cR = pr[NEQ * v4 + iR];
 
-    // Projection variables: simple averages (FlowModelBasicUtilitiesSingleSpecies.cpp:4638-5134).
-    const double ra = double(1) / double(2) * (V[2][0] + V[3][0]);
-    const double ca = double(1) / double(2) * (cL + cR);
-
-    // Characteristic fields (FlowModelBasicUtilitiesSingleSpecies.cpp:6194-6242 and 3D analogues),
-    // reconstructed one at a time.
-    double Wm[NEQ], Wp[NEQ];
-    const LDConst ld{k.p, k.q, k.C, k.alpha_tau};
-    {
-        Six s;
-#pragma unroll
-        for (int m = 0; m < 6; m++)
-            s.v[m] = -double(1) / double(2) * ra * ca * V[m][1 + A] + double(1) / double(2) * V[m][IP];
-        reconstruct_pair<SCHEME, PP, QQ>(s, ld, Wm[0], Wp[0]);
-    }
-    {
-        Six s;
-#pragma unroll
-        for (int m = 0; m < 6; m++) s.v[m] = V[m][0] - double(1) / (ca * ca) * V[m][IP];
-        reconstruct_pair<SCHEME, PP, QQ>(s, ld, Wm[1], Wp[1]);
-    }
-    {
-        int t = 2;
-#pragma unroll
-        for (int b = 0; b < D; b++) {
-            if (b == A) continue;
-            Six s;
-#pragma unroll
-            for (int m = 0; m < 6; m++) s.v[m] = V[m][1 + b];
-            reconstruct_pair<SCHEME, PP, QQ>(s, ld, Wm[t], Wp[t]);
-            t++;
-        }
-    }
-    {
-        Six s;
-#pragma unroll
-        for (int m = 0; m < 6; m++)
-            s.v[m] = double(1) / double(2) * ra * ca * V[m][1 + A] + double(1) / double(2) * V[m][IP];
-        reconstruct_pair<SCHEME, PP, QQ>(s, ld, Wm[IP], Wp[IP]);
-    }
-
-    // Back to primitive variables (FlowModelBasicUtilitiesSingleSpecies.cpp:7279-7284 and analogues).
-    double VL[NEQ], VR[NEQ];
-    auto back = [&](const double* W, double* Vo) {
-        Vo[0] = double(1) / (ca * ca) * W[0] + W[1] + double(1) / (ca * ca) * W[IP];
-        Vo[1 + A] = -double(1) / (ra * ca) * W[0] + double(1) / (ra * ca) * W[IP];
-        int t = 2;
-#pragma unroll
-        for (int b = 0; b < D; b++) {
-            if (b == A) continue;
-            Vo[1 + b] = W[t++];
-        }
-        Vo[IP] = W[0] + W[IP];
-    };
-    back(Wm, VL);
-    back(Wp, VR);
-
-    // Positivity of rho and p on both sides, else first-order values
-    // (FlowModelBasicUtilitiesSingleSpecies.cpp:3205-3222, WCNS56-HLLC-HLL.cpp:947-1032).
-    const bool ok = (VL[0] > double(0)) && (VL[IP] > double(0)) && (VR[0] > double(0)) && (VR[IP] > double(0));
-    if (!ok) {
-#pragma unroll
-        for (int e = 0; e < NEQ; e++) {
-            VL[e] = V[2][e];
-            VR[e] = V[3][e];
-        }
-    }
-
-    // Riemann problem at the midpoint.
-    const double gam = k.gamma[0];
-    const double rhoL = VL[0], rhoR = VR[0];
-    const double pL = VL[IP], pR = VR[IP];
-    const double uL = VL[1 + A], uR = VR[1 + A];
-    const double csL = sqrt(gam * pL / rhoL);                // EquationOfStateIdealGas.cpp:5806
-    const double csR = sqrt(gam * pR / rhoR);
-    const double epsL = pL / ((gam - double(1)) * rhoL);    // EquationOfStateIdealGas.cpp:6135
-    const double epsR = pR / ((gam - double(1)) * rhoR);
-
-    // Wave speeds and contact speed (FlowModelRiemannSolverSingleSpeciesHLLC.cpp:619-631).
-    const double u_avg = double(1) / double(2) * (uL + uR);
-    const double c_avg = double(1) / double(2) * (csL + csR);
-    const double sL = fmin(u_avg - c_avg, uL - csL);
-    const double sR = fmax(u_avg + c_avg, uR + csR);
-    const double s_minus = fmin(double(0), sL);
-    const double s_plus = fmax(double(0), sR);
-    const double s_star =
-        (pR - pL + rhoL * uL * (sL - uL) - rhoR * uR * (sR - uR)) / (rhoL * (sL - uL) - rhoR * (sR - uR));
-
-    // Conservative state and physical flux of one side (:637-648).
-    auto cons_flux = [&](const double* Vs, double eps_int, double* Qs, double* Fs) {
-        Qs[0] = Vs[0];
-#pragma unroll
-        for (int b = 0; b < D; b++) Qs[1 + b] = Vs[0] * Vs[1 + b];
-        double ke;
-        if constexpr (D == 1)
-            ke = Vs[1] * Vs[1];
-        else if constexpr (D == 2)
-            ke = Vs[1] * Vs[1] + Vs[2] * Vs[2];
-        else
-            ke = Vs[1] * Vs[1] + Vs[2] * Vs[2] + Vs[3] * Vs[3];
-        Qs[IP] = Vs[0] * (eps_int + double(1) / double(2) * ke);
-        Fs[0] = Qs[1 + A];
-#pragma unroll
-        for (int b = 0; b < D; b++) Fs[1 + b] = (b == A) ? Qs[1 + A] * Vs[1 + b] + Vs[IP] : Qs[1 + A] * Vs[1 + b];
-        Fs[IP] = Vs[1 + A] * (Qs[IP] + Vs[IP]);
-    };
-
     // Sensor (WCNS56-HLLC-HLL.cpp:1066-1113): only when it exceeds 0.65 is the
     // HLLC-HLL flux used, so HLLC-HLL is evaluated only on those faces.
     bool use_hll = false;
@@ -216,70 +109,7 @@ __device__ __forceinline__ void midflux_single_face(const DevPatch& P, int64_t l
     }
 
     double Fo[NEQ];
-    double QL[NEQ], QR[NEQ], FL[NEQ], FR[NEQ];
-    const bool left = s_star > double(0);
-    if (use_hll) {
-        cons_flux(VL, epsL, QL, FL);
-        cons_flux(VR, epsR, QR, FR);
-    } else if (left) {
-        cons_flux(VL, epsL, QL, FL);
-    } else {
-        cons_flux(VR, epsR, QR, FR);
-    }
-    {
-        // HLLC (:650-688): star state of the upwind side of the contact.
-        const double* Vs = left ? VL : VR;
-        const double* Qs = left ? QL : QR;
-        const double* Fs = left ? FL : FR;
-        const double sK = left ? sL : sR;
-        const double spm = left ? s_minus : s_plus;
-        const double chi = (sK - Vs[1 + A]) / (sK - s_star);
-        double Qst[NEQ];
-        Qst[0] = chi * Vs[0];
-#pragma unroll
-        for (int b = 0; b < D; b++) Qst[1 + b] = (b == A) ? chi * Vs[0] * s_star : chi * Qs[1 + b];
-        Qst[IP] = chi * (Qs[IP] + (s_star - Vs[1 + A]) * (Vs[0] * s_star + Vs[IP] / (sK - Vs[1 + A])));
-#pragma unroll
-        for (int e = 0; e < NEQ; e++) Fo[e] = Fs[e] + spm * (Qst[e] - Qs[e]);
-    }
-    if constexpr (D > 1) {
-        if (use_hll) {
-            // HLL on mass and transverse momentum, blended by the velocity-jump
-            // angle (FlowModelRiemannSolverSingleSpeciesHLLC-HLL.cpp:961-1022).
-            auto hll = [&](int e) {
-                double h = (sR * FL[e] - sL * FR[e] + sR * sL * (QR[e] - QL[e])) / (sR - sL);
-                if (sL > double(0)) h = FL[e];
-                if (sR < double(0)) h = FR[e];
-                return h;
-            };
-            double dsq;
-            {
-                const double du = VR[1] - VL[1];
-                const double dv = VR[2] - VL[2];
-                if constexpr (D == 2) {
-                    dsq = du * du + dv * dv;
-                } else {
-                    const double dw = VR[3] - VL[3];
-                    dsq = du * du + dv * dv + dw * dw;
-                }
-            }
-            const double vel_mag = sqrt(dsq);
-            double al1, al2;
-            if (vel_mag < kEps) {
-                al1 = double(1);
-                al2 = double(0);
-            } else {
-                al1 = fabs(VR[1 + A] - VL[1 + A]) / vel_mag;
-                al2 = sqrt(double(1) - al1 * al1);
-            }
-            const double be1 = double(1) / double(2) * (double(1) + al1 / (al1 + al2));
-            const double be2 = double(1) - be1;
-            Fo[0] = be1 * Fo[0] + be2 * hll(0);
-#pragma unroll
-            for (int b = 0; b < D; b++)
-                if (b != A) Fo[1 + b] = be1 * Fo[1 + b] + be2 * hll(1 + b);
-        }
-    }
+    face_flux_single<D, A, SCHEME, PP, QQ>(V, cL, cR, k.gamma[0], LDConst{k.p, k.q, k.C, k.alpha_tau}, use_hll, Fo);
 
     const int64_t mv = B.mid_vol(A);
     double* out = P.fmid[A];

@@ -22,6 +22,7 @@
 
 #include "hm_ctx.hpp"
 #include "hm_rk.hpp"
+#include "hm_tables.hpp"
 
 namespace hm {
 
@@ -88,49 +89,145 @@ __global__ __launch_bounds__(kBlock) void k_zero_source(SegTableT<LevelPatch> w,
     P->S[stage][loc] = 0.0;
 }
 
-namespace {
 
-size_t up256(size_t b) { return (b + 255) / 256 * 256; }
+// ----------------------------------------------------------------------------
+// Adjoint of the stage loop (discrete adjoint of one level's RK steps).
+// ----------------------------------------------------------------------------
 
-// Segment tables packed into one host image (see the batched RHS plans).
-template <class T>
-struct TableImage {
-    std::vector<T> items;
-    std::vector<int64_t> start, cnt;
-    std::vector<int> seg;
-    int64_t total = 0;
-    void add(int item, int axis, int64_t count) {
-        if (count <= 0) return;
-        start.push_back(total);
-        cnt.push_back(count);
-        seg.push_back(item * 4 + axis);
-        total += (count + kSegAlign - 1) / kSegAlign * kSegAlign;
-    }
-    size_t bytes() const {
-        return up256(sizeof(T) * items.size()) + up256(sizeof(int64_t) * (start.size() + 1)) +
-               up256(sizeof(int64_t) * cnt.size()) + up256(sizeof(int) * seg.size());
-    }
-    // Writes the image at host pointer h, returns the device view for device base d.
-    SegTableT<T> write(char* h, char* d) {
-        std::vector<int64_t> st = start;
-        st.push_back(total);
-        size_t o = 0;
-        SegTableT<T> t{};
-        std::memcpy(h + o, items.data(), sizeof(T) * items.size());
-        t.patches = reinterpret_cast<const T*>(d + o);
-        o += up256(sizeof(T) * items.size());
-        std::memcpy(h + o, st.data(), sizeof(int64_t) * st.size());
-        t.start = reinterpret_cast<const int64_t*>(d + o);
-        o += up256(sizeof(int64_t) * st.size());
-        if (!cnt.empty()) std::memcpy(h + o, cnt.data(), sizeof(int64_t) * cnt.size());
-        t.cnt = reinterpret_cast<const int64_t*>(d + o);
-        o += up256(sizeof(int64_t) * cnt.size());
-        if (!seg.empty()) std::memcpy(h + o, seg.data(), sizeof(int) * seg.size());
-        t.seg = reinterpret_cast<const int*>(d + o);
-        t.nseg = int(cnt.size());
-        return t;
-    }
+struct AdjLevelPatch {
+    int n[3];
+    int64_t gvol;               // ghost-box cells (component stride)
+    double dx[3];
+    double* lam[kMaxRK + 1];    // adjoints of the stage states (interior cells used, ghost-box layout)
+    double* qbar;               // VJP output over the ghost box (neq comps)
+    double* fbar[3];            // face-flux adjoints per axis (neq comps, face layout)
+    int rev_first, rev_count;   // halo items whose source is this patch, in the reverse list
 };
+
+template <int D>
+__device__ __forceinline__ int64_t gbox_idx(const int* n, int g, int i, int j, int k) {
+    const int e0 = n[0] + 2 * g;
+    const int e1 = (D > 1) ? n[1] + 2 * g : 1;
+    return (i + g) + int64_t(j + ((D > 1) ? g : 0)) * e0 + int64_t(k + ((D > 2) ? g : 0)) * e0 * e1;
+}
+
+// Face-flux adjoints from the adjoint of a stage output: the stage update
+// Q_i += beta (-(F_{i+1/2} - F_{i-1/2}) / dx + ...) gives F_bar_f = beta (lam_f - lam_{f-1}) / dx.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_adj_div(SegTableT<AdjLevelPatch> w, int t, double beta, int neq, int g) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const AdjLevelPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    const int f0 = P->n[0] + (ax == 0);
+    const int f1 = (D > 1) ? P->n[1] + (ax == 1) : 1;
+    int c[3];
+    c[0] = int(loc % f0);
+    const int64_t r = loc / f0;
+    c[1] = (D > 1) ? int(r % f1) : 0;
+    c[2] = (D > 2) ? int(r / f1) : 0;
+    const int f = c[ax];
+    const bool hasR = f < P->n[ax], hasL = f - 1 >= 0;
+    int cl[3] = {c[0], c[1], c[2]};
+    cl[ax] -= 1;
+    const int64_t iR = gbox_idx<D>(P->n, g, c[0], c[1], c[2]);
+    const int64_t iL = gbox_idx<D>(P->n, g, cl[0], cl[1], cl[2]);
+    const double* lam = P->lam[t];
+    int64_t fvol = 1;
+    for (int a = 0; a < D; a++) fvol *= P->n[a] + (a == ax);
+    const double sc = beta / P->dx[ax];
+    for (int e = 0; e < neq; e++) {
+        const double lR = hasR ? lam[e * P->gvol + iR] : 0.0;
+        const double lL = hasL ? lam[e * P->gvol + iL] : 0.0;
+        P->fbar[ax][e * fvol + loc] = sc * (lR - lL);
+    }
+}
+
+// lam[m] (interior) += the VJP output of this patch plus the ghost-cell
+// adjoints that the same-level halo copied from this patch's interior
+// (the reverse of k_halo, as a gather: deterministic, no atomics).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_halo_reverse(SegTableT<AdjLevelPatch> w, const HaloItem* rev, int m,
+                                                         int neq, int g) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const AdjLevelPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    const int i = int(loc % P->n[0]);
+    const int64_t r = loc / P->n[0];
+    const int j = (D > 1) ? int(r % P->n[1]) : 0;
+    const int k = (D > 2) ? int(r / P->n[1]) : 0;
+    const int64_t gi = gbox_idx<D>(P->n, g, i, j, k);
+    double acc[HM_MAX_EQN];
+    for (int e = 0; e < neq; e++) acc[e] = P->qbar[e * P->gvol + gi];
+    const int x[3] = {i + g, (D > 1) ? j + g : 0, (D > 2) ? k + g : 0};  // ghost-box coordinates
+    for (int q = P->rev_first; q < P->rev_first + P->rev_count; q++) {
+        const HaloItem& it = rev[q];
+        int y[3];
+        bool in = true;
+        for (int a = 0; a < 3; a++) {
+            y[a] = x[a] - it.off[a];
+            if (y[a] < it.lo[a] || y[a] >= it.lo[a] + it.ext[a]) in = false;
+        }
+        if (!in) continue;
+        const AdjLevelPatch& Pd = w.patches[it.dst];
+        const int e0 = Pd.n[0] + 2 * g;
+        const int e1 = (D > 1) ? Pd.n[1] + 2 * g : 1;
+        const int64_t di = y[0] + int64_t(y[1]) * e0 + int64_t(y[2]) * e0 * e1;
+        for (int e = 0; e < neq; e++) acc[e] += Pd.qbar[e * Pd.gvol + di];
+    }
+    for (int e = 0; e < neq; e++) P->lam[m][e * P->gvol + gi] += acc[e];
+}
+
+// Interior updates of the stage adjoints: op 0: lam[dst] = 0; op 1: lam[dst] += a lam[src];
+// op 2: lam[dst] = lam[src].
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_lam_op(SegTableT<AdjLevelPatch> w, int op, int dst, int src, double a,
+                                                   int neq, int g) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const AdjLevelPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    const int i = int(loc % P->n[0]);
+    const int64_t r = loc / P->n[0];
+    const int j = (D > 1) ? int(r % P->n[1]) : 0;
+    const int k = (D > 2) ? int(r / P->n[1]) : 0;
+    const int64_t gi = gbox_idx<D>(P->n, g, i, j, k);
+    double* L = P->lam[dst];
+    const double* S = P->lam[src];
+    for (int e = 0; e < neq; e++) {
+        const int64_t q = e * P->gvol + gi;
+        if (op == 0)
+            L[q] = 0.0;
+        else if (op == 1)
+            L[q] += a * S[q];
+        else
+            L[q] = S[q];
+    }
+}
+
+// Whole-ghost-box copy between a state buffer and a storage slot
+// (patch p's part of a slot starts at slot + off[p]).  dir 0: buffer -> slot.
+__global__ __launch_bounds__(kBlock) void k_store(SegTableT<LevelPatch> w, int buf, double* slot, const int64_t* off,
+                                                  int nq, int dir) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const LevelPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    double* sp = slot + off[P - w.patches];
+    for (int c = 0; c < nq; c++) {
+        if (dir == 0)
+            sp[c * P->gvol + loc] = P->Q[buf][c * P->gvol + loc];
+        else
+            P->Q[buf][c * P->gvol + loc] = sp[c * P->gvol + loc];
+    }
+}
+
+namespace {
 
 unsigned blocks_for(int64_t items) { return unsigned((items + kBlock - 1) / kBlock); }
 
@@ -145,6 +242,10 @@ class Level {
         (void)hipDeviceSynchronize();
         for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
         for (auto& p : rhs_plan_) rec_->releaseOwnedPlan(p);
+        if (adj_)
+            for (auto& p : vjp_plan_) adj_->release(p);
+        if (adj_mem_) (void)hipFree(adj_mem_);
+        if (store_) (void)hipFree(store_);
         if (rhs_scratch_) (void)hipFree(rhs_scratch_);
         if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
         if (ev_a_) (void)hipEventDestroy(ev_a_);
@@ -273,6 +374,71 @@ class Level {
             return rec_->fail(HM_ERR_DEVICE, "stream ordering failed");
         return HM_OK;
     }
+
+    // ---------------------------------------------------------------------
+    // Discrete adjoint of the stage loop with checkpointing.
+    // ---------------------------------------------------------------------
+
+    // Forward steps that keep a checkpoint of the level state every `every`
+    // steps, for a later adjoint().
+    int advanceRecorded(double dt, int nsteps, int every, bool use_graph, hipStream_t s) {
+        if (nsteps < 1 || every < 1 || !(dt > 0.0)) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "bad dt, step count or interval");
+        if (int st = rec_->selectDevice()) return st;
+        if (int st = ensure_adjoint(nsteps, every)) return st;
+        hipStream_t x = s;
+        hipGraphExec_t exec = nullptr;
+        if (use_graph) {
+            auto itg = graphs_.find(dt);
+            if (itg == graphs_.end()) {
+                if (int st = capture(dt)) return st;
+                itg = graphs_.find(dt);
+            }
+            exec = itg->second;
+            x = cap_stream_;
+            if (hipEventRecord(ev_a_, s) != hipSuccess || hipStreamWaitEvent(cap_stream_, ev_a_, 0) != hipSuccess)
+                return rec_->fail(HM_ERR_DEVICE, "stream ordering failed");
+        }
+        for (int n = 0; n < nsteps; n++) {
+            if (n % every == 0)
+                if (int st = store_op(0, 1 + n / every, 0, x)) return st;
+            if (exec) {
+                if (hipGraphLaunch(exec, x) != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "hipGraphLaunch failed");
+            } else if (int st = step(dt, x)) {
+                return st;
+            }
+        }
+        if (use_graph &&
+            (hipEventRecord(ev_b_, cap_stream_) != hipSuccess || hipStreamWaitEvent(s, ev_b_, 0) != hipSuccess))
+            return rec_->fail(HM_ERR_DEVICE, "stream ordering failed");
+        rec_dt_ = dt;
+        rec_nsteps_ = nsteps;
+        every_ = every;
+        recorded_ = true;
+        return HM_OK;
+    }
+
+    // Adjoint of the last advanceRecorded: the adjoint of the final state
+    // (written with writeAdjoint) becomes the adjoint of the initial state,
+    // segment by segment from the checkpoints.  The level state is restored.
+    int adjoint(hipStream_t s) {
+        if (!recorded_) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "no recorded forward run");
+        if (int st = rec_->selectDevice()) return st;
+        if (int st = store_op(0, 0, 0, s)) return st;  // keep the final state
+        const int nseg = (rec_nsteps_ + every_ - 1) / every_;
+        const int traj0 = 1 + nckpt_;
+        for (int seg = nseg - 1; seg >= 0; seg--) {
+            const int n0 = seg * every_, n1 = std::min(n0 + every_, rec_nsteps_);
+            if (int st = store_op(0, 1 + seg, 1, s)) return st;
+            for (int t = n0; t < n1; t++)
+                if (int st = step(rec_dt_, s, traj0 + (t - n0) * nst_)) return st;
+            for (int t = n1 - 1; t >= n0; t--)
+                if (int st = adjoint_step(traj0 + (t - n0) * nst_, s)) return st;
+        }
+        return store_op(0, 0, 1, s);
+    }
+
+    int writeAdjoint(int p, const double* lam) { return adjoint_io(p, const_cast<double*>(lam), true); }
+    int readAdjoint(int p, double* lam) { return adjoint_io(p, lam, false); }
 
     int numPatches() const { return np_; }
     int ghost() const { return g_; }
@@ -436,6 +602,7 @@ class Level {
             copy.add(p, 0, gvol(p));
             zero.add(p, 0, int64_t(neq_) * ivol(p));
         }
+        halo_host_ = halo.items;
         halo_items_ = halo.total;
         copy_items_ = copy.total;
         zero_items_ = zero.total;
@@ -525,13 +692,17 @@ class Level {
 
     const LevelPatch* patch_dev() const { return copy_.patches; }
 
-    // One Runge-Kutta step (the sequence the graph captures).
-    int step(double dt, hipStream_t s) {
+    // One Runge-Kutta step (the sequence the graph captures).  With
+    // record_slot >= 0 the ghost-filled input of stage sn is also stored in
+    // slot record_slot + sn (recompute pass of the adjoint).
+    int step(double dt, hipStream_t s, int record_slot = -1) {
         if (int st = launch_halo(0, s)) return st;
         for (int sn = 0; sn < nst_; sn++) {
             hipLaunchKernelGGL(k_copy_state, dim3(blocks_for(copy_items_)), dim3(kBlock), 0, s, copy_, 0, 1 + sn, nq_);
             if (sn > 0)
                 if (int st = launch_halo(1 + sn, s)) return st;
+            if (record_slot >= 0)
+                if (int st = store_op(1 + sn, record_slot + sn, 0, s)) return st;
             hipLaunchKernelGGL(k_zero_source, dim3(blocks_for(zero_items_)), dim3(kBlock), 0, s, zero_, sn);
             if (hipGetLastError() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "level launch failed");
             if (int st = rec_->runOwnedPlan(rhs_plan_[sn], dt, s)) return st;
@@ -558,6 +729,208 @@ class Level {
         if (ei != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "hipGraphInstantiate failed");
         if (graphs_.size() >= 4) drop_graphs();
         graphs_[dt] = exec;
+        return HM_OK;
+    }
+
+    // Level state buffer `buf` <-> storage slot; dir 0 stores, 1 restores.
+    int store_op(int buf, int slot, int dir, hipStream_t s) {
+        hipLaunchKernelGGL(k_store, dim3(blocks_for(copy_items_)), dim3(kBlock), 0, s, copy_, buf,
+                           store_ + size_t(slot) * slot_doubles_, slot_off_dev_, nq_, dir);
+        return hipGetLastError() == hipSuccess ? HM_OK : rec_->fail(HM_ERR_DEVICE, "store launch failed");
+    }
+
+    int lam_op(int op, int dst, int src, double a, hipStream_t s) {
+        switch (d_) {
+            case 1: hipLaunchKernelGGL(k_lam_op<1>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, op, dst, src, a, neq_, g_); break;
+            case 2: hipLaunchKernelGGL(k_lam_op<2>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, op, dst, src, a, neq_, g_); break;
+            default: hipLaunchKernelGGL(k_lam_op<3>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, op, dst, src, a, neq_, g_); break;
+        }
+        return hipGetLastError() == hipSuccess ? HM_OK : rec_->fail(HM_ERR_DEVICE, "adjoint launch failed");
+    }
+
+    // Reverse of one step whose stage inputs are stored from slot `slot0`.
+    // lam[nst] holds the adjoint of the step's output on entry and of its
+    // input on return.
+    int adjoint_step(int slot0, hipStream_t s) {
+        for (int m = 0; m < nst_; m++)
+            if (int st = lam_op(0, m, 0, 0.0, s)) return st;
+        for (int sn = nst_ - 1; sn >= 0; sn--) {
+            for (int m = 0; m <= sn; m++) {
+                const double b = beta_[sn][m], a = alpha_[sn][m];
+                if (b != 0.0) {
+                    if (int st = store_op(1 + m, slot0 + m, 1, s)) return st;
+                    switch (d_) {
+                        case 1: hipLaunchKernelGGL(k_adj_div<1>, dim3(blocks_for(adj_face_items_)), dim3(kBlock), 0, s, adj_faces_, sn + 1, b, neq_, g_); break;
+                        case 2: hipLaunchKernelGGL(k_adj_div<2>, dim3(blocks_for(adj_face_items_)), dim3(kBlock), 0, s, adj_faces_, sn + 1, b, neq_, g_); break;
+                        default: hipLaunchKernelGGL(k_adj_div<3>, dim3(blocks_for(adj_face_items_)), dim3(kBlock), 0, s, adj_faces_, sn + 1, b, neq_, g_); break;
+                    }
+                    if (hipGetLastError() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "adjoint launch failed");
+                    if (int st = adj_->run(vjp_plan_[m], rec_dt_, s)) return st;
+                    switch (d_) {
+                        case 1: hipLaunchKernelGGL(k_halo_reverse<1>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, rev_dev_, m, neq_, g_); break;
+                        case 2: hipLaunchKernelGGL(k_halo_reverse<2>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, rev_dev_, m, neq_, g_); break;
+                        default: hipLaunchKernelGGL(k_halo_reverse<3>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, rev_dev_, m, neq_, g_); break;
+                    }
+                    if (hipGetLastError() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "adjoint launch failed");
+                }
+                if (a != 0.0)
+                    if (int st = lam_op(1, m, sn + 1, a, s)) return st;
+            }
+        }
+        return lam_op(2, nst_, 0, 0.0, s);
+    }
+
+    int adjoint_io(int p, double* host, bool write) {
+        if (p < 0 || p >= np_ || !host) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "bad patch index or array");
+        if (int st = ensure_adjoint(0, 0)) return st;
+        std::vector<double> h(size_t(neq_) * gvol(p), 0.0);
+        if (hipDeviceSynchronize() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "device synchronize failed");
+        double* dev = lam_base_[size_t(p) * (kMaxRK + 1) + nst_];
+        const int64_t vol = ivol(p);
+        if (write) {
+            for (int c = 0; c < neq_; c++)
+                for (int64_t l = 0; l < vol; l++) h[size_t(c) * gvol(p) + gidx_of_interior(p, l)] = host[c * vol + l];
+            if (hipMemcpy(dev, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+                return rec_->fail(HM_ERR_DEVICE, "adjoint upload failed");
+        } else {
+            if (hipMemcpy(h.data(), dev, h.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+                return rec_->fail(HM_ERR_DEVICE, "adjoint download failed");
+            for (int c = 0; c < neq_; c++)
+                for (int64_t l = 0; l < vol; l++) host[c * vol + l] = h[size_t(c) * gvol(p) + gidx_of_interior(p, l)];
+        }
+        return HM_OK;
+    }
+
+    // Buffers and tables of the adjoint (built once), and storage slots for
+    // nsteps steps with a checkpoint every `every` steps (grown on demand):
+    // slot 0 final state, slots 1.. checkpoints, then every*nst stage inputs.
+    int ensure_adjoint(int nsteps, int every) {
+        if (rec_->config().model != HM_MODEL_SINGLE_SPECIES)
+            return rec_->fail(HM_ERR_UNSUPPORTED, "the adjoint is available for the single-species model");
+        if (!adj_mem_) {
+            // lam (kMaxRK+1 per patch) and qbar over ghost boxes, fbar over faces
+            size_t tot = 0;
+            std::vector<size_t> off_lam(np_), off_q(np_), off_f(size_t(np_) * 3);
+            for (int p = 0; p < np_; p++) {
+                off_lam[p] = tot;
+                tot += up256(sizeof(double) * size_t(neq_) * gvol(p)) / sizeof(double) * (kMaxRK + 1);
+                off_q[p] = tot;
+                tot += up256(sizeof(double) * size_t(neq_) * gvol(p)) / sizeof(double);
+                for (int a = 0; a < d_; a++) {
+                    off_f[size_t(p) * 3 + a] = tot;
+                    tot += up256(sizeof(double) * size_t(neq_) * fvol(p, a)) / sizeof(double);
+                }
+            }
+            // per-patch offsets within a storage slot
+            std::vector<int64_t> soff(np_);
+            size_t sd = 0;
+            for (int p = 0; p < np_; p++) {
+                soff[p] = int64_t(sd);
+                sd += up256(sizeof(double) * size_t(nq_) * gvol(p)) / sizeof(double);
+            }
+            slot_doubles_ = sd;
+            // tables: AdjLevelPatch faces and cells, reverse halo list, slot offsets
+            std::vector<std::vector<int>> by_src(np_);
+            for (size_t q = 0; q < halo_host_.size(); q++) by_src[halo_host_[q].src].push_back(int(q));
+            std::vector<HaloItem> rev;
+            TableImage<AdjLevelPatch> faces, cells;
+            faces.items.resize(np_);
+            lam_base_.assign(size_t(np_) * (kMaxRK + 1), nullptr);
+            const size_t tables_bytes_guess = 0;
+            (void)tables_bytes_guess;
+            for (int p = 0; p < np_; p++) {
+                AdjLevelPatch& A = faces.items[p];
+                std::memset(&A, 0, sizeof A);
+                for (int a = 0; a < 3; a++) {
+                    A.n[a] = a < d_ ? n_[3 * p + a] : 1;
+                    A.dx[a] = dx_[a];
+                }
+                A.gvol = gvol(p);
+                A.rev_first = int(rev.size());
+                for (int q : by_src[p]) rev.push_back(halo_host_[q]);
+                A.rev_count = int(rev.size()) - A.rev_first;
+                for (int a = 0; a < d_; a++) faces.add(p, a, fvol(p, a));
+                cells.add(p, 0, ivol(p));
+            }
+            cells.items.resize(np_);
+            const size_t tb = faces.bytes() + cells.bytes() + up256(sizeof(HaloItem) * (rev.size() + 1)) +
+                              up256(sizeof(int64_t) * np_);
+            if (hipMalloc(&adj_mem_, tot * sizeof(double) + tb) != hipSuccess) {
+                adj_mem_ = nullptr;
+                return rec_->fail(HM_ERR_OUT_OF_MEMORY, "adjoint buffers");
+            }
+            double* base = static_cast<double*>(adj_mem_);
+            if (hipMemset(adj_mem_, 0, tot * sizeof(double)) != hipSuccess)
+                return rec_->fail(HM_ERR_DEVICE, "adjoint buffer clear");
+            for (int p = 0; p < np_; p++) {
+                AdjLevelPatch& A = faces.items[p];
+                const size_t lstride = up256(sizeof(double) * size_t(neq_) * gvol(p)) / sizeof(double);
+                for (int m = 0; m <= kMaxRK; m++) {
+                    A.lam[m] = base + off_lam[p] + size_t(m) * lstride;
+                    lam_base_[size_t(p) * (kMaxRK + 1) + m] = A.lam[m];
+                }
+                A.qbar = base + off_q[p];
+                for (int a = 0; a < d_; a++) A.fbar[a] = base + off_f[size_t(p) * 3 + a];
+                cells.items[p] = A;
+            }
+            char* tdev = reinterpret_cast<char*>(base + tot);
+            std::vector<char> h(tb, 0);
+            size_t o = 0;
+            adj_faces_ = faces.write(h.data() + o, tdev + o);
+            adj_face_items_ = faces.total;
+            o += faces.bytes();
+            adj_cells_ = cells.write(h.data() + o, tdev + o);
+            adj_cell_items_ = cells.total;
+            o += cells.bytes();
+            if (!rev.empty()) std::memcpy(h.data() + o, rev.data(), sizeof(HaloItem) * rev.size());
+            rev_dev_ = reinterpret_cast<const HaloItem*>(tdev + o);
+            o += up256(sizeof(HaloItem) * (rev.size() + 1));
+            std::memcpy(h.data() + o, soff.data(), sizeof(int64_t) * np_);
+            slot_off_dev_ = reinterpret_cast<const int64_t*>(tdev + o);
+            if (hipMemcpy(tdev, h.data(), tb, hipMemcpyHostToDevice) != hipSuccess)
+                return rec_->fail(HM_ERR_DEVICE, "adjoint table upload");
+            // VJP plans of the stage inputs (state buffers 1..kMaxRK)
+            adj_.reset(new AdjointSingle(rec_));
+            for (int m = 0; m < nst_; m++) {
+                std::vector<hm_vjp_patch_view> vv(np_);
+                for (int p = 0; p < np_; p++) {
+                    hm_vjp_patch_view v{};
+                    for (int a = 0; a < 3; a++) {
+                        v.n[a] = a < d_ ? n_[3 * p + a] : 1;
+                        v.ghost[a] = a < d_ ? g_ : 0;
+                        v.dx[a] = dx_[a];
+                    }
+                    for (int e = 0; e < neq_; e++) {
+                        v.Q[e] = patch_[p].Q[1 + m] + size_t(e) * gvol(p);
+                        v.Q_bar[e] = faces.items[p].qbar + size_t(e) * gvol(p);
+                    }
+                    for (int a = 0; a < d_; a++) v.F_bar[a] = faces.items[p].fbar[a];
+                    vv[p] = v;
+                }
+                if (int st = adj_->build(vv.data(), np_, vjp_plan_[m])) return st;
+            }
+            adj_nst_ = nst_;
+        }
+        if (adj_nst_ != nst_) return rec_->fail(HM_ERR_UNSUPPORTED, "the stage count changed after the adjoint was set up");
+        if (nsteps > 0) {
+            const int nckpt = (nsteps + every - 1) / every;
+            const int nslots = 1 + nckpt + every * nst_;
+            if (nslots > store_slots_) {
+                if (store_) {
+                    (void)hipDeviceSynchronize();
+                    (void)hipFree(store_);
+                    store_ = nullptr;
+                    store_slots_ = 0;
+                }
+                if (hipMalloc(reinterpret_cast<void**>(&store_), size_t(nslots) * slot_doubles_ * sizeof(double)) !=
+                    hipSuccess) {
+                    store_ = nullptr;
+                    return rec_->fail(HM_ERR_OUT_OF_MEMORY, "checkpoint storage");
+                }
+                store_slots_ = nslots;
+            }
+            nckpt_ = nckpt;
+        }
         return HM_OK;
     }
 
@@ -591,6 +964,23 @@ class Level {
     hipStream_t cap_stream_ = nullptr;
     hipEvent_t ev_a_ = nullptr, ev_b_ = nullptr;
     std::map<double, hipGraphExec_t> graphs_;
+    // adjoint
+    std::vector<HaloItem> halo_host_;
+    void* adj_mem_ = nullptr;
+    std::vector<double*> lam_base_;
+    SegTableT<AdjLevelPatch> adj_faces_{}, adj_cells_{};
+    int64_t adj_face_items_ = 0, adj_cell_items_ = 0;
+    const HaloItem* rev_dev_ = nullptr;
+    const int64_t* slot_off_dev_ = nullptr;
+    std::unique_ptr<AdjointSingle> adj_;
+    AdjointSingle::Owned vjp_plan_[kMaxRK];
+    int adj_nst_ = 0;
+    double* store_ = nullptr;
+    size_t slot_doubles_ = 0;
+    int store_slots_ = 0, nckpt_ = 0;
+    bool recorded_ = false;
+    double rec_dt_ = 0.0;
+    int rec_nsteps_ = 0, every_ = 1;
 };
 
 // Stage update for caller-owned patch data (Euler::advanceSingleStepOnPatch).
@@ -794,6 +1184,28 @@ int hm_level_fill_ghosts(hm_level* level, void* stream) {
 int hm_level_advance(hm_level* level, double dt, int num_steps, int use_graph, void* stream) {
     if (!level) return HM_ERR_INVALID_ARGUMENT;
     return level->lvl->advance(dt, num_steps, use_graph != 0, static_cast<hipStream_t>(stream));
+}
+
+int hm_level_advance_recorded(hm_level* level, double dt, int num_steps, int checkpoint_every, int use_graph,
+                              void* stream) {
+    if (!level) return HM_ERR_INVALID_ARGUMENT;
+    return level->lvl->advanceRecorded(dt, num_steps, checkpoint_every, use_graph != 0,
+                                       static_cast<hipStream_t>(stream));
+}
+
+int hm_level_adjoint(hm_level* level, void* stream) {
+    if (!level) return HM_ERR_INVALID_ARGUMENT;
+    return level->lvl->adjoint(static_cast<hipStream_t>(stream));
+}
+
+int hm_level_write_adjoint(hm_level* level, int patch, const double* lam) {
+    if (!level) return HM_ERR_INVALID_ARGUMENT;
+    return level->lvl->writeAdjoint(patch, lam);
+}
+
+int hm_level_read_adjoint(hm_level* level, int patch, double* lam) {
+    if (!level) return HM_ERR_INVALID_ARGUMENT;
+    return level->lvl->readAdjoint(patch, lam);
 }
 
 int hm_advance_single_step_on_patches(hm_ctx* ctx, const hm_rk_patch_view* patches, int num_patches,

@@ -8,7 +8,14 @@
 // branch as the reference on identical inputs.
 #pragma once
 
+// Host+device so that host-side checks can compile the same pointwise code.
+#if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
+#define HM_HD __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#define HM_HD inline
+#endif
 
 namespace hm {
 
@@ -18,7 +25,7 @@ constexpr double kEps = 1.0e-15;  // HAMERS_EPSILON, include/HAMeRS_config.hpp.i
 // multiplication, so base^4 is ((b*b)*b)*b.  E < 0 means "exponent known only
 // at run time".
 template <int E>
-__device__ __forceinline__ double rep_pow(double b, int e_rt) {
+HM_HD double rep_pow(double b, int e_rt) {
     if constexpr (E > 0) {
         double r = b;
 #pragma unroll
@@ -37,11 +44,11 @@ struct Six {
     double v[6];
 };
 
-__device__ __forceinline__ Six flip(const Six& s) { return Six{{s.v[5], s.v[4], s.v[3], s.v[2], s.v[1], s.v[0]}}; }
+HM_HD Six flip(const Six& s) { return Six{{s.v[5], s.v[4], s.v[3], s.v[2], s.v[1], s.v[0]}}; }
 
 // Smoothness indicators of the three 3-point sub-stencils
 // (WCNS6-LD-HLLC-HLL.cpp:55-68, identical in WCNS5-JS-HLLC-HLL.cpp:32-44).
-__device__ __forceinline__ void beta3(const Six& s, double& b0, double& b1, double& b2) {
+HM_HD void beta3(const Six& s, double& b0, double& b1, double& b2) {
     b0 = double(1) / double(3) *
          (s.v[0] * (double(4) * s.v[0] - double(19) * s.v[1] + double(11) * s.v[2]) +
           s.v[1] * (double(25) * s.v[1] - double(31) * s.v[2]) + double(10) * s.v[2] * s.v[2]);
@@ -54,7 +61,7 @@ __device__ __forceinline__ void beta3(const Six& s, double& b0, double& b1, doub
 }
 
 // Six-point smoothness indicator (WCNS6-LD-HLLC-HLL.cpp:70-83).
-__device__ __forceinline__ double beta6(const Six& s) {
+HM_HD double beta6(const Six& s) {
     return double(1) / double(232243200) *
            (s.v[0] * (double(525910327) * s.v[0] - double(4562164630) * s.v[1] + double(7799501420) * s.v[2] -
                     double(6610694540) * s.v[3] + double(2794296070) * s.v[4] - double(472758974) * s.v[5]) +
@@ -71,7 +78,7 @@ __device__ __forceinline__ double beta6(const Six& s) {
 
 // Shock measure sigma (WCNS6-LD-HLLC-HLL.cpp:24-41); both sides use the
 // unflipped stencil.
-__device__ __forceinline__ double ld_sigma(const Six& s) {
+HM_HD double ld_sigma(const Six& s) {
     const double a1 = s.v[2] - s.v[1];
     const double a2 = s.v[3] - s.v[2];
     const double a3 = s.v[4] - s.v[3];
@@ -88,7 +95,7 @@ struct LDConst {
 // One side of the WCNS6-LD nonlinear midpoint interpolation
 // (WCNS6-LD-HLLC-HLL.cpp:133-223; the plus side 229-319 is this on flip(s)).
 template <int P, int Q>
-__device__ __forceinline__ double ld_side(const Six& s, double sigma, const LDConst& k) {
+HM_HD double ld_side(const Six& s, double sigma, const LDConst& k) {
     double b0, b1, b2;
     beta3(s, b0, b1, b2);
     const double b3 = beta6(s);
@@ -128,7 +135,7 @@ __device__ __forceinline__ double ld_side(const Six& s, double sigma, const LDCo
 
 // One side of WCNS5-JS (WCNS5-JS-HLLC-HLL.cpp:78-118; plus side 124-164 on flip(s)).
 template <int P>
-__device__ __forceinline__ double js_side(const Six& s, int p_rt) {
+HM_HD double js_side(const Six& s, int p_rt) {
     double b0, b1, b2;
     beta3(s, b0, b1, b2);
     double w0 = double(1.0) / double(16) / rep_pow<P>((b0 + kEps), p_rt);
@@ -147,7 +154,7 @@ __device__ __forceinline__ double js_side(const Six& s, int p_rt) {
 enum { kWCNS5_JS = 0, kWCNS6_LD = 1 };
 
 template <int SCHEME, int P, int Q>
-__device__ __forceinline__ void reconstruct_pair(const Six& s, const LDConst& k, double& um, double& up) {
+HM_HD void reconstruct_pair(const Six& s, const LDConst& k, double& um, double& up) {
     if constexpr (SCHEME == kWCNS6_LD) {
         const double sig = ld_sigma(s);
         um = ld_side<P, Q>(s, sig, k);

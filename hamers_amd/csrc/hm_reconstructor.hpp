@@ -199,11 +199,20 @@ class ConvectiveFluxReconstructorGPU {
     }
 
     // All five stages of an owned plan; no profiling events (graph capture safe).
-    int runOwnedPlan(const OwnedPlan& p, double dt, hipStream_t s) {
+    int runOwnedPlan(const OwnedPlan& p, double dt, hipStream_t s) { return runOwnedPlanStages(p, dt, 0, kStages, s); }
+
+    // Pipeline stages [first, last) of an owned plan.
+    int runOwnedPlanStages(const OwnedPlan& p, double dt, int first, int last, hipStream_t s) {
         KParams k = k_;
         k.dt = dt;
-        return run_stages(p.plan, k, s, 0, kStages);
+        return run_stages(p.plan, k, s, first, last);
     }
+
+    // Device descriptor of a patch whose scratch starts at `base` (the layout
+    // of scratchDoubles / buildOwnedPlan, patch after patch).
+    DevPatch devPatch(const hm_patch_view& v, double* base) const { return make_dev_patch(v, base, layout(v)); }
+    size_t patchScratchDoubles(const hm_patch_view& v) const { return layout(v).total; }
+    const KParams& kernelParams() const { return k_; }
 
     // Device buffer for per-call tables of the batched stage update.  The
     // previous user's launches are waited for before the buffer is handed out

@@ -206,3 +206,43 @@ class Level:
     def advance(self, dt: float, num_steps: int = 1, use_graph: bool = False, stream=None):
         capi.check(self.rec._ctx, capi.lib.hm_level_advance(self._lvl, float(dt), int(num_steps), int(use_graph),
                                                             self.rec._stream_ptr(stream)))
+
+    # -- discrete adjoint ---------------------------------------------------
+
+    def advance_recorded(self, dt: float, num_steps: int, checkpoint_every: int, use_graph: bool = False,
+                         stream=None):
+        """Advance and keep a checkpoint every ``checkpoint_every`` steps for adjoint()."""
+        capi.check(self.rec._ctx, capi.lib.hm_level_advance_recorded(
+            self._lvl, float(dt), int(num_steps), int(checkpoint_every), int(use_graph), self.rec._stream_ptr(stream)))
+
+    def adjoint(self, stream=None):
+        """Turn the adjoint of the final state into that of the initial state."""
+        capi.check(self.rec._ctx, capi.lib.hm_level_adjoint(self._lvl, self.rec._stream_ptr(stream)))
+
+    def write_adjoint(self, i: int, lam: np.ndarray):
+        bn = self.boxes[i][1]
+        shape = (self.rec.num_eqn,) + tuple(reversed(bn))
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        if lam.shape != shape:
+            raise ValueError(f"patch {i}: expected shape {shape}, got {lam.shape}")
+        capi.check(self.rec._ctx, capi.lib.hm_level_write_adjoint(
+            self._lvl, i, lam.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def read_adjoint(self, i: int) -> np.ndarray:
+        bn = self.boxes[i][1]
+        lam = np.empty((self.rec.num_eqn,) + tuple(reversed(bn)))
+        capi.check(self.rec._ctx, capi.lib.hm_level_read_adjoint(
+            self._lvl, i, lam.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return lam
+
+    def write_adjoint_global(self, lam: np.ndarray):
+        for i, (lo, n) in enumerate(self.boxes):
+            sl = tuple(slice(lo[a], lo[a] + n[a]) for a in reversed(range(self.dim)))
+            self.write_adjoint(i, lam[(slice(None),) + sl])
+
+    def read_adjoint_global(self, shape) -> np.ndarray:
+        lam = np.full(shape, np.nan)
+        for i, (lo, n) in enumerate(self.boxes):
+            sl = tuple(slice(lo[a], lo[a] + n[a]) for a in reversed(range(self.dim)))
+            lam[(slice(None),) + sl] = self.read_adjoint(i)
+        return lam

@@ -178,6 +178,39 @@ class ConvectiveFluxReconstructor:
         capi.check(self._ctx, st)
 
 
+def _vjp_view(pd: PatchData, F_bar, Q_bar, num_arrays: int, neq: int) -> capi.hm_vjp_patch_view:
+    v = capi.hm_vjp_patch_view()
+    d = len(pd.n)
+    for a in range(3):
+        v.n[a] = int(pd.n[a]) if a < d else 1
+        v.ghost[a] = int(pd.ghost[a]) if a < d else 0
+        v.dx[a] = float(pd.dx[a]) if a < d else 1.0
+    qs = list(pd.Q) if isinstance(pd.Q, (list, tuple)) else [pd.Q[e] for e in range(pd.Q.shape[0])]
+    for e in range(num_arrays):
+        _check_tensor(qs[e])
+        v.Q[e] = qs[e].data_ptr()
+    for a in range(d):
+        _check_tensor(F_bar[a])
+        v.F_bar[a] = F_bar[a].data_ptr()
+    qb = list(Q_bar) if isinstance(Q_bar, (list, tuple)) else [Q_bar[e] for e in range(Q_bar.shape[0])]
+    for e in range(neq):
+        _check_tensor(qb[e])
+        v.Q_bar[e] = qb[e].data_ptr()
+    return v
+
+
+def convective_flux_vjp(rec: ConvectiveFluxReconstructor, patches, F_bar, Q_bar, dt: float, stream=None):
+    """Vector-Jacobian product of computeConvectiveFluxAndSourceOnPatch(es):
+    for each patch, Q_bar[i] <- (dF/dQ)^T F_bar[i] over the ghost box of Q.
+    ``F_bar[i]`` is the list of per-axis face-flux adjoints (layout of F)."""
+    views = (capi.hm_vjp_patch_view * len(patches))()
+    for i, pd in enumerate(patches):
+        views[i] = _vjp_view(pd, F_bar[i], Q_bar[i], rec.num_arrays, rec.num_eqn)
+    st = capi.lib.hm_convective_flux_vjp_on_patches(rec._ctx, views, len(patches), 0.0, float(dt), 0,
+                                                    rec._stream_ptr(stream))
+    capi.check(rec._ctx, st)
+
+
 class ConvectiveFluxReconstructorManager:
     """Builds the reconstructor named by the deck key, as
     ConvectiveFluxReconstructorManager.cpp:14-153 does; unknown keys raise."""

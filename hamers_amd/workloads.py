@@ -101,8 +101,9 @@ def kelvin_helmholtz(N: int = 512, amplitudes=None, seed: int = 7, gamma: float 
     u = U0 tanh((y-1/4)/delta) - U0 tanh((y-3/4)/delta) - U0 (periodic pair),
     v = sum_k a_k sin(2 pi k x + phi_k) * envelope(y), rho = 1 (+ 1 inside), p = 2.5."""
     rng = np.random.default_rng(seed)
+    default_amplitudes = 1.0e-2 * rng.uniform(0.5, 1.0, size=8)  # drawn always: the phases follow
     if amplitudes is None:
-        amplitudes = 1.0e-2 * rng.uniform(0.5, 1.0, size=8)
+        amplitudes = default_amplitudes
     phases = rng.uniform(0.0, 2.0 * np.pi, size=8)
     n = (N, N)
     dx = (1.0 / N, 1.0 / N)
@@ -118,6 +119,71 @@ def kelvin_helmholtz(N: int = 512, amplitudes=None, seed: int = 7, gamma: float 
         v = v + amplitudes[k] * np.sin(2.0 * np.pi * (k + 1) * X + phases[k]) * env
     p = 2.5 * np.ones_like(X)
     return conservative_from_primitive_single(rho, [u, v], p, gamma), n, dx, np.asarray(amplitudes), phases
+
+
+def kelvin_helmholtz_mode_derivatives(N: int, Q0: np.ndarray, seed: int = 7):
+    """dQ0/da_k of kelvin_helmholtz: v = sum_k a_k s_k(x, y) enters rho v and
+    E = p/(gamma-1) + rho (u^2 + v^2)/2, so d(rho v)/da_k = rho s_k and
+    dE/da_k = rho v s_k.  Returns an array (8, 4, N, N)."""
+    rng = np.random.default_rng(seed)
+    rng.uniform(0.5, 1.0, size=8)  # the default amplitudes drawn first
+    phases = rng.uniform(0.0, 2.0 * np.pi, size=8)
+    n = (N, N)
+    dx = (1.0 / N, 1.0 / N)
+    X, Y = cell_centres(n, (0.0, 0.0), dx)
+    env = np.exp(-((Y - 0.25) / 0.05) ** 2) + np.exp(-((Y - 0.75) / 0.05) ** 2)
+    rho = Q0[0]
+    v = Q0[2] / rho
+    out = np.zeros((8,) + Q0.shape)
+    for k in range(8):
+        sk = np.sin(2.0 * np.pi * (k + 1) * X + phases[k]) * env
+        out[k, 2] = rho * sk
+        out[k, 3] = rho * v * sk
+    return out
+
+
+def _smooth_mode_shapes(N: int, seed: int):
+    """Phases and the 8 transverse-velocity mode shapes of ``smooth_modes``."""
+    rng = np.random.default_rng(seed)
+    default_amplitudes = 0.05 * rng.uniform(0.5, 1.0, size=8)  # drawn always: the phases follow
+    phases = rng.uniform(0.0, 2.0 * np.pi, size=8)
+    X, Y = cell_centres((N, N), (0.0, 0.0), (1.0 / N, 1.0 / N))
+    shapes = [np.sin(2.0 * np.pi * ((k % 4 + 1) * X + (k // 4 + 1) * Y) + phases[k]) for k in range(8)]
+    return default_amplitudes, phases, X, Y, shapes
+
+
+def smooth_modes(N: int = 64, amplitudes=None, seed: int = 11, gamma: float = 1.4):
+    """Smooth periodic flow on [0,1]^2 in which every primitive variable varies
+    along both axes at O(0.1) amplitude, plus 8 transverse-velocity modes
+    sin(2 pi (kx x + ky y) + phi_k) whose amplitudes are the parameters.
+    No region is near-uniform, so the scheme's smoothness indicators stay far
+    above rounding level and the final state is a smooth function of the
+    amplitudes (a gradient check by differences resolves ~1e-9)."""
+    default_amplitudes, phases, X, Y, shapes = _smooth_mode_shapes(N, seed)
+    if amplitudes is None:
+        amplitudes = default_amplitudes
+    tp = 2.0 * np.pi
+    rho = 1.0 + 0.2 * np.sin(tp * (X + 2.0 * Y))
+    u = 0.3 + 0.2 * np.sin(tp * (2.0 * X + Y) + 1.0)
+    v = 0.2 + 0.1 * np.cos(tp * (X - Y) + 2.0)
+    for k in range(8):
+        v = v + amplitudes[k] * shapes[k]
+    p = 1.0 + 0.1 * np.cos(tp * (3.0 * X + Y))
+    return (conservative_from_primitive_single(rho, [u, v], p, gamma), (N, N), (1.0 / N, 1.0 / N),
+            np.asarray(amplitudes), phases)
+
+
+def smooth_modes_derivatives(N: int, Q0: np.ndarray, seed: int = 11):
+    """dQ0/da_k of ``smooth_modes`` (v is linear in a; rho v and E = p/(gamma-1)
+    + rho |u|^2/2 follow).  Returns an array (8, 4, N, N)."""
+    _, _, _, _, shapes = _smooth_mode_shapes(N, seed)
+    rho = Q0[0]
+    v = Q0[2] / rho
+    out = np.zeros((8,) + Q0.shape)
+    for k in range(8):
+        out[k, 2] = rho * shapes[k]
+        out[k, 3] = rho * v * shapes[k]
+    return out
 
 
 def split_patches(Q_global_interior: np.ndarray, patch: tuple, g: int):

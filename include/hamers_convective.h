@@ -150,6 +150,39 @@ int hm_profile_read(hm_ctx* ctx, double* stage_ms /* [HM_NUM_STAGES] */,
 int hm_synchronize(hm_ctx* ctx, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Discrete adjoint (no counterpart in the reference: a new capability)      */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * One patch of a vector-Jacobian product of the convective RHS: the forward
+ * input Q (as in hm_patch_view), the adjoints F_bar of the face fluxes (the
+ * layout of F) and S_bar of the source, and the output Q_bar = (dF/dQ)^T F_bar
+ * (+ (dS/dQ)^T S_bar) over the ghost box of Q, overwritten (zero outside the
+ * 4-cell stencil halo).
+ */
+typedef struct hm_vjp_patch_view {
+    int n[3];
+    int ghost[3];
+    double dx[3];
+    const double* Q[HM_MAX_EQN];
+    const double* F_bar[3];
+    const double* S_bar;
+    double* Q_bar[HM_MAX_EQN];
+} hm_vjp_patch_view;
+
+/*
+ * Discrete adjoint of hm_compute_convective_flux_and_source_on_patch(es):
+ * the transposed Jacobian of the forward map (branches of the scheme -- WENO
+ * weight selection, positivity fallback, sensor and Riemann wave selection --
+ * fixed at their forward values), for the single-species model
+ * (HM_ERR_UNSUPPORTED otherwise).  Deterministic (no atomics).
+ */
+int hm_convective_flux_vjp_on_patch(hm_ctx* ctx, const hm_vjp_patch_view* patch, double time, double dt,
+                                    int rk_step, void* stream);
+int hm_convective_flux_vjp_on_patches(hm_ctx* ctx, const hm_vjp_patch_view* patches, int num_patches,
+                                      double time, double dt, int rk_step, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Runge-Kutta stage update                                                  */
 /* ------------------------------------------------------------------------ */
 
@@ -232,6 +265,24 @@ int hm_level_fill_ghosts(hm_level* level, void* stream);
 /* num_steps steps of size dt.  use_graph != 0 replays one step captured in a
  * hipGraph (captured once per dt) instead of issuing its launches. */
 int hm_level_advance(hm_level* level, double dt, int num_steps, int use_graph, void* stream);
+
+/*
+ * Discrete adjoint of a level's Runge-Kutta steps (no counterpart in the
+ * reference; single-species model).  hm_level_advance_recorded advances like
+ * hm_level_advance and keeps a copy of the level state every checkpoint_every
+ * steps.  The caller then writes the adjoint of the final state (interior
+ * cells, num_eqn components; e.g. dJ/dQ of an objective J) with
+ * hm_level_write_adjoint and calls hm_level_adjoint, which recomputes each
+ * checkpoint segment forward and runs the reverse stage loop (the transposed
+ * stage update, hm_convective_flux_vjp_on_patches and the transposed ghost
+ * fill); hm_level_read_adjoint then returns dJ/dQ of the initial state.  The
+ * level state is left as hm_level_advance_recorded left it.
+ */
+int hm_level_advance_recorded(hm_level* level, double dt, int num_steps, int checkpoint_every, int use_graph,
+                              void* stream);
+int hm_level_write_adjoint(hm_level* level, int patch, const double* lam);
+int hm_level_adjoint(hm_level* level, void* stream);
+int hm_level_read_adjoint(hm_level* level, int patch, double* lam);
 
 #ifdef __cplusplus
 }

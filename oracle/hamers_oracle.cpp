@@ -63,6 +63,57 @@ namespace {
 constexpr double EPS = HMO_EPS;
 
 // ---------------------------------------------------------------------------
+// Branch tape (single-species path, test instrumentation).  The scheme is only
+// piecewise smooth: the Ducros sensor's HLLC/HLLC-HLL choice, WCNS6-LD's
+// R_tau > alpha_tau switch and the bounds fallback jump at thresholds (the
+// HLLC-HLL |du| < eps switch too, at rounding-level jumps only), and the
+// HLLC side s* > 0 and the fmin/fmax wave speeds have kinks (exact ties are
+// common where a field is uniform along a sweep).  A discrete adjoint
+// differentiates the smooth piece selected at the evaluation point; central
+// differences that cross a switch do not converge to it.  Mode 1 records every
+// decision in evaluation order, mode 2 replays the record, so that differences
+// of a replayed run are differences of that one smooth piece.  Mode 0 is the
+// plain scheme (bit-identical to the code without the tape).  The tape is a
+// single global sequence: use it from one thread only.
+// ---------------------------------------------------------------------------
+struct BranchTape {
+    int mode = 0;
+    std::vector<int8_t> v;
+    size_t pos = 0;
+    bool overrun = false;
+};
+BranchTape g_tape;
+
+inline int tape_sel(int natural) {
+    if (g_tape.mode == 1) {
+        g_tape.v.push_back(int8_t(natural));
+    } else if (g_tape.mode == 2) {
+        if (g_tape.pos >= g_tape.v.size()) {
+            g_tape.overrun = true;
+            return natural;
+        }
+        return g_tape.v[g_tape.pos++];
+    }
+    return natural;
+}
+
+// |x| is continuous and stays untaped: central differences of |x| at x = 0
+// give 0, the derivative the adjoint takes there, while a frozen sign would let
+// WCNS6-LD's sigma denominator |a1| + |a2| + eps pass through zero.
+inline double t_abs(double x) { return std::fabs(x); }
+// fmin(a, b) / fmax(a, b); a tie selects a, as the adjoint does.
+inline double t_min(double a, double b) {
+    if (g_tape.mode == 0) return std::fmin(a, b);
+    return tape_sel(a <= b ? 1 : 0) ? a : b;
+}
+inline double t_max(double a, double b) {
+    if (g_tape.mode == 0) return std::fmax(a, b);
+    return tape_sel(a >= b ? 1 : 0) ? a : b;
+}
+// A plain threshold decision.
+inline bool t_if(bool c) { return g_tape.mode == 0 ? c : tape_sel(c ? 1 : 0) != 0; }
+
+// ---------------------------------------------------------------------------
 // Pointwise reconstruction (WCNS6-LD: ...WCNS6-LD-HLLC-HLL.cpp:9-319,
 // WCNS5-JS: ...WCNS5-JS-HLLC-HLL.cpp:9-164).  U[0..5] are the six stencil
 // values of one characteristic field for one face.
@@ -118,7 +169,7 @@ inline double wcns6ld_side(const Stencil6& s, double sig, int p, int q, double C
     smoothness3(s, b0, b1, b2);
     const double b3 = smoothness6(s);
 
-    const double tau5 = std::fabs(b0 - b2);
+    const double tau5 = t_abs(b0 - b2);
     double wu0 = double(1) / double(16) * (double(1) + ipow_ref(tau5 / (b0 + EPS), p));
     double wu1 = double(5) / double(8) * (double(1) + ipow_ref(tau5 / (b1 + EPS), p));
     double wu2 = double(5) / double(16) * (double(1) + ipow_ref(tau5 / (b2 + EPS), p));
@@ -128,7 +179,7 @@ inline double wcns6ld_side(const Stencil6& s, double sig, int p, int q, double C
     wu2 = wu2 / wus;
 
     const double bavg = double(1) / double(8) * (b0 + b2 + double(6) * b1);
-    const double tau6 = std::fabs(b3 - bavg);
+    const double tau6 = t_abs(b3 - bavg);
     double w0 = double(1) / double(32) * (C + ipow_ref(tau6 / (b0 + EPS), q));
     double w1 = double(15) / double(32) * (C + ipow_ref(tau6 / (b1 + EPS), q));
     double w2 = double(15) / double(32) * (C + ipow_ref(tau6 / (b2 + EPS), q));
@@ -140,7 +191,7 @@ inline double wcns6ld_side(const Stencil6& s, double sig, int p, int q, double C
     w3 = w3 / ws;
 
     const double Rtau = tau6 / (bavg + EPS);
-    if (Rtau > alpha_tau) {
+    if (t_if(Rtau > alpha_tau)) {
         w0 = sig * wu0 + (double(1) - sig) * w0;
         w1 = sig * wu1 + (double(1) - sig) * w1;
         w2 = sig * wu2 + (double(1) - sig) * w2;
@@ -162,9 +213,9 @@ inline double wcns6ld_sigma(const Stencil6& s) {  // WCNS6-LD:24-41
     const double a1 = s.u2 - s.u1;
     const double a2 = s.u3 - s.u2;
     const double a3 = s.u4 - s.u3;
-    const double t1 = std::fabs(a1 - a2) / (std::fabs(a1) + std::fabs(a2) + EPS);
-    const double t2 = std::fabs(a2 - a3) / (std::fabs(a2) + std::fabs(a3) + EPS);
-    return std::fmax(t1, t2);
+    const double t1 = t_abs(a1 - a2) / (t_abs(a1) + t_abs(a2) + EPS);
+    const double t2 = t_abs(a2 - a3) / (t_abs(a2) + t_abs(a3) + EPS);
+    return t_max(t1, t2);
 }
 
 // WCNS5-JS one side (WCNS5-JS:78-118); the plus side is the mirrored stencil.
@@ -423,10 +474,10 @@ void riemann_face_single(int d, int a, double gam, bool use_hll, const double* V
 
     const double u_avg = double(1) / double(2) * (uL + uR);
     const double c_av = double(1) / double(2) * (cL + cR);
-    const double sL = std::fmin(u_avg - c_av, uL - cL);
-    const double sR = std::fmax(u_avg + c_av, uR + cR);
-    const double sm = std::fmin(double(0), sL);
-    const double sp = std::fmax(double(0), sR);
+    const double sL = t_min(u_avg - c_av, uL - cL);
+    const double sR = t_max(u_avg + c_av, uR + cR);
+    const double sm = (g_tape.mode == 0) ? std::fmin(double(0), sL) : (t_if(sL < double(0)) ? sL : double(0));
+    const double sp = (g_tape.mode == 0) ? std::fmax(double(0), sR) : (t_if(sR > double(0)) ? sR : double(0));
     const double sstar =
         (pR - pL + rhoL * uL * (sL - uL) - rhoR * uR * (sR - uR)) / (rhoL * (sL - uL) - rhoR * (sR - uR));
 
@@ -460,7 +511,7 @@ void riemann_face_single(int d, int a, double gam, bool use_hll, const double* V
     flux(VR, QR, FR);
 
     double Fhllc[5];
-    const bool left = (sstar > double(0));
+    const bool left = t_if(sstar > double(0));
     {
         const double* V = left ? VL : VR;
         const double* Qs = left ? QL : QR;
@@ -487,8 +538,8 @@ void riemann_face_single(int d, int a, double gam, bool use_hll, const double* V
         for (int e = 0; e < neq; e++) Fhll[e] = 0.0;
         auto hll = [&](int e) {
             double r = (sR * FL[e] - sL * FR[e] + sR * sL * (QR[e] - QL[e])) / (sR - sL);
-            if (sL > double(0)) r = FL[e];
-            if (sR < double(0)) r = FR[e];
+            if (t_if(sL > double(0))) r = FL[e];
+            if (t_if(sR < double(0))) r = FR[e];
             return r;
         };
         Fhll[0] = hll(0);
@@ -507,11 +558,11 @@ void riemann_face_single(int d, int a, double gam, bool use_hll, const double* V
         }
         const double vel_mag = std::sqrt(dsq);
         double al1, al2;
-        if (vel_mag < EPS) {
+        if (vel_mag < EPS) {  // untaped: a replayed "not tiny" with vel_mag == 0 would divide 0 by 0
             al1 = double(1);
             al2 = double(0);
         } else {
-            al1 = std::fabs(VR[1 + a] - VL[1 + a]) / vel_mag;
+            al1 = t_abs(VR[1 + a] - VL[1 + a]) / vel_mag;
             al2 = std::sqrt(double(1) - al1 * al1);
         }
         const double be1 = double(1) / double(2) * (double(1) + al1 / (al1 + al2));
@@ -640,7 +691,7 @@ void direction_pass_single(const hmo_params& prm, const Model& m, const int* n, 
         fp &= (Vp[0].v[f] > double(0)) ? 1 : 0;
         fp &= (Vp[neq - 1].v[f] > double(0)) ? 1 : 0;
         if (prm.mutation == 3) fm = fp = 1;  // planted mistake: bounds check dropped
-        if (fm == 0 || fp == 0) {
+        if (t_if(fm == 0 || fp == 0)) {
             const int64_t iL = D.V.idx(i - sh[0], j - sh[1], k - sh[2]);
             const int64_t iR = D.V.idx(i, j, k);
             for (int e = 0; e < neq; e++) {
@@ -668,7 +719,7 @@ void direction_pass_single(const hmo_params& prm, const Model& m, const int* n, 
             const double th_avg = 0.5 * (theta.c(0)[tL] + theta.c(0)[tR]);
             const double om_avg = 0.5 * (Omega.c(0)[tL] + Omega.c(0)[tR]);
             const double sensor = -th_avg / (std::fabs(th_avg) + om_avg + EPS);
-            use_hll = sensor > 0.65;
+            use_hll = t_if(sensor > 0.65);
         }
         double Fo[5];
         riemann_face_single(d, a, gam, use_hll, VL, VR, Fo, nullptr);
@@ -1347,9 +1398,9 @@ int hmo_rhs_batch(const hmo_params* prm, int npatch, const hmo_patch* patches, d
     int status = 0;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
-#pragma omp parallel for schedule(dynamic, 1) reduction(max : status)
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : status) if (g_tape.mode == 0)
 #endif
-    for (int ip = 0; ip < npatch; ip++) {
+    for (int ip = 0; ip < npatch; ip++) {  // serial while the branch tape is on
         const hmo_patch& P = patches[ip];
         double* F[3] = {P.F[0], P.F[1], P.F[2]};
         const int st = rhs_patch_impl(prm, P.n, P.g, P.dx, dt, P.Q, F, P.S);
@@ -1400,6 +1451,24 @@ int hmo_riemann_multi(const hmo_params* prm, int axis, int use_hll, const double
     if (!prm || (prm->model != 1 && prm->model != 2) || !make_multi(prm, f)) return 1;
     riemann_face_five(f, axis, use_hll != 0, VL, VR, F, ustar);
     return 0;
+}
+
+// Branch tape control (single-species path): mode 1 records (clearing the
+// tape), mode 2 replays from its start, mode 0 switches it off.  Returns the
+// tape length.
+int64_t hmo_branch_tape(int mode) {
+    if (mode == 1) g_tape.v.clear();
+    g_tape.mode = mode;
+    g_tape.pos = 0;
+    g_tape.overrun = false;
+    return int64_t(g_tape.v.size());
+}
+
+// Replay position and whether a replay ran past the end of the tape.
+void hmo_branch_tape_status(int64_t* length, int64_t* position, int* overrun) {
+    *length = int64_t(g_tape.v.size());
+    *position = int64_t(g_tape.pos);
+    *overrun = g_tape.overrun ? 1 : 0;
 }
 
 int hmo_max_threads(void) {

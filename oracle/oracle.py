@@ -253,3 +253,44 @@ def riemann_single(dim: int, axis: int, gamma: float, use_hll: bool, VL, VR):
     L.hmo_riemann_single(dim, axis, gamma, int(use_hll), VL.ctypes.data, VR.ctypes.data, F.ctypes.data,
                          ctypes.byref(us))
     return F, us.value
+
+
+def branch_tape(mode: int) -> int:
+    """Branch tape of the single-species path: 1 records every branch decision
+    (clearing the tape), 2 replays the recorded decisions from the start, 0
+    switches the tape off.  Returns the tape length."""
+    L = lib()
+    L.hmo_branch_tape.restype = ctypes.c_int64
+    L.hmo_branch_tape.argtypes = [ctypes.c_int]
+    return int(L.hmo_branch_tape(int(mode)))
+
+
+def branch_tape_status() -> tuple:
+    """(length, replay position, overrun) of the branch tape."""
+    L = lib()
+    L.hmo_branch_tape_status.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    n, pos, ov = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+    L.hmo_branch_tape_status(ctypes.byref(n), ctypes.byref(pos), ctypes.byref(ov))
+    return n.value, pos.value, bool(ov.value)
+
+
+class frozen_branches:
+    """Context manager: record the branch decisions of the first oracle run
+    inside it, replay them for every later run (``replay()`` rewinds)."""
+
+    def __enter__(self):
+        branch_tape(1)
+        return self
+
+    def replay(self):
+        n, pos, ov = branch_tape_status()
+        branch_tape(2)
+        return n
+
+    def consumed(self) -> bool:
+        n, pos, ov = branch_tape_status()
+        return pos == n and not ov
+
+    def __exit__(self, *exc):
+        branch_tape(0)
+        return False

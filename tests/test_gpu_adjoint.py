@@ -1,0 +1,261 @@
+"""GPU discrete adjoint of the single-species convective RHS and of the
+level's SSP-RK3 steps.
+
+- Patch VJP: <Q_bar, dQ> against central differences of the GPU forward
+  <F_bar, (F(Q + h dQ) - F(Q - h dQ)) / 2h> (dot-product test), relative
+  error <= 1e-5 on smooth states (the flux is piecewise smooth: wave-speed
+  fmin/fmax near-ties make quotients wobble near 1e-6); with shocks, against
+  the oracle's differences with branch decisions replayed, <= 1e-4.
+- Level adjoint with checkpoint/recompute: gradient of the transverse kinetic
+  energy w.r.t. 8 initial mode amplitudes against central differences of the
+  CPU oracle's RK3 with branch decisions replayed from the base run (the
+  reference-derived gradient of the smooth piece the adjoint differentiates):
+  relative max-norm error <= 1e-6 on a smooth multi-mode flow (WCNS6-LD eager,
+  WCNS5-JS with graph-captured steps), <= 2e-3 on the Kelvin-Helmholtz shear
+  layer (whose objective carries rounding noise, see that test); independent
+  of the checkpoint interval (bit for bit); the thin torch.autograd.Function
+  harness returns the same gradient.
+"""
+import contextlib
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from hamers_amd import workloads as W
+from tests.util import oracle_rk3
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a visible GPU"
+    return torch
+
+
+def _rec(dim, scheme):
+    from hamers_amd import ConvectiveFluxReconstructor
+
+    return ConvectiveFluxReconstructor("adj", dim, "SINGLE_SPECIES", scheme, species_gamma=(1.4,))
+
+
+def _dot_test(torch, dim, scheme, Q, n, dx, seed, tag, tol, trials=6, h=1e-6, consistency=1e-5, oracle_frozen=False):
+    """Several random directions dQ; a direction whose difference quotients at
+    h and h/2 disagree crosses a branch of the scheme (weight selection,
+    fallback, wave-speed choice) and is skipped; all others must agree.
+    With ``oracle_frozen`` the differences are the oracle's with its branch
+    decisions replayed from the unperturbed state (no direction is skipped)."""
+    from hamers_amd import PatchData
+    from hamers_amd.reconstructor import convective_flux_vjp
+
+    rng = np.random.default_rng(seed)
+    rec = _rec(dim, scheme)
+    g = 4
+    dev = "cuda:0"
+    neq = dim + 2
+    fsh = [O.face_shape(neq, n, a) for a in range(dim)]
+    Fb_h = [rng.normal(size=fsh[a]) for a in range(dim)]
+    Fb = [torch.from_numpy(f).to(dev) for f in Fb_h]
+    dt = 1e-3
+
+    prm = O.Params(dim=dim, scheme=1 if scheme.startswith("WCNS6") else 0)
+    fb = O.frozen_branches()
+
+    def forward(Qh):
+        if oracle_frozen:
+            fb.replay()
+            F, _ = O.rhs_patch(prm, np.ascontiguousarray(Qh), n, g, dx, dt)
+            assert fb.consumed()
+            return sum(float(np.sum(F[a] * Fb_h[a])) for a in range(dim))
+        Qd = torch.from_numpy(np.ascontiguousarray(Qh)).to(dev)
+        F = [torch.empty(fsh[a], dtype=torch.float64, device=dev) for a in range(dim)]
+        pd = PatchData(tuple(n), (g,) * dim, tuple(dx), Qd, F, None)
+        rec.computeConvectiveFluxAndSourceOnPatch(pd, 0.0, dt)
+        torch.cuda.synchronize()
+        return sum(float(torch.sum(F[a] * Fb[a])) for a in range(dim))
+
+    Qd = torch.from_numpy(np.ascontiguousarray(Q)).to(dev)
+    pd = PatchData(tuple(n), (g,) * dim, tuple(dx), Qd, [torch.empty(1, dtype=torch.float64, device=dev)] * dim, None)
+    Qb = torch.full(Q.shape, np.nan, dtype=torch.float64, device=dev)
+    _say(f"case {tag}")
+    convective_flux_vjp(rec, [pd], [Fb], [Qb], dt)
+    torch.cuda.synchronize()
+    qb = Qb.cpu().numpy()
+    assert np.all(np.isfinite(qb)), f"{tag}: non-finite Q_bar"
+    compared = 0
+    worst = 0.0
+    with (fb if oracle_frozen else contextlib.nullcontext()):
+        if oracle_frozen:
+            O.rhs_patch(prm, np.ascontiguousarray(Q), n, g, dx, dt)  # records the branch decisions
+        for t in range(trials):
+            dQ = rng.normal(size=Q.shape) * (0.1 + np.abs(Q))
+            fd1 = (forward(Q + h * dQ) - forward(Q - h * dQ)) / (2 * h)
+            fd2 = (forward(Q + 0.5 * h * dQ) - forward(Q - 0.5 * h * dQ)) / h
+            ad = float(np.sum(qb * dQ))
+            # The flux is only piecewise smooth: on smooth flow the two wave-speed
+            # estimates of fmin/fmax nearly tie at every face, so quotients wobble
+            # near 1e-6; a direction whose quotients differ by more crosses a branch.
+            consistent = abs(fd1 - fd2) <= consistency * abs(fd1)
+            err = abs(ad - fd2) / max(abs(fd2), 1e-300)
+            _say(f"{tag} direction {t}: adjoint {ad:.12e} differences {fd2:.12e} (h: {fd1:.12e}) "
+                 f"{'rel err %.2e' % err if consistent else 'crosses a branch, skipped'}")
+            if consistent:
+                compared += 1
+                worst = max(worst, err)
+    if oracle_frozen:
+        assert compared == trials, f"{tag}: replayed differences did not converge"
+    assert compared >= trials // 2, f"{tag}: only {compared} usable directions"
+    assert worst <= tol, f"{tag}: {worst:.2e}"
+
+
+def _smooth(N, dim):
+    """Smooth periodic state on [0,1]^d in which every primitive variable
+    varies along every axis.  (A field constant along a sweep has vanishing
+    smoothness indicators, where WCNS6-LD's scale-free blending switch makes
+    the flux non-differentiable: difference quotients there depend on the
+    perturbation, so such states cannot check a derivative.)"""
+    n = (N,) * dim
+    dx = (1.0 / N,) * dim
+    X = W.cell_centres(n, (0.0,) * dim, dx)
+    ph = lambda k: 2 * np.pi * sum((k + a + 1) * X[a] for a in range(dim))  # noqa: E731
+    rho = 1.0 + 0.2 * np.sin(ph(0))
+    vel = [0.3 + 0.2 * np.sin(ph(1 + b) + b) for b in range(dim)]
+    p = 1.0 + 0.1 * np.cos(ph(5))
+    return W.conservative_from_primitive_single(rho, vel, p, 1.4), n, dx
+
+
+@pytest.mark.parametrize("scheme", ["WCNS6_LD_HLLC_HLL", "WCNS5_JS_HLLC_HLL"])
+def test_vjp_dot_product_2d_smooth(torch_dev, scheme):
+    Qi, n, dx = _smooth(24, 2)
+    Q = W.fill_periodic(Qi, 4)
+    _dot_test(torch_dev, 2, scheme, Q, n, dx, 11, f"VJP 2D smooth {scheme}", 1e-5)
+
+
+def test_vjp_dot_product_2d_shocks(torch_dev):
+    Qi, n, dx = W.four_quadrant_riemann(32, seed=3, noise=1e-2)
+    Q = W.fill_periodic(Qi, 4)
+    # Near shocks branches are dense: the differences are the oracle's with its
+    # branch decisions replayed, which converge at second order (the forward
+    # is bit-identical to the GPU's).
+    _dot_test(torch_dev, 2, "WCNS6_LD_HLLC_HLL", Q, n, dx, 12, "VJP 2D four-quadrant WCNS6-LD", 1e-4, h=1e-7,
+              consistency=1e-4, oracle_frozen=True)
+
+
+def test_vjp_dot_product_1d(torch_dev):
+    Qi, n, dx = _smooth(32, 1)
+    Q = W.fill_periodic(Qi, 4)
+    _dot_test(torch_dev, 1, "WCNS6_LD_HLLC_HLL", Q, n, dx, 13, "VJP 1D", 1e-5)
+
+
+def test_vjp_dot_product_3d(torch_dev):
+    Qi, n, dx = _smooth(10, 3)
+    Q = W.fill_periodic(Qi, 4)
+    _dot_test(torch_dev, 3, "WCNS6_LD_HLLC_HLL", Q, n, dx, 14, "VJP 3D", 1e-5)
+
+
+def _kh(N=64, P=32, nsteps=12, every=5, scheme="WCNS6_LD_HLLC_HLL", use_graph=False):
+    from hamers_amd.adjoint import KHProblem
+
+    return KHProblem(N=N, P=P, nsteps=nsteps, checkpoint_every=every, scheme=scheme, use_graph=use_graph)
+
+
+def _modes(N=64, P=32, nsteps=12, every=5, scheme="WCNS6_LD_HLLC_HLL", use_graph=False):
+    from hamers_amd.adjoint import ModalLevelProblem
+
+    return ModalLevelProblem("smooth_modes", N=N, P=P, nsteps=nsteps, checkpoint_every=every, scheme=scheme,
+                             use_graph=use_graph)
+
+
+def _oracle_J(prob, a):
+    from hamers_amd.adjoint import kinetic_energy_transverse
+
+    scheme = 1 if prob.rec.scheme.startswith("WCNS6") else 0
+    Q0 = prob.initial_state(a)
+    QT = oracle_rk3(O.Params(dim=2, scheme=scheme), Q0, (prob.N, prob.N), prob.dx, prob.dt, prob.nsteps)
+    return kinetic_energy_transverse(QT, prob.dx)
+
+
+def _oracle_frozen_differences(prob, a, rel_h):
+    """Central differences of the oracle's J with every branch decision
+    replayed from the run at ``a`` (the smooth piece the adjoint differentiates)."""
+    fd = np.zeros(len(a))
+    with O.frozen_branches() as fb:
+        J0 = _oracle_J(prob, a)
+        for k in range(len(a)):
+            h = rel_h * a[k]
+            ap, am = a.copy(), a.copy()
+            ap[k] += h
+            am[k] -= h
+            fb.replay()
+            jp = _oracle_J(prob, ap)
+            assert fb.consumed()
+            fb.replay()
+            jm = _oracle_J(prob, am)
+            assert fb.consumed()
+            fd[k] = (jp - jm) / (2 * h)
+    return J0, fd
+
+
+@pytest.mark.parametrize("scheme,use_graph,nsteps,every", [("WCNS6_LD_HLLC_HLL", False, 12, 5),
+                                                           ("WCNS5_JS_HLLC_HLL", True, 8, 3)])
+def test_level_gradient_against_oracle_differences(torch_dev, scheme, use_graph, nsteps, every):
+    prob = _modes(scheme=scheme, use_graph=use_graph, nsteps=nsteps, every=every)
+    a = prob.a0
+    _say(f"case level adjoint gradient, smooth modes 64^2, 4 patches, {scheme}, {nsteps} steps, "
+         f"checkpoint every {every}, graph {use_graph}")
+    J, g = prob.gradient(a)
+    J_oracle, fd = _oracle_frozen_differences(prob, a, 1e-6)
+    assert J == J_oracle  # the level's forward is bit-identical to the oracle's
+    err = float(np.max(np.abs(g - fd)) / np.max(np.abs(fd)))
+    _say(f"J {J:.12e}; adjoint gradient {g}; oracle differences {fd}; rel err {err:.2e}")
+    assert err <= 1e-6
+
+
+def test_level_gradient_kelvin_helmholtz(torch_dev):
+    """KH's objective is only resolvable to ~1e-4 by differences: the shear
+    layer's tanh and Gaussian tails leave large regions varying by 1e-12..1e-7,
+    where the smoothness indicators (an expanded quadratic form on O(1) values)
+    are rounding-dominated at the eps = 1e-15 scale of the weights, so J(a)
+    carries rounding noise of ~3e-7 relative.  A 3e-3 step with replayed
+    branches averages that out to ~3e-4."""
+    prob = _kh()
+    a = prob.a0
+    _say("case level adjoint gradient, Kelvin-Helmholtz 64^2, 4 patches, 12 steps, checkpoint every 5")
+    J, g = prob.gradient(a)
+    J_oracle, fd = _oracle_frozen_differences(prob, a, 3e-3)
+    assert J == J_oracle
+    err = float(np.max(np.abs(g - fd)) / np.max(np.abs(fd)))
+    _say(f"J {J:.12e}; adjoint gradient {g}; oracle differences {fd}; rel err {err:.2e}")
+    assert err <= 2e-3
+
+
+def test_level_gradient_independent_of_checkpoint_interval(torch_dev):
+    grads = []
+    for every in (1, 4, 12):
+        prob = _kh(every=every, nsteps=12)
+        _say(f"case checkpoint every {every}")
+        grads.append(prob.gradient(prob.a0)[1])
+    assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
+
+
+def test_autograd_harness(torch_dev):
+    torch = torch_dev
+    from hamers_amd.adjoint import kh_objective_function
+
+    prob = _kh(nsteps=6, every=4)
+    KH = kh_objective_function()
+    a = torch.tensor(prob.a0, dtype=torch.float64, requires_grad=True)
+    J = KH.apply(a, prob)
+    J.backward()
+    _, g = prob.gradient(prob.a0)
+    assert np.array_equal(a.grad.numpy(), g)
+    assert float(J) == prob.objective(prob.a0)[0]
