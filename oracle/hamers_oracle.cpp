@@ -56,6 +56,18 @@ typedef struct hmo_params {
     int mutation;      // test-only: plants a known mistake (0 = none)
 } hmo_params;
 
+// Constant transport properties per species (the "constant" mixing rules of
+// equations_of_{shear_viscosity,bulk_viscosity,thermal_conductivity,
+// mass_diffusivity}/constant/*MixingRulesConstant.cpp): mu_i, mu_v,i, kappa_i,
+// D_i and the molecular weights M_i of the 1/sqrt(M_i) mixing weights.
+typedef struct hmo_transport {
+    double mu[8];
+    double mu_v[8];
+    double kappa[8];
+    double D[8];
+    double M[8];
+} hmo_transport;
+
 }  // extern "C"
 
 namespace {
@@ -769,6 +781,7 @@ struct Five {
     int mutation;
     double gam[8];
     double cp[8], cv[8];  // species heat capacities (four-equation mixing)
+    double R[8];          // species gas constants
     int iu() const { return ns; }          // first velocity slot of V
     int ip() const { return ns + d; }      // pressure slot of V
     int iz() const { return ns + d + 1; }  // first volume-fraction slot of V
@@ -1323,6 +1336,7 @@ bool make_multi(const hmo_params* prm, Five& f) {
     if (f.ns < (f.four ? 1 : 2) || f.ns > 8) return false;
     for (int i = 0; i < 8; i++) {
         f.gam[i] = prm->gamma[i];
+        f.R[i] = prm->R[i];
         // EquationOfStateMixingRulesIdealGas.cpp:108-120
         f.cp[i] = (i < f.ns) ? prm->gamma[i] / (prm->gamma[i] - double(1)) * prm->R[i] : 0.0;
         f.cv[i] = (i < f.ns) ? double(1) / (prm->gamma[i] - double(1)) * prm->R[i] : 0.0;
@@ -1341,6 +1355,260 @@ int rhs_patch_five(const hmo_params* prm, const int* n, int g, const double* dx,
     FaceArray ustar[3];
     for (int a = 0; a < f.d; a++) direction_pass_five(*prm, f, n, a, D, theta, Omega, dt, F[a], ustar[a]);
     if (S && !f.four) source_five(f, n, g, dx, dt, Q, D, ustar, S);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Diffusive flux of the four-equation model with the node-based SIXTH_ORDER
+// reconstructor (DiffusiveFluxReconstructorNode.cpp:32-1738,
+// DiffusiveFluxReconstructorNodeSixthOrder.cpp:65-939), the variables and
+// diffusivities of FlowModelDiffusiveFluxUtilitiesFourEqnConservative.cpp
+// (764-1821, 1824-2953, 5673-6425) and constant transport mixing rules.
+// ---------------------------------------------------------------------------
+
+constexpr int kDiffGhosts = 6;  // DiffusiveFluxReconstructorNodeSixthOrder.cpp:22
+
+struct DiffCells {  // derived cell data over the 6-ghost box
+    CellArray rho, Y, u, T, h, mu, mu_v, kappa;
+};
+
+// Derived data of the four-equation model (density, mass fractions,
+// velocity, pressure as in compute_derived_five), temperature
+// T = p / ((gamma - 1) c_v rho) (EquationOfStateIdealGas.cpp:6951),
+// species densities rho_i = p / (R_i T) (:8357) and enthalpies
+// h_i = gamma_i p / ((gamma_i - 1) rho_i) (:6464), and the mixture shear and
+// bulk viscosities and conductivity sum(x_i w_i) / sum(w_i), w_i = Y_i / sqrt(M_i)
+// (EquationOfShearViscosityMixingRulesConstant.cpp:497-528 and the bulk /
+// conductivity analogues).
+void compute_diffusive_cells(const Five& f, const hmo_transport& tr, const int* n, int g, const double* Q,
+                             DiffCells& C) {
+    const int d = f.d, ns = f.ns;
+    CellArray Qv;
+    Qv.init(n, g, d, 0);
+    const int64_t qvol = Qv.vol();
+    const int G = kDiffGhosts;
+    C.rho.init(n, G, d, 1);
+    C.Y.init(n, G, d, ns);
+    C.u.init(n, G, d, d);
+    C.T.init(n, G, d, 1);
+    C.h.init(n, G, d, ns);
+    C.mu.init(n, G, d, 1);
+    C.mu_v.init(n, G, d, 1);
+    C.kappa.init(n, G, d, 1);
+    double wfac[8];
+    for (int si = 0; si < ns; si++) wfac[si] = double(1) / (std::sqrt(tr.M[si]));
+    const int lo2 = (d > 2) ? -G : 0, hi2 = (d > 2) ? n[2] + G : 1;
+    const int lo1 = (d > 1) ? -G : 0, hi1 = (d > 1) ? n[1] + G : 1;
+    for (int k = lo2; k < hi2; k++)
+        for (int j = lo1; j < hi1; j++)
+            for (int i = -G; i < n[0] + G; i++) {
+                const int64_t iq = Qv.idx(i, j, k), ic = C.rho.idx(i, j, k);
+                double rho = double(0);
+                for (int si = 0; si < ns; si++) rho += Q[si * qvol + iq];
+                double Y[8];
+                for (int si = 0; si < ns; si++) Y[si] = Q[si * qvol + iq] / rho;
+                double u[3] = {0.0, 0.0, 0.0};
+                for (int b = 0; b < d; b++) u[b] = Q[(ns + b) * qvol + iq] / rho;
+                double ke;
+                if (d == 1)
+                    ke = u[0] * u[0];
+                else if (d == 2)
+                    ke = u[0] * u[0] + u[1] * u[1];
+                else
+                    ke = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+                const double E = Q[(ns + d) * qvol + iq];
+                const double eps_int = E / rho - double(1) / double(2) * ke;
+                double gmix, c_v;
+                mix_mass(f, Y, gmix, c_v);
+                const double p = (gmix - double(1)) * rho * eps_int;
+                const double T = p / ((gmix - double(1)) * c_v * rho);
+                double mu_n = double(0), mu_d = double(0), mv_n = double(0), mv_d = double(0);
+                double ka_n = double(0), ka_d = double(0);
+                for (int si = 0; si < ns; si++) {
+                    const double rho_i = p / (f.R[si] * T);
+                    C.h.c(si)[ic] = f.gam[si] * p / ((f.gam[si] - double(1)) * rho_i);
+                    const double w = Y[si] * wfac[si];
+                    mu_n += tr.mu[si] * w;
+                    mu_d += w;
+                    mv_n += tr.mu_v[si] * w;
+                    mv_d += w;
+                    ka_n += tr.kappa[si] * w;
+                    ka_d += w;
+                }
+                C.rho.c(0)[ic] = rho;
+                for (int si = 0; si < ns; si++) C.Y.c(si)[ic] = Y[si];
+                for (int b = 0; b < d; b++) C.u.c(b)[ic] = u[b];
+                C.T.c(0)[ic] = T;
+                C.mu.c(0)[ic] = mu_n / mu_d;
+                C.mu_v.c(0)[ic] = mv_n / mv_d;
+                C.kappa.c(0)[ic] = ka_n / ka_d;
+            }
+}
+
+// Diffusivity kinds of FlowModelDiffusiveFluxUtilitiesFourEqnConservative.cpp:5673-6425.
+enum DiffKind {
+    DK_MASS_SELF,   // -rho D_si
+    DK_MASS_CROSS,  // rho Y_si D_sj
+    DK_A0,          // -(4/3 mu + mu_v)
+    DK_A1,          // 2/3 mu - mu_v
+    DK_A2,          // -mu
+    DK_U43,         // -u_c (4/3 mu + mu_v)
+    DK_U23,         // u_c (2/3 mu - mu_v)
+    DK_UMU,         // -u_c mu
+    DK_KAP,         // -kappa
+    DK_ENT_SELF,    // -rho D_si h_si
+    DK_ENT_CROSS    // rho Y_si D_sj h_si
+};
+enum DiffVar { DV_Y, DV_U, DV_T };
+
+struct DiffTerm {
+    int var, comp;       // variable whose derivative is taken (mass fraction, velocity component, T)
+    int kind, c, si, sj; // diffusivity
+};
+
+// (variable, diffusivity) terms of equation e of the axis-a flux whose
+// derivative is along axis b, in the reference's order
+// (getCellDataOfDiffusiveFluxVariablesForDerivative:764-1821 and
+// getCellDataOfDiffusiveFluxDiffusivities:1824-2953).
+int diffusive_terms(const Five& f, int a, int b, int e, DiffTerm* out) {
+    const int ns = f.ns, d = f.d;
+    int t = 0;
+    if (e < ns) {
+        if (b == a) {
+            out[t++] = {DV_Y, e, DK_MASS_SELF, 0, e, 0};
+            for (int sj = 0; sj < ns; sj++) out[t++] = {DV_Y, sj, DK_MASS_CROSS, 0, e, sj};
+        }
+    } else if (e < ns + d) {
+        const int c = e - ns;
+        if (b == a) {
+            if (c == a)
+                out[t++] = {DV_U, a, DK_A0, 0, 0, 0};
+            else
+                out[t++] = {DV_U, c, DK_A2, 0, 0, 0};
+        } else if (c == a) {
+            out[t++] = {DV_U, b, DK_A1, 0, 0, 0};
+        } else if (c == b) {
+            out[t++] = {DV_U, a, DK_A2, 0, 0, 0};
+        }
+    } else {
+        if (b == a) {
+            for (int c = 0; c < d; c++) out[t++] = {DV_U, c, c == a ? DK_U43 : DK_UMU, c, 0, 0};
+            out[t++] = {DV_T, 0, DK_KAP, 0, 0, 0};
+            for (int si = 0; si < ns; si++) {
+                out[t++] = {DV_Y, si, DK_ENT_SELF, 0, si, 0};
+                for (int sj = 0; sj < ns; sj++) out[t++] = {DV_Y, sj, DK_ENT_CROSS, 0, si, sj};
+            }
+        } else if (a < b) {
+            out[t++] = {DV_U, a, DK_UMU, b, 0, 0};
+            out[t++] = {DV_U, b, DK_U23, a, 0, 0};
+        } else {
+            out[t++] = {DV_U, b, DK_U23, a, 0, 0};
+            out[t++] = {DV_U, a, DK_UMU, b, 0, 0};
+        }
+    }
+    return t;
+}
+
+inline double diffusivity(const DiffTerm& t, const DiffCells& C, const hmo_transport& tr, int64_t ic, int mut) {
+    const double mu = C.mu.c(0)[ic], mu_v = C.mu_v.c(0)[ic];
+    switch (t.kind) {
+        case DK_MASS_SELF: return -C.rho.c(0)[ic] * tr.D[t.si];
+        case DK_MASS_CROSS:
+            if (mut == 5) return double(0);  // planted mistake: the mass-flux correction dropped
+            return C.rho.c(0)[ic] * C.Y.c(t.si)[ic] * tr.D[t.sj];
+        case DK_A0: return -(double(4) / double(3) * mu + mu_v);
+        case DK_A1:
+            if (mut == 6) return -(double(2) / double(3) * mu - mu_v);  // planted mistake: sign of the dilatation term
+            return double(2) / double(3) * mu - mu_v;
+        case DK_A2: return -mu;
+        case DK_U43: return -C.u.c(t.c)[ic] * (double(4) / double(3) * mu + mu_v);
+        case DK_U23: return C.u.c(t.c)[ic] * (double(2) / double(3) * mu - mu_v);
+        case DK_UMU: return -C.u.c(t.c)[ic] * mu;
+        case DK_KAP: return -C.kappa.c(0)[ic];
+        case DK_ENT_SELF: return -C.rho.c(0)[ic] * tr.D[t.si] * C.h.c(t.si)[ic];
+        default: return C.rho.c(0)[ic] * C.Y.c(t.si)[ic] * tr.D[t.sj] * C.h.c(t.si)[ic];
+    }
+}
+
+// Sixth-order central first derivative along axis b at cell (i, j, k)
+// (DiffusiveFluxReconstructorNodeSixthOrder.cpp:77-112).
+inline double first_derivative6(const CellArray& A, int comp, int b, int i, int j, int k, double dx_inv) {
+    const double a_n = double(3) / double(4);
+    const double b_n = -double(3) / double(20);
+    const double c_n = double(1) / double(60);
+    const int s[3] = {b == 0, b == 1, b == 2};
+    const double* u = A.c(comp);
+    auto at = [&](int o) { return u[A.idx(i + o * s[0], j + o * s[1], k + o * s[2])]; };
+    return (a_n * (at(1) - at(-1)) + b_n * (at(2) - at(-2)) + c_n * (at(3) - at(-3))) * dx_inv;
+}
+
+// computeDiffusiveFluxOnPatch for the four-equation model: node fluxes
+// F_node[e] = sum_b sum_terms D * d(var)/dx_b on nodes [-3, n_a + 3) along a,
+// then F_face = dt (a_r (F_L + F_R) + b_r (F_LL + F_RR) + c_r (F_LLL + F_RRR))
+// (DiffusiveFluxReconstructorNodeSixthOrder.cpp:507-690).
+int diffusive_patch_four(const hmo_params* prm, const hmo_transport* tr, const int* n, int g, const double* dx,
+                         double dt, const double* Q, double* const* F) {
+    Five f;
+    if (!make_multi(prm, f) || !f.four) return 5;
+    if (g < kDiffGhosts) return 3;
+    DiffCells C;
+    compute_diffusive_cells(f, *tr, n, g, Q, C);
+    const int d = f.d, neq = f.neq;
+    double dx_inv[3];
+    for (int b = 0; b < d; b++) dx_inv[b] = double(1) / dx[b];
+    const double a_n = double(3) / double(4);
+    const double b_n = -double(3) / double(20);
+    const double c_n = double(1) / double(60);
+    const double a_r = a_n + b_n + c_n;
+    const double b_r = b_n + c_n;
+    const double c_r = c_n;
+    DiffTerm terms[96];
+    for (int a = 0; a < d; a++) {
+        const int sh[3] = {a == 0, a == 1, a == 2};
+        // node box: [-3, n_a + 3) along a, interior across
+        int lo[3] = {0, 0, 0}, hi[3] = {n[0], n[1], n[2]};
+        lo[a] = -3;
+        hi[a] = n[a] + 3;
+        int next[3];
+        for (int b = 0; b < 3; b++) next[b] = hi[b] - lo[b];
+        const int64_t nvol = int64_t(next[0]) * next[1] * next[2];
+        std::vector<double> Fn(size_t(neq) * nvol, 0.0);
+        auto nidx = [&](int i, int j, int k) {
+            return int64_t(i - lo[0]) + int64_t(j - lo[1]) * next[0] + int64_t(k - lo[2]) * next[0] * next[1];
+        };
+        for (int e = 0; e < neq; e++)
+            for (int b = 0; b < d; b++) {
+                const int nt = diffusive_terms(f, a, b, e, terms);
+                for (int ti = 0; ti < nt; ti++) {
+                    const DiffTerm& t = terms[ti];
+                    const CellArray& A = (t.var == DV_Y) ? C.Y : (t.var == DV_U) ? C.u : C.T;
+                    for (int k = lo[2]; k < hi[2]; k++)
+                        for (int j = lo[1]; j < hi[1]; j++)
+                            for (int i = lo[0]; i < hi[0]; i++) {
+                                const int64_t ic = C.rho.idx(i, j, k);
+                                const double D = diffusivity(t, C, *tr, ic, prm->mutation);
+                                const double der = first_derivative6(A, t.comp, b, i, j, k, dx_inv[b]);
+                                Fn[e * nvol + nidx(i, j, k)] += D * der;
+                            }
+                }
+            }
+        int fext[3];
+        for (int b = 0; b < 3; b++) fext[b] = (b == a) ? n[b] + 1 : n[b];
+        const int64_t fvol = int64_t(fext[0]) * fext[1] * fext[2];
+        for (int e = 0; e < neq; e++) {
+            const double* Fe = Fn.data() + e * nvol;
+            double* out = F[a] + e * fvol;
+            for (int k = 0; k < fext[2]; k++)
+                for (int j = 0; j < fext[1]; j++)
+                    for (int i = 0; i < fext[0]; i++) {
+                        auto node = [&](int o) { return Fe[nidx(i + o * sh[0], j + o * sh[1], k + o * sh[2])]; };
+                        const int64_t of = i + int64_t(j) * fext[0] + int64_t(k) * fext[0] * fext[1];
+                        out[of] = double(0);
+                        out[of] += dt * (a_r * (node(-1) + node(0)) + b_r * (node(-2) + node(1)) +
+                                         c_r * (node(-3) + node(2)));
+                    }
+        }
+    }
     return 0;
 }
 
@@ -1451,6 +1719,22 @@ int hmo_riemann_multi(const hmo_params* prm, int axis, int use_hll, const double
     if (!prm || (prm->model != 1 && prm->model != 2) || !make_multi(prm, f)) return 1;
     riemann_face_five(f, axis, use_hll != 0, VL, VR, F, ustar);
     return 0;
+}
+
+// DiffusiveFluxReconstructorNode::computeDiffusiveFluxOnPatch
+// (DiffusiveFluxReconstructorNode.cpp:32-1738, SIXTH_ORDER) for one patch of
+// the four-equation model: F0/F1/F2 (diffusive flux SideData, 0 ghosts) are
+// overwritten with dt-scaled face fluxes.  Q needs a ghost width >= 6.
+int hmo_diffusive_patch(const hmo_params* prm, const hmo_transport* tr, const int* n_in, int g, const double* dx,
+                        double dt, const double* Q, double* F0, double* F1, double* F2) {
+    if (!prm || !tr || !n_in || !dx || !Q || !F0) return 1;
+    int n[3] = {1, 1, 1};
+    for (int a = 0; a < prm->dim; a++) {
+        n[a] = n_in[a];
+        if (n[a] < 1) return 4;
+    }
+    double* F[3] = {F0, F1, F2};
+    return diffusive_patch_four(prm, tr, n, g, dx, dt, Q, F);
 }
 
 // Branch tape control (single-species path): mode 1 records (clearing the

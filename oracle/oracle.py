@@ -294,3 +294,53 @@ class frozen_branches:
     def __exit__(self, *exc):
         branch_tape(0)
         return False
+
+
+class _Transport(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double * 8) for k in ("mu", "mu_v", "kappa", "D", "M")]
+
+
+@dataclass
+class Transport:
+    """Constant per-species transport properties (shear and bulk viscosity,
+    thermal conductivity, mass diffusivity) and molecular weights."""
+    mu: list = field(default_factory=lambda: [1e-3, 2e-3])
+    mu_v: list = field(default_factory=lambda: [0.0, 0.0])
+    kappa: list = field(default_factory=lambda: [1e-3, 1.5e-3])
+    D: list = field(default_factory=lambda: [1e-3, 2e-3])
+    M: list = field(default_factory=lambda: [1.0, 4.0])
+
+    def to_c(self) -> _Transport:
+        c = _Transport()
+        for k in ("mu", "mu_v", "kappa", "D", "M"):
+            vals = getattr(self, k)
+            arr = getattr(c, k)
+            for i in range(8):
+                arr[i] = vals[i] if i < len(vals) else (1.0 if k == "M" else 0.0)
+        return c
+
+
+def diffusive_patch(prm: Params, tr: Transport, Q: np.ndarray, n: tuple, g: int, dx: tuple, dt: float):
+    """computeDiffusiveFluxOnPatch (node SIXTH_ORDER) of the four-equation
+    model on one patch; ``Q`` has ghost width ``g`` >= 6.  Returns the per-axis
+    face flux arrays (dt-scaled)."""
+    L = lib()
+    L.hmo_diffusive_patch.restype = ctypes.c_int
+    L.hmo_diffusive_patch.argtypes = [ctypes.POINTER(_Params), ctypes.POINTER(_Transport),
+                                      ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p]
+    d = prm.dim
+    neq = prm.neq
+    n = tuple(int(v) for v in n)
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    assert Q.shape == (prm.nq,) + tuple(reversed([v + 2 * g for v in n])), Q.shape
+    F = [np.zeros(face_shape(neq, n, a)) for a in range(d)]
+    nn = (ctypes.c_int * 3)(*(list(n) + [1] * (3 - d)))
+    dd = (ctypes.c_double * 3)(*(list(dx) + [1.0] * (3 - d)))
+    cp, ct = prm.to_c(), tr.to_c()
+    st = L.hmo_diffusive_patch(ctypes.byref(cp), ctypes.byref(ct), nn, g, dd, dt, _ptr(Q), _ptr(F[0]),
+                               _ptr(F[1]) if d > 1 else None, _ptr(F[2]) if d > 2 else None)
+    if st != 0:
+        raise RuntimeError(f"oracle hmo_diffusive_patch failed with status {st}")
+    return F
