@@ -16,6 +16,7 @@ from . import capi  # noqa: E402,F401  (ctypes bindings; raises if the library i
 from .reconstructor import (  # noqa: E402,F401
     ConvectiveFluxReconstructor,
     ConvectiveFluxReconstructorManager,
+    DiffusiveFluxReconstructor,
     PatchData,
     PatchList,
 )

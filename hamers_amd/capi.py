@@ -79,6 +79,16 @@ class hm_rk_patch_view(ctypes.Structure):
     ]
 
 
+class hm_transport(ctypes.Structure):
+    _fields_ = [
+        ("species_mu", ctypes.c_double * HM_MAX_SPECIES),
+        ("species_mu_v", ctypes.c_double * HM_MAX_SPECIES),
+        ("species_kappa", ctypes.c_double * HM_MAX_SPECIES),
+        ("species_D", ctypes.c_double * HM_MAX_SPECIES),
+        ("species_M", ctypes.c_double * HM_MAX_SPECIES),
+    ]
+
+
 class LibraryMissing(RuntimeError):
     pass
 
@@ -145,11 +155,19 @@ def _load():
     L.hm_level_adjoint.argtypes = [P, P]
     L.hm_level_write_adjoint.argtypes = [P, ctypes.c_int, D]
     L.hm_level_read_adjoint.argtypes = [P, ctypes.c_int, D]
+    L.hm_set_diffusive_transport.argtypes = [P, ctypes.POINTER(hm_transport)]
+    L.hm_get_diffusive_num_ghosts.argtypes = [P, I]
+    L.hm_compute_diffusive_flux_on_patch.argtypes = [P, ctypes.POINTER(hm_patch_view), ctypes.c_double,
+                                                     ctypes.c_double, ctypes.c_int, P]
+    L.hm_compute_diffusive_flux_on_patches.argtypes = [P, ctypes.POINTER(hm_patch_view), ctypes.c_int,
+                                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, P]
     for name in ("hm_advance_single_step_on_patch", "hm_advance_single_step_on_patches", "hm_level_create",
                  "hm_level_destroy", "hm_level_set_rk_coefficients", "hm_level_write_patch",
                  "hm_level_read_patch", "hm_level_patch_array", "hm_level_fill_ghosts", "hm_level_advance",
                  "hm_convective_flux_vjp_on_patch", "hm_convective_flux_vjp_on_patches", "hm_level_advance_recorded",
-                 "hm_level_adjoint", "hm_level_write_adjoint", "hm_level_read_adjoint"):
+                 "hm_level_adjoint", "hm_level_write_adjoint", "hm_level_read_adjoint",
+                 "hm_set_diffusive_transport", "hm_get_diffusive_num_ghosts", "hm_compute_diffusive_flux_on_patch",
+                 "hm_compute_diffusive_flux_on_patches"):
         getattr(L, name).restype = ctypes.c_int
     return L
 

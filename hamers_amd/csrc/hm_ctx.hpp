@@ -5,11 +5,13 @@
 #include <string>
 
 #include "hm_adjoint.hpp"
+#include "hm_diffusive.hpp"
 #include "hm_reconstructor.hpp"
 
 struct hm_ctx {
     std::unique_ptr<hm::ConvectiveFluxReconstructorGPU> rec;
     std::unique_ptr<hm::AdjointSingle> adj;  // created on the first VJP call; destroyed before rec
+    std::unique_ptr<hm::DiffusiveFluxReconstructorGPU> diff;  // created by hm_set_diffusive_transport
     std::string err;
     int levels = 0;  // live hm_level objects built on this context
 };

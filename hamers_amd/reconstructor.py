@@ -211,6 +211,45 @@ def convective_flux_vjp(rec: ConvectiveFluxReconstructor, patches, F_bar, Q_bar,
     capi.check(rec._ctx, st)
 
 
+class DiffusiveFluxReconstructor:
+    """The node-based SIXTH_ORDER diffusive flux reconstructor of the
+    four-equation model (DiffusiveFluxReconstructor.hpp:73-79,
+    DiffusiveFluxReconstructorManager.cpp:13-73) with constant transport
+    mixing rules.  It shares the convective reconstructor's context (species
+    gamma and R, device), as NavierStokes holds both reconstructors of one
+    flow model.  Transport values are per species."""
+
+    def __init__(self, convective: ConvectiveFluxReconstructor, species_mu, species_mu_v, species_kappa,
+                 species_D, species_M):
+        self.rec = convective
+        tr = capi.hm_transport()
+        for key, vals in (("species_mu", species_mu), ("species_mu_v", species_mu_v),
+                          ("species_kappa", species_kappa), ("species_D", species_D), ("species_M", species_M)):
+            arr = getattr(tr, key)
+            for i, val in enumerate(vals):
+                arr[i] = float(val)
+        capi.check(convective._ctx, capi.lib.hm_set_diffusive_transport(convective._ctx, ctypes.byref(tr)))
+
+    def getDiffusiveFluxNumberOfGhostCells(self) -> int:
+        g = ctypes.c_int()
+        capi.check(self.rec._ctx, capi.lib.hm_get_diffusive_num_ghosts(self.rec._ctx, ctypes.byref(g)))
+        return g.value
+
+    def computeDiffusiveFluxOnPatch(self, patch: PatchData, time: float, dt: float, RK_step_number: int = 0,
+                                    stream=None):
+        """``patch.F`` are the diffusive flux arrays (overwritten); Q needs 6 ghosts."""
+        v = patch.view(self.rec.num_eqn)
+        st = capi.lib.hm_compute_diffusive_flux_on_patch(self.rec._ctx, ctypes.byref(v), time, dt, RK_step_number,
+                                                         self.rec._stream_ptr(stream))
+        capi.check(self.rec._ctx, st)
+
+    def computeDiffusiveFluxOnPatches(self, plist: PatchList, time: float, dt: float, RK_step_number: int = 0,
+                                      stream=None):
+        st = capi.lib.hm_compute_diffusive_flux_on_patches(self.rec._ctx, plist.views, len(plist), time, dt,
+                                                           RK_step_number, self.rec._stream_ptr(stream))
+        capi.check(self.rec._ctx, st)
+
+
 class ConvectiveFluxReconstructorManager:
     """Builds the reconstructor named by the deck key, as
     ConvectiveFluxReconstructorManager.cpp:14-153 does; unknown keys raise."""

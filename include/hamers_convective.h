@@ -150,6 +150,50 @@ int hm_profile_read(hm_ctx* ctx, double* stage_ms /* [HM_NUM_STAGES] */,
 int hm_synchronize(hm_ctx* ctx, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Diffusive flux (four-equation conservative model)                         */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Constant transport properties per species: the "constant" mixing rules of
+ * the equations of shear viscosity, bulk viscosity, thermal conductivity and
+ * mass diffusivity (src/util/mixing_rules/equations_of_<property>/constant/), i.e. the
+ * deck values species_mu, species_mu_v, species_kappa, species_D and the
+ * molecular weights species_M of the mixing weights Y_i / sqrt(M_i).
+ */
+typedef struct hm_transport {
+    double species_mu[HM_MAX_SPECIES];
+    double species_mu_v[HM_MAX_SPECIES];
+    double species_kappa[HM_MAX_SPECIES];
+    double species_D[HM_MAX_SPECIES];
+    double species_M[HM_MAX_SPECIES];
+} hm_transport;
+
+/*
+ * Give a FOUR_EQN_CONSERVATIVE context its diffusive flux reconstructor:
+ * DiffusiveFluxReconstructorManager with the deck key "SIXTH_ORDER"
+ * (DiffusiveFluxReconstructorManager.cpp:13-73) and the transport mixing rules.
+ */
+int hm_set_diffusive_transport(hm_ctx* ctx, const hm_transport* transport);
+
+/* DiffusiveFluxReconstructor::getDiffusiveFluxNumberOfGhostCells: 6
+ * (DiffusiveFluxReconstructorNodeSixthOrder.cpp:22). */
+int hm_get_diffusive_num_ghosts(const hm_ctx* ctx, int* num_ghosts);
+
+/*
+ * DiffusiveFluxReconstructor::computeDiffusiveFluxOnPatch
+ * (DiffusiveFluxReconstructor.hpp:73-79, node-based SIXTH_ORDER,
+ * DiffusiveFluxReconstructorNode.cpp:32-1738): patch->F[a] is the diffusive
+ * flux SideData of axis a (num_eqn components, overwritten with dt-scaled face
+ * fluxes), patch->Q needs a ghost width >= 6, patch->S is not used.  The stage
+ * update subtracts its divergence beside the convective flux's
+ * (NavierStokes.cpp:1704-1750).
+ */
+int hm_compute_diffusive_flux_on_patch(hm_ctx* ctx, const hm_patch_view* patch, double time, double dt,
+                                       int rk_step, void* stream);
+int hm_compute_diffusive_flux_on_patches(hm_ctx* ctx, const hm_patch_view* patches, int num_patches,
+                                         double time, double dt, int rk_step, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Discrete adjoint (no counterpart in the reference: a new capability)      */
 /* ------------------------------------------------------------------------ */
 

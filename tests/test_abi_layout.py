@@ -38,6 +38,8 @@ def test_struct_layouts_match_header(tmp_path):
         "hm_config": capi.hm_config,
         "hm_patch_view": capi.hm_patch_view,
         "hm_rk_patch_view": capi.hm_rk_patch_view,
+        "hm_vjp_patch_view": capi.hm_vjp_patch_view,
+        "hm_transport": capi.hm_transport,
     }
     c = _c_layout(tmp_path, {k: [f[0] for f in v._fields_] for k, v in structs.items()})
     for name, cls in structs.items():
