@@ -17,7 +17,11 @@ drawn (seeded) from {8, 16, 24, 32, 48, 64}; batched RHS against the
 per-patch path.  Its fields are generated on the GPU with torch (synthetic
 input only; every RHS runs in libhamers_amd.so).
 
-A third leg, reported under "adjoint", times the discrete adjoint of 200
+A third leg, "navier_stokes_3d", times the convective + diffusive RHS of a
+3D four-equation two-species Navier-Stokes level of uneven patches (multimode
+interface, constant transport properties), batched against per-patch calls.
+
+A fourth leg, reported under "adjoint", times the discrete adjoint of 200
 SSP-RK3 steps on a 512^2 Kelvin-Helmholtz level (16 patches of 128^2,
 checkpoints every 20 steps) against the forward run, and checks its gradient
 w.r.t. 8 mode amplitudes against central differences.
@@ -53,6 +57,8 @@ def parse():
     ap.add_argument("--no-per-patch", action="store_true")
     ap.add_argument("--uneven-seg", type=int, default=14, help="patches per axis of the uneven 3D level (0: skip)")
     ap.add_argument("--uneven-steps", type=int, default=5)
+    ap.add_argument("--ns-seg", type=int, default=8, help="patches per axis of the uneven 3D Navier-Stokes level (0: skip)")
+    ap.add_argument("--ns-steps", type=int, default=4)
     ap.add_argument("--adjoint-steps", type=int, default=200, help="RK3 steps of the adjoint leg (0: skip)")
     ap.add_argument("--adjoint-N", type=int, default=512)
     ap.add_argument("--adjoint-patch", type=int, default=128)
@@ -170,6 +176,120 @@ def bench_uneven_3d(args, torch, np, time):
         "batched_speedup": (ms_p / ms_b) if ms_p else None,
         "stage_ms": stage, "bytes_per_cell": bytes_per_cell,
         "rhs_hbm_frac": bytes_per_cell * cells / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
+
+
+def rtm_interface_patch_torch(torch, lo, n, g, dx, modes, gamma=(1.4, 1.67), R=(1.0, 0.25), rho_gas=(3.0, 1.0)):
+    """Conservative four-equation arrays (6, n2+2g, n1+2g, n0+2g) of one patch of a
+    multimode heavy/light interface x_I(y, z) = 0.5 + sum_k a_k (cos(2 pi k y + phi_k)
+    + cos(2 pi k z + psi_k)) / 2 over k in [4, 32] with a_k ~ k^-2, smoothed over
+    two cells, at rest at p = 1 (halos from the same analytic field)."""
+    dev = "cuda:0"
+    ax = [(torch.arange(n[a] + 2 * g, dtype=torch.float64, device=dev) + (lo[a] - g) + 0.5) * dx[a]
+          for a in range(3)]
+    z, y, x = torch.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    xi = torch.full_like(y, 0.5)
+    for k, (amp, ph_y, ph_z) in zip(range(4, 33), modes):
+        xi = xi + amp * 0.5 * (torch.cos(2 * torch.pi * k * y + ph_y) + torch.cos(2 * torch.pi * k * z + ph_z))
+    Y1 = 0.5 * (1.0 - torch.tanh((x - xi) / (2.0 * dx[0])))
+    Y1 = torch.clamp(Y1, 1e-6, 1.0 - 1e-6)
+    rho = 1.0 / (Y1 / rho_gas[0] + (1.0 - Y1) / rho_gas[1])
+    cp = [gm / (gm - 1.0) * r for gm, r in zip(gamma, R)]
+    cv = [1.0 / (gm - 1.0) * r for gm, r in zip(gamma, R)]
+    gmix = (Y1 * cp[0] + (1 - Y1) * cp[1]) / (Y1 * cv[0] + (1 - Y1) * cv[1])
+    E = 1.0 / (gmix - 1.0)
+    zero = torch.zeros_like(x)
+    return torch.stack([rho * Y1, rho * (1.0 - Y1), zero, zero, zero, E]).contiguous()
+
+
+def bench_ns_3d(args, torch, np, time):
+    """Convective + diffusive RHS of an uneven-patch 3D four-equation
+    Navier-Stokes level, batched against per-patch calls."""
+    import ctypes
+
+    from hamers_amd import ConvectiveFluxReconstructor, DiffusiveFluxReconstructor, PatchData, PatchList, capi
+
+    rng = np.random.default_rng(4096)
+    segs = [rng.choice(UNEVEN_CHOICES, size=args.ns_seg) for _ in range(3)]
+    starts = [np.concatenate([[0], np.cumsum(s)[:-1]]) for s in segs]
+    Ng = [int(s.sum()) for s in segs]
+    dx = tuple(1.0 / v for v in Ng)
+    ks = np.arange(4, 33)
+    amps = ks ** -2.0
+    amps = 0.02 * amps / amps.sum()
+    modes = [(float(a), rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi)) for a in amps]
+    g, neq = 6, 6
+    gamma, R = (1.4, 1.67), (1.0, 0.25)
+    rec = ConvectiveFluxReconstructor("ns", 3, "FOUR_EQN_CONSERVATIVE", "WCNS6_LD_HLLC_HLL", num_species=2,
+                                      species_gamma=gamma, species_R=R)
+    diff = DiffusiveFluxReconstructor(rec, species_mu=(1e-3, 2e-3), species_mu_v=(0.0, 0.0),
+                                      species_kappa=(1.5e-3, 2.5e-3), species_D=(1e-3, 1e-3), species_M=(1.0, 4.0))
+    dev = "cuda:0"
+    conv, dif, boxes = [], [], []
+    for k in range(args.ns_seg):
+        for j in range(args.ns_seg):
+            for i in range(args.ns_seg):
+                lo = (int(starts[0][i]), int(starts[1][j]), int(starts[2][k]))
+                n = (int(segs[0][i]), int(segs[1][j]), int(segs[2][k]))
+                Q = rtm_interface_patch_torch(torch, lo, n, g, dx, modes, gamma, R)
+                shp = [(neq, n[2] + (a == 2), n[1] + (a == 1), n[0] + (a == 0)) for a in range(3)]
+                Fc = [torch.empty(s_, dtype=torch.float64, device=dev) for s_ in shp]
+                Fd = [torch.empty(s_, dtype=torch.float64, device=dev) for s_ in shp]
+                S = torch.zeros((neq, n[2], n[1], n[0]), dtype=torch.float64, device=dev)
+                conv.append(PatchData(n, (g, g, g), dx, Q, Fc, S))
+                dif.append(PatchData(n, (g, g, g), dx, Q, Fd, None))
+                boxes.append((lo, n))
+    cells = sum(b[1][0] * b[1][1] * b[1][2] for b in boxes)
+    pc, pdf = PatchList(conv, neq), PatchList(dif, neq)
+    dt = 1.0e-4
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    fc, fd = capi.lib.hm_compute_convective_flux_and_source_on_patch, capi.lib.hm_compute_diffusive_flux_on_patch
+    vc = [p.view(neq) for p in conv]
+    vd = [p.view(neq) for p in dif]
+
+    def batched_conv():
+        rec.computeConvectiveFluxAndSourceOnPatches(pc, 0.0, dt, 0, stream)
+
+    def batched_diff():
+        diff.computeDiffusiveFluxOnPatches(pdf, 0.0, dt, 0, stream)
+
+    def batched():
+        batched_conv()
+        batched_diff()
+
+    def per_patch():
+        for a, b in zip(vc, vd):
+            st = fc(rec._ctx, ctypes.byref(a), 0.0, dt, 0, sp) or fd(rec._ctx, ctypes.byref(b), 0.0, dt, 0, sp)
+            if st:
+                capi.check(rec._ctx, st)
+
+    def timed(f, steps):
+        f()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            f()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    ms_b = timed(batched, args.ns_steps)
+    ms_bc = timed(batched_conv, args.ns_steps)
+    ms_bd = timed(batched_diff, args.ns_steps)
+    ms_p = timed(per_patch, max(2, args.ns_steps // 2)) if not args.no_per_patch else None
+    return {
+        "workload": "3D four-equation two-species Navier-Stokes (constant transport), multimode interface with "
+                    "a k^-2 spectrum over k in [4, 32], WCNS6_LD_HLLC_HLL convective + sixth-order node diffusive "
+                    "flux, one level of uneven patches (extents per axis from {8,16,24,32,48,64}), 6-cell halos",
+        "patches": len(conv), "cells": int(cells), "domain": Ng,
+        "ms_per_rhs_batched": ms_b, "ms_convective_batched": ms_bc, "ms_diffusive_batched": ms_bd,
+        "value_batched": cells / (ms_b * 1e-3),
+        "ms_per_rhs_per_patch": ms_p, "value_per_patch": (cells / (ms_p * 1e-3)) if ms_p else None,
+        "batched_speedup": (ms_p / ms_b) if ms_p else None,
+        "bytes_per_cell": 8 * (neq + 3 * neq + 3 * neq),
     }
 
 
@@ -346,6 +466,11 @@ def main():
         uneven = bench_uneven_3d(args, torch, np, time)
         torch.cuda.empty_cache()
 
+    ns3d = None
+    if args.ns_seg > 0:
+        ns3d = bench_ns_3d(args, torch, np, time)
+        torch.cuda.empty_cache()
+
     adjoint = None
     if args.adjoint_steps > 0:
         adjoint = bench_adjoint(args, torch, np, time)
@@ -377,6 +502,7 @@ def main():
         "host_wall_ms_per_step": wall_batched,
         "cpu_baseline": cpu,
         "uneven_3d": uneven,
+        "navier_stokes_3d": ns3d,
         "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
