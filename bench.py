@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--no-per-patch", action="store_true")
     ap.add_argument("--uneven-seg", type=int, default=14, help="patches per axis of the uneven 3D level (0: skip)")
     ap.add_argument("--uneven-steps", type=int, default=5)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="only copy this many MiB device to device 5 times (calibrates HBM byte counters) and exit")
     ap.add_argument("--ns-seg", type=int, default=8, help="patches per axis of the uneven 3D Navier-Stokes level (0: skip)")
     ap.add_argument("--ns-steps", type=int, default=4)
     ap.add_argument("--adjoint-steps", type=int, default=200, help="RK3 steps of the adjoint leg (0: skip)")
@@ -360,6 +362,17 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_probe_mb > 0:
+        # A plain copy of known size: its FETCH_SIZE / WRITE_SIZE readings give
+        # the counters' scale for the byte figures of the RHS kernels.
+        src = torch.ones(args.copy_probe_mb * (1 << 20) // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 8, "copies": 5}), flush=True)
+        return
 
     from hamers_amd import ConvectiveFluxReconstructor, PatchData, PatchList, capi
     from hamers_amd import workloads as W

@@ -14,24 +14,50 @@ namespace hm {
 // Kernels.
 // ----------------------------------------------------------------------------
 
-// Per midpoint face of axis A: adjoint of the midpoint-to-face combination
-// F_i = dt (1/30 (m_{i+1} + m_{i-1}) - 3/10 (n_i + n_{i-1}) + 23/15 m_i)
-// (WCNS56-HLLC-HLL.cpp:1200-1282) with respect to the midpoint flux m_j, then
-// the reverse sweep of the face computation; writes the stencil adjoints to G.
-template <int D, int A, int SCHEME, int PP, int QQ>
-__device__ __forceinline__ void adj_face(const AdjPatch& P, int64_t loc, const KParams& k) {
-    constexpr int NEQ = D + 2;
-    constexpr int NG = 6 * NEQ + 2;
-    const DevPatch& F = P.f;
-    const Boxes<D> B(F);
+// Addressing shared by the two face-adjoint kernels: the face's cell indices
+// (c0, c1, c2: the cell right of the face) from its midpoint-face number.
+template <int D, int A>
+__device__ __forceinline__ void mid_cell(const Boxes<D>& B, int64_t loc, int& c0, int& c1, int& c2) {
     const int m0 = B.mid_ext(A, 0), m1 = B.mid_ext(A, 1);
-    int c0 = int(loc % m0);
+    c0 = int(loc % m0);
     const int64_t r = loc / m0;
-    int c1 = (D > 1) ? int(r % m1) : 0;
-    int c2 = (D > 2) ? int(r / m1) : 0;
+    c1 = (D > 1) ? int(r % m1) : 0;
+    c2 = (D > 2) ? int(r / m1) : 0;
     if constexpr (A == 0) c0 -= 1;
     if constexpr (A == 1) c1 -= 1;
     if constexpr (A == 2) c2 -= 1;
+}
+
+template <int D, int A>
+__device__ __forceinline__ void load_stencil(const DevPatch& F, const Boxes<D>& B, int c0, int c1, int c2,
+                                             double (&V)[6][D + 2], double& cL, double& cR) {
+    constexpr int NEQ = D + 2;
+    const int64_t v4 = B.vol(kGc);
+    const int64_t st = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGc)) : int64_t(B.ext(0, kGc)) * B.ext(1, kGc);
+    const int64_t iR = B.idx(kGc, c0, c1, c2);
+    const double* pr = F.prim;
+#pragma unroll
+    for (int m = 0; m < 6; m++)
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) V[m][e] = pr[e * v4 + iR + (m - 3) * st];
+    cL = pr[NEQ * v4 + iR - st];
+    cR = pr[NEQ * v4 + iR];
+}
+
+// Per midpoint face of axis A: adjoint of the midpoint-to-face combination
+// F_i = dt (1/30 (m_{i+1} + m_{i-1}) - 3/10 (n_i + n_{i-1}) + 23/15 m_i)
+// (WCNS56-HLLC-HLL.cpp:1200-1282) with respect to the midpoint flux m_j, then
+// the first half of the face's reverse sweep (Riemann flux, fallback,
+// back-projection) into the hand-over buffer H; H's last entry is 2 when the
+// face receives no adjoint at all.
+template <int D, int A, int SCHEME, int PP, int QQ>
+__device__ __forceinline__ void adj_face_flux(const AdjPatch& P, int64_t loc, const KParams& k) {
+    constexpr int NEQ = D + 2;
+    constexpr int NH = face_adj_h_size<D>();
+    const DevPatch& F = P.f;
+    const Boxes<D> B(F);
+    int c0, c1, c2;
+    mid_cell<D, A>(B, loc, c0, c1, c2);
     const int j = (A == 0) ? c0 : (A == 1) ? c1 : c2;
     const int nA = B.n[A];
     const int64_t fv = B.face_vol(A);
@@ -51,23 +77,14 @@ __device__ __forceinline__ void adj_face(const AdjPatch& P, int64_t loc, const K
         any = any || (mb[e] != 0.0);
     }
     const int64_t mv = B.mid_vol(A);
-    double* G = P.G[A];
+    double* H = P.H[A];
     if (!any) {
-#pragma unroll
-        for (int c = 0; c < NG; c++) G[c * mv + loc] = 0.0;
+        H[(NH - 1) * mv + loc] = 2.0;
         return;
     }
-    const int64_t v4 = B.vol(kGc);
-    const int64_t st = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGc)) : int64_t(B.ext(0, kGc)) * B.ext(1, kGc);
-    const int64_t iR = B.idx(kGc, c0, c1, c2);
-    const double* pr = F.prim;
     double V[6][NEQ];
-#pragma unroll
-    for (int m = 0; m < 6; m++)
-#pragma unroll
-        for (int e = 0; e < NEQ; e++) V[m][e] = pr[e * v4 + iR + (m - 3) * st];
-    const double cL = pr[NEQ * v4 + iR - st];
-    const double cR = pr[NEQ * v4 + iR];
+    double cL, cR;
+    load_stencil<D, A>(F, B, c0, c1, c2, V, cL, cR);
     bool use_hll = false;
     if constexpr (D > 1) {
         const int64_t vs = B.vol(kGs);
@@ -79,14 +96,46 @@ __device__ __forceinline__ void adj_face(const AdjPatch& P, int64_t loc, const K
         const double sensor = -th_avg / (fabs(th_avg) + om_avg + kEps);
         use_hll = sensor > 0.65;
     }
+    double h[NH];
+    face_adj_single_flux<D, A, SCHEME, PP, QQ>(V, cL, cR, k.gamma[0], LDConst{k.p, k.q, k.C, k.alpha_tau}, use_hll,
+                                               mb, h);
+#pragma unroll
+    for (int c = 0; c < NH; c++) H[c * mv + loc] = h[c];
+}
+
+// Second half: reverse reconstruction and characteristic projection; writes
+// the six stencil cells' adjoints (and the two sound-speed adjoints) to G.
+template <int D, int A, int SCHEME, int PP, int QQ>
+__device__ __forceinline__ void adj_face_recon(const AdjPatch& P, int64_t loc, const KParams& k) {
+    constexpr int NEQ = D + 2;
+    constexpr int NG = 6 * NEQ + 2;
+    constexpr int NH = face_adj_h_size<D>();
+    const DevPatch& F = P.f;
+    const Boxes<D> B(F);
+    const int64_t mv = B.mid_vol(A);
+    const double* H = P.H[A] + loc;
+    double* G = P.G[A];
+    if (H[(NH - 1) * mv] == 2.0) {
+#pragma unroll
+        for (int c = 0; c < NG; c++) G[c * mv + loc] = 0.0;
+        return;
+    }
+    int c0, c1, c2;
+    mid_cell<D, A>(B, loc, c0, c1, c2);
+    const int64_t v4 = B.vol(kGc);
+    const int64_t st = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGc)) : int64_t(B.ext(0, kGc)) * B.ext(1, kGc);
+    const double* pc = F.prim + B.idx(kGc, c0, c1, c2);  // cell right of the face
+    const double cL = pc[NEQ * v4 - st];
+    const double cR = pc[NEQ * v4];
     double Vb[6][NEQ];
 #pragma unroll
     for (int m = 0; m < 6; m++)
 #pragma unroll
         for (int e = 0; e < NEQ; e++) Vb[m][e] = 0.0;
     double cLb = 0.0, cRb = 0.0;
-    face_adj_single<D, A, SCHEME, PP, QQ>(V, cL, cR, k.gamma[0], LDConst{k.p, k.q, k.C, k.alpha_tau}, use_hll, mb,
-                                          Vb, cLb, cRb);
+    face_adj_single_recon<D, A, SCHEME, PP, QQ>([&](int m, int e) { return pc[e * v4 + (m - 3) * st]; }, cL, cR,
+                                                LDConst{k.p, k.q, k.C, k.alpha_tau}, [&](int c) { return H[c * mv]; },
+                                                Vb, cLb, cRb);
 #pragma unroll
     for (int m = 0; m < 6; m++)
 #pragma unroll
@@ -96,19 +145,37 @@ __device__ __forceinline__ void adj_face(const AdjPatch& P, int64_t loc, const K
 }
 
 template <int D, int SCHEME, int PP, int QQ, class W>
-__global__ __launch_bounds__(kBlock) void k_adj_face(W w, KParams k) {
+__global__ __launch_bounds__(kBlock) void k_adj_face_flux(W w, KParams k) {
     const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     const AdjPatch* P;
     int ax;
     int64_t loc;
     if (!w.find(gid, P, ax, loc)) return;
     if (ax == 0) {
-        adj_face<D, 0, SCHEME, PP, QQ>(*P, loc, k);
+        adj_face_flux<D, 0, SCHEME, PP, QQ>(*P, loc, k);
     } else if constexpr (D > 1) {
         if (ax == 1) {
-            adj_face<D, 1, SCHEME, PP, QQ>(*P, loc, k);
+            adj_face_flux<D, 1, SCHEME, PP, QQ>(*P, loc, k);
         } else if constexpr (D > 2) {
-            adj_face<D, 2, SCHEME, PP, QQ>(*P, loc, k);
+            adj_face_flux<D, 2, SCHEME, PP, QQ>(*P, loc, k);
+        }
+    }
+}
+
+template <int D, int SCHEME, int PP, int QQ, class W>
+__global__ __launch_bounds__(kBlock) void k_adj_face_recon(W w, KParams k) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const AdjPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    if (ax == 0) {
+        adj_face_recon<D, 0, SCHEME, PP, QQ>(*P, loc, k);
+    } else if constexpr (D > 1) {
+        if (ax == 1) {
+            adj_face_recon<D, 1, SCHEME, PP, QQ>(*P, loc, k);
+        } else if constexpr (D > 2) {
+            adj_face_recon<D, 2, SCHEME, PP, QQ>(*P, loc, k);
         }
     }
 }
@@ -239,8 +306,12 @@ template <int D, int SCHEME, int PP, int QQ>
 hipError_t launch_adj_dsq(const AdjPlanDev& pl, const KParams& k, hipStream_t s) {
     using W = SegTableT<AdjPatch>;
     if (pl.n_qbox > 0) hipLaunchKernelGGL((k_adj_zero<D, W>), dim3(nblk(pl.n_qbox)), dim3(kBlock), 0, s, pl.qbox);
-    if (pl.n_face > 0)
-        hipLaunchKernelGGL((k_adj_face<D, SCHEME, PP, QQ, W>), dim3(nblk(pl.n_face)), dim3(kBlock), 0, s, pl.face, k);
+    if (pl.n_face > 0) {
+        hipLaunchKernelGGL((k_adj_face_flux<D, SCHEME, PP, QQ, W>), dim3(nblk(pl.n_face)), dim3(kBlock), 0, s, pl.face,
+                           k);
+        hipLaunchKernelGGL((k_adj_face_recon<D, SCHEME, PP, QQ, W>), dim3(nblk(pl.n_face)), dim3(kBlock), 0, s,
+                           pl.face, k);
+    }
     if (pl.n_cell > 0)
         hipLaunchKernelGGL((k_adj_cells<D, W>), dim3(nblk(pl.n_cell)), dim3(kBlock), 0, s, pl.cell, k);
     return hipGetLastError();
@@ -317,6 +388,7 @@ int AdjointSingle::build(const hm_vjp_patch_view* views, int np, Owned& out) {
             for (int b = 0; b < d; b++) mv *= fv[i].n[b] + ((b == a) ? 3 : 0);
             goff[size_t(i) * 3 + a] = gtot;
             gtot += (size_t(6 * neq + 2) * size_t(mv) + 31) / 32 * 32;
+            gtot += (size_t(2 * neq + 3) * size_t(mv) + 31) / 32 * 32;  // H
         }
     if (hipMalloc(&out.scratch, (fwd_doubles + gtot) * sizeof(double)) != hipSuccess) {
         out.scratch = nullptr;
@@ -335,6 +407,13 @@ int AdjointSingle::build(const hm_vjp_patch_view* views, int np, Owned& out) {
         for (int a = 0; a < 3; a++) {
             A.Fb[a] = (a < d) ? views[i].F_bar[a] : nullptr;
             A.G[a] = (a < d) ? base + fwd_doubles + goff[size_t(i) * 3 + a] : nullptr;
+            if (a < d) {
+                int64_t mv = 1;
+                for (int b = 0; b < d; b++) mv *= fv[i].n[b] + ((b == a) ? 3 : 0);
+                A.H[a] = A.G[a] + (size_t(6 * neq + 2) * size_t(mv) + 31) / 32 * 32;
+            } else {
+                A.H[a] = nullptr;
+            }
         }
         for (int e = 0; e < HM_MAX_EQN; e++) A.Qb[e] = (e < neq) ? views[i].Q_bar[e] : nullptr;
         int64_t v4 = 1, vq = 1;

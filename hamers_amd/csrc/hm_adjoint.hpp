@@ -5,10 +5,13 @@
 // the adjoints F_bar_a of the face fluxes, the VJP returns Q_bar = (dF/dQ)^T
 // F_bar over the ghost box.  It runs as
 //   1. the forward derived-data and sensor stages (prim V, c; theta, Omega),
-//   2. k_adj_face: per midpoint face, the adjoint of the sixth-order
+//   2. k_adj_face_flux: per midpoint face, the adjoint of the sixth-order
 //      midpoint-to-face combination, then the reverse sweep of the face's
-//      projection / reconstruction / Riemann flux (hm_adjoint_face.hpp); the
-//      six stencil cells' adjoints go to a per-face buffer G,
+//      Riemann flux and back-projection (hm_adjoint_face.hpp), handed over in
+//      a per-face buffer H; k_adj_face_recon: the reverse reconstruction and
+//      characteristic projection, one field at a time; the six stencil
+//      cells' adjoints go to a per-face buffer G (two kernels, so that each
+//      half fits the register file without spilling),
 //   3. k_adj_cells: per cell of the 4-ghost box, a deterministic gather of the
 //      G entries of every face whose stencil holds the cell, plus the node
 //      flux adjoint, mapped through the derived-data Jacobian to Q_bar.
@@ -25,6 +28,7 @@ struct AdjPatch {
     DevPatch f;                // forward view: n, gq, dx, Q, prim, sens
     const double* Fb[3];       // face-flux adjoints per axis (layout of F)
     double* G[3];              // per midpoint face: 6*NEQ + 2 stencil adjoints
+    double* H[3];              // per midpoint face: hand-over between the two face-adjoint kernels
     double* Qb[HM_MAX_EQN];    // output over the Q ghost box
 };
 

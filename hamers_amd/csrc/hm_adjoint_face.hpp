@@ -229,8 +229,24 @@ HM_HD void reconstruct_pair_adj(const Six& s, const LDConst& k, double umb, doub
     if constexpr (SCHEME == kWCNS6_LD) {
         const double sig = ld_sigma(s);
         double sigb = 0.0;
-        ld_side_adj<P, Q>(s, sig, k, umb, sb, sigb);
-        ld_side_adj<P, Q>(f, sig, k, upb, fb, sigb);
+        // The two sides one after the other (a runtime loop): their reverse
+        // sweeps are independent, and interleaved they would not fit the
+        // register file.
+#pragma unroll 1
+        for (int side = 0; side < 2; side++) {
+            Six ss;
+#pragma unroll
+            for (int m = 0; m < 6; m++) ss.v[m] = side ? f.v[m] : s.v[m];
+            double tb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            ld_side_adj<P, Q>(ss, sig, k, side ? upb : umb, tb, sigb);
+            if (side) {
+#pragma unroll
+                for (int m = 0; m < 6; m++) fb[m] += tb[m];
+            } else {
+#pragma unroll
+                for (int m = 0; m < 6; m++) sb[m] += tb[m];
+            }
+        }
         ld_sigma_adj(s, sigb, sb);
     } else {
         js_side_adj<P>(s, k.p, umb, sb);
@@ -564,36 +580,57 @@ HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bo
     }
 }
 
-// Adjoint of the whole midpoint-flux computation of one face along axis A:
-// V[m][e] are the primitive variables of stencil cells i-3+m, cL/cR the sound
-// speeds of cells i-1 and i, mb the adjoint of the midpoint flux.  Adds
-// d(flux)/dV * mb to Vb and d(flux)/dc * mb to cLb, cRb.
+// Characteristic field f of stencil point m for axis A (the forward
+// projection's expression trees, FlowModelBasicUtilitiesSingleSpecies.cpp:6194-6242);
+// vat(m, e) reads component e of stencil point m.
+template <int D, int A, class VAt>
+HM_HD double char_field_at(const VAt& vat, double ra, double ca, int f, int m) {
+    constexpr int IP = D + 1;
+    if (f == 0) return -double(1) / double(2) * ra * ca * vat(m, 1 + A) + double(1) / double(2) * vat(m, IP);
+    if (f == 1) return vat(m, 0) - double(1) / (ca * ca) * vat(m, IP);
+    if (f == IP) return double(1) / double(2) * ra * ca * vat(m, 1 + A) + double(1) / double(2) * vat(m, IP);
+    int t = 2;
+    for (int b = 0; b < D; b++) {
+        if (b == A) continue;
+        if (t == f) return vat(m, 1 + b);
+        t++;
+    }
+    return 0.0;
+}
+
+template <int D, int A>
+HM_HD double char_field(const double (&V)[6][D + 2], double ra, double ca, int f, int m) {
+    return char_field_at<D, A>([&](int mm, int e) { return V[mm][e]; }, ra, ca, f, m);
+}
+
+// Size of the per-face hand-over between the two halves of a face adjoint.
+template <int D>
+constexpr int face_adj_h_size() {
+    return 2 * (D + 2) + 3;
+}
+
+// First half of the adjoint of one face's midpoint flux along axis A: the
+// forward reconstruction, back-projection and bounds check, the Riemann-flux
+// adjoint, then the back-projection adjoint.  V[m][e] are the primitive
+// variables of stencil cells i-3+m, cL/cR the sound speeds of cells i-1 and
+// i, mb the adjoint of the midpoint flux.  Writes h[0..NEQ) = adjoint of the
+// minus-side characteristic values (or of V_L where the first-order fallback
+// was taken), h[NEQ..2NEQ) = the plus side's (or V_R's), h[2NEQ], h[2NEQ+1] =
+// the adjoints of rho_avg and c_avg so far, h[2NEQ+2] = 1 (reconstructed) or
+// 0 (fallback).
 template <int D, int A, int SCHEME, int P, int Q>
-HM_HD void face_adj_single(const double (&V)[6][D + 2], double cL, double cR, double gam, const LDConst& ld,
-                           bool use_hll, const double* mb, double (&Vb)[6][D + 2], double& cLb, double& cRb) {
+HM_HD void face_adj_single_flux(const double (&V)[6][D + 2], double cL, double cR, double gam, const LDConst& ld,
+                                bool use_hll, const double* mb, double* h) {
     constexpr int NEQ = D + 2;
     constexpr int IP = D + 1;
     const double ra = double(1) / double(2) * (V[2][0] + V[3][0]);
     const double ca = double(1) / double(2) * (cL + cR);
-    // Characteristic field f at stencil point m (forward expression trees).
-    auto field = [&](int f, int m) -> double {
-        if (f == 0) return -double(1) / double(2) * ra * ca * V[m][1 + A] + double(1) / double(2) * V[m][IP];
-        if (f == 1) return V[m][0] - double(1) / (ca * ca) * V[m][IP];
-        if (f == IP) return double(1) / double(2) * ra * ca * V[m][1 + A] + double(1) / double(2) * V[m][IP];
-        int t = 2;
-        for (int b = 0; b < D; b++) {
-            if (b == A) continue;
-            if (t == f) return V[m][1 + b];
-            t++;
-        }
-        return 0.0;
-    };
     double Wm[NEQ], Wp[NEQ];
 #pragma unroll
     for (int f = 0; f < NEQ; f++) {
         Six s;
 #pragma unroll
-        for (int m = 0; m < 6; m++) s.v[m] = field(f, m);
+        for (int m = 0; m < 6; m++) s.v[m] = char_field<D, A>(V, ra, ca, f, m);
         reconstruct_pair<SCHEME, P, Q>(s, ld, Wm[f], Wp[f]);
     }
     double VL[NEQ], VR[NEQ];
@@ -627,13 +664,15 @@ HM_HD void face_adj_single(const double (&V)[6][D + 2], double cL, double cR, do
     if (!ok) {
 #pragma unroll
         for (int e = 0; e < NEQ; e++) {
-            Vb[2][e] += VLb[e];
-            Vb[3][e] += VRb[e];
+            h[e] = VLb[e];
+            h[NEQ + e] = VRb[e];
         }
+        h[2 * NEQ] = 0.0;
+        h[2 * NEQ + 1] = 0.0;
+        h[2 * NEQ + 2] = 0.0;
         return;
     }
     // Back-projection adjoint for both sides.
-    double Wmb[NEQ], Wpb[NEQ];
     double rab = 0.0, cab = 0.0;
     auto back_adj = [&](const double* W, const double* Vob, double* Wb) {
         const double ic2 = double(1) / (ca * ca);
@@ -650,36 +689,75 @@ HM_HD void face_adj_single(const double (&V)[6][D + 2], double cL, double cR, do
         cab += Vob[1 + A] * (W[IP] - W[0]) * (-irc / ca);
         rab += Vob[1 + A] * (W[IP] - W[0]) * (-irc / ra);
     };
+    double Wmb[NEQ], Wpb[NEQ];
     back_adj(Wm, VLb, Wmb);
     back_adj(Wp, VRb, Wpb);
-    // Reconstruction adjoint, field by field, then the characteristic projection.
 #pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        h[e] = Wmb[e];
+        h[NEQ + e] = Wpb[e];
+    }
+    h[2 * NEQ] = rab;
+    h[2 * NEQ + 1] = cab;
+    h[2 * NEQ + 2] = 1.0;
+}
+
+// Second half: given the hand-over h (read through hat(c)) and the stencil's
+// primitive variables (read through vat(m, e), so that they need not stay in
+// registers), the reverse reconstruction and characteristic projection, one
+// field at a time (a runtime loop, so only one field's WENO adjoint is live),
+// accumulating into Vb, cLb, cRb.
+template <int D, int A, int SCHEME, int P, int Q, class VAt, class HAt>
+HM_HD void face_adj_single_recon(const VAt& vat, double cL, double cR, const LDConst& ld, const HAt& hat,
+                                 double (&Vb)[6][D + 2], double& cLb, double& cRb) {
+    constexpr int NEQ = D + 2;
+    constexpr int IP = D + 1;
+    if (hat(2 * NEQ + 2) == 0.0) {
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) {
+            Vb[2][e] += hat(e);
+            Vb[3][e] += hat(NEQ + e);
+        }
+        return;
+    }
+    const double ra = double(1) / double(2) * (vat(2, 0) + vat(3, 0));
+    const double ca = double(1) / double(2) * (cL + cR);
+    double rab = hat(2 * NEQ), cab = hat(2 * NEQ + 1);
+#pragma unroll 1
     for (int f = 0; f < NEQ; f++) {
         Six s;
 #pragma unroll
-        for (int m = 0; m < 6; m++) s.v[m] = field(f, m);
+        for (int m = 0; m < 6; m++) s.v[m] = char_field_at<D, A>(vat, ra, ca, f, m);
         double sb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        reconstruct_pair_adj<SCHEME, P, Q>(s, ld, Wmb[f], Wpb[f], sb);
+        reconstruct_pair_adj<SCHEME, P, Q>(s, ld, hat(f), hat(NEQ + f), sb);
+        if (f == 0 || f == IP) {
+            const double sgn_ = (f == 0) ? -0.5 : 0.5;
 #pragma unroll
-        for (int m = 0; m < 6; m++) {
-            const double g = sb[m];
-            if (f == 0 || f == IP) {
-                const double sgn_ = (f == 0) ? -0.5 : 0.5;
+            for (int m = 0; m < 6; m++) {
+                const double g = sb[m];
                 Vb[m][1 + A] += g * sgn_ * ra * ca;
                 Vb[m][IP] += 0.5 * g;
-                rab += g * sgn_ * ca * V[m][1 + A];
-                cab += g * sgn_ * ra * V[m][1 + A];
-            } else if (f == 1) {
+                rab += g * sgn_ * ca * vat(m, 1 + A);
+                cab += g * sgn_ * ra * vat(m, 1 + A);
+            }
+        } else if (f == 1) {
+#pragma unroll
+            for (int m = 0; m < 6; m++) {
+                const double g = sb[m];
                 Vb[m][0] += g;
                 Vb[m][IP] -= g / (ca * ca);
-                cab += g * V[m][IP] * 2.0 / (ca * ca * ca);
-            } else {
-                int t = 2;
-                for (int b = 0; b < D; b++) {
-                    if (b == A) continue;
-                    if (t == f) Vb[m][1 + b] += g;
-                    t++;
+                cab += g * vat(m, IP) * 2.0 / (ca * ca * ca);
+            }
+        } else {
+            int t = 2;
+#pragma unroll
+            for (int b = 0; b < D; b++) {
+                if (b == A) continue;
+                if (t == f) {
+#pragma unroll
+                    for (int m = 0; m < 6; m++) Vb[m][1 + b] += sb[m];
                 }
+                t++;
             }
         }
     }
@@ -687,6 +765,17 @@ HM_HD void face_adj_single(const double (&V)[6][D + 2], double cL, double cR, do
     Vb[3][0] += 0.5 * rab;
     cLb += 0.5 * cab;
     cRb += 0.5 * cab;
+}
+
+// Adjoint of the whole midpoint-flux computation of one face along axis A
+// (both halves): adds d(flux)/dV * mb to Vb and d(flux)/dc * mb to cLb, cRb.
+template <int D, int A, int SCHEME, int P, int Q>
+HM_HD void face_adj_single(const double (&V)[6][D + 2], double cL, double cR, double gam, const LDConst& ld,
+                           bool use_hll, const double* mb, double (&Vb)[6][D + 2], double& cLb, double& cRb) {
+    double h[face_adj_h_size<D>()];
+    face_adj_single_flux<D, A, SCHEME, P, Q>(V, cL, cR, gam, ld, use_hll, mb, h);
+    face_adj_single_recon<D, A, SCHEME, P, Q>([&](int m, int e) { return V[m][e]; }, cL, cR, ld,
+                                              [&](int c) { return h[c]; }, Vb, cLb, cRb);
 }
 
 }  // namespace hm
