@@ -8,6 +8,10 @@
 #include "hm_ctx.hpp"
 #include "hm_tables.hpp"
 
+#ifndef HM_RECON_WAVES
+#define HM_RECON_WAVES 1
+#endif
+
 namespace hm {
 
 // ----------------------------------------------------------------------------
@@ -163,7 +167,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_face_flux(W w, KParams k) {
 }
 
 template <int D, int SCHEME, int PP, int QQ, class W>
-__global__ __launch_bounds__(kBlock) void k_adj_face_recon(W w, KParams k) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HM_RECON_WAVES))) void k_adj_face_recon(
+    W w, KParams k) {
     const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     const AdjPatch* P;
     int ax;

@@ -264,46 +264,45 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
     const double ra = double(1) / double(2) * (ld(NEQ + 1, 2) + ld(NEQ + 1, 3));
     const double ca = double(1) / double(2) * (ld(NEQ, 2) + ld(NEQ, 3));
 
-    // Characteristic fields (FlowModelBasicUtilitiesFiveEqnAllaire.cpp:8232-9178), one at a time.
+    // Characteristic fields (FlowModelBasicUtilitiesFiveEqnAllaire.cpp:8232-9178),
+    // one at a time: a runtime loop keeps a single field's WENO interpolation
+    // live, so the kernel fits the register file with room for more waves.
     const LDConst ldk{k.p, k.q, k.C, k.alpha_tau};
     double Wm[NEQ], Wp[NEQ];
-    {
+#pragma unroll 1
+    for (int f = 0; f < NEQ; f++) {
         Six s;
+        if (f == 0) {
 #pragma unroll
-        for (int m = 0; m < 6; m++) s.v[m] = ld(IU, m) - double(1) / (ra * ca) * ld(IP, m);
-        reconstruct_pair<SCHEME, PP, QQ>(s, ldk, Wm[0], Wp[0]);
-    }
+            for (int m = 0; m < 6; m++) s.v[m] = ld(IU, m) - double(1) / (ra * ca) * ld(IP, m);
+        } else if (f <= NS) {
+            const int si = f - 1;
+            double z = Zra[0];
 #pragma unroll
-    for (int si = 0; si < NS; si++) {
-        Six s;
+            for (int q = 1; q < NS; q++)
+                if (q == si) z = Zra[q];
 #pragma unroll
-        for (int m = 0; m < 6; m++) s.v[m] = ld(si, m) - Zra[si] / (ra * ca * ca) * ld(IP, m);
-        reconstruct_pair<SCHEME, PP, QQ>(s, ldk, Wm[1 + si], Wp[1 + si]);
-    }
-    {
-        int t = NS + 1;
-#pragma unroll
-        for (int b = 0; b < D; b++) {
-            if (b == A) continue;
-            Six s;
+            for (int m = 0; m < 6; m++) s.v[m] = ld(si, m) - z / (ra * ca * ca) * ld(IP, m);
+        } else if (f < NS + D) {
+            const int t = f - NS - 1;             // transverse slot
+            const int b = (t < A) ? t : t + 1;    // its axis
 #pragma unroll
             for (int m = 0; m < 6; m++) s.v[m] = ld(NS + b, m);
-            reconstruct_pair<SCHEME, PP, QQ>(s, ldk, Wm[t], Wp[t]);
-            t++;
+        } else if (f < NEQ - 1) {
+#pragma unroll
+            for (int m = 0; m < 6; m++) s.v[m] = ld(IZ + (f - NS - D), m);
+        } else {
+#pragma unroll
+            for (int m = 0; m < 6; m++) s.v[m] = ld(IU, m) + double(1) / (ra * ca) * ld(IP, m);
         }
-    }
+        double wm, wp;
+        reconstruct_pair<SCHEME, PP, QQ>(s, ldk, wm, wp);
 #pragma unroll
-    for (int si = 0; si < NZ; si++) {
-        Six s;
-#pragma unroll
-        for (int m = 0; m < 6; m++) s.v[m] = ld(IZ + si, m);
-        reconstruct_pair<SCHEME, PP, QQ>(s, ldk, Wm[NS + D + si], Wp[NS + D + si]);
-    }
-    {
-        Six s;
-#pragma unroll
-        for (int m = 0; m < 6; m++) s.v[m] = ld(IU, m) + double(1) / (ra * ca) * ld(IP, m);
-        reconstruct_pair<SCHEME, PP, QQ>(s, ldk, Wm[NEQ - 1], Wp[NEQ - 1]);
+        for (int e = 0; e < NEQ; e++)
+            if (e == f) {
+                Wm[e] = wm;
+                Wp[e] = wp;
+            }
     }
 
     // Back-projection (FlowModelBasicUtilitiesFiveEqnAllaire.cpp:9201-9974).

@@ -157,8 +157,19 @@ template <int SCHEME, int P, int Q>
 HM_HD void reconstruct_pair(const Six& s, const LDConst& k, double& um, double& up) {
     if constexpr (SCHEME == kWCNS6_LD) {
         const double sig = ld_sigma(s);
-        um = ld_side<P, Q>(s, sig, k);
-        up = ld_side<P, Q>(flip(s), sig, k);
+        // The two sides one after the other (a runtime loop), so that only one
+        // side's indicators and weights are live at a time.
+#pragma unroll 1
+        for (int side = 0; side < 2; side++) {
+            Six ss;
+#pragma unroll
+            for (int m = 0; m < 6; m++) ss.v[m] = side ? s.v[5 - m] : s.v[m];
+            const double r = ld_side<P, Q>(ss, sig, k);
+            if (side)
+                up = r;
+            else
+                um = r;
+        }
     } else {
         um = js_side<P>(s, k.p);
         up = js_side<P>(flip(s), k.p);
