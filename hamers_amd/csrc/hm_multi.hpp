@@ -25,6 +25,11 @@
 #include "hm_numerics.hpp"
 #include "hm_sensor.hpp"
 
+// Waves per SIMD the multi-species midpoint-flux kernel is compiled for.
+#ifndef HM_MIDFLUX_MULTI_WAVES
+#define HM_MIDFLUX_MULTI_WAVES 3
+#endif
+
 namespace hm {
 
 // Equation layout of a multi-species model.
@@ -495,7 +500,8 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
 }
 
 template <int D, int NS, bool FOUR, int SCHEME, int PP, int QQ, class W>
-__global__ __launch_bounds__(kBlock) void k_midflux_multi(W w, KParams k) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HM_MIDFLUX_MULTI_WAVES))) void k_midflux_multi(
+    W w, KParams k) {
     const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     const DevPatch* P;
     int ax;
