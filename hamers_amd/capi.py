@@ -76,6 +76,8 @@ class hm_rk_patch_view(ctypes.Structure):
         ("S_intermediate", ctypes.c_void_p * HM_MAX_RK_COEFFS),
         ("F_accumulated", ctypes.c_void_p * 3),
         ("S_accumulated", ctypes.c_void_p),
+        ("F_diffusive_intermediate", (ctypes.c_void_p * 3) * HM_MAX_RK_COEFFS),
+        ("F_diffusive_accumulated", ctypes.c_void_p * 3),
     ]
 
 

@@ -1055,14 +1055,23 @@ int advance_single_step(ConvectiveFluxReconstructorGPU* rec, const hm_rk_patch_v
                 return rec->fail(HM_ERR_INVALID_ARGUMENT, msg);
             }
             R.Si[m] = v.S_intermediate[m];
+            // Navier-Stokes form: diffusive fluxes on every axis or on none.
+            int nd = 0;
+            for (int a = 0; a < d; a++) nd += v.F_diffusive_intermediate[m][a] ? 1 : 0;
+            if (nd != 0 && nd != d) {
+                std::snprintf(msg, sizeof msg, "patch %d: intermediate %d diffusive flux given on some axes only", p, m);
+                return rec->fail(HM_ERR_INVALID_ARGUMENT, msg);
+            }
+            for (int a = 0; a < 3; a++) R.Fdi[m][a] = (a < d && nd) ? v.F_diffusive_intermediate[m][a] : nullptr;
         }
         for (int a = 0; a < 3; a++) R.Facc[a] = a < d ? v.F_accumulated[a] : nullptr;
+        for (int a = 0; a < 3; a++) R.Fdacc[a] = a < d ? v.F_diffusive_accumulated[a] : nullptr;
         R.Sacc = v.S_accumulated;
         cells.add(p, 0, ivol);
         acc.items[p] = R;
         if (gamma) {
             for (int a = 0; a < d; a++)
-                if (R.Facc[a]) {
+                if (R.Facc[a] || R.Fdacc[a]) {
                     acc.add(p, a, int64_t(R.n[0] + (a == 0)) * (R.n[1] + (a == 1)) * (R.n[2] + (a == 2)));
                     any_acc = true;
                 }

@@ -64,16 +64,20 @@ __global__ __launch_bounds__(kBlock) void k_rk_update(W w, RKParams r) {
         for (int n = 0; n < r.nc; n++) {
             if (r.alpha[n] != 0.0) q += r.alpha[n] * P->Qi[n][e][ii];
             if (r.beta[n] != 0.0) {
-                const double* Fx = P->Fi[n][0] + e * fvol[0];
-                double t = -(Fx[fR[0]] - Fx[fL[0]]) / P->dx[0];
-                if constexpr (D > 1) {
-                    const double* Fy = P->Fi[n][1] + e * fvol[1];
-                    t = t - (Fy[fR[1]] - Fy[fL[1]]) / P->dx[1];
-                }
-                if constexpr (D > 2) {
-                    const double* Fz = P->Fi[n][2] + e * fvol[2];
-                    t = t - (Fz[fR[2]] - Fz[fL[2]]) / P->dx[2];
-                }
+                // Difference of the face fluxes on axis a: (F_R - F_L), or for the
+                // Navier-Stokes form (F_R - F_L + Fd_R - Fd_L) in the reference's order.
+                auto diff = [&](int a) {
+                    const double* Fa = P->Fi[n][a] + e * fvol[a];
+                    double dF = Fa[fR[a]] - Fa[fL[a]];
+                    if (P->Fdi[n][0]) {
+                        const double* Fd = P->Fdi[n][a] + e * fvol[a];
+                        dF = dF + Fd[fR[a]] - Fd[fL[a]];
+                    }
+                    return dF;
+                };
+                double t = -diff(0) / P->dx[0];
+                if constexpr (D > 1) t = t - diff(1) / P->dx[1];
+                if constexpr (D > 2) t = t - diff(2) / P->dx[2];
                 q += r.beta[n] * (t + P->Si[n][e * svol + loc]);
                 snap = q;
             }
@@ -101,14 +105,24 @@ __global__ __launch_bounds__(kBlock) void k_rk_accumulate(W w, RKParams r) {
         vol = int64_t(P->n[0] + (ax == 0)) * (n1 + (ax == 1)) * (n2 + (ax == 2));
     for (int e = 0; e < r.neq; e++) {
         double* acc = (ax == D) ? P->Sacc : P->Facc[ax];
-        if (!acc) return;
-        double v = acc[e * vol + loc];
-        for (int n = 0; n < r.nc; n++) {
-            if (r.gamma[n] == 0.0) continue;
-            const double* src = (ax == D) ? P->Si[n] : P->Fi[n][ax];
-            v += r.gamma[n] * src[e * vol + loc];
+        if (acc) {
+            double v = acc[e * vol + loc];
+            for (int n = 0; n < r.nc; n++) {
+                if (r.gamma[n] == 0.0) continue;
+                const double* src = (ax == D) ? P->Si[n] : P->Fi[n][ax];
+                v += r.gamma[n] * src[e * vol + loc];
+            }
+            acc[e * vol + loc] = v;
         }
-        acc[e * vol + loc] = v;
+        double* dacc = (ax == D) ? nullptr : P->Fdacc[ax];
+        if (dacc) {
+            double v = dacc[e * vol + loc];
+            for (int n = 0; n < r.nc; n++) {
+                if (r.gamma[n] == 0.0 || !P->Fdi[n][ax]) continue;
+                v += r.gamma[n] * P->Fdi[n][ax][e * vol + loc];
+            }
+            dacc[e * vol + loc] = v;
+        }
     }
 }
 

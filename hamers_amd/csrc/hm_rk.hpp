@@ -20,6 +20,8 @@ struct RKPatch {
     const double* Si[kMaxRK];
     double* Facc[3];                     // optional gamma accumulation (NULL to skip)
     double* Sacc;
+    const double* Fdi[kMaxRK][3];        // Navier-Stokes: intermediate diffusive fluxes (NULL: Euler form)
+    double* Fdacc[3];                    // Navier-Stokes: gamma accumulation of the diffusive flux
 };
 
 struct RKParams {

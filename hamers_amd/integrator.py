@@ -1,7 +1,8 @@
 """Python mirror of the Runge-Kutta stage update and of one level's stage loop.
 
 ``advanceSingleStepOnPatch`` mirrors ``Euler::advanceSingleStepOnPatch``
-(src/apps/Euler/Euler.cpp:1003-1680, the zero fill of
+(src/apps/Euler/Euler.cpp:1003-1680; with diffusive fluxes
+``NavierStokes::advanceSingleStepOnPatch``, NavierStokes.cpp:1003-2400; the zero fill of
 ``RungeKuttaLevelIntegrator::advanceLevel`` included) and ``Level`` runs the
 stage loop of ``RungeKuttaLevelIntegrator::advanceLevel``
 (src/algs/integrator/RungeKuttaLevelIntegrator.cpp:1457-1939) for one level of
@@ -55,6 +56,8 @@ class RKPatchData:
     S_intermediate: list
     F_accumulated: list = None
     S_accumulated: object = None
+    Fd_intermediate: list = None   # Navier-Stokes: per coefficient, per-axis diffusive fluxes
+    Fd_accumulated: list = None
     _keep: list = field(default_factory=list, repr=False)
 
     def view(self, num_arrays: int, num_eqn: int) -> capi.hm_rk_patch_view:
@@ -83,6 +86,11 @@ class RKPatchData:
         for a, t in enumerate(self.F_accumulated or []):
             v.F_accumulated[a] = _ptr(t)
         v.S_accumulated = _ptr(self.S_accumulated)
+        for m, Fm in enumerate(self.Fd_intermediate or []):
+            for a, t in enumerate(Fm or []):
+                v.F_diffusive_intermediate[m][a] = _ptr(t)
+        for a, t in enumerate(self.Fd_accumulated or []):
+            v.F_diffusive_accumulated[a] = _ptr(t)
         return v
 
 

@@ -251,6 +251,11 @@ typedef struct hm_rk_patch_view {
     const double* S_intermediate[HM_MAX_RK_COEFFS];
     double* F_accumulated[3];
     double* S_accumulated;
+    /* Navier-Stokes stage (NavierStokes.cpp:1003-2400): the intermediate
+     * diffusive fluxes of each coefficient (NULL: the Euler form) and the
+     * scratch diffusive flux receiving the gamma-weighted sums (NULL: skip). */
+    const double* F_diffusive_intermediate[HM_MAX_RK_COEFFS][3];
+    double* F_diffusive_accumulated[3];
 } hm_rk_patch_view;
 
 /*
@@ -261,6 +266,9 @@ typedef struct hm_rk_patch_view {
  * fraction = 1 - sum of the others after every nonzero beta
  * (FlowModelFiveEqnAllaire.cpp:1739-1890), and F_acc += gamma[n] F_int[n],
  * S_acc += gamma[n] S_int[n].  num_coeffs <= HM_MAX_RK_COEFFS; gamma may be NULL.
+ * With diffusive fluxes given for a coefficient, its term is the Navier-Stokes
+ * one, -sum_a (F_a,R - F_a,L + Fd_a,R - Fd_a,L)/dx_a + S (NavierStokes.cpp:1743-1748,
+ * 2085-2092), and Fd_acc += gamma[n] Fd_int[n].
  */
 int hm_advance_single_step_on_patch(hm_ctx* ctx, const hm_rk_patch_view* patch, int num_coeffs,
                                     const double* alpha, const double* beta, const double* gamma, void* stream);
