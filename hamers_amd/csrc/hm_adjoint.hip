@@ -9,7 +9,7 @@
 #include "hm_tables.hpp"
 
 #ifndef HM_RECON_WAVES
-#define HM_RECON_WAVES 1
+#define HM_RECON_WAVES 2
 #endif
 
 namespace hm {
@@ -107,23 +107,19 @@ __device__ __forceinline__ void adj_face_flux(const AdjPatch& P, int64_t loc, co
     for (int c = 0; c < NH; c++) H[c * mv + loc] = h[c];
 }
 
-// Second half: reverse reconstruction and characteristic projection; writes
-// the six stencil cells' adjoints (and the two sound-speed adjoints) to G.
+// Second half: the reverse reconstruction; writes the face record G (the
+// characteristic-value adjoints of the six stencil points per field and the
+// projection averages' adjoints; see face_adj_single_recon).  Faces without
+// adjoint (H flag 2) write nothing: the gather skips them.
 template <int D, int A, int SCHEME, int PP, int QQ>
 __device__ __forceinline__ void adj_face_recon(const AdjPatch& P, int64_t loc, const KParams& k) {
     constexpr int NEQ = D + 2;
-    constexpr int NG = 6 * NEQ + 2;
     constexpr int NH = face_adj_h_size<D>();
     const DevPatch& F = P.f;
     const Boxes<D> B(F);
     const int64_t mv = B.mid_vol(A);
     const double* H = P.H[A] + loc;
-    double* G = P.G[A];
-    if (H[(NH - 1) * mv] == 2.0) {
-#pragma unroll
-        for (int c = 0; c < NG; c++) G[c * mv + loc] = 0.0;
-        return;
-    }
+    if (H[(NH - 1) * mv] == 2.0) return;
     int c0, c1, c2;
     mid_cell<D, A>(B, loc, c0, c1, c2);
     const int64_t v4 = B.vol(kGc);
@@ -131,21 +127,10 @@ __device__ __forceinline__ void adj_face_recon(const AdjPatch& P, int64_t loc, c
     const double* pc = F.prim + B.idx(kGc, c0, c1, c2);  // cell right of the face
     const double cL = pc[NEQ * v4 - st];
     const double cR = pc[NEQ * v4];
-    double Vb[6][NEQ];
-#pragma unroll
-    for (int m = 0; m < 6; m++)
-#pragma unroll
-        for (int e = 0; e < NEQ; e++) Vb[m][e] = 0.0;
-    double cLb = 0.0, cRb = 0.0;
+    double* G = P.G[A] + loc;
     face_adj_single_recon<D, A, SCHEME, PP, QQ>([&](int m, int e) { return pc[e * v4 + (m - 3) * st]; }, cL, cR,
                                                 LDConst{k.p, k.q, k.C, k.alpha_tau}, [&](int c) { return H[c * mv]; },
-                                                Vb, cLb, cRb);
-#pragma unroll
-    for (int m = 0; m < 6; m++)
-#pragma unroll
-        for (int e = 0; e < NEQ; e++) G[(m * NEQ + e) * mv + loc] = Vb[m][e];
-    G[(6 * NEQ) * mv + loc] = cLb;
-    G[(6 * NEQ + 1) * mv + loc] = cRb;
+                                                [&](int c, double v) { G[c * mv] = v; });
 }
 
 template <int D, int SCHEME, int PP, int QQ, class W>
@@ -233,16 +218,29 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(W w, KParams k) {
         if (!trans_ok) continue;
         const int na = B.n[a];
         const int64_t mv = B.mid_vol(a);
-        const double* G = P->G[a];
+        const int64_t sta = (a == 0) ? 1 : (a == 1) ? int64_t(e0) : int64_t(e0) * e1;
+        constexpr int NH = face_adj_h_size<D>();
 #pragma unroll
         for (int m = 0; m < 6; m++) {
             const int jf = c[a] + 3 - m;
             if (jf < -1 || jf > na + 1) continue;
             const int64_t fi = B.mid_idx(a, (a == 0) ? jf : c[0], (a == 1) ? jf : c[1], (a == 2) ? jf : c[2]);
-#pragma unroll
-            for (int e = 0; e < NEQ; e++) Vb[e] += G[(m * NEQ + e) * mv + fi];
-            if (m == 2) cb += G[(6 * NEQ) * mv + fi];
-            if (m == 3) cb += G[(6 * NEQ + 1) * mv + fi];
+            const double flag = P->H[a][(NH - 1) * mv + fi];
+            if (flag == 2.0) continue;
+            // projection averages of the face: its cells sit at stencil points 2 and 3
+            const int64_t l2 = loc + (2 - m) * sta, l3 = loc + (3 - m) * sta;
+            const double ra = double(1) / double(2) * (F.prim[l2] + F.prim[l3]);
+            const double ca = double(1) / double(2) * (F.prim[NEQ * v4 + l2] + F.prim[NEQ * v4 + l3]);
+            const double* G = P->G[a] + fi;
+            auto gat = [&](int cc) { return G[cc * mv]; };
+            if (a == 0) {
+                face_record_to_point<D, 0>(flag, gat, ra, ca, m, Vb, cb);
+            } else if constexpr (D > 1) {
+                if (a == 1)
+                    face_record_to_point<D, 1>(flag, gat, ra, ca, m, Vb, cb);
+                else if constexpr (D > 2)
+                    face_record_to_point<D, 2>(flag, gat, ra, ca, m, Vb, cb);
+            }
         }
         // Node flux of this cell enters the faces c_a and c_a + 1 with weight -3/10 dt.
         if (c[a] >= -1 && c[a] <= na) {

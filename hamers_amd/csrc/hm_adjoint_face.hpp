@@ -224,36 +224,31 @@ HM_HD void js_side_adj(const Six& s, int p_rt, double ob, double* sb) {
 template <int SCHEME, int P, int Q>
 HM_HD void reconstruct_pair_adj(const Six& s, const LDConst& k, double umb, double upb, double* sb) {
     if (umb == 0.0 && upb == 0.0) return;
-    const Six f = flip(s);
-    double fb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if constexpr (SCHEME == kWCNS6_LD) {
         const double sig = ld_sigma(s);
         double sigb = 0.0;
         // The two sides one after the other (a runtime loop): their reverse
         // sweeps are independent, and interleaved they would not fit the
-        // register file.
+        // register file.  The plus side works on the mirrored stencil.
 #pragma unroll 1
         for (int side = 0; side < 2; side++) {
             Six ss;
 #pragma unroll
-            for (int m = 0; m < 6; m++) ss.v[m] = side ? f.v[m] : s.v[m];
+            for (int m = 0; m < 6; m++) ss.v[m] = side ? s.v[5 - m] : s.v[m];
             double tb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
             ld_side_adj<P, Q>(ss, sig, k, side ? upb : umb, tb, sigb);
-            if (side) {
 #pragma unroll
-                for (int m = 0; m < 6; m++) fb[m] += tb[m];
-            } else {
-#pragma unroll
-                for (int m = 0; m < 6; m++) sb[m] += tb[m];
-            }
+            for (int m = 0; m < 6; m++) sb[m] += side ? tb[5 - m] : tb[m];
         }
         ld_sigma_adj(s, sigb, sb);
     } else {
+        const Six f = flip(s);
+        double fb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         js_side_adj<P>(s, k.p, umb, sb);
         js_side_adj<P>(f, k.p, upb, fb);
-    }
 #pragma unroll
-    for (int m = 0; m < 6; m++) sb[m] += fb[5 - m];
+        for (int m = 0; m < 6; m++) sb[m] += fb[5 - m];
+    }
 }
 
 // Adjoint of the single-species HLLC / HLLC-HLL midpoint flux of one face
@@ -702,22 +697,30 @@ HM_HD void face_adj_single_flux(const double (&V)[6][D + 2], double cL, double c
     h[2 * NEQ + 2] = 1.0;
 }
 
+// Size of the per-face record the second half writes.
+template <int D>
+constexpr int face_adj_g_size() {
+    return 6 * (D + 2) + 2;
+}
+
 // Second half: given the hand-over h (read through hat(c)) and the stencil's
 // primitive variables (read through vat(m, e), so that they need not stay in
-// registers), the reverse reconstruction and characteristic projection, one
-// field at a time (a runtime loop, so only one field's WENO adjoint is live),
-// accumulating into Vb, cLb, cRb.
-template <int D, int A, int SCHEME, int P, int Q, class VAt, class HAt>
+// registers), the reverse reconstruction, one field at a time (a runtime
+// loop, so only one field's WENO adjoint is live).  Writes the face record
+// through gout(c, value):
+//   reconstructed face (h flag 1): g[f*6 + m] = adjoint of characteristic value
+//     f at stencil point m, g[6 NEQ] = adjoint of rho_avg, g[6 NEQ + 1] = of c_avg;
+//   fallback face (h flag 0): g[e] = adjoint of V_L, g[NEQ + e] = of V_R.
+// The characteristic projection's adjoint is applied per stencil point by
+// face_record_to_point (in the gather), so no per-point accumulators are live here.
+template <int D, int A, int SCHEME, int P, int Q, class VAt, class HAt, class GOut>
 HM_HD void face_adj_single_recon(const VAt& vat, double cL, double cR, const LDConst& ld, const HAt& hat,
-                                 double (&Vb)[6][D + 2], double& cLb, double& cRb) {
+                                 const GOut& gout) {
     constexpr int NEQ = D + 2;
     constexpr int IP = D + 1;
     if (hat(2 * NEQ + 2) == 0.0) {
 #pragma unroll
-        for (int e = 0; e < NEQ; e++) {
-            Vb[2][e] += hat(e);
-            Vb[3][e] += hat(NEQ + e);
-        }
+        for (int e = 0; e < 2 * NEQ; e++) gout(e, hat(e));
         return;
     }
     const double ra = double(1) / double(2) * (vat(2, 0) + vat(3, 0));
@@ -734,48 +737,75 @@ HM_HD void face_adj_single_recon(const VAt& vat, double cL, double cR, const LDC
             const double sgn_ = (f == 0) ? -0.5 : 0.5;
 #pragma unroll
             for (int m = 0; m < 6; m++) {
-                const double g = sb[m];
-                Vb[m][1 + A] += g * sgn_ * ra * ca;
-                Vb[m][IP] += 0.5 * g;
-                rab += g * sgn_ * ca * vat(m, 1 + A);
-                cab += g * sgn_ * ra * vat(m, 1 + A);
+                rab += sb[m] * sgn_ * ca * vat(m, 1 + A);
+                cab += sb[m] * sgn_ * ra * vat(m, 1 + A);
             }
         } else if (f == 1) {
 #pragma unroll
-            for (int m = 0; m < 6; m++) {
-                const double g = sb[m];
-                Vb[m][0] += g;
-                Vb[m][IP] -= g / (ca * ca);
-                cab += g * vat(m, IP) * 2.0 / (ca * ca * ca);
-            }
-        } else {
-            int t = 2;
-#pragma unroll
-            for (int b = 0; b < D; b++) {
-                if (b == A) continue;
-                if (t == f) {
-#pragma unroll
-                    for (int m = 0; m < 6; m++) Vb[m][1 + b] += sb[m];
-                }
-                t++;
-            }
+            for (int m = 0; m < 6; m++) cab += sb[m] * vat(m, IP) * 2.0 / (ca * ca * ca);
         }
+#pragma unroll
+        for (int m = 0; m < 6; m++) gout(f * 6 + m, sb[m]);
     }
-    Vb[2][0] += 0.5 * rab;
-    Vb[3][0] += 0.5 * rab;
-    cLb += 0.5 * cab;
-    cRb += 0.5 * cab;
+    gout(6 * NEQ, rab);
+    gout(6 * NEQ + 1, cab);
+}
+
+// Share of stencil point m (0..5, cells i-3..i+2) of a face record (read
+// through gat(c)): the characteristic projection's adjoint, added to Vbm (the
+// point's primitive-variable adjoint) and cb (its sound-speed adjoint).  flag
+// is the hand-over's last entry (1 reconstructed, 0 fallback); ra, ca are the
+// face's projection averages.
+template <int D, int A, class GAt>
+HM_HD void face_record_to_point(double flag, const GAt& gat, double ra, double ca, int m, double* Vbm, double& cb) {
+    constexpr int NEQ = D + 2;
+    constexpr int IP = D + 1;
+    if (flag == 0.0) {
+        if (m == 2)
+            for (int e = 0; e < NEQ; e++) Vbm[e] += gat(e);
+        if (m == 3)
+            for (int e = 0; e < NEQ; e++) Vbm[e] += gat(NEQ + e);
+        return;
+    }
+    const double g0 = gat(m), gl = gat(IP * 6 + m), g1 = gat(6 + m);
+    Vbm[1 + A] += g0 * -0.5 * ra * ca;
+    Vbm[IP] += 0.5 * g0;
+    Vbm[0] += g1;
+    Vbm[IP] -= g1 / (ca * ca);
+    int t = 2;
+    for (int b = 0; b < D; b++) {
+        if (b == A) continue;
+        Vbm[1 + b] += gat(t * 6 + m);
+        t++;
+    }
+    Vbm[1 + A] += gl * 0.5 * ra * ca;
+    Vbm[IP] += 0.5 * gl;
+    if (m == 2 || m == 3) {
+        Vbm[0] += 0.5 * gat(6 * NEQ);
+        cb += 0.5 * gat(6 * NEQ + 1);
+    }
 }
 
 // Adjoint of the whole midpoint-flux computation of one face along axis A
-// (both halves): adds d(flux)/dV * mb to Vb and d(flux)/dc * mb to cLb, cRb.
+// (both halves and the per-point projection adjoint): adds d(flux)/dV * mb to
+// Vb and d(flux)/dc * mb to cLb, cRb.
 template <int D, int A, int SCHEME, int P, int Q>
 HM_HD void face_adj_single(const double (&V)[6][D + 2], double cL, double cR, double gam, const LDConst& ld,
                            bool use_hll, const double* mb, double (&Vb)[6][D + 2], double& cLb, double& cRb) {
     double h[face_adj_h_size<D>()];
     face_adj_single_flux<D, A, SCHEME, P, Q>(V, cL, cR, gam, ld, use_hll, mb, h);
+    double g[face_adj_g_size<D>()];
+    for (int c = 0; c < face_adj_g_size<D>(); c++) g[c] = 0.0;
     face_adj_single_recon<D, A, SCHEME, P, Q>([&](int m, int e) { return V[m][e]; }, cL, cR, ld,
-                                              [&](int c) { return h[c]; }, Vb, cLb, cRb);
+                                              [&](int c) { return h[c]; }, [&](int c, double v) { g[c] = v; });
+    const double ra = double(1) / double(2) * (V[2][0] + V[3][0]);
+    const double ca = double(1) / double(2) * (cL + cR);
+    for (int m = 0; m < 6; m++) {
+        double cb = 0.0;
+        face_record_to_point<D, A>(h[2 * (D + 2) + 2], [&](int c) { return g[c]; }, ra, ca, m, Vb[m], cb);
+        if (m == 2) cLb += cb;
+        if (m == 3) cRb += cb;
+    }
 }
 
 }  // namespace hm
