@@ -447,7 +447,11 @@ int AdjointSingle::build(const hm_vjp_patch_view* views, int np, Owned& out) {
     out.plan.n_cell = cell.total;
     o += cell.bytes();
     out.plan.qbox = qbox.write(h.data() + o, dv + o);
-    out.plan.n_qbox = qbox.total;
+    // k_adj_cells writes every cell of the 4-ghost box; only wider ghost boxes need the zero fill.
+    bool wider = false;
+    for (int i = 0; i < np; i++)
+        for (int a = 0; a < d; a++) wider = wider || fv[i].ghost[a] > kGc;
+    out.plan.n_qbox = wider ? qbox.total : 0;
     if (hipMemcpy(out.tables, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
         return rec_->fail(HM_ERR_DEVICE, "adjoint table upload");
     return HM_OK;

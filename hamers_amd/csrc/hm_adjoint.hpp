@@ -34,7 +34,7 @@ struct AdjPatch {
 
 struct AdjPlanDev {
     SegTableT<AdjPatch> face, cell, qbox;
-    int64_t n_face = 0, n_cell = 0, n_qbox = 0;
+    int64_t n_face = 0, n_cell = 0, n_qbox = 0;  // n_qbox 0: every Q box is the 4-ghost box (no zero fill)
 };
 
 hipError_t launch_adjoint_single(int dim, int scheme, const AdjPlanDev& plan, const KParams& k, hipStream_t s);

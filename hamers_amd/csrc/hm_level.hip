@@ -160,8 +160,11 @@ __global__ __launch_bounds__(kBlock) void k_halo_reverse(SegTableT<AdjLevelPatch
     const int j = (D > 1) ? int(r % P->n[1]) : 0;
     const int k = (D > 2) ? int(r / P->n[1]) : 0;
     const int64_t gi = gbox_idx<D>(P->n, g, i, j, k);
-    double acc[HM_MAX_EQN];
-    for (int e = 0; e < neq; e++) acc[e] = P->qbar[e * P->gvol + gi];
+    constexpr int NEQ = D + 2;  // the level adjoint is built for the single-species model
+    (void)neq;
+    double acc[NEQ];
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) acc[e] = P->qbar[e * P->gvol + gi];
     const int x[3] = {i + g, (D > 1) ? j + g : 0, (D > 2) ? k + g : 0};  // ghost-box coordinates
     for (int q = P->rev_first; q < P->rev_first + P->rev_count; q++) {
         const HaloItem& it = rev[q];
@@ -176,13 +179,15 @@ __global__ __launch_bounds__(kBlock) void k_halo_reverse(SegTableT<AdjLevelPatch
         const int e0 = Pd.n[0] + 2 * g;
         const int e1 = (D > 1) ? Pd.n[1] + 2 * g : 1;
         const int64_t di = y[0] + int64_t(y[1]) * e0 + int64_t(y[2]) * e0 * e1;
-        for (int e = 0; e < neq; e++) acc[e] += Pd.qbar[e * Pd.gvol + di];
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) acc[e] += Pd.qbar[e * Pd.gvol + di];
     }
-    for (int e = 0; e < neq; e++) P->lam[m][e * P->gvol + gi] += acc[e];
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) P->lam[m][e * P->gvol + gi] += acc[e];
 }
 
 // Interior updates of the stage adjoints: op 0: lam[dst] = 0; op 1: lam[dst] += a lam[src];
-// op 2: lam[dst] = lam[src].
+// op 2: lam[dst] = lam[src]; op 3: lam[0 .. dst) = 0.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_lam_op(SegTableT<AdjLevelPatch> w, int op, int dst, int src, double a,
                                                    int neq, int g) {
@@ -198,6 +203,11 @@ __global__ __launch_bounds__(kBlock) void k_lam_op(SegTableT<AdjLevelPatch> w, i
     const int64_t gi = gbox_idx<D>(P->n, g, i, j, k);
     double* L = P->lam[dst];
     const double* S = P->lam[src];
+    if (op == 3) {  // lam[0 .. dst) = 0
+        for (int mm = 0; mm < dst; mm++)
+            for (int e = 0; e < neq; e++) P->lam[mm][e * P->gvol + gi] = 0.0;
+        return;
+    }
     for (int e = 0; e < neq; e++) {
         const int64_t q = e * P->gvol + gi;
         if (op == 0)
@@ -752,8 +762,7 @@ class Level {
     // lam[nst] holds the adjoint of the step's output on entry and of its
     // input on return.
     int adjoint_step(int slot0, hipStream_t s) {
-        for (int m = 0; m < nst_; m++)
-            if (int st = lam_op(0, m, 0, 0.0, s)) return st;
+        if (int st = lam_op(3, nst_, 0, 0.0, s)) return st;
         for (int sn = nst_ - 1; sn >= 0; sn--) {
             for (int m = 0; m <= sn; m++) {
                 const double b = beta_[sn][m], a = alpha_[sn][m];
