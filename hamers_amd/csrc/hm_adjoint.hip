@@ -8,8 +8,14 @@
 #include "hm_ctx.hpp"
 #include "hm_tables.hpp"
 
+// Waves per SIMD the two face-adjoint kernels are compiled for (a few bytes
+// of spill at these settings cost less than the lower occupancy; the 3D flux
+// half would spill over 100 bytes at 3 waves and stays at 2).
 #ifndef HM_RECON_WAVES
 #define HM_RECON_WAVES 2
+#endif
+#ifndef HM_ADJ_FLUX_WAVES
+#define HM_ADJ_FLUX_WAVES 3
 #endif
 
 namespace hm {
@@ -134,7 +140,8 @@ __device__ __forceinline__ void adj_face_recon(const AdjPatch& P, int64_t loc, c
 }
 
 template <int D, int SCHEME, int PP, int QQ, class W>
-__global__ __launch_bounds__(kBlock) void k_adj_face_flux(W w, KParams k) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D < 3 ? HM_ADJ_FLUX_WAVES : 2))) void k_adj_face_flux(
+    W w, KParams k) {
     const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     const AdjPatch* P;
     int ax;

@@ -22,6 +22,26 @@ namespace hm {
 
 HM_HD double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
 
+// 1/b for the reverse sweep, to within a few ulps: on the device the
+// hardware estimate refined by two Newton-Raphson steps r <- r + r (1 - b r)
+// (each step squares the relative error).  The adjoint's outputs carry no
+// bit-exactness requirement, and a quotient by a shared denominator then costs
+// one multiply instead of a full correctly rounded division.  Quotients that
+// decide a branch (R_tau > alpha_tau, t1 >= t2, s* > 0, ...) keep the forward
+// code's correctly rounded division.  Valid for normal, finite, nonzero b
+// (every denominator here is a sum with a positive floor or a state value).
+HM_HD double adj_rcp(double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(e, r, r);
+    e = fma(-b, r, 1.0);
+    return fma(e, r, r);
+#else
+    return 1.0 / b;
+#endif
+}
+
 // d/dx of x^E (E >= 1) times g.
 template <int E>
 HM_HD double dpow_times(double x, int e_rt, double g) {
@@ -74,17 +94,19 @@ HM_HD void ld_sigma_adj(const Six& s, double sigb, double* sb) {
     const double x1 = a1 - a2, x2 = a2 - a3;
     const double den1 = fabs(a1) + fabs(a2) + kEps;
     const double den2 = fabs(a2) + fabs(a3) + kEps;
-    const double t1 = fabs(x1) / den1;
+    const double t1 = fabs(x1) / den1;  // the branch below compares them as the forward does
     const double t2 = fabs(x2) / den2;
     double a1b = 0.0, a2b = 0.0, a3b = 0.0;
     if (t1 >= t2) {
-        const double xb = sigb * sgn(x1) / den1;
-        const double db = -sigb * t1 / den1;
+        const double id = adj_rcp(den1);
+        const double xb = sigb * sgn(x1) * id;
+        const double db = -sigb * t1 * id;
         a1b += xb + db * sgn(a1);
         a2b += -xb + db * sgn(a2);
     } else {
-        const double xb = sigb * sgn(x2) / den2;
-        const double db = -sigb * t2 / den2;
+        const double id = adj_rcp(den2);
+        const double xb = sigb * sgn(x2) * id;
+        const double db = -sigb * t2 * id;
         a2b += xb + db * sgn(a2);
         a3b += -xb + db * sgn(a3);
     }
@@ -95,42 +117,70 @@ HM_HD void ld_sigma_adj(const Six& s, double sigb, double* sb) {
 }
 
 // Adjoint of ld_side<P,Q>(s, sigma, k): adds d(out)/ds * ob to sb and
-// d(out)/dsigma * ob to sigb.
+// d(out)/dsigma * ob to sigb.  The forward quantities are recomputed with
+// reciprocals of the shared denominators (e_k, the weight sums); only the
+// R_tau > alpha_tau predicate keeps the forward's division.
 template <int P, int Q>
 HM_HD void ld_side_adj(const Six& s, double sigma, const LDConst& k, double ob, double* sb, double& sigb) {
     if (ob == 0.0) return;
     double b0, b1, b2;
     beta3(s, b0, b1, b2);
     const double b3 = beta6(s);
-    const double d02 = b0 - b2;
-    const double tau5 = fabs(d02);
-    const double e0 = b0 + kEps, e1 = b1 + kEps, e2 = b2 + kEps, e3 = b3 + kEps;
-    const double r0 = tau5 / e0, r1 = tau5 / e1, r2 = tau5 / e2;
-    const double u0 = double(1) / double(16) * (double(1) + rep_pow<P>(r0, k.p));
-    const double u1 = double(5) / double(8) * (double(1) + rep_pow<P>(r1, k.p));
-    const double u2 = double(5) / double(16) * (double(1) + rep_pow<P>(r2, k.p));
-    const double us = u0 + u1 + u2;
-    const double uh0 = u0 / us, uh1 = u1 / us, uh2 = u2 / us;
-
+    const double ie0 = adj_rcp(b0 + kEps), ie1 = adj_rcp(b1 + kEps), ie2 = adj_rcp(b2 + kEps),
+                 ie3 = adj_rcp(b3 + kEps);
     const double bavg = double(1) / double(8) * (b0 + b2 + double(6) * b1);
     const double d3 = b3 - bavg;
     const double tau6 = fabs(d3);
-    const double q0 = tau6 / e0, q1 = tau6 / e1, q2 = tau6 / e2, q3 = tau6 / e3;
+    const bool blend = tau6 / (bavg + kEps) > k.alpha_tau;
+    const double q0 = tau6 * ie0, q1 = tau6 * ie1, q2 = tau6 * ie2, q3 = tau6 * ie3;
     const double w0 = double(1) / double(32) * (k.C + rep_pow<Q>(q0, k.q));
     const double w1 = double(15) / double(32) * (k.C + rep_pow<Q>(q1, k.q));
     const double w2 = double(15) / double(32) * (k.C + rep_pow<Q>(q2, k.q));
     const double w3 = double(1) / double(32) * (k.C + rep_pow<Q>(q3, k.q));
-    const double ws = w0 + w1 + w2 + w3;
-    const double wh0 = w0 / ws, wh1 = w1 / ws, wh2 = w2 / ws, wh3 = w3 / ws;
-    const bool blend = tau6 / (bavg + kEps) > k.alpha_tau;
-    double W0 = wh0, W1 = wh1, W2 = wh2, W3 = wh3;
-    if (blend) {
-        W0 = sigma * uh0 + (double(1) - sigma) * wh0;
-        W1 = sigma * uh1 + (double(1) - sigma) * wh1;
-        W2 = sigma * uh2 + (double(1) - sigma) * wh2;
-        W3 = (double(1) - sigma) * wh3;
-    }
+    const double iws = adj_rcp(w0 + w1 + w2 + w3);
+    const double wh0 = w0 * iws, wh1 = w1 * iws, wh2 = w2 * iws, wh3 = w3 * iws;
     const double* v = s.v;
+    const double Wb0 = ob * (3.0 / 8.0 * v[0] - 10.0 / 8.0 * v[1] + 15.0 / 8.0 * v[2]);
+    const double Wb1 = ob * (-1.0 / 8.0 * v[1] + 6.0 / 8.0 * v[2] + 3.0 / 8.0 * v[3]);
+    const double Wb2 = ob * (3.0 / 8.0 * v[2] + 6.0 / 8.0 * v[3] - 1.0 / 8.0 * v[4]);
+    const double Wb3 = ob * (15.0 / 8.0 * v[3] - 10.0 / 8.0 * v[4] + 3.0 / 8.0 * v[5]);
+    double W0 = wh0, W1 = wh1, W2 = wh2, W3 = wh3;
+    double whb0 = Wb0, whb1 = Wb1, whb2 = Wb2, whb3 = Wb3;
+    double bb0 = 0.0, bb1 = 0.0, bb2 = 0.0, bb3 = 0.0;
+    if (blend) {
+        // upwind weights u_k = d_k (1 + r_k^P), r_k = tau5 / e_k, normalized
+        const double d02 = b0 - b2;
+        const double tau5 = fabs(d02);
+        const double r0 = tau5 * ie0, r1 = tau5 * ie1, r2 = tau5 * ie2;
+        const double u0 = double(1) / double(16) * (double(1) + rep_pow<P>(r0, k.p));
+        const double u1 = double(5) / double(8) * (double(1) + rep_pow<P>(r1, k.p));
+        const double u2 = double(5) / double(16) * (double(1) + rep_pow<P>(r2, k.p));
+        const double ius = adj_rcp(u0 + u1 + u2);
+        const double uh0 = u0 * ius, uh1 = u1 * ius, uh2 = u2 * ius;
+        const double om = double(1) - sigma;
+        W0 = sigma * uh0 + om * wh0;
+        W1 = sigma * uh1 + om * wh1;
+        W2 = sigma * uh2 + om * wh2;
+        W3 = om * wh3;
+        whb0 = om * Wb0;
+        whb1 = om * Wb1;
+        whb2 = om * Wb2;
+        whb3 = om * Wb3;
+        sigb += (uh0 - wh0) * Wb0 + (uh1 - wh1) * Wb1 + (uh2 - wh2) * Wb2 - wh3 * Wb3;
+        const double uhb0 = sigma * Wb0, uhb1 = sigma * Wb1, uhb2 = sigma * Wb2;
+        const double dotu = uhb0 * uh0 + uhb1 * uh1 + uhb2 * uh2;
+        const double ub0 = (uhb0 - dotu) * ius, ub1 = (uhb1 - dotu) * ius, ub2 = (uhb2 - dotu) * ius;
+        const double rb0 = dpow_times<P>(r0, k.p, ub0 * (1.0 / 16.0)) * ie0;
+        const double rb1 = dpow_times<P>(r1, k.p, ub1 * (5.0 / 8.0)) * ie1;
+        const double rb2 = dpow_times<P>(r2, k.p, ub2 * (5.0 / 16.0)) * ie2;
+        const double tau5b = rb0 + rb1 + rb2;
+        bb0 -= rb0 * r0;
+        bb1 -= rb1 * r1;
+        bb2 -= rb2 * r2;
+        const double sg = sgn(d02);
+        bb0 += tau5b * sg;
+        bb2 -= tau5b * sg;
+    }
     // out = sum of linear combinations of the stencil values with weights W.
     sb[0] += ob * (3.0 / 8.0 * W0);
     sb[1] += ob * (-10.0 / 8.0 * W0 - 1.0 / 8.0 * W1);
@@ -138,47 +188,19 @@ HM_HD void ld_side_adj(const Six& s, double sigma, const LDConst& k, double ob, 
     sb[3] += ob * (3.0 / 8.0 * W1 + 6.0 / 8.0 * W2 + 15.0 / 8.0 * W3);
     sb[4] += ob * (-1.0 / 8.0 * W2 - 10.0 / 8.0 * W3);
     sb[5] += ob * (3.0 / 8.0 * W3);
-    const double Wb0 = ob * (3.0 / 8.0 * v[0] - 10.0 / 8.0 * v[1] + 15.0 / 8.0 * v[2]);
-    const double Wb1 = ob * (-1.0 / 8.0 * v[1] + 6.0 / 8.0 * v[2] + 3.0 / 8.0 * v[3]);
-    const double Wb2 = ob * (3.0 / 8.0 * v[2] + 6.0 / 8.0 * v[3] - 1.0 / 8.0 * v[4]);
-    const double Wb3 = ob * (15.0 / 8.0 * v[3] - 10.0 / 8.0 * v[4] + 3.0 / 8.0 * v[5]);
-    double whb0 = Wb0, whb1 = Wb1, whb2 = Wb2, whb3 = Wb3;
-    double bb0 = 0.0, bb1 = 0.0, bb2 = 0.0, bb3 = 0.0;
-    if (blend) {
-        const double om = double(1) - sigma;
-        whb0 = om * Wb0;
-        whb1 = om * Wb1;
-        whb2 = om * Wb2;
-        whb3 = om * Wb3;
-        sigb += (uh0 - wh0) * Wb0 + (uh1 - wh1) * Wb1 + (uh2 - wh2) * Wb2 - wh3 * Wb3;
-        // upwind weights: normalization, then u_k = d_k (1 + r_k^P), r_k = tau5 / e_k
-        const double uhb0 = sigma * Wb0, uhb1 = sigma * Wb1, uhb2 = sigma * Wb2;
-        const double dotu = uhb0 * uh0 + uhb1 * uh1 + uhb2 * uh2;
-        const double ub0 = (uhb0 - dotu) / us, ub1 = (uhb1 - dotu) / us, ub2 = (uhb2 - dotu) / us;
-        const double rb0 = dpow_times<P>(r0, k.p, ub0 * (1.0 / 16.0));
-        const double rb1 = dpow_times<P>(r1, k.p, ub1 * (5.0 / 8.0));
-        const double rb2 = dpow_times<P>(r2, k.p, ub2 * (5.0 / 16.0));
-        const double tau5b = rb0 / e0 + rb1 / e1 + rb2 / e2;
-        bb0 -= rb0 * r0 / e0;
-        bb1 -= rb1 * r1 / e1;
-        bb2 -= rb2 * r2 / e2;
-        const double sg = sgn(d02);
-        bb0 += tau5b * sg;
-        bb2 -= tau5b * sg;
-    }
-    // central weights: normalization, then w_k = d_k (C + q_k^Q), q_k = tau6 / e_k
+    // central weights w_k = d_k (C + q_k^Q), q_k = tau6 / e_k, normalized
     const double dotw = whb0 * wh0 + whb1 * wh1 + whb2 * wh2 + whb3 * wh3;
-    const double wb0 = (whb0 - dotw) / ws, wb1 = (whb1 - dotw) / ws, wb2 = (whb2 - dotw) / ws,
-                 wb3 = (whb3 - dotw) / ws;
-    const double qb0 = dpow_times<Q>(q0, k.q, wb0 * (1.0 / 32.0));
-    const double qb1 = dpow_times<Q>(q1, k.q, wb1 * (15.0 / 32.0));
-    const double qb2 = dpow_times<Q>(q2, k.q, wb2 * (15.0 / 32.0));
-    const double qb3 = dpow_times<Q>(q3, k.q, wb3 * (1.0 / 32.0));
-    const double tau6b = qb0 / e0 + qb1 / e1 + qb2 / e2 + qb3 / e3;
-    bb0 -= qb0 * q0 / e0;
-    bb1 -= qb1 * q1 / e1;
-    bb2 -= qb2 * q2 / e2;
-    bb3 -= qb3 * q3 / e3;
+    const double wb0 = (whb0 - dotw) * iws, wb1 = (whb1 - dotw) * iws, wb2 = (whb2 - dotw) * iws,
+                 wb3 = (whb3 - dotw) * iws;
+    const double qb0 = dpow_times<Q>(q0, k.q, wb0 * (1.0 / 32.0)) * ie0;
+    const double qb1 = dpow_times<Q>(q1, k.q, wb1 * (15.0 / 32.0)) * ie1;
+    const double qb2 = dpow_times<Q>(q2, k.q, wb2 * (15.0 / 32.0)) * ie2;
+    const double qb3 = dpow_times<Q>(q3, k.q, wb3 * (1.0 / 32.0)) * ie3;
+    const double tau6b = qb0 + qb1 + qb2 + qb3;
+    bb0 -= qb0 * q0;
+    bb1 -= qb1 * q1;
+    bb2 -= qb2 * q2;
+    bb3 -= qb3 * q3;
     const double sg3 = sgn(d3);
     bb3 += tau6b * sg3;
     const double bavgb = -tau6b * sg3;
@@ -189,18 +211,18 @@ HM_HD void ld_side_adj(const Six& s, double sigma, const LDConst& k, double ob, 
     beta6_adj(s, bb3, sb);
 }
 
-// Adjoint of js_side<P>(s, p).
+// Adjoint of js_side<P>(s, p): w_k = d_k / e_k^p recomputed as d_k (1/e_k)^p.
 template <int P>
 HM_HD void js_side_adj(const Six& s, int p_rt, double ob, double* sb) {
     if (ob == 0.0) return;
     double b0, b1, b2;
     beta3(s, b0, b1, b2);
-    const double e0 = b0 + kEps, e1 = b1 + kEps, e2 = b2 + kEps;
-    const double w0 = double(1.0) / double(16) / rep_pow<P>(e0, p_rt);
-    const double w1 = double(5.0) / double(8) / rep_pow<P>(e1, p_rt);
-    const double w2 = double(5.0) / double(16) / rep_pow<P>(e2, p_rt);
-    const double ws = w0 + w1 + w2;
-    const double W0 = w0 / ws, W1 = w1 / ws, W2 = w2 / ws;
+    const double ie0 = adj_rcp(b0 + kEps), ie1 = adj_rcp(b1 + kEps), ie2 = adj_rcp(b2 + kEps);
+    const double w0 = double(1.0) / double(16) * rep_pow<P>(ie0, p_rt);
+    const double w1 = double(5.0) / double(8) * rep_pow<P>(ie1, p_rt);
+    const double w2 = double(5.0) / double(16) * rep_pow<P>(ie2, p_rt);
+    const double iws = adj_rcp(w0 + w1 + w2);
+    const double W0 = w0 * iws, W1 = w1 * iws, W2 = w2 * iws;
     const double* v = s.v;
     sb[0] += ob * (3.0 / 8.0 * W0);
     sb[1] += ob * (-10.0 / 8.0 * W0 - 1.0 / 8.0 * W1);
@@ -211,12 +233,12 @@ HM_HD void js_side_adj(const Six& s, int p_rt, double ob, double* sb) {
     const double Wb1 = ob * (-1.0 / 8.0 * v[1] + 6.0 / 8.0 * v[2] + 3.0 / 8.0 * v[3]);
     const double Wb2 = ob * (3.0 / 8.0 * v[2] + 6.0 / 8.0 * v[3] - 1.0 / 8.0 * v[4]);
     const double dot = Wb0 * W0 + Wb1 * W1 + Wb2 * W2;
-    const double wb0 = (Wb0 - dot) / ws, wb1 = (Wb1 - dot) / ws, wb2 = (Wb2 - dot) / ws;
+    const double wb0 = (Wb0 - dot) * iws, wb1 = (Wb1 - dot) * iws, wb2 = (Wb2 - dot) * iws;
     const int p = (P > 0) ? P : p_rt;
     // w_k = d_k e_k^-p  ->  dw_k/de_k = -p w_k / e_k
-    const double bb0 = -wb0 * double(p) * w0 / e0;
-    const double bb1 = -wb1 * double(p) * w1 / e1;
-    const double bb2 = -wb2 * double(p) * w2 / e2;
+    const double bb0 = -wb0 * double(p) * w0 * ie0;
+    const double bb1 = -wb1 * double(p) * w1 * ie1;
+    const double bb2 = -wb2 * double(p) * w2 * ie2;
     beta3_adj(s, bb0, bb1, bb2, sb);
 }
 
@@ -375,6 +397,7 @@ HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bo
             const double be2 = double(1) - be1;
             double be1b = 0.0, be2b = 0.0;
             const double dsl = sR - sL;
+            const double idsl = adj_rcp(dsl);
 #pragma unroll
             for (int e = 0; e < NEQ; e++) {
                 const bool hyb = (e == 0) || (e >= 1 && e <= D && e != 1 + A);
@@ -395,8 +418,8 @@ HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bo
                 } else if (sL > double(0)) {
                     Fxb[0][e] += hb;
                 } else {
-                    const double nb = hb / dsl;
-                    const double db = -hb * hv / dsl;
+                    const double nb = hb * idsl;
+                    const double db = -hb * hv * idsl;
                     sRb += nb * (F[0][e] + sL * (Q[1][e] - Q[0][e])) + db;
                     sLb += nb * (-F[1][e] + sR * (Q[1][e] - Q[0][e])) - db;
                     Fxb[0][e] += nb * sR;
@@ -407,14 +430,16 @@ HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bo
             }
             be1b -= be2b;
             if (!tiny) {
-                const double al1b = be1b * 0.5 * al2 / (Ssum * Ssum);
-                double al2b = -be1b * 0.5 * al1 / (Ssum * Ssum);
+                const double iS = adj_rcp(Ssum);
+                const double al1b = be1b * 0.5 * al2 * (iS * iS);
+                double al2b = -be1b * 0.5 * al1 * (iS * iS);
                 double a1b = al1b;
-                if (al2 > 0.0) a1b += al2b * (-al1 / al2);
+                if (al2 > 0.0) a1b += al2b * (-al1 * adj_rcp(al2));
                 const double dua = du[A];
-                const double duab = a1b * sgn(dua) / vm;
-                const double vmb = -a1b * al1 / vm;
-                const double dsqb = (vm > 0.0) ? vmb / (2.0 * vm) : 0.0;
+                const double ivm = adj_rcp(vm);  // vm >= kEps here
+                const double duab = a1b * sgn(dua) * ivm;
+                const double vmb = -a1b * al1 * ivm;
+                const double dsqb = vmb * 0.5 * ivm;
 #pragma unroll
                 for (int b = 0; b < D; b++) {
                     double dub = dsqb * 2.0 * du[b];
@@ -458,15 +483,16 @@ HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bo
         const double hb = innerb * (s_star - uK);
         rhoKb += hb * s_star;
         sstb += hb * rhoK;
-        pKb += hb / g;
-        const double gb = -hb * pK / (g * g);
+        const double ig = adj_rcp(g);
+        pKb += hb * ig;
+        const double gb = -hb * pK * (ig * ig);
         sKb += gb;
         uKb -= gb;
     }
     {
-        const double b2 = sK - s_star;
-        const double ab = chib / b2;
-        const double b2b = -chib * chi / b2;
+        const double ib2 = adj_rcp(sK - s_star);
+        const double ab = chib * ib2;
+        const double b2b = -chib * chi * ib2;
         sKb += ab + b2b;
         uKb -= ab;
         sstb -= b2b;
@@ -528,8 +554,9 @@ HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bo
     }
     // s* = num / den
     {
-        const double numb = sstb / den;
-        const double denb = -sstb * s_star / den;
+        const double iden = adj_rcp(den);
+        const double numb = sstb * iden;
+        const double denb = -sstb * s_star * iden;
         Vb[1][IP] += numb;
         Vb[0][IP] -= numb;
         Vb[0][0] += numb * uL * (sL - uL) + denb * (sL - uL);
@@ -561,12 +588,14 @@ HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bo
     csb[1] += 0.5 * cavgb;
     // c = sqrt(gam p / rho), eps = p / ((gam - 1) rho)
     const double css[2] = {csL, csR};
+    const double igm1 = adj_rcp(gam - 1.0);
 #pragma unroll
     for (int x = 0; x < 2; x++) {
         const double* V = Vs[x];
-        const double ab = csb[x] / (2.0 * css[x]);
-        Vb[x][IP] += ab * gam / V[0] + epsb[x] / ((gam - 1.0) * V[0]);
-        Vb[x][0] -= ab * gam * V[IP] / (V[0] * V[0]) + epsb[x] * epss[x] / V[0];
+        const double irho = adj_rcp(V[0]);
+        const double ab = csb[x] * 0.5 * adj_rcp(css[x]);
+        Vb[x][IP] += ab * gam * irho + epsb[x] * igm1 * irho;
+        Vb[x][0] -= ab * gam * V[IP] * (irho * irho) + epsb[x] * epss[x] * irho;
     }
 #pragma unroll
     for (int e = 0; e < NEQ; e++) {

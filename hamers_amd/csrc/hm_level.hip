@@ -678,6 +678,7 @@ class Level {
             r.nz = five ? ns - 1 : 0;
             r.zfirst = five ? ns + d_ + 1 : 0;
             r.last_beta = -1;
+            r.ns = 0;  // the level integrates the Euler form
             for (int m = 0; m < kMaxRK; m++) {
                 r.alpha[m] = alpha_[s][m];
                 r.beta[m] = beta_[s][m];
@@ -1072,6 +1073,7 @@ int advance_single_step(ConvectiveFluxReconstructorGPU* rec, const hm_rk_patch_v
                 return rec->fail(HM_ERR_INVALID_ARGUMENT, msg);
             }
             for (int a = 0; a < 3; a++) R.Fdi[m][a] = (a < d && nd) ? v.F_diffusive_intermediate[m][a] : nullptr;
+            if (nd) r.ns = 1;
         }
         for (int a = 0; a < 3; a++) R.Facc[a] = a < d ? v.F_accumulated[a] : nullptr;
         for (int a = 0; a < 3; a++) R.Fdacc[a] = a < d ? v.F_diffusive_accumulated[a] : nullptr;

@@ -15,7 +15,7 @@
 
 namespace hm {
 
-template <int D, class W>
+template <int D, bool NS, class W>
 __global__ __launch_bounds__(kBlock) void k_rk_update(W w, RKParams r) {
     const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     const RKPatch* P;
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kBlock) void k_rk_update(W w, RKParams r) {
                 auto diff = [&](int a) {
                     const double* Fa = P->Fi[n][a] + e * fvol[a];
                     double dF = Fa[fR[a]] - Fa[fL[a]];
-                    if (P->Fdi[n][0]) {
+                    if (NS && P->Fdi[n][0]) {
                         const double* Fd = P->Fdi[n][a] + e * fvol[a];
                         dF = dF + Fd[fR[a]] - Fd[fL[a]];
                     }
@@ -131,10 +131,14 @@ hipError_t launch_rk_update(int dim, const W& cells, int64_t cell_items, const W
                             const RKParams& r, hipStream_t s) {
     const unsigned nb = unsigned((cell_items + kBlock - 1) / kBlock);
     if (cell_items > 0) {
-        switch (dim) {
-            case 1: hipLaunchKernelGGL((k_rk_update<1, W>), dim3(nb), dim3(kBlock), 0, s, cells, r); break;
-            case 2: hipLaunchKernelGGL((k_rk_update<2, W>), dim3(nb), dim3(kBlock), 0, s, cells, r); break;
-            case 3: hipLaunchKernelGGL((k_rk_update<3, W>), dim3(nb), dim3(kBlock), 0, s, cells, r); break;
+        // The Euler form is compiled without the diffusive-flux terms.
+        switch (dim * 2 + (r.ns ? 1 : 0)) {
+            case 2: hipLaunchKernelGGL((k_rk_update<1, false, W>), dim3(nb), dim3(kBlock), 0, s, cells, r); break;
+            case 3: hipLaunchKernelGGL((k_rk_update<1, true, W>), dim3(nb), dim3(kBlock), 0, s, cells, r); break;
+            case 4: hipLaunchKernelGGL((k_rk_update<2, false, W>), dim3(nb), dim3(kBlock), 0, s, cells, r); break;
+            case 5: hipLaunchKernelGGL((k_rk_update<2, true, W>), dim3(nb), dim3(kBlock), 0, s, cells, r); break;
+            case 6: hipLaunchKernelGGL((k_rk_update<3, false, W>), dim3(nb), dim3(kBlock), 0, s, cells, r); break;
+            case 7: hipLaunchKernelGGL((k_rk_update<3, true, W>), dim3(nb), dim3(kBlock), 0, s, cells, r); break;
             default: return hipErrorInvalidValue;
         }
     }

@@ -30,6 +30,7 @@ struct RKParams {
     int zfirst;     // first volume-fraction equation (five-equation model)
     int nz;         // number of evolved volume fractions (0: no last-fraction update)
     int last_beta;  // index of the last nonzero beta, -1 if none
+    int ns;         // some patch carries intermediate diffusive fluxes (Navier-Stokes form)
     double alpha[kMaxRK], beta[kMaxRK], gamma[kMaxRK];
 };
 
