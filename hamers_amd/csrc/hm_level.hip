@@ -144,25 +144,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_div(SegTableT<AdjLevelPatch> w, 
     }
 }
 
-// lam[m] (interior) += the VJP output of this patch plus the ghost-cell
-// adjoints that the same-level halo copied from this patch's interior
-// (the reverse of k_halo, as a gather: deterministic, no atomics).
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_halo_reverse(SegTableT<AdjLevelPatch> w, const HaloItem* rev, int m,
-                                                         int neq, int g) {
-    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    const AdjLevelPatch* P;
-    int ax;
-    int64_t loc;
-    if (!w.find(gid, P, ax, loc)) return;
-    const int i = int(loc % P->n[0]);
-    const int64_t r = loc / P->n[0];
-    const int j = (D > 1) ? int(r % P->n[1]) : 0;
-    const int k = (D > 2) ? int(r / P->n[1]) : 0;
-    const int64_t gi = gbox_idx<D>(P->n, g, i, j, k);
-    constexpr int NEQ = D + 2;  // the level adjoint is built for the single-species model
-    (void)neq;
-    double acc[NEQ];
+// The VJP output of this patch at interior cell (i, j, k) plus the ghost-cell
+// adjoints that the same-level halo copied from that cell (the reverse of
+// k_halo, as a gather: deterministic, no atomics).
+template <int D, int NEQ>
+__device__ __forceinline__ void gather_qbar(const SegTableT<AdjLevelPatch>& w, const HaloItem* rev,
+                                            const AdjLevelPatch* P, int i, int j, int k, int g, int64_t gi,
+                                            double (&acc)[NEQ]) {
 #pragma unroll
     for (int e = 0; e < NEQ; e++) acc[e] = P->qbar[e * P->gvol + gi];
     const int x[3] = {i + g, (D > 1) ? j + g : 0, (D > 2) ? k + g : 0};  // ghost-box coordinates
@@ -182,8 +170,81 @@ __global__ __launch_bounds__(kBlock) void k_halo_reverse(SegTableT<AdjLevelPatch
 #pragma unroll
         for (int e = 0; e < NEQ; e++) acc[e] += Pd.qbar[e * Pd.gvol + di];
     }
+}
+
+// lam[m] (interior) += the gathered VJP output (a stage whose update has
+// several nonzero beta: every VJP but the last of the stage).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_halo_reverse(SegTableT<AdjLevelPatch> w, const HaloItem* rev, int m,
+                                                         int g) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const AdjLevelPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    const int i = int(loc % P->n[0]);
+    const int64_t r = loc / P->n[0];
+    const int j = (D > 1) ? int(r % P->n[1]) : 0;
+    const int k = (D > 2) ? int(r / P->n[1]) : 0;
+    const int64_t gi = gbox_idx<D>(P->n, g, i, j, k);
+    constexpr int NEQ = D + 2;  // the level adjoint is built for the single-species model
+    double acc[NEQ];
+    gather_qbar<D, NEQ>(w, rev, P, i, j, k, g, gi, acc);
 #pragma unroll
     for (int e = 0; e < NEQ; e++) P->lam[m][e * P->gvol + gi] += acc[e];
+}
+
+// The stage adjoints' update after the VJPs of stage sn, one launch per stage.
+// Stage sn computes Q_{sn+1} = sum_m alpha_m Q_m + sum_m beta_m dt L(Q_m), so
+// lam[m] receives alpha_m lam[sn+1] and, for the stage input whose VJP ran
+// last, the gathered VJP output.  Entry t writes
+//   lam[dst_t] = (init_t ? 0 : lam[m_t]) + (t == q ? gathered VJP : 0) + a_t lam[sn+1]
+// (the addition order of a zero fill, the halo gather and the alpha update in
+// sequence, so the rounding is the same).  init_t marks the first write to
+// lam[m_t] in the step's reverse sweep, which replaces a zero fill; dst_t
+// differs from m_t only for the step input in the last stage, which lands
+// directly in the step's output slot lam[nst].
+struct StageAdj {
+    int nt;                 // entries
+    int src;                // sn + 1
+    int q;                  // entry receiving the VJP output, -1 if none
+    int m[kMaxRK], dst[kMaxRK], init[kMaxRK];
+    double a[kMaxRK];
+};
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_stage_adjoint(SegTableT<AdjLevelPatch> w, const HaloItem* rev,
+                                                          StageAdj sa, int g) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const AdjLevelPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    const int i = int(loc % P->n[0]);
+    const int64_t r = loc / P->n[0];
+    const int j = (D > 1) ? int(r % P->n[1]) : 0;
+    const int k = (D > 2) ? int(r / P->n[1]) : 0;
+    const int64_t gi = gbox_idx<D>(P->n, g, i, j, k);
+    constexpr int NEQ = D + 2;
+    double acc[NEQ];
+    if (sa.q >= 0) gather_qbar<D, NEQ>(w, rev, P, i, j, k, g, gi, acc);
+    const double* S = P->lam[sa.src];
+    double sv[NEQ];
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) sv[e] = S[e * P->gvol + gi];
+    for (int t = 0; t < sa.nt; t++) {
+        const double* L = P->lam[sa.m[t]];
+        double* O = P->lam[sa.dst[t]];
+        const double a = sa.a[t];
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) {
+            const int64_t idx = e * P->gvol + gi;
+            double v = sa.init[t] ? 0.0 : L[idx];
+            if (t == sa.q) v += acc[e];
+            if (a != 0.0) v += a * sv[e];
+            O[idx] = v;
+        }
+    }
 }
 
 // Interior updates of the stage adjoints: op 0: lam[dst] = 0; op 1: lam[dst] += a lam[src];
@@ -302,6 +363,7 @@ class Level {
                 beta_[s][m] = (s < 3 && m < 3) ? b3[s][m] : 0.0;
                 gamma_[s][m] = (s < 3 && m < 3) ? c3[s][m] : 0.0;
             }
+        plan_stage_adjoint();
         return allocate();
     }
 
@@ -315,6 +377,7 @@ class Level {
                 beta_[s][m] = in ? b[s * nst + m] : 0.0;
                 gamma_[s][m] = in && c ? c[s * nst + m] : 0.0;
             }
+        plan_stage_adjoint();
         // Launches already queued read the tables rewritten below.
         if (hipDeviceSynchronize() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "device synchronize failed");
         drop_graphs();
@@ -761,33 +824,95 @@ class Level {
 
     // Reverse of one step whose stage inputs are stored from slot `slot0`.
     // lam[nst] holds the adjoint of the step's output on entry and of its
-    // input on return.
+    // input on return.  Per stage: the VJP of each stage input with nonzero
+    // beta, then one k_stage_adjoint launch for the alpha terms and the last
+    // VJP's output (earlier VJPs of the same stage, which only tableaux with
+    // several nonzero beta per stage have, add theirs with k_halo_reverse).
     int adjoint_step(int slot0, hipStream_t s) {
-        if (int st = lam_op(3, nst_, 0, 0.0, s)) return st;
+        if (zero_first_) {
+            if (int st = lam_op(3, nst_, 0, 0.0, s)) return st;
+        }
         for (int sn = nst_ - 1; sn >= 0; sn--) {
+            int last_vjp = -1;
+            for (int m = 0; m <= sn; m++)
+                if (beta_[sn][m] != 0.0) last_vjp = m;
             for (int m = 0; m <= sn; m++) {
-                const double b = beta_[sn][m], a = alpha_[sn][m];
-                if (b != 0.0) {
-                    if (int st = store_op(1 + m, slot0 + m, 1, s)) return st;
-                    switch (d_) {
-                        case 1: hipLaunchKernelGGL(k_adj_div<1>, dim3(blocks_for(adj_face_items_)), dim3(kBlock), 0, s, adj_faces_, sn + 1, b, neq_, g_); break;
-                        case 2: hipLaunchKernelGGL(k_adj_div<2>, dim3(blocks_for(adj_face_items_)), dim3(kBlock), 0, s, adj_faces_, sn + 1, b, neq_, g_); break;
-                        default: hipLaunchKernelGGL(k_adj_div<3>, dim3(blocks_for(adj_face_items_)), dim3(kBlock), 0, s, adj_faces_, sn + 1, b, neq_, g_); break;
-                    }
-                    if (hipGetLastError() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "adjoint launch failed");
-                    if (int st = adj_->run(vjp_plan_[m], rec_dt_, s)) return st;
-                    switch (d_) {
-                        case 1: hipLaunchKernelGGL(k_halo_reverse<1>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, rev_dev_, m, neq_, g_); break;
-                        case 2: hipLaunchKernelGGL(k_halo_reverse<2>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, rev_dev_, m, neq_, g_); break;
-                        default: hipLaunchKernelGGL(k_halo_reverse<3>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, rev_dev_, m, neq_, g_); break;
-                    }
-                    if (hipGetLastError() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "adjoint launch failed");
+                if (beta_[sn][m] == 0.0) continue;
+                if (int st = store_op(1 + m, slot0 + m, 1, s)) return st;
+                switch (d_) {
+                    case 1: hipLaunchKernelGGL(k_adj_div<1>, dim3(blocks_for(adj_face_items_)), dim3(kBlock), 0, s, adj_faces_, sn + 1, beta_[sn][m], neq_, g_); break;
+                    case 2: hipLaunchKernelGGL(k_adj_div<2>, dim3(blocks_for(adj_face_items_)), dim3(kBlock), 0, s, adj_faces_, sn + 1, beta_[sn][m], neq_, g_); break;
+                    default: hipLaunchKernelGGL(k_adj_div<3>, dim3(blocks_for(adj_face_items_)), dim3(kBlock), 0, s, adj_faces_, sn + 1, beta_[sn][m], neq_, g_); break;
                 }
-                if (a != 0.0)
-                    if (int st = lam_op(1, m, sn + 1, a, s)) return st;
+                if (hipGetLastError() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "adjoint launch failed");
+                if (int st = adj_->run(vjp_plan_[m], rec_dt_, s)) return st;
+                if (m == last_vjp) break;
+                switch (d_) {
+                    case 1: hipLaunchKernelGGL(k_halo_reverse<1>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, rev_dev_, m, g_); break;
+                    case 2: hipLaunchKernelGGL(k_halo_reverse<2>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, rev_dev_, m, g_); break;
+                    default: hipLaunchKernelGGL(k_halo_reverse<3>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, rev_dev_, m, g_); break;
+                }
+                if (hipGetLastError() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "adjoint launch failed");
+            }
+            const StageAdj& sa = stage_adj_[sn];
+            if (sa.nt == 0) continue;
+            switch (d_) {
+                case 1: hipLaunchKernelGGL(k_stage_adjoint<1>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, rev_dev_, sa, g_); break;
+                case 2: hipLaunchKernelGGL(k_stage_adjoint<2>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, rev_dev_, sa, g_); break;
+                default: hipLaunchKernelGGL(k_stage_adjoint<3>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, rev_dev_, sa, g_); break;
+            }
+            if (hipGetLastError() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "adjoint launch failed");
+        }
+        if (!out_in_place_) return lam_op(2, nst_, 0, 0.0, s);
+        return HM_OK;
+    }
+
+    // Entries of the per-stage adjoint updates (k_stage_adjoint) for the
+    // tableau alpha_, beta_: which stage adjoints each stage writes, with
+    // which alpha, which is first written there (no zero fill needed), and
+    // whether the step input's adjoint can land directly in lam[nst].
+    void plan_stage_adjoint() {
+        bool touched[kMaxRK] = {};
+        zero_first_ = false;
+        for (int sn = nst_ - 1; sn >= 0; sn--) {
+            StageAdj& sa = stage_adj_[sn];
+            sa = StageAdj{};
+            sa.src = sn + 1;
+            sa.q = -1;
+            int last_vjp = -1;
+            for (int m = 0; m <= sn; m++)
+                if (beta_[sn][m] != 0.0) last_vjp = m;
+            // VJPs of this stage before the last add into lam[m] with
+            // k_halo_reverse, which needs lam[m] to hold a value already.
+            for (int m = 0; m < last_vjp; m++)
+                if (beta_[sn][m] != 0.0) {
+                    if (!touched[m]) zero_first_ = true;
+                    touched[m] = true;
+                }
+            for (int m = 0; m <= sn; m++) {
+                if (alpha_[sn][m] == 0.0 && m != last_vjp) continue;
+                const int t = sa.nt++;
+                sa.m[t] = m;
+                sa.dst[t] = m;
+                sa.a[t] = alpha_[sn][m];
+                sa.init[t] = touched[m] ? 0 : 1;
+                if (m == last_vjp) sa.q = t;
+                touched[m] = true;
             }
         }
-        return lam_op(2, nst_, 0, 0.0, s);
+        // A stage adjoint read before any write (an unused stage) needs the zero fill.
+        for (int m = 0; m < nst_; m++)
+            if (!touched[m]) zero_first_ = true;
+        if (zero_first_)
+            for (int sn = 0; sn < nst_; sn++)
+                for (int t = 0; t < stage_adj_[sn].nt; t++) stage_adj_[sn].init[t] = 0;
+        // The step input's last update is in stage 0 (its only entry there is
+        // m = 0): write it straight to lam[nst].
+        out_in_place_ = false;
+        if (stage_adj_[0].nt == 1 && stage_adj_[0].m[0] == 0) {
+            stage_adj_[0].dst[0] = nst_;
+            out_in_place_ = true;
+        }
     }
 
     int adjoint_io(int p, double* host, bool write) {
@@ -981,6 +1106,8 @@ class Level {
     SegTableT<AdjLevelPatch> adj_faces_{}, adj_cells_{};
     int64_t adj_face_items_ = 0, adj_cell_items_ = 0;
     const HaloItem* rev_dev_ = nullptr;
+    StageAdj stage_adj_[kMaxRK] = {};
+    bool zero_first_ = false, out_in_place_ = false;
     const int64_t* slot_off_dev_ = nullptr;
     std::unique_ptr<AdjointSingle> adj_;
     AdjointSingle::Owned vjp_plan_[kMaxRK];

@@ -175,31 +175,32 @@ def _modes(N=64, P=32, nsteps=12, every=5, scheme="WCNS6_LD_HLLC_HLL", use_graph
                              use_graph=use_graph)
 
 
-def _oracle_J(prob, a):
+def _oracle_J(prob, a, tableau=None):
     from hamers_amd.adjoint import kinetic_energy_transverse
 
     scheme = 1 if prob.rec.scheme.startswith("WCNS6") else 0
     Q0 = prob.initial_state(a)
-    QT = oracle_rk3(O.Params(dim=2, scheme=scheme), Q0, (prob.N, prob.N), prob.dx, prob.dt, prob.nsteps)
+    kw = {} if tableau is None else {"alpha": tableau[0], "beta": tableau[1]}
+    QT = oracle_rk3(O.Params(dim=2, scheme=scheme), Q0, (prob.N, prob.N), prob.dx, prob.dt, prob.nsteps, **kw)
     return kinetic_energy_transverse(QT, prob.dx)
 
 
-def _oracle_frozen_differences(prob, a, rel_h):
+def _oracle_frozen_differences(prob, a, rel_h, tableau=None):
     """Central differences of the oracle's J with every branch decision
     replayed from the run at ``a`` (the smooth piece the adjoint differentiates)."""
     fd = np.zeros(len(a))
     with O.frozen_branches() as fb:
-        J0 = _oracle_J(prob, a)
+        J0 = _oracle_J(prob, a, tableau)
         for k in range(len(a)):
             h = rel_h * a[k]
             ap, am = a.copy(), a.copy()
             ap[k] += h
             am[k] -= h
             fb.replay()
-            jp = _oracle_J(prob, ap)
+            jp = _oracle_J(prob, ap, tableau)
             assert fb.consumed()
             fb.replay()
-            jm = _oracle_J(prob, am)
+            jm = _oracle_J(prob, am, tableau)
             assert fb.consumed()
             fd[k] = (jp - jm) / (2 * h)
     return J0, fd
@@ -217,6 +218,26 @@ def test_level_gradient_against_oracle_differences(torch_dev, scheme, use_graph,
     assert J == J_oracle  # the level's forward is bit-identical to the oracle's
     err = float(np.max(np.abs(g - fd)) / np.max(np.abs(fd)))
     _say(f"J {J:.12e}; adjoint gradient {g}; oracle differences {fd}; rel err {err:.2e}")
+    assert err <= 1e-6
+
+
+@pytest.mark.parametrize("name,alpha,beta", [
+    ("forward Euler", [[1.0]], [[1.0]]),
+    ("Heun, two nonzero beta in the second stage", [[1.0], [1.0, 0.0]], [[1.0], [0.5, 0.5]]),
+    ("SSP-RK2 in Shu-Osher form", [[1.0], [0.5, 0.5]], [[1.0], [0.0, 0.5]]),
+])
+def test_level_gradient_other_tableaux(torch_dev, name, alpha, beta):
+    """The stage-adjoint bookkeeping (first writes, several VJPs per stage,
+    the step input's adjoint written in place) for tableaux other than SSP-RK3."""
+    prob = _modes(nsteps=6, every=4)
+    prob.level.set_rk_coefficients(alpha, beta)
+    a = prob.a0
+    _say(f"case level adjoint gradient, smooth modes 64^2, {name}, 6 steps, checkpoint every 4")
+    J, g = prob.gradient(a)
+    J_oracle, fd = _oracle_frozen_differences(prob, a, 1e-6, (alpha, beta))
+    assert J == J_oracle
+    err = float(np.max(np.abs(g - fd)) / np.max(np.abs(fd)))
+    _say(f"J {J:.12e}; rel err {err:.2e}")
     assert err <= 1e-6
 
 
