@@ -155,6 +155,8 @@ def _load():
                                                     ctypes.c_double, ctypes.c_double, ctypes.c_int, P]
     L.hm_level_advance_recorded.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
     L.hm_level_adjoint.argtypes = [P, P]
+    L.hm_level_set_face_state_budget.argtypes = [P, ctypes.c_int64]
+    L.hm_level_face_states_kept.argtypes = [P]
     L.hm_level_write_adjoint.argtypes = [P, ctypes.c_int, D]
     L.hm_level_read_adjoint.argtypes = [P, ctypes.c_int, D]
     L.hm_set_diffusive_transport.argtypes = [P, ctypes.POINTER(hm_transport)]
@@ -168,6 +170,7 @@ def _load():
                  "hm_level_read_patch", "hm_level_patch_array", "hm_level_fill_ghosts", "hm_level_advance",
                  "hm_convective_flux_vjp_on_patch", "hm_convective_flux_vjp_on_patches", "hm_level_advance_recorded",
                  "hm_level_adjoint", "hm_level_write_adjoint", "hm_level_read_adjoint",
+                 "hm_level_set_face_state_budget", "hm_level_face_states_kept",
                  "hm_set_diffusive_transport", "hm_get_diffusive_num_ghosts", "hm_compute_diffusive_flux_on_patch",
                  "hm_compute_diffusive_flux_on_patches"):
         getattr(L, name).restype = ctypes.c_int

@@ -60,7 +60,7 @@ __device__ __forceinline__ void load_stencil(const DevPatch& F, const Boxes<D>& 
 // the first half of the face's reverse sweep (Riemann flux, fallback,
 // back-projection) into the hand-over buffer H; H's last entry is 2 when the
 // face receives no adjoint at all.
-template <int D, int A, int SCHEME, int PP, int QQ>
+template <int D, int A, int SCHEME, int PP, int QQ, bool STORED>
 __device__ __forceinline__ void adj_face_flux(const AdjPatch& P, int64_t loc, const KParams& k) {
     constexpr int NEQ = D + 2;
     constexpr int NH = face_adj_h_size<D>();
@@ -92,23 +92,47 @@ __device__ __forceinline__ void adj_face_flux(const AdjPatch& P, int64_t loc, co
         H[(NH - 1) * mv + loc] = 2.0;
         return;
     }
-    double V[6][NEQ];
-    double cL, cR;
-    load_stencil<D, A>(F, B, c0, c1, c2, V, cL, cR);
-    bool use_hll = false;
-    if constexpr (D > 1) {
-        const int64_t vs = B.vol(kGs);
-        const int64_t tR = B.idx(kGs, c0, c1, c2);
-        const int64_t sts = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGs)) : int64_t(B.ext(0, kGs)) * B.ext(1, kGs);
-        const int64_t tL = tR - sts;
-        const double th_avg = 0.5 * (F.sens[tL] + F.sens[tR]);
-        const double om_avg = 0.5 * (F.sens[vs + tL] + F.sens[vs + tR]);
-        const double sensor = -th_avg / (fabs(th_avg) + om_avg + kEps);
-        use_hll = sensor > 0.65;
-    }
+    auto sensor_hll = [&]() -> bool {
+        if constexpr (D > 1) {
+            const int64_t vs = B.vol(kGs);
+            const int64_t tR = B.idx(kGs, c0, c1, c2);
+            const int64_t sts =
+                (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGs)) : int64_t(B.ext(0, kGs)) * B.ext(1, kGs);
+            const int64_t tL = tR - sts;
+            const double th_avg = 0.5 * (F.sens[tL] + F.sens[tR]);
+            const double om_avg = 0.5 * (F.sens[vs + tL] + F.sens[vs + tR]);
+            const double sensor = -th_avg / (fabs(th_avg) + om_avg + kEps);
+            return sensor > 0.65;
+        }
+        return false;
+    };
     double h[NH];
-    face_adj_single_flux<D, A, SCHEME, PP, QQ>(V, cL, cR, k.gamma[0], LDConst{k.p, k.q, k.C, k.alpha_tau}, use_hll,
-                                               mb, h);
+    if constexpr (STORED) {
+        // W-, W+ as the recording forward pass reconstructed them.
+        const int64_t v4 = B.vol(kGc);
+        const int64_t st = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGc)) : int64_t(B.ext(0, kGc)) * B.ext(1, kGc);
+        const double* pc = F.prim + B.idx(kGc, c0, c1, c2);  // cell right of the face
+        const double* ws = k.wrec + F.wrec[A] + loc;
+        double Wm[NEQ], Wp[NEQ], V2[NEQ], V3[NEQ];
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) {
+            Wm[e] = ws[e * mv];
+            Wp[e] = ws[(NEQ + e) * mv];
+            V2[e] = pc[e * v4 - st];
+            V3[e] = pc[e * v4];
+        }
+        const bool use_hll = sensor_hll();
+        const double ra = double(1) / double(2) * (V2[0] + V3[0]);
+        const double ca = double(1) / double(2) * (pc[NEQ * v4 - st] + pc[NEQ * v4]);
+        face_adj_single_after_recon<D, A>(Wm, Wp, V2, V3, ra, ca, k.gamma[0], use_hll, mb, h);
+    } else {
+        double V[6][NEQ];
+        double cL, cR;
+        load_stencil<D, A>(F, B, c0, c1, c2, V, cL, cR);
+        const bool use_hll = sensor_hll();
+        face_adj_single_flux<D, A, SCHEME, PP, QQ>(V, cL, cR, k.gamma[0], LDConst{k.p, k.q, k.C, k.alpha_tau},
+                                                   use_hll, mb, h);
+    }
 #pragma unroll
     for (int c = 0; c < NH; c++) H[c * mv + loc] = h[c];
 }
@@ -139,7 +163,7 @@ __device__ __forceinline__ void adj_face_recon(const AdjPatch& P, int64_t loc, c
                                                 [&](int c, double v) { G[c * mv] = v; });
 }
 
-template <int D, int SCHEME, int PP, int QQ, class W>
+template <int D, int SCHEME, int PP, int QQ, class W, bool STORED = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D < 3 ? HM_ADJ_FLUX_WAVES : 2))) void k_adj_face_flux(
     W w, KParams k) {
     const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -148,12 +172,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D < 3 ? 
     int64_t loc;
     if (!w.find(gid, P, ax, loc)) return;
     if (ax == 0) {
-        adj_face_flux<D, 0, SCHEME, PP, QQ>(*P, loc, k);
+        adj_face_flux<D, 0, SCHEME, PP, QQ, STORED>(*P, loc, k);
     } else if constexpr (D > 1) {
         if (ax == 1) {
-            adj_face_flux<D, 1, SCHEME, PP, QQ>(*P, loc, k);
+            adj_face_flux<D, 1, SCHEME, PP, QQ, STORED>(*P, loc, k);
         } else if constexpr (D > 2) {
-            adj_face_flux<D, 2, SCHEME, PP, QQ>(*P, loc, k);
+            adj_face_flux<D, 2, SCHEME, PP, QQ, STORED>(*P, loc, k);
         }
     }
 }
@@ -317,8 +341,12 @@ hipError_t launch_adj_dsq(const AdjPlanDev& pl, const KParams& k, hipStream_t s)
     using W = SegTableT<AdjPatch>;
     if (pl.n_qbox > 0) hipLaunchKernelGGL((k_adj_zero<D, W>), dim3(nblk(pl.n_qbox)), dim3(kBlock), 0, s, pl.qbox);
     if (pl.n_face > 0) {
-        hipLaunchKernelGGL((k_adj_face_flux<D, SCHEME, PP, QQ, W>), dim3(nblk(pl.n_face)), dim3(kBlock), 0, s, pl.face,
-                           k);
+        if (k.wrec)
+            hipLaunchKernelGGL((k_adj_face_flux<D, SCHEME, PP, QQ, W, true>), dim3(nblk(pl.n_face)), dim3(kBlock), 0, s,
+                               pl.face, k);
+        else
+            hipLaunchKernelGGL((k_adj_face_flux<D, SCHEME, PP, QQ, W>), dim3(nblk(pl.n_face)), dim3(kBlock), 0, s,
+                               pl.face, k);
         hipLaunchKernelGGL((k_adj_face_recon<D, SCHEME, PP, QQ, W>), dim3(nblk(pl.n_face)), dim3(kBlock), 0, s,
                            pl.face, k);
     }
@@ -362,7 +390,7 @@ void AdjointSingle::release(Owned& o) {
     o.plan = AdjPlanDev{};
 }
 
-int AdjointSingle::build(const hm_vjp_patch_view* views, int np, Owned& out) {
+int AdjointSingle::build(const hm_vjp_patch_view* views, int np, Owned& out, const int64_t* wrec_off) {
     const hm_config& c = rec_->config();
     if (c.model != HM_MODEL_SINGLE_SPECIES)
         return rec_->fail(HM_ERR_UNSUPPORTED, "the adjoint is available for the single-species model");
@@ -412,7 +440,7 @@ int AdjointSingle::build(const hm_vjp_patch_view* views, int np, Owned& out) {
     for (int i = 0; i < np; i++) {
         AdjPatch& A = face.items[i];
         std::memset(&A, 0, sizeof A);
-        A.f = rec_->devPatch(fv[i], pb);
+        A.f = rec_->devPatch(fv[i], pb, wrec_off ? wrec_off + size_t(i) * 3 : nullptr);
         pb += rec_->patchScratchDoubles(fv[i]);
         for (int a = 0; a < 3; a++) {
             A.Fb[a] = (a < d) ? views[i].F_bar[a] : nullptr;
@@ -464,10 +492,11 @@ int AdjointSingle::build(const hm_vjp_patch_view* views, int np, Owned& out) {
     return HM_OK;
 }
 
-int AdjointSingle::run(const Owned& o, double dt, hipStream_t s) {
+int AdjointSingle::run(const Owned& o, double dt, hipStream_t s, const double* wst) {
     if (int st = rec_->runOwnedPlanStages(o.fwd, dt, 0, 2, s)) return st;  // derived data and sensor
     KParams k = rec_->kernelParams();
     k.dt = dt;
+    k.wrec = const_cast<double*>(wst);  // read only here
     const hm_config& c = rec_->config();
     const hipError_t e = launch_adjoint_single(c.dim, c.scheme, o.plan, k, s);
     if (e != hipSuccess) return rec_->fail(HM_ERR_DEVICE, std::string("adjoint launch failed: ") + hipGetErrorString(e));

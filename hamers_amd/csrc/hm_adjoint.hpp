@@ -7,7 +7,9 @@
 //   1. the forward derived-data and sensor stages (prim V, c; theta, Omega),
 //   2. k_adj_face_flux: per midpoint face, the adjoint of the sixth-order
 //      midpoint-to-face combination, then the reverse sweep of the face's
-//      Riemann flux and back-projection (hm_adjoint_face.hpp), handed over in
+//      Riemann flux and back-projection (hm_adjoint_face.hpp) -- from the face
+//      states a recording forward pass kept, when given, else after
+//      reconstructing them again -- handed over in
 //      a per-face buffer H; k_adj_face_recon: the reverse reconstruction and
 //      characteristic projection, one field at a time; the six stencil
 //      cells' adjoints go to a per-face buffer G (two kernels, so that each
@@ -52,9 +54,13 @@ class AdjointSingle {
         AdjPlanDev plan{};
     };
 
-    int build(const hm_vjp_patch_view* views, int np, Owned& out);
+    // wrec_off (optional, 3 per patch): the patches' chunks in a face-state
+    // slot, for runs given one (run's wst).
+    int build(const hm_vjp_patch_view* views, int np, Owned& out, const int64_t* wrec_off = nullptr);
     void release(Owned& o);
-    int run(const Owned& o, double dt, hipStream_t s);
+    // wst: the face-state slot a recording forward pass wrote for these
+    // inputs (the flux half then reads W-, W+ instead of reconstructing), or NULL.
+    int run(const Owned& o, double dt, hipStream_t s, const double* wst = nullptr);
 
     // One-shot VJP of a patch list (the C-ABI path); keeps the last list's plan.
     int vjp(const hm_vjp_patch_view* views, int np, double dt, hipStream_t s);

@@ -633,50 +633,43 @@ constexpr int face_adj_h_size() {
     return 2 * (D + 2) + 3;
 }
 
-// First half of the adjoint of one face's midpoint flux along axis A: the
-// forward reconstruction, back-projection and bounds check, the Riemann-flux
-// adjoint, then the back-projection adjoint.  V[m][e] are the primitive
-// variables of stencil cells i-3+m, cL/cR the sound speeds of cells i-1 and
-// i, mb the adjoint of the midpoint flux.  Writes h[0..NEQ) = adjoint of the
-// minus-side characteristic values (or of V_L where the first-order fallback
-// was taken), h[NEQ..2NEQ) = the plus side's (or V_R's), h[2NEQ], h[2NEQ+1] =
-// the adjoints of rho_avg and c_avg so far, h[2NEQ+2] = 1 (reconstructed) or
-// 0 (fallback).
-template <int D, int A, int SCHEME, int P, int Q>
-HM_HD void face_adj_single_flux(const double (&V)[6][D + 2], double cL, double cR, double gam, const LDConst& ld,
-                                bool use_hll, const double* mb, double* h) {
+// Back-projection of characteristic face values W to primitive variables Vo
+// (the forward code's expression trees, FlowModelBasicUtilitiesSingleSpecies.cpp:7279-7323).
+template <int D, int A>
+HM_HD void back_project_single(const double* W, double ra, double ca, double* Vo) {
+    constexpr int IP = D + 1;
+    Vo[0] = double(1) / (ca * ca) * W[0] + W[1] + double(1) / (ca * ca) * W[IP];
+    Vo[1 + A] = -double(1) / (ra * ca) * W[0] + double(1) / (ra * ca) * W[IP];
+    int t = 2;
+#pragma unroll
+    for (int b = 0; b < D; b++) {
+        if (b == A) continue;
+        Vo[1 + b] = W[t++];
+    }
+    Vo[IP] = W[0] + W[IP];
+}
+
+// The part of the face's first adjoint half after the reconstruction: the
+// positivity check and fallback, the Riemann-flux adjoint and the
+// back-projection adjoint.  Wm, Wp are the reconstructed characteristic
+// values (recomputed, or kept by a recording forward pass), V2, V3 the
+// primitive variables of cells i-1 and i, ra, ca the projection averages.
+template <int D, int A>
+HM_HD void face_adj_single_after_recon(const double* Wm, const double* Wp, const double* V2, const double* V3,
+                                       double ra, double ca, double gam, bool use_hll, const double* mb, double* h) {
     constexpr int NEQ = D + 2;
     constexpr int IP = D + 1;
-    const double ra = double(1) / double(2) * (V[2][0] + V[3][0]);
-    const double ca = double(1) / double(2) * (cL + cR);
-    double Wm[NEQ], Wp[NEQ];
-#pragma unroll
-    for (int f = 0; f < NEQ; f++) {
-        Six s;
-#pragma unroll
-        for (int m = 0; m < 6; m++) s.v[m] = char_field<D, A>(V, ra, ca, f, m);
-        reconstruct_pair<SCHEME, P, Q>(s, ld, Wm[f], Wp[f]);
-    }
     double VL[NEQ], VR[NEQ];
-    auto back = [&](const double* W, double* Vo) {
-        Vo[0] = double(1) / (ca * ca) * W[0] + W[1] + double(1) / (ca * ca) * W[IP];
-        Vo[1 + A] = -double(1) / (ra * ca) * W[0] + double(1) / (ra * ca) * W[IP];
-        int t = 2;
-        for (int b = 0; b < D; b++) {
-            if (b == A) continue;
-            Vo[1 + b] = W[t++];
-        }
-        Vo[IP] = W[0] + W[IP];
-    };
-    back(Wm, VL);
-    back(Wp, VR);
+    back_project_single<D, A>(Wm, ra, ca, VL);
+    back_project_single<D, A>(Wp, ra, ca, VR);
     const bool ok = (VL[0] > double(0)) && (VL[IP] > double(0)) && (VR[0] > double(0)) && (VR[IP] > double(0));
-    if (!ok) {
+    // Element-wise selects (a conditional copy from V2, V3 would make the
+    // compiler keep the stencil in scratch memory to address it).
 #pragma unroll
-        for (int e = 0; e < NEQ; e++) {
-            VL[e] = V[2][e];
-            VR[e] = V[3][e];
-        }
+    for (int e = 0; e < NEQ; e++) {
+        const double l = V2[e], r = V3[e];
+        VL[e] = ok ? VL[e] : l;
+        VR[e] = ok ? VR[e] : r;
     }
     double VLb[NEQ], VRb[NEQ];
 #pragma unroll
@@ -705,6 +698,7 @@ HM_HD void face_adj_single_flux(const double (&V)[6][D + 2], double cL, double c
         Wb[1] = Vob[0];
         Wb[IP] = Vob[0] * ic2 + Vob[1 + A] * irc + Vob[IP];
         int t = 2;
+#pragma unroll
         for (int b = 0; b < D; b++) {
             if (b == A) continue;
             Wb[t++] = Vob[1 + b];
@@ -724,6 +718,32 @@ HM_HD void face_adj_single_flux(const double (&V)[6][D + 2], double cL, double c
     h[2 * NEQ] = rab;
     h[2 * NEQ + 1] = cab;
     h[2 * NEQ + 2] = 1.0;
+}
+
+// First half of the adjoint of one face's midpoint flux along axis A: the
+// forward reconstruction, back-projection and bounds check, the Riemann-flux
+// adjoint, then the back-projection adjoint.  V[m][e] are the primitive
+// variables of stencil cells i-3+m, cL/cR the sound speeds of cells i-1 and
+// i, mb the adjoint of the midpoint flux.  Writes h[0..NEQ) = adjoint of the
+// minus-side characteristic values (or of V_L where the first-order fallback
+// was taken), h[NEQ..2NEQ) = the plus side's (or V_R's), h[2NEQ], h[2NEQ+1] =
+// the adjoints of rho_avg and c_avg so far, h[2NEQ+2] = 1 (reconstructed) or
+// 0 (fallback).
+template <int D, int A, int SCHEME, int P, int Q>
+HM_HD void face_adj_single_flux(const double (&V)[6][D + 2], double cL, double cR, double gam, const LDConst& ld,
+                                bool use_hll, const double* mb, double* h) {
+    constexpr int NEQ = D + 2;
+    const double ra = double(1) / double(2) * (V[2][0] + V[3][0]);
+    const double ca = double(1) / double(2) * (cL + cR);
+    double Wm[NEQ], Wp[NEQ];
+#pragma unroll
+    for (int f = 0; f < NEQ; f++) {
+        Six s;
+#pragma unroll
+        for (int m = 0; m < 6; m++) s.v[m] = char_field<D, A>(V, ra, ca, f, m);
+        reconstruct_pair<SCHEME, P, Q>(s, ld, Wm[f], Wp[f]);
+    }
+    face_adj_single_after_recon<D, A>(Wm, Wp, V[2], V[3], ra, ca, gam, use_hll, mb, h);
 }
 
 // Size of the per-face record the second half writes.

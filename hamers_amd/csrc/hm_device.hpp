@@ -25,6 +25,7 @@ struct DevPatch {
     double* sens;                 // scratch: theta, Omega over the 2-ghost box
     double* fmid[3];              // scratch: midpoint flux per axis, neq comps over (n_a+3) x n_b box
     double* ustar[3];             // scratch: HLLC normal velocity per axis (advective models)
+    int64_t wrec[3];              // offset of this patch's axis-a chunk in a face-state slot (KParams::wrec), -1: none
 };
 
 // Extents of the boxes the kernels walk, for a patch of dimension D.

@@ -123,9 +123,16 @@ HM_HD void riemann_flux_single(const double* VL, const double* VR, double gam, b
     }
 }
 
-template <int D, int A, int SCHEME, int PP, int QQ>
+// Face-state sink of face_flux_single that keeps nothing.
+struct NoFaceStates {
+    HM_HD void operator()(const double*, const double*) const {}
+};
+
+// sink(W-, W+) receives the reconstructed characteristic face values (a
+// recording forward pass keeps them for the adjoint).
+template <int D, int A, int SCHEME, int PP, int QQ, class Sink = NoFaceStates>
 HM_HD void face_flux_single(const double (&V)[6][D + 2], double cL, double cR, double gam, const LDConst& ld,
-                            bool use_hll, double* Fo) {
+                            bool use_hll, double* Fo, const Sink& sink = Sink{}) {
     constexpr int NEQ = D + 2;
     constexpr int IP = D + 1;  // pressure slot of V
 
@@ -168,6 +175,7 @@ HM_HD void face_flux_single(const double (&V)[6][D + 2], double cL, double cR, d
             s.v[m] = double(1) / double(2) * ra * ca * V[m][1 + A] + double(1) / double(2) * V[m][IP];
         reconstruct_pair<SCHEME, PP, QQ>(s, ld, Wm[IP], Wp[IP]);
     }
+    sink(Wm, Wp);
 
     // Back to primitive variables (FlowModelBasicUtilitiesSingleSpecies.cpp:7279-7284 and analogues).
     double VL[NEQ], VR[NEQ];

@@ -17,6 +17,12 @@ struct KParams {
     double cv[HM_MAX_SPECIES];      // species c_v = 1/(gamma-1) R
     int num_species;
     double dt;
+    // Face-state slot (single-species model): the characteristic face values
+    // W-, W+ of every midpoint face, chunk DevPatch::wrec[a] per patch and
+    // axis, [2 NEQ][midpoint faces].  The recording forward pass writes it,
+    // the adjoint's flux half reads it instead of reconstructing again.
+    // NULL: not used.
+    double* wrec;
 };
 
 // Work of one RHS evaluation: one work source per pipeline stage

@@ -12,6 +12,8 @@
 //               (WCNS56-HLLC-HLL.cpp:879-1194, 1746-2320)
 //   k_faceflux  every face: sixth-order midpoint-to-face flux times dt
 //               (WCNS56-HLLC-HLL.cpp:1197-1282, 2327-2490)
+#include <type_traits>
+
 #include "hm_kernels.hpp"
 #include "hm_face_single.hpp"
 #include "hm_numerics.hpp"
@@ -67,7 +69,7 @@ __global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
 // ----------------------------------------------------------------------------
 // Fused midpoint flux of one face (single-species).
 // ----------------------------------------------------------------------------
-template <int D, int A, int SCHEME, int PP, int QQ>
+template <int D, int A, int SCHEME, int PP, int QQ, bool REC>
 __device__ __forceinline__ void midflux_single_face(const DevPatch& P, int64_t loc, const KParams& k) {
     constexpr int NEQ = D + 2;
     const Boxes<D> B(P);
@@ -109,15 +111,29 @@ __device__ __forceinline__ void midflux_single_face(const DevPatch& P, int64_t l
     }
 
     double Fo[NEQ];
-    face_flux_single<D, A, SCHEME, PP, QQ>(V, cL, cR, k.gamma[0], LDConst{k.p, k.q, k.C, k.alpha_tau}, use_hll, Fo);
-
     const int64_t mv = B.mid_vol(A);
+    if constexpr (REC) {
+        // Recording pass: keep W-, W+ in the face-state slot.
+        double* wr = k.wrec + P.wrec[A] + loc;
+        face_flux_single<D, A, SCHEME, PP, QQ>(V, cL, cR, k.gamma[0], LDConst{k.p, k.q, k.C, k.alpha_tau}, use_hll,
+                                               Fo, [&](const double* Wm, const double* Wp) {
+#pragma unroll
+                                                   for (int c = 0; c < NEQ; c++) {
+                                                       wr[c * mv] = Wm[c];
+                                                       wr[(NEQ + c) * mv] = Wp[c];
+                                                   }
+                                               });
+    } else {
+        face_flux_single<D, A, SCHEME, PP, QQ>(V, cL, cR, k.gamma[0], LDConst{k.p, k.q, k.C, k.alpha_tau}, use_hll,
+                                               Fo);
+    }
+
     double* out = P.fmid[A];
 #pragma unroll
     for (int e = 0; e < NEQ; e++) out[e * mv + loc] = Fo[e];
 }
 
-template <int D, int SCHEME, int PP, int QQ, class W>
+template <int D, int SCHEME, int PP, int QQ, class W, bool REC = false>
 __global__ __launch_bounds__(kBlock) void k_midflux_single(W w, KParams k) {
     const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     const DevPatch* P;
@@ -125,12 +141,12 @@ __global__ __launch_bounds__(kBlock) void k_midflux_single(W w, KParams k) {
     int64_t loc;
     if (!w.find(gid, P, ax, loc)) return;
     if (ax == 0) {
-        midflux_single_face<D, 0, SCHEME, PP, QQ>(*P, loc, k);
+        midflux_single_face<D, 0, SCHEME, PP, QQ, REC>(*P, loc, k);
     } else if constexpr (D > 1) {
         if (ax == 1) {
-            midflux_single_face<D, 1, SCHEME, PP, QQ>(*P, loc, k);
+            midflux_single_face<D, 1, SCHEME, PP, QQ, REC>(*P, loc, k);
         } else if constexpr (D > 2) {
-            midflux_single_face<D, 2, SCHEME, PP, QQ>(*P, loc, k);
+            midflux_single_face<D, 2, SCHEME, PP, QQ, REC>(*P, loc, k);
         }
     }
 }
@@ -222,9 +238,18 @@ hipError_t launch_single_dsq(const Plan<W>& plan, const KParams& k, hipStream_t 
             hipLaunchKernelGGL((k_sensor<D, 1, W>), dim3(unsigned((plan.items[1] + kBlock - 1) / kBlock)),
                                dim3(kBlock), 0, s, plan.work[1], k);
     }
-    if (plan.items[2] > 0)
-        hipLaunchKernelGGL((k_midflux_single<D, SCHEME, PP, QQ, W>),
-                           dim3(unsigned((plan.items[2] + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, plan.work[2], k);
+    if (plan.items[2] > 0) {
+        const dim3 grid(unsigned((plan.items[2] + kBlock - 1) / kBlock));
+        bool rec = false;
+        if constexpr (std::is_same_v<W, SegTable>) rec = k.wrec != nullptr;  // recording runs use patch tables
+        if (rec) {
+            if constexpr (std::is_same_v<W, SegTable>)
+                hipLaunchKernelGGL((k_midflux_single<D, SCHEME, PP, QQ, W, true>), grid, dim3(kBlock), 0, s,
+                                   plan.work[2], k);
+        } else {
+            hipLaunchKernelGGL((k_midflux_single<D, SCHEME, PP, QQ, W>), grid, dim3(kBlock), 0, s, plan.work[2], k);
+        }
+    }
     if (plan.items[3] > 0)
         hipLaunchKernelGGL((k_faceflux_single<D, W>), dim3(unsigned((plan.items[3] + kBlock - 1) / kBlock)),
                            dim3(kBlock), 0, s, plan.work[3], k);

@@ -317,6 +317,7 @@ class Level {
             for (auto& p : vjp_plan_) adj_->release(p);
         if (adj_mem_) (void)hipFree(adj_mem_);
         if (store_) (void)hipFree(store_);
+        if (wst_) (void)hipFree(wst_);
         if (rhs_scratch_) (void)hipFree(rhs_scratch_);
         if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
         if (ev_a_) (void)hipEventDestroy(ev_a_);
@@ -503,9 +504,9 @@ class Level {
             const int n0 = seg * every_, n1 = std::min(n0 + every_, rec_nsteps_);
             if (int st = store_op(0, 1 + seg, 1, s)) return st;
             for (int t = n0; t < n1; t++)
-                if (int st = step(rec_dt_, s, traj0 + (t - n0) * nst_)) return st;
+                if (int st = step(rec_dt_, s, traj0 + (t - n0) * nst_, (t - n0) * nst_)) return st;
             for (int t = n1 - 1; t >= n0; t--)
-                if (int st = adjoint_step(traj0 + (t - n0) * nst_, s)) return st;
+                if (int st = adjoint_step(traj0 + (t - n0) * nst_, (t - n0) * nst_, s)) return st;
         }
         return store_op(0, 0, 1, s);
     }
@@ -600,8 +601,22 @@ class Level {
         const size_t sd = rec_->scratchDoubles(views_[0].data(), np_);
         if (hipMalloc(&rhs_scratch_, sd * sizeof(double)) != hipSuccess)
             return rec_->fail(HM_ERR_OUT_OF_MEMORY, "level RHS scratch");
+        // Face-state slot layout (single-species model): per patch and axis,
+        // 2 NEQ values per midpoint face.
+        const bool single = rec_->config().model == HM_MODEL_SINGLE_SPECIES;
+        wst_off_.assign(size_t(np_) * 3, -1);
+        wst_slot_doubles_ = 0;
+        if (single)
+            for (int p = 0; p < np_; p++)
+                for (int a = 0; a < d_; a++) {
+                    int64_t mv = 1;
+                    for (int b = 0; b < d_; b++) mv *= n_[3 * p + b] + (b == a ? 3 : 0);
+                    wst_off_[size_t(p) * 3 + a] = int64_t(wst_slot_doubles_);
+                    wst_slot_doubles_ += up256(sizeof(double) * size_t(2 * neq_) * size_t(mv)) / sizeof(double);
+                }
         for (int s = 0; s < kMaxRK; s++)
-            if (int st = rec_->buildOwnedPlan(views_[s].data(), np_, static_cast<double*>(rhs_scratch_), rhs_plan_[s]))
+            if (int st = rec_->buildOwnedPlan(views_[s].data(), np_, static_cast<double*>(rhs_scratch_), rhs_plan_[s],
+                                              single ? wst_off_.data() : nullptr))
                 return st;
         if (hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&ev_a_, hipEventDisableTiming) != hipSuccess ||
@@ -768,8 +783,10 @@ class Level {
 
     // One Runge-Kutta step (the sequence the graph captures).  With
     // record_slot >= 0 the ghost-filled input of stage sn is also stored in
-    // slot record_slot + sn (recompute pass of the adjoint).
-    int step(double dt, hipStream_t s, int record_slot = -1) {
+    // slot record_slot + sn (recompute pass of the adjoint), and with
+    // face_slot >= 0 and a face-state store its RHS keeps the face states in
+    // face-state slot face_slot + sn.
+    int step(double dt, hipStream_t s, int record_slot = -1, int face_slot = -1) {
         if (int st = launch_halo(0, s)) return st;
         for (int sn = 0; sn < nst_; sn++) {
             hipLaunchKernelGGL(k_copy_state, dim3(blocks_for(copy_items_)), dim3(kBlock), 0, s, copy_, 0, 1 + sn, nq_);
@@ -779,7 +796,8 @@ class Level {
                 if (int st = store_op(1 + sn, record_slot + sn, 0, s)) return st;
             hipLaunchKernelGGL(k_zero_source, dim3(blocks_for(zero_items_)), dim3(kBlock), 0, s, zero_, sn);
             if (hipGetLastError() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "level launch failed");
-            if (int st = rec_->runOwnedPlan(rhs_plan_[sn], dt, s)) return st;
+            if (int st = rec_->runOwnedPlan(rhs_plan_[sn], dt, s, face_slot >= 0 ? face_states(face_slot + sn) : nullptr))
+                return st;
             if (launch_rk_update<SegTableT<RKPatch>>(d_, rk_[sn], rk_items_, nullptr, 0, rkp_[sn], s) != hipSuccess)
                 return rec_->fail(HM_ERR_DEVICE, "stage update launch failed");
         }
@@ -828,7 +846,7 @@ class Level {
     // beta, then one k_stage_adjoint launch for the alpha terms and the last
     // VJP's output (earlier VJPs of the same stage, which only tableaux with
     // several nonzero beta per stage have, add theirs with k_halo_reverse).
-    int adjoint_step(int slot0, hipStream_t s) {
+    int adjoint_step(int slot0, int face_slot0, hipStream_t s) {
         if (zero_first_) {
             if (int st = lam_op(3, nst_, 0, 0.0, s)) return st;
         }
@@ -845,7 +863,7 @@ class Level {
                     default: hipLaunchKernelGGL(k_adj_div<3>, dim3(blocks_for(adj_face_items_)), dim3(kBlock), 0, s, adj_faces_, sn + 1, beta_[sn][m], neq_, g_); break;
                 }
                 if (hipGetLastError() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "adjoint launch failed");
-                if (int st = adj_->run(vjp_plan_[m], rec_dt_, s)) return st;
+                if (int st = adj_->run(vjp_plan_[m], rec_dt_, s, face_states(face_slot0 + m))) return st;
                 if (m == last_vjp) break;
                 switch (d_) {
                     case 1: hipLaunchKernelGGL(k_halo_reverse<1>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, rev_dev_, m, g_); break;
@@ -1042,7 +1060,7 @@ class Level {
                     for (int a = 0; a < d_; a++) v.F_bar[a] = faces.items[p].fbar[a];
                     vv[p] = v;
                 }
-                if (int st = adj_->build(vv.data(), np_, vjp_plan_[m])) return st;
+                if (int st = adj_->build(vv.data(), np_, vjp_plan_[m], wst_off_.data())) return st;
             }
             adj_nst_ = nst_;
         }
@@ -1065,9 +1083,57 @@ class Level {
                 store_slots_ = nslots;
             }
             nckpt_ = nckpt;
+            // Face states of one segment's stages, when they fit the budget;
+            // else the adjoint's flux half reconstructs them again.
+            const size_t nfs = size_t(every) * nst_;
+            const size_t want = nfs * wst_slot_doubles_ * sizeof(double);
+            if (nfs > wst_slots_ || want == 0) {
+                if (wst_) {
+                    (void)hipDeviceSynchronize();
+                    (void)hipFree(wst_);
+                    wst_ = nullptr;
+                    wst_slots_ = 0;
+                }
+                size_t budget = 0;
+                if (face_state_budget_ < 0) {
+                    size_t fr = 0, tot = 0;
+                    if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget = fr / 2;
+                } else {
+                    budget = size_t(face_state_budget_);
+                }
+                if (want > 0 && want <= budget) {
+                    if (hipMalloc(reinterpret_cast<void**>(&wst_), want) == hipSuccess)
+                        wst_slots_ = nfs;
+                    else
+                        wst_ = nullptr;  // not fatal: reconstruct instead
+                }
+            }
         }
         return HM_OK;
     }
+
+    // Face-state slot i, or NULL without a store.
+    double* face_states(int i) const {
+        return (wst_ && i >= 0 && size_t(i) < wst_slots_) ? wst_ + size_t(i) * wst_slot_doubles_ : nullptr;
+    }
+
+   public:
+    // Device memory the adjoint may use to keep face states between its
+    // recompute and reverse passes: < 0 half of the free memory (default),
+    // 0 none.  Takes effect at the next advanceRecorded.
+    int setFaceStateBudget(int64_t bytes) {
+        face_state_budget_ = bytes;
+        if (wst_) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(wst_);
+            wst_ = nullptr;
+            wst_slots_ = 0;
+        }
+        return HM_OK;
+    }
+    bool faceStatesKept() const { return wst_ != nullptr; }
+
+   private:
 
     void drop_graphs() {
         if (graphs_.empty()) return;
@@ -1114,6 +1180,10 @@ class Level {
     int adj_nst_ = 0;
     double* store_ = nullptr;
     size_t slot_doubles_ = 0;
+    double* wst_ = nullptr;              // face-state store: wst_slots_ slots of wst_slot_doubles_
+    size_t wst_slots_ = 0, wst_slot_doubles_ = 0;
+    std::vector<int64_t> wst_off_;       // per patch and axis: chunk offset inside a slot
+    int64_t face_state_budget_ = -1;
     int store_slots_ = 0, nckpt_ = 0;
     bool recorded_ = false;
     double rec_dt_ = 0.0;
@@ -1338,6 +1408,16 @@ int hm_level_advance_recorded(hm_level* level, double dt, int num_steps, int che
     if (!level) return HM_ERR_INVALID_ARGUMENT;
     return level->lvl->advanceRecorded(dt, num_steps, checkpoint_every, use_graph != 0,
                                        static_cast<hipStream_t>(stream));
+}
+
+int hm_level_set_face_state_budget(hm_level* level, int64_t bytes) {
+    if (!level) return HM_ERR_INVALID_ARGUMENT;
+    return level->lvl->setFaceStateBudget(bytes);
+}
+
+int hm_level_face_states_kept(hm_level* level) {
+    if (!level) return HM_ERR_INVALID_ARGUMENT;
+    return level->lvl->faceStatesKept() ? 1 : 0;
 }
 
 int hm_level_adjoint(hm_level* level, void* stream) {

@@ -70,6 +70,7 @@ class ConvectiveFluxReconstructorGPU {
         }
         k_.num_species = cfg.num_species;
         k_.dt = 0.0;
+        k_.wrec = nullptr;
         has_advective_ = (cfg.model == HM_MODEL_FIVE_EQN_ALLAIRE && cfg.num_species > 1);
         plans_.reserve(kMaxPlans);  // plan addresses stay valid while cached
     }
@@ -185,12 +186,15 @@ class ConvectiveFluxReconstructorGPU {
         return total;
     }
 
-    int buildOwnedPlan(const hm_patch_view* views, int np, double* scratch, OwnedPlan& out) {
+    // wrec_off (optional, 3 per patch): the patches' chunks in a face-state
+    // slot, for runs that record the face states (runOwnedPlan's wrec).
+    int buildOwnedPlan(const hm_patch_view* views, int np, double* scratch, OwnedPlan& out,
+                       const int64_t* wrec_off = nullptr) {
         for (int i = 0; i < np; i++)
             if (int st = check_view(views[i], i)) return st;
         if (int st = select_device()) return st;
         releaseOwnedPlan(out);
-        return upload_tables(views, np, scratch, out.dev, out.plan);
+        return upload_tables(views, np, scratch, out.dev, out.plan, wrec_off);
     }
 
     void releaseOwnedPlan(OwnedPlan& p) {
@@ -198,19 +202,29 @@ class ConvectiveFluxReconstructorGPU {
         p.dev = nullptr;
     }
 
-    // All five stages of an owned plan; no profiling events (graph capture safe).
-    int runOwnedPlan(const OwnedPlan& p, double dt, hipStream_t s) { return runOwnedPlanStages(p, dt, 0, kStages, s); }
+    // All five stages of an owned plan; no profiling events (graph capture
+    // safe).  wrec: face-state slot to record into (single-species model,
+    // plans built with wrec_off), NULL for none.
+    int runOwnedPlan(const OwnedPlan& p, double dt, hipStream_t s, double* wrec = nullptr) {
+        return runOwnedPlanStages(p, dt, 0, kStages, s, wrec);
+    }
 
     // Pipeline stages [first, last) of an owned plan.
-    int runOwnedPlanStages(const OwnedPlan& p, double dt, int first, int last, hipStream_t s) {
+    int runOwnedPlanStages(const OwnedPlan& p, double dt, int first, int last, hipStream_t s, double* wrec = nullptr) {
         KParams k = k_;
         k.dt = dt;
+        k.wrec = wrec;
         return run_stages(p.plan, k, s, first, last);
     }
 
     // Device descriptor of a patch whose scratch starts at `base` (the layout
     // of scratchDoubles / buildOwnedPlan, patch after patch).
-    DevPatch devPatch(const hm_patch_view& v, double* base) const { return make_dev_patch(v, base, layout(v)); }
+    DevPatch devPatch(const hm_patch_view& v, double* base, const int64_t* wrec_off = nullptr) const {
+        DevPatch p = make_dev_patch(v, base, layout(v));
+        if (wrec_off)
+            for (int a = 0; a < 3; a++) p.wrec[a] = wrec_off[a];
+        return p;
+    }
     size_t patchScratchDoubles(const hm_patch_view& v) const { return layout(v).total; }
     const KParams& kernelParams() const { return k_; }
 
@@ -338,6 +352,7 @@ class ConvectiveFluxReconstructorGPU {
         DevPatch p{};
         const int d = cfg_.dim;
         for (int a = 0; a < 3; a++) {
+            p.wrec[a] = -1;
             p.n[a] = (a < d) ? v.n[a] : 1;
             p.gq[a] = (a < d) ? v.ghost[a] : 0;
             p.dx[a] = (a < d) ? v.dx[a] : 1.0;
@@ -465,7 +480,8 @@ class ConvectiveFluxReconstructorGPU {
 
     // Device work tables of a patch list whose scratch starts at `base`:
     // DevPatch[np], then per stage start/cnt/seg, in one allocation.
-    int upload_tables(const hm_patch_view* views, int np, double* base, void*& dev, Plan<SegTable>& plan) {
+    int upload_tables(const hm_patch_view* views, int np, double* base, void*& dev, Plan<SegTable>& plan,
+                      const int64_t* wrec_off = nullptr) {
         const int d = cfg_.dim;
         std::vector<DevPatch> patches(np);
         std::vector<int64_t> start[kStages], cnt[kStages];
@@ -473,6 +489,8 @@ class ConvectiveFluxReconstructorGPU {
         for (int i = 0; i < np; i++) {
             const ScratchLayout L = layout(views[i]);
             patches[i] = make_dev_patch(views[i], base, L);
+            if (wrec_off)
+                for (int a = 0; a < 3; a++) patches[i].wrec[a] = wrec_off[size_t(i) * 3 + a];
             base += L.total;
         }
         int64_t acc[kStages] = {0, 0, 0, 0, 0};

@@ -227,6 +227,17 @@ class Level:
         """Turn the adjoint of the final state into that of the initial state."""
         capi.check(self.rec._ctx, capi.lib.hm_level_adjoint(self._lvl, self.rec._stream_ptr(stream)))
 
+    def set_face_state_budget(self, nbytes: int):
+        """Bytes of device memory adjoint() may keep face states in between its
+        recompute and reverse passes (< 0: half the free memory, 0: none)."""
+        capi.check(self.rec._ctx, capi.lib.hm_level_set_face_state_budget(self._lvl, int(nbytes)))
+
+    @property
+    def face_states_kept(self) -> bool:
+        r = capi.lib.hm_level_face_states_kept(self._lvl)
+        capi.check(self.rec._ctx, min(r, 0))
+        return r == 1
+
     def write_adjoint(self, i: int, lam: np.ndarray):
         bn = self.boxes[i][1]
         shape = (self.rec.num_eqn,) + tuple(reversed(bn))

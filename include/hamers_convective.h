@@ -336,6 +336,20 @@ int hm_level_write_adjoint(hm_level* level, int patch, const double* lam);
 int hm_level_adjoint(hm_level* level, void* stream);
 int hm_level_read_adjoint(hm_level* level, int patch, double* lam);
 
+/*
+ * Trajectory storage of hm_level_adjoint (no counterpart in the reference).
+ * Its recompute pass can keep the reconstructed characteristic face values of
+ * every stage of a checkpoint segment (2 num_eqn doubles per midpoint face and
+ * stage, checkpoint_every * num_stages stages), so that the reverse pass need
+ * not reconstruct them again; the gradient is the same bit for bit either way.
+ * hm_level_set_face_state_budget bounds that store in bytes: < 0 half of the
+ * device memory free when hm_level_advance_recorded sets it up (the default),
+ * 0 never.  hm_level_face_states_kept returns 1 when the last
+ * hm_level_advance_recorded set a store up, else 0.
+ */
+int hm_level_set_face_state_budget(hm_level* level, int64_t bytes);
+int hm_level_face_states_kept(hm_level* level);
+
 #ifdef __cplusplus
 }
 #endif

@@ -268,6 +268,43 @@ def test_level_gradient_independent_of_checkpoint_interval(torch_dev):
     assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
 
 
+def _level3d_adjoint(budget, N=16, P=8, nsteps=4, every=2):
+    """dJ/dQ0 of J = <w, Q(T)> on a periodic 3D level of (N/P)^3 patches."""
+    from hamers_amd import ConvectiveFluxReconstructor, Level
+
+    rec = ConvectiveFluxReconstructor("adjoint 3D", 3, "SINGLE_SPECIES", "WCNS6_LD_HLLC_HLL", species_gamma=(1.4,))
+    boxes = [((i, j, k), (P, P, P)) for k in range(0, N, P) for j in range(0, N, P) for i in range(0, N, P)]
+    Q0, n, dx = _smooth(N, 3)
+    lvl = Level(rec, boxes, ((0, 0, 0), n), (1, 1, 1), dx)
+    lvl.set_face_state_budget(budget)
+    lvl.write_global(Q0)
+    lvl.advance_recorded(0.2 * dx[0], nsteps, every)
+    w = np.random.default_rng(5).standard_normal(Q0.shape)
+    lvl.write_adjoint_global(w)
+    lvl.adjoint()
+    return lvl.read_adjoint_global(Q0.shape), lvl.face_states_kept
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_level_gradient_with_and_without_face_states(torch_dev, dim):
+    """Face states kept by the recompute pass are the values the adjoint's
+    flux half would reconstruct again: the gradient is the same bit for bit."""
+    out = []
+    for budget in (-1, 0):
+        _say(f"case {dim}D level adjoint, face-state budget {budget}")
+        if dim == 2:
+            prob = _kh(every=4, nsteps=8)
+            prob.level.set_face_state_budget(budget)
+            g = prob.gradient(prob.a0)[1]
+            kept = prob.level.face_states_kept
+        else:
+            g, kept = _level3d_adjoint(budget)
+        assert kept == (budget != 0)
+        out.append(g)
+    assert np.all(np.isfinite(out[0]))
+    assert np.array_equal(out[0], out[1])
+
+
 def test_autograd_harness(torch_dev):
     torch = torch_dev
     from hamers_amd.adjoint import kh_objective_function
