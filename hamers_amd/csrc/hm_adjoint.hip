@@ -54,6 +54,36 @@ __device__ __forceinline__ void load_stencil(const DevPatch& F, const Boxes<D>& 
     cR = pr[NEQ * v4 + iR];
 }
 
+// Level runs (KParams::lam_src >= 0): the stage adjoint along the line of
+// axis a through cell (c0, c1, c2), and the face-flux adjoint it implies,
+// F_bar_f = lam_scale (lam_f - lam_{f-1}) / dx_a for output faces f = 0..n_a
+// (lam is zero outside the interior).
+template <int D>
+struct LamLine {
+    const double* L;  // component 0 at the line's cell 0
+    int64_t vol, st;
+    int n;
+    double sc;
+    __device__ __forceinline__ LamLine(const AdjPatch& P, const Boxes<D>& B, const KParams& k, int a, int c0, int c1,
+                                       int c2) {
+        const int lg = P.lam.g;
+        const int64_t le0 = B.n[0] + 2 * lg, le1 = (D > 1) ? B.n[1] + 2 * lg : 1;
+        vol = le0 * le1 * ((D > 2) ? B.n[2] + 2 * lg : 1);
+        st = (a == 0) ? 1 : (a == 1) ? le0 : le0 * le1;
+        const int x0 = (a == 0) ? 0 : c0, x1 = (a == 1) ? 0 : c1, x2 = (a == 2) ? 0 : c2;
+        const int64_t i0 = (x0 + lg) + int64_t(x1 + ((D > 1) ? lg : 0)) * le0 + int64_t(x2 + ((D > 2) ? lg : 0)) * le0 * le1;
+        L = P.lam.L0 + int64_t(k.lam_src) * P.lam.stride + i0;
+        n = B.n[a];
+        sc = k.lam_scale / P.f.dx[a];
+    }
+    __device__ __forceinline__ double cell(int e, int x) const {
+        return (x >= 0 && x < n) ? L[e * vol + int64_t(x) * st] : 0.0;
+    }
+    __device__ __forceinline__ double face(int e, int f) const {
+        return (f >= 0 && f <= n) ? sc * (cell(e, f) - cell(e, f - 1)) : 0.0;
+    }
+};
+
 // Per midpoint face of axis A: adjoint of the midpoint-to-face combination
 // F_i = dt (1/30 (m_{i+1} + m_{i-1}) - 3/10 (n_i + n_{i-1}) + 23/15 m_i)
 // (WCNS56-HLLC-HLL.cpp:1200-1282) with respect to the midpoint flux m_j, then
@@ -74,17 +104,27 @@ __device__ __forceinline__ void adj_face_flux(const AdjPatch& P, int64_t loc, co
     auto fidx = [&](int jj) -> int64_t {
         return B.face_idx(A, (A == 0) ? jj : c0, (A == 1) ? jj : c1, (A == 2) ? jj : c2);
     };
-    const double* Fb = P.Fb[A];
     const double dt = k.dt;
     double mb[NEQ];
     bool any = false;
+    if (k.lam_src >= 0) {
+        const LamLine<D> Ll(P, B, k, A, c0, c1, c2);
 #pragma unroll
-    for (int e = 0; e < NEQ; e++) {
-        const double fl = (j - 1 >= 0 && j - 1 <= nA) ? Fb[e * fv + fidx(j - 1)] : 0.0;
-        const double fr = (j + 1 >= 0 && j + 1 <= nA) ? Fb[e * fv + fidx(j + 1)] : 0.0;
-        const double fc = (j >= 0 && j <= nA) ? Fb[e * fv + fidx(j)] : 0.0;
-        mb[e] = dt * (1.0 / 30.0 * (fl + fr) + 23.0 / 15.0 * fc);
-        any = any || (mb[e] != 0.0);
+        for (int e = 0; e < NEQ; e++) {
+            const double fl = Ll.face(e, j - 1), fc = Ll.face(e, j), fr = Ll.face(e, j + 1);
+            mb[e] = dt * (1.0 / 30.0 * (fl + fr) + 23.0 / 15.0 * fc);
+            any = any || (mb[e] != 0.0);
+        }
+    } else {
+        const double* Fb = P.Fb[A];
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) {
+            const double fl = (j - 1 >= 0 && j - 1 <= nA) ? Fb[e * fv + fidx(j - 1)] : 0.0;
+            const double fr = (j + 1 >= 0 && j + 1 <= nA) ? Fb[e * fv + fidx(j + 1)] : 0.0;
+            const double fc = (j >= 0 && j <= nA) ? Fb[e * fv + fidx(j)] : 0.0;
+            mb[e] = dt * (1.0 / 30.0 * (fl + fr) + 23.0 / 15.0 * fc);
+            any = any || (mb[e] != 0.0);
+        }
     }
     const int64_t mv = B.mid_vol(A);
     double* H = P.H[A];
@@ -282,11 +322,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(W w, KParams k) {
             };
             const bool hasL = c[a] >= 0, hasR = c[a] + 1 <= na;
             double nb[NEQ];
+            if (k.lam_src >= 0) {
+                const LamLine<D> Ll(*P, B, k, a, c[0], c[1], c[2]);
 #pragma unroll
-            for (int e = 0; e < NEQ; e++) {
-                const double fL = hasL ? Fb[e * fv + fidx(c[a])] : 0.0;
-                const double fR = hasR ? Fb[e * fv + fidx(c[a] + 1)] : 0.0;
-                nb[e] = -dt * (3.0 / 10.0) * (fL + fR);
+                for (int e = 0; e < NEQ; e++) nb[e] = -dt * (3.0 / 10.0) * (Ll.face(e, c[a]) + Ll.face(e, c[a] + 1));
+            } else {
+#pragma unroll
+                for (int e = 0; e < NEQ; e++) {
+                    const double fL = hasL ? Fb[e * fv + fidx(c[a])] : 0.0;
+                    const double fR = hasR ? Fb[e * fv + fidx(c[a] + 1)] : 0.0;
+                    nb[e] = -dt * (3.0 / 10.0) * (fL + fR);
+                }
             }
             // n[0] = Q[1+a]; n[1+b] = u_a Q[1+b] (+ p if b == a); n[IP] = u_a (E + p)
             Qbd[1 + a] += nb[0];
@@ -390,7 +436,8 @@ void AdjointSingle::release(Owned& o) {
     o.plan = AdjPlanDev{};
 }
 
-int AdjointSingle::build(const hm_vjp_patch_view* views, int np, Owned& out, const int64_t* wrec_off) {
+int AdjointSingle::build(const hm_vjp_patch_view* views, int np, Owned& out, const int64_t* wrec_off,
+                         const AdjLamSource* lam) {
     const hm_config& c = rec_->config();
     if (c.model != HM_MODEL_SINGLE_SPECIES)
         return rec_->fail(HM_ERR_UNSUPPORTED, "the adjoint is available for the single-species model");
@@ -441,6 +488,7 @@ int AdjointSingle::build(const hm_vjp_patch_view* views, int np, Owned& out, con
         AdjPatch& A = face.items[i];
         std::memset(&A, 0, sizeof A);
         A.f = rec_->devPatch(fv[i], pb, wrec_off ? wrec_off + size_t(i) * 3 : nullptr);
+        if (lam) A.lam = lam[i];
         pb += rec_->patchScratchDoubles(fv[i]);
         for (int a = 0; a < 3; a++) {
             A.Fb[a] = (a < d) ? views[i].F_bar[a] : nullptr;
@@ -492,11 +540,13 @@ int AdjointSingle::build(const hm_vjp_patch_view* views, int np, Owned& out, con
     return HM_OK;
 }
 
-int AdjointSingle::run(const Owned& o, double dt, hipStream_t s, const double* wst) {
+int AdjointSingle::run(const Owned& o, double dt, hipStream_t s, const double* wst, int lam_src, double lam_scale) {
     if (int st = rec_->runOwnedPlanStages(o.fwd, dt, 0, 2, s)) return st;  // derived data and sensor
     KParams k = rec_->kernelParams();
     k.dt = dt;
     k.wrec = const_cast<double*>(wst);  // read only here
+    k.lam_src = lam_src;
+    k.lam_scale = lam_scale;
     const hm_config& c = rec_->config();
     const hipError_t e = launch_adjoint_single(c.dim, c.scheme, o.plan, k, s);
     if (e != hipSuccess) return rec_->fail(HM_ERR_DEVICE, std::string("adjoint launch failed: ") + hipGetErrorString(e));

@@ -26,9 +26,18 @@
 
 namespace hm {
 
+// Stage adjoints of a level patch: stage m at L0 + m * stride, num_eqn
+// components over the box grown by g (interior cells used).
+struct AdjLamSource {
+    const double* L0;
+    int64_t stride;
+    int g;
+};
+
 struct AdjPatch {
     DevPatch f;                // forward view: n, gq, dx, Q, prim, sens
     const double* Fb[3];       // face-flux adjoints per axis (layout of F)
+    AdjLamSource lam;          // level runs (KParams::lam_src >= 0): the stage adjoints instead of Fb
     double* G[3];              // per midpoint face: 6*NEQ + 2 stencil adjoints
     double* H[3];              // per midpoint face: hand-over between the two face-adjoint kernels
     double* Qb[HM_MAX_EQN];    // output over the Q ghost box
@@ -56,11 +65,16 @@ class AdjointSingle {
 
     // wrec_off (optional, 3 per patch): the patches' chunks in a face-state
     // slot, for runs given one (run's wst).
-    int build(const hm_vjp_patch_view* views, int np, Owned& out, const int64_t* wrec_off = nullptr);
+    // lam (optional, 1 per patch): the level's stage adjoints, for runs given lam_src.
+    int build(const hm_vjp_patch_view* views, int np, Owned& out, const int64_t* wrec_off = nullptr,
+              const AdjLamSource* lam = nullptr);
     void release(Owned& o);
     // wst: the face-state slot a recording forward pass wrote for these
-    // inputs (the flux half then reads W-, W+ instead of reconstructing), or NULL.
-    int run(const Owned& o, double dt, hipStream_t s, const double* wst = nullptr);
+    // inputs (the flux half then reads W-, W+ instead of reconstructing), or
+    // NULL.  lam_src >= 0: the face-flux adjoints are lam_scale times the
+    // face differences of stage adjoint lam_src over dx (else read from F_bar).
+    int run(const Owned& o, double dt, hipStream_t s, const double* wst = nullptr, int lam_src = -1,
+            double lam_scale = 0.0);
 
     // One-shot VJP of a patch list (the C-ABI path); keeps the last list's plan.
     int vjp(const hm_vjp_patch_view* views, int np, double dt, hipStream_t s);

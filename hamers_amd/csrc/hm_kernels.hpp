@@ -23,6 +23,12 @@ struct KParams {
     // the adjoint's flux half reads it instead of reconstructing again.
     // NULL: not used.
     double* wrec;
+    // Adjoint runs of a level (single-species model): with lam_src >= 0 the
+    // face-flux adjoints are not read from AdjPatch::Fb but formed from the
+    // level's stage adjoint lam_src, F_bar_f = lam_scale (lam_f - lam_{f-1}) / dx
+    // (the transposed stage update; AdjPatch::L).
+    int lam_src;
+    double lam_scale;
 };
 
 // Work of one RHS evaluation: one work source per pipeline stage

@@ -100,7 +100,6 @@ struct AdjLevelPatch {
     double dx[3];
     double* lam[kMaxRK + 1];    // adjoints of the stage states (interior cells used, ghost-box layout)
     double* qbar;               // VJP output over the ghost box (neq comps)
-    double* fbar[3];            // face-flux adjoints per axis (neq comps, face layout)
     int rev_first, rev_count;   // halo items whose source is this patch, in the reverse list
 };
 
@@ -109,39 +108,6 @@ __device__ __forceinline__ int64_t gbox_idx(const int* n, int g, int i, int j, i
     const int e0 = n[0] + 2 * g;
     const int e1 = (D > 1) ? n[1] + 2 * g : 1;
     return (i + g) + int64_t(j + ((D > 1) ? g : 0)) * e0 + int64_t(k + ((D > 2) ? g : 0)) * e0 * e1;
-}
-
-// Face-flux adjoints from the adjoint of a stage output: the stage update
-// Q_i += beta (-(F_{i+1/2} - F_{i-1/2}) / dx + ...) gives F_bar_f = beta (lam_f - lam_{f-1}) / dx.
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_adj_div(SegTableT<AdjLevelPatch> w, int t, double beta, int neq, int g) {
-    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    const AdjLevelPatch* P;
-    int ax;
-    int64_t loc;
-    if (!w.find(gid, P, ax, loc)) return;
-    const int f0 = P->n[0] + (ax == 0);
-    const int f1 = (D > 1) ? P->n[1] + (ax == 1) : 1;
-    int c[3];
-    c[0] = int(loc % f0);
-    const int64_t r = loc / f0;
-    c[1] = (D > 1) ? int(r % f1) : 0;
-    c[2] = (D > 2) ? int(r / f1) : 0;
-    const int f = c[ax];
-    const bool hasR = f < P->n[ax], hasL = f - 1 >= 0;
-    int cl[3] = {c[0], c[1], c[2]};
-    cl[ax] -= 1;
-    const int64_t iR = gbox_idx<D>(P->n, g, c[0], c[1], c[2]);
-    const int64_t iL = gbox_idx<D>(P->n, g, cl[0], cl[1], cl[2]);
-    const double* lam = P->lam[t];
-    int64_t fvol = 1;
-    for (int a = 0; a < D; a++) fvol *= P->n[a] + (a == ax);
-    const double sc = beta / P->dx[ax];
-    for (int e = 0; e < neq; e++) {
-        const double lR = hasR ? lam[e * P->gvol + iR] : 0.0;
-        const double lL = hasL ? lam[e * P->gvol + iL] : 0.0;
-        P->fbar[ax][e * fvol + loc] = sc * (lR - lL);
-    }
 }
 
 // The VJP output of this patch at interior cell (i, j, k) plus the ghost-cell
@@ -153,6 +119,13 @@ __device__ __forceinline__ void gather_qbar(const SegTableT<AdjLevelPatch>& w, c
                                             double (&acc)[NEQ]) {
 #pragma unroll
     for (int e = 0; e < NEQ; e++) acc[e] = P->qbar[e * P->gvol + gi];
+    // A cell at least g cells from every face of its patch is no ghost
+    // image: a ghost cell lies within g cells of its own patch, which is
+    // disjoint from this one.
+    bool deep = i >= g && i < P->n[0] - g;
+    if constexpr (D > 1) deep = deep && j >= g && j < P->n[1] - g;
+    if constexpr (D > 2) deep = deep && k >= g && k < P->n[2] - g;
+    if (deep) return;
     const int x[3] = {i + g, (D > 1) ? j + g : 0, (D > 2) ? k + g : 0};  // ghost-box coordinates
     for (int q = P->rev_first; q < P->rev_first + P->rev_count; q++) {
         const HaloItem& it = rev[q];
@@ -857,13 +830,10 @@ class Level {
             for (int m = 0; m <= sn; m++) {
                 if (beta_[sn][m] == 0.0) continue;
                 if (int st = store_op(1 + m, slot0 + m, 1, s)) return st;
-                switch (d_) {
-                    case 1: hipLaunchKernelGGL(k_adj_div<1>, dim3(blocks_for(adj_face_items_)), dim3(kBlock), 0, s, adj_faces_, sn + 1, beta_[sn][m], neq_, g_); break;
-                    case 2: hipLaunchKernelGGL(k_adj_div<2>, dim3(blocks_for(adj_face_items_)), dim3(kBlock), 0, s, adj_faces_, sn + 1, beta_[sn][m], neq_, g_); break;
-                    default: hipLaunchKernelGGL(k_adj_div<3>, dim3(blocks_for(adj_face_items_)), dim3(kBlock), 0, s, adj_faces_, sn + 1, beta_[sn][m], neq_, g_); break;
-                }
-                if (hipGetLastError() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "adjoint launch failed");
-                if (int st = adj_->run(vjp_plan_[m], rec_dt_, s, face_states(face_slot0 + m))) return st;
+                // The VJP forms the face-flux adjoints beta (lam_f - lam_{f-1}) / dx
+                // of stage adjoint sn + 1 itself (the transposed stage update).
+                if (int st = adj_->run(vjp_plan_[m], rec_dt_, s, face_states(face_slot0 + m), sn + 1, beta_[sn][m]))
+                    return st;
                 if (m == last_vjp) break;
                 switch (d_) {
                     case 1: hipLaunchKernelGGL(k_halo_reverse<1>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, rev_dev_, m, g_); break;
@@ -961,18 +931,14 @@ class Level {
         if (rec_->config().model != HM_MODEL_SINGLE_SPECIES)
             return rec_->fail(HM_ERR_UNSUPPORTED, "the adjoint is available for the single-species model");
         if (!adj_mem_) {
-            // lam (kMaxRK+1 per patch) and qbar over ghost boxes, fbar over faces
+            // lam (kMaxRK+1 per patch) and qbar over ghost boxes
             size_t tot = 0;
-            std::vector<size_t> off_lam(np_), off_q(np_), off_f(size_t(np_) * 3);
+            std::vector<size_t> off_lam(np_), off_q(np_);
             for (int p = 0; p < np_; p++) {
                 off_lam[p] = tot;
                 tot += up256(sizeof(double) * size_t(neq_) * gvol(p)) / sizeof(double) * (kMaxRK + 1);
                 off_q[p] = tot;
                 tot += up256(sizeof(double) * size_t(neq_) * gvol(p)) / sizeof(double);
-                for (int a = 0; a < d_; a++) {
-                    off_f[size_t(p) * 3 + a] = tot;
-                    tot += up256(sizeof(double) * size_t(neq_) * fvol(p, a)) / sizeof(double);
-                }
             }
             // per-patch offsets within a storage slot
             std::vector<int64_t> soff(np_);
@@ -982,17 +948,15 @@ class Level {
                 sd += up256(sizeof(double) * size_t(nq_) * gvol(p)) / sizeof(double);
             }
             slot_doubles_ = sd;
-            // tables: AdjLevelPatch faces and cells, reverse halo list, slot offsets
+            // tables: AdjLevelPatch cells, reverse halo list, slot offsets
             std::vector<std::vector<int>> by_src(np_);
             for (size_t q = 0; q < halo_host_.size(); q++) by_src[halo_host_[q].src].push_back(int(q));
             std::vector<HaloItem> rev;
-            TableImage<AdjLevelPatch> faces, cells;
-            faces.items.resize(np_);
+            TableImage<AdjLevelPatch> cells;
+            std::vector<AdjLevelPatch> items(np_);
             lam_base_.assign(size_t(np_) * (kMaxRK + 1), nullptr);
-            const size_t tables_bytes_guess = 0;
-            (void)tables_bytes_guess;
             for (int p = 0; p < np_; p++) {
-                AdjLevelPatch& A = faces.items[p];
+                AdjLevelPatch& A = items[p];
                 std::memset(&A, 0, sizeof A);
                 for (int a = 0; a < 3; a++) {
                     A.n[a] = a < d_ ? n_[3 * p + a] : 1;
@@ -1002,11 +966,10 @@ class Level {
                 A.rev_first = int(rev.size());
                 for (int q : by_src[p]) rev.push_back(halo_host_[q]);
                 A.rev_count = int(rev.size()) - A.rev_first;
-                for (int a = 0; a < d_; a++) faces.add(p, a, fvol(p, a));
                 cells.add(p, 0, ivol(p));
             }
             cells.items.resize(np_);
-            const size_t tb = faces.bytes() + cells.bytes() + up256(sizeof(HaloItem) * (rev.size() + 1)) +
+            const size_t tb = cells.bytes() + up256(sizeof(HaloItem) * (rev.size() + 1)) +
                               up256(sizeof(int64_t) * np_);
             if (hipMalloc(&adj_mem_, tot * sizeof(double) + tb) != hipSuccess) {
                 adj_mem_ = nullptr;
@@ -1016,22 +979,18 @@ class Level {
             if (hipMemset(adj_mem_, 0, tot * sizeof(double)) != hipSuccess)
                 return rec_->fail(HM_ERR_DEVICE, "adjoint buffer clear");
             for (int p = 0; p < np_; p++) {
-                AdjLevelPatch& A = faces.items[p];
+                AdjLevelPatch& A = items[p];
                 const size_t lstride = up256(sizeof(double) * size_t(neq_) * gvol(p)) / sizeof(double);
                 for (int m = 0; m <= kMaxRK; m++) {
                     A.lam[m] = base + off_lam[p] + size_t(m) * lstride;
                     lam_base_[size_t(p) * (kMaxRK + 1) + m] = A.lam[m];
                 }
                 A.qbar = base + off_q[p];
-                for (int a = 0; a < d_; a++) A.fbar[a] = base + off_f[size_t(p) * 3 + a];
                 cells.items[p] = A;
             }
             char* tdev = reinterpret_cast<char*>(base + tot);
             std::vector<char> h(tb, 0);
             size_t o = 0;
-            adj_faces_ = faces.write(h.data() + o, tdev + o);
-            adj_face_items_ = faces.total;
-            o += faces.bytes();
             adj_cells_ = cells.write(h.data() + o, tdev + o);
             adj_cell_items_ = cells.total;
             o += cells.bytes();
@@ -1043,6 +1002,11 @@ class Level {
             if (hipMemcpy(tdev, h.data(), tb, hipMemcpyHostToDevice) != hipSuccess)
                 return rec_->fail(HM_ERR_DEVICE, "adjoint table upload");
             // VJP plans of the stage inputs (state buffers 1..kMaxRK)
+            std::vector<AdjLamSource> lam_src(np_);
+            for (int p = 0; p < np_; p++) {
+                const AdjLevelPatch& A = items[p];
+                lam_src[p] = AdjLamSource{A.lam[0], A.lam[1] - A.lam[0], g_};
+            }
             adj_.reset(new AdjointSingle(rec_));
             for (int m = 0; m < nst_; m++) {
                 std::vector<hm_vjp_patch_view> vv(np_);
@@ -1055,12 +1019,14 @@ class Level {
                     }
                     for (int e = 0; e < neq_; e++) {
                         v.Q[e] = patch_[p].Q[1 + m] + size_t(e) * gvol(p);
-                        v.Q_bar[e] = faces.items[p].qbar + size_t(e) * gvol(p);
+                        v.Q_bar[e] = items[p].qbar + size_t(e) * gvol(p);
                     }
-                    for (int a = 0; a < d_; a++) v.F_bar[a] = faces.items[p].fbar[a];
+                    // Not read: level runs form the face-flux adjoints from the
+                    // stage adjoints (lam_src); the view only needs an address.
+                    for (int a = 0; a < d_; a++) v.F_bar[a] = items[p].qbar;
                     vv[p] = v;
                 }
-                if (int st = adj_->build(vv.data(), np_, vjp_plan_[m], wst_off_.data())) return st;
+                if (int st = adj_->build(vv.data(), np_, vjp_plan_[m], wst_off_.data(), lam_src.data())) return st;
             }
             adj_nst_ = nst_;
         }
@@ -1169,8 +1135,8 @@ class Level {
     std::vector<HaloItem> halo_host_;
     void* adj_mem_ = nullptr;
     std::vector<double*> lam_base_;
-    SegTableT<AdjLevelPatch> adj_faces_{}, adj_cells_{};
-    int64_t adj_face_items_ = 0, adj_cell_items_ = 0;
+    SegTableT<AdjLevelPatch> adj_cells_{};
+    int64_t adj_cell_items_ = 0;
     const HaloItem* rev_dev_ = nullptr;
     StageAdj stage_adj_[kMaxRK] = {};
     bool zero_first_ = false, out_in_place_ = false;

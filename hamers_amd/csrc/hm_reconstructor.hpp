@@ -71,6 +71,8 @@ class ConvectiveFluxReconstructorGPU {
         k_.num_species = cfg.num_species;
         k_.dt = 0.0;
         k_.wrec = nullptr;
+        k_.lam_src = -1;
+        k_.lam_scale = 0.0;
         has_advective_ = (cfg.model == HM_MODEL_FIVE_EQN_ALLAIRE && cfg.num_species > 1);
         plans_.reserve(kMaxPlans);  // plan addresses stay valid while cached
     }
