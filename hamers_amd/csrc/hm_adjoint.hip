@@ -29,10 +29,11 @@ namespace hm {
 template <int D, int A>
 __device__ __forceinline__ void mid_cell(const Boxes<D>& B, int64_t loc, int& c0, int& c1, int& c2) {
     const int m0 = B.mid_ext(A, 0), m1 = B.mid_ext(A, 1);
-    c0 = int(loc % m0);
-    const int64_t r = loc / m0;
-    c1 = (D > 1) ? int(r % m1) : 0;
-    c2 = (D > 2) ? int(r / m1) : 0;
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(m0);
+    c0 = int(uint32_t(loc) - r * uint32_t(m0));
+    c1 = (D > 1) ? int((D > 2) ? r % uint32_t(m1) : r) : 0;
+    c2 = (D > 2) ? int(r / uint32_t(m1)) : 0;
     if constexpr (A == 0) c0 -= 1;
     if constexpr (A == 1) c1 -= 1;
     if constexpr (A == 2) c2 -= 1;
@@ -258,10 +259,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(W w, KParams k) {
     const Boxes<D> B(F);
     const int e0 = B.ext(0, kGc), e1 = B.ext(1, kGc);
     int c[3];
-    c[0] = int(loc % e0) - kGc;
-    const int64_t r = loc / e0;
-    c[1] = (D > 1) ? int(r % e1) - kGc : 0;
-    c[2] = (D > 2) ? int(r / e1) - kGc : 0;
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(e0);
+    c[0] = int(uint32_t(loc) - r * uint32_t(e0)) - kGc;
+    c[1] = (D > 1) ? int((D > 2) ? r % uint32_t(e1) : r) - kGc : 0;
+    c[2] = (D > 2) ? int(r / uint32_t(e1)) - kGc : 0;
     const double gam = k.gamma[0];
     const double dt = k.dt;
     const int64_t v4 = B.vol(kGc);

@@ -100,6 +100,13 @@ struct SegSingleT {  // one patch, up to 4 segments (one per axis, one extra), a
     }
 };
 
+// The same 64-bit value in every lane of the wavefront (lane 0's).
+__device__ __forceinline__ int64_t wave_uniform(int64_t v) {
+    const int lo = __builtin_amdgcn_readfirstlane(int(uint64_t(v) & 0xffffffffu));
+    const int hi = __builtin_amdgcn_readfirstlane(int(uint64_t(v) >> 32));
+    return int64_t((uint64_t(uint32_t(hi)) << 32) | uint32_t(lo));
+}
+
 template <class T>
 struct SegTableT {  // many patches: device arrays
     const T* patches;
@@ -107,12 +114,18 @@ struct SegTableT {  // many patches: device arrays
     const int64_t* cnt;    // nseg entries
     const int* seg;        // per segment: patch * 4 + axis
     int nseg;
+    // gid = blockIdx.x * (a multiple of 64) + threadIdx.x: a wavefront's items
+    // are 64 consecutive numbers from a multiple of 64, which lie in one
+    // segment.  So the segment is looked up once per wavefront, on the
+    // wavefront's first item: the search, the patch pointer and the patch's
+    // fields are then wave-uniform (scalar loads and registers).
     __device__ __forceinline__ bool find(int64_t gid, const T*& P, int& seg_axis, int64_t& loc) const {
-        if (gid >= start[nseg]) return false;
-        int lo = 0, hi = nseg - 1;  // last segment with start <= gid
+        const int64_t wb = wave_uniform(gid & ~int64_t(kSegAlign - 1));
+        if (wb >= start[nseg]) return false;
+        int lo = 0, hi = nseg - 1;  // last segment with start <= wb
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (start[mid] <= gid)
+            if (start[mid] <= wb)
                 lo = mid;
             else
                 hi = mid - 1;

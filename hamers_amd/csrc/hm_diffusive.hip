@@ -57,10 +57,11 @@ __global__ __launch_bounds__(kBlock) void k_diff_cells(W w, DParams k) {
     if (!w.find(gid, P, ax, loc)) return;
     using S = DiffSlots<D, NS>;
     const int e0 = dext<D>(*P, 0), e1 = dext<D>(*P, 1);
-    const int i = int(loc % e0) - kGd;
-    const int64_t r = loc / e0;
-    const int j = (D > 1) ? int(r % e1) - kGd : 0;
-    const int kk = (D > 2) ? int(r / e1) - kGd : 0;
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(e0);
+    const int i = int(uint32_t(loc) - r * uint32_t(e0)) - kGd;
+    const int j = (D > 1) ? int((D > 2) ? r % uint32_t(e1) : r) - kGd : 0;
+    const int kk = (D > 2) ? int(r / uint32_t(e1)) - kGd : 0;
     const int64_t iq = dq_idx<D>(*P, i, j, kk);
     double rho = double(0);
     double Y[NS];
@@ -145,10 +146,11 @@ __global__ __launch_bounds__(kBlock) void k_diff_nodes(W w, DParams k) {
 #pragma unroll
     for (int b = 0; b < 3; b++) ext[b] = (b < D) ? P->n[b] + ((b == A) ? 6 : 0) : 1;
     int idx[3];
-    idx[0] = int(loc % ext[0]);
-    const int64_t r = loc / ext[0];
-    idx[1] = int(r % ext[1]);
-    idx[2] = int(r / ext[1]);
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(ext[0]);
+    idx[0] = int(uint32_t(loc) - r * uint32_t(ext[0]));
+    idx[1] = int(r % uint32_t(ext[1]));
+    idx[2] = int(r / uint32_t(ext[1]));
     idx[A] -= 3;
     const int64_t c = dcell<D>(*P, idx[0], idx[1], idx[2]);
     const int64_t vol = dvol<D>(*P);
@@ -269,10 +271,11 @@ __global__ __launch_bounds__(kBlock) void k_diff_faces(W w, DParams k) {
         next[b] = (b < D) ? P->n[b] + ((b == A) ? 6 : 0) : 1;
     }
     int idx[3];
-    idx[0] = int(loc % fext[0]);
-    const int64_t r = loc / fext[0];
-    idx[1] = int(r % fext[1]);
-    idx[2] = int(r / fext[1]);
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(fext[0]);
+    idx[0] = int(uint32_t(loc) - r * uint32_t(fext[0]));
+    idx[1] = int(r % uint32_t(fext[1]));
+    idx[2] = int(r / uint32_t(fext[1]));
     // node of cell m along A sits at m + 3; the face f reads cells f-3 .. f+2
     idx[A] += 3;
     const int64_t nst = (A == 0) ? 1 : (A == 1) ? int64_t(next[0]) : int64_t(next[0]) * next[1];

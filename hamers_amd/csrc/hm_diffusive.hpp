@@ -238,6 +238,13 @@ class DiffusiveFluxReconstructorGPU {
                 return rec_.fail(HM_ERR_INVALID_ARGUMENT, buf);
             }
         }
+        // Kernels number the items of a patch's work segment in 32 bits.
+        double box = 1.0;
+        for (int a = 0; a < d; a++) box *= double(v.n[a]) + 2.0 * double(v.ghost[a]);
+        if (box >= 2147483648.0) {
+            snprintf(buf, sizeof buf, "patch %d: ghost box of 2^31 cells or more; split the patch", which);
+            return rec_.fail(HM_ERR_INVALID_ARGUMENT, buf);
+        }
         for (int e = 0; e < neq_; e++)
             if (!v.Q[e]) {
                 snprintf(buf, sizeof buf, "patch %d: conservative variable %d is NULL", which, e);

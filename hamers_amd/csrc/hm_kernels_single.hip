@@ -34,10 +34,11 @@ __global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
     constexpr int NEQ = D + 2;
     const Boxes<D> B(*P);
     const int e0 = B.ext(0, kGc), e1 = B.ext(1, kGc);
-    const int i = int(loc % e0) - kGc;
-    const int64_t r = loc / e0;
-    const int j = (D > 1) ? int(r % e1) - kGc : 0;
-    const int kk = (D > 2) ? int(r / e1) - kGc : 0;
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(e0);
+    const int i = int(uint32_t(loc) - r * uint32_t(e0)) - kGc;
+    const int j = (D > 1) ? int((D > 2) ? r % uint32_t(e1) : r) - kGc : 0;
+    const int kk = (D > 2) ? int(r / uint32_t(e1)) - kGc : 0;
     const int64_t iq = q_idx<D>(*P, i, j, kk);
     const double gam = k.gamma[0];
 
@@ -74,10 +75,11 @@ __device__ __forceinline__ void midflux_single_face(const DevPatch& P, int64_t l
     constexpr int NEQ = D + 2;
     const Boxes<D> B(P);
     const int m0 = B.mid_ext(A, 0), m1 = B.mid_ext(A, 1);
-    int c0 = int(loc % m0);
-    const int64_t r = loc / m0;
-    int c1 = (D > 1) ? int(r % m1) : 0;
-    int c2 = (D > 2) ? int(r / m1) : 0;
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(m0);
+    int c0 = int(uint32_t(loc) - r * uint32_t(m0));
+    int c1 = (D > 1) ? int((D > 2) ? r % uint32_t(m1) : r) : 0;
+    int c2 = (D > 2) ? int(r / uint32_t(m1)) : 0;
     if constexpr (A == 0) c0 -= 1;
     if constexpr (A == 1) c1 -= 1;
     if constexpr (A == 2) c2 -= 1;
@@ -134,7 +136,8 @@ __device__ __forceinline__ void midflux_single_face(const DevPatch& P, int64_t l
 }
 
 template <int D, int SCHEME, int PP, int QQ, class W, bool REC = false>
-__global__ __launch_bounds__(kBlock) void k_midflux_single(W w, KParams k) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SCHEME == kWCNS5_JS && D == 2 ? 4 : 1))) void k_midflux_single(
+    W w, KParams k) {
     const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     const DevPatch* P;
     int ax;
@@ -160,10 +163,11 @@ __device__ __forceinline__ void faceflux_single_face(const DevPatch& P, int64_t 
     constexpr int IP = D + 1;
     const Boxes<D> B(P);
     const int f0 = B.face_ext(A, 0), f1 = B.face_ext(A, 1);
-    const int c0 = int(loc % f0);
-    const int64_t r = loc / f0;
-    const int c1 = (D > 1) ? int(r % f1) : 0;
-    const int c2 = (D > 2) ? int(r / f1) : 0;
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(f0);
+    const int c0 = int(uint32_t(loc) - r * uint32_t(f0));
+    const int c1 = (D > 1) ? int((D > 2) ? r % uint32_t(f1) : r) : 0;
+    const int c2 = (D > 2) ? int(r / uint32_t(f1)) : 0;
 
     const int64_t mv = B.mid_vol(A);
     const int64_t mC = B.mid_idx(A, c0, c1, c2);

@@ -51,10 +51,11 @@ __global__ __launch_bounds__(kBlock) void k_halo(SegTableT<HaloItem> w, const Le
     int ax;
     int64_t loc;
     if (!w.find(gid, it, ax, loc)) return;
-    const int i = int(loc % it->ext[0]);
-    const int64_t r = loc / it->ext[0];
-    const int j = (D > 1) ? int(r % it->ext[1]) : 0;
-    const int k = (D > 2) ? int(r / it->ext[1]) : 0;
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(it->ext[0]);
+    const int i = int(uint32_t(loc) - r * uint32_t(it->ext[0]));
+    const int j = (D > 1) ? int((D > 2) ? r % uint32_t(it->ext[1]) : r) : 0;
+    const int k = (D > 2) ? int(r / uint32_t(it->ext[1])) : 0;
     const LevelPatch& Pd = patches[it->dst];
     const LevelPatch& Ps = patches[it->src];
     auto gidx = [&](const LevelPatch& P, int a0, int a1, int a2) -> int64_t {
@@ -155,10 +156,11 @@ __global__ __launch_bounds__(kBlock) void k_halo_reverse(SegTableT<AdjLevelPatch
     int ax;
     int64_t loc;
     if (!w.find(gid, P, ax, loc)) return;
-    const int i = int(loc % P->n[0]);
-    const int64_t r = loc / P->n[0];
-    const int j = (D > 1) ? int(r % P->n[1]) : 0;
-    const int k = (D > 2) ? int(r / P->n[1]) : 0;
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(P->n[0]);
+    const int i = int(uint32_t(loc) - r * uint32_t(P->n[0]));
+    const int j = (D > 1) ? int((D > 2) ? r % uint32_t(P->n[1]) : r) : 0;
+    const int k = (D > 2) ? int(r / uint32_t(P->n[1])) : 0;
     const int64_t gi = gbox_idx<D>(P->n, g, i, j, k);
     constexpr int NEQ = D + 2;  // the level adjoint is built for the single-species model
     double acc[NEQ];
@@ -193,10 +195,11 @@ __global__ __launch_bounds__(kBlock) void k_stage_adjoint(SegTableT<AdjLevelPatc
     int ax;
     int64_t loc;
     if (!w.find(gid, P, ax, loc)) return;
-    const int i = int(loc % P->n[0]);
-    const int64_t r = loc / P->n[0];
-    const int j = (D > 1) ? int(r % P->n[1]) : 0;
-    const int k = (D > 2) ? int(r / P->n[1]) : 0;
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(P->n[0]);
+    const int i = int(uint32_t(loc) - r * uint32_t(P->n[0]));
+    const int j = (D > 1) ? int((D > 2) ? r % uint32_t(P->n[1]) : r) : 0;
+    const int k = (D > 2) ? int(r / uint32_t(P->n[1])) : 0;
     const int64_t gi = gbox_idx<D>(P->n, g, i, j, k);
     constexpr int NEQ = D + 2;
     double acc[NEQ];
@@ -230,10 +233,11 @@ __global__ __launch_bounds__(kBlock) void k_lam_op(SegTableT<AdjLevelPatch> w, i
     int ax;
     int64_t loc;
     if (!w.find(gid, P, ax, loc)) return;
-    const int i = int(loc % P->n[0]);
-    const int64_t r = loc / P->n[0];
-    const int j = (D > 1) ? int(r % P->n[1]) : 0;
-    const int k = (D > 2) ? int(r / P->n[1]) : 0;
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(P->n[0]);
+    const int i = int(uint32_t(loc) - r * uint32_t(P->n[0]));
+    const int j = (D > 1) ? int((D > 2) ? r % uint32_t(P->n[1]) : r) : 0;
+    const int k = (D > 2) ? int(r / uint32_t(P->n[1])) : 0;
     const int64_t gi = gbox_idx<D>(P->n, g, i, j, k);
     double* L = P->lam[dst];
     const double* S = P->lam[src];
@@ -1199,6 +1203,13 @@ int advance_single_step(ConvectiveFluxReconstructorGPU* rec, const hm_rk_patch_v
                 return rec->fail(HM_ERR_INVALID_ARGUMENT, msg);
             }
             ivol *= R.n[a];
+        }
+        // Kernels number the items of a patch's work segment in 32 bits.
+        double items = double(ivol);
+        for (int a = 0; a < d; a++) items = std::max(items, double(ivol) / R.n[a] * (R.n[a] + 1));
+        if (items >= 2147483648.0) {
+            std::snprintf(msg, sizeof msg, "patch %d: 2^31 cells or faces or more; split the patch", p);
+            return rec->fail(HM_ERR_INVALID_ARGUMENT, msg);
         }
         for (int e = 0; e < nq; e++) {
             if (!v.Q[e]) {

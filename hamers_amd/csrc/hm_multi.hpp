@@ -124,10 +124,11 @@ __global__ __launch_bounds__(kBlock) void k_derived_multi(W w, KParams k) {
     constexpr int NEQ = M::NEQ;
     const Boxes<D> B(*P);
     const int e0 = B.ext(0, kGc), e1 = B.ext(1, kGc);
-    const int i = int(loc % e0) - kGc;
-    const int64_t r = loc / e0;
-    const int j = (D > 1) ? int(r % e1) - kGc : 0;
-    const int kk = (D > 2) ? int(r / e1) - kGc : 0;
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(e0);
+    const int i = int(uint32_t(loc) - r * uint32_t(e0)) - kGc;
+    const int j = (D > 1) ? int((D > 2) ? r % uint32_t(e1) : r) - kGc : 0;
+    const int kk = (D > 2) ? int(r / uint32_t(e1)) - kGc : 0;
     const int64_t iq = q_idx<D>(*P, i, j, kk);
 
     double Zr[NS], Z[NS];
@@ -248,10 +249,11 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
     constexpr int IZ = NS + D + 1;
     const Boxes<D> B(P);
     const int m0 = B.mid_ext(A, 0), m1 = B.mid_ext(A, 1);
-    int c0 = int(loc % m0);
-    const int64_t r = loc / m0;
-    int c1 = (D > 1) ? int(r % m1) : 0;
-    int c2 = (D > 2) ? int(r / m1) : 0;
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(m0);
+    int c0 = int(uint32_t(loc) - r * uint32_t(m0));
+    int c1 = (D > 1) ? int((D > 2) ? r % uint32_t(m1) : r) : 0;
+    int c2 = (D > 2) ? int(r / uint32_t(m1)) : 0;
     if constexpr (A == 0) c0 -= 1;
     if constexpr (A == 1) c1 -= 1;
     if constexpr (A == 2) c2 -= 1;
@@ -528,10 +530,11 @@ __device__ __forceinline__ void faceflux_multi_face(const DevPatch& P, int64_t l
     constexpr int IP = NS + D;
     const Boxes<D> B(P);
     const int f0 = B.face_ext(A, 0), f1 = B.face_ext(A, 1);
-    const int c0 = int(loc % f0);
-    const int64_t r = loc / f0;
-    const int c1 = (D > 1) ? int(r % f1) : 0;
-    const int c2 = (D > 2) ? int(r / f1) : 0;
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(f0);
+    const int c0 = int(uint32_t(loc) - r * uint32_t(f0));
+    const int c1 = (D > 1) ? int((D > 2) ? r % uint32_t(f1) : r) : 0;
+    const int c2 = (D > 2) ? int(r / uint32_t(f1)) : 0;
     const int64_t mv = B.mid_vol(A);
     const int64_t mC = B.mid_idx(A, c0, c1, c2);
     const int64_t mst = (A == 0) ? 1 : (A == 1) ? int64_t(B.mid_ext(A, 0)) : int64_t(B.mid_ext(A, 0)) * B.mid_ext(A, 1);
@@ -597,10 +600,11 @@ __global__ __launch_bounds__(kBlock) void k_source_five(W w, KParams k) {
     int64_t loc;
     if (!w.find(gid, P, ax, loc)) return;
     const Boxes<D> B(*P);
-    const int i = int(loc % B.n[0]);
-    const int64_t r = loc / B.n[0];
-    const int j = (D > 1) ? int(r % B.n[1]) : 0;
-    const int kk = (D > 2) ? int(r / B.n[1]) : 0;
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(B.n[0]);
+    const int i = int(uint32_t(loc) - r * uint32_t(B.n[0]));
+    const int j = (D > 1) ? int((D > 2) ? r % uint32_t(B.n[1]) : r) : 0;
+    const int kk = (D > 2) ? int(r / uint32_t(B.n[1])) : 0;
     const int64_t v4 = B.vol(kGc);
     const int64_t pc = B.idx(kGc, i, j, kk);
     double acc = 0.0;

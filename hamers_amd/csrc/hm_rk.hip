@@ -23,10 +23,11 @@ __global__ __launch_bounds__(kBlock) void k_rk_update(W w, RKParams r) {
     int64_t loc;
     if (!w.find(gid, P, ax, loc)) return;
     const int n0 = P->n[0], n1 = (D > 1) ? P->n[1] : 1;
-    const int i = int(loc % n0);
-    const int64_t rr = loc / n0;
-    const int j = (D > 1) ? int(rr % n1) : 0;
-    const int k = (D > 2) ? int(rr / n1) : 0;
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t rr = uint32_t(loc) / uint32_t(n0);
+    const int i = int(uint32_t(loc) - rr * uint32_t(n0));
+    const int j = (D > 1) ? int((D > 2) ? rr % uint32_t(n1) : rr) : 0;
+    const int k = (D > 2) ? int(rr / uint32_t(n1)) : 0;
     // Scratch (output) and intermediate (input) ghost boxes.
     auto box_idx = [&](const int* g) -> int64_t {
         const int e0 = n0 + 2 * g[0];

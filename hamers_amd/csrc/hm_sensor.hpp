@@ -20,10 +20,11 @@ __global__ __launch_bounds__(kBlock) void k_sensor(W w, KParams k) {
     if (!w.find(gid, P, ax, loc)) return;
     const Boxes<D> B(*P);
     const int e0 = B.ext(0, kGs), e1 = B.ext(1, kGs);
-    const int i = int(loc % e0) - kGs;
-    const int64_t r = loc / e0;
-    const int j = int(r % e1) - kGs;
-    const int kk = (D > 2) ? int(r / e1) - kGs : 0;
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(e0);
+    const int i = int(uint32_t(loc) - r * uint32_t(e0)) - kGs;
+    const int j = int((D > 2) ? r % uint32_t(e1) : r) - kGs;
+    const int kk = (D > 2) ? int(r / uint32_t(e1)) - kGs : 0;
     const int64_t v4 = B.vol(kGc);
     const int64_t c = B.idx(kGc, i, j, kk);
     const int64_t st[3] = {1, B.ext(0, kGc), int64_t(B.ext(0, kGc)) * B.ext(1, kGc)};
