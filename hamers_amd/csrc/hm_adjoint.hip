@@ -273,7 +273,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(W w, KParams k) {
     for (int b = 0; b < D; b++) u[b] = F.prim[(1 + b) * v4 + loc];
     const double p = F.prim[IP * v4 + loc];
     const double cs = F.prim[NEQ * v4 + loc];
-    const int64_t iq = q_idx<D>(F, c[0], c[1], c[2]);
+    const int64_t iq = q_idx<D>(F, c[0], c[1], c[2]);  // Q_bar index
+    const int64_t iqr = iq + k.qoff;                    // Q index (reads)
 
     double Vb[NEQ], Qbd[NEQ];
 #pragma unroll
@@ -340,12 +341,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(W w, KParams k) {
             Qbd[1 + a] += nb[0];
 #pragma unroll
             for (int b = 0; b < D; b++) {
-                const double Qv = F.Q[1 + b][iq];
+                const double Qv = F.Q[1 + b][iqr];
                 Qbd[1 + b] += u[a] * nb[1 + b];
                 Vb[1 + a] += Qv * nb[1 + b];
                 if (b == a) Vb[IP] += nb[1 + b];
             }
-            const double E = F.Q[IP][iq];
+            const double E = F.Q[IP][iqr];
             Vb[1 + a] += (E + p) * nb[IP];
             Qbd[IP] += u[a] * nb[IP];
             Vb[IP] += u[a] * nb[IP];
@@ -542,13 +543,15 @@ int AdjointSingle::build(const hm_vjp_patch_view* views, int np, Owned& out, con
     return HM_OK;
 }
 
-int AdjointSingle::run(const Owned& o, double dt, hipStream_t s, const double* wst, int lam_src, double lam_scale) {
-    if (int st = rec_->runOwnedPlanStages(o.fwd, dt, 0, 2, s)) return st;  // derived data and sensor
+int AdjointSingle::run(const Owned& o, double dt, hipStream_t s, const double* wst, int lam_src, double lam_scale,
+                       int64_t qoff) {
+    if (int st = rec_->runOwnedPlanStages(o.fwd, dt, 0, 2, s, nullptr, qoff)) return st;  // derived data and sensor
     KParams k = rec_->kernelParams();
     k.dt = dt;
     k.wrec = const_cast<double*>(wst);  // read only here
     k.lam_src = lam_src;
     k.lam_scale = lam_scale;
+    k.qoff = qoff;
     const hm_config& c = rec_->config();
     const hipError_t e = launch_adjoint_single(c.dim, c.scheme, o.plan, k, s);
     if (e != hipSuccess) return rec_->fail(HM_ERR_DEVICE, std::string("adjoint launch failed: ") + hipGetErrorString(e));

@@ -73,8 +73,9 @@ class AdjointSingle {
     // inputs (the flux half then reads W-, W+ instead of reconstructing), or
     // NULL.  lam_src >= 0: the face-flux adjoints are lam_scale times the
     // face differences of stage adjoint lam_src over dx (else read from F_bar).
+    // qoff: offset added to the indices into the views' Q arrays.
     int run(const Owned& o, double dt, hipStream_t s, const double* wst = nullptr, int lam_src = -1,
-            double lam_scale = 0.0);
+            double lam_scale = 0.0, int64_t qoff = 0);
 
     // One-shot VJP of a patch list (the C-ABI path); keeps the last list's plan.
     int vjp(const hm_vjp_patch_view* views, int np, double dt, hipStream_t s);

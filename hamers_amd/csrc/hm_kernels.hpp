@@ -29,6 +29,10 @@ struct KParams {
     // (the transposed stage update; AdjPatch::L).
     int lam_src;
     double lam_scale;
+    // Offset added to every index into DevPatch::Q by the derived-data stage
+    // and the adjoint's cell gather (a level adjoint reads the stage inputs
+    // from a slot of its trajectory store); 0 otherwise.
+    int64_t qoff;
 };
 
 // Work of one RHS evaluation: one work source per pipeline stage

@@ -39,7 +39,7 @@ __global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
     const int i = int(uint32_t(loc) - r * uint32_t(e0)) - kGc;
     const int j = (D > 1) ? int((D > 2) ? r % uint32_t(e1) : r) - kGc : 0;
     const int kk = (D > 2) ? int(r / uint32_t(e1)) - kGc : 0;
-    const int64_t iq = q_idx<D>(*P, i, j, kk);
+    const int64_t iq = q_idx<D>(*P, i, j, kk) + k.qoff;
     const double gam = k.gamma[0];
 
     const double rho = P->Q[0][iq];

@@ -291,7 +291,7 @@ class Level {
         for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
         for (auto& p : rhs_plan_) rec_->releaseOwnedPlan(p);
         if (adj_)
-            for (auto& p : vjp_plan_) adj_->release(p);
+            adj_->release(vjp_plan_);
         if (adj_mem_) (void)hipFree(adj_mem_);
         if (store_) (void)hipFree(store_);
         if (wst_) (void)hipFree(wst_);
@@ -833,10 +833,11 @@ class Level {
                 if (beta_[sn][m] != 0.0) last_vjp = m;
             for (int m = 0; m <= sn; m++) {
                 if (beta_[sn][m] == 0.0) continue;
-                if (int st = store_op(1 + m, slot0 + m, 1, s)) return st;
-                // The VJP forms the face-flux adjoints beta (lam_f - lam_{f-1}) / dx
-                // of stage adjoint sn + 1 itself (the transposed stage update).
-                if (int st = adj_->run(vjp_plan_[m], rec_dt_, s, face_states(face_slot0 + m), sn + 1, beta_[sn][m]))
+                // The VJP reads stage input m from its trajectory slot and forms
+                // the face-flux adjoints beta (lam_f - lam_{f-1}) / dx of stage
+                // adjoint sn + 1 itself (the transposed stage update).
+                if (int st = adj_->run(vjp_plan_, rec_dt_, s, face_states(face_slot0 + m), sn + 1, beta_[sn][m],
+                                       int64_t(slot0 + m) * int64_t(slot_doubles_)))
                     return st;
                 if (m == last_vjp) break;
                 switch (d_) {
@@ -1005,33 +1006,10 @@ class Level {
             slot_off_dev_ = reinterpret_cast<const int64_t*>(tdev + o);
             if (hipMemcpy(tdev, h.data(), tb, hipMemcpyHostToDevice) != hipSuccess)
                 return rec_->fail(HM_ERR_DEVICE, "adjoint table upload");
-            // VJP plans of the stage inputs (state buffers 1..kMaxRK)
-            std::vector<AdjLamSource> lam_src(np_);
-            for (int p = 0; p < np_; p++) {
-                const AdjLevelPatch& A = items[p];
-                lam_src[p] = AdjLamSource{A.lam[0], A.lam[1] - A.lam[0], g_};
-            }
+            // The VJP plan (built once the trajectory store exists) needs these.
+            adj_items_ = items;
+            slot_off_host_ = soff;
             adj_.reset(new AdjointSingle(rec_));
-            for (int m = 0; m < nst_; m++) {
-                std::vector<hm_vjp_patch_view> vv(np_);
-                for (int p = 0; p < np_; p++) {
-                    hm_vjp_patch_view v{};
-                    for (int a = 0; a < 3; a++) {
-                        v.n[a] = a < d_ ? n_[3 * p + a] : 1;
-                        v.ghost[a] = a < d_ ? g_ : 0;
-                        v.dx[a] = dx_[a];
-                    }
-                    for (int e = 0; e < neq_; e++) {
-                        v.Q[e] = patch_[p].Q[1 + m] + size_t(e) * gvol(p);
-                        v.Q_bar[e] = items[p].qbar + size_t(e) * gvol(p);
-                    }
-                    // Not read: level runs form the face-flux adjoints from the
-                    // stage adjoints (lam_src); the view only needs an address.
-                    for (int a = 0; a < d_; a++) v.F_bar[a] = items[p].qbar;
-                    vv[p] = v;
-                }
-                if (int st = adj_->build(vv.data(), np_, vjp_plan_[m], wst_off_.data(), lam_src.data())) return st;
-            }
             adj_nst_ = nst_;
         }
         if (adj_nst_ != nst_) return rec_->fail(HM_ERR_UNSUPPORTED, "the stage count changed after the adjoint was set up");
@@ -1053,6 +1031,33 @@ class Level {
                 store_slots_ = nslots;
             }
             nckpt_ = nckpt;
+            // The VJP plan reads each stage input from its trajectory slot
+            // (KParams::qoff selects the slot): its Q views are slot 0.
+            if (vjp_store_ != store_) {
+                std::vector<AdjLamSource> lam_src(np_);
+                std::vector<hm_vjp_patch_view> vv(np_);
+                for (int p = 0; p < np_; p++) {
+                    const AdjLevelPatch& A = adj_items_[p];
+                    lam_src[p] = AdjLamSource{A.lam[0], A.lam[1] - A.lam[0], g_};
+                    hm_vjp_patch_view v{};
+                    for (int a = 0; a < 3; a++) {
+                        v.n[a] = a < d_ ? n_[3 * p + a] : 1;
+                        v.ghost[a] = a < d_ ? g_ : 0;
+                        v.dx[a] = dx_[a];
+                    }
+                    for (int e = 0; e < neq_; e++) {
+                        v.Q[e] = store_ + slot_off_host_[p] + size_t(e) * gvol(p);
+                        v.Q_bar[e] = A.qbar + size_t(e) * gvol(p);
+                    }
+                    // Not read: level runs form the face-flux adjoints from the
+                    // stage adjoints (lam_src); the view only needs an address.
+                    for (int a = 0; a < d_; a++) v.F_bar[a] = A.qbar;
+                    vv[p] = v;
+                }
+                vjp_store_ = nullptr;
+                if (int st = adj_->build(vv.data(), np_, vjp_plan_, wst_off_.data(), lam_src.data())) return st;
+                vjp_store_ = store_;
+            }
             // Face states of one segment's stages, when they fit the budget;
             // else the adjoint's flux half reconstructs them again.
             const size_t nfs = size_t(every) * nst_;
@@ -1146,7 +1151,10 @@ class Level {
     bool zero_first_ = false, out_in_place_ = false;
     const int64_t* slot_off_dev_ = nullptr;
     std::unique_ptr<AdjointSingle> adj_;
-    AdjointSingle::Owned vjp_plan_[kMaxRK];
+    AdjointSingle::Owned vjp_plan_{};
+    const double* vjp_store_ = nullptr;  // the store the VJP plan's Q views point into
+    std::vector<AdjLevelPatch> adj_items_;
+    std::vector<int64_t> slot_off_host_;
     int adj_nst_ = 0;
     double* store_ = nullptr;
     size_t slot_doubles_ = 0;

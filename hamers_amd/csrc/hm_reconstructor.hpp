@@ -73,6 +73,7 @@ class ConvectiveFluxReconstructorGPU {
         k_.wrec = nullptr;
         k_.lam_src = -1;
         k_.lam_scale = 0.0;
+        k_.qoff = 0;
         has_advective_ = (cfg.model == HM_MODEL_FIVE_EQN_ALLAIRE && cfg.num_species > 1);
         plans_.reserve(kMaxPlans);  // plan addresses stay valid while cached
     }
@@ -212,10 +213,12 @@ class ConvectiveFluxReconstructorGPU {
     }
 
     // Pipeline stages [first, last) of an owned plan.
-    int runOwnedPlanStages(const OwnedPlan& p, double dt, int first, int last, hipStream_t s, double* wrec = nullptr) {
+    int runOwnedPlanStages(const OwnedPlan& p, double dt, int first, int last, hipStream_t s, double* wrec = nullptr,
+                           int64_t qoff = 0) {
         KParams k = k_;
         k.dt = dt;
         k.wrec = wrec;
+        k.qoff = qoff;
         return run_stages(p.plan, k, s, first, last);
     }
 
