@@ -142,7 +142,7 @@ __device__ __forceinline__ void adj_face_flux(const AdjPatch& P, int64_t loc, co
             const int64_t tL = tR - sts;
             const double th_avg = 0.5 * (F.sens[tL] + F.sens[tR]);
             const double om_avg = 0.5 * (F.sens[vs + tL] + F.sens[vs + tR]);
-            const double sensor = -th_avg / (fabs(th_avg) + om_avg + kEps);
+            const double sensor = div_rn(-th_avg, fabs(th_avg) + om_avg + kEps);
             return sensor > 0.65;
         }
         return false;

@@ -611,7 +611,7 @@ template <int D, int A, class VAt>
 HM_HD double char_field_at(const VAt& vat, double ra, double ca, int f, int m) {
     constexpr int IP = D + 1;
     if (f == 0) return -double(1) / double(2) * ra * ca * vat(m, 1 + A) + double(1) / double(2) * vat(m, IP);
-    if (f == 1) return vat(m, 0) - double(1) / (ca * ca) * vat(m, IP);
+    if (f == 1) return vat(m, 0) - div_rn(double(1), ca * ca) * vat(m, IP);
     if (f == IP) return double(1) / double(2) * ra * ca * vat(m, 1 + A) + double(1) / double(2) * vat(m, IP);
     int t = 2;
     for (int b = 0; b < D; b++) {
@@ -638,8 +638,9 @@ constexpr int face_adj_h_size() {
 template <int D, int A>
 HM_HD void back_project_single(const double* W, double ra, double ca, double* Vo) {
     constexpr int IP = D + 1;
-    Vo[0] = double(1) / (ca * ca) * W[0] + W[1] + double(1) / (ca * ca) * W[IP];
-    Vo[1 + A] = -double(1) / (ra * ca) * W[0] + double(1) / (ra * ca) * W[IP];
+    const double ic2 = div_rn(double(1), ca * ca), irc = div_rn(double(1), ra * ca);
+    Vo[0] = ic2 * W[0] + W[1] + ic2 * W[IP];
+    Vo[1 + A] = -irc * W[0] + irc * W[IP];
     int t = 2;
 #pragma unroll
     for (int b = 0; b < D; b++) {
@@ -692,8 +693,8 @@ HM_HD void face_adj_single_after_recon(const double* Wm, const double* Wp, const
     // Back-projection adjoint for both sides.
     double rab = 0.0, cab = 0.0;
     auto back_adj = [&](const double* W, const double* Vob, double* Wb) {
-        const double ic2 = double(1) / (ca * ca);
-        const double irc = double(1) / (ra * ca);
+        const double ic2 = div_rn(double(1), ca * ca);
+        const double irc = div_rn(double(1), ra * ca);
         Wb[0] = Vob[0] * ic2 - Vob[1 + A] * irc + Vob[IP];
         Wb[1] = Vob[0];
         Wb[IP] = Vob[0] * ic2 + Vob[1 + A] * irc + Vob[IP];

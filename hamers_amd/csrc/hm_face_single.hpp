@@ -23,10 +23,11 @@ HM_HD void riemann_flux_single(const double* VL, const double* VR, double gam, b
     const double rhoL = VL[0], rhoR = VR[0];
     const double pL = VL[IP], pR = VR[IP];
     const double uL = VL[1 + A], uR = VR[1 + A];
-    const double csL = sqrt(gam * pL / rhoL);                // EquationOfStateIdealGas.cpp:5806
-    const double csR = sqrt(gam * pR / rhoR);
-    const double epsL = pL / ((gam - double(1)) * rhoL);    // EquationOfStateIdealGas.cpp:6135
-    const double epsR = pR / ((gam - double(1)) * rhoR);
+    // Quotients through div_rn / quot: correctly rounded, as a / b.
+    const double csL = sqrt(div_rn(gam * pL, rhoL));             // EquationOfStateIdealGas.cpp:5806
+    const double csR = sqrt(div_rn(gam * pR, rhoR));
+    const double epsL = div_rn(pL, (gam - double(1)) * rhoL);    // EquationOfStateIdealGas.cpp:6135
+    const double epsR = div_rn(pR, (gam - double(1)) * rhoR);
 
     // Wave speeds and contact speed (FlowModelRiemannSolverSingleSpeciesHLLC.cpp:619-631).
     const double u_avg = double(1) / double(2) * (uL + uR);
@@ -36,7 +37,7 @@ HM_HD void riemann_flux_single(const double* VL, const double* VR, double gam, b
     const double s_minus = fmin(double(0), sL);
     const double s_plus = fmax(double(0), sR);
     const double s_star =
-        (pR - pL + rhoL * uL * (sL - uL) - rhoR * uR * (sR - uR)) / (rhoL * (sL - uL) - rhoR * (sR - uR));
+        div_rn(pR - pL + rhoL * uL * (sL - uL) - rhoR * uR * (sR - uR), rhoL * (sL - uL) - rhoR * (sR - uR));
 
     // Conservative state and physical flux of one side (:637-648).
     auto cons_flux = [&](const double* Vs, double eps_int, double* Qs, double* Fs) {
@@ -74,12 +75,12 @@ HM_HD void riemann_flux_single(const double* VL, const double* VR, double gam, b
         const double* Fs = left ? FL : FR;
         const double sK = left ? sL : sR;
         const double spm = left ? s_minus : s_plus;
-        const double chi = (sK - Vs[1 + A]) / (sK - s_star);
+        const double chi = div_rn(sK - Vs[1 + A], sK - s_star);
         double Qst[NEQ];
         Qst[0] = chi * Vs[0];
 #pragma unroll
         for (int b = 0; b < D; b++) Qst[1 + b] = (b == A) ? chi * Vs[0] * s_star : chi * Qs[1 + b];
-        Qst[IP] = chi * (Qs[IP] + (s_star - Vs[1 + A]) * (Vs[0] * s_star + Vs[IP] / (sK - Vs[1 + A])));
+        Qst[IP] = chi * (Qs[IP] + (s_star - Vs[1 + A]) * (Vs[0] * s_star + div_rn(Vs[IP], sK - Vs[1 + A])));
 #pragma unroll
         for (int e = 0; e < NEQ; e++) Fo[e] = Fs[e] + spm * (Qst[e] - Qs[e]);
     }
@@ -87,8 +88,9 @@ HM_HD void riemann_flux_single(const double* VL, const double* VR, double gam, b
         if (use_hll) {
             // HLL on mass and transverse momentum, blended by the velocity-jump
             // angle (FlowModelRiemannSolverSingleSpeciesHLLC-HLL.cpp:961-1022).
+            const double rdsl = rcp_rn(sR - sL);
             auto hll = [&](int e) {
-                double h = (sR * FL[e] - sL * FR[e] + sR * sL * (QR[e] - QL[e])) / (sR - sL);
+                double h = quot(sR * FL[e] - sL * FR[e] + sR * sL * (QR[e] - QL[e]), sR - sL, rdsl);
                 if (sL > double(0)) h = FL[e];
                 if (sR < double(0)) h = FR[e];
                 return h;
@@ -110,10 +112,10 @@ HM_HD void riemann_flux_single(const double* VL, const double* VR, double gam, b
                 al1 = double(1);
                 al2 = double(0);
             } else {
-                al1 = fabs(VR[1 + A] - VL[1 + A]) / vel_mag;
+                al1 = div_rn(fabs(VR[1 + A] - VL[1 + A]), vel_mag);
                 al2 = sqrt(double(1) - al1 * al1);
             }
-            const double be1 = double(1) / double(2) * (double(1) + al1 / (al1 + al2));
+            const double be1 = double(1) / double(2) * (double(1) + div_rn(al1, al1 + al2));
             const double be2 = double(1) - be1;
             Fo[0] = be1 * Fo[0] + be2 * hll(0);
 #pragma unroll
@@ -153,7 +155,7 @@ HM_HD void face_flux_single(const double (&V)[6][D + 2], double cL, double cR, d
     {
         Six s;
 #pragma unroll
-        for (int m = 0; m < 6; m++) s.v[m] = V[m][0] - double(1) / (ca * ca) * V[m][IP];
+        for (int m = 0; m < 6; m++) s.v[m] = V[m][0] - div_rn(double(1), ca * ca) * V[m][IP];
         reconstruct_pair<SCHEME, PP, QQ>(s, ld, Wm[1], Wp[1]);
     }
     {
@@ -180,8 +182,9 @@ HM_HD void face_flux_single(const double (&V)[6][D + 2], double cL, double cR, d
     // Back to primitive variables (FlowModelBasicUtilitiesSingleSpecies.cpp:7279-7284 and analogues).
     double VL[NEQ], VR[NEQ];
     auto back = [&](const double* W, double* Vo) {
-        Vo[0] = double(1) / (ca * ca) * W[0] + W[1] + double(1) / (ca * ca) * W[IP];
-        Vo[1 + A] = -double(1) / (ra * ca) * W[0] + double(1) / (ra * ca) * W[IP];
+        const double ic2 = div_rn(double(1), ca * ca), irc = div_rn(double(1), ra * ca);
+        Vo[0] = ic2 * W[0] + W[1] + ic2 * W[IP];
+        Vo[1 + A] = -irc * W[0] + irc * W[IP];
         int t = 2;
 #pragma unroll
         for (int b = 0; b < D; b++) {

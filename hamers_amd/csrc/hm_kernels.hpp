@@ -15,6 +15,7 @@ struct KParams {
     double R[HM_MAX_SPECIES];       // species gas constants
     double cp[HM_MAX_SPECIES];      // species c_p = gamma/(gamma-1) R (four-equation mixing)
     double cv[HM_MAX_SPECIES];      // species c_v = 1/(gamma-1) R
+    double igm1[HM_MAX_SPECIES];    // species 1/(gamma-1) (volume-fraction mixing)
     int num_species;
     double dt;
     // Face-state slot (single-species model): the characteristic face values

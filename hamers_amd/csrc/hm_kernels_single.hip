@@ -108,7 +108,7 @@ __device__ __forceinline__ void midflux_single_face(const DevPatch& P, int64_t l
         const int64_t tL = tR - sts;
         const double th_avg = 0.5 * (P.sens[tL] + P.sens[tR]);
         const double om_avg = 0.5 * (P.sens[vs + tL] + P.sens[vs + tR]);
-        const double sensor = -th_avg / (fabs(th_avg) + om_avg + kEps);
+        const double sensor = div_rn(-th_avg, fabs(th_avg) + om_avg + kEps);
         use_hll = sensor > 0.65;
     }
 

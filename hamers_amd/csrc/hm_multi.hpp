@@ -55,7 +55,7 @@ __device__ __forceinline__ void mix_mass(const KParams& k, const double* Y, doub
         c_p += Y[si] * k.cp[si];
         c_v += Y[si] * k.cv[si];
     }
-    gamma = c_p / c_v;
+    gamma = div_rn(c_p, c_v);
 }
 
 // Four-equation sound speed squared: Gamma p / rho + sum Y_i Psi_i with
@@ -68,11 +68,12 @@ __device__ __forceinline__ double sound_speed_sq_four(const KParams& k, double r
     double gamma, c_v;
     mix_mass<NS>(k, Y, gamma, c_v);
     const double Gamma = gamma - double(1);
-    eps = p / ((gamma - double(1)) * rho);
-    double c2 = Gamma * p / rho;
+    eps = div_rn(p, (gamma - double(1)) * rho);
+    double c2 = div_rn(Gamma * p, rho);
+    const double rcv = rcp_rn(c_v);
 #pragma unroll
     for (int si = 0; si < NS; si++) {
-        const double Psi = ((k.cp[si] - gamma * k.cv[si]) / c_v + gamma - double(1)) * eps;
+        const double Psi = (quot(k.cp[si] - gamma * k.cv[si], c_v, rcv) + gamma - double(1)) * eps;
         c2 += Y[si] * Psi;
     }
     return c2;
@@ -85,10 +86,10 @@ __device__ __forceinline__ double mix_gamma_all(const KParams& k, const double* 
     double xi = double(0);
 #pragma unroll
     for (int si = 0; si < NS; si++) {
-        const double one_over_denominator = double(1) / (k.gamma[si] - double(1));
+        const double one_over_denominator = k.igm1[si];  // double(1) / (gamma_i - 1)
         xi += Z[si] * one_over_denominator;
     }
-    return double(1) / xi + double(1);
+    return div_rn(double(1), xi) + double(1);
 }
 
 // Mixture ratio from NS-1 volume fractions, the last one being 1 - sum
@@ -99,12 +100,12 @@ __device__ __forceinline__ double mix_gamma_last(const KParams& k, const double*
     double Z_last = double(1);
 #pragma unroll
     for (int si = 0; si < NS - 1; si++) {
-        const double one_over_denominator = double(1) / (k.gamma[si] - double(1));
+        const double one_over_denominator = k.igm1[si];  // double(1) / (gamma_i - 1)
         xi += Z[si] * one_over_denominator;
         Z_last -= Z[si];
     }
-    xi += Z_last / (k.gamma[NS - 1] - double(1));
-    return double(1) / xi + double(1);
+    xi += quot(Z_last, k.gamma[NS - 1] - double(1), k.igm1[NS - 1]);
+    return div_rn(double(1), xi) + double(1);
 }
 
 }  // namespace
@@ -142,8 +143,9 @@ __global__ __launch_bounds__(kBlock) void k_derived_multi(W w, KParams k) {
 #pragma unroll
     for (int si = 0; si < NS; si++) rho += Zr[si];
     double u[3] = {0.0, 0.0, 0.0};
+    const double rrho = rcp_rn(rho);  // quotients by rho, correctly rounded (quot)
 #pragma unroll
-    for (int b = 0; b < D; b++) u[b] = P->Q[NS + b][iq] / rho;
+    for (int b = 0; b < D; b++) u[b] = quot(P->Q[NS + b][iq], rho, rrho);
     double ke;
     if constexpr (D == 1)
         ke = u[0] * u[0];
@@ -152,12 +154,12 @@ __global__ __launch_bounds__(kBlock) void k_derived_multi(W w, KParams k) {
     else
         ke = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
     const double E = P->Q[NS + D][iq];
-    const double eps_int = E / rho - double(1) / double(2) * ke;
+    const double eps_int = quot(E, rho, rrho) - double(1) / double(2) * ke;
     double p, c2;
     if constexpr (FOUR) {
         double Y[NS];
 #pragma unroll
-        for (int si = 0; si < NS; si++) Y[si] = Zr[si] / rho;
+        for (int si = 0; si < NS; si++) Y[si] = quot(Zr[si], rho, rrho);
         double gmix, c_v, eps_psi;
         mix_mass<NS>(k, Y, gmix, c_v);
         p = (gmix - double(1)) * rho * eps_int;  // EquationOfStateIdealGas.cpp:5633
@@ -166,11 +168,11 @@ __global__ __launch_bounds__(kBlock) void k_derived_multi(W w, KParams k) {
         const double gmix = mix_gamma_all<NS>(k, Z);
         p = (gmix - double(1)) * rho * eps_int;  // EquationOfStateIdealGas.cpp:5633
         const double Gamma = gmix - double(1);
-        c2 = Gamma * p / rho;  // FlowModelFiveEqnAllaire.cpp:4600-4640
+        c2 = quot(Gamma * p, rho, rrho);  // FlowModelFiveEqnAllaire.cpp:4600-4640
 #pragma unroll
         for (int si = 0; si < NS; si++) {
-            const double Psi = p / rho;  // EquationOfStateMixingRulesIdealGas.cpp:6608
-            c2 += (Zr[si] / rho) * Psi;
+            const double Psi = quot(p, rho, rrho);  // EquationOfStateMixingRulesIdealGas.cpp:6608
+            c2 += quot(Zr[si], rho, rrho) * Psi;
         }
     }
 
@@ -199,9 +201,10 @@ __device__ __forceinline__ bool bounded_multi(const double* V, const KParams& k)
 #pragma unroll
         for (int si = 0; si < NS; si++) rho += V[si];
         bool ok = true;
+        const double rrho = rcp_rn(rho);
 #pragma unroll
         for (int si = 0; si < NS; si++) {
-            const double Y = V[si] / rho;
+            const double Y = quot(V[si], rho, rrho);
             ok = ok && (Y > double(-0.001) && Y < double(1.001));
         }
         return ok && (rho > double(0)) && (V[NS + D] > double(0));
@@ -221,19 +224,20 @@ __device__ __forceinline__ bool bounded_multi(const double* V, const KParams& k)
 #pragma unroll
     for (int si = 0; si < NS; si++) rho += V[si];
     double Y[NS];
+    const double rrho = rcp_rn(rho);
 #pragma unroll
     for (int si = 0; si < NS; si++) {
-        Y[si] = V[si] / rho;
+        Y[si] = quot(V[si], rho, rrho);
         ok = ok && (Y[si] > double(-0.001) && Y[si] < double(1.001));
     }
 #pragma unroll
     for (int si = 0; si < NS; si++) ok = ok && (V[si] > double(0));
     const double p = V[NS + D];
     const double Gamma = mix_gamma_all<NS>(k, Z) - double(1);
-    double c2 = Gamma * p / rho;
+    double c2 = quot(Gamma * p, rho, rrho);
 #pragma unroll
     for (int si = 0; si < NS; si++) {
-        const double Psi = p / rho;
+        const double Psi = quot(p, rho, rrho);
         c2 += Y[si] * Psi;
     }
     return ok && (c2 > double(0));
@@ -281,7 +285,7 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
         Six s;
         if (f == 0) {
 #pragma unroll
-            for (int m = 0; m < 6; m++) s.v[m] = ld(IU, m) - double(1) / (ra * ca) * ld(IP, m);
+            for (int m = 0; m < 6; m++) s.v[m] = ld(IU, m) - div_rn(double(1), ra * ca) * ld(IP, m);
         } else if (f <= NS) {
             const int si = f - 1;
             double z = Zra[0];
@@ -289,7 +293,7 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
             for (int q = 1; q < NS; q++)
                 if (q == si) z = Zra[q];
 #pragma unroll
-            for (int m = 0; m < 6; m++) s.v[m] = ld(si, m) - z / (ra * ca * ca) * ld(IP, m);
+            for (int m = 0; m < 6; m++) s.v[m] = ld(si, m) - div_rn(z, ra * ca * ca) * ld(IP, m);
         } else if (f < NS + D) {
             const int t = f - NS - 1;             // transverse slot
             const int b = (t < A) ? t : t + 1;    // its axis
@@ -300,7 +304,7 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
             for (int m = 0; m < 6; m++) s.v[m] = ld(IZ + (f - NS - D), m);
         } else {
 #pragma unroll
-            for (int m = 0; m < 6; m++) s.v[m] = ld(IU, m) + double(1) / (ra * ca) * ld(IP, m);
+            for (int m = 0; m < 6; m++) s.v[m] = ld(IU, m) + div_rn(double(1), ra * ca) * ld(IP, m);
         }
         double wm, wp;
         reconstruct_pair<SCHEME, PP, QQ>(s, ldk, wm, wp);
@@ -314,11 +318,17 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
 
     // Back-projection (FlowModelBasicUtilitiesFiveEqnAllaire.cpp:9201-9974).
     double VL[NEQ], VR[NEQ];
+    // (1/2 Z_i rho_i) / c: the forward's quotient, shared by both sides; the
+    // minus form (-1/2 Z_i rho_i) / c is its exact negative.
+    double zc[NS];
+    {
+        const double rca = rcp_rn(ca);
+#pragma unroll
+        for (int si = 0; si < NS; si++) zc[si] = quot(double(1) / double(2) * Zra[si], ca, rca);
+    }
     auto back = [&](const double* Wc, double* Vo) {
 #pragma unroll
-        for (int si = 0; si < NS; si++)
-            Vo[si] = -double(1) / double(2) * Zra[si] / ca * Wc[0] + Wc[si + 1] +
-                     double(1) / double(2) * Zra[si] / ca * Wc[NEQ - 1];
+        for (int si = 0; si < NS; si++) Vo[si] = -zc[si] * Wc[0] + Wc[si + 1] + zc[si] * Wc[NEQ - 1];
         Vo[IU] = double(1) / double(2) * Wc[0] + double(1) / double(2) * Wc[NEQ - 1];
         int t = NS + 1;
 #pragma unroll
@@ -350,22 +360,24 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
         if constexpr (FOUR) {
             // FlowModelRiemannSolverFourEqnConservativeHLLC.cpp:4345-4700
             double Y[NS];
+            const double rrho = rcp_rn(rho);
 #pragma unroll
-            for (int si = 0; si < NS; si++) Y[si] = V[si] / rho;
+            for (int si = 0; si < NS; si++) Y[si] = quot(V[si], rho, rrho);
             c = sqrt(sound_speed_sq_four<NS>(k, rho, V[IP], Y, eps_int));
             return;
         }
         const double gmix = mix_gamma_last<NS>(k, V + IZ);
         const double p = V[IP];
         const double Gamma = gmix - double(1);
-        double c2 = Gamma * p / rho;
+        const double rrho = rcp_rn(rho);
+        double c2 = quot(Gamma * p, rho, rrho);
 #pragma unroll
         for (int si = 0; si < NS; si++) {
-            const double Psi = p / rho;
-            c2 += (V[si] / rho) * Psi;
+            const double Psi = quot(p, rho, rrho);
+            c2 += quot(V[si], rho, rrho) * Psi;
         }
         c = sqrt(c2);
-        eps_int = p / ((gmix - double(1)) * rho);
+        eps_int = div_rn(p, (gmix - double(1)) * rho);
     };
     double rhoL, rhoR, cL, cR, epsL, epsR;
     face_eos(VL, rhoL, cL, epsL);
@@ -378,7 +390,7 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
     const double s_minus = fmin(double(0), sL);
     const double s_plus = fmax(double(0), sR);
     const double s_star =
-        (pR - pL + rhoL * uL * (sL - uL) - rhoR * uR * (sR - uR)) / (rhoL * (sL - uL) - rhoR * (sR - uR));
+        div_rn(pR - pL + rhoL * uL * (sL - uL) - rhoR * uR * (sR - uR), rhoL * (sL - uL) - rhoR * (sR - uR));
 
     // Sensor (WCNS56-HLLC-HLL.cpp:1066-1113).
     bool use_hll = false;
@@ -389,7 +401,7 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
         const int64_t tL = tR - sts;
         const double th_avg = 0.5 * (P.sens[tL] + P.sens[tR]);
         const double om_avg = 0.5 * (P.sens[vs + tL] + P.sens[vs + tR]);
-        const double sensor = -th_avg / (fabs(th_avg) + om_avg + kEps);
+        const double sensor = div_rn(-th_avg, fabs(th_avg) + om_avg + kEps);
         use_hll = sensor > 0.65;
     }
 
@@ -439,13 +451,13 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
         const double rho = left ? rhoL : rhoR;
         const double sK = left ? sL : sR;
         const double spm = left ? s_minus : s_plus;
-        const double chi = (sK - Vs[IU]) / (sK - s_star);
+        const double chi = div_rn(sK - Vs[IU], sK - s_star);
         double Qst[NEQ];
 #pragma unroll
         for (int si = 0; si < NS; si++) Qst[si] = chi * Vs[si];
 #pragma unroll
         for (int b = 0; b < D; b++) Qst[NS + b] = (b == A) ? chi * rho * s_star : chi * Qs[NS + b];
-        Qst[IP] = chi * (Qs[IP] + (s_star - Vs[IU]) * (rho * s_star + Vs[IP] / (sK - Vs[IU])));
+        Qst[IP] = chi * (Qs[IP] + (s_star - Vs[IU]) * (rho * s_star + div_rn(Vs[IP], sK - Vs[IU])));
 #pragma unroll
         for (int si = 0; si < NZ; si++) Qst[IZ + si] = chi * Vs[IZ + si];
 #pragma unroll
@@ -456,8 +468,9 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
         if (use_hll) {
             // HLL on partial densities, transverse momentum and volume fractions
             // (FlowModelRiemannSolverFiveEqnAllaireHLLC-HLL.cpp:1283-1504).
+            const double rdsl = rcp_rn(sR - sL);
             auto hll = [&](int e) {
-                double h = (sR * FL[e] - sL * FR[e] + sR * sL * (QR[e] - QL[e])) / (sR - sL);
+                double h = quot(sR * FL[e] - sL * FR[e] + sR * sL * (QR[e] - QL[e]), sR - sL, rdsl);
                 if (sL > double(0)) h = FL[e];
                 if (sR < double(0)) h = FR[e];
                 return h;
@@ -479,10 +492,10 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
                 al1 = double(1);
                 al2 = double(0);
             } else {
-                al1 = fabs(VR[IU] - VL[IU]) / vel_mag;
+                al1 = div_rn(fabs(VR[IU] - VL[IU]), vel_mag);
                 al2 = sqrt(double(1) - al1 * al1);
             }
-            const double be1 = double(1) / double(2) * (double(1) + al1 / (al1 + al2));
+            const double be1 = double(1) / double(2) * (double(1) + div_rn(al1, al1 + al2));
             const double be2 = double(1) - be1;
 #pragma unroll
             for (int si = 0; si < NS; si++) Fo[si] = be1 * Fo[si] + be2 * hll(si);

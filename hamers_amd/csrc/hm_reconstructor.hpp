@@ -67,6 +67,9 @@ class ConvectiveFluxReconstructorGPU {
             const bool used = i < cfg.num_species;
             k_.cp[i] = used ? cfg.species_gamma[i] / (cfg.species_gamma[i] - double(1)) * cfg.species_R[i] : 0.0;
             k_.cv[i] = used ? double(1) / (cfg.species_gamma[i] - double(1)) * cfg.species_R[i] : 0.0;
+            // EquationOfStateMixingRulesIdealGas.cpp:7404-7800 divides per cell; the
+            // quotient is the same correctly rounded number computed once here.
+            k_.igm1[i] = used ? double(1) / (cfg.species_gamma[i] - double(1)) : 0.0;
         }
         k_.num_species = cfg.num_species;
         k_.dt = 0.0;
