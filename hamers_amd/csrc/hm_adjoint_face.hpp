@@ -22,25 +22,11 @@ namespace hm {
 
 HM_HD double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
 
-// 1/b for the reverse sweep, to within a few ulps: on the device the
-// hardware estimate refined by two Newton-Raphson steps r <- r + r (1 - b r)
-// (each step squares the relative error).  The adjoint's outputs carry no
-// bit-exactness requirement, and a quotient by a shared denominator then costs
-// one multiply instead of a full correctly rounded division.  Quotients that
-// decide a branch (R_tau > alpha_tau, t1 >= t2, s* > 0, ...) keep the forward
-// code's correctly rounded division.  Valid for normal, finite, nonzero b
-// (every denominator here is a sum with a positive floor or a state value).
-HM_HD double adj_rcp(double b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    double r = __builtin_amdgcn_rcp(b);
-    double e = fma(-b, r, 1.0);
-    r = fma(e, r, r);
-    e = fma(-b, r, 1.0);
-    return fma(e, r, r);
-#else
-    return 1.0 / b;
-#endif
-}
+// The reverse sweep multiplies by reciprocals of shared denominators
+// (rcp_rn): its outputs carry no bit-exactness requirement, and one multiply
+// replaces a quotient.  Quotients that decide a branch (R_tau > alpha_tau,
+// t1 >= t2, s* > 0, ...) are recomputed with the forward pass's correctly
+// rounded quotients (div_rn, quot), so every branch goes the forward's way.
 
 // d/dx of x^E (E >= 1) times g.
 template <int E>
@@ -94,17 +80,17 @@ HM_HD void ld_sigma_adj(const Six& s, double sigb, double* sb) {
     const double x1 = a1 - a2, x2 = a2 - a3;
     const double den1 = fabs(a1) + fabs(a2) + kEps;
     const double den2 = fabs(a2) + fabs(a3) + kEps;
-    const double t1 = fabs(x1) / den1;  // the branch below compares them as the forward does
-    const double t2 = fabs(x2) / den2;
+    const double t1 = div_rn(fabs(x1), den1);  // the branch below compares them as the forward does
+    const double t2 = div_rn(fabs(x2), den2);
     double a1b = 0.0, a2b = 0.0, a3b = 0.0;
     if (t1 >= t2) {
-        const double id = adj_rcp(den1);
+        const double id = rcp_rn(den1);
         const double xb = sigb * sgn(x1) * id;
         const double db = -sigb * t1 * id;
         a1b += xb + db * sgn(a1);
         a2b += -xb + db * sgn(a2);
     } else {
-        const double id = adj_rcp(den2);
+        const double id = rcp_rn(den2);
         const double xb = sigb * sgn(x2) * id;
         const double db = -sigb * t2 * id;
         a2b += xb + db * sgn(a2);
@@ -126,18 +112,18 @@ HM_HD void ld_side_adj(const Six& s, double sigma, const LDConst& k, double ob, 
     double b0, b1, b2;
     beta3(s, b0, b1, b2);
     const double b3 = beta6(s);
-    const double ie0 = adj_rcp(b0 + kEps), ie1 = adj_rcp(b1 + kEps), ie2 = adj_rcp(b2 + kEps),
-                 ie3 = adj_rcp(b3 + kEps);
+    const double ie0 = rcp_rn(b0 + kEps), ie1 = rcp_rn(b1 + kEps), ie2 = rcp_rn(b2 + kEps),
+                 ie3 = rcp_rn(b3 + kEps);
     const double bavg = double(1) / double(8) * (b0 + b2 + double(6) * b1);
     const double d3 = b3 - bavg;
     const double tau6 = fabs(d3);
-    const bool blend = tau6 / (bavg + kEps) > k.alpha_tau;
+    const bool blend = div_rn(tau6, bavg + kEps) > k.alpha_tau;
     const double q0 = tau6 * ie0, q1 = tau6 * ie1, q2 = tau6 * ie2, q3 = tau6 * ie3;
     const double w0 = double(1) / double(32) * (k.C + rep_pow<Q>(q0, k.q));
     const double w1 = double(15) / double(32) * (k.C + rep_pow<Q>(q1, k.q));
     const double w2 = double(15) / double(32) * (k.C + rep_pow<Q>(q2, k.q));
     const double w3 = double(1) / double(32) * (k.C + rep_pow<Q>(q3, k.q));
-    const double iws = adj_rcp(w0 + w1 + w2 + w3);
+    const double iws = rcp_rn(w0 + w1 + w2 + w3);
     const double wh0 = w0 * iws, wh1 = w1 * iws, wh2 = w2 * iws, wh3 = w3 * iws;
     const double* v = s.v;
     const double Wb0 = ob * (3.0 / 8.0 * v[0] - 10.0 / 8.0 * v[1] + 15.0 / 8.0 * v[2]);
@@ -155,7 +141,7 @@ HM_HD void ld_side_adj(const Six& s, double sigma, const LDConst& k, double ob, 
         const double u0 = double(1) / double(16) * (double(1) + rep_pow<P>(r0, k.p));
         const double u1 = double(5) / double(8) * (double(1) + rep_pow<P>(r1, k.p));
         const double u2 = double(5) / double(16) * (double(1) + rep_pow<P>(r2, k.p));
-        const double ius = adj_rcp(u0 + u1 + u2);
+        const double ius = rcp_rn(u0 + u1 + u2);
         const double uh0 = u0 * ius, uh1 = u1 * ius, uh2 = u2 * ius;
         const double om = double(1) - sigma;
         W0 = sigma * uh0 + om * wh0;
@@ -217,11 +203,11 @@ HM_HD void js_side_adj(const Six& s, int p_rt, double ob, double* sb) {
     if (ob == 0.0) return;
     double b0, b1, b2;
     beta3(s, b0, b1, b2);
-    const double ie0 = adj_rcp(b0 + kEps), ie1 = adj_rcp(b1 + kEps), ie2 = adj_rcp(b2 + kEps);
+    const double ie0 = rcp_rn(b0 + kEps), ie1 = rcp_rn(b1 + kEps), ie2 = rcp_rn(b2 + kEps);
     const double w0 = double(1.0) / double(16) * rep_pow<P>(ie0, p_rt);
     const double w1 = double(5.0) / double(8) * rep_pow<P>(ie1, p_rt);
     const double w2 = double(5.0) / double(16) * rep_pow<P>(ie2, p_rt);
-    const double iws = adj_rcp(w0 + w1 + w2);
+    const double iws = rcp_rn(w0 + w1 + w2);
     const double W0 = w0 * iws, W1 = w1 * iws, W2 = w2 * iws;
     const double* v = s.v;
     sb[0] += ob * (3.0 / 8.0 * W0);
@@ -283,10 +269,11 @@ HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bo
     const double rhoL = VL[0], rhoR = VR[0];
     const double pL = VL[IP], pR = VR[IP];
     const double uL = VL[1 + A], uR = VR[1 + A];
-    const double csL = sqrt(gam * pL / rhoL);
-    const double csR = sqrt(gam * pR / rhoR);
-    const double epsL = pL / ((gam - double(1)) * rhoL);
-    const double epsR = pR / ((gam - double(1)) * rhoR);
+    // Forward quotients recomputed exactly as the forward pass rounds them.
+    const double csL = sqrt(div_rn(gam * pL, rhoL));
+    const double csR = sqrt(div_rn(gam * pR, rhoR));
+    const double epsL = div_rn(pL, (gam - double(1)) * rhoL);
+    const double epsR = div_rn(pR, (gam - double(1)) * rhoR);
     const double u_avg = double(1) / double(2) * (uL + uR);
     const double c_avg = double(1) / double(2) * (csL + csR);
     const double A1 = u_avg - c_avg, A2 = uL - csL;
@@ -297,7 +284,7 @@ HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bo
     const double s_plus = fmax(double(0), sR);
     const double num = pR - pL + rhoL * uL * (sL - uL) - rhoR * uR * (sR - uR);
     const double den = rhoL * (sL - uL) - rhoR * (sR - uR);
-    const double s_star = num / den;
+    const double s_star = div_rn(num, den);
 
     // Conservative states and fluxes of both sides.
     double Q[2][NEQ], F[2][NEQ];
@@ -334,9 +321,9 @@ HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bo
         QK[e] = left ? Q[0][e] : Q[1][e];
         FK[e] = left ? F[0][e] : F[1][e];
     }
-    const double chi = (sK - uK) / (sK - s_star);
+    const double chi = div_rn(sK - uK, sK - s_star);
     const double g = sK - uK;
-    const double h = rhoK * s_star + pK / g;
+    const double h = rhoK * s_star + div_rn(pK, g);
     const double inner = QK[IP] + (s_star - uK) * h;
     double Qst[NEQ], Fh[NEQ];
     Qst[0] = chi * rhoK;
@@ -389,15 +376,15 @@ HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bo
                 al1 = double(1);
                 al2 = double(0);
             } else {
-                al1 = fabs(VR[1 + A] - VL[1 + A]) / vm;
+                al1 = div_rn(fabs(VR[1 + A] - VL[1 + A]), vm);
                 al2 = sqrt(double(1) - al1 * al1);
             }
             const double Ssum = al1 + al2;
-            const double be1 = double(1) / double(2) * (double(1) + al1 / Ssum);
+            const double be1 = double(1) / double(2) * (double(1) + div_rn(al1, Ssum));
             const double be2 = double(1) - be1;
             double be1b = 0.0, be2b = 0.0;
             const double dsl = sR - sL;
-            const double idsl = adj_rcp(dsl);
+            const double idsl = rcp_rn(dsl);
 #pragma unroll
             for (int e = 0; e < NEQ; e++) {
                 const bool hyb = (e == 0) || (e >= 1 && e <= D && e != 1 + A);
@@ -405,7 +392,7 @@ HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bo
                 // recompute the HLL flux of component e
                 double hv;
                 const double numh = sR * F[0][e] - sL * F[1][e] + sR * sL * (Q[1][e] - Q[0][e]);
-                hv = numh / dsl;
+                hv = quot(numh, dsl, idsl);
                 if (sL > double(0)) hv = F[0][e];
                 if (sR < double(0)) hv = F[1][e];
                 const double fb = Fb[e];
@@ -430,13 +417,13 @@ HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bo
             }
             be1b -= be2b;
             if (!tiny) {
-                const double iS = adj_rcp(Ssum);
+                const double iS = rcp_rn(Ssum);
                 const double al1b = be1b * 0.5 * al2 * (iS * iS);
                 double al2b = -be1b * 0.5 * al1 * (iS * iS);
                 double a1b = al1b;
-                if (al2 > 0.0) a1b += al2b * (-al1 * adj_rcp(al2));
+                if (al2 > 0.0) a1b += al2b * (-al1 * rcp_rn(al2));
                 const double dua = du[A];
-                const double ivm = adj_rcp(vm);  // vm >= kEps here
+                const double ivm = rcp_rn(vm);  // vm >= kEps here
                 const double duab = a1b * sgn(dua) * ivm;
                 const double vmb = -a1b * al1 * ivm;
                 const double dsqb = vmb * 0.5 * ivm;
@@ -483,14 +470,14 @@ HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bo
         const double hb = innerb * (s_star - uK);
         rhoKb += hb * s_star;
         sstb += hb * rhoK;
-        const double ig = adj_rcp(g);
+        const double ig = rcp_rn(g);
         pKb += hb * ig;
         const double gb = -hb * pK * (ig * ig);
         sKb += gb;
         uKb -= gb;
     }
     {
-        const double ib2 = adj_rcp(sK - s_star);
+        const double ib2 = rcp_rn(sK - s_star);
         const double ab = chib * ib2;
         const double b2b = -chib * chi * ib2;
         sKb += ab + b2b;
@@ -554,7 +541,7 @@ HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bo
     }
     // s* = num / den
     {
-        const double iden = adj_rcp(den);
+        const double iden = rcp_rn(den);
         const double numb = sstb * iden;
         const double denb = -sstb * s_star * iden;
         Vb[1][IP] += numb;
@@ -588,12 +575,12 @@ HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bo
     csb[1] += 0.5 * cavgb;
     // c = sqrt(gam p / rho), eps = p / ((gam - 1) rho)
     const double css[2] = {csL, csR};
-    const double igm1 = adj_rcp(gam - 1.0);
+    const double igm1 = rcp_rn(gam - 1.0);
 #pragma unroll
     for (int x = 0; x < 2; x++) {
         const double* V = Vs[x];
-        const double irho = adj_rcp(V[0]);
-        const double ab = csb[x] * 0.5 * adj_rcp(css[x]);
+        const double irho = rcp_rn(V[0]);
+        const double ab = csb[x] * 0.5 * rcp_rn(css[x]);
         Vb[x][IP] += ab * gam * irho + epsb[x] * igm1 * irho;
         Vb[x][0] -= ab * gam * V[IP] * (irho * irho) + epsb[x] * epss[x] * irho;
     }
