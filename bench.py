@@ -41,6 +41,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+MIDFLUX_TRAFFIC_2048_WCNS5 = 836.6e6 + 274.7e6  # bytes per k_midflux launch, 2048^2 / 128^2 patches, WCNS5-JS
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector
 
 
@@ -445,6 +446,13 @@ def main():
     # F on d axes (d*neq): 8*(4 + 2*4) = 96 B for 2D single-species.
     bytes_per_cell = 8 * (neq + 2 * neq)
     achieved_gbs = bytes_per_cell * cells / (stage_avg[dominant] * 1e-3) / 1e9
+    # DRAM bytes per launch of the dominant kernel, measured with rocprofv3
+    # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a copy of known size)
+    # for the default workload only; other sizes report null.
+    traffic, traffic_source = None, None
+    if (args.N, args.patch, args.scheme, dominant) == (2048, 128, "WCNS5_JS_HLLC_HLL", "midflux"):
+        traffic = MIDFLUX_TRAFFIC_2048_WCNS5
+        traffic_source = "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/k_midflux_single_2048sq_counters.md"
     rhs_gbs = bytes_per_cell * cells / (ms_batched * 1e-3) / 1e9
     value = cells / (ms_batched * 1e-3)
 
@@ -505,7 +513,8 @@ def main():
                                "128^2, 4-cell halos, WCNS5_JS_HLLC_HLL, batched RHS", "cells": cells,
                    "patches": npatches, "patch": [P, P], "scheme": args.scheme},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_source,
                      "kernel": f"k_{dominant}", "kernel_ms": stage_avg[dominant],
                      "bytes_per_cell": bytes_per_cell},
         "rhs_hbm_frac": rhs_gbs / HBM_PEAK_GBS,
