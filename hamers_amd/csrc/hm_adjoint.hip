@@ -75,7 +75,7 @@ struct LamLine {
         const int64_t i0 = (x0 + lg) + int64_t(x1 + ((D > 1) ? lg : 0)) * le0 + int64_t(x2 + ((D > 2) ? lg : 0)) * le0 * le1;
         L = P.lam.L0 + int64_t(k.lam_src) * P.lam.stride + i0;
         n = B.n[a];
-        sc = k.lam_scale / P.f.dx[a];
+        sc = div_rn(k.lam_scale, P.f.dx[a]);
     }
     __device__ __forceinline__ double cell(int e, int x) const {
         return (x >= 0 && x < n) ? L[e * vol + int64_t(x) * st] : 0.0;
@@ -188,20 +188,22 @@ __device__ __forceinline__ void adj_face_recon(const AdjPatch& P, int64_t loc, c
     constexpr int NH = face_adj_h_size<D>();
     const DevPatch& F = P.f;
     const Boxes<D> B(F);
-    const int64_t mv = B.mid_vol(A);
-    const double* H = P.H[A] + loc;
-    if (H[(NH - 1) * mv] == 2.0) return;
+    const uint32_t mv = uint32_t(B.mid_vol(A)), l32 = uint32_t(loc);
+    const double* H = P.H[A];
+    if (ldg(H, (NH - 1) * mv + l32) == 2.0) return;
     int c0, c1, c2;
     mid_cell<D, A>(B, loc, c0, c1, c2);
     const int64_t v4 = B.vol(kGc);
-    const int64_t st = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGc)) : int64_t(B.ext(0, kGc)) * B.ext(1, kGc);
-    const double* pc = F.prim + B.idx(kGc, c0, c1, c2);  // cell right of the face
-    const double cL = pc[NEQ * v4 - st];
-    const double cR = pc[NEQ * v4];
-    double* G = P.G[A] + loc;
-    face_adj_single_recon<D, A, SCHEME, PP, QQ>([&](int m, int e) { return pc[e * v4 + (m - 3) * st]; }, cL, cR,
-                                                LDConst{k.p, k.q, k.C, k.alpha_tau}, [&](int c) { return H[c * mv]; },
-                                                [&](int c, double v) { G[c * mv] = v; });
+    const uint32_t st = (A == 0) ? 1u : (A == 1) ? uint32_t(B.ext(0, kGc)) : uint32_t(B.ext(0, kGc) * B.ext(1, kGc));
+    const uint32_t iR = uint32_t(B.idx(kGc, c0, c1, c2));  // cell right of the face
+    const double* pr = F.prim;
+    const double cL = ldg(pr + NEQ * v4, iR - st);
+    const double cR = ldg(pr + NEQ * v4, iR);
+    double* G = P.G[A];
+    face_adj_single_recon<D, A, SCHEME, PP, QQ>(
+        [&](int m, int e) { return ldg(pr + e * v4, iR + uint32_t(m - 3) * st); }, cL, cR,
+        LDConst{k.p, k.q, k.C, k.alpha_tau}, [&](int c) { return ldg(H, uint32_t(c) * mv + l32); },
+        [&](int c, double v) { stg(G, uint32_t(c) * mv + l32, v); });
 }
 
 template <int D, int SCHEME, int PP, int QQ, class W, bool STORED = false>
@@ -299,14 +301,15 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(W w, KParams k) {
             const int jf = c[a] + 3 - m;
             if (jf < -1 || jf > na + 1) continue;
             const int64_t fi = B.mid_idx(a, (a == 0) ? jf : c[0], (a == 1) ? jf : c[1], (a == 2) ? jf : c[2]);
-            const double flag = P->H[a][(NH - 1) * mv + fi];
+            const uint32_t fi32 = uint32_t(fi), mv32 = uint32_t(mv);
+            const double flag = ldg(P->H[a], (NH - 1) * mv32 + fi32);
             if (flag == 2.0) continue;
             // projection averages of the face: its cells sit at stencil points 2 and 3
             const int64_t l2 = loc + (2 - m) * sta, l3 = loc + (3 - m) * sta;
             const double ra = double(1) / double(2) * (F.prim[l2] + F.prim[l3]);
             const double ca = double(1) / double(2) * (F.prim[NEQ * v4 + l2] + F.prim[NEQ * v4 + l3]);
-            const double* G = P->G[a] + fi;
-            auto gat = [&](int cc) { return G[cc * mv]; };
+            const double* G = P->G[a];
+            auto gat = [&](int cc) { return ldg(G, uint32_t(cc) * mv32 + fi32); };
             if (a == 0) {
                 face_record_to_point<D, 0>(flag, gat, ra, ca, m, Vb, cb);
             } else if constexpr (D > 1) {
@@ -353,8 +356,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(W w, KParams k) {
         }
     }
     // Derived-data Jacobian.
-    const double pb = Vb[IP] + cb * gam / (2.0 * cs * rho);
-    double rhob = Vb[0] - cb * gam * p / (2.0 * cs * rho * rho);
+    // Reciprocals instead of quotients: the adjoint carries no bit-exactness requirement.
+    const double irho = rcp_rn(rho), i2cr = rcp_rn(2.0 * cs * rho);
+    const double pb = Vb[IP] + cb * gam * i2cr;
+    double rhob = Vb[0] - cb * gam * p * i2cr * irho;
     double ke = 0.0;
 #pragma unroll
     for (int b = 0; b < D; b++) ke += u[b] * u[b];
@@ -362,8 +367,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(W w, KParams k) {
     double* const* Qb = P->Qb;
 #pragma unroll
     for (int b = 0; b < D; b++) {
-        Qb[1 + b][iq] = -pb * (gam - 1.0) * u[b] + Vb[1 + b] / rho + Qbd[1 + b];
-        rhob -= Vb[1 + b] * u[b] / rho;
+        Qb[1 + b][iq] = -pb * (gam - 1.0) * u[b] + Vb[1 + b] * irho + Qbd[1 + b];
+        rhob -= Vb[1 + b] * u[b] * irho;
     }
     Qb[0][iq] = rhob + Qbd[0];
     Qb[IP][iq] = pb * (gam - 1.0) + Qbd[IP];

@@ -61,7 +61,7 @@ HM_HD void beta6_adj(const Six& s, double bb3, double* sb) {
     const double c33 = 20.0 * 1607794163.0, c34 = -20.0 * 1486026707.0, c35 = 20.0 * 268747951.0;
     const double c44 = 5.0 * 1432381427.0, c45 = -5.0 * 536951582.0;
     const double c55 = 263126407.0;
-    const double g = bb3 / 232243200.0;
+    const double g = bb3 * (1.0 / 232243200.0);
     const double* v = s.v;
     sb[0] += g * (2.0 * c00 * v[0] + c01 * v[1] + c02 * v[2] + c03 * v[3] + c04 * v[4] + c05 * v[5]);
     sb[1] += g * (c01 * v[0] + 2.0 * c11 * v[1] + c12 * v[2] + c13 * v[3] + c14 * v[4] + c15 * v[5]);
@@ -679,6 +679,7 @@ HM_HD void face_adj_single_after_recon(const double* Wm, const double* Wp, const
     }
     // Back-projection adjoint for both sides.
     double rab = 0.0, cab = 0.0;
+    const double ica = rcp_rn(ca), ira = rcp_rn(ra);
     auto back_adj = [&](const double* W, const double* Vob, double* Wb) {
         const double ic2 = div_rn(double(1), ca * ca);
         const double irc = div_rn(double(1), ra * ca);
@@ -691,9 +692,9 @@ HM_HD void face_adj_single_after_recon(const double* Wm, const double* Wp, const
             if (b == A) continue;
             Wb[t++] = Vob[1 + b];
         }
-        cab += Vob[0] * (W[0] + W[IP]) * (-2.0 * ic2 / ca);
-        cab += Vob[1 + A] * (W[IP] - W[0]) * (-irc / ca);
-        rab += Vob[1 + A] * (W[IP] - W[0]) * (-irc / ra);
+        cab += Vob[0] * (W[0] + W[IP]) * (-2.0 * ic2 * ica);
+        cab += Vob[1 + A] * (W[IP] - W[0]) * (-irc * ica);
+        rab += Vob[1 + A] * (W[IP] - W[0]) * (-irc * ira);
     };
     double Wmb[NEQ], Wpb[NEQ];
     back_adj(Wm, VLb, Wmb);
@@ -763,6 +764,7 @@ HM_HD void face_adj_single_recon(const VAt& vat, double cL, double cR, const LDC
     const double ra = double(1) / double(2) * (vat(2, 0) + vat(3, 0));
     const double ca = double(1) / double(2) * (cL + cR);
     double rab = hat(2 * NEQ), cab = hat(2 * NEQ + 1);
+    const double ic3 = rcp_rn(ca * ca * ca);
 #pragma unroll 1
     for (int f = 0; f < NEQ; f++) {
         Six s;
@@ -779,7 +781,7 @@ HM_HD void face_adj_single_recon(const VAt& vat, double cL, double cR, const LDC
             }
         } else if (f == 1) {
 #pragma unroll
-            for (int m = 0; m < 6; m++) cab += sb[m] * vat(m, IP) * 2.0 / (ca * ca * ca);
+            for (int m = 0; m < 6; m++) cab += sb[m] * vat(m, IP) * 2.0 * ic3;
         }
 #pragma unroll
         for (int m = 0; m < 6; m++) gout(f * 6 + m, sb[m]);
@@ -808,7 +810,7 @@ HM_HD void face_record_to_point(double flag, const GAt& gat, double ra, double c
     Vbm[1 + A] += g0 * -0.5 * ra * ca;
     Vbm[IP] += 0.5 * g0;
     Vbm[0] += g1;
-    Vbm[IP] -= g1 / (ca * ca);
+    Vbm[IP] -= g1 * rcp_rn(ca * ca);
     int t = 2;
     for (int b = 0; b < D; b++) {
         if (b == A) continue;

@@ -12,6 +12,25 @@ constexpr int kGc = 4;     // convective ghost width (WCNS56-HLLC-HLL.cpp:22)
 constexpr int kGs = 2;     // ghost width of dilatation / vorticity (WCNS56-HLLC-HLL.cpp:116-123)
 constexpr int kBlock = 256;
 
+// Element i of a per-patch array (one component of a patch's field), read as
+// global memory at a 32-bit byte offset from a wave-uniform base: the load
+// then takes its base in scalar registers and one 32-bit vector offset
+// instead of a 64-bit flat address per lane.  Every entry point keeps a
+// patch's ghost box below 2^29 cells, so i * 8 fits 32 bits.  i may be
+// formed with wrap-around (base + i - k, i >= k) as unsigned arithmetic.
+__device__ __forceinline__ double ldg(const double* base, uint32_t i) {
+    typedef __attribute__((address_space(1))) const char gchar;
+    typedef __attribute__((address_space(1))) const double gdouble;
+    return *(gdouble*)((gchar*)base + i * 8u);
+}
+
+// Store counterpart of ldg.
+__device__ __forceinline__ void stg(double* base, uint32_t i, double v) {
+    typedef __attribute__((address_space(1))) char gchar;
+    typedef __attribute__((address_space(1))) double gdouble;
+    *(gdouble*)((gchar*)base + i * 8u) = v;
+}
+
 // Per-patch descriptor as the kernels see it: caller arrays plus this
 // library's scratch for the patch.
 struct DevPatch {

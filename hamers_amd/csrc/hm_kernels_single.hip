@@ -90,13 +90,14 @@ __device__ __forceinline__ void midflux_single_face(const DevPatch& P, int64_t l
     const double* pr = P.prim;
 
     // Stencil of primitive variables, cells i-3 .. i+2.
+    const uint32_t i32 = uint32_t(iR), st32 = uint32_t(st);
     double V[6][NEQ];
 #pragma unroll
     for (int m = 0; m < 6; m++)
 #pragma unroll
-        for (int e = 0; e < NEQ; e++) V[m][e] = pr[e * v4 + iR + (m - 3) * st];
-    const double cL = pr[NEQ * v4 + iR - st];
-    const double cR = pr[NEQ * v4 + iR];
+        for (int e = 0; e < NEQ; e++) V[m][e] = ldg(pr + e * v4, i32 + uint32_t(m - 3) * st32);
+    const double cL = ldg(pr + NEQ * v4, i32 - st32);
+    const double cR = ldg(pr + NEQ * v4, i32);
 
     // Sensor (WCNS56-HLLC-HLL.cpp:1066-1113): only when it exceeds 0.65 is the
     // HLLC-HLL flux used, so HLLC-HLL is evaluated only on those faces.
