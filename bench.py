@@ -48,8 +48,10 @@ FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=30)
+    # The GPU's clocks ramp over the first few milliseconds of load; ten
+    # untimed RHS evaluations (about 8 ms) keep the ramp out of the timing.
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--N", type=int, default=2048)
     ap.add_argument("--patch", type=int, default=128)
     ap.add_argument("--scheme", default="WCNS5_JS_HLLC_HLL")
