@@ -36,6 +36,19 @@ struct KParams {
     int64_t qoff;
 };
 
+// Fused midpoint and face flux of the single-species model along axis 0
+// (k_flux_single): a wavefront computes kChunk + 2 consecutive midpoint faces
+// and outputs kChunk face fluxes.
+constexpr int kChunk = 62;
+
+// Work items of the axis-0 segment of one patch in the fused kernel (whole
+// wavefronts): n0 + 3 midpoint faces per row, rows of the other axes.
+__host__ __device__ inline int64_t fused_items_axis0(const int* n, int dim) {
+    int64_t mid = n[0] + 3;
+    for (int b = 1; b < dim; b++) mid *= n[b];
+    return int64_t(64) * ((mid - 3) / kChunk + 1);
+}
+
 // Work of one RHS evaluation: one work source per pipeline stage
 // (0 derived cells, 1 sensor cells, 2 midpoint faces, 3 output faces,
 // 4 interior cells for the advective source) and the number of work items

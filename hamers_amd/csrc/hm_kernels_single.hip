@@ -136,8 +136,132 @@ __device__ __forceinline__ void midflux_single_face(const DevPatch& P, int64_t l
     for (int e = 0; e < NEQ; e++) out[e * mv + loc] = Fo[e];
 }
 
+
+// ----------------------------------------------------------------------------
+// Fused midpoint and face flux along axis 0 (single-species): a wavefront
+// computes kChunk + 2 consecutive midpoint faces of the patch's flattened
+// face list (rows of faces -1 .. n+1, unit stride in memory), kChunk apart
+// from the next wavefront's, and its lanes 1..kChunk finish their output faces
+// with their neighbours' midpoint fluxes (cross-lane shuffles) and the node
+// fluxes of their two cells (WCNS56-HLLC-HLL.cpp:1197-1282, 2327-2490), so
+// the midpoint flux of axis 0 never goes to memory.  A row is complete in the
+// list, so the neighbours of an output face are in its row.  On axes 1 and 2
+// the lanes of a wavefront lie across the sweep (coalesced loads); there the
+// midpoint flux goes through memory to the face-flux kernel.
+// ----------------------------------------------------------------------------
+
+// Midpoint flux of the face whose right cell is (c0, c1, c2) along A, from its
+// stencil V (cells i-3 .. i+2) and the sound speeds of cells i-1, i.
+template <int D, int A, int SCHEME, int PP, int QQ, bool REC>
+__device__ __forceinline__ void midpoint_flux_at(const DevPatch& P, const Boxes<D>& B, const KParams& k,
+                                                 const double (&V)[6][D + 2], double cL, double cR, int c0, int c1,
+                                                 int c2, double (&Fo)[D + 2]) {
+    constexpr int NEQ = D + 2;
+    bool use_hll = false;
+    if constexpr (D > 1) {
+        const int64_t vs = B.vol(kGs);
+        const uint32_t tR = uint32_t(B.idx(kGs, c0, c1, c2));
+        const uint32_t sts = (A == 0) ? 1u : (A == 1) ? uint32_t(B.ext(0, kGs)) : uint32_t(B.ext(0, kGs) * B.ext(1, kGs));
+        const double th_avg = 0.5 * (ldg(P.sens, tR - sts) + ldg(P.sens, tR));
+        const double om_avg = 0.5 * (ldg(P.sens + vs, tR - sts) + ldg(P.sens + vs, tR));
+        const double sensor = div_rn(-th_avg, fabs(th_avg) + om_avg + kEps);
+        use_hll = sensor > 0.65;
+    }
+    if constexpr (REC) {
+        const uint32_t mv = uint32_t(B.mid_vol(A)), mi = uint32_t(B.mid_idx(A, c0, c1, c2));
+        double* wr = k.wrec + P.wrec[A];
+        face_flux_single<D, A, SCHEME, PP, QQ>(V, cL, cR, k.gamma[0], LDConst{k.p, k.q, k.C, k.alpha_tau}, use_hll,
+                                               Fo, [&](const double* Wm, const double* Wp) {
+#pragma unroll
+                                                   for (int c = 0; c < NEQ; c++) {
+                                                       stg(wr, uint32_t(c) * mv + mi, Wm[c]);
+                                                       stg(wr, uint32_t(NEQ + c) * mv + mi, Wp[c]);
+                                                   }
+                                               });
+    } else {
+        face_flux_single<D, A, SCHEME, PP, QQ>(V, cL, cR, k.gamma[0], LDConst{k.p, k.q, k.C, k.alpha_tau}, use_hll,
+                                               Fo);
+    }
+}
+
+// Node flux of equation e at a cell (FlowModelSingleSpecies.cpp:3329-3331,
+// 3388-3391, 3470-3474) from its conservative variables (index q) and its
+// velocity ua along A and pressure p.
+template <int D, int A>
+__device__ __forceinline__ double node_flux_single(const DevPatch& P, int64_t q, double ua, double p, int e) {
+    constexpr int IP = D + 1;
+    if (e == 0) return P.Q[1 + A][q];
+    if (e == IP) return ua * (P.Q[IP][q] + p);
+    if (e == 1 + A) return ua * P.Q[e][q] + p;
+    return ua * P.Q[e][q];
+}
+
+// Output flux of face f along A: dt (1/30 (m_{f+1} + m_{f-1}) - 3/10 (n_f + n_{f-1}) + 23/15 m_f).
+template <int D>
+__device__ __forceinline__ void store_face_flux(double* out, int64_t fv, int64_t fi, double dt, const double* mL,
+                                                const double* mM, const double* mR, const double* nL,
+                                                const double* nR) {
+#pragma unroll
+    for (int e = 0; e < D + 2; e++)
+        out[e * fv + fi] = dt * (double(1) / double(30) * (mR[e] + mL[e]) - double(3) / double(10) * (nR[e] + nL[e]) +
+                                 double(23) / double(15) * mM[e]);
+}
+
+template <int D, int SCHEME, int PP, int QQ, bool REC>
+__device__ __forceinline__ void fused_axis0(const DevPatch& P, int64_t loc, const KParams& k) {
+    constexpr int NEQ = D + 2;
+    constexpr int IP = D + 1;
+    const Boxes<D> B(P);
+    const uint32_t lane = uint32_t(loc) & 63u;
+    const uint32_t M = (uint32_t(loc) >> 6) * kChunk + lane;  // midpoint face, flattened over the patch's rows
+    const uint32_t mv = uint32_t(B.mid_vol(0));
+    const bool valid = M < mv;
+    const uint32_t m0 = uint32_t(B.mid_ext(0, 0)), m1 = uint32_t(B.mid_ext(0, 1));
+    const uint32_t Mc = valid ? M : mv - 1;  // lanes past the end compute a real face and drop it
+    const uint32_t r = Mc / m0;
+    const int cm = int(Mc - r * m0);  // position in the row: face between cells cm-2 and cm-1
+    const int c0 = cm - 1;
+    const int c1 = (D > 1) ? int((D > 2) ? r % m1 : r) : 0;
+    const int c2 = (D > 2) ? int(r / m1) : 0;
+
+    const int64_t v4 = B.vol(kGc);
+    const uint32_t iR = uint32_t(B.idx(kGc, c0, c1, c2));
+    const double* pr = P.prim;
+    double V[6][NEQ];
+#pragma unroll
+    for (int m = 0; m < 6; m++)
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) V[m][e] = ldg(pr + e * v4, iR + uint32_t(m - 3));
+    const double cL = ldg(pr + NEQ * v4, iR - 1u);
+    const double cR = ldg(pr + NEQ * v4, iR);
+    double mM[NEQ];
+    midpoint_flux_at<D, 0, SCHEME, PP, QQ, REC>(P, B, k, V, cL, cR, c0, c1, c2, mM);
+
+    // Neighbours' midpoint fluxes (every lane takes part in the shuffles).
+    double mL[NEQ], mR[NEQ];
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        mL[e] = __shfl_up(mM[e], 1);
+        mR[e] = __shfl_down(mM[e], 1);
+    }
+    // Output face f = c0 of the row, for faces 0..n0 held by lanes 1..62.
+    if (!valid || lane == 0u || lane > uint32_t(kChunk) || cm < 1 || cm > B.n[0] + 1) return;
+    const int64_t qR = q_idx<D>(P, c0, c1, c2), qL = qR - 1;
+    double nL[NEQ], nR[NEQ];
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        nL[e] = node_flux_single<D, 0>(P, qL, V[2][1], V[2][IP], e);
+        nR[e] = node_flux_single<D, 0>(P, qR, V[3][1], V[3][IP], e);
+    }
+    store_face_flux<D>(P.F[0], B.face_vol(0), B.face_idx(0, c0, c1, c2), k.dt, mL, mM, mR, nL, nR);
+}
+
+
+// Stage 2 of the single-species pipeline: axis 0 runs the fused midpoint and
+// face flux; axes 1 and 2 write midpoint fluxes for the face-flux kernel
+// (stage 3), their lanes lying across the sweep so that the loads coalesce.
 template <int D, int SCHEME, int PP, int QQ, class W, bool REC = false>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SCHEME == kWCNS5_JS && D == 2 ? 4 : 1))) void k_midflux_single(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SCHEME == kWCNS5_JS && D == 2 ? 4 : 1))) void k_flux_single(
     W w, KParams k) {
     const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     const DevPatch* P;
@@ -145,7 +269,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SCHEME =
     int64_t loc;
     if (!w.find(gid, P, ax, loc)) return;
     if (ax == 0) {
-        midflux_single_face<D, 0, SCHEME, PP, QQ, REC>(*P, loc, k);
+        fused_axis0<D, SCHEME, PP, QQ, REC>(*P, loc, k);
     } else if constexpr (D > 1) {
         if (ax == 1) {
             midflux_single_face<D, 1, SCHEME, PP, QQ, REC>(*P, loc, k);
@@ -154,6 +278,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SCHEME =
         }
     }
 }
+
 
 // ----------------------------------------------------------------------------
 // Midpoint-to-face flux (sixth order), times dt.
@@ -243,16 +368,18 @@ hipError_t launch_single_dsq(const Plan<W>& plan, const KParams& k, hipStream_t 
             hipLaunchKernelGGL((k_sensor<D, 1, W>), dim3(unsigned((plan.items[1] + kBlock - 1) / kBlock)),
                                dim3(kBlock), 0, s, plan.work[1], k);
     }
+    // Stage 2: fused flux on axis 0, midpoint fluxes on axes 1 and 2; stage 3:
+    // the face flux of axes 1 and 2.
+    bool rec = false;
+    if constexpr (std::is_same_v<W, SegTable>) rec = k.wrec != nullptr;  // recording runs use patch tables
     if (plan.items[2] > 0) {
         const dim3 grid(unsigned((plan.items[2] + kBlock - 1) / kBlock));
-        bool rec = false;
-        if constexpr (std::is_same_v<W, SegTable>) rec = k.wrec != nullptr;  // recording runs use patch tables
         if (rec) {
             if constexpr (std::is_same_v<W, SegTable>)
-                hipLaunchKernelGGL((k_midflux_single<D, SCHEME, PP, QQ, W, true>), grid, dim3(kBlock), 0, s,
-                                   plan.work[2], k);
+                hipLaunchKernelGGL((k_flux_single<D, SCHEME, PP, QQ, W, true>), grid, dim3(kBlock), 0, s, plan.work[2],
+                                   k);
         } else {
-            hipLaunchKernelGGL((k_midflux_single<D, SCHEME, PP, QQ, W>), grid, dim3(kBlock), 0, s, plan.work[2], k);
+            hipLaunchKernelGGL((k_flux_single<D, SCHEME, PP, QQ, W>), grid, dim3(kBlock), 0, s, plan.work[2], k);
         }
     }
     if (plan.items[3] > 0)

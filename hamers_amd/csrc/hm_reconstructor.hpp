@@ -78,6 +78,7 @@ class ConvectiveFluxReconstructorGPU {
         k_.lam_scale = 0.0;
         k_.qoff = 0;
         has_advective_ = (cfg.model == HM_MODEL_FIVE_EQN_ALLAIRE && cfg.num_species > 1);
+        fused_ = (cfg.model == HM_MODEL_SINGLE_SPECIES);
         plans_.reserve(kMaxPlans);  // plan addresses stay valid while cached
     }
 
@@ -141,7 +142,7 @@ class ConvectiveFluxReconstructorGPU {
             int64_t acc = 0;
             for (int a = 0; a < d; a++) {
                 w.start[a] = acc;
-                w.cnt[a] = (stage == 2) ? e.mid[a] : e.face[a];
+                w.cnt[a] = (stage == 2) ? e.work2[a] : e.work3[a];
                 acc += align_items(w.cnt[a]);
             }
             w.start[d] = acc;
@@ -303,6 +304,7 @@ class ConvectiveFluxReconstructorGPU {
    private:
     struct Extents {
         int64_t vol4, vol2, vol0, mid[3], face[3];
+        int64_t work2[3], work3[3];  // items of the midpoint-flux and face-flux stages per axis
     };
 
     struct TablePlan {
@@ -338,6 +340,11 @@ class ConvectiveFluxReconstructorGPU {
             }
             e.mid[a] = m;
             e.face[a] = f;
+            // The single-species model's axis 0 runs the fused midpoint and
+            // face flux in stage 2 (no stage-3 work on that axis).
+            const int nn[3] = {int(n[0]), int(n[1]), int(n[2])};
+            e.work2[a] = (fused_ && a == 0) ? fused_items_axis0(nn, d) : m;
+            e.work3[a] = (fused_ && a == 0) ? 0 : f;
         }
         return e;
     }
@@ -349,7 +356,8 @@ class ConvectiveFluxReconstructorGPU {
         L.sens = (cfg_.dim > 1) ? align_up(size_t(2) * e.vol2, kAlignDoubles) : 0;
         L.total = L.prim + L.sens;
         for (int a = 0; a < 3; a++) {
-            L.fmid[a] = align_up(size_t(neq_) * e.mid[a], kAlignDoubles);
+            // (the fused flux of axis 0 keeps its midpoint fluxes in registers)
+            L.fmid[a] = (fused_ && a == 0) ? 0 : align_up(size_t(neq_) * e.mid[a], kAlignDoubles);
             L.ustar[a] = has_advective_ ? align_up(size_t(e.mid[a]), kAlignDoubles) : 0;
             L.total += L.fmid[a] + L.ustar[a];
         }
@@ -520,8 +528,8 @@ class ConvectiveFluxReconstructorGPU {
             };
             add(0, 0, e.vol4);
             if (d > 1) add(1, 0, e.vol2);
-            for (int a = 0; a < d; a++) add(2, a, e.mid[a]);
-            for (int a = 0; a < d; a++) add(3, a, e.face[a]);
+            for (int a = 0; a < d; a++) add(2, a, e.work2[a]);
+            for (int a = 0; a < d; a++) add(3, a, e.work3[a]);
             if (has_advective_) add(4, 0, e.vol0);
         }
         for (int st = 0; st < kStages; st++) start[st].push_back(acc[st]);
@@ -622,6 +630,7 @@ class ConvectiveFluxReconstructorGPU {
     hm_config cfg_;
     int neq_ = 0;
     bool has_advective_ = false;
+    bool fused_ = false;  // single-species model: axis 0's midpoint and face flux in one kernel (stage 2)
     KParams k_{};
     double* arena_ = nullptr;
     size_t arena_doubles_ = 0;
