@@ -36,16 +36,25 @@ struct KParams {
     int64_t qoff;
 };
 
-// Fused midpoint and face flux of the single-species model along axis 0
-// (k_flux_single): a wavefront computes kChunk + 2 consecutive midpoint faces
-// and outputs kChunk face fluxes.
+// Fused midpoint and face flux of the single-species model (k_flux_single):
+// a wavefront computes kChunk + 2 consecutive midpoint faces along the axis
+// and outputs kChunk face fluxes.  HM_FUSED_AXES: bit a set = axis a fused
+// (else midpoint fluxes go through memory to the face-flux kernel).  Only axis
+// 0 is fused by default: on axes 1 and 2 the lanes of a fused wavefront lie
+// along the sweep, one memory row apart, and their loads do not coalesce (the
+// 2048^2 RHS's stage 2 took 0.90 ms with all axes fused against 0.42 ms with
+// axis 0 alone).
 constexpr int kChunk = 62;
+#ifndef HM_FUSED_AXES
+#define HM_FUSED_AXES 1
+#endif
 
-// Work items of the axis-0 segment of one patch in the fused kernel (whole
-// wavefronts): n0 + 3 midpoint faces per row, rows of the other axes.
-__host__ __device__ inline int64_t fused_items_axis0(const int* n, int dim) {
-    int64_t mid = n[0] + 3;
-    for (int b = 1; b < dim; b++) mid *= n[b];
+__host__ __device__ inline bool fused_axis(int axis) { return (HM_FUSED_AXES >> axis) & 1; }
+
+// Work items of the fused axis-a segment of one patch (whole wavefronts).
+__host__ __device__ inline int64_t fused_items(int axis, const int* n, int dim) {
+    int64_t mid = 1;
+    for (int b = 0; b < dim; b++) mid *= (b == axis) ? n[b] + 3 : n[b];
     return int64_t(64) * ((mid - 3) / kChunk + 1);
 }
 

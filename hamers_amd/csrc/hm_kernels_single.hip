@@ -207,35 +207,43 @@ __device__ __forceinline__ void store_face_flux(double* out, int64_t fv, int64_t
                                  double(23) / double(15) * mM[e]);
 }
 
-template <int D, int SCHEME, int PP, int QQ, bool REC>
-__device__ __forceinline__ void fused_axis0(const DevPatch& P, int64_t loc, const KParams& k) {
+template <int D, int A, int SCHEME, int PP, int QQ, bool REC>
+__device__ __forceinline__ void fused_chunk(const DevPatch& P, int64_t loc, const KParams& k) {
     constexpr int NEQ = D + 2;
     constexpr int IP = D + 1;
     const Boxes<D> B(P);
     const uint32_t lane = uint32_t(loc) & 63u;
-    const uint32_t M = (uint32_t(loc) >> 6) * kChunk + lane;  // midpoint face, flattened over the patch's rows
-    const uint32_t mv = uint32_t(B.mid_vol(0));
+    const uint32_t M = (uint32_t(loc) >> 6) * kChunk + lane;  // midpoint face, flattened with axis A fastest
+    const uint32_t mv = uint32_t(B.mid_vol(A));
     const bool valid = M < mv;
-    const uint32_t m0 = uint32_t(B.mid_ext(0, 0)), m1 = uint32_t(B.mid_ext(0, 1));
+    const uint32_t mA = uint32_t(B.n[A] + 3);
     const uint32_t Mc = valid ? M : mv - 1;  // lanes past the end compute a real face and drop it
-    const uint32_t r = Mc / m0;
-    const int cm = int(Mc - r * m0);  // position in the row: face between cells cm-2 and cm-1
-    const int c0 = cm - 1;
-    const int c1 = (D > 1) ? int((D > 2) ? r % m1 : r) : 0;
-    const int c2 = (D > 2) ? int(r / m1) : 0;
+    const uint32_t r = Mc / mA;
+    const int cm = int(Mc - r * mA);  // position along A: face between cells cm-2 and cm-1
+    // the other axes from r (x first)
+    constexpr int T0 = (A == 0) ? 1 : 0, T1 = (A == 2) ? 1 : 2;
+    const uint32_t nT0 = uint32_t(B.n[T0]);
+    const int t0 = (D > 1) ? int((D > 2) ? r % nT0 : r) : 0;
+    const int t1 = (D > 2) ? int(r / nT0) : 0;
+    int c[3] = {0, 0, 0};
+    c[A] = cm - 1;
+    if constexpr (D > 1) c[T0] = t0;
+    if constexpr (D > 2) c[T1] = t1;
+    const int c0 = c[0], c1 = c[1], c2 = c[2];
 
     const int64_t v4 = B.vol(kGc);
+    const uint32_t st = (A == 0) ? 1u : (A == 1) ? uint32_t(B.ext(0, kGc)) : uint32_t(B.ext(0, kGc) * B.ext(1, kGc));
     const uint32_t iR = uint32_t(B.idx(kGc, c0, c1, c2));
     const double* pr = P.prim;
     double V[6][NEQ];
 #pragma unroll
     for (int m = 0; m < 6; m++)
 #pragma unroll
-        for (int e = 0; e < NEQ; e++) V[m][e] = ldg(pr + e * v4, iR + uint32_t(m - 3));
-    const double cL = ldg(pr + NEQ * v4, iR - 1u);
+        for (int e = 0; e < NEQ; e++) V[m][e] = ldg(pr + e * v4, iR + uint32_t(m - 3) * st);
+    const double cL = ldg(pr + NEQ * v4, iR - st);
     const double cR = ldg(pr + NEQ * v4, iR);
     double mM[NEQ];
-    midpoint_flux_at<D, 0, SCHEME, PP, QQ, REC>(P, B, k, V, cL, cR, c0, c1, c2, mM);
+    midpoint_flux_at<D, A, SCHEME, PP, QQ, REC>(P, B, k, V, cL, cR, c0, c1, c2, mM);
 
     // Neighbours' midpoint fluxes (every lane takes part in the shuffles).
     double mL[NEQ], mR[NEQ];
@@ -244,16 +252,17 @@ __device__ __forceinline__ void fused_axis0(const DevPatch& P, int64_t loc, cons
         mL[e] = __shfl_up(mM[e], 1);
         mR[e] = __shfl_down(mM[e], 1);
     }
-    // Output face f = c0 of the row, for faces 0..n0 held by lanes 1..62.
-    if (!valid || lane == 0u || lane > uint32_t(kChunk) || cm < 1 || cm > B.n[0] + 1) return;
-    const int64_t qR = q_idx<D>(P, c0, c1, c2), qL = qR - 1;
+    // Output face f = c[A] of the line, for faces 0..n_A held by lanes 1..62.
+    if (!valid || lane == 0u || lane > uint32_t(kChunk) || cm < 1 || cm > B.n[A] + 1) return;
+    const int64_t qst = (A == 0) ? 1 : (A == 1) ? (P.n[0] + 2 * P.gq[0]) : int64_t(P.n[0] + 2 * P.gq[0]) * (P.n[1] + 2 * P.gq[1]);
+    const int64_t qR = q_idx<D>(P, c0, c1, c2), qL = qR - qst;
     double nL[NEQ], nR[NEQ];
 #pragma unroll
     for (int e = 0; e < NEQ; e++) {
-        nL[e] = node_flux_single<D, 0>(P, qL, V[2][1], V[2][IP], e);
-        nR[e] = node_flux_single<D, 0>(P, qR, V[3][1], V[3][IP], e);
+        nL[e] = node_flux_single<D, A>(P, qL, V[2][1 + A], V[2][IP], e);
+        nR[e] = node_flux_single<D, A>(P, qR, V[3][1 + A], V[3][IP], e);
     }
-    store_face_flux<D>(P.F[0], B.face_vol(0), B.face_idx(0, c0, c1, c2), k.dt, mL, mM, mR, nL, nR);
+    store_face_flux<D>(P.F[A], B.face_vol(A), B.face_idx(A, c0, c1, c2), k.dt, mL, mM, mR, nL, nR);
 }
 
 
@@ -269,12 +278,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SCHEME =
     int64_t loc;
     if (!w.find(gid, P, ax, loc)) return;
     if (ax == 0) {
-        fused_axis0<D, SCHEME, PP, QQ, REC>(*P, loc, k);
+        fused_chunk<D, 0, SCHEME, PP, QQ, REC>(*P, loc, k);
     } else if constexpr (D > 1) {
         if (ax == 1) {
-            midflux_single_face<D, 1, SCHEME, PP, QQ, REC>(*P, loc, k);
+            if constexpr (HM_FUSED_AXES & 2)
+                fused_chunk<D, 1, SCHEME, PP, QQ, REC>(*P, loc, k);
+            else
+                midflux_single_face<D, 1, SCHEME, PP, QQ, REC>(*P, loc, k);
         } else if constexpr (D > 2) {
-            midflux_single_face<D, 2, SCHEME, PP, QQ, REC>(*P, loc, k);
+            if constexpr (HM_FUSED_AXES & 4)
+                fused_chunk<D, 2, SCHEME, PP, QQ, REC>(*P, loc, k);
+            else
+                midflux_single_face<D, 2, SCHEME, PP, QQ, REC>(*P, loc, k);
         }
     }
 }

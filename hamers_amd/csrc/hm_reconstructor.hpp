@@ -340,11 +340,12 @@ class ConvectiveFluxReconstructorGPU {
             }
             e.mid[a] = m;
             e.face[a] = f;
-            // The single-species model's axis 0 runs the fused midpoint and
-            // face flux in stage 2 (no stage-3 work on that axis).
+            // The single-species model's fused axes run the midpoint and face
+            // flux in stage 2 (no stage-3 work on them).
             const int nn[3] = {int(n[0]), int(n[1]), int(n[2])};
-            e.work2[a] = (fused_ && a == 0) ? fused_items_axis0(nn, d) : m;
-            e.work3[a] = (fused_ && a == 0) ? 0 : f;
+            const bool fa = fused_ && fused_axis(a);
+            e.work2[a] = fa ? fused_items(a, nn, d) : m;
+            e.work3[a] = fa ? 0 : f;
         }
         return e;
     }
@@ -357,7 +358,7 @@ class ConvectiveFluxReconstructorGPU {
         L.total = L.prim + L.sens;
         for (int a = 0; a < 3; a++) {
             // (the fused flux of axis 0 keeps its midpoint fluxes in registers)
-            L.fmid[a] = (fused_ && a == 0) ? 0 : align_up(size_t(neq_) * e.mid[a], kAlignDoubles);
+            L.fmid[a] = (fused_ && fused_axis(a)) ? 0 : align_up(size_t(neq_) * e.mid[a], kAlignDoubles);
             L.ustar[a] = has_advective_ ? align_up(size_t(e.mid[a]), kAlignDoubles) : 0;
             L.total += L.fmid[a] + L.ustar[a];
         }
