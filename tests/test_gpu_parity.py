@@ -229,6 +229,47 @@ def test_batched_uneven_patches_match_per_patch_and_oracle(torch_dev, dim):
         compare(f"batched patch {i} {sizes[i][:dim]}", batched[i], refs[i][0])
 
 
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_tiny_and_odd_extents(torch_dev, dim):
+    """Patches of 1..7 cells per axis (batched and per patch): the axis-0
+    fused flux packs rows of n0+3 midpoint faces into wavefronts of 64 with
+    62 output lanes, so short rows, rows split between wavefronts and the
+    last partial wavefront all occur."""
+    torch = torch_dev
+    g = 4
+    sizes = {1: [(1,), (2,), (3,), (5,), (61,), (62,), (63,), (124,)],
+             2: [(1, 1), (1, 5), (2, 3), (3, 7), (7, 2), (59, 3), (60, 1)],
+             3: [(1, 1, 1), (1, 2, 3), (3, 1, 2), (2, 5, 1), (7, 3, 2)]}[dim]
+    rec = _rec(dim, 1)
+    pds, refs = [], []
+    for i, n in enumerate(sizes):
+        Q = _random_smooth_3d(n, g, seed=40 + i) if dim == 3 else None
+        if dim < 3:
+            rng = np.random.default_rng(100 + i)
+            shp = tuple(reversed([v + 2 * g for v in n]))
+            rho = 10 ** rng.uniform(-1, 0, shp)
+            p = 10 ** rng.uniform(-1, 0, shp)
+            vel = [rng.normal(size=shp) for _ in range(dim)]
+            Q = W.conservative_from_primitive_single(rho, vel, p, 1.4)
+        dx = tuple(1.0 / (v + 1) for v in n)
+        pds.append(_patchdata(torch, Q, n, g, dx, rec.num_eqn))
+        refs.append(O.rhs_patch(O.Params(dim=dim, scheme=1), Q, n, g, dx, 1e-3))
+    _say(f"case tiny and odd extents dim={dim} ({len(sizes)} patches)")
+    rec.computeConvectiveFluxAndSourceOnPatches(pds, 0.0, 1e-3)
+    torch.cuda.synchronize()
+    batched = [[f.cpu().numpy().copy() for f in pd.F] for pd in pds]
+    for pd in pds:
+        for f in pd.F:
+            f.fill_(np.nan)
+        rec.computeConvectiveFluxAndSourceOnPatch(pd, 0.0, 1e-3)
+    torch.cuda.synchronize()
+    for i, pd in enumerate(pds):
+        single = [f.cpu().numpy() for f in pd.F]
+        for a in range(dim):
+            assert np.array_equal(batched[i][a], single[a]), f"patch {i} axis {a}: batched != per-patch"
+        compare(f"tiny patch {i} {sizes[i]}", batched[i], refs[i][0])
+
+
 def test_invalid_ghost_width_is_an_error_not_a_fault(torch_dev):
     from hamers_amd.capi import HamersError
 
