@@ -41,7 +41,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
-MIDFLUX_TRAFFIC_2048_WCNS5 = 836.6e6 + 274.7e6  # bytes per k_midflux launch, 2048^2 / 128^2 patches, WCNS5-JS
+# Kernels of the single-species pipeline's two flux stages.
+STAGE_KERNEL = {"midflux": "k_flux_single", "faceflux": "k_flux_tile_single"}
+# DRAM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0) of each flux
+# kernel, 2048^2 in 128^2 patches, WCNS5-JS.
+FLUX_TRAFFIC_2048_WCNS5 = {"midflux": 381.95e6 + 136.40e6, "faceflux": 637.54e6 + 135.27e6}
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector
 
 
@@ -447,14 +451,19 @@ def main():
     # Algorithmic (compulsory) HBM bytes per cell per RHS: read Q (neq), write
     # F on d axes (d*neq): 8*(4 + 2*4) = 96 B for 2D single-species.
     bytes_per_cell = 8 * (neq + 2 * neq)
-    achieved_gbs = bytes_per_cell * cells / (stage_avg[dominant] * 1e-3) / 1e9
+    # Each flux kernel covers one axis here (axis 0, axis 1): its compulsory
+    # bytes are Q read for the node fluxes and F of its axis written,
+    # 8*(4 + 4) = 64 B per cell per launch.
+    kernel_bytes_per_cell = 8 * (neq + neq)
+    achieved_gbs = kernel_bytes_per_cell * cells / (stage_avg[dominant] * 1e-3) / 1e9
     # DRAM bytes per launch of the dominant kernel, measured with rocprofv3
     # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a copy of known size)
     # for the default workload only; other sizes report null.
     traffic, traffic_source = None, None
-    if (args.N, args.patch, args.scheme, dominant) == (2048, 128, "WCNS5_JS_HLLC_HLL", "midflux"):
-        traffic = MIDFLUX_TRAFFIC_2048_WCNS5
-        traffic_source = "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/k_midflux_single_2048sq_counters.md"
+    if (args.N, args.patch, args.scheme) == (2048, 128, "WCNS5_JS_HLLC_HLL") and \
+            FLUX_TRAFFIC_2048_WCNS5.get(dominant) is not None:
+        traffic = FLUX_TRAFFIC_2048_WCNS5[dominant]
+        traffic_source = "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/flux_kernels_2048sq_counters.md"
     rhs_gbs = bytes_per_cell * cells / (ms_batched * 1e-3) / 1e9
     value = cells / (ms_batched * 1e-3)
 
@@ -517,8 +526,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_source,
-                     "kernel": f"k_{dominant}", "kernel_ms": stage_avg[dominant],
-                     "bytes_per_cell": bytes_per_cell},
+                     "kernel": STAGE_KERNEL.get(dominant, dominant), "kernel_ms": stage_avg[dominant],
+                     "bytes_per_cell": kernel_bytes_per_cell, "rhs_bytes_per_cell": bytes_per_cell},
         "rhs_hbm_frac": rhs_gbs / HBM_PEAK_GBS,
         "stage_ms": stage_avg,
         "per_patch": {"ms_per_step": ms_pp, "value": (cells / (ms_pp * 1e-3)) if ms_pp else None,

@@ -260,7 +260,7 @@ HM_HD void reconstruct_pair_adj(const Six& s, const LDConst& k, double umb, doub
 }
 
 // Adjoint of the single-species HLLC / HLLC-HLL midpoint flux of one face
-// along axis A (forward: midflux_single_face).  Fb -> VLb, VRb (accumulated).
+// along axis A (forward: midpoint_flux_at).  Fb -> VLb, VRb (accumulated).
 template <int D, int A>
 HM_HD void riemann_single_adj(const double* VL, const double* VR, double gam, bool use_hll, const double* Fb,
                               double* VLb, double* VRb) {

@@ -36,20 +36,42 @@ struct KParams {
     int64_t qoff;
 };
 
-// Fused midpoint and face flux of the single-species model (k_flux_single):
-// a wavefront computes kChunk + 2 consecutive midpoint faces along the axis
-// and outputs kChunk face fluxes.  HM_FUSED_AXES: bit a set = axis a fused
-// (else midpoint fluxes go through memory to the face-flux kernel).  Only axis
-// 0 is fused by default: on axes 1 and 2 the lanes of a fused wavefront lie
-// along the sweep, one memory row apart, and their loads do not coalesce (the
-// 2048^2 RHS's stage 2 took 0.90 ms with all axes fused against 0.42 ms with
-// axis 0 alone).
+// Fused midpoint and face flux of the single-species model on axis 0
+// (k_flux_single): a wavefront computes kChunk + 2 consecutive midpoint faces
+// along the axis and outputs kChunk face fluxes.
 constexpr int kChunk = 62;
-#ifndef HM_FUSED_AXES
-#define HM_FUSED_AXES 1
+
+// Fused midpoint and face flux of the single-species model on axes 1 and 2
+// (k_flux_tile_single): a 256-thread block owns a tile of 64 x-positions by
+// R midpoint-face rows along the axis (each wavefront R/4 rows), keeps the
+// tile's midpoint fluxes in LDS and outputs the R - 2 inner rows.  R is chosen
+// per call from the work it holds (tile_rows_for): 24 rows recompute the
+// fewest shared rows, 8 and 4 give small calls (a lone patch, a small level)
+// enough blocks and short wavefront latency.
+constexpr int kTileX = 64;
+#ifndef HM_TILE_WAVES
+#define HM_TILE_WAVES 3  // wavefronts per SIMD asked of the 2D WCNS5-JS build
 #endif
 
-__host__ __device__ inline bool fused_axis(int axis) { return (HM_FUSED_AXES >> axis) & 1; }
+// Work items of the tiled axis-a (1 or 2) segment of one patch, R rows per
+// tile: 256 per tile.
+__host__ __device__ inline int64_t tile_items(int axis, const int* n, int dim, int rows) {
+    int64_t trans = 1;  // the other transverse axis (3D)
+    for (int b = 1; b < dim; b++)
+        if (b != axis) trans *= n[b];
+    const int64_t xt = (n[0] + kTileX - 1) / kTileX;
+    const int64_t jt = (n[axis] + 1 + (rows - 2) - 1) / (rows - 2);
+    return int64_t(256) * xt * trans * jt;
+}
+
+// Tile rows of a call whose 24-row tiles number blocks24 and whose 8-row
+// tiles number blocks8 (sums over its patches and axes 1, 2): 24 when that
+// gives at least 8 blocks per CU of the 256, else 8 when that gives 2, else 4.
+inline int tile_rows_for(int64_t blocks24, int64_t blocks8) {
+    if (blocks24 >= 2048) return 24;
+    if (blocks8 >= 512) return 8;
+    return 4;
+}
 
 // Work items of the fused axis-a segment of one patch (whole wavefronts).
 __host__ __device__ inline int64_t fused_items(int axis, const int* n, int dim) {
@@ -68,6 +90,7 @@ template <class W>
 struct Plan {
     W work[kStages];
     int64_t items[kStages];
+    int tile_rows;  // rows of the tiled flux stage (single-species model): 4, 8 or 24
 };
 
 template <class W>

@@ -2,16 +2,16 @@
 //
 // Pipeline per RHS evaluation (each stage is one launch over all work items
 // of all patches handed to it):
-//   k_derived   cells of the 4-ghost box: V = (rho, u.., p) and c
-//               (FlowModelSingleSpecies.cpp:2649-3179)
-//   k_sensor    cells of the 2-ghost box: dilatation and vorticity magnitude
-//               (WCNS56-HLLC-HLL.cpp:626-731 (2D), 1470-1656 (3D))
-//   k_midflux   every midpoint face of every axis: characteristic projection,
-//               WCNS interpolation, back-projection, positivity fallback,
-//               HLLC / HLLC-HLL and sensor select, fused in registers
-//               (WCNS56-HLLC-HLL.cpp:879-1194, 1746-2320)
-//   k_faceflux  every face: sixth-order midpoint-to-face flux times dt
-//               (WCNS56-HLLC-HLL.cpp:1197-1282, 2327-2490)
+//   k_derived_single    cells of the 4-ghost box: V = (rho, u.., p) and c
+//                       (FlowModelSingleSpecies.cpp:2649-3179)
+//   k_sensor            cells of the 2-ghost box: dilatation and vorticity
+//                       magnitude (WCNS56-HLLC-HLL.cpp:626-731 (2D), 1470-1656 (3D))
+//   k_flux_single       axis 0: midpoint fluxes (characteristic projection, WCNS
+//                       interpolation, back-projection, positivity fallback,
+//                       HLLC / HLLC-HLL and sensor select; WCNS56-HLLC-HLL.cpp:
+//                       879-1194, 1746-2320) and the sixth-order midpoint-to-face
+//                       flux times dt (1197-1282, 2327-2490), fused per wavefront
+//   k_flux_tile_single  axes 1 and 2: the same, fused per 64 x 24 tile
 #include <type_traits>
 
 #include "hm_kernels.hpp"
@@ -68,76 +68,6 @@ __global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
 }
 
 // ----------------------------------------------------------------------------
-// Fused midpoint flux of one face (single-species).
-// ----------------------------------------------------------------------------
-template <int D, int A, int SCHEME, int PP, int QQ, bool REC>
-__device__ __forceinline__ void midflux_single_face(const DevPatch& P, int64_t loc, const KParams& k) {
-    constexpr int NEQ = D + 2;
-    const Boxes<D> B(P);
-    const int m0 = B.mid_ext(A, 0), m1 = B.mid_ext(A, 1);
-    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
-    const uint32_t r = uint32_t(loc) / uint32_t(m0);
-    int c0 = int(uint32_t(loc) - r * uint32_t(m0));
-    int c1 = (D > 1) ? int((D > 2) ? r % uint32_t(m1) : r) : 0;
-    int c2 = (D > 2) ? int(r / uint32_t(m1)) : 0;
-    if constexpr (A == 0) c0 -= 1;
-    if constexpr (A == 1) c1 -= 1;
-    if constexpr (A == 2) c2 -= 1;
-
-    const int64_t v4 = B.vol(kGc);
-    const int64_t st = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGc)) : int64_t(B.ext(0, kGc)) * B.ext(1, kGc);
-    const int64_t iR = B.idx(kGc, c0, c1, c2);  // cell i (right of the face); cell i-1 is iR - st
-    const double* pr = P.prim;
-
-    // Stencil of primitive variables, cells i-3 .. i+2.
-    const uint32_t i32 = uint32_t(iR), st32 = uint32_t(st);
-    double V[6][NEQ];
-#pragma unroll
-    for (int m = 0; m < 6; m++)
-#pragma unroll
-        for (int e = 0; e < NEQ; e++) V[m][e] = ldg(pr + e * v4, i32 + uint32_t(m - 3) * st32);
-    const double cL = ldg(pr + NEQ * v4, i32 - st32);
-    const double cR = ldg(pr + NEQ * v4, i32);
-
-    // Sensor (WCNS56-HLLC-HLL.cpp:1066-1113): only when it exceeds 0.65 is the
-    // HLLC-HLL flux used, so HLLC-HLL is evaluated only on those faces.
-    bool use_hll = false;
-    if constexpr (D > 1) {
-        const int64_t vs = B.vol(kGs);
-        const int64_t tR = B.idx(kGs, c0, c1, c2);
-        const int64_t sts = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGs)) : int64_t(B.ext(0, kGs)) * B.ext(1, kGs);
-        const int64_t tL = tR - sts;
-        const double th_avg = 0.5 * (P.sens[tL] + P.sens[tR]);
-        const double om_avg = 0.5 * (P.sens[vs + tL] + P.sens[vs + tR]);
-        const double sensor = div_rn(-th_avg, fabs(th_avg) + om_avg + kEps);
-        use_hll = sensor > 0.65;
-    }
-
-    double Fo[NEQ];
-    const int64_t mv = B.mid_vol(A);
-    if constexpr (REC) {
-        // Recording pass: keep W-, W+ in the face-state slot.
-        double* wr = k.wrec + P.wrec[A] + loc;
-        face_flux_single<D, A, SCHEME, PP, QQ>(V, cL, cR, k.gamma[0], LDConst{k.p, k.q, k.C, k.alpha_tau}, use_hll,
-                                               Fo, [&](const double* Wm, const double* Wp) {
-#pragma unroll
-                                                   for (int c = 0; c < NEQ; c++) {
-                                                       wr[c * mv] = Wm[c];
-                                                       wr[(NEQ + c) * mv] = Wp[c];
-                                                   }
-                                               });
-    } else {
-        face_flux_single<D, A, SCHEME, PP, QQ>(V, cL, cR, k.gamma[0], LDConst{k.p, k.q, k.C, k.alpha_tau}, use_hll,
-                                               Fo);
-    }
-
-    double* out = P.fmid[A];
-#pragma unroll
-    for (int e = 0; e < NEQ; e++) out[e * mv + loc] = Fo[e];
-}
-
-
-// ----------------------------------------------------------------------------
 // Fused midpoint and face flux along axis 0 (single-species): a wavefront
 // computes kChunk + 2 consecutive midpoint faces of the patch's flattened
 // face list (rows of faces -1 .. n+1, unit stride in memory), kChunk apart
@@ -145,9 +75,8 @@ __device__ __forceinline__ void midflux_single_face(const DevPatch& P, int64_t l
 // with their neighbours' midpoint fluxes (cross-lane shuffles) and the node
 // fluxes of their two cells (WCNS56-HLLC-HLL.cpp:1197-1282, 2327-2490), so
 // the midpoint flux of axis 0 never goes to memory.  A row is complete in the
-// list, so the neighbours of an output face are in its row.  On axes 1 and 2
-// the lanes of a wavefront lie across the sweep (coalesced loads); there the
-// midpoint flux goes through memory to the face-flux kernel.
+// list, so the neighbours of an output face are in its row.  Axes 1 and 2 run
+// in tiles whose lanes lie across the sweep (coalesced loads), below.
 // ----------------------------------------------------------------------------
 
 // Midpoint flux of the face whose right cell is (c0, c1, c2) along A, from its
@@ -266,9 +195,8 @@ __device__ __forceinline__ void fused_chunk(const DevPatch& P, int64_t loc, cons
 }
 
 
-// Stage 2 of the single-species pipeline: axis 0 runs the fused midpoint and
-// face flux; axes 1 and 2 write midpoint fluxes for the face-flux kernel
-// (stage 3), their lanes lying across the sweep so that the loads coalesce.
+// Stage 2 of the single-species pipeline: the fused midpoint and face flux of
+// axis 0 (the stage's only segments).
 template <int D, int SCHEME, int PP, int QQ, class W, bool REC = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SCHEME == kWCNS5_JS && D == 2 ? 4 : 1))) void k_flux_single(
     W w, KParams k) {
@@ -277,93 +205,126 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SCHEME =
     int ax;
     int64_t loc;
     if (!w.find(gid, P, ax, loc)) return;
-    if (ax == 0) {
-        fused_chunk<D, 0, SCHEME, PP, QQ, REC>(*P, loc, k);
-    } else if constexpr (D > 1) {
-        if (ax == 1) {
-            if constexpr (HM_FUSED_AXES & 2)
-                fused_chunk<D, 1, SCHEME, PP, QQ, REC>(*P, loc, k);
-            else
-                midflux_single_face<D, 1, SCHEME, PP, QQ, REC>(*P, loc, k);
-        } else if constexpr (D > 2) {
-            if constexpr (HM_FUSED_AXES & 4)
-                fused_chunk<D, 2, SCHEME, PP, QQ, REC>(*P, loc, k);
-            else
-                midflux_single_face<D, 2, SCHEME, PP, QQ, REC>(*P, loc, k);
-        }
-    }
+    if (ax == 0) fused_chunk<D, 0, SCHEME, PP, QQ, REC>(*P, loc, k);
 }
 
-
-// ----------------------------------------------------------------------------
-// Midpoint-to-face flux (sixth order), times dt.
-// ----------------------------------------------------------------------------
-template <int D, int A>
-__device__ __forceinline__ void faceflux_single_face(const DevPatch& P, int64_t loc, const KParams& k) {
+// Axes 1 and 2 of the fused midpoint and face flux, one tile per 256-thread
+// block: 64 x-positions (threads tx) by R midpoint faces along A (wavefront ty
+// takes rows ty, ty+4, ty+8, ...), coalesced along x.  The tile's midpoint
+// fluxes go to LDS; after a barrier the inner R - 2 rows become output faces (1/30, 3/10, 23/15 combination with the node fluxes
+// of their cells).  Adjacent tiles share two midpoint rows, computed by both.
+// The stage's segments hold whole tiles (256 items each; every segment of the
+// stage is a tile segment, so each starts on a multiple of 256) and every
+// block lies in one tile: the barrier is reached by all four wavefronts.
+template <int D, int A, int R, int SCHEME, int PP, int QQ, bool REC>
+__device__ __forceinline__ void fused_tile(const DevPatch& P, int64_t loc, const KParams& k, double* lds) {
     constexpr int NEQ = D + 2;
     constexpr int IP = D + 1;
+    constexpr int TB = (A == 1) ? 2 : 1;  // the other transverse axis (3D)
     const Boxes<D> B(P);
-    const int f0 = B.face_ext(A, 0), f1 = B.face_ext(A, 1);
-    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
-    const uint32_t r = uint32_t(loc) / uint32_t(f0);
-    const int c0 = int(uint32_t(loc) - r * uint32_t(f0));
-    const int c1 = (D > 1) ? int((D > 2) ? r % uint32_t(f1) : r) : 0;
-    const int c2 = (D > 2) ? int(r / uint32_t(f1)) : 0;
-
-    const int64_t mv = B.mid_vol(A);
-    const int64_t mC = B.mid_idx(A, c0, c1, c2);
-    const int64_t mst = (A == 0) ? 1 : (A == 1) ? int64_t(B.mid_ext(A, 0)) : int64_t(B.mid_ext(A, 0)) * B.mid_ext(A, 1);
-
-    // Node flux at cells i-1 and i (FlowModelSingleSpecies.cpp:3329-3331, 3388-3391, 3470-3474).
+    const uint32_t tile = uint32_t(loc) >> 8, tid = uint32_t(loc) & 255u;
+    const int tx = int(tid & 63u), ty = int(tid >> 6);
+    const uint32_t nxt = uint32_t(B.n[0] + kTileX - 1) / uint32_t(kTileX);
+    const uint32_t nt = (D > 2) ? uint32_t(B.n[TB]) : 1u;
+    const uint32_t xt = tile % nxt, r = tile / nxt;
+    const int t = (D > 2) ? int(r % nt) : 0;
+    const int jt = int(r / nt);
+    const int x = int(xt) * kTileX + tx;
+    const bool vx = x < B.n[0];
+    const int xc = vx ? x : B.n[0] - 1;  // lanes past the patch edge compute a real face and drop it
+    const int nA = B.n[A];
+    const int j0 = jt * (R - 2) - 1;  // midpoint face of tile row 0
+    auto cell = [&](int j, int& c0, int& c1, int& c2) {
+        c0 = xc;
+        c1 = (A == 1) ? j : t;
+        c2 = (A == 2) ? j : ((D > 2) ? t : 0);
+    };
     const int64_t v4 = B.vol(kGc);
-    const int64_t st4 = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGc)) : int64_t(B.ext(0, kGc)) * B.ext(1, kGc);
-    const int64_t pR = B.idx(kGc, c0, c1, c2), pL = pR - st4;
-    const int qst = (A == 0) ? 1 : (A == 1) ? (P.n[0] + 2 * P.gq[0]) : (P.n[0] + 2 * P.gq[0]) * (P.n[1] + 2 * P.gq[1]);
-    const int64_t qR = q_idx<D>(P, c0, c1, c2), qL = qR - qst;
-    const double uaL = P.prim[(1 + A) * v4 + pL], uaR = P.prim[(1 + A) * v4 + pR];
-    const double prL = P.prim[IP * v4 + pL], prR = P.prim[IP * v4 + pR];
-
-    const int64_t fv = B.face_vol(A);
-    const double* fm = P.fmid[A];
-    double* out = P.F[A];
-    const double dt = k.dt;
+    const uint32_t st = (A == 1) ? uint32_t(B.ext(0, kGc)) : uint32_t(B.ext(0, kGc) * B.ext(1, kGc));
+    const double* pr = P.prim;
+#pragma unroll 1
+    for (int q = 0; q < R / 4; q++) {
+        const int row = ty + 4 * q;
+        const int j = j0 + row;
+        if (j > nA + 1) break;  // past the last midpoint face (uniform per wavefront: one row each)
+        int c0, c1, c2;
+        cell(j, c0, c1, c2);
+        const uint32_t iR = uint32_t(B.idx(kGc, c0, c1, c2));
+        double V[6][NEQ];
 #pragma unroll
-    for (int e = 0; e < NEQ; e++) {
-        double nL, nR;
-        if (e == 0) {
-            nL = P.Q[1 + A][qL];
-            nR = P.Q[1 + A][qR];
-        } else if (e == IP) {
-            nL = uaL * (P.Q[IP][qL] + prL);
-            nR = uaR * (P.Q[IP][qR] + prR);
-        } else if (e == 1 + A) {
-            nL = uaL * P.Q[e][qL] + prL;
-            nR = uaR * P.Q[e][qR] + prR;
-        } else {
-            nL = uaL * P.Q[e][qL];
-            nR = uaR * P.Q[e][qR];
+        for (int m = 0; m < 6; m++)
+#pragma unroll
+            for (int e = 0; e < NEQ; e++) V[m][e] = ldg(pr + e * v4, iR + uint32_t(m - 3) * st);
+        const double cL = ldg(pr + NEQ * v4, iR - st), cR = ldg(pr + NEQ * v4, iR);
+        double m[NEQ];
+        midpoint_flux_at<D, A, SCHEME, PP, QQ, REC>(P, B, k, V, cL, cR, c0, c1, c2, m);
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) lds[(e * R + row) * kTileX + tx] = m[e];
+    }
+    __syncthreads();
+    const int64_t qst = (A == 1) ? (P.n[0] + 2 * P.gq[0]) : int64_t(P.n[0] + 2 * P.gq[0]) * (P.n[1] + 2 * P.gq[1]);
+#pragma unroll 1
+    for (int q = 0; q < R / 4; q++) {
+        const int row = ty + 4 * q;
+        const int f = j0 + row;  // output face of this row
+        if (row < 1 || row > R - 2 || !vx || f > nA) continue;
+        int c0, c1, c2;
+        cell(f, c0, c1, c2);
+        const uint32_t pR = uint32_t(B.idx(kGc, c0, c1, c2)), pL = pR - st;
+        const int64_t qR = q_idx<D>(P, c0, c1, c2), qL = qR - qst;
+        const double uaL = ldg(pr + (1 + A) * v4, pL), uaR = ldg(pr + (1 + A) * v4, pR);
+        const double prL = ldg(pr + IP * v4, pL), prR = ldg(pr + IP * v4, pR);
+        double mL[NEQ], mM[NEQ], mR[NEQ], nL[NEQ], nR[NEQ];
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) {
+            mL[e] = lds[(e * R + row - 1) * kTileX + tx];
+            mM[e] = lds[(e * R + row) * kTileX + tx];
+            mR[e] = lds[(e * R + row + 1) * kTileX + tx];
+            nL[e] = node_flux_single<D, A>(P, qL, uaL, prL, e);
+            nR[e] = node_flux_single<D, A>(P, qR, uaR, prR, e);
         }
-        const double* f = fm + e * mv;
-        const double mR = f[mC + mst], mL = f[mC - mst], mM = f[mC];
-        out[e * fv + loc] = dt * (double(1) / double(30) * (mR + mL) - double(3) / double(10) * (nR + nL) +
-                                  double(23) / double(15) * mM);
+        store_face_flux<D>(P.F[A], B.face_vol(A), B.face_idx(A, c0, c1, c2), k.dt, mL, mM, mR, nL, nR);
     }
 }
 
-template <int D, class W>
-__global__ __launch_bounds__(kBlock) void k_faceflux_single(W w, KParams k) {
+template <int D, int R, int SCHEME, int PP, int QQ, class W, bool REC = false>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SCHEME == kWCNS5_JS && D == 2 ? HM_TILE_WAVES : 1))) void k_flux_tile_single(
+    W w, KParams k) {
+    static_assert(R % 4 == 0 && R >= 4, "tile rows: a multiple of the block's four wavefronts");
+    if constexpr (D > 1) {
+        __shared__ double lds[R * kTileX * (D + 2)];
+        const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+        const DevPatch* P;
+        int ax;
+        int64_t loc;
+        if (!w.find(gid, P, ax, loc)) return;  // whole blocks only: the stage's items are whole tiles
+        if (ax == 1) {
+            fused_tile<D, 1, R, SCHEME, PP, QQ, REC>(*P, loc, k, lds);
+        } else if constexpr (D > 2) {
+            fused_tile<D, 2, R, SCHEME, PP, QQ, REC>(*P, loc, k, lds);
+        }
+    }
+}
+
+// One patch (the per-patch entry point): axis 0's chunks and the tiles of axes
+// 1 and 2 in one launch, so that the two flux stages of a small call overlap
+// instead of running one after the other.  The axis-0 segment is padded to a
+// multiple of 256 items, so a block holds either chunks or one tile.
+template <int D, int R, int SCHEME, int PP, int QQ, class W>
+__global__ __launch_bounds__(kBlock) void k_flux_patch_single(W w, KParams k) {
+    __shared__ double lds[(D > 1) ? R * kTileX * (D + 2) : 1];
     const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     const DevPatch* P;
     int ax;
     int64_t loc;
-    if (!w.find(gid, P, ax, loc)) return;
+    if (!w.find(gid, P, ax, loc)) return;  // holes lie in axis-0 blocks only
     if (ax == 0) {
-        faceflux_single_face<D, 0>(*P, loc, k);
+        fused_chunk<D, 0, SCHEME, PP, QQ, false>(*P, loc, k);
     } else if constexpr (D > 1) {
         if (ax == 1) {
-            faceflux_single_face<D, 1>(*P, loc, k);
+            fused_tile<D, 1, R, SCHEME, PP, QQ, false>(*P, loc, k, lds);
         } else if constexpr (D > 2) {
-            faceflux_single_face<D, 2>(*P, loc, k);
+            fused_tile<D, 2, R, SCHEME, PP, QQ, false>(*P, loc, k, lds);
         }
     }
 }
@@ -383,24 +344,59 @@ hipError_t launch_single_dsq(const Plan<W>& plan, const KParams& k, hipStream_t 
             hipLaunchKernelGGL((k_sensor<D, 1, W>), dim3(unsigned((plan.items[1] + kBlock - 1) / kBlock)),
                                dim3(kBlock), 0, s, plan.work[1], k);
     }
-    // Stage 2: fused flux on axis 0, midpoint fluxes on axes 1 and 2; stage 3:
-    // the face flux of axes 1 and 2.
-    bool rec = false;
-    if constexpr (std::is_same_v<W, SegTable>) rec = k.wrec != nullptr;  // recording runs use patch tables
-    if (plan.items[2] > 0) {
-        const dim3 grid(unsigned((plan.items[2] + kBlock - 1) / kBlock));
-        if (rec) {
-            if constexpr (std::is_same_v<W, SegTable>)
-                hipLaunchKernelGGL((k_flux_single<D, SCHEME, PP, QQ, W, true>), grid, dim3(kBlock), 0, s, plan.work[2],
-                                   k);
-        } else {
-            hipLaunchKernelGGL((k_flux_single<D, SCHEME, PP, QQ, W>), grid, dim3(kBlock), 0, s, plan.work[2], k);
+    // Stage 2: fused flux on axis 0; stage 3: fused flux on axes 1 and 2 (tiles).
+    if constexpr (std::is_same_v<W, SegSingle>) {
+        // one patch: both flux stages in one launch (stage 2's table holds all axes)
+        if (plan.items[2] > 0) {
+            const dim3 grid(unsigned((plan.items[2] + kBlock - 1) / kBlock));
+            switch (plan.tile_rows) {
+                case 24:
+                    hipLaunchKernelGGL((k_flux_patch_single<D, 24, SCHEME, PP, QQ, W>), grid, dim3(kBlock), 0, s,
+                                       plan.work[2], k);
+                    break;
+                case 8:
+                    hipLaunchKernelGGL((k_flux_patch_single<D, 8, SCHEME, PP, QQ, W>), grid, dim3(kBlock), 0, s,
+                                       plan.work[2], k);
+                    break;
+                case 4:
+                    hipLaunchKernelGGL((k_flux_patch_single<D, 4, SCHEME, PP, QQ, W>), grid, dim3(kBlock), 0, s,
+                                       plan.work[2], k);
+                    break;
+                default: return hipErrorInvalidValue;
+            }
         }
+        return hipGetLastError();
+    } else {
+        const bool rec = k.wrec != nullptr;  // recording runs (patch tables only)
+        if (plan.items[2] > 0) {
+            const dim3 grid(unsigned((plan.items[2] + kBlock - 1) / kBlock));
+            if (rec)
+                hipLaunchKernelGGL((k_flux_single<D, SCHEME, PP, QQ, W, true>), grid, dim3(kBlock), 0, s, plan.work[2], k);
+            else
+                hipLaunchKernelGGL((k_flux_single<D, SCHEME, PP, QQ, W>), grid, dim3(kBlock), 0, s, plan.work[2], k);
+        }
+        if (plan.items[3] > 0) {
+            if constexpr (D > 1) {
+                const dim3 grid(unsigned((plan.items[3] + kBlock - 1) / kBlock));
+                auto tiles = [&](auto rows) {
+                    constexpr int R = decltype(rows)::value;
+                    if (rec)
+                        hipLaunchKernelGGL((k_flux_tile_single<D, R, SCHEME, PP, QQ, W, true>), grid, dim3(kBlock), 0, s,
+                                           plan.work[3], k);
+                    else
+                        hipLaunchKernelGGL((k_flux_tile_single<D, R, SCHEME, PP, QQ, W>), grid, dim3(kBlock), 0, s,
+                                           plan.work[3], k);
+                };
+                switch (plan.tile_rows) {
+                    case 24: tiles(std::integral_constant<int, 24>{}); break;
+                    case 8: tiles(std::integral_constant<int, 8>{}); break;
+                    case 4: tiles(std::integral_constant<int, 4>{}); break;
+                    default: return hipErrorInvalidValue;
+                }
+            }
+        }
+        return hipGetLastError();
     }
-    if (plan.items[3] > 0)
-        hipLaunchKernelGGL((k_faceflux_single<D, W>), dim3(unsigned((plan.items[3] + kBlock - 1) / kBlock)),
-                           dim3(kBlock), 0, s, plan.work[3], k);
-    return hipGetLastError();
 }
 
 template <int D, class W>

@@ -119,7 +119,8 @@ class ConvectiveFluxReconstructorGPU {
         DevPatch dp = make_dev_patch(v, arena_, L);
         Plan<SegSingle> plan{};
         const int d = cfg_.dim;
-        const Extents e = extents(v);
+        plan.tile_rows = tile_rows_of(&v, 1);
+        const Extents e = extents(v, plan.tile_rows);
         // derived cells
         plan.work[0].p = dp;
         plan.work[0].nseg = 1;
@@ -147,6 +148,21 @@ class ConvectiveFluxReconstructorGPU {
             }
             w.start[d] = acc;
             plan.items[stage] = acc;
+        }
+        if (fused_) {
+            // The single-species model runs both flux stages in one launch on
+            // one patch: stage 2 takes axis 0's chunks, padded to whole blocks,
+            // and the tiles of axes 1 and 2; stage 3 is empty.
+            SegSingle& w = plan.work[2];
+            int64_t acc = align_up(size_t(e.work2[0]), kBlock);
+            for (int a = 1; a < d; a++) {
+                w.start[a] = acc;
+                w.cnt[a] = e.work3[a];
+                acc += e.work3[a];
+            }
+            w.start[d] = acc;
+            plan.items[2] = acc;
+            plan.items[3] = 0;
         }
         // advective source cells
         plan.work[4].p = dp;
@@ -316,7 +332,24 @@ class ConvectiveFluxReconstructorGPU {
         uint64_t use = 0;
     };
 
-    Extents extents(const hm_patch_view& v) const {
+    // Tile rows of the single-species model's tiled flux stage for a call over
+    // these patches (4 when the model does not tile).
+    int tile_rows_of(const hm_patch_view* views, int np) const {
+        if (!fused_ || cfg_.dim < 2) return 4;
+        int64_t b24 = 0, b8 = 0;
+        for (int i = 0; i < np; i++) {
+            const int nn[3] = {views[i].n[0], cfg_.dim > 1 ? views[i].n[1] : 1, cfg_.dim > 2 ? views[i].n[2] : 1};
+            for (int a = 1; a < cfg_.dim; a++) {
+                b24 += tile_items(a, nn, cfg_.dim, 24) / 256;
+                b8 += tile_items(a, nn, cfg_.dim, 8) / 256;
+            }
+        }
+        return tile_rows_for(b24, b8);
+    }
+
+    // rows: the call's tile rows (the work-item counts of the tiled stage
+    // depend on it; the scratch layout does not).
+    Extents extents(const hm_patch_view& v, int rows = 24) const {
         const int d = cfg_.dim;
         Extents e{};
         int64_t n[3] = {1, 1, 1};
@@ -340,12 +373,19 @@ class ConvectiveFluxReconstructorGPU {
             }
             e.mid[a] = m;
             e.face[a] = f;
-            // The single-species model's fused axes run the midpoint and face
-            // flux in stage 2 (no stage-3 work on them).
+            // The single-species model fuses the midpoint and face flux: axis 0
+            // in stage 2, axes 1 and 2 (tiles) in stage 3.
             const int nn[3] = {int(n[0]), int(n[1]), int(n[2])};
-            const bool fa = fused_ && fused_axis(a);
-            e.work2[a] = fa ? fused_items(a, nn, d) : m;
-            e.work3[a] = fa ? 0 : f;
+            if (!fused_) {
+                e.work2[a] = m;
+                e.work3[a] = f;
+            } else if (a == 0) {
+                e.work2[a] = fused_items(0, nn, d);
+                e.work3[a] = 0;
+            } else {
+                e.work2[a] = 0;
+                e.work3[a] = tile_items(a, nn, d, rows);
+            }
         }
         return e;
     }
@@ -357,8 +397,8 @@ class ConvectiveFluxReconstructorGPU {
         L.sens = (cfg_.dim > 1) ? align_up(size_t(2) * e.vol2, kAlignDoubles) : 0;
         L.total = L.prim + L.sens;
         for (int a = 0; a < 3; a++) {
-            // (the fused flux of axis 0 keeps its midpoint fluxes in registers)
-            L.fmid[a] = (fused_ && fused_axis(a)) ? 0 : align_up(size_t(neq_) * e.mid[a], kAlignDoubles);
+            // (the fused flux keeps its midpoint fluxes in registers or LDS)
+            L.fmid[a] = fused_ ? 0 : align_up(size_t(neq_) * e.mid[a], kAlignDoubles);
             L.ustar[a] = has_advective_ ? align_up(size_t(e.mid[a]), kAlignDoubles) : 0;
             L.total += L.fmid[a] + L.ustar[a];
         }
@@ -518,8 +558,9 @@ class ConvectiveFluxReconstructorGPU {
             base += L.total;
         }
         int64_t acc[kStages] = {0, 0, 0, 0, 0};
+        plan.tile_rows = tile_rows_of(views, np);
         for (int i = 0; i < np; i++) {
-            const Extents e = extents(views[i]);
+            const Extents e = extents(views[i], plan.tile_rows);
             auto add = [&](int stage, int axis, int64_t c) {
                 if (c <= 0) return;
                 start[stage].push_back(acc[stage]);
@@ -631,7 +672,7 @@ class ConvectiveFluxReconstructorGPU {
     hm_config cfg_;
     int neq_ = 0;
     bool has_advective_ = false;
-    bool fused_ = false;  // single-species model: axis 0's midpoint and face flux in one kernel (stage 2)
+    bool fused_ = false;  // single-species model: midpoint and face flux fused (axis 0 in stage 2, axes 1, 2 in stage 3)
     KParams k_{};
     double* arena_ = nullptr;
     size_t arena_doubles_ = 0;

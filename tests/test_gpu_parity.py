@@ -270,6 +270,55 @@ def test_tiny_and_odd_extents(torch_dev, dim):
         compare(f"tiny patch {i} {sizes[i]}", batched[i], refs[i][0])
 
 
+def _tile_rows(shapes):
+    """Rows per tile of the single-species kernel's axis-1/2 stage for a call
+    over patches of these interior shapes: 24 when 24-row tiles give 2048
+    blocks, else 8 when 8-row tiles give 512, else 4."""
+    def blocks(rows):
+        b = 0
+        for n in shapes:
+            for a in range(1, len(n)):
+                trans = int(np.prod([n[c] for c in range(1, len(n)) if c != a]))
+                b += -(-n[0] // 64) * trans * -(-(n[a] + 1) // (rows - 2))
+        return b
+    return 24 if blocks(24) >= 2048 else (8 if blocks(8) >= 512 else 4)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_tile_row_bands_agree(torch_dev, dim):
+    """The axis-1/2 tiles come in 24, 8 and 4 rows by the size of the call:
+    a mid-size level runs batched on 8-row tiles and per patch on 4-row tiles;
+    both must equal each other bit for bit and the oracle."""
+    torch = torch_dev
+    g = 4
+    if dim == 2:
+        Qi, _, dx = W.four_quadrant_riemann(N=512, seed=77)
+        patches = [Qp for _, Qp in W.split_patches(Qi, (128, 128), g)]
+        shapes = [(128, 128)] * len(patches)
+    else:
+        shapes = [(40, 36, 30), (64, 20, 24), (33, 50, 17), (70, 40, 28)]
+        patches = [_random_smooth_3d(n, g, seed=300 + i) for i, n in enumerate(shapes)]
+        dx = (0.02, 0.025, 0.03)
+    assert _tile_rows(shapes) == 8 and _tile_rows(shapes[:1]) == 4
+    rec = _rec(dim, 0)
+    pds = [_patchdata(torch, Qp, n, g, dx, rec.num_eqn) for Qp, n in zip(patches, shapes)]
+    _say(f"case tile-row bands dim={dim} ({len(pds)} patches)")
+    rec.computeConvectiveFluxAndSourceOnPatches(pds, 0.0, 1e-4)
+    torch.cuda.synchronize()
+    batched = [[f.cpu().numpy().copy() for f in pd.F] for pd in pds]
+    for pd in pds:
+        for f in pd.F:
+            f.fill_(np.nan)
+        rec.computeConvectiveFluxAndSourceOnPatch(pd, 0.0, 1e-4)
+    torch.cuda.synchronize()
+    for i, pd in enumerate(pds):
+        for a in range(dim):
+            assert np.array_equal(batched[i][a], pd.F[a].cpu().numpy()), f"patch {i} axis {a}: 8 rows != 4 rows"
+    for i in (0, len(pds) - 1):
+        Fr, _ = O.rhs_patch(O.Params(dim=dim, scheme=0), patches[i], shapes[i], g, dx, 1e-4)
+        compare(f"tile-row bands patch {i}", batched[i], Fr)
+
+
 def test_invalid_ghost_width_is_an_error_not_a_fault(torch_dev):
     from hamers_amd.capi import HamersError
 
