@@ -43,9 +43,10 @@ __global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
     const double gam = k.gamma[0];
 
     const double rho = P->Q[0][iq];
+    const double rr = rcp_rn(rho);  // quot(a, rho, rr) rounds exactly as a / rho
     double u[3];
 #pragma unroll
-    for (int b = 0; b < D; b++) u[b] = P->Q[1 + b][iq] / rho;  // FlowModelSingleSpecies.cpp:2719
+    for (int b = 0; b < D; b++) u[b] = quot(P->Q[1 + b][iq], rho, rr);  // FlowModelSingleSpecies.cpp:2719
     double ke;
     if constexpr (D == 1)
         ke = u[0] * u[0];
@@ -54,9 +55,9 @@ __global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
     else
         ke = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
     const double E = P->Q[D + 1][iq];
-    const double eps_int = E / rho - double(1) / double(2) * ke;  // :2934
-    const double p = (gam - double(1)) * rho * eps_int;           // EquationOfStateIdealGas.cpp:5477
-    const double c = sqrt(gam * p / rho);                         // EquationOfStateIdealGas.cpp:5806
+    const double eps_int = quot(E, rho, rr) - double(1) / double(2) * ke;  // :2934
+    const double p = (gam - double(1)) * rho * eps_int;                   // EquationOfStateIdealGas.cpp:5477
+    const double c = sqrt(quot(gam * p, rho, rr));                        // EquationOfStateIdealGas.cpp:5806
 
     const int64_t vol = B.vol(kGc);
     double* pr = P->prim;

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "hm_kernels.hpp"
+#include "hm_numerics.hpp"
 
 namespace hm {
 
@@ -26,16 +27,19 @@ __global__ __launch_bounds__(kBlock) void k_sensor(W w, KParams k) {
     const int j = int((D > 2) ? r % uint32_t(e1) : r) - kGs;
     const int kk = (D > 2) ? int(r / uint32_t(e1)) - kGs : 0;
     const int64_t v4 = B.vol(kGc);
-    const int64_t c = B.idx(kGc, i, j, kk);
-    const int64_t st[3] = {1, B.ext(0, kGc), int64_t(B.ext(0, kGc)) * B.ext(1, kGc)};
-    // d u_b / d x_a
+    const uint32_t c = uint32_t(B.idx(kGc, i, j, kk));
+    const uint32_t st[3] = {1u, uint32_t(B.ext(0, kGc)), uint32_t(B.ext(0, kGc) * B.ext(1, kGc))};
+    // d u_b / d x_a; quot rounds exactly as the division by dx_a does
+    double rdx[3];
+#pragma unroll
+    for (int a = 0; a < D; a++) rdx[a] = rcp_rn(P->dx[a]);
     double g[3][3];
 #pragma unroll
     for (int b = 0; b < D; b++) {
         const double* ub = P->prim + (VEL0 + b) * v4;
 #pragma unroll
         for (int a = 0; a < D; a++)
-            g[b][a] = (double(1) / double(2) * (ub[c + st[a]] - ub[c - st[a]])) / P->dx[a];
+            g[b][a] = quot(double(1) / double(2) * (ldg(ub, c + st[a]) - ldg(ub, c - st[a])), P->dx[a], rdx[a]);
     }
     double theta, Omega;
     if constexpr (D == 2) {
@@ -49,8 +53,8 @@ __global__ __launch_bounds__(kBlock) void k_sensor(W w, KParams k) {
         Omega = sqrt(ox * ox + oy * oy + oz * oz);
     }
     const int64_t vs = B.vol(kGs);
-    P->sens[loc] = theta;
-    P->sens[vs + loc] = Omega;
+    stg(P->sens, uint32_t(loc), theta);
+    stg(P->sens + vs, uint32_t(loc), Omega);
 }
 
 }  // namespace hm
