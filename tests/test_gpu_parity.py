@@ -284,25 +284,40 @@ def _tile_rows(shapes):
     return 24 if blocks(24) >= 2048 else (8 if blocks(8) >= 512 else 4)
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_tile_row_bands_agree(torch_dev, dim):
+@pytest.mark.parametrize("case", ["2d_level_8_rows", "3d_level_8_rows", "2d_uneven_24_rows"])
+def test_tile_row_bands_agree(torch_dev, case):
     """The axis-1/2 tiles come in 24, 8 and 4 rows by the size of the call:
-    a mid-size level runs batched on 8-row tiles and per patch on 4-row tiles;
-    both must equal each other bit for bit and the oracle."""
+    a level runs batched on 8-row (or 24-row) tiles and per patch on 8- or
+    4-row tiles; all must equal each other bit for bit and the oracle."""
     torch = torch_dev
     g = 4
-    if dim == 2:
+    if case == "2d_level_8_rows":
         Qi, _, dx = W.four_quadrant_riemann(N=512, seed=77)
         patches = [Qp for _, Qp in W.split_patches(Qi, (128, 128), g)]
         shapes = [(128, 128)] * len(patches)
-    else:
+        rows, single_rows = 8, 4
+    elif case == "3d_level_8_rows":
         shapes = [(40, 36, 30), (64, 20, 24), (33, 50, 17), (70, 40, 28)]
         patches = [_random_smooth_3d(n, g, seed=300 + i) for i, n in enumerate(shapes)]
         dx = (0.02, 0.025, 0.03)
-    assert _tile_rows(shapes) == 8 and _tile_rows(shapes[:1]) == 4
+        rows, single_rows = 8, 4
+    else:
+        # x extents off the 64-lane tile width, rows off the 22-face tile height
+        shapes = [(1000, 1000), (777, 1203), (1500, 600), (333, 1111), (65, 1)]
+        patches = []
+        for i, n in enumerate(shapes):
+            rng = np.random.default_rng(500 + i)
+            shp = tuple(reversed([v + 2 * g for v in n]))
+            rho = 10 ** rng.uniform(-1, 0, shp)
+            p = 10 ** rng.uniform(-1, 0, shp)
+            patches.append(W.conservative_from_primitive_single(rho, [rng.normal(size=shp) for _ in range(2)], p, 1.4))
+        dx = (1.0e-3, 1.0e-3)
+        rows, single_rows = 24, 8
+    dim = len(shapes[0])
+    assert _tile_rows(shapes) == rows and _tile_rows(shapes[:1]) == single_rows
     rec = _rec(dim, 0)
     pds = [_patchdata(torch, Qp, n, g, dx, rec.num_eqn) for Qp, n in zip(patches, shapes)]
-    _say(f"case tile-row bands dim={dim} ({len(pds)} patches)")
+    _say(f"case tile-row bands {case} ({len(pds)} patches)")
     rec.computeConvectiveFluxAndSourceOnPatches(pds, 0.0, 1e-4)
     torch.cuda.synchronize()
     batched = [[f.cpu().numpy().copy() for f in pd.F] for pd in pds]
@@ -313,7 +328,7 @@ def test_tile_row_bands_agree(torch_dev, dim):
     torch.cuda.synchronize()
     for i, pd in enumerate(pds):
         for a in range(dim):
-            assert np.array_equal(batched[i][a], pd.F[a].cpu().numpy()), f"patch {i} axis {a}: 8 rows != 4 rows"
+            assert np.array_equal(batched[i][a], pd.F[a].cpu().numpy()), f"patch {i} axis {a}: batched != per-patch"
     for i in (0, len(pds) - 1):
         Fr, _ = O.rhs_patch(O.Params(dim=dim, scheme=0), patches[i], shapes[i], g, dx, 1e-4)
         compare(f"tile-row bands patch {i}", batched[i], Fr)
