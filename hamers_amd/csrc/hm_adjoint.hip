@@ -190,7 +190,9 @@ __device__ __forceinline__ void adj_face_recon(const AdjPatch& P, int64_t loc, c
     const Boxes<D> B(F);
     const uint32_t mv = uint32_t(B.mid_vol(A)), l32 = uint32_t(loc);
     const double* H = P.H[A];
-    if (ldg(H, (NH - 1) * mv + l32) == 2.0) return;
+    // component c of a face record starts at H + c * mv (64-bit base), so the
+    // 32-bit offset stays below one component's size
+    if (ldg(H + size_t(NH - 1) * mv, l32) == 2.0) return;
     int c0, c1, c2;
     mid_cell<D, A>(B, loc, c0, c1, c2);
     const int64_t v4 = B.vol(kGc);
@@ -202,8 +204,8 @@ __device__ __forceinline__ void adj_face_recon(const AdjPatch& P, int64_t loc, c
     double* G = P.G[A];
     face_adj_single_recon<D, A, SCHEME, PP, QQ>(
         [&](int m, int e) { return ldg(pr + e * v4, iR + uint32_t(m - 3) * st); }, cL, cR,
-        LDConst{k.p, k.q, k.C, k.alpha_tau}, [&](int c) { return ldg(H, uint32_t(c) * mv + l32); },
-        [&](int c, double v) { stg(G, uint32_t(c) * mv + l32, v); });
+        LDConst{k.p, k.q, k.C, k.alpha_tau}, [&](int c) { return ldg(H + size_t(c) * mv, l32); },
+        [&](int c, double v) { stg(G + size_t(c) * mv, l32, v); });
 }
 
 template <int D, int SCHEME, int PP, int QQ, class W, bool STORED = false>
@@ -302,14 +304,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_cells(W w, KParams k) {
             if (jf < -1 || jf > na + 1) continue;
             const int64_t fi = B.mid_idx(a, (a == 0) ? jf : c[0], (a == 1) ? jf : c[1], (a == 2) ? jf : c[2]);
             const uint32_t fi32 = uint32_t(fi), mv32 = uint32_t(mv);
-            const double flag = ldg(P->H[a], (NH - 1) * mv32 + fi32);
+            const double flag = ldg(P->H[a] + size_t(NH - 1) * mv32, fi32);
             if (flag == 2.0) continue;
             // projection averages of the face: its cells sit at stencil points 2 and 3
             const int64_t l2 = loc + (2 - m) * sta, l3 = loc + (3 - m) * sta;
             const double ra = double(1) / double(2) * (F.prim[l2] + F.prim[l3]);
             const double ca = double(1) / double(2) * (F.prim[NEQ * v4 + l2] + F.prim[NEQ * v4 + l3]);
             const double* G = P->G[a];
-            auto gat = [&](int cc) { return ldg(G, uint32_t(cc) * mv32 + fi32); };
+            auto gat = [&](int cc) { return ldg(G + size_t(cc) * mv32, fi32); };
             if (a == 0) {
                 face_record_to_point<D, 0>(flag, gat, ra, ca, m, Vb, cb);
             } else if constexpr (D > 1) {

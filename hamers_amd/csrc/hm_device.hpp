@@ -16,8 +16,10 @@ constexpr int kBlock = 256;
 // global memory at a 32-bit byte offset from a wave-uniform base: the load
 // then takes its base in scalar registers and one 32-bit vector offset
 // instead of a 64-bit flat address per lane.  Every entry point keeps a
-// patch's ghost box below 2^29 cells, so i * 8 fits 32 bits.  i may be
-// formed with wrap-around (base + i - k, i >= k) as unsigned arithmetic.
+// patch's ghost box below 2^29 cells, and callers index one component
+// (cells or faces of one box) from that component's own base, so i * 8 fits
+// 32 bits.  i may be formed with wrap-around (base + i - k, i >= k) as
+// unsigned arithmetic.
 __device__ __forceinline__ double ldg(const double* base, uint32_t i) {
     typedef __attribute__((address_space(1))) const char gchar;
     typedef __attribute__((address_space(1))) const double gdouble;

@@ -3,9 +3,9 @@
 // Pipeline per RHS evaluation (each stage is one launch over all work items
 // of all patches handed to it):
 //   k_derived_single    cells of the 4-ghost box: V = (rho, u.., p) and c
-//                       (FlowModelSingleSpecies.cpp:2649-3179)
-//   k_sensor            cells of the 2-ghost box: dilatation and vorticity
-//                       magnitude (WCNS56-HLLC-HLL.cpp:626-731 (2D), 1470-1656 (3D))
+//                       (FlowModelSingleSpecies.cpp:2649-3179); on the 2-ghost
+//                       box also the dilatation and vorticity magnitude
+//                       (WCNS56-HLLC-HLL.cpp:626-731 (2D), 1470-1656 (3D))
 //   k_flux_single       axis 0: midpoint fluxes (characteristic projection, WCNS
 //                       interpolation, back-projection, positivity fallback,
 //                       HLLC / HLLC-HLL and sensor select; WCNS56-HLLC-HLL.cpp:
@@ -22,7 +22,11 @@
 namespace hm {
 
 // ----------------------------------------------------------------------------
-// Derived cell data.
+// Derived cell data, and on the cells of the 2-ghost box the dilatation and
+// vorticity magnitude (second-order central derivatives, DerivativeFirstOrder.cpp:
+// 216-228) from the velocities of the four (six) face neighbours, recomputed
+// from Q as this kernel computes its own cell's: no second pass over the
+// velocities in memory.
 // ----------------------------------------------------------------------------
 template <int D, class W>
 __global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
@@ -66,6 +70,47 @@ __global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
     for (int b = 0; b < D; b++) pr[(1 + b) * vol + loc] = u[b];
     pr[(D + 1) * vol + loc] = p;
     pr[NEQ * vol + loc] = c;
+
+    if constexpr (D > 1) {
+        // sensor on the 2-ghost box (the same expressions as k_sensor)
+        const bool in2 = i >= -kGs && i < B.n[0] + kGs && j >= -kGs && j < B.n[1] + kGs &&
+                         (D < 3 || (kk >= -kGs && kk < B.n[2] + kGs));
+        if (!in2) return;
+        const int64_t qs[3] = {1, int64_t(P->n[0] + 2 * P->gq[0]),
+                               int64_t(P->n[0] + 2 * P->gq[0]) * (P->n[1] + 2 * P->gq[1])};
+        // velocity b of the neighbour at offset o, as computed for that cell above
+        auto vel = [&](int64_t o, double (&v)[3]) {
+            const double rn = P->Q[0][iq + o];
+            const double rrn = rcp_rn(rn);
+#pragma unroll
+            for (int b = 0; b < D; b++) v[b] = quot(P->Q[1 + b][iq + o], rn, rrn);
+        };
+        double g[3][3];
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            double vp[3], vm[3];
+            vel(qs[a], vp);
+            vel(-qs[a], vm);
+            const double rdx = rcp_rn(P->dx[a]);
+#pragma unroll
+            for (int b = 0; b < D; b++) g[b][a] = quot(double(1) / double(2) * (vp[b] - vm[b]), P->dx[a], rdx);
+        }
+        double theta, Omega;
+        if constexpr (D == 2) {
+            theta = g[0][0] + g[1][1];
+            Omega = fabs(g[1][0] - g[0][1]);
+        } else {
+            theta = g[0][0] + g[1][1] + g[2][2];
+            const double ox = g[2][1] - g[1][2];
+            const double oy = g[0][2] - g[2][0];
+            const double oz = g[1][0] - g[0][1];
+            Omega = sqrt(ox * ox + oy * oy + oz * oz);
+        }
+        const int64_t vs = B.vol(kGs);
+        const int64_t ls = B.idx(kGs, i, j, kk);
+        P->sens[ls] = theta;
+        P->sens[vs + ls] = Omega;
+    }
 }
 
 // ----------------------------------------------------------------------------
@@ -104,8 +149,8 @@ __device__ __forceinline__ void midpoint_flux_at(const DevPatch& P, const Boxes<
                                                Fo, [&](const double* Wm, const double* Wp) {
 #pragma unroll
                                                    for (int c = 0; c < NEQ; c++) {
-                                                       stg(wr, uint32_t(c) * mv + mi, Wm[c]);
-                                                       stg(wr, uint32_t(NEQ + c) * mv + mi, Wp[c]);
+                                                       stg(wr + size_t(c) * mv, mi, Wm[c]);
+                                                       stg(wr + size_t(NEQ + c) * mv, mi, Wp[c]);
                                                    }
                                                });
     } else {
@@ -340,11 +385,7 @@ hipError_t launch_single_dsq(const Plan<W>& plan, const KParams& k, hipStream_t 
     if (plan.items[0] > 0)
         hipLaunchKernelGGL((k_derived_single<D, W>), dim3(unsigned((plan.items[0] + kBlock - 1) / kBlock)),
                            dim3(kBlock), 0, s, plan.work[0], k);
-    if constexpr (D > 1) {
-        if (plan.items[1] > 0)
-            hipLaunchKernelGGL((k_sensor<D, 1, W>), dim3(unsigned((plan.items[1] + kBlock - 1) / kBlock)),
-                               dim3(kBlock), 0, s, plan.work[1], k);
-    }
+    // (the dilatation and vorticity are computed by k_derived_single: no sensor stage)
     // Stage 2: fused flux on axis 0; stage 3: fused flux on axes 1 and 2 (tiles).
     if constexpr (std::is_same_v<W, SegSingle>) {
         // one patch: both flux stages in one launch (stage 2's table holds all axes)

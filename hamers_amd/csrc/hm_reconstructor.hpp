@@ -132,7 +132,7 @@ class ConvectiveFluxReconstructorGPU {
         plan.work[1].p = dp;
         plan.work[1].nseg = 1;
         plan.work[1].start[0] = 0;
-        plan.work[1].cnt[0] = (d > 1) ? e.vol2 : 0;
+        plan.work[1].cnt[0] = (d > 1 && !fused_) ? e.vol2 : 0;  // single species: in the derived stage
         plan.work[1].start[1] = align_items(plan.work[1].cnt[0]);
         plan.items[1] = plan.work[1].start[1];
         // midpoint faces and output faces, one segment per axis
@@ -454,11 +454,12 @@ class ConvectiveFluxReconstructorGPU {
                 return fail(HM_ERR_INVALID_ARGUMENT, buf);
             }
         }
-        // Kernels number the items of a patch's work segment in 32 bits.
+        // Kernels number the items of a patch's work segment in 32 bits and
+        // address one component of a patch array at a 32-bit byte offset.
         double box = 1.0;
         for (int a = 0; a < d; a++) box *= double(v.n[a]) + 2.0 * double(v.ghost[a] > 6 ? v.ghost[a] : 6);
-        if (box >= 2147483648.0) {
-            snprintf(buf, sizeof buf, "patch %d: ghost box of 2^31 cells or more; split the patch", which);
+        if (box >= 536870912.0) {
+            snprintf(buf, sizeof buf, "patch %d: ghost box of 2^29 cells or more; split the patch", which);
             return fail(HM_ERR_INVALID_ARGUMENT, buf);
         }
         for (int e = 0; e < numConservativeArrays(); e++)
@@ -569,7 +570,7 @@ class ConvectiveFluxReconstructorGPU {
                 acc[stage] += align_items(c);
             };
             add(0, 0, e.vol4);
-            if (d > 1) add(1, 0, e.vol2);
+            if (d > 1 && !fused_) add(1, 0, e.vol2);  // single species: in the derived stage
             for (int a = 0; a < d; a++) add(2, a, e.work2[a]);
             for (int a = 0; a < d; a++) add(3, a, e.work3[a]);
             if (has_advective_) add(4, 0, e.vol0);

@@ -1,4 +1,5 @@
-// hm_sensor.hpp -- dilatation and vorticity magnitude kernel shared by the flow models.
+// hm_sensor.hpp -- dilatation and vorticity magnitude kernel of the multi-species
+// models (the single-species model computes them in its derived-data kernel).
 #pragma once
 
 #include "hm_kernels.hpp"

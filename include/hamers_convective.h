@@ -74,7 +74,10 @@ typedef struct hm_config {
 /*
  * One patch as the reconstructor sees it: the SAMRAI hier::Patch box, the
  * CartesianPatchGeometry dx and the patch-data pointers that
- * computeConvectiveFluxAndSourceOnPatch reads and writes.
+ * computeConvectiveFluxAndSourceOnPatch reads and writes.  The patch's box
+ * grown by max(ghost, 6) cells per side must hold fewer than 2^29 cells
+ * (the kernels address one component of a patch array at a 32-bit byte
+ * offset); a larger patch is rejected with HM_ERR_INVALID_ARGUMENT.
  */
 typedef struct hm_patch_view {
     int n[3];                     /* interior cells per axis (1 on unused axes) */
