@@ -43,14 +43,17 @@ __global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
     const int i = int(uint32_t(loc) - r * uint32_t(e0)) - kGc;
     const int j = (D > 1) ? int((D > 2) ? r % uint32_t(e1) : r) - kGc : 0;
     const int kk = (D > 2) ? int(r / uint32_t(e1)) - kGc : 0;
-    const int64_t iq = q_idx<D>(*P, i, j, kk) + k.qoff;
+    // Q component e from a wave-uniform base (trajectory-slot offset included)
+    // at a 32-bit offset within the patch's Q box
+    const uint32_t iq = uint32_t(q_idx<D>(*P, i, j, kk));
+    auto q = [&](int e, uint32_t at) { return ldg(P->Q[e] + k.qoff, at); };
     const double gam = k.gamma[0];
 
-    const double rho = P->Q[0][iq];
+    const double rho = q(0, iq);
     const double rr = rcp_rn(rho);  // quot(a, rho, rr) rounds exactly as a / rho
     double u[3];
 #pragma unroll
-    for (int b = 0; b < D; b++) u[b] = quot(P->Q[1 + b][iq], rho, rr);  // FlowModelSingleSpecies.cpp:2719
+    for (int b = 0; b < D; b++) u[b] = quot(q(1 + b, iq), rho, rr);  // FlowModelSingleSpecies.cpp:2719
     double ke;
     if constexpr (D == 1)
         ke = u[0] * u[0];
@@ -58,39 +61,40 @@ __global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
         ke = u[0] * u[0] + u[1] * u[1];
     else
         ke = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    const double E = P->Q[D + 1][iq];
+    const double E = q(D + 1, iq);
     const double eps_int = quot(E, rho, rr) - double(1) / double(2) * ke;  // :2934
     const double p = (gam - double(1)) * rho * eps_int;                   // EquationOfStateIdealGas.cpp:5477
     const double c = sqrt(quot(gam * p, rho, rr));                        // EquationOfStateIdealGas.cpp:5806
 
     const int64_t vol = B.vol(kGc);
     double* pr = P->prim;
-    pr[loc] = rho;
+    const uint32_t l32 = uint32_t(loc);
+    stg(pr, l32, rho);
 #pragma unroll
-    for (int b = 0; b < D; b++) pr[(1 + b) * vol + loc] = u[b];
-    pr[(D + 1) * vol + loc] = p;
-    pr[NEQ * vol + loc] = c;
+    for (int b = 0; b < D; b++) stg(pr + (1 + b) * vol, l32, u[b]);
+    stg(pr + (D + 1) * vol, l32, p);
+    stg(pr + NEQ * vol, l32, c);
 
     if constexpr (D > 1) {
         // sensor on the 2-ghost box (the same expressions as k_sensor)
         const bool in2 = i >= -kGs && i < B.n[0] + kGs && j >= -kGs && j < B.n[1] + kGs &&
                          (D < 3 || (kk >= -kGs && kk < B.n[2] + kGs));
         if (!in2) return;
-        const int64_t qs[3] = {1, int64_t(P->n[0] + 2 * P->gq[0]),
-                               int64_t(P->n[0] + 2 * P->gq[0]) * (P->n[1] + 2 * P->gq[1])};
-        // velocity b of the neighbour at offset o, as computed for that cell above
-        auto vel = [&](int64_t o, double (&v)[3]) {
-            const double rn = P->Q[0][iq + o];
+        const uint32_t qs[3] = {1u, uint32_t(P->n[0] + 2 * P->gq[0]),
+                                uint32_t((P->n[0] + 2 * P->gq[0]) * (P->n[1] + 2 * P->gq[1]))};
+        // velocity b of the neighbour at index at, as computed for that cell above
+        auto vel = [&](uint32_t at, double (&v)[3]) {
+            const double rn = q(0, at);
             const double rrn = rcp_rn(rn);
 #pragma unroll
-            for (int b = 0; b < D; b++) v[b] = quot(P->Q[1 + b][iq + o], rn, rrn);
+            for (int b = 0; b < D; b++) v[b] = quot(q(1 + b, at), rn, rrn);
         };
         double g[3][3];
 #pragma unroll
         for (int a = 0; a < D; a++) {
             double vp[3], vm[3];
-            vel(qs[a], vp);
-            vel(-qs[a], vm);
+            vel(iq + qs[a], vp);
+            vel(iq - qs[a], vm);
             const double rdx = rcp_rn(P->dx[a]);
 #pragma unroll
             for (int b = 0; b < D; b++) g[b][a] = quot(double(1) / double(2) * (vp[b] - vm[b]), P->dx[a], rdx);
@@ -107,9 +111,9 @@ __global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
             Omega = sqrt(ox * ox + oy * oy + oz * oz);
         }
         const int64_t vs = B.vol(kGs);
-        const int64_t ls = B.idx(kGs, i, j, kk);
-        P->sens[ls] = theta;
-        P->sens[vs + ls] = Omega;
+        const uint32_t ls = uint32_t(B.idx(kGs, i, j, kk));
+        stg(P->sens, ls, theta);
+        stg(P->sens + vs, ls, Omega);
     }
 }
 
