@@ -76,44 +76,16 @@ __global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
     stg(pr + NEQ * vol, l32, c);
 
     if constexpr (D > 1) {
-        // sensor on the 2-ghost box (the same expressions as k_sensor)
-        const bool in2 = i >= -kGs && i < B.n[0] + kGs && j >= -kGs && j < B.n[1] + kGs &&
-                         (D < 3 || (kk >= -kGs && kk < B.n[2] + kGs));
-        if (!in2) return;
+        if (!in_sensor_box<D>(B, i, j, kk)) return;
         const uint32_t qs[3] = {1u, uint32_t(P->n[0] + 2 * P->gq[0]),
                                 uint32_t((P->n[0] + 2 * P->gq[0]) * (P->n[1] + 2 * P->gq[1]))};
-        // velocity b of the neighbour at index at, as computed for that cell above
-        auto vel = [&](uint32_t at, double (&v)[3]) {
+        sensor_cell<D>(*P, B, i, j, kk, [&](int a, int side, double (&v)[3]) {
+            const uint32_t at = side > 0 ? iq + qs[a] : iq - qs[a];
             const double rn = q(0, at);
             const double rrn = rcp_rn(rn);
 #pragma unroll
             for (int b = 0; b < D; b++) v[b] = quot(q(1 + b, at), rn, rrn);
-        };
-        double g[3][3];
-#pragma unroll
-        for (int a = 0; a < D; a++) {
-            double vp[3], vm[3];
-            vel(iq + qs[a], vp);
-            vel(iq - qs[a], vm);
-            const double rdx = rcp_rn(P->dx[a]);
-#pragma unroll
-            for (int b = 0; b < D; b++) g[b][a] = quot(double(1) / double(2) * (vp[b] - vm[b]), P->dx[a], rdx);
-        }
-        double theta, Omega;
-        if constexpr (D == 2) {
-            theta = g[0][0] + g[1][1];
-            Omega = fabs(g[1][0] - g[0][1]);
-        } else {
-            theta = g[0][0] + g[1][1] + g[2][2];
-            const double ox = g[2][1] - g[1][2];
-            const double oy = g[0][2] - g[2][0];
-            const double oz = g[1][0] - g[0][1];
-            Omega = sqrt(ox * ox + oy * oy + oz * oz);
-        }
-        const int64_t vs = B.vol(kGs);
-        const uint32_t ls = uint32_t(B.idx(kGs, i, j, kk));
-        stg(P->sens, ls, theta);
-        stg(P->sens + vs, ls, Omega);
+        });
     }
 }
 

@@ -16,7 +16,8 @@
 // Scratch prim holds V = (partial densities, u.., p[, Z_i (NS-1)]), then c,
 // then the mixture density rho, over the 4-ghost box.
 //
-// Pipeline: k_derived_multi, k_sensor (shared), k_midflux_multi (five-equation:
+// Pipeline: k_derived_multi (with the dilatation and vorticity on the 2-ghost
+// box), k_midflux_multi (five-equation:
 // also the HLLC normal velocity u* of every midpoint face), k_faceflux_multi,
 // k_source_five.
 #pragma once
@@ -130,14 +131,16 @@ __global__ __launch_bounds__(kBlock) void k_derived_multi(W w, KParams k) {
     const int i = int(uint32_t(loc) - r * uint32_t(e0)) - kGc;
     const int j = (D > 1) ? int((D > 2) ? r % uint32_t(e1) : r) - kGc : 0;
     const int kk = (D > 2) ? int(r / uint32_t(e1)) - kGc : 0;
-    const int64_t iq = q_idx<D>(*P, i, j, kk);
+    // Q components at 32-bit offsets within the patch's Q box (below 2^29 cells)
+    const uint32_t iq = uint32_t(q_idx<D>(*P, i, j, kk));
+    auto q = [&](int e, uint32_t at) { return ldg(P->Q[e], at); };
 
     double Zr[NS], Z[NS];
 #pragma unroll
-    for (int si = 0; si < NS; si++) Zr[si] = P->Q[si][iq];
+    for (int si = 0; si < NS; si++) Zr[si] = q(si, iq);
     if constexpr (!FOUR) {
 #pragma unroll
-        for (int si = 0; si < NS; si++) Z[si] = P->Q[NS + D + 1 + si][iq];
+        for (int si = 0; si < NS; si++) Z[si] = q(NS + D + 1 + si, iq);
     }
     double rho = double(0);  // EquationOfStateMixingRules.cpp:735-860
 #pragma unroll
@@ -145,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_derived_multi(W w, KParams k) {
     double u[3] = {0.0, 0.0, 0.0};
     const double rrho = rcp_rn(rho);  // quotients by rho, correctly rounded (quot)
 #pragma unroll
-    for (int b = 0; b < D; b++) u[b] = quot(P->Q[NS + b][iq], rho, rrho);
+    for (int b = 0; b < D; b++) u[b] = quot(q(NS + b, iq), rho, rrho);
     double ke;
     if constexpr (D == 1)
         ke = u[0] * u[0];
@@ -153,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_derived_multi(W w, KParams k) {
         ke = u[0] * u[0] + u[1] * u[1];
     else
         ke = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    const double E = P->Q[NS + D][iq];
+    const double E = q(NS + D, iq);
     const double eps_int = quot(E, rho, rrho) - double(1) / double(2) * ke;
     double p, c2;
     if constexpr (FOUR) {
@@ -178,15 +181,33 @@ __global__ __launch_bounds__(kBlock) void k_derived_multi(W w, KParams k) {
 
     const int64_t vol = B.vol(kGc);
     double* pr = P->prim;
+    const uint32_t l32 = uint32_t(loc);
 #pragma unroll
-    for (int si = 0; si < NS; si++) pr[si * vol + loc] = Zr[si];
+    for (int si = 0; si < NS; si++) stg(pr + si * vol, l32, Zr[si]);
 #pragma unroll
-    for (int b = 0; b < D; b++) pr[(NS + b) * vol + loc] = u[b];
-    pr[(NS + D) * vol + loc] = p;
+    for (int b = 0; b < D; b++) stg(pr + (NS + b) * vol, l32, u[b]);
+    stg(pr + (NS + D) * vol, l32, p);
 #pragma unroll
-    for (int si = 0; si < M::NZ; si++) pr[(NS + D + 1 + si) * vol + loc] = Z[si];
-    pr[NEQ * vol + loc] = sqrt(c2);
-    pr[(NEQ + 1) * vol + loc] = rho;
+    for (int si = 0; si < M::NZ; si++) stg(pr + (NS + D + 1 + si) * vol, l32, Z[si]);
+    stg(pr + NEQ * vol, l32, sqrt(c2));
+    stg(pr + (NEQ + 1) * vol, l32, rho);
+
+    if constexpr (D > 1) {
+        // dilatation and vorticity on the 2-ghost box, from the neighbours'
+        // velocities computed as above
+        if (!in_sensor_box<D>(B, i, j, kk)) return;
+        const uint32_t qs[3] = {1u, uint32_t(P->n[0] + 2 * P->gq[0]),
+                                uint32_t((P->n[0] + 2 * P->gq[0]) * (P->n[1] + 2 * P->gq[1]))};
+        sensor_cell<D>(*P, B, i, j, kk, [&](int a, int side, double (&v)[3]) {
+            const uint32_t at = side > 0 ? iq + qs[a] : iq - qs[a];
+            double rn = double(0);
+#pragma unroll
+            for (int si = 0; si < NS; si++) rn += q(si, at);
+            const double rrn = rcp_rn(rn);
+#pragma unroll
+            for (int b = 0; b < D; b++) v[b] = quot(q(NS + b, at), rn, rrn);
+        });
+    }
 }
 
 // ----------------------------------------------------------------------------
@@ -659,10 +680,7 @@ hipError_t launch_multi_dsq(const Plan<W>& plan, const KParams& k, hipStream_t s
     if (plan.items[0] > 0)
         hipLaunchKernelGGL((k_derived_multi<D, NS, FOUR, W>), dim3(nblocks(plan, 0)), dim3(kBlock), 0, s,
                            plan.work[0], k);
-    if constexpr (D > 1) {
-        if (plan.items[1] > 0)
-            hipLaunchKernelGGL((k_sensor<D, NS, W>), dim3(nblocks(plan, 1)), dim3(kBlock), 0, s, plan.work[1], k);
-    }
+    // (stage 1 is empty: the derived-data kernel computes the dilatation and vorticity)
     if (plan.items[2] > 0)
         hipLaunchKernelGGL((k_midflux_multi<D, NS, FOUR, SCHEME, PP, QQ, W>), dim3(nblocks(plan, 2)), dim3(kBlock),
                            0, s, plan.work[2], k);

@@ -132,7 +132,7 @@ class ConvectiveFluxReconstructorGPU {
         plan.work[1].p = dp;
         plan.work[1].nseg = 1;
         plan.work[1].start[0] = 0;
-        plan.work[1].cnt[0] = (d > 1 && !fused_) ? e.vol2 : 0;  // single species: in the derived stage
+        plan.work[1].cnt[0] = 0;  // the derived-data kernels compute the sensor (stage 1 is empty)
         plan.work[1].start[1] = align_items(plan.work[1].cnt[0]);
         plan.items[1] = plan.work[1].start[1];
         // midpoint faces and output faces, one segment per axis
@@ -570,7 +570,7 @@ class ConvectiveFluxReconstructorGPU {
                 acc[stage] += align_items(c);
             };
             add(0, 0, e.vol4);
-            if (d > 1 && !fused_) add(1, 0, e.vol2);  // single species: in the derived stage
+            // (stage 1 stays empty: the derived-data kernels compute the sensor)
             for (int a = 0; a < d; a++) add(2, a, e.work2[a]);
             for (int a = 0; a < d; a++) add(3, a, e.work3[a]);
             if (has_advective_) add(4, 0, e.vol0);

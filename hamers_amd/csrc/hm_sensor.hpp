@@ -1,5 +1,5 @@
-// hm_sensor.hpp -- dilatation and vorticity magnitude kernel of the multi-species
-// models (the single-species model computes them in its derived-data kernel).
+// hm_sensor.hpp -- dilatation and vorticity magnitude of one cell, computed by
+// the derived-data kernels of every flow model on the cells of the 2-ghost box.
 #pragma once
 
 #include "hm_kernels.hpp"
@@ -7,40 +7,25 @@
 
 namespace hm {
 
-// ----------------------------------------------------------------------------
-// Dilatation and vorticity magnitude (second-order central derivatives,
-// DerivativeFirstOrder.cpp:216-228).
-// ----------------------------------------------------------------------------
-// VEL0 is the slot of the first velocity component in the scratch array prim.
-template <int D, int VEL0, class W>
-__global__ __launch_bounds__(kBlock) void k_sensor(W w, KParams k) {
-    (void)k;
-    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    const DevPatch* P;
-    int ax;
-    int64_t loc;
-    if (!w.find(gid, P, ax, loc)) return;
-    const Boxes<D> B(*P);
-    const int e0 = B.ext(0, kGs), e1 = B.ext(1, kGs);
-    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
-    const uint32_t r = uint32_t(loc) / uint32_t(e0);
-    const int i = int(uint32_t(loc) - r * uint32_t(e0)) - kGs;
-    const int j = int((D > 2) ? r % uint32_t(e1) : r) - kGs;
-    const int kk = (D > 2) ? int(r / uint32_t(e1)) - kGs : 0;
-    const int64_t v4 = B.vol(kGc);
-    const uint32_t c = uint32_t(B.idx(kGc, i, j, kk));
-    const uint32_t st[3] = {1u, uint32_t(B.ext(0, kGc)), uint32_t(B.ext(0, kGc) * B.ext(1, kGc))};
-    // d u_b / d x_a; quot rounds exactly as the division by dx_a does
-    double rdx[3];
+// Dilatation theta and vorticity magnitude Omega of cell (i, j, kk) from
+// second-order central derivatives of the velocity (DerivativeFirstOrder.cpp:
+// 216-228; WCNS56-HLLC-HLL.cpp:626-731 (2D), 1470-1656 (3D)), written to the
+// patch's sensor array.  vel(a, s, v) returns the velocity components v[b] of
+// the face neighbour at side s (+1, -1) along axis a, computed exactly as the
+// derived-data kernel computes a cell's own velocity, so the values equal
+// those the kernel stores for that neighbour.  Quotients by dx_a use quot,
+// which rounds as the division does.
+template <int D, class Vel>
+__device__ __forceinline__ void sensor_cell(const DevPatch& P, const Boxes<D>& B, int i, int j, int kk, Vel vel) {
+    double g[3][3];  // d u_b / d x_a
 #pragma unroll
-    for (int a = 0; a < D; a++) rdx[a] = rcp_rn(P->dx[a]);
-    double g[3][3];
+    for (int a = 0; a < D; a++) {
+        double vp[3], vm[3];
+        vel(a, 1, vp);
+        vel(a, -1, vm);
+        const double rdx = rcp_rn(P.dx[a]);
 #pragma unroll
-    for (int b = 0; b < D; b++) {
-        const double* ub = P->prim + (VEL0 + b) * v4;
-#pragma unroll
-        for (int a = 0; a < D; a++)
-            g[b][a] = quot(double(1) / double(2) * (ldg(ub, c + st[a]) - ldg(ub, c - st[a])), P->dx[a], rdx[a]);
+        for (int b = 0; b < D; b++) g[b][a] = quot(double(1) / double(2) * (vp[b] - vm[b]), P.dx[a], rdx);
     }
     double theta, Omega;
     if constexpr (D == 2) {
@@ -54,8 +39,16 @@ __global__ __launch_bounds__(kBlock) void k_sensor(W w, KParams k) {
         Omega = sqrt(ox * ox + oy * oy + oz * oz);
     }
     const int64_t vs = B.vol(kGs);
-    stg(P->sens, uint32_t(loc), theta);
-    stg(P->sens + vs, uint32_t(loc), Omega);
+    const uint32_t ls = uint32_t(B.idx(kGs, i, j, kk));
+    stg(P.sens, ls, theta);
+    stg(P.sens + vs, ls, Omega);
+}
+
+// Whether cell (i, j, kk) lies in the 2-ghost box, where the sensor is needed.
+template <int D>
+__device__ __forceinline__ bool in_sensor_box(const Boxes<D>& B, int i, int j, int kk) {
+    return i >= -kGs && i < B.n[0] + kGs && j >= -kGs && j < B.n[1] + kGs &&
+           (D < 3 || (kk >= -kGs && kk < B.n[2] + kGs));
 }
 
 }  // namespace hm
