@@ -223,11 +223,12 @@ __global__ __launch_bounds__(kBlock) void k_stage_adjoint(SegTableT<AdjLevelPatc
     }
 }
 
-// Interior updates of the stage adjoints: op 0: lam[dst] = 0; op 1: lam[dst] += a lam[src];
-// op 2: lam[dst] = lam[src]; op 3: lam[0 .. dst) = 0.
+// Interior updates of the stage adjoints: kLamCopy: lam[dst] = lam[src];
+// kLamZeroBelow: lam[0 .. dst) = 0.
+enum { kLamCopy = 2, kLamZeroBelow = 3 };
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_lam_op(SegTableT<AdjLevelPatch> w, int op, int dst, int src, double a,
-                                                   int neq, int g) {
+__global__ __launch_bounds__(kBlock) void k_lam_op(SegTableT<AdjLevelPatch> w, int op, int dst, int src, int neq,
+                                                   int g) {
     const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     const AdjLevelPatch* P;
     int ax;
@@ -241,20 +242,12 @@ __global__ __launch_bounds__(kBlock) void k_lam_op(SegTableT<AdjLevelPatch> w, i
     const int64_t gi = gbox_idx<D>(P->n, g, i, j, k);
     double* L = P->lam[dst];
     const double* S = P->lam[src];
-    if (op == 3) {  // lam[0 .. dst) = 0
+    if (op == kLamZeroBelow) {
         for (int mm = 0; mm < dst; mm++)
             for (int e = 0; e < neq; e++) P->lam[mm][e * P->gvol + gi] = 0.0;
         return;
     }
-    for (int e = 0; e < neq; e++) {
-        const int64_t q = e * P->gvol + gi;
-        if (op == 0)
-            L[q] = 0.0;
-        else if (op == 1)
-            L[q] += a * S[q];
-        else
-            L[q] = S[q];
-    }
+    for (int e = 0; e < neq; e++) L[e * P->gvol + gi] = S[e * P->gvol + gi];  // kLamCopy
 }
 
 // Whole-ghost-box copy between a state buffer and a storage slot
@@ -808,11 +801,11 @@ class Level {
         return hipGetLastError() == hipSuccess ? HM_OK : rec_->fail(HM_ERR_DEVICE, "store launch failed");
     }
 
-    int lam_op(int op, int dst, int src, double a, hipStream_t s) {
+    int lam_op(int op, int dst, int src, hipStream_t s) {
         switch (d_) {
-            case 1: hipLaunchKernelGGL(k_lam_op<1>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, op, dst, src, a, neq_, g_); break;
-            case 2: hipLaunchKernelGGL(k_lam_op<2>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, op, dst, src, a, neq_, g_); break;
-            default: hipLaunchKernelGGL(k_lam_op<3>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, op, dst, src, a, neq_, g_); break;
+            case 1: hipLaunchKernelGGL(k_lam_op<1>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, op, dst, src, neq_, g_); break;
+            case 2: hipLaunchKernelGGL(k_lam_op<2>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, op, dst, src, neq_, g_); break;
+            default: hipLaunchKernelGGL(k_lam_op<3>, dim3(blocks_for(adj_cell_items_)), dim3(kBlock), 0, s, adj_cells_, op, dst, src, neq_, g_); break;
         }
         return hipGetLastError() == hipSuccess ? HM_OK : rec_->fail(HM_ERR_DEVICE, "adjoint launch failed");
     }
@@ -825,7 +818,7 @@ class Level {
     // several nonzero beta per stage have, add theirs with k_halo_reverse).
     int adjoint_step(int slot0, int face_slot0, hipStream_t s) {
         if (zero_first_) {
-            if (int st = lam_op(3, nst_, 0, 0.0, s)) return st;
+            if (int st = lam_op(kLamZeroBelow, nst_, 0, s)) return st;
         }
         for (int sn = nst_ - 1; sn >= 0; sn--) {
             int last_vjp = -1;
@@ -856,7 +849,7 @@ class Level {
             }
             if (hipGetLastError() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "adjoint launch failed");
         }
-        if (!out_in_place_) return lam_op(2, nst_, 0, 0.0, s);
+        if (!out_in_place_) return lam_op(kLamCopy, nst_, 0, s);
         return HM_OK;
     }
 
