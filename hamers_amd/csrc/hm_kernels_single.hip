@@ -136,15 +136,15 @@ __device__ __forceinline__ void midpoint_flux_at(const DevPatch& P, const Boxes<
 }
 
 // Node flux of equation e at a cell (FlowModelSingleSpecies.cpp:3329-3331,
-// 3388-3391, 3470-3474) from its conservative variables (index q) and its
-// velocity ua along A and pressure p.
+// 3388-3391, 3470-3474) from its conservative variables (index q in the Q
+// box, below 2^29) and its velocity ua along A and pressure p.
 template <int D, int A>
-__device__ __forceinline__ double node_flux_single(const DevPatch& P, int64_t q, double ua, double p, int e) {
+__device__ __forceinline__ double node_flux_single(const DevPatch& P, uint32_t q, double ua, double p, int e) {
     constexpr int IP = D + 1;
-    if (e == 0) return P.Q[1 + A][q];
-    if (e == IP) return ua * (P.Q[IP][q] + p);
-    if (e == 1 + A) return ua * P.Q[e][q] + p;
-    return ua * P.Q[e][q];
+    if (e == 0) return ldg(P.Q[1 + A], q);
+    if (e == IP) return ua * (ldg(P.Q[IP], q) + p);
+    if (e == 1 + A) return ua * ldg(P.Q[e], q) + p;
+    return ua * ldg(P.Q[e], q);
 }
 
 // Output flux of face f along A: dt (1/30 (m_{f+1} + m_{f-1}) - 3/10 (n_f + n_{f-1}) + 23/15 m_f).
@@ -154,8 +154,9 @@ __device__ __forceinline__ void store_face_flux(double* out, int64_t fv, int64_t
                                                 const double* nR) {
 #pragma unroll
     for (int e = 0; e < D + 2; e++)
-        out[e * fv + fi] = dt * (double(1) / double(30) * (mR[e] + mL[e]) - double(3) / double(10) * (nR[e] + nL[e]) +
-                                 double(23) / double(15) * mM[e]);
+        stg(out + e * fv, uint32_t(fi),
+            dt * (double(1) / double(30) * (mR[e] + mL[e]) - double(3) / double(10) * (nR[e] + nL[e]) +
+                  double(23) / double(15) * mM[e]));
 }
 
 template <int D, int A, int SCHEME, int PP, int QQ, bool REC>
@@ -205,8 +206,8 @@ __device__ __forceinline__ void fused_chunk(const DevPatch& P, int64_t loc, cons
     }
     // Output face f = c[A] of the line, for faces 0..n_A held by lanes 1..62.
     if (!valid || lane == 0u || lane > uint32_t(kChunk) || cm < 1 || cm > B.n[A] + 1) return;
-    const int64_t qst = (A == 0) ? 1 : (A == 1) ? (P.n[0] + 2 * P.gq[0]) : int64_t(P.n[0] + 2 * P.gq[0]) * (P.n[1] + 2 * P.gq[1]);
-    const int64_t qR = q_idx<D>(P, c0, c1, c2), qL = qR - qst;
+    const uint32_t qst = (A == 0) ? 1u : (A == 1) ? uint32_t(P.n[0] + 2 * P.gq[0]) : uint32_t((P.n[0] + 2 * P.gq[0]) * (P.n[1] + 2 * P.gq[1]));
+    const uint32_t qR = uint32_t(q_idx<D>(P, c0, c1, c2)), qL = qR - qst;
     double nL[NEQ], nR[NEQ];
 #pragma unroll
     for (int e = 0; e < NEQ; e++) {
@@ -284,7 +285,7 @@ __device__ __forceinline__ void fused_tile(const DevPatch& P, int64_t loc, const
         for (int e = 0; e < NEQ; e++) lds[(e * R + row) * kTileX + tx] = m[e];
     }
     __syncthreads();
-    const int64_t qst = (A == 1) ? (P.n[0] + 2 * P.gq[0]) : int64_t(P.n[0] + 2 * P.gq[0]) * (P.n[1] + 2 * P.gq[1]);
+    const uint32_t qst = (A == 1) ? uint32_t(P.n[0] + 2 * P.gq[0]) : uint32_t((P.n[0] + 2 * P.gq[0]) * (P.n[1] + 2 * P.gq[1]));
 #pragma unroll 1
     for (int q = 0; q < R / 4; q++) {
         const int row = ty + 4 * q;
@@ -293,7 +294,7 @@ __device__ __forceinline__ void fused_tile(const DevPatch& P, int64_t loc, const
         int c0, c1, c2;
         cell(f, c0, c1, c2);
         const uint32_t pR = uint32_t(B.idx(kGc, c0, c1, c2)), pL = pR - st;
-        const int64_t qR = q_idx<D>(P, c0, c1, c2), qL = qR - qst;
+        const uint32_t qR = uint32_t(q_idx<D>(P, c0, c1, c2)), qL = qR - qst;
         const double uaL = ldg(pr + (1 + A) * v4, pL), uaR = ldg(pr + (1 + A) * v4, pR);
         const double prL = ldg(pr + IP * v4, pL), prR = ldg(pr + IP * v4, pR);
         double mL[NEQ], mM[NEQ], mR[NEQ], nL[NEQ], nR[NEQ];
