@@ -26,6 +26,17 @@ __device__ __forceinline__ double ldg(const double* base, uint32_t i) {
     return *(gdouble*)((gchar*)base + i * 8u);
 }
 
+// Block number a kernel works on for hardware block blockIdx.x of a grid of
+// nb blocks.  The eight XCDs of the chip (each with its own L2) take blocks
+// round-robin: hardware blocks b, b + 8, b + 16, ... run on XCD b % 8.  The
+// map hands each XCD one contiguous range of block numbers instead, so that
+// neighbouring blocks of a patch, which re-read each other's stencil rows,
+// share an L2.  A bijection on [0, nb).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t xcd = b & 7u, local = b >> 3, q = nb >> 3, r = nb & 7u;
+    return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
 // Store counterpart of ldg.
 __device__ __forceinline__ void stg(double* base, uint32_t i, double v) {
     typedef __attribute__((address_space(1))) char gchar;
