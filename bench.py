@@ -45,7 +45,7 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 STAGE_KERNEL = {"midflux": "k_flux_single", "faceflux": "k_flux_tile_single"}
 # DRAM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0) of each flux
 # kernel, 2048^2 in 128^2 patches, WCNS5-JS.
-FLUX_TRAFFIC_2048_WCNS5 = {"midflux": 381.95e6 + 136.40e6, "faceflux": 637.54e6 + 135.27e6}
+FLUX_TRAFFIC_2048_WCNS5 = {"midflux": 354.92e6 + 135.27e6, "faceflux": 506.47e6 + 135.27e6}
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector
 
 
