@@ -569,6 +569,8 @@ int AdjointSingle::vjp(const hm_vjp_patch_view* views, int np, double dt, hipStr
     if (np < 0 || (np > 0 && !views)) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "invalid patch list");
     if (np == 0) return HM_OK;
     if (int st = rec_->selectDevice()) return st;
+    // calls on other streams finish with the cached scratch first
+    if (order_.enter(s) != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "hipStreamWaitEvent failed");
     const bool same = int(cached_views_.size()) == np && cached_.tables &&
                       std::memcmp(cached_views_.data(), views, sizeof(hm_vjp_patch_view) * size_t(np)) == 0;
     if (!same) {
@@ -577,7 +579,9 @@ int AdjointSingle::vjp(const hm_vjp_patch_view* views, int np, double dt, hipStr
         if (int st = build(views, np, cached_)) return st;
         cached_views_.assign(views, views + np);
     }
-    return run(cached_, dt, s);
+    if (int st = run(cached_, dt, s)) return st;
+    if (order_.leave(s) != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "hipEventRecord failed");
+    return HM_OK;
 }
 
 }  // namespace hm

@@ -84,6 +84,7 @@ class AdjointSingle {
     ConvectiveFluxReconstructorGPU* rec_;
     Owned cached_{};
     std::vector<hm_vjp_patch_view> cached_views_;
+    StreamOrder order_;  // calls on different streams take the cached scratch in turn
 };
 
 }  // namespace hm

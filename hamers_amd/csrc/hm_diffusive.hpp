@@ -103,6 +103,7 @@ class DiffusiveFluxReconstructorGPU {
         if (int st = check_view(v, 0)) return st;
         if (int st = rec_.selectDevice()) return st;
         const Layout L = layout(v);
+        if (order_.enter(s) != hipSuccess) return rec_.fail(HM_ERR_DEVICE, "hipStreamWaitEvent failed");
         if (int st = ensure_arena(L.total, s)) return st;
         const DiffPatch p = make_patch(v, arena_, L);
         DPlan<SegSingleT<DiffPatch>> plan{};
@@ -124,7 +125,7 @@ class DiffusiveFluxReconstructorGPU {
         }
         DParams k = k_;
         k.dt = dt;
-        return launch(plan, k, s);
+        return launch_ordered(plan, k, s);
     }
 
     int computeDiffusiveFluxOnPatches(const hm_patch_view* views, int np, double time, double dt, int rk_step,
@@ -138,15 +139,24 @@ class DiffusiveFluxReconstructorGPU {
         if (int st = rec_.selectDevice()) return st;
         size_t total = 0;
         for (int i = 0; i < np; i++) total += layout(views[i]).total;
+        if (order_.enter(s) != hipSuccess) return rec_.fail(HM_ERR_DEVICE, "hipStreamWaitEvent failed");
         if (int st = ensure_arena(total, s)) return st;
         CachedPlan* cp = nullptr;
         if (int st = get_plan(views, np, s, cp)) return st;
         DParams k = k_;
         k.dt = dt;
-        return launch(cp->plan, k, s);
+        return launch_ordered(cp->plan, k, s);
     }
 
    private:
+    // launch() on the arena, then mark the arena in use by stream s.
+    template <class W>
+    int launch_ordered(const DPlan<W>& plan, const DParams& k, hipStream_t s) {
+        if (int st = launch(plan, k, s)) return st;
+        if (order_.leave(s) != hipSuccess) return rec_.fail(HM_ERR_DEVICE, "hipEventRecord failed");
+        return HM_OK;
+    }
+
     struct Ext {
         int64_t cells, nodes[3], faces[3];
     };
@@ -348,6 +358,7 @@ class DiffusiveFluxReconstructorGPU {
     int neq_ = 0;
     double* arena_ = nullptr;
     size_t arena_doubles_ = 0;
+    StreamOrder order_;  // calls on different streams take the arena in turn
     std::vector<CachedPlan> plans_;
     uint64_t use_clock_ = 0;
 };

@@ -213,7 +213,8 @@ __global__ __launch_bounds__(kBlock) void k_derived_multi(W w, KParams k) {
 // ----------------------------------------------------------------------------
 // Fused midpoint flux of one face (multi-species models).
 // ----------------------------------------------------------------------------
-template <int D, int NS, bool FOUR>
+// Bounds of one side of a face normal to axis A.
+template <int D, int NS, bool FOUR, int A>
 __device__ __forceinline__ bool bounded_multi(const double* V, const KParams& k) {
     if constexpr (FOUR) {
         // FlowModelBasicUtilitiesFourEqnConservative.cpp:3737-4511 (bounds in the .hpp:24-25):
@@ -231,6 +232,9 @@ __device__ __forceinline__ bool bounded_multi(const double* V, const KParams& k)
         return ok && (rho > double(0)) && (V[NS + D] > double(0));
     }
     // FlowModelBasicUtilitiesFiveEqnAllaire.cpp:5446-7347 (bounds in the .hpp:24-27).
+    // The sound-speed test differs by axis there: x-faces add Y_i Psi_i to
+    // c^2 = Gamma p / rho (:5819, 6091, 6678), y- and z-faces assign
+    // c^2 = Gamma p / rho again in the species loop (:6359, 6992, 7305).
     bool ok = true;
     double Z[NS];
     Z[NS - 1] = double(1);
@@ -256,10 +260,12 @@ __device__ __forceinline__ bool bounded_multi(const double* V, const KParams& k)
     const double p = V[NS + D];
     const double Gamma = mix_gamma_all<NS>(k, Z) - double(1);
     double c2 = quot(Gamma * p, rho, rrho);
+    if constexpr (A == 0) {
 #pragma unroll
-    for (int si = 0; si < NS; si++) {
-        const double Psi = quot(p, rho, rrho);
-        c2 += Y[si] * Psi;
+        for (int si = 0; si < NS; si++) {
+            const double Psi = quot(p, rho, rrho);
+            c2 += Y[si] * Psi;
+        }
     }
     return ok && (c2 > double(0));
 }
@@ -365,7 +371,7 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
     back(Wp, VR);
 
     // Bounds on both sides, else first-order values (WCNS56-HLLC-HLL.cpp:947-1032).
-    if (!(bounded_multi<D, NS, FOUR>(VL, k) && bounded_multi<D, NS, FOUR>(VR, k))) {
+    if (!(bounded_multi<D, NS, FOUR, A>(VL, k) && bounded_multi<D, NS, FOUR, A>(VR, k))) {
 #pragma unroll
         for (int e = 0; e < NEQ; e++) {
             VL[e] = ld(e, 2);

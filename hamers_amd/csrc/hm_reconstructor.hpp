@@ -47,6 +47,46 @@ inline int num_equations(int model, int dim, int ns) {
     }
 }
 
+// Orders the calls that share one scratch region of a context across streams.
+// Each call records an event on its stream after its launches; a call that
+// arrives on another stream first makes that stream wait for the event, so
+// two streams never write the shared scratch at the same time.  Calls on one
+// stream are ordered by the stream itself.
+class StreamOrder {
+   public:
+    StreamOrder() = default;
+    StreamOrder(const StreamOrder&) = delete;
+    StreamOrder& operator=(const StreamOrder&) = delete;
+    ~StreamOrder() {
+        if (ev_) (void)hipEventDestroy(ev_);
+    }
+    // Before the call's first launch on s.
+    hipError_t enter(hipStream_t s) {
+        if (recorded_ && s != last_) return hipStreamWaitEvent(s, ev_, 0);
+        return hipSuccess;
+    }
+    // After the call's last launch on s.
+    hipError_t leave(hipStream_t s) {
+        if (!ev_) {
+            const hipError_t e = hipEventCreateWithFlags(&ev_, hipEventDisableTiming);
+            if (e != hipSuccess) {
+                ev_ = nullptr;
+                return e;
+            }
+        }
+        const hipError_t e = hipEventRecord(ev_, s);
+        if (e != hipSuccess) return e;
+        last_ = s;
+        recorded_ = true;
+        return hipSuccess;
+    }
+
+   private:
+    hipEvent_t ev_ = nullptr;
+    hipStream_t last_ = nullptr;
+    bool recorded_ = false;
+};
+
 // Sizes (in doubles) of one patch's scratch arrays.
 struct ScratchLayout {
     size_t prim, sens, fmid[3], ustar[3], total;
@@ -115,65 +155,13 @@ class ConvectiveFluxReconstructorGPU {
         if (int st = check_view(v, 0)) return st;
         if (int st = select_device()) return st;
         const ScratchLayout L = layout(v);
+        // (wait first: growing the arena synchronizes only this stream)
+        if (arena_order_.enter(s) != hipSuccess) return fail(HM_ERR_DEVICE, "hipStreamWaitEvent failed");
         if (int st = ensure_arena(L.total, s)) return st;
-        DevPatch dp = make_dev_patch(v, arena_, L);
-        Plan<SegSingle> plan{};
-        const int d = cfg_.dim;
-        plan.tile_rows = tile_rows_of(&v, 1);
-        const Extents e = extents(v, plan.tile_rows);
-        // derived cells
-        plan.work[0].p = dp;
-        plan.work[0].nseg = 1;
-        plan.work[0].start[0] = 0;
-        plan.work[0].cnt[0] = e.vol4;
-        plan.work[0].start[1] = align_items(e.vol4);
-        plan.items[0] = plan.work[0].start[1];
-        // sensor cells
-        plan.work[1].p = dp;
-        plan.work[1].nseg = 1;
-        plan.work[1].start[0] = 0;
-        plan.work[1].cnt[0] = 0;  // the derived-data kernels compute the sensor (stage 1 is empty)
-        plan.work[1].start[1] = align_items(plan.work[1].cnt[0]);
-        plan.items[1] = plan.work[1].start[1];
-        // midpoint faces and output faces, one segment per axis
-        for (int stage = 2; stage < 4; stage++) {
-            SegSingle& w = plan.work[stage];
-            w.p = dp;
-            w.nseg = d;
-            int64_t acc = 0;
-            for (int a = 0; a < d; a++) {
-                w.start[a] = acc;
-                w.cnt[a] = (stage == 2) ? e.work2[a] : e.work3[a];
-                acc += align_items(w.cnt[a]);
-            }
-            w.start[d] = acc;
-            plan.items[stage] = acc;
-        }
-        if (fused_) {
-            // The single-species model runs both flux stages in one launch on
-            // one patch: stage 2 takes axis 0's chunks, padded to whole blocks,
-            // and the tiles of axes 1 and 2; stage 3 is empty.
-            SegSingle& w = plan.work[2];
-            int64_t acc = align_up(size_t(e.work2[0]), kBlock);
-            for (int a = 1; a < d; a++) {
-                w.start[a] = acc;
-                w.cnt[a] = e.work3[a];
-                acc += e.work3[a];
-            }
-            w.start[d] = acc;
-            plan.items[2] = acc;
-            plan.items[3] = 0;
-        }
-        // advective source cells
-        plan.work[4].p = dp;
-        plan.work[4].nseg = 1;
-        plan.work[4].start[0] = 0;
-        plan.work[4].cnt[0] = has_advective_ ? e.vol0 : 0;
-        plan.work[4].start[1] = align_items(plan.work[4].cnt[0]);
-        plan.items[4] = plan.work[4].start[1];
+        const Plan<SegSingle> plan = patch_plan(v, arena_, L);
         KParams k = k_;
         k.dt = dt;
-        return run(plan, k, s);
+        return run_ordered(plan, k, s);
     }
 
     int computeConvectiveFluxAndSourceOnPatches(const hm_patch_view* views, int np, double time, double dt,
@@ -187,12 +175,13 @@ class ConvectiveFluxReconstructorGPU {
         if (int st = select_device()) return st;
         size_t total = 0;
         for (int i = 0; i < np; i++) total += layout(views[i]).total;
+        if (arena_order_.enter(s) != hipSuccess) return fail(HM_ERR_DEVICE, "hipStreamWaitEvent failed");
         if (int st = ensure_arena(total, s)) return st;
         TablePlan* tp = nullptr;
         if (int st = get_table_plan(views, np, s, tp)) return st;
         KParams k = k_;
         k.dt = dt;
-        return run(tp->plan, k, s);
+        return run_ordered(tp->plan, k, s);
     }
 
     // Work tables of a fixed patch list over a caller-owned scratch region,
@@ -318,6 +307,66 @@ class ConvectiveFluxReconstructorGPU {
     }
 
    private:
+    // Work of one patch's RHS with its scratch at `base` (layout L).
+    Plan<SegSingle> patch_plan(const hm_patch_view& v, double* base, const ScratchLayout& L) const {
+        const DevPatch dp = make_dev_patch(v, base, L);
+        Plan<SegSingle> plan{};
+        const int d = cfg_.dim;
+        plan.tile_rows = tile_rows_of(&v, 1);
+        const Extents e = extents(v, plan.tile_rows);
+        // derived cells
+        plan.work[0].p = dp;
+        plan.work[0].nseg = 1;
+        plan.work[0].start[0] = 0;
+        plan.work[0].cnt[0] = e.vol4;
+        plan.work[0].start[1] = align_items(e.vol4);
+        plan.items[0] = plan.work[0].start[1];
+        // sensor cells
+        plan.work[1].p = dp;
+        plan.work[1].nseg = 1;
+        plan.work[1].start[0] = 0;
+        plan.work[1].cnt[0] = 0;  // the derived-data kernels compute the sensor (stage 1 is empty)
+        plan.work[1].start[1] = align_items(plan.work[1].cnt[0]);
+        plan.items[1] = plan.work[1].start[1];
+        // midpoint faces and output faces, one segment per axis
+        for (int stage = 2; stage < 4; stage++) {
+            SegSingle& w = plan.work[stage];
+            w.p = dp;
+            w.nseg = d;
+            int64_t acc = 0;
+            for (int a = 0; a < d; a++) {
+                w.start[a] = acc;
+                w.cnt[a] = (stage == 2) ? e.work2[a] : e.work3[a];
+                acc += align_items(w.cnt[a]);
+            }
+            w.start[d] = acc;
+            plan.items[stage] = acc;
+        }
+        if (fused_) {
+            // The single-species model runs both flux stages in one launch on
+            // one patch: stage 2 takes axis 0's chunks, padded to whole blocks,
+            // and the tiles of axes 1 and 2; stage 3 is empty.
+            SegSingle& w = plan.work[2];
+            int64_t acc = align_up(size_t(e.work2[0]), kBlock);
+            for (int a = 1; a < d; a++) {
+                w.start[a] = acc;
+                w.cnt[a] = e.work3[a];
+                acc += e.work3[a];
+            }
+            w.start[d] = acc;
+            plan.items[2] = acc;
+            plan.items[3] = 0;
+        }
+        // advective source cells
+        plan.work[4].p = dp;
+        plan.work[4].nseg = 1;
+        plan.work[4].start[0] = 0;
+        plan.work[4].cnt[0] = has_advective_ ? e.vol0 : 0;
+        plan.work[4].start[1] = align_items(plan.work[4].cnt[0]);
+        plan.items[4] = plan.work[4].start[1];
+        return plan;
+    }
+
     struct Extents {
         int64_t vol4, vol2, vol0, mid[3], face[3];
         int64_t work2[3], work3[3];  // items of the midpoint-flux and face-flux stages per axis
@@ -614,6 +663,14 @@ class ConvectiveFluxReconstructorGPU {
         return HM_OK;
     }
 
+    // run() on the context's arena, then mark the arena in use by stream s.
+    template <class W>
+    int run_ordered(const Plan<W>& plan, const KParams& k, hipStream_t s) {
+        if (int st = run(plan, k, s)) return st;
+        if (arena_order_.leave(s) != hipSuccess) return fail(HM_ERR_DEVICE, "hipEventRecord failed");
+        return HM_OK;
+    }
+
     template <class W>
     int run(const Plan<W>& plan, const KParams& k, hipStream_t s) {
         if (!profiling_) return run_stages(plan, k, s, 0, kStages);
@@ -677,6 +734,7 @@ class ConvectiveFluxReconstructorGPU {
     KParams k_{};
     double* arena_ = nullptr;
     size_t arena_doubles_ = 0;
+    StreamOrder arena_order_;  // calls on different streams take the arena in turn
     std::vector<TablePlan> plans_;
     uint64_t use_clock_ = 0;
     void* stage_buf_ = nullptr;

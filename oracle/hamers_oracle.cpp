@@ -474,7 +474,7 @@ void compute_theta_omega(const Model& m, const int* n, const Derived& D, const d
 // (EquationOfStateIdealGas.cpp:5806, 6135).  When `ustar` is non-null it
 // receives the HLLC normal velocity (FlowModelRiemannSolverSingleSpeciesHLLC.cpp:3213-3220).
 void riemann_face_single(int d, int a, double gam, bool use_hll, const double* VL, const double* VR, double* Fout,
-                         double* ustar) {
+                         double* ustar, int mutation = 0) {
     const int neq = d + 2;
     const double rhoL = VL[0], rhoR = VR[0];
     const double pL = VL[neq - 1], pR = VR[neq - 1];
@@ -577,7 +577,8 @@ void riemann_face_single(int d, int a, double gam, bool use_hll, const double* V
             al1 = t_abs(VR[1 + a] - VL[1 + a]) / vel_mag;
             al2 = std::sqrt(double(1) - al1 * al1);
         }
-        const double be1 = double(1) / double(2) * (double(1) + al1 / (al1 + al2));
+        double be1 = double(1) / double(2) * (double(1) + al1 / (al1 + al2));
+        if (mutation == 8) be1 = double(1) / double(2) * (double(1) + al2 / (al1 + al2));  // planted: alphas swapped
         const double be2 = double(1) - be1;
         Fout[0] = be1 * Fhllc[0] + be2 * Fhll[0];
         for (int b = 0; b < d; b++) {
@@ -617,8 +618,17 @@ inline void prim_from_char_single(int d, int a, double ra, double ca, const doub
 
 // One axis of the WCNS56 driver for the single-species model
 // (WCNS56-HLLC-HLL.cpp:879-1282 in 2D, 1746-2490 in 3D), pass by pass.
+// Test probe of one direction pass: the sensor value and the midpoint flux of
+// every midpoint face (FaceArray index space: i_a in [-1, n_a+1], others
+// interior).  Filled only when passed.
+struct FaceProbe {
+    double* sensor;  // [faces]
+    double* fmid;    // [neq][faces]
+};
+
 void direction_pass_single(const hmo_params& prm, const Model& m, const int* n, int a, const Derived& D,
-                           const CellArray& theta, const CellArray& Omega, double dt, double* F_out) {
+                           const CellArray& theta, const CellArray& Omega, double dt, double* F_out,
+                           FaceProbe* probe = nullptr) {
     const int d = m.dim, neq = m.neq;
     const double gam = m.gamma[0];
     const int sh[3] = {a == 0, a == 1, a == 2};
@@ -703,6 +713,7 @@ void direction_pass_single(const hmo_params& prm, const Model& m, const int* n, 
         fp &= (Vp[0].v[f] > double(0)) ? 1 : 0;
         fp &= (Vp[neq - 1].v[f] > double(0)) ? 1 : 0;
         if (prm.mutation == 3) fm = fp = 1;  // planted mistake: bounds check dropped
+        if (prm.mutation == 5) fm = 0;       // test hook: first-order face states on every face
         if (t_if(fm == 0 || fp == 0)) {
             const int64_t iL = D.V.idx(i - sh[0], j - sh[1], k - sh[2]);
             const int64_t iR = D.V.idx(i, j, k);
@@ -728,14 +739,22 @@ void direction_pass_single(const hmo_params& prm, const Model& m, const int* n, 
             // Ducros-type sensor from the cells either side (WCNS56:1066-1088).
             const int64_t tL = theta.idx(i - sh[0], j - sh[1], k - sh[2]);
             const int64_t tR = theta.idx(i, j, k);
-            const double th_avg = 0.5 * (theta.c(0)[tL] + theta.c(0)[tR]);
-            const double om_avg = 0.5 * (Omega.c(0)[tL] + Omega.c(0)[tR]);
-            const double sensor = -th_avg / (std::fabs(th_avg) + om_avg + EPS);
+            double th_avg = 0.5 * (theta.c(0)[tL] + theta.c(0)[tR]);
+            double om_avg = 0.5 * (Omega.c(0)[tL] + Omega.c(0)[tR]);
+            if (prm.mutation == 7) {  // planted mistake: the right cell's values, no face average
+                th_avg = theta.c(0)[tR];
+                om_avg = Omega.c(0)[tR];
+            }
+            double sensor = -th_avg / (std::fabs(th_avg) + om_avg + EPS);
+            if (prm.mutation == 6) sensor = -sensor;  // planted mistake: sign of the dilatation
+            if (probe) probe->sensor[f] = sensor;
             use_hll = t_if(sensor > 0.65);
         }
         double Fo[5];
-        riemann_face_single(d, a, gam, use_hll, VL, VR, Fo, nullptr);
+        riemann_face_single(d, a, gam, use_hll, VL, VR, Fo, nullptr, prm.mutation);
         for (int e = 0; e < neq; e++) Fmid.c(e)[f] = Fo[e];
+        if (probe)
+            for (int e = 0; e < neq; e++) probe->fmid[e * Fmid.vol() + f] = Fo[e];
     });
 
     // Midpoint-to-face sixth-order flux, scaled by dt (WCNS56:1197-1282, 2327-2490).
@@ -960,9 +979,14 @@ inline void prim_from_char_five(const Five& f, int a, const double* Zra, double 
     for (int si = 0; si < f.nz; si++) V[ns + d + 1 + si] = W[ns + d + si];
 }
 
-// Bounds of one side (FlowModelBasicUtilitiesFiveEqnAllaire.cpp:5446-7347,
-// bounds FlowModelBasicUtilitiesFiveEqnAllaire.hpp:24-27).
-inline int bounded_five(const Five& f, const double* V) {
+// Bounds of one side of a face normal to axis a
+// (FlowModelBasicUtilitiesFiveEqnAllaire.cpp:5446-7347, bounds
+// FlowModelBasicUtilitiesFiveEqnAllaire.hpp:24-27).  The sound-speed check
+// differs by axis in the reference: on x-faces the species loop adds
+// Y_i Psi_i to c^2 = Gamma p / rho (:5819, 6091, 6678), on y- and z-faces it
+// assigns c^2 = Gamma p / rho again (:6359, 6992, 7305), so there the test is
+// Gamma p / rho > 0.
+inline int bounded_five(const Five& f, const double* V, int a) {
     const int ns = f.ns, d = f.d;
     int ok = 1;
     if (f.four) {
@@ -998,8 +1022,12 @@ inline int bounded_five(const Five& f, const double* V) {
     const double Gamma = mix_gamma_all(f, Z) - double(1);
     double c2 = Gamma * p / rho;
     for (int si = 0; si < ns; si++) {
-        const double Psi = p / rho;
-        c2 += Y[si] * Psi;
+        if (a == 0) {
+            const double Psi = p / rho;
+            c2 += Y[si] * Psi;
+        } else {
+            c2 = Gamma * p / rho;
+        }
     }
     ok &= (c2 > double(0)) ? 1 : 0;
     return ok;
@@ -1218,7 +1246,7 @@ void direction_pass_five(const hmo_params& prm, const Five& f, const int* n, int
             VL[e] = Vm[e].v[fc];
             VR[e] = Vp[e].v[fc];
         }
-        int fm = bounded_five(f, VL), fp = bounded_five(f, VR);
+        int fm = bounded_five(f, VL, a), fp = bounded_five(f, VR, a);
         if (prm.mutation == 3) fm = fp = 1;
         if (fm == 0 || fp == 0) {
             const int64_t iL = D.V.idx(i - sh[0], j - sh[1], k - sh[2]);
@@ -1709,6 +1737,47 @@ void hmo_char_roundtrip_single(int dim, int axis, double ra, double ca, const do
 void hmo_riemann_single(int dim, int axis, double gamma, int use_hll, const double* VL, const double* VR, double* F,
                         double* ustar) {
     riemann_face_single(dim, axis, gamma, use_hll != 0, VL, VR, F, ustar);
+}
+
+// The same with a planted mistake (Params.mutation; 8: HLLC-HLL blend weights
+// alpha_1 and alpha_2 swapped), for the tests that show the pins catch it.
+void hmo_riemann_single_mutated(int dim, int axis, double gamma, int use_hll, const double* VL, const double* VR,
+                                double* F, double* ustar, int mutation) {
+    riemann_face_single(dim, axis, gamma, use_hll != 0, VL, VR, F, ustar, mutation);
+}
+
+// Sensor value and midpoint flux of every midpoint face of axis `axis` of one
+// single-species patch (faces i_a in [-1, n_a+1] x interior, x fastest; fmid
+// component-major), from the same pass hmo_rhs_patch runs.
+int hmo_midpoint_probe_single(const hmo_params* prm, const int* n_in, int g, const double* dx, const double* Q,
+                              int axis, double* sensor, double* fmid) {
+    if (!prm || !n_in || !dx || !Q || !sensor || !fmid || prm->model != 0) return 1;
+    Model m = make_model(*prm);
+    if (g < 4 || axis < 0 || axis >= m.dim) return 3;
+    int n[3] = {1, 1, 1};
+    for (int a = 0; a < m.dim; a++) {
+        n[a] = n_in[a];
+        if (n[a] < 1) return 4;
+    }
+    Derived D;
+    compute_derived_single(m, n, g, Q, D);
+    CellArray theta, Omega;
+    compute_theta_omega(m, n, D, dx, theta, Omega);
+    int64_t fvol = 1;
+    for (int b = 0; b < 3; b++) fvol *= (b == axis) ? n[b] + 1 : n[b];
+    std::vector<double> F(size_t(m.neq) * fvol);
+    FaceProbe probe{sensor, fmid};
+    direction_pass_single(*prm, m, n, axis, D, theta, Omega, 1.0, F.data(), &probe);
+    return 0;
+}
+
+// Bounds check of one side of a face normal to `axis`, four- or five-equation
+// model (FlowModelBasicUtilities{FourEqnConservative,FiveEqnAllaire}.cpp):
+// 1 bounded, 0 not, -1 bad arguments.
+int hmo_bounded_multi(const hmo_params* prm, int axis, const double* V) {
+    Five f;
+    if (!prm || !V || (prm->model != 1 && prm->model != 2) || !make_multi(prm, f)) return -1;
+    return bounded_five(f, V, axis);
 }
 
 // Midpoint flux of one face from left/right primitive states, four- or

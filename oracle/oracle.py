@@ -242,17 +242,58 @@ def riemann_multi(prm: Params, axis: int, use_hll: bool, VL, VR):
     return F, us.value
 
 
-def riemann_single(dim: int, axis: int, gamma: float, use_hll: bool, VL, VR):
+def riemann_single(dim: int, axis: int, gamma: float, use_hll: bool, VL, VR, mutation: int = 0):
+    """Midpoint flux of one face, single-species model; ``mutation`` plants a
+    known mistake (8: HLLC-HLL blend weights alpha_1, alpha_2 swapped)."""
     L = lib()
-    L.hmo_riemann_single.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
-                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.hmo_riemann_single_mutated.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_int]
     VL = _arr(VL, dim + 2)
     VR = _arr(VR, dim + 2)
     F = np.zeros(dim + 2)
     us = ctypes.c_double()
-    L.hmo_riemann_single(dim, axis, gamma, int(use_hll), VL.ctypes.data, VR.ctypes.data, F.ctypes.data,
-                         ctypes.byref(us))
+    L.hmo_riemann_single_mutated(dim, axis, gamma, int(use_hll), VL.ctypes.data, VR.ctypes.data, F.ctypes.data,
+                                 ctypes.byref(us), int(mutation))
     return F, us.value
+
+
+def midpoint_probe_single(prm: Params, Q: np.ndarray, n: tuple, g: int, dx: tuple, axis: int):
+    """Sensor value and midpoint flux of every midpoint face of one axis of a
+    single-species patch, from the pass ``rhs_patch`` runs.  Returns
+    (sensor, fmid) shaped like the face box (i_a in [-1, n_a+1], others
+    interior; numpy order z, y, x) and (neq,) + that."""
+    L = lib()
+    L.hmo_midpoint_probe_single.restype = ctypes.c_int
+    L.hmo_midpoint_probe_single.argtypes = [ctypes.POINTER(_Params), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.c_int,
+                                            ctypes.c_void_p, ctypes.c_void_p]
+    d = prm.dim
+    ext = [n[b] + 3 if b == axis else n[b] for b in range(d)]
+    shape = tuple(reversed(ext))
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    sensor = np.zeros(shape)
+    fmid = np.zeros((prm.neq,) + shape)
+    nn = (ctypes.c_int * 3)(*(list(n) + [1] * (3 - d)))
+    dd = (ctypes.c_double * 3)(*(list(dx) + [1.0] * (3 - d)))
+    st = L.hmo_midpoint_probe_single(ctypes.byref(prm.to_c()), nn, g, dd, Q.ctypes.data, axis, sensor.ctypes.data,
+                                     fmid.ctypes.data)
+    if st != 0:
+        raise ValueError(f"oracle midpoint_probe_single status {st}")
+    return sensor, fmid
+
+
+def bounded_multi(prm: Params, axis: int, V) -> bool:
+    """Bounds check of one side of a face normal to ``axis`` (four- or
+    five-equation model)."""
+    L = lib()
+    L.hmo_bounded_multi.restype = ctypes.c_int
+    L.hmo_bounded_multi.argtypes = [ctypes.POINTER(_Params), ctypes.c_int, ctypes.c_void_p]
+    V = _arr(V, prm.neq)
+    st = L.hmo_bounded_multi(ctypes.byref(prm.to_c()), axis, V.ctypes.data)
+    if st < 0:
+        raise ValueError("oracle bounded_multi: bad arguments")
+    return bool(st)
 
 
 def branch_tape(mode: int) -> int:

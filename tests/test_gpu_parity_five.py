@@ -180,3 +180,42 @@ def test_five_eqn_needs_last_volume_fraction(torch_dev):
     pd = _patchdata(torch, Q[:-1].copy(), n, 4, (0.1, 0.1), rec.num_eqn)
     with pytest.raises(ValueError):
         rec.computeConvectiveFluxAndSourceOnPatch(pd, 0.0, 1e-3)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_five_eqn_bounds_by_axis_gpu(torch_dev, dim):
+    """A patch in the Gamma in (-1, 0) window: the reference's bounds check passes
+    on x-faces ((Gamma + 1) p / rho > 0) and fails on y- and z-faces
+    (Gamma p / rho < 0), which then take first-order states."""
+    from test_oracle_five import negative_gamma_patch
+
+    n, g = (12,) * dim, 4
+    Q, dx = negative_gamma_patch(n, g)
+    gpu_vs_oracle(torch_dev, f"five-eqn {dim}D Gamma<0 bounds by axis", Q, n, g, dx, 1e-3, dim, 1)
+
+
+def test_five_eqn_null_last_volume_fraction_status(torch_dev):
+    """Through the C-ABI itself: a five-equation view without the last volume
+    fraction (Q[num_eqn] = NULL) is refused with HM_ERR_INVALID_ARGUMENT and a
+    message naming the array, before any launch."""
+    import ctypes
+
+    from hamers_amd import capi
+
+    torch = torch_dev
+    n = (8, 8)
+    Q = shock_interface(n, 4, seed=2)
+    rec = _rec(2, 1)
+    pd = _patchdata(torch, Q, n, 4, (0.1, 0.1), rec.num_eqn)
+    v = pd.view(rec.num_arrays)
+    neq = rec.num_eqn
+    assert rec.num_arrays == neq + 1
+    v.Q[neq] = None
+    st = capi.lib.hm_compute_convective_flux_and_source_on_patch(rec._ctx, ctypes.byref(v), 0.0, 1e-3, 0, None)
+    assert st == capi.HM_ERR_INVALID_ARGUMENT
+    msg = capi.lib.hm_last_error(rec._ctx).decode()
+    assert f"conservative variable {neq} is NULL" in msg, msg
+    # the batched entry point refuses it the same way
+    arr = (capi.hm_patch_view * 1)(v)
+    st = capi.lib.hm_compute_convective_flux_and_source_on_patches(rec._ctx, arr, 1, 0.0, 1e-3, 0, None)
+    assert st == capi.HM_ERR_INVALID_ARGUMENT

@@ -301,6 +301,37 @@ def test_convergence_rate_wcns5js_2d():
     assert rates[-1] > 4.8
 
 
+# 3D: tests/3D_convergence_test_single_species/convergence_test.py:13-23, 200-205
+# (N = 8, 16, 32, 64 on [-1,1]^3, dt = 0.001 dx, 1, 2, 4, 8 steps),
+# problems/Euler/initial_conditions/ConvergenceSingleSpecies.cpp:104-175 and the
+# exact solution 1 + 1/2 sin(pi (x + y + z - 3 t)).
+def _l2_rates_3d(prm, levels=4):
+    errs = []
+    for lev in range(levels):
+        N = 8 * 2 ** lev
+        Q0, n, dx = W.density_wave_convergence(N, dim=3)
+        dt = 0.001 * dx[0]
+        nsteps = 2 ** lev
+        Q = oracle_rk3(prm, Q0, n, dx, dt, nsteps)
+        X = W.cell_centres(n, (-1.0,) * 3, dx)
+        exact = 1.0 + 0.5 * np.sin(np.pi * (X[0] + X[1] + X[2] - 3.0 * dt * nsteps))
+        e = np.abs(exact - Q[0])
+        errs.append(np.sqrt(np.mean(e * e)))
+    return [np.log2(errs[i - 1] / errs[i]) for i in range(1, len(errs))]
+
+
+def test_convergence_rate_wcns6ld_3d():
+    assert _l2_rates_3d(O.Params(dim=3, scheme=O.WCNS6_LD_HLLC_HLL))[-1] > 5.8
+
+
+def test_convergence_rate_wcns5js_3d():
+    assert _l2_rates_3d(O.Params(dim=3, scheme=O.WCNS5_JS_HLLC_HLL))[-1] > 4.8
+
+
+def test_convergence_3d_catches_dropped_stencil_point():
+    assert _l2_rates_3d(O.Params(dim=3, scheme=O.WCNS6_LD_HLLC_HLL, mutation=1), levels=3)[-1] < 5.8
+
+
 def test_convergence_catches_dropped_stencil_point():
     rates = _l2_rates(O.Params(dim=2, scheme=O.WCNS6_LD_HLLC_HLL, mutation=1), levels=3)
     assert rates[-1] < 5.8

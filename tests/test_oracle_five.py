@@ -20,19 +20,63 @@ def five(dim, scheme=1, gamma=(1.6, 1.4), **kw):
     return O.Params(dim=dim, model=O.FIVE_EQN_ALLAIRE, num_species=2, scheme=scheme, gamma=list(gamma), **kw)
 
 
-def _l2_rates_five(prm, levels=4):
+def _l2_rates_five(prm, levels=4, dim=2):
+    """2D: 8, 16, 32, 64 cells per axis with 8, 16, 32, 64 steps; 3D
+    (tests/3D_convergence_test_five_eqn_allaire/convergence_test.py:18-23):
+    the same sizes with 1, 2, 4, 8 steps; exact Z_1 = 1/2 + 1/4 sin(pi (sum x - d t))
+    (problems/Euler/error_statistics/ConvergenceFiveEqnAllaire.cpp:85, 199)."""
     errs = []
     for lev in range(levels):
         N = 8 * 2 ** lev
-        Q0, n, dx = W.interface_advection_convergence(N, dim=2)
+        Q0, n, dx = W.interface_advection_convergence(N, dim=dim)
         dt = 0.001 * dx[0]
-        nsteps = 8 * 2 ** lev
+        nsteps = (8 if dim == 2 else 1) * 2 ** lev
         Q = oracle_rk3(prm, Q0, n, dx, dt, nsteps)
-        X = W.cell_centres(n, (-1.0, -1.0), dx)
-        exact = 0.5 + 0.25 * np.sin(np.pi * (X[0] + X[1] - 2.0 * dt * nsteps))
-        e = np.abs(exact - Q[2 + 2 + 1])  # Z_1 slot: ns + d + 1
-        errs.append(np.sqrt(np.sum(e * e) / (N * N)))
+        X = W.cell_centres(n, (-1.0,) * dim, dx)
+        exact = 0.5 + 0.25 * np.sin(np.pi * (sum(X) - dim * dt * nsteps))
+        e = np.abs(exact - Q[2 + dim + 1])  # Z_1 slot: ns + d + 1
+        errs.append(np.sqrt(np.mean(e * e)))
     return [np.log2(errs[i - 1] / errs[i]) for i in range(1, len(errs))]
+
+
+def test_convergence_rate_wcns6ld_five_eqn_3d():
+    assert _l2_rates_five(five(3, 1), dim=3)[-1] > 5.8
+
+
+def test_convergence_rate_wcns5js_five_eqn_3d():
+    assert _l2_rates_five(five(3, 0), dim=3)[-1] > 4.8
+
+
+def _mixture_Gamma(Z1, gamma=(1.6, 1.4)):
+    """Gamma = gamma_mix - 1 = 1 / sum_i Z_i / (gamma_i - 1) (isobaric mixing)."""
+    return 1.0 / (Z1 / (gamma[0] - 1) + (1 - Z1) / (gamma[1] - 1))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_bounds_sound_speed_check_differs_by_axis(dim):
+    """FlowModelBasicUtilitiesFiveEqnAllaire.cpp: on x-faces c^2 = Gamma p / rho
+    + sum Y_i Psi_i = (Gamma + 1) p / rho (Psi_i = p / rho, :5819, 6091, 6678); on
+    y- and z-faces the species loop assigns c^2 = Gamma p / rho (:6359, 6992,
+    7305).  With Gamma in (-1, 0) the two tests disagree in sign."""
+    prm = five(dim)
+    Z1 = 6.0  # inside the +-1000 bound; Gamma = 1 / (6/0.6 - 5/0.4) = -0.4 for gamma = (1.6, 1.4)
+    Gam = _mixture_Gamma(Z1)
+    assert -1.0 < Gam < 0.0
+    vel = [0.1, 0.2, -0.1][:dim]
+    for p in (1.0, -1.0):
+        V = np.array([0.5, 0.5] + vel + [p, Z1])
+        x_ok = (Gam + 1.0) * p > 0.0
+        yz_ok = Gam * p > 0.0
+        assert x_ok != yz_ok
+        assert O.bounded_multi(prm, 0, V) == x_ok
+        for a in range(1, dim):
+            assert O.bounded_multi(prm, a, V) == yz_ok
+    # a state with Gamma > 0 is bounded on every axis either way
+    V = np.array([0.5, 0.5] + vel + [1.0, 0.3])
+    assert all(O.bounded_multi(prm, a, V) for a in range(dim))
+    # and the other bounds still apply: Z beyond +-1000, a negative partial density
+    assert not any(O.bounded_multi(prm, a, np.array([0.5, 0.5] + vel + [1.0, 1001.0])) for a in range(dim))
+    assert not any(O.bounded_multi(prm, a, np.array([-0.1, 0.5] + vel + [1.0, 0.3])) for a in range(dim))
 
 
 def test_convergence_rate_wcns6ld_five_eqn_2d():
@@ -129,3 +173,37 @@ def test_five_eqn_source_is_z_times_divergence_of_hllc_velocity():
     Zi = Z1[g:-g, g:-g]
     assert np.allclose(S[5], dt * Zi * 1e-3, rtol=1e-6)
     assert np.all(S[:5] == 0.0)
+
+
+def negative_gamma_patch(n, g, Z1=-3.0, gamma=(1.4, 1.67)):
+    """Five-equation patch with a uniform volume fraction Z_1 in the window where
+    Gamma = gamma_mix - 1 lies in (-1, 0) (Z_1 = -3 for gamma = (1.4, 1.67)),
+    smooth positive partial densities and pressure, varying velocity."""
+    d = len(n)
+    dx = tuple(1.0 / v for v in n)
+    X = W.cell_centres(tuple(v + 2 * g for v in n), tuple(-g * h for h in dx), dx)
+    s = sum(np.sin(2 * np.pi * (a + 1) * X[a]) for a in range(d))
+    Zr = [0.5 * (1.0 + 0.1 * s), 0.4 * (1.0 - 0.1 * s)]
+    vel = [0.3 * np.cos(2 * np.pi * X[(a + 1) % d]) for a in range(d)]
+    p = 1.0 + 0.2 * s
+    Zf = np.full(X[0].shape, Z1)
+    return W.conservative_from_primitive_five(Zr, vel, p, [Zf, 1.0 - Zf], gamma), dx
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_patch_fallback_on_yz_faces_only_when_gamma_negative(dim):
+    """On this patch every face state has (Gamma + 1) p / rho > 0 but
+    Gamma p / rho < 0: the x-faces pass the reference's bounds check and keep
+    the high-order states, the y- and z-faces fail it and fall back to first
+    order.  So dropping the bounds check (planted mutation 3) changes the y and z
+    fluxes and leaves the x flux bit for bit."""
+    gamma = (1.4, 1.67)
+    assert -1.0 < _mixture_Gamma(-3.0, gamma) < 0.0
+    n, g = (10,) * dim, 4
+    Q, dx = negative_gamma_patch(n, g)
+    F, _ = O.rhs_patch(five(dim, 1, gamma), Q, n, g, dx, 1e-3)
+    Fm, _ = O.rhs_patch(five(dim, 1, gamma, mutation=3), Q, n, g, dx, 1e-3)
+    assert all(np.all(np.isfinite(f)) for f in F)
+    assert np.array_equal(F[0], Fm[0])
+    for a in range(1, dim):
+        assert not np.allclose(F[a], Fm[a], rtol=1e-10, atol=0.0)
