@@ -1,30 +1,25 @@
 """Benchmark of the convective-flux RHS on one MI355X.
 
-Workload (one "step" = one RHS evaluation of a whole AMR level): 2D
-single-species Euler (gamma 1.4), FP64, a 2048^2 periodic domain split into
-256 patches of 128^2 with 4-cell halos, four-quadrant Riemann data (seeded)
-with 1e-3 white noise, WCNS5-JS + hybrid HLLC/HLLC-HLL.  Inputs are resident
-in HBM before the timed region.  The batched path (one work-list launch per
-pipeline stage for the whole level) is the headline; the per-patch path (one
-launch sequence per patch, the plain drop-in) is timed beside it, and the
-CPU oracle is timed on a bounded sample of patches on the host cores.
+Headline workload (one "step" = one RHS evaluation of a whole AMR level): 3D
+five-equation (Allaire) two-gas Euler, FP64, a Mach-1.5 shock in gas 1
+(gamma 1.4) approaching a sinusoidally perturbed interface with gas 2
+(gamma 1.67), WCNS6-LD + hybrid HLLC/HLLC-HLL, on a level tiled by uneven
+patches whose extents per axis are drawn (seeded) from {8, 16, 24, 32, 48, 64}:
+14^3 = 2744 patches, 68.7 M cells, 4-cell halos from the analytic field.  The
+fields are generated on the GPU with torch (synthetic input; every RHS runs
+in libhamers_amd.so) and are resident in HBM before the timed region.  The
+batched path (one work-list launch per pipeline stage for the whole level) is
+the headline; the per-patch path (one launch sequence per patch, the plain
+drop-in) is timed beside it; one SSP-RK3 step of the same level through the
+level integrator with the step captured in a hipGraph is timed too; the CPU
+oracle is timed on a bounded sample of the level's patches on the host cores.
 
-A second workload is timed beside it and reported under "uneven_3d": 3D
-five-equation (Allaire) two-gas Euler, a Mach-1.5 shock in gas 1 (gamma 1.4)
-approaching a perturbed interface with gas 2 (gamma 1.67), WCNS6-LD + hybrid
-HLLC/HLLC-HLL, on a level tiled by uneven patches whose extents per axis are
-drawn (seeded) from {8, 16, 24, 32, 48, 64}; batched RHS against the
-per-patch path.  Its fields are generated on the GPU with torch (synthetic
-input only; every RHS runs in libhamers_amd.so).
-
-A third leg, "navier_stokes_3d", times the convective + diffusive RHS of a
-3D four-equation two-species Navier-Stokes level of uneven patches (multimode
-interface, constant transport properties), batched against per-patch calls.
-
-A fourth leg, reported under "adjoint", times the discrete adjoint of 200
-SSP-RK3 steps on a 512^2 Kelvin-Helmholtz level (16 patches of 128^2,
-checkpoints every 20 steps) against the forward run, and checks its gradient
-w.r.t. 8 mode amplitudes against central differences.
+Secondary legs: "rhs_2d" (2D single-species four-quadrant Riemann problem,
+2048^2 in 256 patches of 128^2, WCNS5-JS, batched vs per-patch),
+"navier_stokes_3d" (3D four-equation Navier-Stokes level of uneven patches,
+convective + diffusive RHS, batched vs per-patch) and "adjoint" (discrete
+adjoint of 200 SSP-RK3 steps on a 512^2 Kelvin-Helmholtz level, checkpoints
+every 20 steps, gradient w.r.t. 8 mode amplitudes).
 
 Prints one JSON line (rank 0).
 """
@@ -34,6 +29,7 @@ import argparse
 import ctypes
 import json
 import os
+import platform
 import sys
 import time
 
@@ -41,31 +37,42 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
-# Kernels of the single-species pipeline's two flux stages.
-STAGE_KERNEL = {"midflux": "k_flux_single", "faceflux": "k_flux_tile_single"}
-# DRAM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0) of each flux
-# kernel, 2048^2 in 128^2 patches, WCNS5-JS.
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix)
+# Counted per-launch figures of the headline's dominant kernel (rocprofv3 on
+# the default headline workload; profiles/ holds the summaries and commands):
+# FP64 operations per launch (64 lanes x (ADD + MUL + TRANS + 2 FMA) wave
+# instructions of SQ_INSTS_VALU_*_F64), DRAM bytes per launch (FETCH_SIZE x 2.0
+# + WRITE_SIZE x 1.0, factors from a copy of known size) and the fraction of
+# the VALU issue peak its SQ_INSTS_VALU reaches.  None until measured.
+HEADLINE_KERNEL_COUNTS = {}
+# The same for the 2D single-species leg's flux kernels (2048^2 in 128^2
+# patches, WCNS5-JS): DRAM bytes per launch.
 FLUX_TRAFFIC_2048_WCNS5 = {"midflux": 354.92e6 + 135.27e6, "faceflux": 506.47e6 + 135.27e6}
-FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector
+STAGE_KERNEL_SINGLE = {"midflux": "k_flux_single", "faceflux": "k_flux_tile_single"}
+STAGE_KERNEL_MULTI = {"derived": "k_derived_multi", "midflux": "k_midflux_multi", "faceflux": "k_faceflux_multi",
+                      "source": "k_source_five"}
+UNEVEN_CHOICES = (8, 16, 24, 32, 48, 64)
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    # The GPU's clocks ramp over the first few milliseconds of load; ten
-    # untimed RHS evaluations (about 8 ms) keep the ramp out of the timing.
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--N", type=int, default=2048)
-    ap.add_argument("--patch", type=int, default=128)
-    ap.add_argument("--scheme", default="WCNS5_JS_HLLC_HLL")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work")
+    ap.add_argument("--steps", type=int, default=20, help="timed RHS evaluations of the headline")
+    # The GPU's clocks ramp over the first milliseconds of load: warm-up RHS
+    # evaluations keep the ramp out of the timing.
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--seg", type=int, default=14, help="patches per axis of the headline's uneven 3D level")
+    ap.add_argument("--rk-steps", type=int, default=3, help="timed SSP-RK3 steps of the headline level (0: skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work (multi-thread run)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-patch", action="store_true")
-    ap.add_argument("--uneven-seg", type=int, default=14, help="patches per axis of the uneven 3D level (0: skip)")
-    ap.add_argument("--uneven-steps", type=int, default=5)
+    ap.add_argument("--headline-only", action="store_true", help="skip the secondary legs")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only copy this many MiB device to device 5 times (calibrates HBM byte counters) and exit")
+    ap.add_argument("--N", type=int, default=2048, help="2D leg: cells per axis")
+    ap.add_argument("--patch", type=int, default=128, help="2D leg: patch size")
+    ap.add_argument("--scheme", default="WCNS5_JS_HLLC_HLL", help="2D leg: scheme")
+    ap.add_argument("--steps-2d", type=int, default=30)
     ap.add_argument("--ns-seg", type=int, default=8, help="patches per axis of the uneven 3D Navier-Stokes level (0: skip)")
     ap.add_argument("--ns-steps", type=int, default=4)
     ap.add_argument("--adjoint-steps", type=int, default=200, help="RK3 steps of the adjoint leg (0: skip)")
@@ -75,12 +82,44 @@ def parse():
     return ap.parse_args()
 
 
-UNEVEN_CHOICES = (8, 16, 24, 32, 48, 64)
+def host_description():
+    """nproc, the CPU model (from /proc/cpuinfo, as lscpu prints it) and the
+    OpenMP thread count of this host."""
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model, "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def timed_events(torch, stream, f, steps, warmup):
+    """Average ms per call of f over `steps` calls after `warmup`, from HIP
+    events on `stream`, bracketed by device synchronizations."""
+    for _ in range(warmup):
+        f()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / steps, wall / steps * 1e3
 
 
 def shock_interface_patch_torch(torch, lo, n, g, dx, modes, gamma=(1.4, 1.67), rho_gas=(1.0, 0.2)):
-    """Conservative five-equation arrays (8, n2+2g, n1+2g, n0+2g) of one patch of the
-    shock / perturbed-interface field, halos from the same analytic field."""
+    """Conservative five-equation arrays (8, n2+2g, n1+2g, n0+2g) of the box of
+    cells lo - g .. lo + n + g - 1 of the shock / perturbed-interface field
+    (one patch with halos from the same analytic field, or the whole level
+    grown by its halo)."""
     dev = "cuda:0"
     ax = [(torch.arange(n[a] + 2 * g, dtype=torch.float64, device=dev) + (lo[a] - g) + 0.5) * dx[a]
           for a in range(3)]
@@ -107,43 +146,221 @@ def shock_interface_patch_torch(torch, lo, n, g, dx, modes, gamma=(1.4, 1.67), r
     return torch.stack([Zr1, Zr2, rho * u, zero, zero, E, Z1, Z2]).contiguous()
 
 
-def bench_uneven_3d(args, torch, np, time):
-    """Batched vs per-patch RHS of the uneven-patch 3D five-equation level."""
-    import ctypes
+class UnevenLevel:
+    """The headline's level: 3D five-equation two-gas Euler, shock and perturbed
+    interface, seg^3 uneven patches (extents per axis from UNEVEN_CHOICES,
+    seeded), 4-cell halos from the analytic field, on the GPU."""
 
-    from hamers_amd import ConvectiveFluxReconstructor, PatchData, PatchList, capi
+    def __init__(self, args, torch, np):
+        from hamers_amd import ConvectiveFluxReconstructor, PatchData, PatchList
 
-    rng = np.random.default_rng(2024)
-    segs = [rng.choice(UNEVEN_CHOICES, size=args.uneven_seg) for _ in range(3)]
-    starts = [np.concatenate([[0], np.cumsum(s)[:-1]]) for s in segs]
-    Ng = [int(s.sum()) for s in segs]
-    dx = tuple(1.0 / v for v in Ng)
-    modes = [(rng.uniform(0.0, 1.0), rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi)) for _ in range(7)]
-    tot = sum(m[0] for m in modes)
-    modes = [(m[0] / tot, m[1], m[2]) for m in modes]
-    g, neq = 4, 7
-    rec = ConvectiveFluxReconstructor("uneven", 3, "FIVE_EQN_ALLAIRE", "WCNS6_LD_HLLC_HLL", num_species=2,
-                                      species_gamma=(1.4, 1.67), species_R=(1.0, 1.0))
-    dev = "cuda:0"
-    pds, boxes = [], []
-    for k in range(args.uneven_seg):
-        for j in range(args.uneven_seg):
-            for i in range(args.uneven_seg):
-                lo = (int(starts[0][i]), int(starts[1][j]), int(starts[2][k]))
-                n = (int(segs[0][i]), int(segs[1][j]), int(segs[2][k]))
-                Q = shock_interface_patch_torch(torch, lo, n, g, dx, modes)
-                F = [torch.empty((neq, n[2] + (a == 2), n[1] + (a == 1), n[0] + (a == 0)), dtype=torch.float64,
-                                 device=dev) for a in range(3)]
-                S = torch.zeros((neq, n[2], n[1], n[0]), dtype=torch.float64, device=dev)
-                pds.append(PatchData(n, (g, g, g), dx, Q, F, S))
-                boxes.append((lo, n))
-    cells = sum(b[1][0] * b[1][1] * b[1][2] for b in boxes)
-    plist = PatchList(pds, rec.num_arrays)
-    dt = 1.0e-4
+        rng = np.random.default_rng(2024)
+        self.seg = args.seg
+        self.segs = [rng.choice(UNEVEN_CHOICES, size=args.seg) for _ in range(3)]
+        starts = [np.concatenate([[0], np.cumsum(s)[:-1]]) for s in self.segs]
+        self.Ng = [int(s.sum()) for s in self.segs]
+        self.dx = tuple(1.0 / v for v in self.Ng)
+        modes = [(rng.uniform(0.0, 1.0), rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi)) for _ in range(7)]
+        tot = sum(m[0] for m in modes)
+        self.modes = [(m[0] / tot, m[1], m[2]) for m in modes]
+        self.g, self.neq = 4, 7
+        self.rec = ConvectiveFluxReconstructor("uneven", 3, "FIVE_EQN_ALLAIRE", "WCNS6_LD_HLLC_HLL", num_species=2,
+                                               species_gamma=(1.4, 1.67), species_R=(1.0, 1.0))
+        dev = "cuda:0"
+        self.pds, self.boxes = [], []
+        g, neq = self.g, self.neq
+        # the field over the whole level grown by the halo, one evaluation; each
+        # patch takes its ghost box out of it (the same values as evaluating
+        # the field per patch: the coordinates are formed from the global index)
+        whole = shock_interface_patch_torch(torch, (0, 0, 0), tuple(self.Ng), g, self.dx, self.modes)
+        for k in range(args.seg):
+            for j in range(args.seg):
+                for i in range(args.seg):
+                    lo = (int(starts[0][i]), int(starts[1][j]), int(starts[2][k]))
+                    n = (int(self.segs[0][i]), int(self.segs[1][j]), int(self.segs[2][k]))
+                    Q = whole[:, lo[2]:lo[2] + n[2] + 2 * g, lo[1]:lo[1] + n[1] + 2 * g,
+                              lo[0]:lo[0] + n[0] + 2 * g].contiguous()
+                    F = [torch.empty((neq, n[2] + (a == 2), n[1] + (a == 1), n[0] + (a == 0)), dtype=torch.float64,
+                                     device=dev) for a in range(3)]
+                    S = torch.zeros((neq, n[2], n[1], n[0]), dtype=torch.float64, device=dev)
+                    self.pds.append(PatchData(n, (g, g, g), self.dx, Q, F, S))
+                    self.boxes.append((lo, n))
+        del whole
+        torch.cuda.empty_cache()
+        self.cells = int(sum(b[1][0] * b[1][1] * b[1][2] for b in self.boxes))
+        self.plist = PatchList(self.pds, self.rec.num_arrays)
+        self.dt = 1.0e-4
+
+    def describe(self, np):
+        ext = np.bincount(np.concatenate(self.segs), minlength=65)[list(UNEVEN_CHOICES)].tolist()
+        return {"workload": "3D five-equation Allaire two-gas Euler, Mach-1.5 shock and perturbed interface, "
+                            "WCNS6_LD_HLLC_HLL, uneven patches (extents per axis from {8,16,24,32,48,64}), "
+                            "batched RHS of the whole level",
+                "patches": len(self.pds), "cells": self.cells, "domain": self.Ng, "extent_counts": ext,
+                "scheme": "WCNS6_LD_HLLC_HLL", "model": "FIVE_EQN_ALLAIRE", "num_species": 2}
+
+
+def bench_headline(args, torch, np, L):
+    """Batched RHS of the uneven level (headline), stage times with HIP events
+    on the launch stream, the per-patch path beside it."""
+    from hamers_amd import capi
+
+    rec = L.rec
     stream = torch.cuda.current_stream()
     sp = ctypes.c_void_p(stream.cuda_stream)
     fn = capi.lib.hm_compute_convective_flux_and_source_on_patch
-    views = [pd.view(rec.num_arrays) for pd in pds]
+    views = [pd.view(rec.num_arrays) for pd in L.pds]
+
+    def batched():
+        rec.computeConvectiveFluxAndSourceOnPatches(L.plist, 0.0, L.dt, 0, stream)
+
+    def per_patch():
+        for v in views:
+            st = fn(rec._ctx, ctypes.byref(v), 0.0, L.dt, 0, sp)
+            if st:
+                capi.check(rec._ctx, st)
+
+    ms, wall = timed_events(torch, stream, batched, args.steps, args.warmup)
+    rec.enable_profiling(True)
+    for _ in range(args.steps):
+        batched()
+    prof = rec.read_profile()
+    rec.enable_profiling(False)
+    stage = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in prof.items()}
+    ms_p = None
+    if not args.no_per_patch:
+        ms_p, _ = timed_events(torch, stream, per_patch, max(2, args.steps // 4), 1)
+    return ms, wall, stage, ms_p
+
+
+def headline_roofline(L, stage, ms):
+    """Roofline of the dominant kernel of the headline RHS.  Its bound is FP64
+    VALU issue (counters: profiles/), so the figure is its FP64 operation rate
+    against MI355X's FP64 peak; the HBM figures are reported beside it: the
+    compulsory bytes of the whole RHS (SURVEY-style count) per RHS time, and the
+    kernel's measured DRAM traffic per launch."""
+    neq, nadv = L.neq, 1
+    rhs_bytes_per_cell = 8 * (neq + 3 * neq + 2 * nadv)  # Q read, F on 3 axes, S read-modify-write
+    dominant = max(stage, key=stage.get)
+    kernel = STAGE_KERNEL_MULTI.get(dominant, dominant)
+    kms = stage[dominant]
+    c = HEADLINE_KERNEL_COUNTS.get(kernel)
+    rhs_gbs = rhs_bytes_per_cell * L.cells / (ms * 1e-3) / 1e9
+    out = {"kernel": kernel, "kernel_ms": kms, "rhs_bytes_per_cell": rhs_bytes_per_cell,
+           "rhs_hbm": {"achieved": rhs_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rhs_gbs / HBM_PEAK_GBS}}
+    if c is not None and (L.seg, L.cells) == (14, c["cells"]):
+        tflops = c["fp64_ops"] / (kms * 1e-3) / 1e12
+        out.update({"bound": "valu", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": tflops / FP64_PEAK_TFLOPS, "traffic": c["dram_bytes"],
+                    "valu_issue_frac": c["valu_issue_frac"], "fp64_ops_per_launch": c["fp64_ops"],
+                    "counts_source": c["source"],
+                    "dram_gbs": c["dram_bytes"] / (kms * 1e-3) / 1e9})
+    else:
+        # without counted figures for this workload: the compulsory-byte view only
+        out.update({"bound": "hbm", "achieved": rhs_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": rhs_gbs / HBM_PEAK_GBS, "traffic": None})
+    return out
+
+
+def bench_level_rk3(args, torch, np, L):
+    """SSP-RK3 steps of the headline level through the level integrator
+    (hm_level_advance), one step captured in a hipGraph and replayed."""
+    from hamers_amd.integrator import Level
+
+    g = L.g
+    interiors = [pd.Q[:, g:-g, g:-g, g:-g].cpu().numpy() for pd in L.pds]
+    # the RHS leg's arrays are no longer needed: free them before the level's storage
+    L.plist = None
+    L.pds = None
+    torch.cuda.empty_cache()
+    lvl = Level(L.rec, L.boxes, ((0, 0, 0), tuple(L.Ng)), (1, 1, 1), L.dx)
+    for i, Qi in enumerate(interiors):
+        lvl.write_patch(i, Qi)
+    del interiors
+    dt = 2.0e-5
+    lvl.advance(dt, 1, use_graph=True)  # capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lvl.advance(dt, args.rk_steps, use_graph=True)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.rk_steps
+    ok = bool(np.all(np.isfinite(lvl.read_patch(0))))
+    del lvl
+    return {"ms_per_step": t * 1e3, "steps": args.rk_steps, "dt": dt, "graph": True,
+            "cell_rhs_per_s": 3 * L.cells / t, "finite": ok,
+            "note": "one SSP-RK3 step = ghost fill + 3 batched RHS + 3 stage updates, replayed from a hipGraph"}
+
+
+def cpu_baseline_uneven(args, np, L):
+    """The CPU oracle (C++ restatement, -O3 -fopenmp, OpenMP over patches) on a
+    bounded sample of the headline level's patches: every 28th patch, on the
+    host's OpenMP thread count, then a smaller sample on one thread."""
+    from oracle import oracle as O
+
+    prm = O.Params(dim=3, model=O.FIVE_EQN_ALLAIRE, num_species=2, scheme=O.WCNS6_LD_HLLC_HLL, gamma=[1.4, 1.67])
+
+    def sample(stride):
+        out = []
+        for i in range(0, len(L.pds), stride):
+            pd = L.pds[i]
+            n = pd.n
+            out.append({"Q": pd.Q.cpu().numpy(), "n": n, "g": L.g, "dx": L.dx,
+                        "F": [np.empty(O.face_shape(L.neq, n, a)) for a in range(3)],
+                        "S": np.zeros((L.neq,) + tuple(reversed(n)))})
+        return out, sum(int(np.prod(p["n"])) for p in out)
+
+    host = host_description()
+    nthreads = O.max_threads()
+    big, cells_big = sample(28)
+    t0 = time.perf_counter()
+    O.rhs_batch(prm, big, L.dt, nthreads)
+    t1 = time.perf_counter() - t0
+    reps = max(1, int(round(args.cpu_seconds / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        O.rhs_batch(prm, big, L.dt, nthreads)
+    tk = time.perf_counter() - t0
+    small, cells_small = sample(280)
+    t0 = time.perf_counter()
+    O.rhs_batch(prm, small, L.dt, 1)
+    t_one = time.perf_counter() - t0
+    v_multi = reps * cells_big / tk
+    v_one = cells_small / t_one
+    return {"value": v_multi, "unit": "cell-RHS-evals/s", "cores": nthreads, "kind": "port",
+            "sample": f"{len(big)} of the level's {len(L.pds)} patches (every 28th, {cells_big} cells), "
+                      f"{reps} RHS evaluations in {tk:.1f} s; oracle built -O3 -fopenmp -ffp-contract=off, "
+                      f"{nthreads} OpenMP threads over patches (the GPU box's CPU share); 1-thread run on every "
+                      f"280th patch ({cells_small} cells)",
+            "one_thread_value": v_one, "host": host,
+            "full_host_estimate": v_one * (host["nproc"] or 1)}
+
+
+def bench_2d(args, torch, np):
+    """2D single-species four-quadrant Riemann level: batched vs per-patch RHS,
+    stage times, the flux kernels' HBM roofline (measured traffic)."""
+    from hamers_amd import ConvectiveFluxReconstructor, PatchData, PatchList, capi
+    from hamers_amd import workloads as W
+
+    dev = "cuda:0"
+    N, P, g = args.N, args.patch, 4
+    Qi, _, dx = W.four_quadrant_riemann(N=N, seed=1234)
+    patches = W.split_patches(Qi, (P, P), g)
+    neq = 4
+    rec = ConvectiveFluxReconstructor("bench2d", 2, "SINGLE_SPECIES", args.scheme, species_gamma=(1.4,))
+    pds = []
+    for _, Qp in patches:
+        Qd = torch.from_numpy(Qp).to(dev)
+        F = [torch.empty((neq, P, P + 1), dtype=torch.float64, device=dev),
+             torch.empty((neq, P + 1, P), dtype=torch.float64, device=dev)]
+        S = torch.zeros((neq, P, P), dtype=torch.float64, device=dev)
+        pds.append(PatchData((P, P), (g, g), dx, Qd, F, S))
+    cells = N * N
+    dt = 1.0e-4
+    stream = torch.cuda.current_stream()
+    plist = PatchList(pds, neq)
+    views = [pd.view(neq) for pd in pds]
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    fn = capi.lib.hm_compute_convective_flux_and_source_on_patch
 
     def batched():
         rec.computeConvectiveFluxAndSourceOnPatches(plist, 0.0, dt, 0, stream)
@@ -154,38 +371,32 @@ def bench_uneven_3d(args, torch, np, time):
             if st:
                 capi.check(rec._ctx, st)
 
-    def timed(f, steps):
-        f()
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(steps):
-            f()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / steps
-
-    ms_b = timed(batched, args.uneven_steps)
+    ms, _ = timed_events(torch, stream, batched, args.steps_2d, 10)
     rec.enable_profiling(True)
-    for _ in range(args.uneven_steps):
+    for _ in range(args.steps_2d):
         batched()
     prof = rec.read_profile()
     rec.enable_profiling(False)
     stage = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in prof.items()}
-    ms_p = timed(per_patch, max(2, args.uneven_steps // 2)) if not args.no_per_patch else None
-    bytes_per_cell = 8 * (neq + 3 * neq + 2 * 1)  # Q read, F on 3 axes, S read-modify-write (1 advective eq)
-    ext = [np.bincount(np.concatenate(segs), minlength=65)[list(UNEVEN_CHOICES)].tolist()]
-    return {
-        "workload": "3D five-equation Allaire two-gas Euler, Mach-1.5 shock and perturbed interface, "
-                    "WCNS6_LD_HLLC_HLL, uneven patches (extents per axis from {8,16,24,32,48,64})",
-        "patches": len(pds), "cells": int(cells), "domain": Ng, "extent_counts": ext[0],
-        "ms_per_rhs_batched": ms_b, "value_batched": cells / (ms_b * 1e-3),
-        "ms_per_rhs_per_patch": ms_p, "value_per_patch": (cells / (ms_p * 1e-3)) if ms_p else None,
-        "batched_speedup": (ms_p / ms_b) if ms_p else None,
-        "stage_ms": stage, "bytes_per_cell": bytes_per_cell,
-        "rhs_hbm_frac": bytes_per_cell * cells / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS,
-    }
+    dominant = max(stage, key=stage.get)
+    ms_p = None if args.no_per_patch else timed_events(torch, stream, per_patch, max(3, args.steps_2d // 4), 1)[0]
+    # each flux kernel covers one axis: Q read for the node fluxes, F of its axis written, 8 (4 + 4) B per cell
+    kernel_bytes = 8 * (neq + neq) * cells
+    kgbs = kernel_bytes / (stage[dominant] * 1e-3) / 1e9
+    traffic = None
+    if (args.N, args.patch, args.scheme) == (2048, 128, "WCNS5_JS_HLLC_HLL"):
+        traffic = FLUX_TRAFFIC_2048_WCNS5.get(dominant)
+    return {"workload": "2D single-species Euler four-quadrant Riemann, 2048^2 in 256 patches of 128^2, "
+                        "4-cell halos, WCNS5_JS_HLLC_HLL, batched RHS" if (N, P) == (2048, 128) else
+                        f"2D single-species four-quadrant Riemann, {N}^2 in patches of {P}^2, {args.scheme}",
+            "cells": cells, "patches": len(pds), "ms_per_rhs_batched": ms, "value_batched": cells / (ms * 1e-3),
+            "stage_ms": stage, "ms_per_rhs_per_patch": ms_p,
+            "batched_speedup": (ms_p / ms) if ms_p else None,
+            "flux_kernel": {"kernel": STAGE_KERNEL_SINGLE.get(dominant, dominant), "kernel_ms": stage[dominant],
+                            "bytes_per_cell": 8 * (neq + neq), "achieved_gbs": kgbs, "hbm_frac": kgbs / HBM_PEAK_GBS,
+                            "traffic": traffic, "bound": "valu",
+                            "counts_source": "profiles/flux_kernels_2048sq_counters.md (73-77 % of VALU issue)"},
+            "rhs_hbm_frac": 8 * (neq + 2 * neq) * cells / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
 def rtm_interface_patch_torch(torch, lo, n, g, dx, modes, gamma=(1.4, 1.67), R=(1.0, 0.25), rho_gas=(3.0, 1.0)):
@@ -381,160 +592,50 @@ def main():
         print(json.dumps({"copy_probe_bytes": src.numel() * 8, "copies": 5}), flush=True)
         return
 
-    from hamers_amd import ConvectiveFluxReconstructor, PatchData, PatchList, capi
-    from hamers_amd import workloads as W
-
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    dev = "cuda:0"
-    N, P, g = args.N, args.patch, 4
-    Qi, n_glob, dx = W.four_quadrant_riemann(N=N, seed=1234)
-    patches = W.split_patches(Qi, (P, P), g)
-    neq = 4
-    rec = ConvectiveFluxReconstructor("bench", 2, "SINGLE_SPECIES", args.scheme, species_gamma=(1.4,))
-    pds = []
-    for _, Qp in patches:
-        Qd = torch.from_numpy(Qp).to(dev)
-        F = [torch.empty((neq, P, P + 1), dtype=torch.float64, device=dev),
-             torch.empty((neq, P + 1, P), dtype=torch.float64, device=dev)]
-        S = torch.zeros((neq, P, P), dtype=torch.float64, device=dev)
-        pds.append(PatchData((P, P), (g, g), dx, Qd, F, S))
-    cells = N * N
-    dt = 1.0e-4
-    stream = torch.cuda.current_stream()
+    L = UnevenLevel(args, torch, np)
+    ms, wall, stage, ms_p = bench_headline(args, torch, np, L)
+    roof = headline_roofline(L, stage, ms)
+    cpu = None if args.no_cpu else cpu_baseline_uneven(args, np, L)
+    config = L.describe(np)
+    rk = bench_level_rk3(args, torch, np, L) if args.rk_steps > 0 else None
+    del L
+    torch.cuda.empty_cache()
 
-    plist = PatchList(pds, neq)  # the level's patch list, built once as a C++ caller would hold it
-
-    def step_batched():
-        rec.computeConvectiveFluxAndSourceOnPatches(plist, 0.0, dt, 0, stream)
-
-    # Per-patch path: views prebuilt so the loop is one C call per patch.
-    views = [pd.view(neq) for pd in pds]
-    sp = ctypes.c_void_p(stream.cuda_stream)
-    fn = capi.lib.hm_compute_convective_flux_and_source_on_patch
-
-    def step_per_patch():
-        for v in views:
-            st = fn(rec._ctx, ctypes.byref(v), 0.0, dt, 0, sp)
-            if st:
-                capi.check(rec._ctx, st)
-
-    def timed(fn_step, steps, warmup):
-        for _ in range(warmup):
-            fn_step()
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            fn_step()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        return e0.elapsed_time(e1) / steps, wall / steps * 1e3
-
-    ms_batched, wall_batched = timed(step_batched, args.steps, args.warmup)
-
-    # Stage timing of the batched path (events around each stage launch).
-    rec.enable_profiling(True)
-    for _ in range(args.steps):
-        step_batched()
-    prof = rec.read_profile()
-    rec.enable_profiling(False)
-    stage_avg = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in prof.items()}
-    dominant = max(stage_avg, key=stage_avg.get)
-
-    ms_pp = None
-    if not args.no_per_patch:
-        ms_pp, _ = timed(step_per_patch, max(3, args.steps // 4), 1)
-
-    # Algorithmic (compulsory) HBM bytes per cell per RHS: read Q (neq), write
-    # F on d axes (d*neq): 8*(4 + 2*4) = 96 B for 2D single-species.
-    bytes_per_cell = 8 * (neq + 2 * neq)
-    # Each flux kernel covers one axis here (axis 0, axis 1): its compulsory
-    # bytes are Q read for the node fluxes and F of its axis written,
-    # 8*(4 + 4) = 64 B per cell per launch.
-    kernel_bytes_per_cell = 8 * (neq + neq)
-    achieved_gbs = kernel_bytes_per_cell * cells / (stage_avg[dominant] * 1e-3) / 1e9
-    # DRAM bytes per launch of the dominant kernel, measured with rocprofv3
-    # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a copy of known size)
-    # for the default workload only; other sizes report null.
-    traffic, traffic_source = None, None
-    if (args.N, args.patch, args.scheme) == (2048, 128, "WCNS5_JS_HLLC_HLL") and \
-            FLUX_TRAFFIC_2048_WCNS5.get(dominant) is not None:
-        traffic = FLUX_TRAFFIC_2048_WCNS5[dominant]
-        traffic_source = "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/flux_kernels_2048sq_counters.md"
-    rhs_gbs = bytes_per_cell * cells / (ms_batched * 1e-3) / 1e9
-    value = cells / (ms_batched * 1e-3)
-
-    cpu = None
-    if not args.no_cpu:
-        from oracle import oracle as O
-
-        nthreads = min(16, os.cpu_count() or 1)
-        prm = O.Params(dim=2, scheme=1 if args.scheme.startswith("WCNS6") else 0)
-
-        cpu_list = [{"Q": Qp, "n": (P, P), "g": g, "dx": dx,
-                  "F": [np.empty((neq, P, P + 1)), np.empty((neq, P + 1, P))],
-                  "S": np.zeros((neq, P, P))} for _, Qp in patches]
-        t0 = time.perf_counter()
-        O.rhs_batch(prm, cpu_list, dt, nthreads)        # one pass over the whole level, to size the sample
-        t1 = time.perf_counter() - t0
-        reps = max(1, int(round(args.cpu_seconds / max(t1, 1e-3))))
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            O.rhs_batch(prm, cpu_list, dt, nthreads)
-        tk = time.perf_counter() - t0
-        cpu = {"value": reps * cells / tk, "unit": "cell-RHS-evals/s", "cores": nthreads, "kind": "port",
-               "sample": f"the whole 256-patch level, {reps} RHS evaluations ({tk:.1f} s), oracle built -O3 "
-                         f"-fopenmp -ffp-contract=off, {nthreads} OpenMP threads over patches"}
-
-    uneven = None
-    npatches = len(pds)
-    if args.uneven_seg > 0:
-        # free the 2D level first
-        del plist, pds, views
+    rhs2d = ns3d = adjoint = None
+    if not args.headline_only:
+        rhs2d = bench_2d(args, torch, np)
         torch.cuda.empty_cache()
-        uneven = bench_uneven_3d(args, torch, np, time)
-        torch.cuda.empty_cache()
+        if args.ns_seg > 0:
+            ns3d = bench_ns_3d(args, torch, np, time)
+            torch.cuda.empty_cache()
+        if args.adjoint_steps > 0:
+            adjoint = bench_adjoint(args, torch, np, time)
 
-    ns3d = None
-    if args.ns_seg > 0:
-        ns3d = bench_ns_3d(args, torch, np, time)
-        torch.cuda.empty_cache()
-
-    adjoint = None
-    if args.adjoint_steps > 0:
-        adjoint = bench_adjoint(args, torch, np, time)
-
+    value = config["cells"] / (ms * 1e-3)
     out = {
-        "metric": "FP64 cell-RHS-evals/s (batched convective RHS, WCNS5-JS + hybrid HLLC/HLLC-HLL)",
+        "metric": "FP64 cell-RHS-evals/s (batched convective RHS of an uneven-patch 3D five-equation level, "
+                  "WCNS6-LD + hybrid HLLC/HLLC-HLL)",
         "value": value,
         "unit": "cell-RHS-evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_batched,
+        "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "2D single-species Euler four-quadrant Riemann, 2048^2 in 256 patches of "
-                               "128^2, 4-cell halos, WCNS5_JS_HLLC_HLL, batched RHS", "cells": cells,
-                   "patches": npatches, "patch": [P, P], "scheme": args.scheme},
-        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": traffic_source,
-                     "kernel": STAGE_KERNEL.get(dominant, dominant), "kernel_ms": stage_avg[dominant],
-                     "bytes_per_cell": kernel_bytes_per_cell, "rhs_bytes_per_cell": bytes_per_cell},
-        "rhs_hbm_frac": rhs_gbs / HBM_PEAK_GBS,
-        "stage_ms": stage_avg,
-        "per_patch": {"ms_per_step": ms_pp, "value": (cells / (ms_pp * 1e-3)) if ms_pp else None,
-                      "batched_speedup": (ms_pp / ms_batched) if ms_pp else None},
-        "host_wall_ms_per_step": wall_batched,
+        "config": config,
+        "roofline": roof,
+        "stage_ms": stage,
+        "per_patch": {"ms_per_step": ms_p, "value": (config["cells"] / (ms_p * 1e-3)) if ms_p else None,
+                      "batched_speedup": (ms_p / ms) if ms_p else None},
+        "host_wall_ms_per_step": wall,
+        "rk3_level_step": rk,
         "cpu_baseline": cpu,
-        "uneven_3d": uneven,
+        "rhs_2d": rhs2d,
         "navier_stokes_3d": ns3d,
         "adjoint": adjoint,
     }
