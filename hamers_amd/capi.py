@@ -13,6 +13,7 @@ HM_MAX_RK_COEFFS = 5
 
 HM_OK = 0
 HM_ERR_INVALID_ARGUMENT = 1
+HM_DT_INFO_SLOTS = 8
 HM_ERR_UNSUPPORTED = 2
 HM_ERR_DEVICE = 3
 HM_ERR_OUT_OF_MEMORY = 4
@@ -165,6 +166,13 @@ def _load():
                                                      ctypes.c_double, ctypes.c_int, P]
     L.hm_compute_diffusive_flux_on_patches.argtypes = [P, ctypes.POINTER(hm_patch_view), ctypes.c_int,
                                                        ctypes.c_double, ctypes.c_double, ctypes.c_int, P]
+    L.hm_compute_spectral_radiuses_and_stable_dt_on_patches.argtypes = [P, ctypes.POINTER(hm_patch_view), ctypes.c_int,
+                                                                         D, P]
+    L.hm_level_advance_cfl.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.c_int, P]
+    L.hm_level_dt_info.argtypes = [P, D]
+    for name in ("hm_compute_spectral_radiuses_and_stable_dt_on_patches", "hm_level_advance_cfl",
+                 "hm_level_dt_info"):
+        getattr(L, name).restype = ctypes.c_int
     for name in ("hm_advance_single_step_on_patch", "hm_advance_single_step_on_patches", "hm_level_create",
                  "hm_level_destroy", "hm_level_set_rk_coefficients", "hm_level_write_patch",
                  "hm_level_read_patch", "hm_level_patch_array", "hm_level_fill_ghosts", "hm_level_advance",

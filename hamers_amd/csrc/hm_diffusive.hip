@@ -290,11 +290,12 @@ __global__ __launch_bounds__(kBlock) void k_diff_faces(W w, DParams k) {
     const double c_r = c_n;
     const double* Fn = P->fnode[A];
     double* out = P->F[A];
+    const double dt = k.dtp ? *k.dtp : k.dt;
 #pragma unroll
     for (int e = 0; e < NEQ; e++) {
         const double* f = Fn + e * nvol + n0;
         double acc = double(0);
-        acc += k.dt * (a_r * (f[-nst] + f[0]) + b_r * (f[-2 * nst] + f[nst]) + c_r * (f[-3 * nst] + f[2 * nst]));
+        acc += dt * (a_r * (f[-nst] + f[0]) + b_r * (f[-2 * nst] + f[nst]) + c_r * (f[-3 * nst] + f[2 * nst]));
         out[e * fvol + loc] = acc;
     }
 }
@@ -359,6 +360,9 @@ int hm_set_diffusive_transport(hm_ctx* ctx, const hm_transport* tr) {
             !(tr->species_D[i] >= 0.0))
             return ctx->rec->fail(HM_ERR_INVALID_ARGUMENT, "transport coefficients must be non-negative");
     }
+    if (ctx->levels > 0)
+        return ctx->rec->fail(HM_ERR_INVALID_ARGUMENT,
+                              "set the transport properties before creating levels on this context");
     if (int st = ctx->rec->selectDevice()) return st;
     ctx->diff.reset();
     ctx->diff.reset(new (std::nothrow) hm::DiffusiveFluxReconstructorGPU(*ctx->rec, *tr));

@@ -52,6 +52,7 @@ struct DParams {
     double wfac[HM_MAX_SPECIES];  // 1 / sqrt(M_i), the mixing weights' factors
     int num_species;
     double dt;
+    const double* dtp;  // non-NULL: dt in device memory (KParams::dtp)
 };
 
 template <class W>
@@ -84,6 +85,7 @@ class DiffusiveFluxReconstructorGPU {
         }
         k_.num_species = c.num_species;
         k_.dt = 0.0;
+        k_.dtp = nullptr;
         neq_ = rec.numEqn();
     }
 
@@ -146,6 +148,41 @@ class DiffusiveFluxReconstructorGPU {
         DParams k = k_;
         k.dt = dt;
         return launch_ordered(cp->plan, k, s);
+    }
+
+    // Work tables of a fixed patch list over a caller-owned scratch region
+    // (scratchDoubles), outside the plan cache: a captured graph can hold their
+    // addresses for as long as the owner keeps them.
+    struct OwnedPlan {
+        void* dev = nullptr;
+        DPlan<SegTableT<DiffPatch>> plan{};
+    };
+
+    size_t scratchDoubles(const hm_patch_view* views, int np) const {
+        size_t total = 0;
+        for (int i = 0; i < np; i++) total += layout(views[i]).total;
+        return total;
+    }
+
+    int buildOwnedPlan(const hm_patch_view* views, int np, double* scratch, OwnedPlan& out) {
+        for (int i = 0; i < np; i++)
+            if (int st = check_view(views[i], i)) return st;
+        if (int st = rec_.selectDevice()) return st;
+        releaseOwnedPlan(out);
+        return upload(views, np, scratch, out.dev, out.plan);
+    }
+
+    void releaseOwnedPlan(OwnedPlan& p) {
+        if (p.dev) (void)hipFree(p.dev);
+        p.dev = nullptr;
+    }
+
+    // No allocation, no events: graph-capture safe.  dtp: dt in device memory, or NULL.
+    int runOwnedPlan(const OwnedPlan& p, double dt, hipStream_t s, const double* dtp = nullptr) {
+        DParams k = k_;
+        k.dt = dt;
+        k.dtp = dtp;
+        return launch(p.plan, k, s);
     }
 
    private:
@@ -307,10 +344,19 @@ class DiffusiveFluxReconstructorGPU {
             p->dev = nullptr;
         }
         p->views.clear();
+        if (int st = upload(views, np, arena_, p->dev, p->plan)) return st;
+        p->views.assign(views, views + np);
+        p->arena_at_build = arena_;
+        p->use = use_clock_;
+        out = p;
+        return HM_OK;
+    }
+
+    // Device work tables of a patch list whose scratch starts at `base`.
+    int upload(const hm_patch_view* views, int np, double* base, void*& dev, DPlan<SegTableT<DiffPatch>>& plan) {
         const int d = rec_.config().dim;
         TableImage<DiffPatch> img[kDiffStages];
         for (auto& t : img) t.items.resize(np);
-        double* base = arena_;
         for (int i = 0; i < np; i++) {
             const Layout L = layout(views[i]);
             const DiffPatch dp = make_patch(views[i], base, L);
@@ -323,24 +369,20 @@ class DiffusiveFluxReconstructorGPU {
         }
         size_t bytes = 0;
         for (auto& t : img) bytes += t.bytes();
-        if (hipMalloc(&p->dev, bytes) != hipSuccess) {
-            p->dev = nullptr;
+        if (hipMalloc(&dev, bytes) != hipSuccess) {
+            dev = nullptr;
             return rec_.fail(HM_ERR_OUT_OF_MEMORY, "hipMalloc of the diffusive work tables failed");
         }
         std::vector<char> h(bytes, 0);
         size_t o = 0;
         for (int st = 0; st < kDiffStages; st++) {
-            p->plan.work[st] = img[st].write(h.data() + o, static_cast<char*>(p->dev) + o);
-            p->plan.items[st] = img[st].total;
+            plan.work[st] = img[st].write(h.data() + o, static_cast<char*>(dev) + o);
+            plan.items[st] = img[st].total;
             o += img[st].bytes();
         }
         // built once per distinct patch list: a blocking copy is fine here
-        if (hipMemcpy(p->dev, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+        if (hipMemcpy(dev, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
             return rec_.fail(HM_ERR_DEVICE, "upload of the diffusive work tables failed");
-        p->views.assign(views, views + np);
-        p->arena_at_build = arena_;
-        p->use = use_clock_;
-        out = p;
         return HM_OK;
     }
 

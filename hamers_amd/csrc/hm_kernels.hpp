@@ -18,6 +18,10 @@ struct KParams {
     double igm1[HM_MAX_SPECIES];    // species 1/(gamma-1) (volume-fraction mixing)
     int num_species;
     double dt;
+    // With dtp non-NULL the kernels take dt from device memory instead (a
+    // level advanced with its stable dt computed on the device each step:
+    // one captured graph serves every dt).
+    const double* dtp;
     // Face-state slot (single-species model): the characteristic face values
     // W-, W+ of every midpoint face, chunk DevPatch::wrec[a] per patch and
     // axis, [2 NEQ][midpoint faces].  The recording forward pass writes it,
@@ -35,6 +39,9 @@ struct KParams {
     // from a slot of its trajectory store); 0 otherwise.
     int64_t qoff;
 };
+
+// The time step a kernel scales its fluxes with.
+__device__ __forceinline__ double kdt(const KParams& k) { return k.dtp ? *k.dtp : k.dt; }
 
 // Fused midpoint and face flux of the single-species model on axis 0
 // (k_flux_single): a wavefront computes kChunk + 2 consecutive midpoint faces

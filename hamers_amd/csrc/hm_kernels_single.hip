@@ -14,6 +14,7 @@
 //   k_flux_tile_single  axes 1 and 2: the same, fused per 64 x 24 tile
 #include <type_traits>
 
+#include "hm_cell.hpp"
 #include "hm_kernels.hpp"
 #include "hm_face_single.hpp"
 #include "hm_numerics.hpp"
@@ -47,24 +48,8 @@ __global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
     // at a 32-bit offset within the patch's Q box
     const uint32_t iq = uint32_t(q_idx<D>(*P, i, j, kk));
     auto q = [&](int e, uint32_t at) { return ldg(P->Q[e] + k.qoff, at); };
-    const double gam = k.gamma[0];
-
-    const double rho = q(0, iq);
-    const double rr = rcp_rn(rho);  // quot(a, rho, rr) rounds exactly as a / rho
-    double u[3];
-#pragma unroll
-    for (int b = 0; b < D; b++) u[b] = quot(q(1 + b, iq), rho, rr);  // FlowModelSingleSpecies.cpp:2719
-    double ke;
-    if constexpr (D == 1)
-        ke = u[0] * u[0];
-    else if constexpr (D == 2)
-        ke = u[0] * u[0] + u[1] * u[1];
-    else
-        ke = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    const double E = q(D + 1, iq);
-    const double eps_int = quot(E, rho, rr) - double(1) / double(2) * ke;  // :2934
-    const double p = (gam - double(1)) * rho * eps_int;                   // EquationOfStateIdealGas.cpp:5477
-    const double c = sqrt(quot(gam * p, rho, rr));                        // EquationOfStateIdealGas.cpp:5806
+    double rho, u[3], p, c;
+    cell_state_single<D>(q, iq, k.gamma[0], rho, u, p, c);
 
     const int64_t vol = B.vol(kGc);
     double* pr = P->prim;
@@ -214,7 +199,7 @@ __device__ __forceinline__ void fused_chunk(const DevPatch& P, int64_t loc, cons
         nL[e] = node_flux_single<D, A>(P, qL, V[2][1 + A], V[2][IP], e);
         nR[e] = node_flux_single<D, A>(P, qR, V[3][1 + A], V[3][IP], e);
     }
-    store_face_flux<D>(P.F[A], B.face_vol(A), B.face_idx(A, c0, c1, c2), k.dt, mL, mM, mR, nL, nR);
+    store_face_flux<D>(P.F[A], B.face_vol(A), B.face_idx(A, c0, c1, c2), kdt(k), mL, mM, mR, nL, nR);
 }
 
 
@@ -306,7 +291,7 @@ __device__ __forceinline__ void fused_tile(const DevPatch& P, int64_t loc, const
             nL[e] = node_flux_single<D, A>(P, qL, uaL, prL, e);
             nR[e] = node_flux_single<D, A>(P, qR, uaR, prR, e);
         }
-        store_face_flux<D>(P.F[A], B.face_vol(A), B.face_idx(A, c0, c1, c2), k.dt, mL, mM, mR, nL, nR);
+        store_face_flux<D>(P.F[A], B.face_vol(A), B.face_idx(A, c0, c1, c2), kdt(k), mL, mM, mR, nL, nR);
     }
 }
 

@@ -21,6 +21,7 @@
 #include <map>
 
 #include "hm_ctx.hpp"
+#include "hm_dt.hpp"
 #include "hm_rk.hpp"
 #include "hm_tables.hpp"
 
@@ -276,19 +277,27 @@ unsigned blocks_for(int64_t items) { return unsigned((items + kBlock - 1) / kBlo
 
 class Level {
    public:
-    Level(ConvectiveFluxReconstructorGPU* rec) : rec_(rec) {}
+    // diff: the context's diffusive flux reconstructor (four-equation model
+    // with transport properties), or NULL for the Euler form.  With it the
+    // level keeps 6 ghost cells and its steps add the diffusive flux to every
+    // stage in the Navier-Stokes form (NavierStokes.cpp:1108-1247, 1704-1750).
+    Level(ConvectiveFluxReconstructorGPU* rec, DiffusiveFluxReconstructorGPU* diff = nullptr) : rec_(rec), diff_(diff) {}
 
     ~Level() {
         (void)rec_->selectDevice();
         (void)hipDeviceSynchronize();
         for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
         for (auto& p : rhs_plan_) rec_->releaseOwnedPlan(p);
+        if (diff_)
+            for (auto& p : diff_plan_) diff_->releaseOwnedPlan(p);
+        if (diff_scratch_) (void)hipFree(diff_scratch_);
         if (adj_)
             adj_->release(vjp_plan_);
         if (adj_mem_) (void)hipFree(adj_mem_);
         if (store_) (void)hipFree(store_);
         if (wst_) (void)hipFree(wst_);
         if (rhs_scratch_) (void)hipFree(rhs_scratch_);
+        if (dt_mem_) (void)hipFree(dt_mem_);
         if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
         if (ev_a_) (void)hipEventDestroy(ev_a_);
         if (ev_b_) (void)hipEventDestroy(ev_b_);
@@ -301,7 +310,7 @@ class Level {
         d_ = c.dim;
         neq_ = rec_->numEqn();
         nq_ = rec_->numConservativeArrays();
-        g_ = kGc;
+        g_ = diff_ ? kGd : kGc;
         np_ = np;
         if (np < 1) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "a level needs at least one patch");
         for (int a = 0; a < 3; a++) {
@@ -419,6 +428,53 @@ class Level {
         return HM_OK;
     }
 
+    // num_steps steps whose dt the device computes at the start of each step
+    // from the level state (the stable dt of RungeKuttaLevelIntegrator::getLevelDt
+    // times cfl, hm_dt.hpp): the RHS kernels read it from device memory, so the
+    // host never waits for it and one captured graph serves every step.
+    int advanceCfl(double cfl, int nsteps, bool use_graph, hipStream_t s) {
+        if (nsteps < 0 || !(cfl > 0.0)) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "bad cfl or step count");
+        if (diff_)
+            return rec_->fail(HM_ERR_UNSUPPORTED,
+                              "the stable dt of the Navier-Stokes form (maximum diffusivity, NavierStokes.cpp:"
+                              "846-893) is not built: advance the level with a given dt");
+        if (int st = rec_->selectDevice()) return st;
+        if (int st = ensure_dt()) return st;
+        if (launch_dt_reset_time(dt_buf_, s) != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "dt launch failed");
+        if (!use_graph) {
+            for (int it = 0; it < nsteps; it++)
+                if (int st = step_cfl(cfl, s)) return st;
+            return HM_OK;
+        }
+        if (nsteps == 0) return HM_OK;
+        const double key = -cfl;  // fixed-dt graphs are keyed by dt > 0
+        auto itg = graphs_.find(key);
+        if (itg == graphs_.end()) {
+            if (int st = capture_with(key, [&](hipStream_t x) { return step_cfl(cfl, x); })) return st;
+            itg = graphs_.find(key);
+        }
+        if (hipEventRecord(ev_a_, s) != hipSuccess || hipStreamWaitEvent(cap_stream_, ev_a_, 0) != hipSuccess)
+            return rec_->fail(HM_ERR_DEVICE, "stream ordering failed");
+        for (int it = 0; it < nsteps; it++)
+            if (hipGraphLaunch(itg->second, cap_stream_) != hipSuccess)
+                return rec_->fail(HM_ERR_DEVICE, "hipGraphLaunch failed");
+        if (hipEventRecord(ev_b_, cap_stream_) != hipSuccess || hipStreamWaitEvent(s, ev_b_, 0) != hipSuccess)
+            return rec_->fail(HM_ERR_DEVICE, "stream ordering failed");
+        return HM_OK;
+    }
+
+    // The last advanceCfl step's reduction (kDtSlots doubles: spectral
+    // radiuses, dt of the patches, of the level, of the step, the sum of the
+    // step dts of that call); waits for the device.
+    int dtInfo(double* out) {
+        if (!out) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "NULL output");
+        if (!dt_mem_) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "no stable-dt step has run on this level");
+        if (hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(out, dt_buf_.res, sizeof(double) * kDtSlots, hipMemcpyDeviceToHost) != hipSuccess)
+            return rec_->fail(HM_ERR_DEVICE, "dt read failed");
+        return HM_OK;
+    }
+
     // ---------------------------------------------------------------------
     // Discrete adjoint of the stage loop with checkpointing.
     // ---------------------------------------------------------------------
@@ -528,12 +584,15 @@ class Level {
             for (int s = 0; s < kMaxRK; s++) {
                 for (int a = 0; a < d_; a++) total += up256(sizeof(double) * size_t(neq_) * fvol(p, a));
                 total += up256(sizeof(double) * size_t(neq_) * ivol(p));
+                if (diff_)
+                    for (int a = 0; a < d_; a++) total += up256(sizeof(double) * size_t(neq_) * fvol(p, a));
             }
         }
         if (hipMalloc(&storage_, total) != hipSuccess) return rec_->fail(HM_ERR_OUT_OF_MEMORY, "level storage");
         if (hipMemset(storage_, 0, total) != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "level storage clear");
         patch_.resize(np_);
         fl_.assign(size_t(np_) * kMaxRK * 3, nullptr);
+        fdl_.assign(size_t(np_) * kMaxRK * 3, nullptr);
         for (int p = 0; p < np_; p++) {
             char* b = static_cast<char*>(storage_) + off[p];
             LevelPatch& L = patch_[p];
@@ -550,6 +609,11 @@ class Level {
                 }
                 L.S[s] = reinterpret_cast<double*>(b);
                 b += up256(sizeof(double) * size_t(neq_) * ivol(p));
+                if (diff_)
+                    for (int a = 0; a < d_; a++) {
+                        fdl_[(size_t(p) * kMaxRK + s) * 3 + a] = reinterpret_cast<double*>(b);
+                        b += up256(sizeof(double) * size_t(neq_) * fvol(p, a));
+                    }
             }
         }
         // RHS views of every intermediate buffer.
@@ -588,6 +652,27 @@ class Level {
             if (int st = rec_->buildOwnedPlan(views_[s].data(), np_, static_cast<double*>(rhs_scratch_), rhs_plan_[s],
                                               single ? wst_off_.data() : nullptr))
                 return st;
+        if (diff_) {
+            // Diffusive RHS of every intermediate buffer into the stage's
+            // diffusive fluxes; the stages share one scratch region.
+            std::vector<hm_patch_view> dv(np_);
+            for (int s = 0; s < kMaxRK; s++) {
+                for (int p = 0; p < np_; p++) {
+                    dv[p] = views_[s][p];
+                    for (int a = 0; a < d_; a++) dv[p].F[a] = fdl_[(size_t(p) * kMaxRK + s) * 3 + a];
+                    dv[p].S = nullptr;
+                }
+                if (s == 0) {
+                    const size_t dd = diff_->scratchDoubles(dv.data(), np_);
+                    if (hipMalloc(&diff_scratch_, dd * sizeof(double)) != hipSuccess) {
+                        diff_scratch_ = nullptr;
+                        return rec_->fail(HM_ERR_OUT_OF_MEMORY, "level diffusive scratch");
+                    }
+                }
+                if (int st = diff_->buildOwnedPlan(dv.data(), np_, static_cast<double*>(diff_scratch_), diff_plan_[s]))
+                    return st;
+            }
+        }
         if (hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&ev_a_, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&ev_b_, hipEventDisableTiming) != hipSuccess)
@@ -712,6 +797,8 @@ class Level {
                 for (int m = 0; m <= s && m < kMaxRK; m++) {
                     for (int e = 0; e < nq_; e++) R.Qi[m][e] = patch_[p].Q[1 + m] + size_t(e) * gvol(p);
                     for (int a = 0; a < d_; a++) R.Fi[m][a] = fl_[(size_t(p) * kMaxRK + m) * 3 + a];
+                    if (diff_)
+                        for (int a = 0; a < d_; a++) R.Fdi[m][a] = fdl_[(size_t(p) * kMaxRK + m) * 3 + a];
                     R.Si[m] = patch_[p].S[m];
                 }
                 t.add(p, 0, ivol(p));
@@ -726,7 +813,7 @@ class Level {
             r.nz = five ? ns - 1 : 0;
             r.zfirst = five ? ns + d_ + 1 : 0;
             r.last_beta = -1;
-            r.ns = 0;  // the level integrates the Euler form
+            r.ns = diff_ ? 1 : 0;  // Navier-Stokes form with the diffusive fluxes, else Euler
             for (int m = 0; m < kMaxRK; m++) {
                 r.alpha[m] = alpha_[s][m];
                 r.beta[m] = beta_[s][m];
@@ -756,7 +843,7 @@ class Level {
     // slot record_slot + sn (recompute pass of the adjoint), and with
     // face_slot >= 0 and a face-state store its RHS keeps the face states in
     // face-state slot face_slot + sn.
-    int step(double dt, hipStream_t s, int record_slot = -1, int face_slot = -1) {
+    int step(double dt, hipStream_t s, int record_slot = -1, int face_slot = -1, const double* dtp = nullptr) {
         if (int st = launch_halo(0, s)) return st;
         for (int sn = 0; sn < nst_; sn++) {
             hipLaunchKernelGGL(k_copy_state, dim3(blocks_for(copy_items_)), dim3(kBlock), 0, s, copy_, 0, 1 + sn, nq_);
@@ -766,8 +853,11 @@ class Level {
                 if (int st = store_op(1 + sn, record_slot + sn, 0, s)) return st;
             hipLaunchKernelGGL(k_zero_source, dim3(blocks_for(zero_items_)), dim3(kBlock), 0, s, zero_, sn);
             if (hipGetLastError() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "level launch failed");
-            if (int st = rec_->runOwnedPlan(rhs_plan_[sn], dt, s, face_slot >= 0 ? face_states(face_slot + sn) : nullptr))
+            if (int st = rec_->runOwnedPlan(rhs_plan_[sn], dt, s, face_slot >= 0 ? face_states(face_slot + sn) : nullptr,
+                                            dtp))
                 return st;
+            if (diff_)
+                if (int st = diff_->runOwnedPlan(diff_plan_[sn], dt, s, dtp)) return st;
             if (launch_rk_update<SegTableT<RKPatch>>(d_, rk_[sn], rk_items_, nullptr, 0, rkp_[sn], s) != hipSuccess)
                 return rec_->fail(HM_ERR_DEVICE, "stage update launch failed");
         }
@@ -775,10 +865,59 @@ class Level {
     }
 
     int capture(double dt) {
+        return capture_with(dt, [&](hipStream_t x) { return step(dt, x); });
+    }
+
+    // Stable dt of the level state, then one step with that dt (both on s).
+    int step_cfl(double cfl, hipStream_t s) {
+        if (launch_stable_dt(rec_->config(), dt_work_, dt_items_, rec_->kernelParams(), cfl, dt_buf_, s) != hipSuccess)
+            return rec_->fail(HM_ERR_DEVICE, "stable dt launch failed");
+        return step(0.0, s, -1, -1, dt_buf_.res + kDtStep);
+    }
+
+    // Work table of the stable-dt kernel over the interiors of the scratch
+    // state, and its accumulators (built on first use).
+    int ensure_dt() {
+        if (dt_mem_) return HM_OK;
+        TableImage<DevPatch> tab;
+        tab.items.resize(np_);
+        for (int p = 0; p < np_; p++) {
+            DevPatch& d = tab.items[p];
+            std::memset(&d, 0, sizeof d);
+            int64_t cells = 1;
+            for (int a = 0; a < 3; a++) {
+                d.n[a] = a < d_ ? n_[3 * p + a] : 1;
+                d.gq[a] = a < d_ ? g_ : 0;
+                d.dx[a] = dx_[a];
+                d.wrec[a] = -1;
+                cells *= d.n[a];
+            }
+            for (int e = 0; e < nq_; e++) d.Q[e] = patch_[p].Q[0] + size_t(e) * gvol(p);
+            tab.add(p, 0, cells);
+        }
+        const size_t tb = tab.bytes();
+        const size_t bytes = tb + up256(sizeof(unsigned long long) * kDtAcc) + up256(sizeof(double) * kDtSlots);
+        if (hipMalloc(&dt_mem_, bytes) != hipSuccess) {
+            dt_mem_ = nullptr;
+            return rec_->fail(HM_ERR_OUT_OF_MEMORY, "level dt tables");
+        }
+        std::vector<char> img(bytes, 0);
+        dt_work_ = tab.write(img.data(), static_cast<char*>(dt_mem_));
+        dt_items_ = tab.total;
+        dt_buf_.acc = reinterpret_cast<unsigned long long*>(static_cast<char*>(dt_mem_) + tb);
+        dt_buf_.res = reinterpret_cast<double*>(static_cast<char*>(dt_mem_) + tb +
+                                                up256(sizeof(unsigned long long) * kDtAcc));
+        if (hipMemcpy(dt_mem_, img.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+            return rec_->fail(HM_ERR_DEVICE, "level dt table upload failed");
+        return HM_OK;
+    }
+
+    template <class F>
+    int capture_with(double key, F&& body) {
         hipGraph_t graph = nullptr;
         if (hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeThreadLocal) != hipSuccess)
             return rec_->fail(HM_ERR_DEVICE, "hipStreamBeginCapture failed");
-        const int st = step(dt, cap_stream_);
+        const int st = body(cap_stream_);
         const hipError_t e = hipStreamEndCapture(cap_stream_, &graph);
         if (st != HM_OK) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -790,7 +929,7 @@ class Level {
         (void)hipGraphDestroy(graph);
         if (ei != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "hipGraphInstantiate failed");
         if (graphs_.size() >= 4) drop_graphs();
-        graphs_[dt] = exec;
+        graphs_[key] = exec;
         return HM_OK;
     }
 
@@ -1123,6 +1262,14 @@ class Level {
     std::vector<double*> fl_;
     std::vector<std::vector<hm_patch_view>> views_;
     void* rhs_scratch_ = nullptr;
+    DiffusiveFluxReconstructorGPU* diff_ = nullptr;
+    std::vector<double*> fdl_;  // diffusive flux per patch, stage and axis (with diff_)
+    void* diff_scratch_ = nullptr;
+    DiffusiveFluxReconstructorGPU::OwnedPlan diff_plan_[kMaxRK];
+    void* dt_mem_ = nullptr;  // stable-dt work table and accumulators
+    SegTableT<DevPatch> dt_work_{};
+    int64_t dt_items_ = 0;
+    DtBuffers dt_buf_;
     ConvectiveFluxReconstructorGPU::OwnedPlan rhs_plan_[kMaxRK];
     SegTableT<HaloItem> halo_{};
     SegTableT<LevelPatch> copy_{}, zero_{};
@@ -1325,7 +1472,7 @@ int hm_level_create(hm_ctx* ctx, int num_patches, const int* patch_lo, const int
     if (!l) return HM_ERR_OUT_OF_MEMORY;
     l->ctx = ctx;
     ctx->levels++;
-    l->lvl.reset(new (std::nothrow) hm::Level(ctx->rec.get()));
+    l->lvl.reset(new (std::nothrow) hm::Level(ctx->rec.get(), ctx->diff.get()));
     if (!l->lvl) {
         ctx->levels--;
         delete l;
@@ -1379,6 +1526,16 @@ int hm_level_fill_ghosts(hm_level* level, void* stream) {
 int hm_level_advance(hm_level* level, double dt, int num_steps, int use_graph, void* stream) {
     if (!level) return HM_ERR_INVALID_ARGUMENT;
     return level->lvl->advance(dt, num_steps, use_graph != 0, static_cast<hipStream_t>(stream));
+}
+
+int hm_level_advance_cfl(hm_level* level, double cfl, int num_steps, int use_graph, void* stream) {
+    if (!level) return HM_ERR_INVALID_ARGUMENT;
+    return level->lvl->advanceCfl(cfl, num_steps, use_graph != 0, static_cast<hipStream_t>(stream));
+}
+
+int hm_level_dt_info(hm_level* level, double* info) {
+    if (!level) return HM_ERR_INVALID_ARGUMENT;
+    return level->lvl->dtInfo(info);
 }
 
 int hm_level_advance_recorded(hm_level* level, double dt, int num_steps, int checkpoint_every, int use_graph,

@@ -111,6 +111,57 @@ __device__ __forceinline__ double mix_gamma_last(const KParams& k, const double*
 
 }  // namespace
 
+// Primitive state of one cell and its sound speed squared from its
+// conservative variables (q(e, at): variable e of the cell at index `at`),
+// exactly as the derived-data kernel computes them; shared with the stable-dt
+// kernel.  Zr: partial densities; Z: the NS volume fractions (five-equation).
+template <int D, int NS, bool FOUR, class QF>
+__device__ __forceinline__ void cell_state_multi(const KParams& k, QF&& q, uint32_t iq, double (&Zr)[NS],
+                                                 double (&Z)[NS], double& rho, double (&u)[3], double& p,
+                                                 double& c2) {
+#pragma unroll
+    for (int si = 0; si < NS; si++) Zr[si] = q(si, iq);
+    if constexpr (!FOUR) {
+#pragma unroll
+        for (int si = 0; si < NS; si++) Z[si] = q(NS + D + 1 + si, iq);
+    }
+    rho = double(0);  // EquationOfStateMixingRules.cpp:735-860
+#pragma unroll
+    for (int si = 0; si < NS; si++) rho += Zr[si];
+    u[0] = u[1] = u[2] = 0.0;
+    const double rrho = rcp_rn(rho);  // quotients by rho, correctly rounded (quot)
+#pragma unroll
+    for (int b = 0; b < D; b++) u[b] = quot(q(NS + b, iq), rho, rrho);
+    double ke;
+    if constexpr (D == 1)
+        ke = u[0] * u[0];
+    else if constexpr (D == 2)
+        ke = u[0] * u[0] + u[1] * u[1];
+    else
+        ke = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const double E = q(NS + D, iq);
+    const double eps_int = quot(E, rho, rrho) - double(1) / double(2) * ke;
+    if constexpr (FOUR) {
+        double Y[NS];
+#pragma unroll
+        for (int si = 0; si < NS; si++) Y[si] = quot(Zr[si], rho, rrho);
+        double gmix, c_v, eps_psi;
+        mix_mass<NS>(k, Y, gmix, c_v);
+        p = (gmix - double(1)) * rho * eps_int;  // EquationOfStateIdealGas.cpp:5633
+        c2 = sound_speed_sq_four<NS>(k, rho, p, Y, eps_psi);
+    } else {
+        const double gmix = mix_gamma_all<NS>(k, Z);
+        p = (gmix - double(1)) * rho * eps_int;  // EquationOfStateIdealGas.cpp:5633
+        const double Gamma = gmix - double(1);
+        c2 = quot(Gamma * p, rho, rrho);  // FlowModelFiveEqnAllaire.cpp:4600-4640
+#pragma unroll
+        for (int si = 0; si < NS; si++) {
+            const double Psi = quot(p, rho, rrho);  // EquationOfStateMixingRulesIdealGas.cpp:6608
+            c2 += quot(Zr[si], rho, rrho) * Psi;
+        }
+    }
+}
+
 // ----------------------------------------------------------------------------
 // Derived cell data (FlowModelFiveEqnAllaire.cpp:3734-4876,
 // FlowModelFourEqnConservative.cpp:3877-5440).
@@ -135,49 +186,8 @@ __global__ __launch_bounds__(kBlock) void k_derived_multi(W w, KParams k) {
     const uint32_t iq = uint32_t(q_idx<D>(*P, i, j, kk));
     auto q = [&](int e, uint32_t at) { return ldg(P->Q[e], at); };
 
-    double Zr[NS], Z[NS];
-#pragma unroll
-    for (int si = 0; si < NS; si++) Zr[si] = q(si, iq);
-    if constexpr (!FOUR) {
-#pragma unroll
-        for (int si = 0; si < NS; si++) Z[si] = q(NS + D + 1 + si, iq);
-    }
-    double rho = double(0);  // EquationOfStateMixingRules.cpp:735-860
-#pragma unroll
-    for (int si = 0; si < NS; si++) rho += Zr[si];
-    double u[3] = {0.0, 0.0, 0.0};
-    const double rrho = rcp_rn(rho);  // quotients by rho, correctly rounded (quot)
-#pragma unroll
-    for (int b = 0; b < D; b++) u[b] = quot(q(NS + b, iq), rho, rrho);
-    double ke;
-    if constexpr (D == 1)
-        ke = u[0] * u[0];
-    else if constexpr (D == 2)
-        ke = u[0] * u[0] + u[1] * u[1];
-    else
-        ke = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    const double E = q(NS + D, iq);
-    const double eps_int = quot(E, rho, rrho) - double(1) / double(2) * ke;
-    double p, c2;
-    if constexpr (FOUR) {
-        double Y[NS];
-#pragma unroll
-        for (int si = 0; si < NS; si++) Y[si] = quot(Zr[si], rho, rrho);
-        double gmix, c_v, eps_psi;
-        mix_mass<NS>(k, Y, gmix, c_v);
-        p = (gmix - double(1)) * rho * eps_int;  // EquationOfStateIdealGas.cpp:5633
-        c2 = sound_speed_sq_four<NS>(k, rho, p, Y, eps_psi);
-    } else {
-        const double gmix = mix_gamma_all<NS>(k, Z);
-        p = (gmix - double(1)) * rho * eps_int;  // EquationOfStateIdealGas.cpp:5633
-        const double Gamma = gmix - double(1);
-        c2 = quot(Gamma * p, rho, rrho);  // FlowModelFiveEqnAllaire.cpp:4600-4640
-#pragma unroll
-        for (int si = 0; si < NS; si++) {
-            const double Psi = quot(p, rho, rrho);  // EquationOfStateMixingRulesIdealGas.cpp:6608
-            c2 += quot(Zr[si], rho, rrho) * Psi;
-        }
-    }
+    double Zr[NS], Z[NS], rho, u[3], p, c2;
+    cell_state_multi<D, NS, FOUR>(k, q, iq, Zr, Z, rho, u, p, c2);
 
     const int64_t vol = B.vol(kGc);
     double* pr = P->prim;
@@ -588,7 +598,7 @@ __device__ __forceinline__ void faceflux_multi_face(const DevPatch& P, int64_t l
     const int64_t fv = B.face_vol(A);
     const double* fm = P.fmid[A];
     double* out = P.F[A];
-    const double dt = k.dt;
+    const double dt = kdt(k);
 #pragma unroll
     for (int e = 0; e < NEQ; e++) {
         double nL, nR;
@@ -667,7 +677,7 @@ __global__ __launch_bounds__(kBlock) void k_source_five(W w, KParams k) {
 #pragma unroll
     for (int si = 0; si < NS - 1; si++) {
         const int e = NS + D + 1 + si;
-        P->S[e * svol + loc] += k.dt * P->Q[e][iq] * acc;
+        P->S[e * svol + loc] += kdt(k) * P->Q[e][iq] * acc;
     }
 }
 

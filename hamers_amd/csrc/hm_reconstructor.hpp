@@ -113,6 +113,7 @@ class ConvectiveFluxReconstructorGPU {
         }
         k_.num_species = cfg.num_species;
         k_.dt = 0.0;
+        k_.dtp = nullptr;
         k_.wrec = nullptr;
         k_.lam_src = -1;
         k_.lam_scale = 0.0;
@@ -217,15 +218,17 @@ class ConvectiveFluxReconstructorGPU {
     // All five stages of an owned plan; no profiling events (graph capture
     // safe).  wrec: face-state slot to record into (single-species model,
     // plans built with wrec_off), NULL for none.
-    int runOwnedPlan(const OwnedPlan& p, double dt, hipStream_t s, double* wrec = nullptr) {
-        return runOwnedPlanStages(p, dt, 0, kStages, s, wrec);
+    int runOwnedPlan(const OwnedPlan& p, double dt, hipStream_t s, double* wrec = nullptr,
+                     const double* dtp = nullptr) {
+        return runOwnedPlanStages(p, dt, 0, kStages, s, wrec, 0, dtp);
     }
 
     // Pipeline stages [first, last) of an owned plan.
     int runOwnedPlanStages(const OwnedPlan& p, double dt, int first, int last, hipStream_t s, double* wrec = nullptr,
-                           int64_t qoff = 0) {
+                           int64_t qoff = 0, const double* dtp = nullptr) {
         KParams k = k_;
         k.dt = dt;
+        k.dtp = dtp;
         k.wrec = wrec;
         k.qoff = qoff;
         return run_stages(p.plan, k, s, first, last);

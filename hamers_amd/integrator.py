@@ -215,6 +215,19 @@ class Level:
         capi.check(self.rec._ctx, capi.lib.hm_level_advance(self._lvl, float(dt), int(num_steps), int(use_graph),
                                                             self.rec._stream_ptr(stream)))
 
+    def advance_cfl(self, cfl: float, num_steps: int = 1, use_graph: bool = False, stream=None):
+        """Steps of dt = cfl * the level's stable dt, computed on the device at
+        the start of every step (no host round trip)."""
+        capi.check(self.rec._ctx, capi.lib.hm_level_advance_cfl(self._lvl, float(cfl), int(num_steps),
+                                                                int(use_graph), self.rec._stream_ptr(stream)))
+
+    def dt_info(self) -> dict:
+        """The last advance_cfl step's reduction."""
+        info = (ctypes.c_double * capi.HM_DT_INFO_SLOTS)()
+        capi.check(self.rec._ctx, capi.lib.hm_level_dt_info(self._lvl, info))
+        return {"spectral_radiuses": list(info[0:self.dim]), "dt_patches": info[3], "dt_level": info[4],
+                "dt": info[5], "time": info[6]}
+
     # -- discrete adjoint ---------------------------------------------------
 
     def advance_recorded(self, dt: float, num_steps: int, checkpoint_every: int, use_graph: bool = False,

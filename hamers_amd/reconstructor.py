@@ -178,6 +178,18 @@ class ConvectiveFluxReconstructor:
         capi.check(self._ctx, st)
 
 
+def computeSpectralRadiusesAndStableDtOnPatches(rec: "ConvectiveFluxReconstructor", patches, stream=None):
+    """Euler::computeSpectralRadiusesAndStableDtOnPatch over the interior cells
+    of every patch, reduced over the list as RungeKuttaLevelIntegrator::getLevelDt
+    does before its CFL factor: [s_0 .. s_{d-1}, min over patches of the
+    patch dt, the level dt].  Only the patches' conservative arrays are read."""
+    plist = patches if isinstance(patches, PatchList) else PatchList(patches, rec.num_arrays)
+    out = (ctypes.c_double * (rec.dim + 2))()
+    capi.check(rec._ctx, capi.lib.hm_compute_spectral_radiuses_and_stable_dt_on_patches(
+        rec._ctx, plist.views, len(plist), out, rec._stream_ptr(stream)))
+    return list(out)
+
+
 def _vjp_view(pd: PatchData, F_bar, Q_bar, num_arrays: int, neq: int) -> capi.hm_vjp_patch_view:
     v = capi.hm_vjp_patch_view()
     d = len(pd.n)

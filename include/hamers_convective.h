@@ -126,7 +126,8 @@ int hm_compute_convective_flux_and_source_on_patch(hm_ctx* ctx, const hm_patch_v
                                                     double dt, int rk_step, void* stream);
 
 /*
- * Batched form of the call above for all patches of a level (uneven sizes
+ * Batched form of hm_compute_convective_flux_and_source_on_patch for all
+ * patches of a level (uneven sizes
  * allowed): one work-list driven launch per kernel stage instead of one
  * launch sequence per patch.  Results are identical to calling the per-patch
  * entry point on every patch.  Replaces the serial patch loop of
@@ -148,6 +149,22 @@ int hm_compute_convective_flux_and_source_on_patches(hm_ctx* ctx, const hm_patch
 int hm_profile_enable(hm_ctx* ctx, int enable);
 int hm_profile_read(hm_ctx* ctx, double* stage_ms /* [HM_NUM_STAGES] */,
                     int64_t* stage_launches /* [HM_NUM_STAGES] */);
+
+/*
+ * Euler::computeSpectralRadiusesAndStableDtOnPatch (Euler.cpp:489-888) over
+ * the interior cells of every given patch (the reference's default,
+ * use_ghosts_to_compute_dt = false, reads current data without ghost cells),
+ * reduced over the list as RungeKuttaLevelIntegrator::getLevelDt
+ * (RungeKuttaLevelIntegrator.cpp:1175-1346) does before its CFL factor.  With
+ * lambda_a = |u_a| + c (the model's sound speed): out[a] (a < dim) = max over
+ * cells of lambda_a / dx_a; out[dim] = min over patches of the patch's stable
+ * dt, 1 / max over cells of sum_a lambda_a / dx_a (for one patch, the vector
+ * the reference returns); out[dim + 1] = the level dt before the CFL factor,
+ * min(out[dim], 1 / sum_a out[a]).  Only the views' conservative arrays are
+ * read (any ghost width); F and S are not used.  Synchronizes `stream`.
+ */
+int hm_compute_spectral_radiuses_and_stable_dt_on_patches(hm_ctx* ctx, const hm_patch_view* patches,
+                                                          int num_patches, double* out, void* stream);
 
 /* Block the host until all work this context queued on `stream` is done. */
 int hm_synchronize(hm_ctx* ctx, void* stream);
@@ -320,6 +337,23 @@ int hm_level_fill_ghosts(hm_level* level, void* stream);
 /* num_steps steps of size dt.  use_graph != 0 replays one step captured in a
  * hipGraph (captured once per dt) instead of issuing its launches. */
 int hm_level_advance(hm_level* level, double dt, int num_steps, int use_graph, void* stream);
+
+/*
+ * num_steps steps whose dt is cfl times the level's stable dt, computed on the
+ * device from the level state at the start of every step (the reduction of
+ * hm_compute_spectral_radiuses_and_stable_dt_on_patches: getLevelDt with
+ * cfl = min(cfl_init, cfl), RungeKuttaLevelIntegrator.cpp:1175-1346).  The RHS
+ * kernels read that dt from device memory, so the host never waits for it;
+ * use_graph != 0 replays one captured step for every dt.
+ */
+int hm_level_advance_cfl(hm_level* level, double cfl, int num_steps, int use_graph, void* stream);
+
+/* The last hm_level_advance_cfl step's reduction (waits for the device):
+ * info[0..2] spectral radiuses per axis, info[3] the patches' dt, info[4] the
+ * level dt, info[5] the step's dt (info[4] * cfl), info[6] the sum of the step
+ * dts of that hm_level_advance_cfl call. */
+#define HM_DT_INFO_SLOTS 8
+int hm_level_dt_info(hm_level* level, double* info);
 
 /*
  * Discrete adjoint of a level's Runge-Kutta steps (no counterpart in the

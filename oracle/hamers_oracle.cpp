@@ -1771,6 +1771,64 @@ int hmo_midpoint_probe_single(const hmo_params* prm, const int* n_in, int g, con
     return 0;
 }
 
+// Euler::computeSpectralRadiusesAndStableDtOnPatch (Euler.cpp:489-888) of one
+// patch, over its interior cells: RungeKuttaLevelIntegrator's default
+// use_ghosts_to_compute_dt = false (RungeKuttaLevelIntegrator.cpp:3787) reads
+// the current data, which has no ghost cells.  Maximum wave speed
+// lambda_a = |u_a| + c (FlowModelSingleSpecies.cpp:3884-4070; the
+// multi-species models use their mixture sound speed); out[a] = max over cells
+// of lambda_a / dx_a (a < dim) and out[dim] = 1 / max over cells of the sum of
+// those (Euler.cpp:592-605, 713-729, 861-890), the maxima by fmax from 0.
+int hmo_spectral_radiuses_and_dt(const hmo_params* prm, const int* n_in, int g, const double* dx, const double* Q,
+                                 double* out) {
+    if (!prm || !n_in || !dx || !Q || !out) return 1;
+    if (g < 4) return 3;  // the derived data covers the 4-cell halo
+    const int d = prm->dim;
+    int n[3] = {1, 1, 1};
+    for (int a = 0; a < d; a++) {
+        n[a] = n_in[a];
+        if (n[a] < 1) return 4;
+    }
+    CellArray V, C;
+    int iu = 1;  // first velocity slot of V
+    if (prm->model == 0) {
+        Model m = make_model(*prm);
+        Derived D;
+        compute_derived_single(m, n, g, Q, D);
+        V = D.V;
+        C = D.c;
+    } else {
+        Five f;
+        if (!make_multi(prm, f)) return 5;
+        DerivedFive D;
+        compute_derived_five(f, n, g, Q, D);
+        V = D.V;
+        C = D.c;
+        iu = f.ns;
+    }
+    double s[3] = {double(0), double(0), double(0)}, ssum = double(0);
+    for (int k = 0; k < n[2]; k++)
+        for (int j = 0; j < n[1]; j++)
+            for (int i = 0; i < n[0]; i++) {
+                const int64_t ic = V.idx(i, j, k);
+                const double c = C.c(0)[C.idx(i, j, k)];
+                double sr[3] = {double(0), double(0), double(0)};
+                for (int a = 0; a < d; a++) {
+                    const double lambda = std::fabs(V.c(iu + a)[ic]) + c;
+                    sr[a] = lambda / dx[a];
+                    s[a] = std::fmax(s[a], sr[a]);
+                }
+                double sum = sr[0];
+                if (d == 2) sum = sr[0] + sr[1];
+                if (d == 3) sum = sr[0] + sr[1] + sr[2];
+                if (prm->mutation == 9) sum = std::fmax(sr[0], std::fmax(sr[1], sr[2]));  // planted: max, not sum
+                ssum = std::fmax(ssum, sum);
+            }
+    for (int a = 0; a < d; a++) out[a] = s[a];
+    out[d] = double(1) / ssum;
+    return 0;
+}
+
 // Bounds check of one side of a face normal to `axis`, four- or five-equation
 // model (FlowModelBasicUtilities{FourEqnConservative,FiveEqnAllaire}.cpp):
 // 1 bounded, 0 not, -1 bad arguments.

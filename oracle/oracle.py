@@ -283,6 +283,43 @@ def midpoint_probe_single(prm: Params, Q: np.ndarray, n: tuple, g: int, dx: tupl
     return sensor, fmid
 
 
+def spectral_radiuses_and_dt(prm: Params, Q: np.ndarray, n: tuple, g: int, dx: tuple) -> np.ndarray:
+    """Euler::computeSpectralRadiusesAndStableDtOnPatch of one patch (interior
+    cells): [max (|u_a| + c) / dx_a for each axis a, 1 / max of their sum]."""
+    L = lib()
+    L.hmo_spectral_radiuses_and_dt.restype = ctypes.c_int
+    L.hmo_spectral_radiuses_and_dt.argtypes = [ctypes.POINTER(_Params), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                               ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.c_void_p]
+    d = prm.dim
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    out = np.zeros(d + 1)
+    nn = (ctypes.c_int * 3)(*(list(n) + [1] * (3 - d)))
+    dd = (ctypes.c_double * 3)(*(list(dx) + [1.0] * (3 - d)))
+    st = L.hmo_spectral_radiuses_and_dt(ctypes.byref(prm.to_c()), nn, g, dd, Q.ctypes.data, out.ctypes.data)
+    if st != 0:
+        raise ValueError(f"oracle spectral_radiuses_and_dt status {st}")
+    return out
+
+
+def level_dt(patch_results, cfl: float = 1.0) -> float:
+    """RungeKuttaLevelIntegrator::getLevelDt (RungeKuttaLevelIntegrator.cpp:1175-1346)
+    from the patches' [s_0 .. s_{d-1}, dt_patch]: the per-axis maxima over the
+    patches, dt = min over patches of dt_patch, then min(dt, 1 / sum of the
+    maxima) (tbox Min: a < b ? a : b), times cfl = min(cfl_init, cfl)."""
+    d = len(patch_results[0]) - 1
+    s = [0.0] * d
+    dt = float(np.finfo(np.float64).max)
+    for r in patch_results:
+        for a in range(d):
+            s[a] = s[a] if s[a] >= r[a] else float(r[a])
+        dt = dt if dt < r[d] else float(r[d])
+    ssum = 0.0
+    for a in range(d):
+        ssum += s[a]
+    dt = dt if dt < 1.0 / ssum else 1.0 / ssum
+    return dt * cfl
+
+
 def bounded_multi(prm: Params, axis: int, V) -> bool:
     """Bounds check of one side of a face normal to ``axis`` (four- or
     five-equation model)."""

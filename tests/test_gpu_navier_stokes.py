@@ -138,3 +138,63 @@ def test_navier_stokes_rk3_step_matches_oracle(torch_dev):
     # the diffusive flux matters at this viscosity: the Euler step differs
     euler = oracle_rk3(prm, Q0, n, dx, dt, 1)
     assert max_rel_err(euler, ref) > 1e-8
+
+
+def _smooth_state_3d(N):
+    n, dx = (N, N, N), (1.0 / N,) * 3
+    X = W.cell_centres(n, (0.0, 0.0, 0.0), dx)
+    ph = lambda k: 2 * np.pi * ((k + 1) * X[0] + (k + 2) * X[1] + (3 - k % 2) * X[2]) + k  # noqa: E731
+    rho = 1.0 + 0.2 * np.sin(ph(0))
+    Y0 = 0.5 + 0.3 * np.sin(ph(1))
+    vel = [0.3 * np.sin(ph(2)), 0.2 * np.cos(ph(3)), 0.1 * np.sin(ph(4))]
+    p = 1.0 + 0.1 * np.cos(ph(5))
+    return W.conservative_from_primitive_four([rho * Y0, rho * (1.0 - Y0)], vel, p, GAM, RR), n, dx
+
+
+@pytest.mark.parametrize("dim,use_graph", [(2, False), (2, True), (3, True)])
+def test_navier_stokes_level_rk3_matches_oracle(torch_dev, dim, use_graph):
+    """A level built on a context with transport properties keeps 6 ghost cells
+    and adds the diffusive flux to every stage (Navier-Stokes form): SSP-RK3
+    steps of an uneven periodic level against the oracle's Navier-Stokes RK3."""
+    from hamers_amd import Level
+
+    torch = torch_dev
+    if dim == 2:
+        Q0, n, dx = _smooth_state(32)
+        boxes = [((0, 0), (20, 12)), ((20, 0), (12, 12)), ((0, 12), (8, 20)), ((8, 12), (24, 20))]
+        dt, nsteps = 2e-3, 2
+    else:
+        Q0, n, dx = _smooth_state_3d(16)
+        boxes = [((i, j, k), (8, 8, 8)) for k in (0, 8) for j in (0, 8) for i in (0, 8)]
+        dt, nsteps = 2e-3, 1
+    rec, diff = _recs(dim)
+    lvl = Level(rec, boxes, ((0,) * dim, n), (1,) * dim, dx)
+    lvl.write_global(Q0)
+    _say(f"case Navier-Stokes level RK3, {dim}D, {len(boxes)} patches, {nsteps} steps, graph {use_graph}")
+    lvl.advance(dt, nsteps, use_graph=use_graph)
+    torch.cuda.synchronize()
+    got = lvl.read_global(Q0.shape)
+    prm = O.Params(dim=dim, model=O.FOUR_EQN_CONSERVATIVE, num_species=2, gamma=list(GAM), R=list(RR))
+    ref = oracle_rk3(prm, Q0, n, dx, dt, nsteps, transport=TR)
+    err = max_rel_err(got, ref)
+    same = int(np.sum(got == ref))
+    _say(f"Navier-Stokes level: max rel err {err:.3e}, bit-identical {same}/{ref.size}")
+    assert np.all(np.isfinite(got))
+    assert err <= 1e-12
+    euler = oracle_rk3(prm, Q0, n, dx, dt, nsteps)
+    assert max_rel_err(euler, ref) > 1e-8  # the diffusive flux is in the step
+
+
+def test_navier_stokes_level_refuses_stable_dt(torch_dev):
+    """The stable dt of the Navier-Stokes form needs the maximum diffusivity,
+    which is not built: hm_level_advance_cfl says so instead of using the
+    Euler dt."""
+    from hamers_amd import Level
+    from hamers_amd.capi import HamersError
+
+    Q0, n, dx = _smooth_state(16)
+    rec, diff = _recs(2)
+    lvl = Level(rec, [((0, 0), (16, 16))], ((0, 0), n), (1, 1), dx)
+    lvl.write_global(Q0)
+    with pytest.raises(HamersError, match="Navier-Stokes"):
+        lvl.advance_cfl(0.4, 1)
