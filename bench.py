@@ -571,6 +571,11 @@ def bench_adjoint(args, torch, np, time):
     }
 
 
+def progress(msg):
+    """One line per finished leg on stderr (the JSON line stays alone on stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -594,23 +599,30 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     L = UnevenLevel(args, torch, np)
+    progress("uneven 3D level built")
     ms, wall, stage, ms_p = bench_headline(args, torch, np, L)
+    progress(f"headline RHS {ms:.3f} ms (per-patch {ms_p} ms)")
     roof = headline_roofline(L, stage, ms)
     cpu = None if args.no_cpu else cpu_baseline_uneven(args, np, L)
+    progress("CPU baseline done")
     config = L.describe(np)
     rk = bench_level_rk3(args, torch, np, L) if args.rk_steps > 0 else None
+    progress("level RK3 step done")
     del L
     torch.cuda.empty_cache()
 
     rhs2d = ns3d = adjoint = None
     if not args.headline_only:
         rhs2d = bench_2d(args, torch, np)
+        progress("2D leg done")
         torch.cuda.empty_cache()
         if args.ns_seg > 0:
             ns3d = bench_ns_3d(args, torch, np, time)
+            progress("Navier-Stokes leg done")
             torch.cuda.empty_cache()
         if args.adjoint_steps > 0:
             adjoint = bench_adjoint(args, torch, np, time)
+            progress("adjoint leg done")
 
     value = config["cells"] / (ms * 1e-3)
     out = {

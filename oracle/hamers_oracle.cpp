@@ -88,23 +88,44 @@ constexpr double EPS = HMO_EPS;
 // plain scheme (bit-identical to the code without the tape).  The tape is a
 // single global sequence: use it from one thread only.
 // ---------------------------------------------------------------------------
+// Every decision belongs to a category (kTape*); a replay can leave chosen
+// categories free (decided afresh, their tape entries still consumed), which
+// measures how much of the gap between free and replayed differences one kind
+// of switch accounts for.
+enum {
+    kTapeOther = 0,
+    kTapeRtau = 1,      // WCNS6-LD R_tau > alpha_tau
+    kTapeSigma = 2,     // WCNS6-LD sigma = fmax(t1, t2)
+    kTapeWaveSpeed = 3, // sL = fmin, sR = fmax
+    kTapeSpm = 4,       // s_minus = fmin(0, sL), s_plus = fmax(0, sR)
+    kTapeSstar = 5,     // HLLC side s* > 0
+    kTapeHll = 6,       // HLL upwind sL > 0, sR < 0
+    kTapeFallback = 7,  // bounds fallback
+    kTapeSensor = 8,    // sensor > 0.65
+};
 struct BranchTape {
     int mode = 0;
-    std::vector<int8_t> v;
+    std::vector<uint64_t> bits;  // one bit per decision
+    size_t size = 0;             // decisions recorded
     size_t pos = 0;
     bool overrun = false;
+    int free_mask = 0;  // replay: bit c set = category c decided afresh
 };
 BranchTape g_tape;
 
-inline int tape_sel(int natural) {
+inline int tape_sel(int natural, int cat) {
     if (g_tape.mode == 1) {
-        g_tape.v.push_back(int8_t(natural));
+        if ((g_tape.size & 63) == 0) g_tape.bits.push_back(0);
+        if (natural) g_tape.bits.back() |= uint64_t(1) << (g_tape.size & 63);
+        g_tape.size++;
     } else if (g_tape.mode == 2) {
-        if (g_tape.pos >= g_tape.v.size()) {
+        if (g_tape.pos >= g_tape.size) {
             g_tape.overrun = true;
             return natural;
         }
-        return g_tape.v[g_tape.pos++];
+        const int r = int((g_tape.bits[g_tape.pos >> 6] >> (g_tape.pos & 63)) & 1);
+        g_tape.pos++;
+        return ((g_tape.free_mask >> cat) & 1) ? natural : r;
     }
     return natural;
 }
@@ -114,16 +135,16 @@ inline int tape_sel(int natural) {
 // WCNS6-LD's sigma denominator |a1| + |a2| + eps pass through zero.
 inline double t_abs(double x) { return std::fabs(x); }
 // fmin(a, b) / fmax(a, b); a tie selects a, as the adjoint does.
-inline double t_min(double a, double b) {
+inline double t_min(double a, double b, int cat) {
     if (g_tape.mode == 0) return std::fmin(a, b);
-    return tape_sel(a <= b ? 1 : 0) ? a : b;
+    return tape_sel(a <= b ? 1 : 0, cat) ? a : b;
 }
-inline double t_max(double a, double b) {
+inline double t_max(double a, double b, int cat) {
     if (g_tape.mode == 0) return std::fmax(a, b);
-    return tape_sel(a >= b ? 1 : 0) ? a : b;
+    return tape_sel(a >= b ? 1 : 0, cat) ? a : b;
 }
 // A plain threshold decision.
-inline bool t_if(bool c) { return g_tape.mode == 0 ? c : tape_sel(c ? 1 : 0) != 0; }
+inline bool t_if(bool c, int cat) { return g_tape.mode == 0 ? c : tape_sel(c ? 1 : 0, cat) != 0; }
 
 // ---------------------------------------------------------------------------
 // Pointwise reconstruction (WCNS6-LD: ...WCNS6-LD-HLLC-HLL.cpp:9-319,
@@ -203,7 +224,7 @@ inline double wcns6ld_side(const Stencil6& s, double sig, int p, int q, double C
     w3 = w3 / ws;
 
     const double Rtau = tau6 / (bavg + EPS);
-    if (t_if(Rtau > alpha_tau)) {
+    if (t_if(Rtau > alpha_tau, kTapeRtau)) {
         w0 = sig * wu0 + (double(1) - sig) * w0;
         w1 = sig * wu1 + (double(1) - sig) * w1;
         w2 = sig * wu2 + (double(1) - sig) * w2;
@@ -227,7 +248,7 @@ inline double wcns6ld_sigma(const Stencil6& s) {  // WCNS6-LD:24-41
     const double a3 = s.u4 - s.u3;
     const double t1 = t_abs(a1 - a2) / (t_abs(a1) + t_abs(a2) + EPS);
     const double t2 = t_abs(a2 - a3) / (t_abs(a2) + t_abs(a3) + EPS);
-    return t_max(t1, t2);
+    return t_max(t1, t2, kTapeSigma);
 }
 
 // WCNS5-JS one side (WCNS5-JS:78-118); the plus side is the mirrored stencil.
@@ -486,10 +507,10 @@ void riemann_face_single(int d, int a, double gam, bool use_hll, const double* V
 
     const double u_avg = double(1) / double(2) * (uL + uR);
     const double c_av = double(1) / double(2) * (cL + cR);
-    const double sL = t_min(u_avg - c_av, uL - cL);
-    const double sR = t_max(u_avg + c_av, uR + cR);
-    const double sm = (g_tape.mode == 0) ? std::fmin(double(0), sL) : (t_if(sL < double(0)) ? sL : double(0));
-    const double sp = (g_tape.mode == 0) ? std::fmax(double(0), sR) : (t_if(sR > double(0)) ? sR : double(0));
+    const double sL = t_min(u_avg - c_av, uL - cL, kTapeWaveSpeed);
+    const double sR = t_max(u_avg + c_av, uR + cR, kTapeWaveSpeed);
+    const double sm = (g_tape.mode == 0) ? std::fmin(double(0), sL) : (t_if(sL < double(0), kTapeSpm) ? sL : double(0));
+    const double sp = (g_tape.mode == 0) ? std::fmax(double(0), sR) : (t_if(sR > double(0), kTapeSpm) ? sR : double(0));
     const double sstar =
         (pR - pL + rhoL * uL * (sL - uL) - rhoR * uR * (sR - uR)) / (rhoL * (sL - uL) - rhoR * (sR - uR));
 
@@ -523,7 +544,7 @@ void riemann_face_single(int d, int a, double gam, bool use_hll, const double* V
     flux(VR, QR, FR);
 
     double Fhllc[5];
-    const bool left = t_if(sstar > double(0));
+    const bool left = t_if(sstar > double(0), kTapeSstar);
     {
         const double* V = left ? VL : VR;
         const double* Qs = left ? QL : QR;
@@ -550,8 +571,8 @@ void riemann_face_single(int d, int a, double gam, bool use_hll, const double* V
         for (int e = 0; e < neq; e++) Fhll[e] = 0.0;
         auto hll = [&](int e) {
             double r = (sR * FL[e] - sL * FR[e] + sR * sL * (QR[e] - QL[e])) / (sR - sL);
-            if (t_if(sL > double(0))) r = FL[e];
-            if (t_if(sR < double(0))) r = FR[e];
+            if (t_if(sL > double(0), kTapeHll)) r = FL[e];
+            if (t_if(sR < double(0), kTapeHll)) r = FR[e];
             return r;
         };
         Fhll[0] = hll(0);
@@ -714,7 +735,7 @@ void direction_pass_single(const hmo_params& prm, const Model& m, const int* n, 
         fp &= (Vp[neq - 1].v[f] > double(0)) ? 1 : 0;
         if (prm.mutation == 3) fm = fp = 1;  // planted mistake: bounds check dropped
         if (prm.mutation == 5) fm = 0;       // test hook: first-order face states on every face
-        if (t_if(fm == 0 || fp == 0)) {
+        if (t_if(fm == 0 || fp == 0, kTapeFallback)) {
             const int64_t iL = D.V.idx(i - sh[0], j - sh[1], k - sh[2]);
             const int64_t iR = D.V.idx(i, j, k);
             for (int e = 0; e < neq; e++) {
@@ -748,7 +769,7 @@ void direction_pass_single(const hmo_params& prm, const Model& m, const int* n, 
             double sensor = -th_avg / (std::fabs(th_avg) + om_avg + EPS);
             if (prm.mutation == 6) sensor = -sensor;  // planted mistake: sign of the dilatation
             if (probe) probe->sensor[f] = sensor;
-            use_hll = t_if(sensor > 0.65);
+            use_hll = t_if(sensor > 0.65, kTapeSensor);
         }
         double Fo[5];
         riemann_face_single(d, a, gam, use_hll, VL, VR, Fo, nullptr, prm.mutation);
@@ -1868,16 +1889,22 @@ int hmo_diffusive_patch(const hmo_params* prm, const hmo_transport* tr, const in
 // tape), mode 2 replays from its start, mode 0 switches it off.  Returns the
 // tape length.
 int64_t hmo_branch_tape(int mode) {
-    if (mode == 1) g_tape.v.clear();
+    if (mode == 1) {
+        g_tape.bits.clear();
+        g_tape.size = 0;
+    }
     g_tape.mode = mode;
     g_tape.pos = 0;
     g_tape.overrun = false;
-    return int64_t(g_tape.v.size());
+    return int64_t(g_tape.size);
 }
+
+// Categories (bit c = kTape c) that a replay decides afresh; 0 replays all.
+void hmo_branch_tape_free_mask(int mask) { g_tape.free_mask = mask; }
 
 // Replay position and whether a replay ran past the end of the tape.
 void hmo_branch_tape_status(int64_t* length, int64_t* position, int* overrun) {
-    *length = int64_t(g_tape.v.size());
+    *length = int64_t(g_tape.size);
     *position = int64_t(g_tape.pos);
     *overrun = g_tape.overrun ? 1 : 0;
 }

@@ -352,6 +352,18 @@ def branch_tape_status() -> tuple:
     return n.value, pos.value, bool(ov.value)
 
 
+# Categories of branch decisions (bit c of branch_tape_free_mask's mask).
+TAPE_RTAU, TAPE_SIGMA, TAPE_WAVE_SPEED, TAPE_SPM, TAPE_SSTAR, TAPE_HLL, TAPE_FALLBACK, TAPE_SENSOR = range(1, 9)
+
+
+def branch_tape_free_mask(mask: int) -> None:
+    """Categories (bit c = TAPE_* c) that a replay decides afresh, their tape
+    entries still consumed; 0 replays every decision."""
+    L = lib()
+    L.hmo_branch_tape_free_mask.argtypes = [ctypes.c_int]
+    L.hmo_branch_tape_free_mask(int(mask))
+
+
 class frozen_branches:
     """Context manager: record the branch decisions of the first oracle run
     inside it, replay them for every later run (``replay()`` rewinds)."""
