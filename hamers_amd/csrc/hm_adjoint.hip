@@ -552,7 +552,7 @@ int AdjointSingle::build(const hm_vjp_patch_view* views, int np, Owned& out, con
 
 int AdjointSingle::run(const Owned& o, double dt, hipStream_t s, const double* wst, int lam_src, double lam_scale,
                        int64_t qoff) {
-    if (int st = rec_->runOwnedPlanStages(o.fwd, dt, 0, 2, s, nullptr, qoff)) return st;  // derived data and sensor
+    if (int st = rec_->runOwnedPlanStages(o.fwd, dt, 0, 1, s, nullptr, qoff)) return st;  // derived data and sensor
     KParams k = rec_->kernelParams();
     k.dt = dt;
     k.wrec = const_cast<double*>(wst);  // read only here

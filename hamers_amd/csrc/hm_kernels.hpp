@@ -88,9 +88,10 @@ __host__ __device__ inline int64_t fused_items(int axis, const int* n, int dim) 
 }
 
 // Work of one RHS evaluation: one work source per pipeline stage
-// (0 derived cells, 1 sensor cells, 2 midpoint faces, 3 output faces,
-// 4 interior cells for the advective source) and the number of work items
-// (padded) each stage launches over.
+// (0 derived cells, 1 sensor cells -- empty: the derived-data kernels compute
+// the sensor --, 2 midpoint faces, 3 output faces, 4 interior cells for the
+// advective source) and the number of work items (padded) each stage launches
+// over.
 constexpr int kStages = 5;
 
 template <class W>

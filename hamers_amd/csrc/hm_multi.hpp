@@ -280,24 +280,20 @@ __device__ __forceinline__ bool bounded_multi(const double* V, const KParams& k)
     return ok && (c2 > double(0));
 }
 
+// Midpoint flux Fo of the face normal to axis A between cells (c0, c1, c2)
+// - e_A and (c0, c1, c2), and its HLLC normal velocity u* (five-equation).
+// The prim and sensor arrays are read at 32-bit offsets from per-component
+// bases (every patch's ghost box is below 2^29 cells).
 template <int D, int NS, bool FOUR, int A, int SCHEME, int PP, int QQ>
-__device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t loc, const KParams& k) {
+__device__ __forceinline__ void midflux_multi_at(const DevPatch& P, const Boxes<D>& B, int c0, int c1, int c2,
+                                                 const KParams& k, double (&Fo)[Multi<D, NS, FOUR>::NEQ],
+                                                 double& ustar) {
     using M = Multi<D, NS, FOUR>;
     constexpr int NEQ = M::NEQ;
     constexpr int NZ = M::NZ;
     constexpr int IU = NS + A;   // normal velocity slot
     constexpr int IP = NS + D;   // pressure slot
     constexpr int IZ = NS + D + 1;
-    const Boxes<D> B(P);
-    const int m0 = B.mid_ext(A, 0), m1 = B.mid_ext(A, 1);
-    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
-    const uint32_t r = uint32_t(loc) / uint32_t(m0);
-    int c0 = int(uint32_t(loc) - r * uint32_t(m0));
-    int c1 = (D > 1) ? int((D > 2) ? r % uint32_t(m1) : r) : 0;
-    int c2 = (D > 2) ? int(r / uint32_t(m1)) : 0;
-    if constexpr (A == 0) c0 -= 1;
-    if constexpr (A == 1) c1 -= 1;
-    if constexpr (A == 2) c2 -= 1;
 
     const int64_t v4 = B.vol(kGc);
     const int64_t st = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGc)) : int64_t(B.ext(0, kGc)) * B.ext(1, kGc);
@@ -433,11 +429,11 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
     bool use_hll = false;
     if constexpr (D > 1) {
         const int64_t vs = B.vol(kGs);
-        const int64_t tR = B.idx(kGs, c0, c1, c2);
-        const int64_t sts = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGs)) : int64_t(B.ext(0, kGs)) * B.ext(1, kGs);
-        const int64_t tL = tR - sts;
-        const double th_avg = 0.5 * (P.sens[tL] + P.sens[tR]);
-        const double om_avg = 0.5 * (P.sens[vs + tL] + P.sens[vs + tR]);
+        const uint32_t tR = uint32_t(B.idx(kGs, c0, c1, c2));
+        const uint32_t sts = (A == 0) ? 1u : (A == 1) ? uint32_t(B.ext(0, kGs)) : uint32_t(B.ext(0, kGs) * B.ext(1, kGs));
+        const uint32_t tL = tR - sts;
+        const double th_avg = 0.5 * (ldg(P.sens, tL) + ldg(P.sens, tR));
+        const double om_avg = 0.5 * (ldg(P.sens + vs, tL) + ldg(P.sens + vs, tR));
         const double sensor = div_rn(-th_avg, fabs(th_avg) + om_avg + kEps);
         use_hll = sensor > 0.65;
     }
@@ -479,8 +475,6 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
     } else {
         cons_flux(VR, rhoR, epsR, QR, FR);
     }
-    double Fo[NEQ];
-    double ustar;
     {
         const double* Vs = left ? VL : VR;
         const double* Qs = left ? QL : QR;
@@ -544,13 +538,36 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
         }
     }
 
-    const int64_t mv = B.mid_vol(A);
-    double* out = P.fmid[A];
-#pragma unroll
-    for (int e = 0; e < NEQ; e++) out[e * mv + loc] = Fo[e];
-    if constexpr (!FOUR) P.ustar[A][loc] = ustar;
 }
 
+// Midpoint face `loc` (numbered with x fastest over faces -1..n_A+1 along A)
+// of the unfused pipeline: its flux to fmid[A], u* to ustar[A].
+template <int D, int NS, bool FOUR, int A, int SCHEME, int PP, int QQ>
+__device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t loc, const KParams& k) {
+    constexpr int NEQ = Multi<D, NS, FOUR>::NEQ;
+    const Boxes<D> B(P);
+    const int m0 = B.mid_ext(A, 0), m1 = B.mid_ext(A, 1);
+    // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
+    const uint32_t r = uint32_t(loc) / uint32_t(m0);
+    int c0 = int(uint32_t(loc) - r * uint32_t(m0));
+    int c1 = (D > 1) ? int((D > 2) ? r % uint32_t(m1) : r) : 0;
+    int c2 = (D > 2) ? int(r / uint32_t(m1)) : 0;
+    if constexpr (A == 0) c0 -= 1;
+    if constexpr (A == 1) c1 -= 1;
+    if constexpr (A == 2) c2 -= 1;
+    double Fo[NEQ];
+    double ustar;
+    midflux_multi_at<D, NS, FOUR, A, SCHEME, PP, QQ>(P, B, c0, c1, c2, k, Fo, ustar);
+    const int64_t mv = B.mid_vol(A);
+    double* out = P.fmid[A];
+    const uint32_t l32 = uint32_t(loc);
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) stg(out + e * mv, l32, Fo[e]);
+    if constexpr (!FOUR) stg(P.ustar[A], l32, ustar);
+}
+
+// Midpoint-flux stage: the midpoint faces of one axis per segment, fluxes to
+// memory (face flux in stage 3).
 template <int D, int NS, bool FOUR, int SCHEME, int PP, int QQ, class W>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HM_MIDFLUX_MULTI_WAVES))) void k_midflux_multi(
     W w, KParams k) {
@@ -574,10 +591,54 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HM_MIDFL
 // Midpoint-to-face flux with the model's node flux
 // (FlowModelFiveEqnAllaire.cpp:4877-5901, FlowModelFourEqnConservative.cpp:5499-6338).
 // ----------------------------------------------------------------------------
-template <int D, int NS, bool FOUR, int A>
-__device__ __forceinline__ void faceflux_multi_face(const DevPatch& P, int64_t loc, const KParams& k) {
+// Node flux of equation e at a cell (FlowModelFiveEqnAllaire.cpp:4877-5101,
+// FlowModelFourEqnConservative.cpp:5499-5700) along A, from its conservative
+// variables at `q` in the Q box (below 2^29 cells) and its velocity ua along A
+// and pressure p.
+template <int D, int NS, int A>
+__device__ __forceinline__ double node_flux_multi(const DevPatch& P, uint32_t q, double ua, double p, int e) {
+    constexpr int IP = NS + D;
+    if (e == IP) return ua * (ldg(P.Q[IP], q) + p);
+    if (e == NS + A) return ua * ldg(P.Q[e], q) + p;
+    // partial densities, transverse momentum and volume fractions: u_a * Q
+    return ua * ldg(P.Q[e], q);
+}
+
+// Output flux of face (c0, c1, c2) along A from the midpoint fluxes of faces
+// f - 1, f, f + 1 (m(side, e): side -1, 0, +1) and the node fluxes of cells
+// f - 1 and f: dt (1/30 (m_{f+1} + m_{f-1}) - 3/10 (n_f + n_{f-1}) + 23/15 m_f)
+// (WCNS56-HLLC-HLL.cpp:461-485, 1200-1282, 2327-2490).
+template <int D, int NS, bool FOUR, int A, class MF>
+__device__ __forceinline__ void store_face_multi(const DevPatch& P, const Boxes<D>& B, int c0, int c1, int c2,
+                                                 double dt, MF&& m) {
     constexpr int NEQ = Multi<D, NS, FOUR>::NEQ;
     constexpr int IP = NS + D;
+    const int64_t v4 = B.vol(kGc);
+    const uint32_t st4 = (A == 0) ? 1u : (A == 1) ? uint32_t(B.ext(0, kGc)) : uint32_t(B.ext(0, kGc) * B.ext(1, kGc));
+    const uint32_t pR = uint32_t(B.idx(kGc, c0, c1, c2)), pL = pR - st4;
+    const uint32_t qst = (A == 0) ? 1u
+                         : (A == 1) ? uint32_t(P.n[0] + 2 * P.gq[0])
+                                    : uint32_t((P.n[0] + 2 * P.gq[0]) * (P.n[1] + 2 * P.gq[1]));
+    const uint32_t qR = uint32_t(q_idx<D>(P, c0, c1, c2)), qL = qR - qst;
+    const double uaL = ldg(P.prim + (NS + A) * v4, pL), uaR = ldg(P.prim + (NS + A) * v4, pR);
+    const double prL = ldg(P.prim + IP * v4, pL), prR = ldg(P.prim + IP * v4, pR);
+    const int64_t fv = B.face_vol(A);
+    const uint32_t fi = uint32_t(B.face_idx(A, c0, c1, c2));
+    double* out = P.F[A];
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        const double nL = node_flux_multi<D, NS, A>(P, qL, uaL, prL, e);
+        const double nR = node_flux_multi<D, NS, A>(P, qR, uaR, prR, e);
+        const double mR = m(1, e), mL = m(-1, e), mM = m(0, e);
+        stg(out + e * fv, fi,
+            dt * (double(1) / double(30) * (mR + mL) - double(3) / double(10) * (nR + nL) +
+                  double(23) / double(15) * mM));
+    }
+}
+
+// Output face `loc` of the unfused pipeline, from the midpoint fluxes in fmid[A].
+template <int D, int NS, bool FOUR, int A>
+__device__ __forceinline__ void faceflux_multi_face(const DevPatch& P, int64_t loc, const KParams& k) {
     const Boxes<D> B(P);
     const int f0 = B.face_ext(A, 0), f1 = B.face_ext(A, 1);
     // 32-bit index arithmetic (a work segment has fewer than 2^31 items).
@@ -586,38 +647,11 @@ __device__ __forceinline__ void faceflux_multi_face(const DevPatch& P, int64_t l
     const int c1 = (D > 1) ? int((D > 2) ? r % uint32_t(f1) : r) : 0;
     const int c2 = (D > 2) ? int(r / uint32_t(f1)) : 0;
     const int64_t mv = B.mid_vol(A);
-    const int64_t mC = B.mid_idx(A, c0, c1, c2);
-    const int64_t mst = (A == 0) ? 1 : (A == 1) ? int64_t(B.mid_ext(A, 0)) : int64_t(B.mid_ext(A, 0)) * B.mid_ext(A, 1);
-    const int64_t v4 = B.vol(kGc);
-    const int64_t st4 = (A == 0) ? 1 : (A == 1) ? int64_t(B.ext(0, kGc)) : int64_t(B.ext(0, kGc)) * B.ext(1, kGc);
-    const int64_t pR = B.idx(kGc, c0, c1, c2), pL = pR - st4;
-    const int qst = (A == 0) ? 1 : (A == 1) ? (P.n[0] + 2 * P.gq[0]) : (P.n[0] + 2 * P.gq[0]) * (P.n[1] + 2 * P.gq[1]);
-    const int64_t qR = q_idx<D>(P, c0, c1, c2), qL = qR - qst;
-    const double uaL = P.prim[(NS + A) * v4 + pL], uaR = P.prim[(NS + A) * v4 + pR];
-    const double prL = P.prim[IP * v4 + pL], prR = P.prim[IP * v4 + pR];
-    const int64_t fv = B.face_vol(A);
+    const uint32_t mC = uint32_t(B.mid_idx(A, c0, c1, c2));
+    const uint32_t mst = (A == 0) ? 1u : (A == 1) ? uint32_t(B.mid_ext(A, 0)) : uint32_t(B.mid_ext(A, 0) * B.mid_ext(A, 1));
     const double* fm = P.fmid[A];
-    double* out = P.F[A];
-    const double dt = kdt(k);
-#pragma unroll
-    for (int e = 0; e < NEQ; e++) {
-        double nL, nR;
-        if (e == IP) {
-            nL = uaL * (P.Q[IP][qL] + prL);
-            nR = uaR * (P.Q[IP][qR] + prR);
-        } else if (e == NS + A) {
-            nL = uaL * P.Q[e][qL] + prL;
-            nR = uaR * P.Q[e][qR] + prR;
-        } else {
-            // partial densities, transverse momentum and volume fractions: u_a * Q
-            nL = uaL * P.Q[e][qL];
-            nR = uaR * P.Q[e][qR];
-        }
-        const double* f = fm + e * mv;
-        const double mR = f[mC + mst], mL = f[mC - mst], mM = f[mC];
-        out[e * fv + loc] = dt * (double(1) / double(30) * (mR + mL) - double(3) / double(10) * (nR + nL) +
-                                  double(23) / double(15) * mM);
-    }
+    store_face_multi<D, NS, FOUR, A>(P, B, c0, c1, c2, kdt(k),
+                                     [&](int side, int e) { return ldg(fm + e * mv, mC + uint32_t(side) * mst); });
 }
 
 template <int D, int NS, bool FOUR, class W>
