@@ -125,6 +125,9 @@ def _load():
     L.hm_compute_convective_flux_and_source_on_patch.argtypes = [
         P, ctypes.POINTER(hm_patch_view), ctypes.c_double, ctypes.c_double, ctypes.c_int, P]
     L.hm_compute_convective_flux_and_source_on_patch.restype = ctypes.c_int
+    L.hm_compute_convective_flux_and_source_on_patch_noalloc.argtypes = [
+        P, ctypes.POINTER(hm_patch_view), ctypes.c_double, ctypes.c_double, ctypes.c_int, P]
+    L.hm_compute_convective_flux_and_source_on_patch_noalloc.restype = ctypes.c_int
     L.hm_compute_convective_flux_and_source_on_patches.argtypes = [
         P, ctypes.POINTER(hm_patch_view), ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, P]
     L.hm_compute_convective_flux_and_source_on_patches.restype = ctypes.c_int

@@ -10,9 +10,11 @@ int hm_ctx_create(const hm_config* cfg, hm_ctx** out) {
     hm_ctx* c = new (std::nothrow) hm_ctx();
     if (!c) return HM_ERR_OUT_OF_MEMORY;
     std::string err;
-    const int st = hm::make_reconstructor(*cfg, c->rec, err);
+    int st = hm::make_reconstructor(*cfg, c->rec, err);
+    if (st == HM_OK) st = hm::noalloc_setup(cfg->device, err);
     if (st != HM_OK) {
         // The failed context is still returned so that hm_last_error can explain the failure.
+        c->rec.reset();
         c->err = err;
         *out = c;
         return st;

@@ -165,6 +165,30 @@ class ConvectiveFluxReconstructorGPU {
         return run_ordered(plan, k, s);
     }
 
+    // The per-patch RHS with its scratch in a caller-given device region of
+    // `cap` doubles starting on a 256-byte boundary (the allocation-free
+    // entry point's static array): no arena, no tables, no profiling events,
+    // no allocation.
+    int computeOnPatchInScratch(const hm_patch_view& v, double dt, hipStream_t s, double* scratch, size_t cap) {
+        if (int st = check_view(v, 0)) return st;
+        if (int st = select_device()) return st;
+        const ScratchLayout L = layout(v);
+        if (L.total > cap) {
+            char buf[200];
+            snprintf(buf, sizeof buf,
+                     "patch needs %zu doubles of scratch; the allocation-free entry point holds %zu "
+                     "(use hm_compute_convective_flux_and_source_on_patch)",
+                     L.total, cap);
+            return fail(HM_ERR_INVALID_ARGUMENT, buf);
+        }
+        const Plan<SegSingle> plan = patch_plan(v, scratch, L);
+        KParams k = k_;
+        k.dt = dt;
+        return run_stages(plan, k, s, 0, kStages);
+    }
+
+    int device() const { return cfg_.device; }
+
     int computeConvectiveFluxAndSourceOnPatches(const hm_patch_view* views, int np, double time, double dt,
                                                 int rk_step, hipStream_t s) {
         (void)time;

@@ -126,6 +126,24 @@ int hm_compute_convective_flux_and_source_on_patch(hm_ctx* ctx, const hm_patch_v
                                                     double dt, int rk_step, void* stream);
 
 /*
+ * Allocation-free variant of hm_compute_convective_flux_and_source_on_patch
+ * (no counterpart in the reference): for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Same arguments, semantics and results (bit for bit); the
+ * pipeline's scratch is a static device array of HM_NOALLOC_SCRATCH_DOUBLES
+ * doubles (64 MiB) per device, set up by hm_ctx_create and held until the
+ * process ends, instead of the context's arena.  A patch whose scratch does
+ * not fit (a 2D 128^2 single-species patch needs 0.2 Mi doubles, a 3D 64^3
+ * five-equation patch 6.8 Mi) is rejected with HM_ERR_INVALID_ARGUMENT.
+ * Every call on a device shares that device's array: calls that overlap on
+ * one device (on different streams, or from several threads or contexts)
+ * race; order them on one stream or synchronize between them.
+ */
+#define HM_NOALLOC_SCRATCH_DOUBLES (8u << 20)
+int hm_compute_convective_flux_and_source_on_patch_noalloc(hm_ctx* ctx, const hm_patch_view* patch, double time,
+                                                            double dt, int rk_step, void* stream);
+
+/*
  * Batched form of hm_compute_convective_flux_and_source_on_patch for all
  * patches of a level (uneven sizes
  * allowed): one work-list driven launch per kernel stage instead of one
