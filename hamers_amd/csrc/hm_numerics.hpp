@@ -132,8 +132,12 @@ struct LDConst {
 
 // One side of the WCNS6-LD nonlinear midpoint interpolation
 // (WCNS6-LD-HLLC-HLL.cpp:133-223; the plus side 229-319 is this on flip(s)).
+// The upwind weights u_k and the shock measure sigma enter only where the
+// ratio R_tau = tau6 / (beta_avg + eps) exceeds alpha_tau, so they are formed
+// only there (the same expressions, evaluated on demand): sigma of the
+// unflipped stencil s0 is computed once per field, into sig (< 0: not yet).
 template <int P, int Q>
-HM_HD double ld_side(const Six& s, double sigma, const LDConst& k) {
+HM_HD double ld_side(const Six& s, const Six& s0, double& sig, const LDConst& k) {
     double b0, b1, b2;
     beta3(s, b0, b1, b2);
     const double b3 = beta6(s);
@@ -141,15 +145,6 @@ HM_HD double ld_side(const Six& s, double sigma, const LDConst& k) {
     // e_k = b_k + eps divides both tau5 and tau6: one reciprocal each.
     const double e0 = b0 + kEps, e1 = b1 + kEps, e2 = b2 + kEps, e3 = b3 + kEps;
     const double r0 = rcp_rn(e0), r1 = rcp_rn(e1), r2 = rcp_rn(e2);
-    const double tau5 = fabs(b0 - b2);
-    double u0 = double(1) / double(16) * (double(1) + rep_pow<P>(quot(tau5, e0, r0), k.p));
-    double u1 = double(5) / double(8) * (double(1) + rep_pow<P>(quot(tau5, e1, r1), k.p));
-    double u2 = double(5) / double(16) * (double(1) + rep_pow<P>(quot(tau5, e2, r2), k.p));
-    const double us = u0 + u1 + u2;
-    const double rus = rcp_rn(us);
-    u0 = quot(u0, us, rus);
-    u1 = quot(u1, us, rus);
-    u2 = quot(u2, us, rus);
 
     const double bavg = double(1) / double(8) * (b0 + b2 + double(6) * b1);
     const double tau6 = fabs(b3 - bavg);
@@ -165,6 +160,17 @@ HM_HD double ld_side(const Six& s, double sigma, const LDConst& k) {
     w3 = quot(w3, ws, rws);
 
     if (div_rn(tau6, bavg + kEps) > k.alpha_tau) {
+        const double tau5 = fabs(b0 - b2);
+        double u0 = double(1) / double(16) * (double(1) + rep_pow<P>(quot(tau5, e0, r0), k.p));
+        double u1 = double(5) / double(8) * (double(1) + rep_pow<P>(quot(tau5, e1, r1), k.p));
+        double u2 = double(5) / double(16) * (double(1) + rep_pow<P>(quot(tau5, e2, r2), k.p));
+        const double us = u0 + u1 + u2;
+        const double rus = rcp_rn(us);
+        u0 = quot(u0, us, rus);
+        u1 = quot(u1, us, rus);
+        u2 = quot(u2, us, rus);
+        if (sig < double(0)) sig = ld_sigma(s0);
+        const double sigma = sig;
         w0 = sigma * u0 + (double(1) - sigma) * w0;
         w1 = sigma * u1 + (double(1) - sigma) * w1;
         w2 = sigma * u2 + (double(1) - sigma) * w2;
@@ -200,7 +206,7 @@ enum { kWCNS5_JS = 0, kWCNS6_LD = 1 };
 template <int SCHEME, int P, int Q>
 HM_HD void reconstruct_pair(const Six& s, const LDConst& k, double& um, double& up) {
     if constexpr (SCHEME == kWCNS6_LD) {
-        const double sig = ld_sigma(s);
+        double sig = double(-1);  // sigma >= 0, formed on the first side that blends
         // The two sides one after the other (a runtime loop), so that only one
         // side's indicators and weights are live at a time.
 #pragma unroll 1
@@ -208,7 +214,7 @@ HM_HD void reconstruct_pair(const Six& s, const LDConst& k, double& um, double& 
             Six ss;
 #pragma unroll
             for (int m = 0; m < 6; m++) ss.v[m] = side ? s.v[5 - m] : s.v[m];
-            const double r = ld_side<P, Q>(ss, sig, k);
+            const double r = ld_side<P, Q>(ss, s, sig, k);
             if (side)
                 up = r;
             else
