@@ -22,6 +22,8 @@ HM_MODEL_SINGLE_SPECIES = 0
 HM_MODEL_FOUR_EQN_CONSERVATIVE = 1
 HM_MODEL_FIVE_EQN_ALLAIRE = 2
 
+HM_BC_NONE, HM_BC_FLOW, HM_BC_REFLECT, HM_BC_SYMMETRY, HM_BC_DIRICHLET = range(5)
+
 HM_SCHEME_WCNS5_JS_HLLC_HLL = 0
 HM_SCHEME_WCNS6_LD_HLLC_HLL = 1
 
@@ -152,6 +154,8 @@ def _load():
     L.hm_level_read_patch.argtypes = [P, ctypes.c_int, D]
     L.hm_level_patch_array.argtypes = [P, ctypes.c_int, ctypes.POINTER(P), I]
     L.hm_level_fill_ghosts.argtypes = [P, P]
+    L.hm_level_set_boundary_conditions.argtypes = [P, I, D]
+    L.hm_level_set_boundary_conditions.restype = ctypes.c_int
     L.hm_level_advance.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.c_int, P]
     L.hm_convective_flux_vjp_on_patch.argtypes = [P, ctypes.POINTER(hm_vjp_patch_view), ctypes.c_double,
                                                   ctypes.c_double, ctypes.c_int, P]

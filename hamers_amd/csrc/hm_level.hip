@@ -2,17 +2,21 @@
 // RungeKuttaLevelIntegrator::advanceLevel (RungeKuttaLevelIntegrator.cpp:1457-1939,
 // stage loop 1672-1745, default SSP-RK3 weights 3907-3924), entirely on the GPU:
 //
-//   fill ghosts of the scratch state (same level + periodic images)      :1568
+//   fill ghosts of the scratch state (same level + periodic images,
+//   then the physical boundary conditions)                              :1568
 //   for each stage sn:
 //     intermediate[sn] <- scratch (whole ghost box)                       :1677
-//     sn > 0: fill ghosts of intermediate[sn] (same level only)           :1686-1711
+//     sn > 0: fill ghosts of intermediate[sn] (the same fill)             :1686-1711
 //     S[sn] <- 0, F[sn], S[sn] <- batched convective RHS of intermediate  Euler.cpp:904-1001
 //     scratch interior <- stage update from intermediate[0..sn]           Euler.cpp:1003-1680
 //
 // One step is a fixed sequence of launches on one stream, so it is captured
-// once into a hipGraph and replayed.  Ghost cells that no same-level patch
-// covers (coarse-fine boundaries) keep the values written by the caller, as
-// the reference's intermediate fills do not use the coarser level.
+// once into a hipGraph and replayed.  Ghost cells inside the domain that no
+// same-level patch covers (coarse-fine boundaries) keep the values written by
+// the caller, as the reference's intermediate fills do not use the coarser
+// level; ghost cells outside the domain get the physical boundary conditions
+// set for their sides (hm_level_set_boundary_conditions), else also keep the
+// caller's values.
 //
 // Also here: the stage-update entry points for callers that keep their own
 // patch data (per patch and batched).
@@ -68,6 +72,64 @@ __global__ __launch_bounds__(kBlock) void k_halo(SegTableT<HaloItem> w, const Le
     const int64_t id = gidx(Pd, d0, d1, d2);
     const int64_t is = gidx(Ps, d0 + it->off[0], d1 + it->off[1], d2 + it->off[2]);
     for (int c = 0; c < nq; c++) Pd.Q[buf][c * Pd.gvol + id] = Ps.Q[buf][c * Ps.gvol + is];
+}
+
+// Physical boundary conditions of the ghost cells outside the domain
+// (BasicCartesianBoundaryUtilities1.cpp, BasicCartesianBoundaryUtilities2.cpp:215-440
+// edge and 590-900 node fills, BasicCartesianBoundaryUtilities3.cpp face, edge
+// and node fills), applied to every conservative variable as
+// EulerBoundaryConditions::setPhysicalBoundaryConditions does
+// (EulerBoundaryConditions.cpp:749-1100): FLOW copies the boundary cell,
+// REFLECT and SYMMETRY copy the mirror image about the boundary face, REFLECT
+// negating the momentum component normal to it (scalar variables take
+// SYMMETRY where the deck says REFLECT, EulerBoundaryConditions.cpp:196-345),
+// DIRICHLET writes fixed values.  A cell outside the domain along several axes
+// (an edge or node box) composes the rules of those axes -- the reference's
+// node and edge conditions named after the x axis, whose pivot lies in the
+// box of the next axis -- so every item reads only cells inside the domain
+// (interior or same-level ghosts, filled before) and the items run in one
+// launch.  With a DIRICHLET axis among them, the first such axis (x, y, z
+// order) gives the values.
+struct BcItem {
+    int dst;
+    int lo[3], ext[3];  // region in the patch's ghost-box coordinates
+    int mode[3];        // per axis: 0 same coordinate, 1 mirror (source = par - x), 2 clamp (source = par)
+    int par[3];
+    int flip;           // bit b: momentum component b negated
+    int side;           // >= 0: the DIRICHLET values of side `side` (2 axis + upper)
+};
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_bc(SegTableT<BcItem> w, const LevelPatch* patches, int buf, int nq,
+                                               int g, int mom0, const double* dir) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const BcItem* it;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, it, ax, loc)) return;
+    const uint32_t r = uint32_t(loc) / uint32_t(it->ext[0]);
+    const int x[3] = {it->lo[0] + int(uint32_t(loc) - r * uint32_t(it->ext[0])),
+                      it->lo[1] + ((D > 1) ? int((D > 2) ? r % uint32_t(it->ext[1]) : r) : 0),
+                      it->lo[2] + ((D > 2) ? int(r / uint32_t(it->ext[1])) : 0)};
+    int y[3];
+    for (int a = 0; a < 3; a++) y[a] = it->mode[a] == 0 ? x[a] : (it->mode[a] == 1 ? it->par[a] - x[a] : it->par[a]);
+    const LevelPatch& P = patches[it->dst];
+    const int e0 = P.n[0] + 2 * g;
+    const int e1 = (D > 1) ? P.n[1] + 2 * g : 1;
+    const int64_t id = x[0] + int64_t(x[1]) * e0 + int64_t(x[2]) * e0 * e1;
+    const int64_t is = y[0] + int64_t(y[1]) * e0 + int64_t(y[2]) * e0 * e1;
+    double* Q = P.Q[buf];
+    for (int c = 0; c < nq; c++) {
+        double v;
+        if (it->side >= 0) {
+            v = dir[it->side * nq + c];
+        } else {
+            v = Q[c * P.gvol + is];
+            const int b = c - mom0;
+            if (b >= 0 && b < D && ((it->flip >> b) & 1)) v = -v;
+        }
+        Q[c * P.gvol + id] = v;
+    }
 }
 
 // Whole-ghost-box copy between two state buffers of every patch.
@@ -401,6 +463,35 @@ class Level {
 
     int fillGhosts(hipStream_t s) { return launch_halo(0, s); }
 
+    // Physical boundary conditions of the sides of non-periodic axes
+    // (HM_BC_*, xlo, xhi, ylo, yhi, zlo, zhi); dirichlet: [2 dim][nq] values.
+    int setBoundaryConditions(const int* cond, const double* dirichlet) {
+        int c[6] = {HM_BC_NONE, HM_BC_NONE, HM_BC_NONE, HM_BC_NONE, HM_BC_NONE, HM_BC_NONE};
+        for (int sd = 0; sd < 2 * d_; sd++) {
+            c[sd] = cond[sd];
+            if (c[sd] < HM_BC_NONE || c[sd] > HM_BC_DIRICHLET)
+                return rec_->fail(HM_ERR_INVALID_ARGUMENT, "unknown boundary condition");
+            if (c[sd] != HM_BC_NONE && periodic_[sd / 2])
+                return rec_->fail(HM_ERR_INVALID_ARGUMENT, "a periodic axis takes no boundary condition");
+            if (c[sd] != HM_BC_NONE && dn_[sd / 2] < g_)
+                return rec_->fail(HM_ERR_INVALID_ARGUMENT, "domain narrower than the ghost width on a boundary axis");
+            if (c[sd] == HM_BC_DIRICHLET && !dirichlet)
+                return rec_->fail(HM_ERR_INVALID_ARGUMENT, "DIRICHLET sides need their values");
+        }
+        // Launches already queued read the tables rewritten below.
+        if (hipDeviceSynchronize() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "device synchronize failed");
+        drop_graphs();
+        for (int sd = 0; sd < 6; sd++) {
+            bc_cond_[sd] = c[sd];
+            for (int e = 0; e < nq_; e++)
+                bc_values_[sd * nq_ + e] = (c[sd] == HM_BC_DIRICHLET) ? dirichlet[sd * nq_ + e] : 0.0;
+        }
+        has_bc_ = false;
+        for (int sd = 0; sd < 6; sd++) has_bc_ = has_bc_ || bc_cond_[sd] != HM_BC_NONE;
+        return build_tables();
+    }
+    bool hasBoundaryConditions() const { return has_bc_; }
+
     // num_steps steps of the configured Runge-Kutta scheme with fixed dt.
     int advance(double dt, int nsteps, bool use_graph, hipStream_t s) {
         if (nsteps < 0 || !(dt > 0.0)) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "bad dt or step count");
@@ -483,6 +574,10 @@ class Level {
     // steps, for a later adjoint().
     int advanceRecorded(double dt, int nsteps, int every, bool use_graph, hipStream_t s) {
         if (nsteps < 1 || every < 1 || !(dt > 0.0)) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "bad dt, step count or interval");
+        if (has_bc_)
+            return rec_->fail(HM_ERR_UNSUPPORTED,
+                              "the adjoint of the physical boundary conditions is not built (periodic and "
+                              "same-level ghost fills only)");
         if (int st = rec_->selectDevice()) return st;
         if (int st = ensure_adjoint(nsteps, every)) return st;
         hipStream_t x = s;
@@ -735,9 +830,98 @@ class Level {
         }
     }
 
+    // Ghost regions outside the domain along the axes with a physical
+    // boundary condition: for each patch, the ghost box split per axis into
+    // below / inside / above the domain, one item per region outside along at
+    // least one such axis.
+    void build_bc(TableImage<BcItem>& img) {
+        for (int p = 0; p < np_; p++) {
+            int rlo[3][3], rhi[3][3], nr[3];  // ranges per axis (global, inclusive); range kind 0 inside, 1 below, 2 above
+            int kind[3][3];
+            for (int a = 0; a < 3; a++) {
+                nr[a] = 0;
+                if (a >= d_) {
+                    rlo[a][0] = rhi[a][0] = 0;
+                    kind[a][0] = 0;
+                    nr[a] = 1;
+                    continue;
+                }
+                const int glo = lo_[3 * p + a] - g_, ghi = lo_[3 * p + a] + n_[3 * p + a] - 1 + g_;
+                const int dlo = dlo_[a], dhi = dlo_[a] + dn_[a] - 1;
+                const bool bcl = !periodic_[a] && bc_cond_[2 * a] != HM_BC_NONE;
+                const bool bch = !periodic_[a] && bc_cond_[2 * a + 1] != HM_BC_NONE;
+                const int ilo = bcl ? std::max(glo, dlo) : glo, ihi = bch ? std::min(ghi, dhi) : ghi;
+                if (bcl && glo < dlo) {
+                    rlo[a][nr[a]] = glo, rhi[a][nr[a]] = dlo - 1, kind[a][nr[a]] = 1;
+                    nr[a]++;
+                }
+                if (ilo <= ihi) {
+                    rlo[a][nr[a]] = ilo, rhi[a][nr[a]] = ihi, kind[a][nr[a]] = 0;
+                    nr[a]++;
+                }
+                if (bch && ghi > dhi) {
+                    rlo[a][nr[a]] = dhi + 1, rhi[a][nr[a]] = ghi, kind[a][nr[a]] = 2;
+                    nr[a]++;
+                }
+            }
+            for (int r0 = 0; r0 < nr[0]; r0++)
+                for (int r1 = 0; r1 < nr[1]; r1++)
+                    for (int r2 = 0; r2 < nr[2]; r2++) {
+                        const int rr[3] = {r0, r1, r2};
+                        if (kind[0][r0] == 0 && kind[1][r1] == 0 && kind[2][r2] == 0) continue;
+                        BcItem it{};
+                        it.dst = p;
+                        it.side = -1;
+                        int64_t cells = 1;
+                        for (int a = 0; a < 3; a++) {
+                            const int k = kind[a][rr[a]];
+                            const int base = (a < d_) ? lo_[3 * p + a] - g_ : 0;
+                            it.lo[a] = rlo[a][rr[a]] - base;
+                            it.ext[a] = rhi[a][rr[a]] - rlo[a][rr[a]] + 1;
+                            cells *= it.ext[a];
+                            if (k == 0) continue;
+                            const int sd = 2 * a + (k == 2 ? 1 : 0);
+                            const int face = (k == 1) ? 2 * dlo_[a] - 1 : 2 * (dlo_[a] + dn_[a] - 1) + 1;
+                            switch (bc_cond_[sd]) {
+                                case HM_BC_FLOW:
+                                    it.mode[a] = 2;
+                                    it.par[a] = ((k == 1) ? dlo_[a] : dlo_[a] + dn_[a] - 1) - base;
+                                    break;
+                                case HM_BC_REFLECT:
+                                    it.flip |= 1 << a;
+                                    it.mode[a] = 1;
+                                    it.par[a] = face - 2 * base;
+                                    break;
+                                case HM_BC_SYMMETRY:
+                                    it.mode[a] = 1;
+                                    it.par[a] = face - 2 * base;
+                                    break;
+                                default:  // HM_BC_DIRICHLET
+                                    if (it.side < 0) it.side = sd;
+                                    break;
+                            }
+                        }
+                        // every source cell lies in the patch's ghost box (the kernel does not check)
+                        for (int a = 0; a < 3; a++) {
+                            const int ge = (a < d_) ? n_[3 * p + a] + 2 * g_ : 1;
+                            for (int x : {it.lo[a], it.lo[a] + it.ext[a] - 1}) {
+                                const int y = it.mode[a] == 0 ? x : (it.mode[a] == 1 ? it.par[a] - x : it.par[a]);
+                                if (x < 0 || x >= ge || y < 0 || y >= ge) bc_bad_ = true;
+                            }
+                        }
+                        img.items.push_back(it);
+                        img.add(int(img.items.size()) - 1, 0, cells);
+                    }
+        }
+    }
+
     int build_tables() {
         TableImage<HaloItem> halo;
         build_halo(halo);
+        TableImage<BcItem> bc;
+        bc_bad_ = false;
+        build_bc(bc);
+        if (bc_bad_) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "boundary-condition source outside a ghost box");
         TableImage<LevelPatch> copy, zero;
         copy.items = patch_;
         zero.items = patch_;
@@ -747,9 +931,11 @@ class Level {
         }
         halo_host_ = halo.items;
         halo_items_ = halo.total;
+        bc_items_ = bc.total;
         copy_items_ = copy.total;
         zero_items_ = zero.total;
-        const size_t bytes = halo.bytes() + copy.bytes() + zero.bytes() + rk_bytes();
+        const size_t dir_bytes = up256(sizeof(double) * 6 * size_t(nq_));
+        const size_t bytes = halo.bytes() + bc.bytes() + dir_bytes + copy.bytes() + zero.bytes() + rk_bytes();
         if (tables_) (void)hipFree(tables_);
         tables_ = nullptr;
         if (hipMalloc(&tables_, bytes) != hipSuccess) return rec_->fail(HM_ERR_OUT_OF_MEMORY, "level tables");
@@ -758,6 +944,11 @@ class Level {
         size_t o = 0;
         halo_ = halo.write(h.data() + o, d + o);
         o += halo.bytes();
+        bc_table_ = bc.write(h.data() + o, d + o);
+        o += bc.bytes();
+        std::memcpy(h.data() + o, bc_values_, sizeof(double) * 6 * size_t(nq_));
+        bc_dir_ = reinterpret_cast<const double*>(d + o);
+        o += dir_bytes;
         copy_ = copy.write(h.data() + o, d + o);
         o += copy.bytes();
         zero_ = zero.write(h.data() + o, d + o);
@@ -826,15 +1017,27 @@ class Level {
         return HM_OK;
     }
 
+    // Ghost fill of state buffer buf: same-level copies (periodic images
+    // included), then the physical boundary conditions, which read them.
     int launch_halo(int buf, hipStream_t s) {
-        if (halo_items_ == 0) return HM_OK;
-        switch (d_) {
-            case 1: hipLaunchKernelGGL(k_halo<1>, dim3(blocks_for(halo_items_)), dim3(kBlock), 0, s, halo_, patch_dev(), buf, nq_, g_); break;
-            case 2: hipLaunchKernelGGL(k_halo<2>, dim3(blocks_for(halo_items_)), dim3(kBlock), 0, s, halo_, patch_dev(), buf, nq_, g_); break;
-            default: hipLaunchKernelGGL(k_halo<3>, dim3(blocks_for(halo_items_)), dim3(kBlock), 0, s, halo_, patch_dev(), buf, nq_, g_); break;
+        if (halo_items_ > 0) {
+            switch (d_) {
+                case 1: hipLaunchKernelGGL(k_halo<1>, dim3(blocks_for(halo_items_)), dim3(kBlock), 0, s, halo_, patch_dev(), buf, nq_, g_); break;
+                case 2: hipLaunchKernelGGL(k_halo<2>, dim3(blocks_for(halo_items_)), dim3(kBlock), 0, s, halo_, patch_dev(), buf, nq_, g_); break;
+                default: hipLaunchKernelGGL(k_halo<3>, dim3(blocks_for(halo_items_)), dim3(kBlock), 0, s, halo_, patch_dev(), buf, nq_, g_); break;
+            }
         }
-        return hipGetLastError() == hipSuccess ? HM_OK : rec_->fail(HM_ERR_DEVICE, "halo launch failed");
+        if (bc_items_ > 0) {
+            const int mom0 = rec_->config().model == HM_MODEL_SINGLE_SPECIES ? 1 : rec_->config().num_species;
+            switch (d_) {
+                case 1: hipLaunchKernelGGL(k_bc<1>, dim3(blocks_for(bc_items_)), dim3(kBlock), 0, s, bc_table_, patch_dev(), buf, nq_, g_, mom0, bc_dir_); break;
+                case 2: hipLaunchKernelGGL(k_bc<2>, dim3(blocks_for(bc_items_)), dim3(kBlock), 0, s, bc_table_, patch_dev(), buf, nq_, g_, mom0, bc_dir_); break;
+                default: hipLaunchKernelGGL(k_bc<3>, dim3(blocks_for(bc_items_)), dim3(kBlock), 0, s, bc_table_, patch_dev(), buf, nq_, g_, mom0, bc_dir_); break;
+            }
+        }
+        return hipGetLastError() == hipSuccess ? HM_OK : rec_->fail(HM_ERR_DEVICE, "ghost fill launch failed");
     }
+
 
     const LevelPatch* patch_dev() const { return copy_.patches; }
 
@@ -1272,6 +1475,13 @@ class Level {
     DtBuffers dt_buf_;
     ConvectiveFluxReconstructorGPU::OwnedPlan rhs_plan_[kMaxRK];
     SegTableT<HaloItem> halo_{};
+    SegTableT<BcItem> bc_table_{};
+    const double* bc_dir_ = nullptr;
+    int64_t bc_items_ = 0;
+    int bc_cond_[6] = {HM_BC_NONE, HM_BC_NONE, HM_BC_NONE, HM_BC_NONE, HM_BC_NONE, HM_BC_NONE};
+    double bc_values_[6 * (HM_MAX_EQN + 1)] = {};
+    bool has_bc_ = false;
+    bool bc_bad_ = false;
     SegTableT<LevelPatch> copy_{}, zero_{};
     SegTableT<RKPatch> rk_[kMaxRK] = {};
     RKParams rkp_[kMaxRK] = {};
@@ -1516,6 +1726,12 @@ int hm_level_read_patch(hm_level* level, int patch, double* Q) {
 int hm_level_patch_array(hm_level* level, int patch, double** Q, int* ghost) {
     if (!level) return HM_ERR_INVALID_ARGUMENT;
     return level->lvl->patchArray(patch, Q, ghost);
+}
+
+int hm_level_set_boundary_conditions(hm_level* level, const int* conditions, const double* dirichlet) {
+    if (!level || !conditions) return HM_ERR_INVALID_ARGUMENT;
+    if (int st = level->ctx->rec->selectDevice()) return st;
+    return level->lvl->setBoundaryConditions(conditions, dirichlet);
 }
 
 int hm_level_fill_ghosts(hm_level* level, void* stream) {

@@ -208,6 +208,25 @@ class Level:
         capi.check(self.rec._ctx, capi.lib.hm_level_patch_array(self._lvl, i, ctypes.byref(ptr), ctypes.byref(g)))
         return ptr.value, g.value
 
+    # Physical boundary conditions (include/hamers_convective.h HM_BC_*).
+    BC = {"NONE": 0, "FLOW": 1, "REFLECT": 2, "SYMMETRY": 3, "DIRICHLET": 4}
+
+    def set_boundary_conditions(self, conditions, dirichlet=None):
+        """Conditions of the non-periodic sides, (xlo, xhi, ylo, yhi, zlo, zhi)
+        as HM_BC_* codes or names; ``dirichlet`` [2 dim][num_arrays] values of
+        the DIRICHLET sides (EulerBoundaryConditions::setPhysicalBoundaryConditions)."""
+        d = self.dim
+        codes = [self.BC[c] if isinstance(c, str) else int(c) for c in conditions]
+        if len(codes) != 2 * d:
+            raise ValueError(f"{2 * d} side conditions expected")
+        cc = (ctypes.c_int * (2 * d))(*codes)
+        dv = None
+        if dirichlet is not None:
+            arr = np.ascontiguousarray(dirichlet, dtype=np.float64).reshape(2 * d, self.rec.num_arrays)
+            self._dirichlet = arr
+            dv = arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        capi.check(self.rec._ctx, capi.lib.hm_level_set_boundary_conditions(self._lvl, cc, dv))
+
     def fill_ghosts(self, stream=None):
         capi.check(self.rec._ctx, capi.lib.hm_level_fill_ghosts(self._lvl, self.rec._stream_ptr(stream)))
 

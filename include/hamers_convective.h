@@ -327,8 +327,10 @@ int hm_advance_single_step_on_patches(hm_ctx* ctx, const hm_rk_patch_view* patch
  * convective RHS (hm_compute_convective_flux_and_source_on_patches) and the
  * stage update, with SSP-RK3 weights (RungeKuttaLevelIntegrator.cpp:3907-3924)
  * unless set otherwise.  Ghost cells no same-level patch covers keep what the
- * caller wrote there.  Patch boxes are (lo, n) in level index space; all
- * (x, y, z)-ordered arrays of 3 per patch.
+ * caller wrote there, unless they lie outside the domain on a side with a
+ * physical boundary condition (hm_level_set_boundary_conditions).  Patch boxes
+ * are (lo, n) in level index space; all (x, y, z)-ordered arrays of 3 per
+ * patch.
  */
 typedef struct hm_level hm_level;
 int hm_level_create(hm_ctx* ctx, int num_patches, const int* patch_lo, const int* patch_n,
@@ -349,7 +351,38 @@ int hm_level_read_patch(hm_level* level, int patch, double* Q);
  * component stride = ghost-box volume) and the ghost width. */
 int hm_level_patch_array(hm_level* level, int patch, double** Q, int* ghost);
 
-/* Same-level ghost fill of the scratch arrays. */
+/*
+ * Physical boundary conditions of the level's non-periodic sides, applied to
+ * every conservative variable after each same-level ghost fill (every Runge-
+ * Kutta stage), as EulerBoundaryConditions::setPhysicalBoundaryConditions
+ * (EulerBoundaryConditions.cpp:749-1100) applies the basic conditions of
+ * BasicCartesianBoundaryUtilities1/2/3 (BasicCartesianBoundaryUtilities2.cpp:215-440
+ * edge fills, 590-900 node fills; BasicBoundaryConditions.hpp:14-34):
+ *   HM_BC_FLOW      copy of the boundary cell (zero gradient);
+ *   HM_BC_REFLECT   mirror image about the boundary face, the momentum
+ *                   component normal to it negated (scalar variables mirror);
+ *   HM_BC_SYMMETRY  mirror image;
+ *   HM_BC_DIRICHLET fixed conservative values;
+ *   HM_BC_NONE      ghost cells outside the domain keep the caller's values
+ *                   (the default).
+ * conditions: 2 dim entries (xlo, xhi, ylo, yhi, zlo, zhi); periodic axes take
+ * HM_BC_NONE.  dirichlet: [2 dim][num_conservative_arrays] values (rows of
+ * non-DIRICHLET sides ignored), may be NULL when no side is DIRICHLET.  Cells
+ * outside the domain along several axes (edges, corners) compose the rules of
+ * those axes, as the reference's x-named node and edge conditions (XFLOW,
+ * XREFLECT, ...) do; with DIRICHLET among them the first such axis in x, y, z
+ * order gives the values.  A level with boundary conditions has no adjoint
+ * (hm_level_advance_recorded returns HM_ERR_UNSUPPORTED).
+ */
+#define HM_BC_NONE 0
+#define HM_BC_FLOW 1
+#define HM_BC_REFLECT 2
+#define HM_BC_SYMMETRY 3
+#define HM_BC_DIRICHLET 4
+int hm_level_set_boundary_conditions(hm_level* level, const int* conditions, const double* dirichlet);
+
+/* Ghost fill of the scratch arrays: same level (periodic images included),
+ * then the physical boundary conditions. */
 int hm_level_fill_ghosts(hm_level* level, void* stream);
 
 /* num_steps steps of size dt.  use_graph != 0 replays one step captured in a

@@ -434,3 +434,83 @@ def diffusive_patch(prm: Params, tr: Transport, Q: np.ndarray, n: tuple, g: int,
     if st != 0:
         raise RuntimeError(f"oracle hmo_diffusive_patch failed with status {st}")
     return F
+
+
+# Physical boundary conditions (codes of include/hamers_convective.h).
+BC_NONE, BC_FLOW, BC_REFLECT, BC_SYMMETRY, BC_DIRICHLET = range(5)
+
+
+def fill_boundary(Qi: np.ndarray, g: int, periodic, conds, dirichlet=None, mom0: int = 1) -> np.ndarray:
+    """Ghosted array of a one-patch domain (ncomp, *rev(n)) -> (ncomp, *rev(n + 2g))
+    with the basic physical boundary conditions of
+    BasicCartesianBoundaryUtilities2.cpp:215-440 (edges) / 590-900 (nodes) and
+    BasicCartesianBoundaryUtilities3.cpp, applied to every conservative
+    variable as EulerBoundaryConditions::setPhysicalBoundaryConditions does
+    (EulerBoundaryConditions.cpp:749-1100):
+      FLOW      the boundary cell (pivot interior_box_lo / hi, :285-310);
+      REFLECT   the mirror cell lo + (fill_hi - i) / hi - (i - fill_lo), the
+                vector variable's component normal to the side negated (:335-375);
+                scalar variables take SYMMETRY (EulerBoundaryConditions.cpp:196-345);
+      SYMMETRY  the mirror cell (:383-420);
+      DIRICHLET the side's values (:262-276).
+    conds[2a + hi] per side; periodic axes wrap.  Momentum components are
+    mom0 .. mom0 + d - 1.  A cell outside the domain along several axes
+    composes the axes' rules (the x-named node and edge conditions of the
+    reference, whose pivot lies in the next axis's ghost box); with DIRICHLET
+    among them the first such axis (x, y, z order) gives the values."""
+    d = Qi.ndim - 1
+    n = [Qi.shape[d - a] for a in range(d)]  # x first
+    src, flip, side = [], [], []
+    for a in range(d):
+        x = np.arange(-g, n[a] + g)
+        s = x.copy()
+        f = np.zeros(x.size, dtype=bool)
+        sd = np.full(x.size, -1)
+        if periodic[a]:
+            s = np.mod(x, n[a])
+        else:
+            for hi in (0, 1):
+                c = conds[2 * a + hi]
+                out = (x < 0) if hi == 0 else (x >= n[a])
+                if c == BC_NONE or not np.any(out):
+                    continue
+                if c == BC_FLOW:
+                    s[out] = 0 if hi == 0 else n[a] - 1
+                elif c in (BC_REFLECT, BC_SYMMETRY):
+                    s[out] = (-1 - x[out]) if hi == 0 else (2 * n[a] - 1 - x[out])
+                    f[out] = c == BC_REFLECT
+                elif c == BC_DIRICHLET:
+                    s[out] = 0 if hi == 0 else n[a] - 1  # any interior cell: overwritten below
+                    sd[out] = 2 * a + hi
+                else:
+                    raise ValueError(f"unknown boundary condition {c}")
+            if np.any((s < 0) | (s >= n[a])):
+                raise ValueError("cells outside the domain on a side without a condition")
+        src.append(s)
+        flip.append(f)
+        side.append(sd)
+    # gather: axis a of the array is numpy axis d - a
+    idx = np.ix_(*[src[a] for a in reversed(range(d))])
+    Q = np.stack([Qi[c][idx] for c in range(Qi.shape[0])])
+    for a in range(d):
+        if not np.any(flip[a]):
+            continue
+        shp = [1] * d
+        shp[d - 1 - a] = flip[a].size
+        sign = np.where(flip[a], -1.0, 1.0).reshape(shp)
+        for b in range(d):
+            if a == b:
+                Q[mom0 + b] = Q[mom0 + b] * sign
+    if any(np.any(sd >= 0) for sd in side):
+        dirichlet = np.asarray(dirichlet, dtype=np.float64)
+        first = np.full(Q.shape[1:], -1)
+        for a in range(d):  # the first DIRICHLET axis in x, y, z order wins
+            shp = [1] * d
+            shp[d - 1 - a] = side[a].size
+            sd = np.broadcast_to(side[a].reshape(shp), first.shape)
+            first = np.where((first < 0) & (sd >= 0), sd, first)
+        for sdv in np.unique(first[first >= 0]):
+            m = first == sdv
+            for c in range(Q.shape[0]):
+                Q[c][m] = dirichlet[sdv][c]
+    return Q

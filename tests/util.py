@@ -33,14 +33,25 @@ def flux_divergence(F, S, dx, Fd=None):
     return acc + S
 
 
-def oracle_rhs_periodic(prm, Qi, n, dx, dt, g=4, transport=None):
+def ghost_fill(Qi, g, prm, bc=None):
+    """Ghosted one-patch domain: periodic, or with the physical boundary
+    conditions bc = (periodic per axis, side conditions, dirichlet values)."""
+    if bc is None:
+        return W.fill_periodic(Qi, g)
+    periodic, conds, dirichlet = bc
+    mom0 = 1 if prm.model == O.SINGLE_SPECIES else prm.num_species
+    return O.fill_boundary(Qi, g, periodic, conds, dirichlet, mom0)
+
+
+def oracle_rhs_periodic(prm, Qi, n, dx, dt, g=4, transport=None, bc=None):
     """Oracle RHS (flux divergence, dt-scaled) on a periodic domain treated as
-    one patch; for the five-equation model the source is part of the RHS and
-    the last volume fraction's row is zero.  With ``transport`` (four-equation
-    model) the diffusive flux is added in the Navier-Stokes form."""
+    one patch (or with physical boundary conditions ``bc``, see ghost_fill);
+    for the five-equation model the source is part of the RHS and the last
+    volume fraction's row is zero.  With ``transport`` (four-equation model)
+    the diffusive flux is added in the Navier-Stokes form."""
     if transport is not None:
         g = max(g, 6)
-    Qg = W.fill_periodic(Qi, g)
+    Qg = ghost_fill(Qi, g, prm, bc)
     F, S = O.rhs_patch(prm, Qg, n, g, dx, dt)
     Fd = O.diffusive_patch(prm, transport, Qg, n, g, dx, dt) if transport is not None else None
     R = flux_divergence(F, S, dx, Fd)
@@ -49,7 +60,7 @@ def oracle_rhs_periodic(prm, Qi, n, dx, dt, g=4, transport=None):
     return R, F, S
 
 
-def oracle_rk3(prm, Qi, n, dx, dt, nsteps, alpha=RK_ALPHA, beta=RK_BETA, transport=None):
+def oracle_rk3(prm, Qi, n, dx, dt, nsteps, alpha=RK_ALPHA, beta=RK_BETA, transport=None, bc=None):
     """SSP-RK3 steps on a periodic domain with the oracle RHS; the stage
     update follows Euler::advanceSingleStepOnPatch (Euler.cpp:1003-1680), or
     with ``transport`` NavierStokes::advanceSingleStepOnPatch."""
@@ -59,7 +70,7 @@ def oracle_rk3(prm, Qi, n, dx, dt, nsteps, alpha=RK_ALPHA, beta=RK_BETA, transpo
         # (the zero-fill of Euler::advanceSingleStepOnPatch covers all stored arrays)
         for sn in range(len(alpha)):
             Qint.append(Q.copy())
-            R.append(oracle_rhs_periodic(prm, Q, n, dx, dt, transport=transport)[0])
+            R.append(oracle_rhs_periodic(prm, Q, n, dx, dt, transport=transport, bc=bc)[0])
             Qn = np.zeros_like(Q)
             for m in range(sn + 1):
                 if alpha[sn][m] != 0.0:
