@@ -66,6 +66,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work (multi-thread run)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-patch", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch schedule")
     ap.add_argument("--headline-only", action="store_true", help="skip the secondary legs")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only copy this many MiB device to device 5 times (calibrates HBM byte counters) and exit")
@@ -204,6 +205,7 @@ def bench_headline(args, torch, np, L):
     """Batched RHS of the uneven level (headline), stage times with HIP events
     on the launch stream, the per-patch path beside it."""
     from hamers_amd import capi
+    from hamers_amd.reconstructor import autotune
 
     rec = L.rec
     stream = torch.cuda.current_stream()
@@ -220,6 +222,8 @@ def bench_headline(args, torch, np, L):
             if st:
                 capi.check(rec._ctx, st)
 
+    # the autotuner picks the level's launch schedule (timed trials on this level)
+    sched = autotune(rec, L.plist, L.dt, repetitions=3, stream=stream) if not args.no_autotune else None
     ms, wall = timed_events(torch, stream, batched, args.steps, args.warmup)
     rec.enable_profiling(True)
     for _ in range(args.steps):
@@ -230,7 +234,7 @@ def bench_headline(args, torch, np, L):
     ms_p = None
     if not args.no_per_patch:
         ms_p, _ = timed_events(torch, stream, per_patch, max(2, args.steps // 4), 1)
-    return ms, wall, stage, ms_p
+    return ms, wall, stage, ms_p, sched
 
 
 def headline_roofline(L, stage, ms):
@@ -278,6 +282,7 @@ def bench_level_rk3(args, torch, np, L):
         lvl.write_patch(i, Qi)
     del interiors
     dt = 2.0e-5
+    sched = lvl.autotune(dt, repetitions=2) if not args.no_autotune else None
     lvl.advance(dt, 1, use_graph=True)  # capture
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -286,7 +291,7 @@ def bench_level_rk3(args, torch, np, L):
     t = (time.perf_counter() - t0) / args.rk_steps
     ok = bool(np.all(np.isfinite(lvl.read_patch(0))))
     del lvl
-    return {"ms_per_step": t * 1e3, "steps": args.rk_steps, "dt": dt, "graph": True,
+    return {"ms_per_step": t * 1e3, "steps": args.rk_steps, "dt": dt, "graph": True, "schedule": sched,
             "cell_rhs_per_s": 3 * L.cells / t, "finite": ok,
             "note": "one SSP-RK3 step = ghost fill + 3 batched RHS + 3 stage updates, replayed from a hipGraph"}
 
@@ -371,6 +376,9 @@ def bench_2d(args, torch, np):
             if st:
                 capi.check(rec._ctx, st)
 
+    from hamers_amd.reconstructor import autotune
+
+    sched = autotune(rec, plist, dt, repetitions=3, stream=stream) if not args.no_autotune else None
     ms, _ = timed_events(torch, stream, batched, args.steps_2d, 10)
     rec.enable_profiling(True)
     for _ in range(args.steps_2d):
@@ -390,7 +398,7 @@ def bench_2d(args, torch, np):
                         "4-cell halos, WCNS5_JS_HLLC_HLL, batched RHS" if (N, P) == (2048, 128) else
                         f"2D single-species four-quadrant Riemann, {N}^2 in patches of {P}^2, {args.scheme}",
             "cells": cells, "patches": len(pds), "ms_per_rhs_batched": ms, "value_batched": cells / (ms * 1e-3),
-            "stage_ms": stage, "ms_per_rhs_per_patch": ms_p,
+            "stage_ms": stage, "schedule": sched, "ms_per_rhs_per_patch": ms_p,
             "batched_speedup": (ms_p / ms) if ms_p else None,
             "flux_kernel": {"kernel": STAGE_KERNEL_SINGLE.get(dominant, dominant), "kernel_ms": stage[dominant],
                             "bytes_per_cell": 8 * (neq + neq), "achieved_gbs": kgbs, "hbm_frac": kgbs / HBM_PEAK_GBS,
@@ -600,7 +608,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     L = UnevenLevel(args, torch, np)
     progress("uneven 3D level built")
-    ms, wall, stage, ms_p = bench_headline(args, torch, np, L)
+    ms, wall, stage, ms_p, sched = bench_headline(args, torch, np, L)
     progress(f"headline RHS {ms:.3f} ms (per-patch {ms_p} ms)")
     roof = headline_roofline(L, stage, ms)
     cpu = None if args.no_cpu else cpu_baseline_uneven(args, np, L)
@@ -642,6 +650,7 @@ def main():
         "config": config,
         "roofline": roof,
         "stage_ms": stage,
+        "schedule": sched,
         "per_patch": {"ms_per_step": ms_p, "value": (config["cells"] / (ms_p * 1e-3)) if ms_p else None,
                       "batched_speedup": (ms_p / ms) if ms_p else None},
         "host_wall_ms_per_step": wall,

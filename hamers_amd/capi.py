@@ -84,6 +84,13 @@ class hm_rk_patch_view(ctypes.Structure):
     ]
 
 
+class hm_schedule(ctypes.Structure):
+    _fields_ = [("tile_rows", ctypes.c_int), ("xcd_remap", ctypes.c_int), ("order", ctypes.c_int)]
+
+    def as_dict(self):
+        return {"tile_rows": self.tile_rows, "xcd_remap": self.xcd_remap, "order": self.order}
+
+
 class hm_transport(ctypes.Structure):
     _fields_ = [
         ("species_mu", ctypes.c_double * HM_MAX_SPECIES),
@@ -156,6 +163,17 @@ def _load():
     L.hm_level_fill_ghosts.argtypes = [P, P]
     L.hm_level_set_boundary_conditions.argtypes = [P, I, D]
     L.hm_level_set_boundary_conditions.restype = ctypes.c_int
+    S = ctypes.POINTER(hm_schedule)
+    L.hm_autotune_on_patches.argtypes = [P, ctypes.POINTER(hm_patch_view), ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_int, P, S, D]
+    L.hm_set_schedule_on_patches.argtypes = [P, ctypes.POINTER(hm_patch_view), ctypes.c_int, S]
+    L.hm_get_schedule_on_patches.argtypes = [P, ctypes.POINTER(hm_patch_view), ctypes.c_int, S]
+    L.hm_level_autotune.argtypes = [P, ctypes.c_double, ctypes.c_int, S, D]
+    L.hm_level_set_schedule.argtypes = [P, S]
+    L.hm_level_get_schedule.argtypes = [P, S]
+    for name in ("hm_autotune_on_patches", "hm_set_schedule_on_patches", "hm_get_schedule_on_patches",
+                 "hm_level_autotune", "hm_level_set_schedule", "hm_level_get_schedule"):
+        getattr(L, name).restype = ctypes.c_int
     L.hm_level_advance.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.c_int, P]
     L.hm_convective_flux_vjp_on_patch.argtypes = [P, ctypes.POINTER(hm_vjp_patch_view), ctypes.c_double,
                                                   ctypes.c_double, ctypes.c_int, P]

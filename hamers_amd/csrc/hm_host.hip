@@ -65,6 +65,52 @@ int hm_compute_convective_flux_and_source_on_patches(hm_ctx* ctx, const hm_patch
                                                              static_cast<hipStream_t>(stream));
 }
 
+namespace {
+hm::Schedule to_sched(const hm_schedule& c) {
+    hm::Schedule s;
+    s.tile_rows = c.tile_rows;
+    s.xcd = c.xcd_remap;
+    s.order = c.order;
+    return s;
+}
+hm_schedule from_sched(const hm::Schedule& s) {
+    hm_schedule c;
+    c.tile_rows = s.tile_rows;
+    c.xcd_remap = s.xcd;
+    c.order = s.order;
+    return c;
+}
+}  // namespace
+
+int hm_autotune_on_patches(hm_ctx* ctx, const hm_patch_view* patches, int num_patches, double dt, int repetitions,
+                           void* stream, hm_schedule* chosen, double* ms) {
+    if (!ctx || !ctx->rec) return HM_ERR_INVALID_ARGUMENT;
+    hm::Schedule best;
+    double best_ms = 0.0, default_ms = 0.0;
+    const int st = ctx->rec->autotuneOnPatches(patches, num_patches, dt, repetitions, static_cast<hipStream_t>(stream),
+                                               best, best_ms, default_ms);
+    if (st) return st;
+    if (chosen) *chosen = from_sched(best);
+    if (ms) {
+        ms[0] = best_ms;
+        ms[1] = default_ms;
+    }
+    return HM_OK;
+}
+
+int hm_set_schedule_on_patches(hm_ctx* ctx, const hm_patch_view* patches, int num_patches,
+                               const hm_schedule* schedule) {
+    if (!ctx || !ctx->rec || !schedule) return HM_ERR_INVALID_ARGUMENT;
+    if (int st = ctx->rec->selectDevice()) return st;
+    return ctx->rec->setSchedule(patches, num_patches, to_sched(*schedule));
+}
+
+int hm_get_schedule_on_patches(hm_ctx* ctx, const hm_patch_view* patches, int num_patches, hm_schedule* schedule) {
+    if (!ctx || !ctx->rec || !schedule || !patches || num_patches < 1) return HM_ERR_INVALID_ARGUMENT;
+    *schedule = from_sched(ctx->rec->scheduleOf(patches, num_patches));
+    return HM_OK;
+}
+
 int hm_profile_enable(hm_ctx* ctx, int enable) {
     if (!ctx || !ctx->rec) return HM_ERR_INVALID_ARGUMENT;
     return ctx->rec->setProfiling(enable);

@@ -38,6 +38,26 @@ struct KParams {
     // and the adjoint's cell gather (a level adjoint reads the stage inputs
     // from a slot of its trajectory store); 0 otherwise.
     int64_t qoff;
+    // Block numbering of the RHS kernels (Schedule::xcd): 1 hands each XCD a
+    // contiguous range of block numbers (xcd_block), 0 keeps the hardware's
+    // round-robin order.
+    int xcd;
+};
+
+// Block number a RHS kernel works on.
+__device__ __forceinline__ uint32_t rhs_block(const KParams& k) {
+    return k.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+}
+
+// Launch schedule of a batched RHS (chosen by the autotuner, or the built-in
+// rule): tile rows of the single-species tiled flux stage (4, 8 or 24;
+// ignored by the multi-species models), the block numbering (KParams::xcd)
+// and the order of the work segments within each flux stage (0 patch order,
+// 1 largest segments first, so the long ones do not trail the launch).
+struct Schedule {
+    int tile_rows = 24;
+    int xcd = 0;
+    int order = 0;
 };
 
 // The time step a kernel scales its fluxes with.
@@ -99,6 +119,7 @@ struct Plan {
     W work[kStages];
     int64_t items[kStages];
     int tile_rows;  // rows of the tiled flux stage (single-species model): 4, 8 or 24
+    int xcd;        // KParams::xcd of the plan's launches
 };
 
 template <class W>

@@ -31,7 +31,7 @@ namespace hm {
 // ----------------------------------------------------------------------------
 template <int D, class W>
 __global__ __launch_bounds__(kBlock) void k_derived_single(W w, KParams k) {
-    const int64_t gid = int64_t(xcd_block(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
+    const int64_t gid = int64_t(rhs_block(k)) * kBlock + threadIdx.x;
     const DevPatch* P;
     int ax;
     int64_t loc;
@@ -208,7 +208,7 @@ __device__ __forceinline__ void fused_chunk(const DevPatch& P, int64_t loc, cons
 template <int D, int SCHEME, int PP, int QQ, class W, bool REC = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SCHEME == kWCNS5_JS && D == 2 ? 4 : 1))) void k_flux_single(
     W w, KParams k) {
-    const int64_t gid = int64_t(xcd_block(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
+    const int64_t gid = int64_t(rhs_block(k)) * kBlock + threadIdx.x;
     const DevPatch* P;
     int ax;
     int64_t loc;
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SCHEME =
     static_assert(R % 4 == 0 && R >= 4, "tile rows: a multiple of the block's four wavefronts");
     if constexpr (D > 1) {
         __shared__ double lds[R * kTileX * (D + 2)];
-        const int64_t gid = int64_t(xcd_block(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
+        const int64_t gid = int64_t(rhs_block(k)) * kBlock + threadIdx.x;
         const DevPatch* P;
         int ax;
         int64_t loc;
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SCHEME =
 template <int D, int R, int SCHEME, int PP, int QQ, class W>
 __global__ __launch_bounds__(kBlock) void k_flux_patch_single(W w, KParams k) {
     __shared__ double lds[(D > 1) ? R * kTileX * (D + 2) : 1];
-    const int64_t gid = int64_t(xcd_block(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
+    const int64_t gid = int64_t(rhs_block(k)) * kBlock + threadIdx.x;
     const DevPatch* P;
     int ax;
     int64_t loc;

@@ -492,6 +492,53 @@ class Level {
     }
     bool hasBoundaryConditions() const { return has_bc_; }
 
+    // Autotuner of the level's batched RHS: every candidate schedule is timed
+    // on the first stage's RHS of the current state (ghosts filled), the
+    // fastest becomes the schedule of every stage's RHS.  Captured steps are
+    // dropped (their launches used the old tables).
+    int autotune(double dt, int reps, Schedule& best, double& best_ms, double& default_ms) {
+        if (!(dt > 0.0)) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "bad dt");
+        if (int st = rec_->selectDevice()) return st;
+        hipStream_t s = cap_stream_;
+        if (hipDeviceSynchronize() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "device synchronize failed");
+        drop_graphs();
+        hipLaunchKernelGGL(k_copy_state, dim3(blocks_for(copy_items_)), dim3(kBlock), 0, s, copy_, 0, 1, nq_);
+        if (int st = launch_halo(1, s)) return st;
+        hipLaunchKernelGGL(k_zero_source, dim3(blocks_for(zero_items_)), dim3(kBlock), 0, s, zero_, 0);
+        const bool single = rec_->config().model == HM_MODEL_SINGLE_SPECIES;
+        ConvectiveFluxReconstructorGPU::OwnedPlan trial;
+        const Schedule dflt = rec_->defaultSchedule(views_[0].data(), np_);
+        int st = rec_->tuneSchedules(
+            s, reps,
+            [&](const Schedule& sc) {
+                if (hipStreamSynchronize(s) != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "stream synchronize failed");
+                return rec_->buildOwnedPlan(views_[0].data(), np_, static_cast<double*>(rhs_scratch_), trial,
+                                            single ? wst_off_.data() : nullptr, &sc);
+            },
+            [&]() {
+                if (int e = rec_->runOwnedPlan(trial, dt, s)) return e;
+                if (diff_) return diff_->runOwnedPlan(diff_plan_[0], dt, s);
+                return int(HM_OK);
+            },
+            best, best_ms, default_ms, dflt);
+        if (hipStreamSynchronize(s) != hipSuccess && st == HM_OK) st = rec_->fail(HM_ERR_DEVICE, "stream synchronize failed");
+        rec_->releaseOwnedPlan(trial);
+        if (st) return st;
+        return setSchedule(best);
+    }
+
+    int setSchedule(const Schedule& sc) {
+        if (sc.tile_rows != 4 && sc.tile_rows != 8 && sc.tile_rows != 24)
+            return rec_->fail(HM_ERR_INVALID_ARGUMENT, "tile rows must be 4, 8 or 24");
+        if ((sc.xcd != 0 && sc.xcd != 1) || (sc.order != 0 && sc.order != 1))
+            return rec_->fail(HM_ERR_INVALID_ARGUMENT, "xcd and order are 0 or 1");
+        if (hipDeviceSynchronize() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "device synchronize failed");
+        drop_graphs();
+        sched_ = sc;
+        return build_rhs_plans();
+    }
+    const Schedule& schedule() const { return sched_; }
+
     // num_steps steps of the configured Runge-Kutta scheme with fixed dt.
     int advance(double dt, int nsteps, bool use_graph, hipStream_t s) {
         if (nsteps < 0 || !(dt > 0.0)) return rec_->fail(HM_ERR_INVALID_ARGUMENT, "bad dt or step count");
@@ -743,10 +790,8 @@ class Level {
                     wst_off_[size_t(p) * 3 + a] = int64_t(wst_slot_doubles_);
                     wst_slot_doubles_ += up256(sizeof(double) * size_t(2 * neq_) * size_t(mv)) / sizeof(double);
                 }
-        for (int s = 0; s < kMaxRK; s++)
-            if (int st = rec_->buildOwnedPlan(views_[s].data(), np_, static_cast<double*>(rhs_scratch_), rhs_plan_[s],
-                                              single ? wst_off_.data() : nullptr))
-                return st;
+        sched_ = rec_->defaultSchedule(views_[0].data(), np_);
+        if (int st = build_rhs_plans()) return st;
         if (diff_) {
             // Diffusive RHS of every intermediate buffer into the stage's
             // diffusive fluxes; the stages share one scratch region.
@@ -773,6 +818,16 @@ class Level {
             hipEventCreateWithFlags(&ev_b_, hipEventDisableTiming) != hipSuccess)
             return rec_->fail(HM_ERR_DEVICE, "stream or event creation failed");
         return build_tables();
+    }
+
+    // Work tables of every stage's batched RHS with the level's schedule.
+    int build_rhs_plans() {
+        const bool single = rec_->config().model == HM_MODEL_SINGLE_SPECIES;
+        for (int s = 0; s < kMaxRK; s++)
+            if (int st = rec_->buildOwnedPlan(views_[s].data(), np_, static_cast<double*>(rhs_scratch_), rhs_plan_[s],
+                                              single ? wst_off_.data() : nullptr, &sched_))
+                return st;
+        return HM_OK;
     }
 
     // Same-level ghost copies, periodic images included.
@@ -1482,6 +1537,7 @@ class Level {
     double bc_values_[6 * (HM_MAX_EQN + 1)] = {};
     bool has_bc_ = false;
     bool bc_bad_ = false;
+    Schedule sched_{};  // schedule of the batched RHS plans
     SegTableT<LevelPatch> copy_{}, zero_{};
     SegTableT<RKPatch> rk_[kMaxRK] = {};
     RKParams rkp_[kMaxRK] = {};
@@ -1732,6 +1788,42 @@ int hm_level_set_boundary_conditions(hm_level* level, const int* conditions, con
     if (!level || !conditions) return HM_ERR_INVALID_ARGUMENT;
     if (int st = level->ctx->rec->selectDevice()) return st;
     return level->lvl->setBoundaryConditions(conditions, dirichlet);
+}
+
+int hm_level_autotune(hm_level* level, double dt, int repetitions, hm_schedule* chosen, double* ms) {
+    if (!level) return HM_ERR_INVALID_ARGUMENT;
+    hm::Schedule best;
+    double best_ms = 0.0, default_ms = 0.0;
+    if (int st = level->lvl->autotune(dt, repetitions, best, best_ms, default_ms)) return st;
+    if (chosen) {
+        chosen->tile_rows = best.tile_rows;
+        chosen->xcd_remap = best.xcd;
+        chosen->order = best.order;
+    }
+    if (ms) {
+        ms[0] = best_ms;
+        ms[1] = default_ms;
+    }
+    return HM_OK;
+}
+
+int hm_level_set_schedule(hm_level* level, const hm_schedule* schedule) {
+    if (!level || !schedule) return HM_ERR_INVALID_ARGUMENT;
+    if (int st = level->ctx->rec->selectDevice()) return st;
+    hm::Schedule sc;
+    sc.tile_rows = schedule->tile_rows;
+    sc.xcd = schedule->xcd_remap;
+    sc.order = schedule->order;
+    return level->lvl->setSchedule(sc);
+}
+
+int hm_level_get_schedule(hm_level* level, hm_schedule* schedule) {
+    if (!level || !schedule) return HM_ERR_INVALID_ARGUMENT;
+    const hm::Schedule& sc = level->lvl->schedule();
+    schedule->tile_rows = sc.tile_rows;
+    schedule->xcd_remap = sc.xcd;
+    schedule->order = sc.order;
+    return HM_OK;
 }
 
 int hm_level_fill_ghosts(hm_level* level, void* stream) {

@@ -168,7 +168,7 @@ __device__ __forceinline__ void cell_state_multi(const KParams& k, QF&& q, uint3
 // ----------------------------------------------------------------------------
 template <int D, int NS, bool FOUR, class W>
 __global__ __launch_bounds__(kBlock) void k_derived_multi(W w, KParams k) {
-    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t gid = int64_t(rhs_block(k)) * kBlock + threadIdx.x;
     const DevPatch* P;
     int ax;
     int64_t loc;
@@ -571,7 +571,7 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
 template <int D, int NS, bool FOUR, int SCHEME, int PP, int QQ, class W>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HM_MIDFLUX_MULTI_WAVES))) void k_midflux_multi(
     W w, KParams k) {
-    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t gid = int64_t(rhs_block(k)) * kBlock + threadIdx.x;
     const DevPatch* P;
     int ax;
     int64_t loc;
@@ -656,7 +656,7 @@ __device__ __forceinline__ void faceflux_multi_face(const DevPatch& P, int64_t l
 
 template <int D, int NS, bool FOUR, class W>
 __global__ __launch_bounds__(kBlock) void k_faceflux_multi(W w, KParams k) {
-    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t gid = int64_t(rhs_block(k)) * kBlock + threadIdx.x;
     const DevPatch* P;
     int ax;
     int64_t loc;
@@ -678,7 +678,7 @@ __global__ __launch_bounds__(kBlock) void k_faceflux_multi(W w, KParams k) {
 // ----------------------------------------------------------------------------
 template <int D, int NS, class W>
 __global__ __launch_bounds__(kBlock) void k_source_five(W w, KParams k) {
-    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t gid = int64_t(rhs_block(k)) * kBlock + threadIdx.x;
     const DevPatch* P;
     int ax;
     int64_t loc;

@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -118,6 +119,7 @@ class ConvectiveFluxReconstructorGPU {
         k_.lam_src = -1;
         k_.lam_scale = 0.0;
         k_.qoff = 0;
+        k_.xcd = 0;
         has_advective_ = (cfg.model == HM_MODEL_FIVE_EQN_ALLAIRE && cfg.num_species > 1);
         fused_ = (cfg.model == HM_MODEL_SINGLE_SPECIES);
         plans_.reserve(kMaxPlans);  // plan addresses stay valid while cached
@@ -226,12 +228,152 @@ class ConvectiveFluxReconstructorGPU {
     // wrec_off (optional, 3 per patch): the patches' chunks in a face-state
     // slot, for runs that record the face states (runOwnedPlan's wrec).
     int buildOwnedPlan(const hm_patch_view* views, int np, double* scratch, OwnedPlan& out,
-                       const int64_t* wrec_off = nullptr) {
+                       const int64_t* wrec_off = nullptr, const Schedule* sched = nullptr) {
         for (int i = 0; i < np; i++)
             if (int st = check_view(views[i], i)) return st;
         if (int st = select_device()) return st;
         releaseOwnedPlan(out);
-        return upload_tables(views, np, scratch, out.dev, out.plan, wrec_off);
+        return upload_tables(views, np, scratch, out.dev, out.plan, wrec_off, sched);
+    }
+
+    // The built-in schedule of a patch list: tile rows by the work the call
+    // holds (tile_rows_for), XCD block numbering for the single-species
+    // kernels (measured faster there) and hardware order for the multi-species
+    // ones, segments in patch order.
+    Schedule defaultSchedule(const hm_patch_view* views, int np) const {
+        Schedule sc;
+        sc.tile_rows = tile_rows_of(views, np);
+        sc.xcd = fused_ ? 1 : 0;
+        sc.order = 0;
+        return sc;
+    }
+
+    // Candidate schedules of the autotuner for this model.
+    std::vector<Schedule> scheduleCandidates() const {
+        std::vector<Schedule> c;
+        const int rows[3] = {4, 8, 24};
+        for (int r = 0; r < (fused_ && cfg_.dim > 1 ? 3 : 1); r++)
+            for (int x = 0; x < 2; x++)
+                for (int o = 0; o < 2; o++) {
+                    Schedule sc;
+                    sc.tile_rows = (fused_ && cfg_.dim > 1) ? rows[r] : 4;
+                    sc.xcd = x;
+                    sc.order = o;
+                    c.push_back(sc);
+                }
+        return c;
+    }
+
+    // Times `run` (one batched RHS of a plan) for each candidate schedule,
+    // built by `build` (plan for a schedule), repetitions times after one
+    // warm-up, and returns the fastest (median of the repetitions).
+    template <class Build, class Run>
+    int tuneSchedules(hipStream_t s, int reps, Build&& build, Run&& run, Schedule& best, double& best_ms,
+                      double& default_ms, const Schedule& dflt) {
+        if (reps < 1) return fail(HM_ERR_INVALID_ARGUMENT, "autotune needs at least one repetition");
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+            return fail(HM_ERR_DEVICE, "hipEventCreate failed");
+        best_ms = -1.0;
+        default_ms = -1.0;
+        int st = HM_OK;
+        for (const Schedule& sc : scheduleCandidates()) {
+            if ((st = build(sc))) break;
+            if ((st = run())) break;  // warm-up
+            std::vector<float> t(size_t(reps), 0.f);
+            for (int r = 0; r < reps && st == HM_OK; r++) {
+                if (hipEventRecord(e0, s) != hipSuccess) st = fail(HM_ERR_DEVICE, "hipEventRecord failed");
+                if (st == HM_OK) st = run();
+                if (st == HM_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                    hipEventElapsedTime(&t[size_t(r)], e0, e1) != hipSuccess))
+                    st = fail(HM_ERR_DEVICE, "event timing failed");
+            }
+            if (st) break;
+            std::sort(t.begin(), t.end());
+            const double ms = t[t.size() / 2];
+            if (sc.tile_rows == dflt.tile_rows && sc.xcd == dflt.xcd && sc.order == dflt.order) default_ms = ms;
+            if (best_ms < 0.0 || ms < best_ms) {
+                best_ms = ms;
+                best = sc;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return st;
+    }
+
+    // Autotuner of the batched RHS of a patch list: times every candidate
+    // schedule on these views (their F is overwritten and their S accumulated
+    // by every trial: call it on scratch outputs or before the real call) and
+    // keeps the fastest for the list; later batched calls on the same list use
+    // it.
+    int autotuneOnPatches(const hm_patch_view* views, int np, double dt, int reps, hipStream_t s, Schedule& best,
+                          double& best_ms, double& default_ms) {
+        if (np < 1 || !views) return fail(HM_ERR_INVALID_ARGUMENT, "invalid patch list");
+        for (int i = 0; i < np; i++)
+            if (int st = check_view(views[i], i)) return st;
+        if (int st = select_device()) return st;
+        size_t total = 0;
+        for (int i = 0; i < np; i++) total += layout(views[i]).total;
+        if (arena_order_.enter(s) != hipSuccess) return fail(HM_ERR_DEVICE, "hipStreamWaitEvent failed");
+        if (int st = ensure_arena(total, s)) return st;
+        KParams k = k_;
+        k.dt = dt;
+        OwnedPlan trial;
+        const Schedule dflt = defaultSchedule(views, np);
+        int st = tuneSchedules(
+            s, reps,
+            [&](const Schedule& sc) {
+                if (hipStreamSynchronize(s) != hipSuccess) return fail(HM_ERR_DEVICE, "stream synchronize failed");
+                releaseOwnedPlan(trial);
+                return upload_tables(views, np, arena_, trial.dev, trial.plan, nullptr, &sc);
+            },
+            [&]() { return run_stages(trial.plan, k, s, 0, kStages); }, best, best_ms, default_ms, dflt);
+        if (hipStreamSynchronize(s) != hipSuccess && st == HM_OK) st = fail(HM_ERR_DEVICE, "stream synchronize failed");
+        releaseOwnedPlan(trial);
+        if (st) return st;
+        (void)arena_order_.leave(s);
+        return setSchedule(views, np, best);
+    }
+
+    // Pins the schedule of a patch list's batched calls (the autotuner's
+    // choice, or a caller's).
+    int setSchedule(const hm_patch_view* views, int np, const Schedule& sc) {
+        if (np < 1 || !views) return fail(HM_ERR_INVALID_ARGUMENT, "invalid patch list");
+        if (sc.tile_rows != 4 && sc.tile_rows != 8 && sc.tile_rows != 24)
+            return fail(HM_ERR_INVALID_ARGUMENT, "tile rows must be 4, 8 or 24");
+        if ((sc.xcd != 0 && sc.xcd != 1) || (sc.order != 0 && sc.order != 1))
+            return fail(HM_ERR_INVALID_ARGUMENT, "xcd and order are 0 or 1");
+        const uint64_t key = fnv1a(views, sizeof(hm_patch_view) * size_t(np), fnv1a(&np, sizeof np));
+        bool found = false;
+        for (auto& t : tuned_)
+            if (t.key == key && t.views.size() == size_t(np) &&
+                std::memcmp(t.views.data(), views, sizeof(hm_patch_view) * size_t(np)) == 0) {
+                t.sched = sc;
+                found = true;
+            }
+        if (!found) {
+            if (tuned_.size() >= kMaxPlans) tuned_.erase(tuned_.begin());
+            tuned_.push_back(TunedSchedule{key, std::vector<hm_patch_view>(views, views + np), sc});
+        }
+        // cached tables of this list were built with the previous schedule
+        if (hipDeviceSynchronize() != hipSuccess) return fail(HM_ERR_DEVICE, "device synchronize failed");
+        for (auto& pl : plans_)
+            if (pl.key == key) {
+                free_plan(pl);
+                pl.key = 0;
+            }
+        return HM_OK;
+    }
+
+    // The schedule a batched call on this list uses.
+    Schedule scheduleOf(const hm_patch_view* views, int np) const {
+        const uint64_t key = fnv1a(views, sizeof(hm_patch_view) * size_t(np), fnv1a(&np, sizeof np));
+        for (const auto& t : tuned_)
+            if (t.key == key && t.views.size() == size_t(np) &&
+                std::memcmp(t.views.data(), views, sizeof(hm_patch_view) * size_t(np)) == 0)
+                return t.sched;
+        return defaultSchedule(views, np);
     }
 
     void releaseOwnedPlan(OwnedPlan& p) {
@@ -340,6 +482,7 @@ class ConvectiveFluxReconstructorGPU {
         Plan<SegSingle> plan{};
         const int d = cfg_.dim;
         plan.tile_rows = tile_rows_of(&v, 1);
+        plan.xcd = fused_ ? 1 : 0;
         const Extents e = extents(v, plan.tile_rows);
         // derived cells
         plan.work[0].p = dp;
@@ -610,7 +753,12 @@ class ConvectiveFluxReconstructorGPU {
             free_plan(*pl);
         }
         pl->key = 0;
-        if (int st = upload_tables(views, np, arena_, pl->dev, pl->plan)) return st;
+        const Schedule* sc = nullptr;
+        for (const auto& t : tuned_)
+            if (t.key == key && t.views.size() == size_t(np) &&
+                std::memcmp(t.views.data(), views, sizeof(hm_patch_view) * size_t(np)) == 0)
+                sc = &t.sched;
+        if (int st = upload_tables(views, np, arena_, pl->dev, pl->plan, nullptr, sc)) return st;
         pl->key = key;
         pl->views.assign(views, views + np);
         pl->arena_at_build = arena_;
@@ -622,7 +770,8 @@ class ConvectiveFluxReconstructorGPU {
     // Device work tables of a patch list whose scratch starts at `base`:
     // DevPatch[np], then per stage start/cnt/seg, in one allocation.
     int upload_tables(const hm_patch_view* views, int np, double* base, void*& dev, Plan<SegTable>& plan,
-                      const int64_t* wrec_off = nullptr) {
+                      const int64_t* wrec_off = nullptr, const Schedule* sched = nullptr) {
+        const Schedule sc = sched ? *sched : defaultSchedule(views, np);
         const int d = cfg_.dim;
         std::vector<DevPatch> patches(np);
         std::vector<int64_t> start[kStages], cnt[kStages];
@@ -635,15 +784,14 @@ class ConvectiveFluxReconstructorGPU {
             base += L.total;
         }
         int64_t acc[kStages] = {0, 0, 0, 0, 0};
-        plan.tile_rows = tile_rows_of(views, np);
+        plan.tile_rows = fused_ ? sc.tile_rows : 4;
+        plan.xcd = sc.xcd;
+        // (count, patch * 4 + axis) per stage, in patch order
+        std::vector<std::pair<int64_t, int>> segs[kStages];
         for (int i = 0; i < np; i++) {
             const Extents e = extents(views[i], plan.tile_rows);
             auto add = [&](int stage, int axis, int64_t c) {
-                if (c <= 0) return;
-                start[stage].push_back(acc[stage]);
-                cnt[stage].push_back(c);
-                seg[stage].push_back(i * 4 + axis);
-                acc[stage] += align_items(c);
+                if (c > 0) segs[stage].push_back({c, i * 4 + axis});
             };
             add(0, 0, e.vol4);
             // (stage 1 stays empty: the derived-data kernels compute the sensor)
@@ -651,7 +799,21 @@ class ConvectiveFluxReconstructorGPU {
             for (int a = 0; a < d; a++) add(3, a, e.work3[a]);
             if (has_advective_) add(4, 0, e.vol0);
         }
-        for (int st = 0; st < kStages; st++) start[st].push_back(acc[st]);
+        if (sc.order == 1)
+            for (int st = 2; st < 4; st++)
+                std::stable_sort(segs[st].begin(), segs[st].end(),
+                                 [](const std::pair<int64_t, int>& x, const std::pair<int64_t, int>& y) {
+                                     return x.first > y.first;
+                                 });
+        for (int st = 0; st < kStages; st++) {
+            for (const auto& sg : segs[st]) {
+                start[st].push_back(acc[st]);
+                cnt[st].push_back(sg.first);
+                seg[st].push_back(sg.second);
+                acc[st] += align_items(sg.first);
+            }
+            start[st].push_back(acc[st]);
+        }
         size_t bytes = align_up(sizeof(DevPatch) * size_t(np), 256);
         size_t off_start[kStages], off_cnt[kStages], off_seg[kStages];
         for (int st = 0; st < kStages; st++) {
@@ -717,10 +879,12 @@ class ConvectiveFluxReconstructorGPU {
     }
 
     template <class W>
-    int run_stages(const Plan<W>& plan, const KParams& k, hipStream_t s, int first, int last) {
+    int run_stages(const Plan<W>& plan, const KParams& k0, hipStream_t s, int first, int last) {
         Plan<W> sub = plan;
         for (int st = 0; st < kStages; st++)
             if (st < first || st >= last) sub.items[st] = 0;
+        KParams k = k0;
+        k.xcd = plan.xcd;
         hipError_t e = hipSuccess;
         switch (cfg_.model) {
             case HM_MODEL_SINGLE_SPECIES: e = launch_rhs_single<W>(cfg_.dim, cfg_.scheme, sub, k, s); break;
@@ -763,6 +927,12 @@ class ConvectiveFluxReconstructorGPU {
     size_t arena_doubles_ = 0;
     StreamOrder arena_order_;  // calls on different streams take the arena in turn
     std::vector<TablePlan> plans_;
+    struct TunedSchedule {
+        uint64_t key;
+        std::vector<hm_patch_view> views;
+        Schedule sched;
+    };
+    std::vector<TunedSchedule> tuned_;  // schedules pinned per patch list (autotuner or caller)
     uint64_t use_clock_ = 0;
     void* stage_buf_ = nullptr;
     size_t stage_bytes_ = 0;

@@ -227,6 +227,25 @@ class Level:
             dv = arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         capi.check(self.rec._ctx, capi.lib.hm_level_set_boundary_conditions(self._lvl, cc, dv))
 
+    def autotune(self, dt: float, repetitions: int = 3) -> dict:
+        """hm_level_autotune: the fastest schedule of the level's batched RHS
+        (timed on the current state), with its time and the built-in one's."""
+        sc = capi.hm_schedule()
+        ms = (ctypes.c_double * 2)()
+        capi.check(self.rec._ctx, capi.lib.hm_level_autotune(self._lvl, float(dt), int(repetitions),
+                                                             ctypes.byref(sc), ms))
+        return dict(sc.as_dict(), ms=ms[0], default_ms=ms[1])
+
+    def set_schedule(self, sched: dict):
+        sc = capi.hm_schedule()
+        sc.tile_rows, sc.xcd_remap, sc.order = int(sched["tile_rows"]), int(sched["xcd_remap"]), int(sched["order"])
+        capi.check(self.rec._ctx, capi.lib.hm_level_set_schedule(self._lvl, ctypes.byref(sc)))
+
+    def schedule(self) -> dict:
+        sc = capi.hm_schedule()
+        capi.check(self.rec._ctx, capi.lib.hm_level_get_schedule(self._lvl, ctypes.byref(sc)))
+        return sc.as_dict()
+
     def fill_ghosts(self, stream=None):
         capi.check(self.rec._ctx, capi.lib.hm_level_fill_ghosts(self._lvl, self.rec._stream_ptr(stream)))
 

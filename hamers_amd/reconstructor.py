@@ -178,6 +178,41 @@ class ConvectiveFluxReconstructor:
         capi.check(self._ctx, st)
 
 
+def _schedule(sched):
+    if isinstance(sched, capi.hm_schedule):
+        return sched
+    c = capi.hm_schedule()
+    c.tile_rows, c.xcd_remap, c.order = int(sched["tile_rows"]), int(sched["xcd_remap"]), int(sched["order"])
+    return c
+
+
+def autotune(rec: "ConvectiveFluxReconstructor", patches, dt: float, repetitions: int = 3, stream=None) -> dict:
+    """Autotuner of the batched RHS of a patch list (hm_autotune_on_patches):
+    every candidate schedule is timed on these patches (their F is overwritten,
+    their S accumulated), the fastest is pinned for the list.  Returns the
+    schedule with its time and the built-in schedule's (ms)."""
+    plist = patches if isinstance(patches, PatchList) else PatchList(patches, rec.num_arrays)
+    sc = capi.hm_schedule()
+    ms = (ctypes.c_double * 2)()
+    capi.check(rec._ctx, capi.lib.hm_autotune_on_patches(rec._ctx, plist.views, len(plist), float(dt),
+                                                          int(repetitions), rec._stream_ptr(stream),
+                                                          ctypes.byref(sc), ms))
+    return dict(sc.as_dict(), ms=ms[0], default_ms=ms[1])
+
+
+def set_schedule(rec: "ConvectiveFluxReconstructor", patches, sched):
+    plist = patches if isinstance(patches, PatchList) else PatchList(patches, rec.num_arrays)
+    capi.check(rec._ctx, capi.lib.hm_set_schedule_on_patches(rec._ctx, plist.views, len(plist),
+                                                              ctypes.byref(_schedule(sched))))
+
+
+def get_schedule(rec: "ConvectiveFluxReconstructor", patches) -> dict:
+    plist = patches if isinstance(patches, PatchList) else PatchList(patches, rec.num_arrays)
+    sc = capi.hm_schedule()
+    capi.check(rec._ctx, capi.lib.hm_get_schedule_on_patches(rec._ctx, plist.views, len(plist), ctypes.byref(sc)))
+    return sc.as_dict()
+
+
 def computeSpectralRadiusesAndStableDtOnPatches(rec: "ConvectiveFluxReconstructor", patches, stream=None):
     """Euler::computeSpectralRadiusesAndStableDtOnPatch over the interior cells
     of every patch, reduced over the list as RungeKuttaLevelIntegrator::getLevelDt

@@ -156,6 +156,37 @@ int hm_compute_convective_flux_and_source_on_patches(hm_ctx* ctx, const hm_patch
                                                       double time, double dt, int rk_step, void* stream);
 
 /*
+ * Launch schedule of the batched RHS and its autotuner (no counterpart in the
+ * reference; the reference runs one patch at a time).  A schedule fixes how
+ * a patch list's work is laid out over the GPU, not what is computed: every
+ * schedule gives the same results bit for bit.
+ *   tile_rows  single-species model: midpoint-face rows per tile of the fused
+ *              flux along axes 1 and 2 (4, 8 or 24); others: 4 (ignored)
+ *   xcd_remap  1: each of the eight XCDs (own L2) takes a contiguous range of
+ *              block numbers; 0: the hardware's round-robin order
+ *   order      0: work segments in patch order; 1: largest segments first
+ * Without a pinned schedule a list uses the built-in rule (tile rows by the
+ * call's size, xcd_remap 1 for the single-species model, order 0).
+ * hm_autotune_on_patches times every candidate schedule on these views
+ * (`repetitions` timed RHS calls each after one warm-up, median, HIP events on
+ * `stream`), pins the fastest for the list and returns it with its time and
+ * the built-in schedule's in ms[0], ms[1] (ms may be NULL).  The trials write
+ * the views' F and accumulate into their S: tune on scratch outputs or before
+ * the real call.  hm_set_schedule_on_patches pins a schedule;
+ * hm_get_schedule_on_patches returns the one a call on the list would use.
+ */
+typedef struct hm_schedule {
+    int tile_rows;
+    int xcd_remap;
+    int order;
+} hm_schedule;
+int hm_autotune_on_patches(hm_ctx* ctx, const hm_patch_view* patches, int num_patches, double dt, int repetitions,
+                           void* stream, hm_schedule* chosen, double* ms);
+int hm_set_schedule_on_patches(hm_ctx* ctx, const hm_patch_view* patches, int num_patches,
+                               const hm_schedule* schedule);
+int hm_get_schedule_on_patches(hm_ctx* ctx, const hm_patch_view* patches, int num_patches, hm_schedule* schedule);
+
+/*
  * Stage timing (instrumentation; no counterpart in the reference).  While
  * enabled, every RHS call records HIP events on its stream around each of the
  * pipeline stages: 0 derived cell data, 1 dilatation/vorticity, 2 fused
@@ -384,6 +415,15 @@ int hm_level_set_boundary_conditions(hm_level* level, const int* conditions, con
 /* Ghost fill of the scratch arrays: same level (periodic images included),
  * then the physical boundary conditions. */
 int hm_level_fill_ghosts(hm_level* level, void* stream);
+
+/* Schedule of the level's batched RHS (hm_schedule above): hm_level_autotune
+ * times every candidate on the first stage's RHS of the current state and
+ * makes the fastest the schedule of every stage (ms[0], ms[1]: its time and
+ * the built-in schedule's; ms may be NULL); hm_level_set_schedule pins one.
+ * Both drop the level's captured steps. */
+int hm_level_autotune(hm_level* level, double dt, int repetitions, hm_schedule* chosen, double* ms);
+int hm_level_set_schedule(hm_level* level, const hm_schedule* schedule);
+int hm_level_get_schedule(hm_level* level, hm_schedule* schedule);
 
 /* num_steps steps of size dt.  use_graph != 0 replays one step captured in a
  * hipGraph (captured once per dt) instead of issuing its launches. */
