@@ -71,6 +71,28 @@ HM_HD void beta6_adj(const Six& s, double bb3, double* sb) {
     sb[5] += g * (c05 * v[0] + c15 * v[1] + c25 * v[2] + c35 * v[3] + c45 * v[4] + 2.0 * c55 * v[5]);
 }
 
+// Rounding-noise floors of the smoothness indicators, per m^2 with
+// m = max |s_i| over the stencil.  beta3 and beta6 are evaluated in the
+// reference's expanded quadratic forms; their rounding error is below about
+// 10 u times the sum of the magnitudes of their terms (u = 2^-53): at most
+// 33.3 m^2 for each of beta0..beta2 and 1462 m^2 for beta6 (the coefficient
+// magnitudes summed and divided by 232243200).  Where the difference tau6 =
+// |beta6 - beta_avg| (or tau5 = |beta0 - beta2|) is below its floor, its sign
+// and size are rounding noise: on near-uniform O(1) data (variations below
+// ~1e-6 relative) the nonlinear weights then scatter with the last bits of the
+// stencil and carry no derivative in s.  The reverse sweep holds such weights
+// constant -- it propagates only what varies with s above the noise -- while
+// the forward values and every branch stay as the forward pass computed them.
+constexpr double kNoiseBeta3 = 3.7e-14;  // 10 u * 33.3
+constexpr double kNoiseBeta6 = 1.6e-12;  // 10 u * 1462
+
+HM_HD double stencil_mag2(const Six& s) {
+    double m = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) m = fmax(m, fabs(s.v[i]));
+    return m * m;
+}
+
 // Shock measure sigma of ld_sigma: adjoint sigb into sb.
 HM_HD void ld_sigma_adj(const Six& s, double sigb, double* sb) {
     if (sigb == 0.0) return;
@@ -105,7 +127,9 @@ HM_HD void ld_sigma_adj(const Six& s, double sigb, double* sb) {
 // Adjoint of ld_side<P,Q>(s, sigma, k): adds d(out)/ds * ob to sb and
 // d(out)/dsigma * ob to sigb.  The forward quantities are recomputed with
 // reciprocals of the shared denominators (e_k, the weight sums); only the
-// R_tau > alpha_tau predicate keeps the forward's division.
+// R_tau > alpha_tau predicate keeps the forward's division.  The central
+// (upwind) weights are held constant where tau6 (tau5) is below its rounding
+// noise floor.
 template <int P, int Q>
 HM_HD void ld_side_adj(const Six& s, double sigma, const LDConst& k, double ob, double* sb, double& sigb) {
     if (ob == 0.0) return;
@@ -133,6 +157,7 @@ HM_HD void ld_side_adj(const Six& s, double sigma, const LDConst& k, double ob, 
     double W0 = wh0, W1 = wh1, W2 = wh2, W3 = wh3;
     double whb0 = Wb0, whb1 = Wb1, whb2 = Wb2, whb3 = Wb3;
     double bb0 = 0.0, bb1 = 0.0, bb2 = 0.0, bb3 = 0.0;
+    const double mag2 = stencil_mag2(s);
     if (blend) {
         // upwind weights u_k = d_k (1 + r_k^P), r_k = tau5 / e_k, normalized
         const double d02 = b0 - b2;
@@ -153,19 +178,21 @@ HM_HD void ld_side_adj(const Six& s, double sigma, const LDConst& k, double ob, 
         whb2 = om * Wb2;
         whb3 = om * Wb3;
         sigb += (uh0 - wh0) * Wb0 + (uh1 - wh1) * Wb1 + (uh2 - wh2) * Wb2 - wh3 * Wb3;
-        const double uhb0 = sigma * Wb0, uhb1 = sigma * Wb1, uhb2 = sigma * Wb2;
-        const double dotu = uhb0 * uh0 + uhb1 * uh1 + uhb2 * uh2;
-        const double ub0 = (uhb0 - dotu) * ius, ub1 = (uhb1 - dotu) * ius, ub2 = (uhb2 - dotu) * ius;
-        const double rb0 = dpow_times<P>(r0, k.p, ub0 * (1.0 / 16.0)) * ie0;
-        const double rb1 = dpow_times<P>(r1, k.p, ub1 * (5.0 / 8.0)) * ie1;
-        const double rb2 = dpow_times<P>(r2, k.p, ub2 * (5.0 / 16.0)) * ie2;
-        const double tau5b = rb0 + rb1 + rb2;
-        bb0 -= rb0 * r0;
-        bb1 -= rb1 * r1;
-        bb2 -= rb2 * r2;
-        const double sg = sgn(d02);
-        bb0 += tau5b * sg;
-        bb2 -= tau5b * sg;
+        if (tau5 > 2.0 * kNoiseBeta3 * mag2) {
+            const double uhb0 = sigma * Wb0, uhb1 = sigma * Wb1, uhb2 = sigma * Wb2;
+            const double dotu = uhb0 * uh0 + uhb1 * uh1 + uhb2 * uh2;
+            const double ub0 = (uhb0 - dotu) * ius, ub1 = (uhb1 - dotu) * ius, ub2 = (uhb2 - dotu) * ius;
+            const double rb0 = dpow_times<P>(r0, k.p, ub0 * (1.0 / 16.0)) * ie0;
+            const double rb1 = dpow_times<P>(r1, k.p, ub1 * (5.0 / 8.0)) * ie1;
+            const double rb2 = dpow_times<P>(r2, k.p, ub2 * (5.0 / 16.0)) * ie2;
+            const double tau5b = rb0 + rb1 + rb2;
+            bb0 -= rb0 * r0;
+            bb1 -= rb1 * r1;
+            bb2 -= rb2 * r2;
+            const double sg = sgn(d02);
+            bb0 += tau5b * sg;
+            bb2 -= tau5b * sg;
+        }
     }
     // out = sum of linear combinations of the stencil values with weights W.
     sb[0] += ob * (3.0 / 8.0 * W0);
@@ -175,6 +202,10 @@ HM_HD void ld_side_adj(const Six& s, double sigma, const LDConst& k, double ob, 
     sb[4] += ob * (-1.0 / 8.0 * W2 - 10.0 / 8.0 * W3);
     sb[5] += ob * (3.0 / 8.0 * W3);
     // central weights w_k = d_k (C + q_k^Q), q_k = tau6 / e_k, normalized
+    if (!(tau6 > kNoiseBeta6 * mag2)) {
+        if (bb0 != 0.0 || bb1 != 0.0 || bb2 != 0.0) beta3_adj(s, bb0, bb1, bb2, sb);
+        return;
+    }
     const double dotw = whb0 * wh0 + whb1 * wh1 + whb2 * wh2 + whb3 * wh3;
     const double wb0 = (whb0 - dotw) * iws, wb1 = (whb1 - dotw) * iws, wb2 = (whb2 - dotw) * iws,
                  wb3 = (whb3 - dotw) * iws;
@@ -198,6 +229,8 @@ HM_HD void ld_side_adj(const Six& s, double sigma, const LDConst& k, double ob, 
 }
 
 // Adjoint of js_side<P>(s, p): w_k = d_k / e_k^p recomputed as d_k (1/e_k)^p.
+// The weights are held constant where every indicator is below its rounding
+// noise floor.
 template <int P>
 HM_HD void js_side_adj(const Six& s, int p_rt, double ob, double* sb) {
     if (ob == 0.0) return;
@@ -215,6 +248,7 @@ HM_HD void js_side_adj(const Six& s, int p_rt, double ob, double* sb) {
     sb[2] += ob * (15.0 / 8.0 * W0 + 6.0 / 8.0 * W1 + 3.0 / 8.0 * W2);
     sb[3] += ob * (3.0 / 8.0 * W1 + 6.0 / 8.0 * W2);
     sb[4] += ob * (-1.0 / 8.0 * W2);
+    if (!(fmax(fmax(b0, b1), b2) > kNoiseBeta3 * stencil_mag2(s))) return;
     const double Wb0 = ob * (3.0 / 8.0 * v[0] - 10.0 / 8.0 * v[1] + 15.0 / 8.0 * v[2]);
     const double Wb1 = ob * (-1.0 / 8.0 * v[1] + 6.0 / 8.0 * v[2] + 3.0 / 8.0 * v[3]);
     const double Wb2 = ob * (3.0 / 8.0 * v[2] + 6.0 / 8.0 * v[3] - 1.0 / 8.0 * v[4]);
