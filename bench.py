@@ -43,8 +43,18 @@ FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix)
 # FP64 operations per launch (64 lanes x (ADD + MUL + TRANS + 2 FMA) wave
 # instructions of SQ_INSTS_VALU_*_F64), DRAM bytes per launch (FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0, factors from a copy of known size) and the fraction of
-# the VALU issue peak its SQ_INSTS_VALU reaches.  None until measured.
-HEADLINE_KERNEL_COUNTS = {}
+# the VALU issue peak its SQ_INSTS_VALU reaches.
+HEADLINE_KERNEL_COUNTS = {
+    "k_midflux_multi": {
+        "cells": 68726784,
+        # 64 x (3 808 445 963 + 5 664 677 342 + 423 476 106 + 2 x 3 067 398 558)
+        "fp64_ops": 64 * (3808445963 + 5664677342 + 423476106 + 2 * 3067398558),
+        # FETCH_SIZE 20 998 064 KiB x 2.0 + WRITE_SIZE 14 210 952 KiB x 1.0
+        "dram_bytes": 20998064 * 1024 * 2 + 14210952 * 1024,
+        "valu_issue_frac": 0.827,
+        "source": "profiles/rhs_uneven3d_five_eqn_counters.md",
+    },
+}
 # The same for the 2D single-species leg's flux kernels (2048^2 in 128^2
 # patches, WCNS5-JS): DRAM bytes per launch.
 FLUX_TRAFFIC_2048_WCNS5 = {"midflux": 354.92e6 + 135.27e6, "faceflux": 506.47e6 + 135.27e6}
