@@ -24,6 +24,7 @@ from .integrator import (  # noqa: E402,F401
     SSP_RK3_ALPHA,
     SSP_RK3_BETA,
     SSP_RK3_GAMMA,
+    Hierarchy,
     Level,
     RKPatchData,
     advanceSingleStepOnPatch,

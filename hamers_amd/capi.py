@@ -171,6 +171,11 @@ def _load():
     L.hm_level_autotune.argtypes = [P, ctypes.c_double, ctypes.c_int, S, D]
     L.hm_level_set_schedule.argtypes = [P, S]
     L.hm_level_get_schedule.argtypes = [P, S]
+    L.hm_hierarchy_create.argtypes = [P, P, ctypes.c_int, ctypes.POINTER(P)]
+    L.hm_hierarchy_destroy.argtypes = [P]
+    L.hm_hierarchy_advance.argtypes = [P, ctypes.c_double, ctypes.c_int, P]
+    for name in ("hm_hierarchy_create", "hm_hierarchy_destroy", "hm_hierarchy_advance"):
+        getattr(L, name).restype = ctypes.c_int
     for name in ("hm_autotune_on_patches", "hm_set_schedule_on_patches", "hm_get_schedule_on_patches",
                  "hm_level_autotune", "hm_level_set_schedule", "hm_level_get_schedule"):
         getattr(L, name).restype = ctypes.c_int

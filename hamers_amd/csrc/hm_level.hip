@@ -365,6 +365,7 @@ class Level {
         if (ev_b_) (void)hipEventDestroy(ev_b_);
         if (tables_) (void)hipFree(tables_);
         if (storage_) (void)hipFree(storage_);
+        if (acc_mem_) (void)hipFree(acc_mem_);
     }
 
     int init(int np, const int* lo, const int* n, const int* dlo, const int* dn, const int* periodic, const double* dx) {
@@ -538,6 +539,119 @@ class Level {
         return build_rhs_plans();
     }
     const Schedule& schedule() const { return sched_; }
+
+    // Flux sums for the synchronization of a two-level hierarchy: F_acc of
+    // every axis and S_acc accumulate gamma_s F_s and gamma_s S_s over the
+    // stages of every step (Euler.cpp:1230-1270, 1360-1420, 1560-1660) until
+    // clearFluxSums.  Drops the captured steps (their tables change).
+    int enableFluxSums() {
+        if (acc_mem_) return HM_OK;
+        size_t total = 0;
+        std::vector<size_t> off(size_t(np_) * 4);
+        for (int p = 0; p < np_; p++) {
+            for (int a = 0; a < d_; a++) {
+                off[size_t(p) * 4 + a] = total;
+                total += up256(sizeof(double) * size_t(neq_) * fvol(p, a));
+            }
+            off[size_t(p) * 4 + 3] = total;
+            total += up256(sizeof(double) * size_t(neq_) * ivol(p));
+        }
+        if (hipDeviceSynchronize() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "device synchronize failed");
+        if (hipMalloc(&acc_mem_, total) != hipSuccess) {
+            acc_mem_ = nullptr;
+            return rec_->fail(HM_ERR_OUT_OF_MEMORY, "level flux sums");
+        }
+        if (hipMemset(acc_mem_, 0, total) != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "flux-sum clear");
+        acc_bytes_ = total;
+        facc_.assign(size_t(np_) * 3, nullptr);
+        sacc_.assign(size_t(np_), nullptr);
+        char* b = static_cast<char*>(acc_mem_);
+        for (int p = 0; p < np_; p++) {
+            for (int a = 0; a < d_; a++) facc_[size_t(p) * 3 + a] = reinterpret_cast<double*>(b + off[size_t(p) * 4 + a]);
+            sacc_[p] = reinterpret_cast<double*>(b + off[size_t(p) * 4 + 3]);
+        }
+        drop_graphs();
+        return build_tables();
+    }
+
+    int clearFluxSums(hipStream_t s) {
+        if (!acc_mem_) return HM_OK;
+        return hipMemsetAsync(acc_mem_, 0, acc_bytes_, s) == hipSuccess ? HM_OK
+                                                                        : rec_->fail(HM_ERR_DEVICE, "flux-sum clear");
+    }
+
+    // Pieces of the level a two-level hierarchy drives.
+    int stepOnce(double dt, hipStream_t s) { return step(dt, s); }
+    int fillGhostsOf(int buf, hipStream_t s) { return launch_halo(buf, s); }
+    const LevelPatch* devPatches() const { return patch_dev(); }
+    const LevelPatch& hostPatch(int p) const { return patch_[p]; }
+    double* fluxSum(int p, int a) const { return facc_[size_t(p) * 3 + a]; }
+    double* sourceSum(int p) const { return sacc_[p]; }
+    int dim() const { return d_; }
+    bool diffusive() const { return diff_ != nullptr; }
+    int numEqn() const { return neq_; }
+    const int* patchLo(int p) const { return &lo_[3 * p]; }
+    const int* patchN(int p) const { return &n_[3 * p]; }
+    const int* domainLo() const { return dlo_; }
+    const int* domainN() const { return dn_; }
+    bool periodic(int a) const { return periodic_[a]; }
+    const double* dx() const { return dx_; }
+    int64_t ghostVolume(int p) const { return gvol(p); }
+    ConvectiveFluxReconstructorGPU* reconstructor() const { return rec_; }
+    // The patch whose interior holds cell x (global index, wrapped on
+    // periodic axes), or -1.  Patches are looked up through buckets of
+    // kBucket^d cells over the domain, built on first use.
+    int patchOf(const int* x) {
+        if (bucket_start_.empty()) build_buckets();
+        int v[3] = {0, 0, 0};
+        int64_t b = 0, st = 1;
+        for (int a = 0; a < d_; a++) {
+            v[a] = x[a];
+            if (periodic_[a]) v[a] = dlo_[a] + ((v[a] - dlo_[a]) % dn_[a] + dn_[a]) % dn_[a];
+            if (v[a] < dlo_[a] || v[a] >= dlo_[a] + dn_[a]) return -1;
+            b += int64_t((v[a] - dlo_[a]) / kBucket) * st;
+            st *= nbucket_[a];
+        }
+        for (int i = bucket_start_[b]; i < bucket_start_[b + 1]; i++) {
+            const int p = bucket_items_[i];
+            bool in = true;
+            for (int a = 0; a < d_ && in; a++) in = v[a] >= lo_[3 * p + a] && v[a] < lo_[3 * p + a] + n_[3 * p + a];
+            if (in) return p;
+        }
+        return -1;
+    }
+
+    static constexpr int kBucket = 16;
+    void build_buckets() {
+        int64_t nb = 1;
+        for (int a = 0; a < 3; a++) {
+            nbucket_[a] = a < d_ ? (dn_[a] + kBucket - 1) / kBucket : 1;
+            nb *= nbucket_[a];
+        }
+        std::vector<int> count(size_t(nb) + 1, 0);
+        auto each = [&](int p, auto&& f) {  // the buckets patch p's box (clipped to the domain) overlaps
+            int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+            for (int a = 0; a < 3; a++) {
+                if (a < d_) {
+                    const int l = std::max(lo_[3 * p + a], dlo_[a]) - dlo_[a];
+                    const int h = std::min(lo_[3 * p + a] + n_[3 * p + a], dlo_[a] + dn_[a]) - dlo_[a];
+                    if (h <= l) return;
+                    lo[a] = l / kBucket;
+                    hi[a] = (h - 1) / kBucket;
+                }
+            }
+            for (int z = lo[2]; z <= hi[2]; z++)
+                for (int y = lo[1]; y <= hi[1]; y++)
+                    for (int x = lo[0]; x <= hi[0]; x++) f((int64_t(z) * nbucket_[1] + y) * nbucket_[0] + x);
+        };
+        for (int p = 0; p < np_; p++) each(p, [&](int64_t b) { count[size_t(b) + 1]++; });
+        for (int64_t b = 0; b < nb; b++) count[size_t(b) + 1] += count[size_t(b)];
+        bucket_start_ = count;
+        bucket_items_.assign(size_t(count[size_t(nb)]), 0);
+        for (int p = 0; p < np_; p++) each(p, [&](int64_t b) { bucket_items_[size_t(count[size_t(b)]++)] = p; });
+    }
+    int nbucket_[3] = {1, 1, 1};
+    std::vector<int> bucket_start_, bucket_items_;
 
     // num_steps steps of the configured Runge-Kutta scheme with fixed dt.
     int advance(double dt, int nsteps, bool use_graph, hipStream_t s) {
@@ -1018,7 +1132,21 @@ class Level {
         TableImage<RKPatch> t;
         t.items.resize(np_);
         for (int p = 0; p < np_; p++) t.add(p, 0, ivol(p));
-        return t.bytes() * kMaxRK;
+        size_t b = t.bytes() * kMaxRK;
+        if (acc_mem_) b += acc_table(0).bytes() * kMaxRK;
+        return b;
+    }
+
+    // Work table of the gamma accumulation (k_rk_accumulate): per patch the
+    // faces of each axis, then the cells (axis code dim).
+    TableImage<RKPatch> acc_table(int) const {
+        TableImage<RKPatch> t;
+        t.items.resize(np_);
+        for (int p = 0; p < np_; p++) {
+            for (int a = 0; a < d_; a++) t.add(p, a, fvol(p, a));
+            t.add(p, d_, ivol(p));
+        }
+        return t;
     }
 
     // One stage-update table per stage (the intermediate pointers differ).
@@ -1047,11 +1175,22 @@ class Level {
                         for (int a = 0; a < d_; a++) R.Fdi[m][a] = fdl_[(size_t(p) * kMaxRK + m) * 3 + a];
                     R.Si[m] = patch_[p].S[m];
                 }
+                if (acc_mem_) {
+                    for (int a = 0; a < d_; a++) R.Facc[a] = facc_[size_t(p) * 3 + a];
+                    R.Sacc = sacc_[p];
+                }
                 t.add(p, 0, ivol(p));
             }
             rk_[s] = t.write(h.data() + o, d + o);
             rk_items_ = t.total;
             o += t.bytes();
+            if (acc_mem_) {
+                TableImage<RKPatch> at = acc_table(s);
+                at.items = t.items;
+                acc_[s] = at.write(h.data() + o, d + o);
+                acc_items_ = at.total;
+                o += at.bytes();
+            }
             RKParams& r = rkp_[s];
             r.nc = s + 1;
             r.neq = neq_;
@@ -1116,7 +1255,8 @@ class Level {
                 return st;
             if (diff_)
                 if (int st = diff_->runOwnedPlan(diff_plan_[sn], dt, s, dtp)) return st;
-            if (launch_rk_update<SegTableT<RKPatch>>(d_, rk_[sn], rk_items_, nullptr, 0, rkp_[sn], s) != hipSuccess)
+            if (launch_rk_update<SegTableT<RKPatch>>(d_, rk_[sn], rk_items_, acc_mem_ ? &acc_[sn] : nullptr,
+                                                     acc_items_, rkp_[sn], s) != hipSuccess)
                 return rec_->fail(HM_ERR_DEVICE, "stage update launch failed");
         }
         return HM_OK;
@@ -1540,6 +1680,11 @@ class Level {
     Schedule sched_{};  // schedule of the batched RHS plans
     SegTableT<LevelPatch> copy_{}, zero_{};
     SegTableT<RKPatch> rk_[kMaxRK] = {};
+    SegTableT<RKPatch> acc_[kMaxRK] = {};
+    int64_t acc_items_ = 0;
+    void* acc_mem_ = nullptr;           // flux sums (enableFluxSums)
+    size_t acc_bytes_ = 0;
+    std::vector<double*> facc_, sacc_;  // per patch and axis / per patch
     RKParams rkp_[kMaxRK] = {};
     int64_t halo_items_ = 0, copy_items_ = 0, zero_items_ = 0, rk_items_ = 0;
     size_t rk_off_ = 0;
@@ -1719,11 +1864,524 @@ int advance_single_step(ConvectiveFluxReconstructorGPU* rec, const hm_rk_patch_v
     return rec->stageTablesInUse(s);
 }
 
+// ----------------------------------------------------------------------------
+// Two-level hierarchy (refinement ratio 2): the time-refined advance of
+// SAMRAI's TimeRefinementIntegrator as RungeKuttaLevelIntegrator drives it --
+// one coarse step, two fine steps of half the size, then
+// RungeKuttaLevelIntegrator::synchronizeLevelWithCoarser
+// (RungeKuttaLevelIntegrator.cpp:2131-2225).
+//   * Coarse-fine ghost cells of the fine level (inside the domain, covered by
+//     no fine patch) are filled before each fine step from the coarse level,
+//     time-interpolated between its states at the start and end of the coarse
+//     step (the advance fill with the coarser level, :1500-1548, linear time
+//     interpolation) and refined by conservative linear interpolation (the
+//     CONSERVATIVE_LINEAR_REFINE operator the flow models register,
+//     FlowModelSingleSpecies.cpp:578-597): monotonized-central limited slopes.
+//     The intermediate stages keep them (their fills are same-level only,
+//     :2513-2520).
+//   * Both levels accumulate gamma-weighted flux sums; the coarse faces on the
+//     coarse-fine boundary take the area average of the fine flux sums over
+//     the two fine steps (upfluxsum, algs_upfluxsum2d.m4, coarsened to the
+//     coarse level), the coarse update is repeated from the coarse state at the
+//     start of the step with these fluxes (Euler::synchronizeFluxes,
+//     Euler.cpp:1682-1990: Q += (-(F_R - F_L)/dx ... + S)), and the coarse
+//     cells under the fine level take the average of their fine cells
+//     (CONSERVATIVE_COARSEN).
+// SAMRAI's refine and coarsen operators are not part of the reference; the
+// ones here are the published forms those operator names denote, and they
+// are pinned by conservation, exactness and monotonicity checks.
+// ----------------------------------------------------------------------------
+
+struct CfItem {      // one coarse-fine ghost cell of a fine patch
+    int fp, cq;      // fine patch, coarse patch holding the parent cell
+    int64_t fidx;    // cell in the fine patch's ghost box
+    int64_t cidx;    // parent in the coarse patch's ghost box
+    int child;       // bit a: the cell is the upper child along axis a
+};
+
+struct RefluxItem {  // one coarse face on the coarse-fine boundary, in a coarse patch's face array
+    int cq, axis, fp, nf;
+    int64_t cface;
+    int64_t fface[4];
+};
+
+struct CoarsenItem { // one coarse cell under the fine level
+    int cq, fp;
+    int64_t cidx;    // coarse ghost-box index
+    int64_t fidx;    // ghost-box index of the lower-corner child in the fine patch
+};
+
+struct SyncPatch {   // coarse patch data the synchronization kernels use
+    int n[3];
+    int64_t gvol;
+    double dx[3];
+    const double* Qold;  // state at the start of the coarse step (nq comps over the ghost box)
+    double* Q;           // state (nq comps over the ghost box)
+    double* Facc[3];     // flux sums (neq comps over the faces of each axis)
+    const double* Sacc;  // source sums (neq comps over the interior)
+};
+
+// MC-limited slope of the coarse values vL, v0, vR (per unit coarse cell).
+__device__ __forceinline__ double mc_slope(double vL, double v0, double vR) {
+    const double dR = vR - v0, dL = v0 - vL;
+    if (!(dR * dL > 0.0)) return 0.0;
+    const double c = 0.5 * (dR + dL);
+    const double b = 2.0 * fmin(fabs(dR), fabs(dL));
+    return copysign(fmin(fabs(c), b), c);
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_cf_fill(const CfItem* items, int64_t n_items, const LevelPatch* fine,
+                                                    const LevelPatch* coarse, double alpha, int nq, int g) {
+    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= n_items) return;
+    const CfItem it = items[t];
+    const LevelPatch& C = coarse[it.cq];
+    const int64_t st[3] = {1, int64_t(C.n[0] + 2 * g), int64_t(C.n[0] + 2 * g) * ((D > 1) ? C.n[1] + 2 * g : 1)};
+    const double* old = C.Q[1];  // the coarse step's first stage input: its start state, ghosts filled
+    const double* nw = C.Q[0];
+    const LevelPatch& F = fine[it.fp];
+    for (int c = 0; c < nq; c++) {
+        auto val = [&](int64_t i) {
+            const double o = old[c * C.gvol + i];
+            return o + alpha * (nw[c * C.gvol + i] - o);
+        };
+        const double v0 = val(it.cidx);
+        double v = v0;
+        for (int a = 0; a < D; a++) {
+            const double sl = mc_slope(val(it.cidx - st[a]), v0, val(it.cidx + st[a]));
+            v = v + sl * (((it.child >> a) & 1) ? 0.25 : -0.25);
+        }
+        F.Q[0][c * F.gvol + it.fidx] = v;
+    }
+}
+
+// Flux sums of the fine faces on the coarse-fine boundary: after each fine step
+// its gamma-weighted flux is added to a sum that starts at zero
+// (RungeKuttaLevelIntegrator::postprocessFluxAndSourceData,
+// RungeKuttaLevelIntegrator.cpp:2920-2967, 3060-3180; algs_upfluxsum2d.m4).
+// fsum holds [item][fine face j < 4][equation].
+__global__ __launch_bounds__(kBlock) void k_fsum_add(const RefluxItem* items, int64_t n_items,
+                                                     double* const* fine_facc, const int64_t* fine_fvol, int neq,
+                                                     double* fsum, int first) {
+    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= n_items) return;
+    const RefluxItem it = items[t];
+    const double* ff = fine_facc[it.fp * 3 + it.axis];
+    const int64_t fv = fine_fvol[it.fp * 3 + it.axis];
+    for (int j = 0; j < it.nf; j++)
+        for (int e = 0; e < neq; e++) {
+            double* o = fsum + (t * 4 + j) * neq + e;
+            const double v = ff[e * fv + it.fface[j]];
+            *o = first ? 0.0 + v : *o + v;
+        }
+}
+
+// A coarse face on the coarse-fine boundary takes the area average of the
+// fine flux sums over it: their sum times 2^(1-d).
+__global__ __launch_bounds__(kBlock) void k_reflux(const RefluxItem* items, int64_t n_items, const SyncPatch* coarse,
+                                                   const double* fsum, int neq) {
+    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= n_items) return;
+    const RefluxItem it = items[t];
+    const SyncPatch& C = coarse[it.cq];
+    int64_t cv = 1;
+    for (int b = 0; b < 3; b++) cv *= C.n[b] + (b == it.axis);
+    const double w = 1.0 / double(it.nf);
+    for (int e = 0; e < neq; e++) {
+        double sum = fsum[(t * 4) * neq + e];
+        for (int j = 1; j < it.nf; j++) sum += fsum[(t * 4 + j) * neq + e];
+        C.Facc[it.axis][e * cv + it.cface] = sum * w;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_sync_update(SegTableT<SyncPatch> w, int neq, int g, int nz, int zfirst) {
+    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const SyncPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;
+    const int n0 = P->n[0], n1 = (D > 1) ? P->n[1] : 1, n2 = (D > 2) ? P->n[2] : 1;
+    const int64_t r = loc / n0;
+    const int i = int(loc - r * n0);
+    const int j = (D > 1) ? int(r % n1) : 0;
+    const int k = (D > 2) ? int(r / n1) : 0;
+    const int e0 = n0 + 2 * g, e1 = (D > 1) ? n1 + 2 * g : 1;
+    const int64_t iq = (i + g) + int64_t(j + ((D > 1) ? g : 0)) * e0 + int64_t(k + ((D > 2) ? g : 0)) * e0 * e1;
+    const int64_t fL0 = i + int64_t(j) * (n0 + 1) + int64_t(k) * (n0 + 1) * n1, fR0 = fL0 + 1;
+    const int64_t fL1 = i + int64_t(j) * n0 + int64_t(k) * n0 * (n1 + 1), fR1 = fL1 + n0;
+    const int64_t fL2 = loc, fR2 = loc + int64_t(n0) * n1;
+    const int64_t fv0 = int64_t(n0 + 1) * n1 * n2, fv1 = int64_t(n0) * (n1 + 1) * n2, fv2 = int64_t(n0) * n1 * (n2 + 1);
+    const int64_t sv = int64_t(n0) * n1 * n2;
+    for (int e = 0; e < neq; e++) {
+        // Euler::synchronizeFluxes (Euler.cpp:1682-1943): Q += (-(F_x,R - F_x,L)/dx_0 - ... + S)
+        double t = -(P->Facc[0][e * fv0 + fR0] - P->Facc[0][e * fv0 + fL0]) / P->dx[0];
+        if constexpr (D > 1) t = t - (P->Facc[1][e * fv1 + fR1] - P->Facc[1][e * fv1 + fL1]) / P->dx[1];
+        if constexpr (D > 2) t = t - (P->Facc[2][e * fv2 + fR2] - P->Facc[2][e * fv2 + fL2]) / P->dx[2];
+        P->Q[e * P->gvol + iq] = P->Qold[e * P->gvol + iq] + (t + P->Sacc[e * sv + loc]);
+    }
+    // Five-equation model: the last volume fraction is 1 minus the others, in
+    // species order (FlowModelFiveEqnAllaire::updateCellDataOfConservativeVariables,
+    // FlowModelFiveEqnAllaire.cpp:1739-1890, called at Euler.cpp:1937-1939).
+    if (nz > 0) {
+        double zl = 1.0;
+        for (int z = 0; z < nz; z++) zl -= P->Q[(zfirst + z) * P->gvol + iq];
+        P->Q[neq * P->gvol + iq] = zl;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_coarsen(const CoarsenItem* items, int64_t n_items, const LevelPatch* coarse,
+                                                    const LevelPatch* fine, int nq, int g) {
+    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= n_items) return;
+    const CoarsenItem it = items[t];
+    const LevelPatch& C = coarse[it.cq];
+    const LevelPatch& F = fine[it.fp];
+    const int64_t s1 = F.n[0] + 2 * g, s2 = s1 * ((D > 1) ? F.n[1] + 2 * g : 1);
+    constexpr int NC = 1 << D;
+    const double w = 1.0 / double(NC);
+    for (int c = 0; c < nq; c++) {
+        const double* f = F.Q[0] + c * F.gvol + it.fidx;
+        double sum = f[0];
+        for (int q = 1; q < NC; q++) sum += f[(q & 1) + ((q >> 1) & 1) * s1 + ((q >> 2) & 1) * s2];
+        C.Q[0][c * C.gvol + it.cidx] = sum * w;
+    }
+}
+
+class Hierarchy {
+   public:
+    Hierarchy(Level* coarse, Level* fine) : c_(coarse), f_(fine) {}
+    ~Hierarchy() {
+        if (mem_) (void)hipFree(mem_);
+    }
+
+    int init() {
+        ConvectiveFluxReconstructorGPU* rec = c_->reconstructor();
+        const int d = c_->dim();
+        if (f_->reconstructor() != rec) return rec->fail(HM_ERR_INVALID_ARGUMENT, "levels of different contexts");
+        if (f_->dim() != d || f_->ghost() != c_->ghost())
+            return rec->fail(HM_ERR_INVALID_ARGUMENT, "levels of different dimension or ghost width");
+        if (c_->diffusive())
+            return rec->fail(HM_ERR_UNSUPPORTED,
+                             "the two-level advance keeps convective flux sums only (no Navier-Stokes form)");
+        for (int a = 0; a < d; a++) {
+            if (f_->domainLo()[a] != 2 * c_->domainLo()[a] || f_->domainN()[a] != 2 * c_->domainN()[a])
+                return rec->fail(HM_ERR_INVALID_ARGUMENT, "the fine domain must be the coarse domain refined by 2");
+            if (f_->periodic(a) != c_->periodic(a))
+                return rec->fail(HM_ERR_INVALID_ARGUMENT, "the levels' periodic axes differ");
+            if (std::fabs(c_->dx()[a] - 2.0 * f_->dx()[a]) > 1e-12 * c_->dx()[a])
+                return rec->fail(HM_ERR_INVALID_ARGUMENT, "coarse dx must be twice the fine dx");
+        }
+        for (int p = 0; p < f_->numPatches(); p++)
+            for (int a = 0; a < d; a++)
+                if ((f_->patchLo(p)[a] & 1) || (f_->patchN(p)[a] & 1))
+                    return rec->fail(HM_ERR_INVALID_ARGUMENT, "fine patch boxes must start and end on coarse cells");
+        g_ = c_->ghost();
+        if (int st = c_->enableFluxSums()) return st;
+        if (int st = f_->enableFluxSums()) return st;
+        return build();
+    }
+
+    // num_steps coarse steps of dt, each with two fine steps of dt / 2 and the
+    // synchronization.
+    int advance(double dt, int nsteps, hipStream_t s) {
+        ConvectiveFluxReconstructorGPU* rec = c_->reconstructor();
+        if (nsteps < 0 || !(dt > 0.0)) return rec->fail(HM_ERR_INVALID_ARGUMENT, "bad dt or step count");
+        if (int st = rec->selectDevice()) return st;
+        const int d = c_->dim();
+        for (int it = 0; it < nsteps; it++) {
+            if (int st = c_->clearFluxSums(s)) return st;
+            if (int st = c_->stepOnce(dt, s)) return st;
+            if (int st = c_->fillGhostsOf(0, s)) return st;  // the coarse state at the end of the step
+            for (int k = 0; k < 2; k++) {
+                if (int st = f_->clearFluxSums(s)) return st;  // the fine step's own flux sums
+                // coarse-fine ghosts first: the fine step's own fill (same level,
+                // then the physical boundary conditions, which may read them) follows
+                if (ncf_ > 0) {
+                    const double alpha = 0.5 * k;
+                    switch (d) {
+                        case 1: hipLaunchKernelGGL(k_cf_fill<1>, dim3(blocks_for(ncf_)), dim3(kBlock), 0, s, cf_, ncf_, f_->devPatches(), c_->devPatches(), alpha, f_->numArrays(), g_); break;
+                        case 2: hipLaunchKernelGGL(k_cf_fill<2>, dim3(blocks_for(ncf_)), dim3(kBlock), 0, s, cf_, ncf_, f_->devPatches(), c_->devPatches(), alpha, f_->numArrays(), g_); break;
+                        default: hipLaunchKernelGGL(k_cf_fill<3>, dim3(blocks_for(ncf_)), dim3(kBlock), 0, s, cf_, ncf_, f_->devPatches(), c_->devPatches(), alpha, f_->numArrays(), g_); break;
+                    }
+                }
+                if (int st = f_->stepOnce(0.5 * dt, s)) return st;
+                if (nrf_ > 0)
+                    hipLaunchKernelGGL(k_fsum_add, dim3(blocks_for(nrf_)), dim3(kBlock), 0, s, rf_, nrf_, ffacc_, ffvol_,
+                                       c_->numEqn(), fsum_, int(k == 0));
+            }
+            if (nrf_ > 0)
+                hipLaunchKernelGGL(k_reflux, dim3(blocks_for(nrf_)), dim3(kBlock), 0, s, rf_, nrf_, sync_.patches, fsum_,
+                                   c_->numEqn());
+            const hm_config& cfg = rec->config();
+            const bool five = cfg.model == HM_MODEL_FIVE_EQN_ALLAIRE;
+            const int nz = five ? cfg.num_species - 1 : 0, zfirst = five ? cfg.num_species + d + 1 : 0;
+            switch (d) {
+                case 1: hipLaunchKernelGGL(k_sync_update<1>, dim3(blocks_for(sync_items_)), dim3(kBlock), 0, s, sync_, c_->numEqn(), g_, nz, zfirst); break;
+                case 2: hipLaunchKernelGGL(k_sync_update<2>, dim3(blocks_for(sync_items_)), dim3(kBlock), 0, s, sync_, c_->numEqn(), g_, nz, zfirst); break;
+                default: hipLaunchKernelGGL(k_sync_update<3>, dim3(blocks_for(sync_items_)), dim3(kBlock), 0, s, sync_, c_->numEqn(), g_, nz, zfirst); break;
+            }
+            if (ncs_ > 0) {
+                switch (d) {
+                    case 1: hipLaunchKernelGGL(k_coarsen<1>, dim3(blocks_for(ncs_)), dim3(kBlock), 0, s, cs_, ncs_, c_->devPatches(), f_->devPatches(), c_->numArrays(), g_); break;
+                    case 2: hipLaunchKernelGGL(k_coarsen<2>, dim3(blocks_for(ncs_)), dim3(kBlock), 0, s, cs_, ncs_, c_->devPatches(), f_->devPatches(), c_->numArrays(), g_); break;
+                    default: hipLaunchKernelGGL(k_coarsen<3>, dim3(blocks_for(ncs_)), dim3(kBlock), 0, s, cs_, ncs_, c_->devPatches(), f_->devPatches(), c_->numArrays(), g_); break;
+                }
+            }
+            if (hipGetLastError() != hipSuccess) return rec->fail(HM_ERR_DEVICE, "hierarchy launch failed");
+        }
+        return HM_OK;
+    }
+
+   private:
+    // Whether coarse cell x (global, wrapped on periodic axes) lies under a fine patch; its fine patch.
+    int fine_over(const int* xc) const {
+        const int d = c_->dim();
+        int xf[3] = {0, 0, 0};
+        for (int a = 0; a < d; a++) xf[a] = 2 * xc[a];
+        return f_->patchOf(xf);
+    }
+
+    int build() {
+        ConvectiveFluxReconstructorGPU* rec = c_->reconstructor();
+        const int d = c_->dim(), g = g_;
+        std::vector<CfItem> cf;
+        std::vector<RefluxItem> rf;
+        std::vector<CoarsenItem> cs;
+        auto ghost_index = [&](const int* n, const int* x) {  // x in ghost-box coordinates
+            int64_t i = 0, st = 1;
+            for (int a = 0; a < d; a++) {
+                i += x[a] * st;
+                st *= n[a] + 2 * g;
+            }
+            return i;
+        };
+        auto wrap = [&](const Level* L, int a, int v) {
+            if (!L->periodic(a)) return v;
+            const int lo = L->domainLo()[a], n = L->domainN()[a];
+            return lo + ((v - lo) % n + n) % n;
+        };
+        // coarse-fine ghost cells of every fine patch
+        for (int p = 0; p < f_->numPatches(); p++) {
+            const int* lo = f_->patchLo(p);
+            const int* n = f_->patchN(p);
+            int ext[3] = {1, 1, 1};
+            for (int a = 0; a < d; a++) ext[a] = n[a] + 2 * g;
+            for (int z = 0; z < ext[2]; z++)
+                for (int y = 0; y < ext[1]; y++)
+                    for (int x = 0; x < ext[0]; x++) {
+                        const int gx[3] = {x, y, z};
+                        int xg[3] = {0, 0, 0};  // global fine index
+                        bool interior = true, outside = false;
+                        for (int a = 0; a < d; a++) {
+                            xg[a] = lo[a] - g + gx[a];
+                            if (gx[a] < g || gx[a] >= g + n[a]) interior = false;
+                            if (!f_->periodic(a) &&
+                                (xg[a] < f_->domainLo()[a] || xg[a] >= f_->domainLo()[a] + f_->domainN()[a]))
+                                outside = true;
+                        }
+                        if (interior || outside || f_->patchOf(xg) >= 0) continue;
+                        int xw[3] = {0, 0, 0}, xc[3] = {0, 0, 0}, child = 0;
+                        for (int a = 0; a < d; a++) {
+                            xw[a] = wrap(f_, a, xg[a]);
+                            xc[a] = (xw[a] >= 0) ? xw[a] / 2 : -((-xw[a] + 1) / 2);
+                            if (xw[a] - 2 * xc[a]) child |= 1 << a;
+                        }
+                        const int q = c_->patchOf(xc);
+                        if (q < 0)
+                            return rec->fail(HM_ERR_INVALID_ARGUMENT,
+                                             "a coarse-fine ghost cell has no coarse parent: the fine level is not nested");
+                        int cgx[3] = {0, 0, 0};
+                        for (int a = 0; a < d; a++) {
+                            cgx[a] = wrap(c_, a, xc[a]) - c_->patchLo(q)[a] + g;
+                            if (cgx[a] < 1 || cgx[a] > c_->patchN(q)[a] + 2 * g - 2)
+                                return rec->fail(HM_ERR_INVALID_ARGUMENT, "coarse parent too close to its ghost-box edge");
+                        }
+                        cf.push_back(CfItem{p, q, ghost_index(n, gx), ghost_index(c_->patchN(q), cgx), child});
+                    }
+        }
+        // coarse cells under the fine level, and coarse faces on the coarse-fine boundary
+        for (int q = 0; q < c_->numPatches(); q++) {
+            const int* lo = c_->patchLo(q);
+            const int* n = c_->patchN(q);
+            int ext[3] = {1, 1, 1};
+            for (int a = 0; a < d; a++) ext[a] = n[a];
+            for (int z = 0; z < ext[2]; z++)
+                for (int y = 0; y < ext[1]; y++)
+                    for (int x = 0; x < ext[0]; x++) {
+                        const int ix[3] = {x, y, z};
+                        int xc[3] = {0, 0, 0}, gx[3] = {0, 0, 0};
+                        for (int a = 0; a < d; a++) {
+                            xc[a] = lo[a] + ix[a];
+                            gx[a] = ix[a] + g;
+                        }
+                        const int p = fine_over(xc);
+                        if (p < 0) continue;
+                        int fx[3] = {0, 0, 0};
+                        for (int a = 0; a < d; a++) fx[a] = 2 * xc[a] - f_->patchLo(p)[a] + g;
+                        cs.push_back(CoarsenItem{q, p, ghost_index(n, gx), ghost_index(f_->patchN(p), fx)});
+                    }
+            for (int a = 0; a < d; a++) {
+                int fe[3] = {1, 1, 1};
+                for (int b = 0; b < d; b++) fe[b] = n[b] + (b == a);
+                for (int z = 0; z < fe[2]; z++)
+                    for (int y = 0; y < fe[1]; y++)
+                        for (int x = 0; x < fe[0]; x++) {
+                            const int ix[3] = {x, y, z};
+                            int R[3] = {0, 0, 0}, Lc[3] = {0, 0, 0};
+                            for (int b = 0; b < d; b++) {
+                                R[b] = lo[b] + ix[b];
+                                Lc[b] = R[b] - (b == a);
+                            }
+                            int Rw[3] = {0, 0, 0}, Lw[3] = {0, 0, 0};
+                            bool outL = false, outR = false;
+                            for (int b = 0; b < d; b++) {
+                                Rw[b] = wrap(c_, b, R[b]);
+                                Lw[b] = wrap(c_, b, Lc[b]);
+                                const int dlo = c_->domainLo()[b], dhi = dlo + c_->domainN()[b];
+                                if (Rw[b] < dlo || Rw[b] >= dhi) outR = true;
+                                if (Lw[b] < dlo || Lw[b] >= dhi) outL = true;
+                            }
+                            if (outL || outR) continue;  // a physical boundary face
+                            const int pL = fine_over(Lw), pR = fine_over(Rw);
+                            if ((pL >= 0) == (pR >= 0)) continue;
+                            const int p = (pL >= 0) ? pL : pR;
+                            RefluxItem it{};
+                            it.cq = q;
+                            it.axis = a;
+                            it.fp = p;
+                            int64_t cf_i = 0, st = 1;
+                            for (int b = 0; b < d; b++) {
+                                cf_i += ix[b] * st;
+                                st *= fe[b];
+                            }
+                            it.cface = cf_i;
+                            // the fine faces: along a at 2 R (the covered side's fine patch face), 2^(d-1) transverse children
+                            const int* flo = f_->patchLo(p);
+                            const int* fn = f_->patchN(p);
+                            const int* Rref = (pL >= 0) ? Lw : Rw;  // a cell of the covered side
+                            int ffe[3] = {1, 1, 1};
+                            for (int b = 0; b < d; b++) ffe[b] = fn[b] + (b == a);
+                            int nf = 0;
+                            for (int t = 0; t < (1 << (d - 1)); t++) {
+                                int fx[3] = {0, 0, 0}, bit = 0;
+                                for (int b = 0; b < d; b++) {
+                                    if (b == a) {
+                                        // the face between the two sides, in the fine patch's face numbering
+                                        const int face_global = (pL >= 0) ? 2 * (Lw[a] + 1) : 2 * Rw[a];
+                                        fx[b] = face_global - flo[b];
+                                    } else {
+                                        fx[b] = 2 * Rref[b] + ((t >> bit) & 1) - flo[b];
+                                        bit++;
+                                    }
+                                    if (fx[b] < 0 || fx[b] >= ffe[b])
+                                        return rec->fail(HM_ERR_INVALID_ARGUMENT, "fine face outside its patch");
+                                }
+                                int64_t fi = 0, fst = 1;
+                                for (int b = 0; b < d; b++) {
+                                    fi += fx[b] * fst;
+                                    fst *= ffe[b];
+                                }
+                                it.fface[nf++] = fi;
+                            }
+                            it.nf = nf;
+                            rf.push_back(it);
+                        }
+            }
+        }
+        // sync tables: coarse patch descriptors and their cells
+        const int ncp = c_->numPatches(), nfp = f_->numPatches();
+        TableImage<SyncPatch> sync;
+        sync.items.resize(ncp);
+        for (int q = 0; q < ncp; q++) {
+            SyncPatch& P = sync.items[q];
+            std::memset(&P, 0, sizeof P);
+            for (int a = 0; a < 3; a++) {
+                P.n[a] = a < d ? c_->patchN(q)[a] : 1;
+                P.dx[a] = a < d ? c_->dx()[a] : 1.0;
+            }
+            P.gvol = c_->ghostVolume(q);
+            P.Qold = c_->hostPatch(q).Q[1];
+            P.Q = c_->hostPatch(q).Q[0];
+            for (int a = 0; a < d; a++) P.Facc[a] = c_->fluxSum(q, a);
+            P.Sacc = c_->sourceSum(q);
+            int64_t v = 1;
+            for (int a = 0; a < d; a++) v *= c_->patchN(q)[a];
+            sync.add(q, 0, v);
+        }
+        std::vector<double*> ffacc(size_t(nfp) * 3, nullptr);
+        std::vector<int64_t> ffvol(size_t(nfp) * 3, 0);
+        for (int p = 0; p < nfp; p++)
+            for (int a = 0; a < d; a++) {
+                ffacc[size_t(p) * 3 + a] = f_->fluxSum(p, a);
+                int64_t v = 1;
+                for (int b = 0; b < d; b++) v *= f_->patchN(p)[b] + (b == a);
+                ffvol[size_t(p) * 3 + a] = v;
+            }
+        const size_t fsum_bytes = up256(sizeof(double) * rf.size() * 4 * size_t(c_->numEqn()));
+        const size_t bytes = up256(sizeof(CfItem) * cf.size()) + up256(sizeof(RefluxItem) * rf.size()) +
+                             up256(sizeof(CoarsenItem) * cs.size()) + sync.bytes() +
+                             up256(sizeof(double*) * ffacc.size()) + up256(sizeof(int64_t) * ffvol.size()) +
+                             fsum_bytes;
+        if (hipMalloc(&mem_, bytes) != hipSuccess) {
+            mem_ = nullptr;
+            return rec->fail(HM_ERR_OUT_OF_MEMORY, "hierarchy tables");
+        }
+        std::vector<char> h(bytes, 0);
+        char* dv = static_cast<char*>(mem_);
+        size_t o = 0;
+        auto put = [&](const void* src, size_t n) {
+            if (n) std::memcpy(h.data() + o, src, n);
+            const size_t at = o;
+            o += up256(n);
+            return dv + at;
+        };
+        cf_ = reinterpret_cast<const CfItem*>(put(cf.data(), sizeof(CfItem) * cf.size()));
+        rf_ = reinterpret_cast<const RefluxItem*>(put(rf.data(), sizeof(RefluxItem) * rf.size()));
+        cs_ = reinterpret_cast<const CoarsenItem*>(put(cs.data(), sizeof(CoarsenItem) * cs.size()));
+        sync_ = sync.write(h.data() + o, dv + o);
+        sync_items_ = sync.total;
+        o += sync.bytes();
+        ffacc_ = reinterpret_cast<double* const*>(put(ffacc.data(), sizeof(double*) * ffacc.size()));
+        ffvol_ = reinterpret_cast<const int64_t*>(put(ffvol.data(), sizeof(int64_t) * ffvol.size()));
+        fsum_ = reinterpret_cast<double*>(dv + o);  // written by k_fsum_add before it is read
+        o += fsum_bytes;
+        ncf_ = int64_t(cf.size());
+        nrf_ = int64_t(rf.size());
+        ncs_ = int64_t(cs.size());
+        if (hipMemcpy(mem_, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+            return rec->fail(HM_ERR_DEVICE, "hierarchy table upload");
+        return HM_OK;
+    }
+
+    Level* c_;
+    Level* f_;
+    int g_ = kGc;
+    void* mem_ = nullptr;
+    const CfItem* cf_ = nullptr;
+    const RefluxItem* rf_ = nullptr;
+    const CoarsenItem* cs_ = nullptr;
+    SegTableT<SyncPatch> sync_{};
+    double* const* ffacc_ = nullptr;
+    const int64_t* ffvol_ = nullptr;
+    double* fsum_ = nullptr;
+    int64_t ncf_ = 0, nrf_ = 0, ncs_ = 0, sync_items_ = 0;
+};
+
 }  // namespace hm
 
 struct hm_level {
     std::unique_ptr<hm::Level> lvl;
     hm_ctx* ctx;
+    int hierarchies = 0;  // live hierarchies built on this level
+};
+
+struct hm_hierarchy {
+    std::unique_ptr<hm::Hierarchy> h;
+    hm_level* coarse;
+    hm_level* fine;
 };
 
 extern "C" {
@@ -1755,8 +2413,50 @@ int hm_level_create(hm_ctx* ctx, int num_patches, const int* patch_lo, const int
     return HM_OK;
 }
 
+int hm_hierarchy_create(hm_level* coarse, hm_level* fine, int ratio, hm_hierarchy** out) {
+    if (!coarse || !fine || !out || coarse == fine) return HM_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (ratio != 2) return coarse->ctx->rec->fail(HM_ERR_UNSUPPORTED, "only refinement ratio 2 is built");
+    if (coarse->ctx != fine->ctx) return coarse->ctx->rec->fail(HM_ERR_INVALID_ARGUMENT, "levels of different contexts");
+    if (int st = coarse->ctx->rec->selectDevice()) return st;
+    hm_hierarchy* h = new (std::nothrow) hm_hierarchy();
+    if (!h) return HM_ERR_OUT_OF_MEMORY;
+    h->coarse = coarse;
+    h->fine = fine;
+    h->h.reset(new (std::nothrow) hm::Hierarchy(coarse->lvl.get(), fine->lvl.get()));
+    if (!h->h) {
+        delete h;
+        return HM_ERR_OUT_OF_MEMORY;
+    }
+    if (int st = h->h->init()) {
+        delete h;
+        return st;
+    }
+    coarse->hierarchies++;
+    fine->hierarchies++;
+    *out = h;
+    return HM_OK;
+}
+
+int hm_hierarchy_destroy(hm_hierarchy* h) {
+    if (!h) return HM_OK;
+    h->coarse->hierarchies--;
+    h->fine->hierarchies--;
+    delete h;
+    return HM_OK;
+}
+
+int hm_hierarchy_advance(hm_hierarchy* h, double dt_coarse, int num_steps, void* stream) {
+    if (!h) return HM_ERR_INVALID_ARGUMENT;
+    return h->h->advance(dt_coarse, num_steps, static_cast<hipStream_t>(stream));
+}
+
 int hm_level_destroy(hm_level* level) {
     if (!level) return HM_OK;
+    if (level->hierarchies > 0) {
+        level->ctx->rec->fail(HM_ERR_INVALID_ARGUMENT, "destroy the hierarchies of this level first");
+        return HM_ERR_INVALID_ARGUMENT;
+    }
     level->lvl.reset();
     level->ctx->levels--;
     delete level;

@@ -316,3 +316,24 @@ class Level:
             sl = tuple(slice(lo[a], lo[a] + n[a]) for a in reversed(range(self.dim)))
             lam[(slice(None),) + sl] = self.read_adjoint(i)
         return lam
+
+
+class Hierarchy:
+    """Two levels, refinement ratio 2 (hm_hierarchy_create): ``advance`` runs
+    coarse steps of dt, each with two fine steps of dt / 2, the coarse-fine
+    ghost fill and the synchronization (refluxing, then the coarse cells under
+    the fine level averaged from it)."""
+
+    def __init__(self, coarse: Level, fine: Level):
+        self.coarse, self.fine = coarse, fine  # the levels must outlive the hierarchy
+        self._h = ctypes.c_void_p()
+        capi.check(coarse.rec._ctx, capi.lib.hm_hierarchy_create(coarse._lvl, fine._lvl, 2, ctypes.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            capi.lib.hm_hierarchy_destroy(self._h)
+            self._h = None
+
+    def advance(self, dt: float, num_steps: int = 1, stream=None):
+        capi.check(self.coarse.rec._ctx, capi.lib.hm_hierarchy_advance(self._h, float(dt), int(num_steps),
+                                                                        self.coarse.rec._stream_ptr(stream)))

@@ -465,6 +465,38 @@ int hm_level_adjoint(hm_level* level, void* stream);
 int hm_level_read_adjoint(hm_level* level, int patch, double* lam);
 
 /*
+ * Two-level hierarchy (refinement ratio 2): SAMRAI's time-refined advance as
+ * RungeKuttaLevelIntegrator drives it (advanceLevel on the coarse level, twice
+ * on the fine level with half the step, then synchronizeLevelWithCoarser,
+ * RungeKuttaLevelIntegrator.cpp:2131-2225).  Before each fine step the fine
+ * level's coarse-fine ghost cells (inside the domain, under no fine patch) are
+ * filled from the coarse level, linearly interpolated in time between its
+ * states at the start and the end of the coarse step and refined by
+ * conservative linear interpolation (monotonized-central limited slopes, the
+ * CONSERVATIVE_LINEAR_REFINE operator of FlowModelSingleSpecies.cpp:578-597);
+ * after the two fine steps the coarse faces on the coarse-fine boundary take
+ * the area average of the fine flux sums (each fine step's gamma-weighted flux
+ * added to a sum that starts at zero, algs_upfluxsum2d.m4,
+ * RungeKuttaLevelIntegrator.cpp:3060-3180), the coarse update is repeated from
+ * the coarse state at the start of the step with those fluxes
+ * (Euler::synchronizeFluxes, Euler.cpp:1682-1943; five-equation model: the
+ * last volume fraction then 1 minus the others) and the coarse cells under the
+ * fine level take the average of their fine cells (CONSERVATIVE_COARSEN).
+ * Euler form only: a context with diffusive fluxes returns HM_ERR_UNSUPPORTED.
+ * Both levels must belong to one context; the fine
+ * domain is the coarse domain refined by 2, fine patch boxes start and end on
+ * coarse cells, the fine level lies inside the coarse level with room for the
+ * refinement stencil, and both levels carry the same periodic axes and
+ * physical boundary conditions.  The levels keep working on their own
+ * (hm_level_advance); destroy the hierarchy before its levels.
+ */
+typedef struct hm_hierarchy hm_hierarchy;
+int hm_hierarchy_create(hm_level* coarse, hm_level* fine, int ratio, hm_hierarchy** out);
+int hm_hierarchy_destroy(hm_hierarchy* hierarchy);
+/* num_steps coarse steps of dt_coarse (each with two fine steps of dt_coarse / 2). */
+int hm_hierarchy_advance(hm_hierarchy* hierarchy, double dt_coarse, int num_steps, void* stream);
+
+/*
  * Trajectory storage of hm_level_adjoint (no counterpart in the reference).
  * Its recompute pass can keep the reconstructed characteristic face values of
  * every stage of a checkpoint segment (2 num_eqn doubles per midpoint face and

@@ -1,0 +1,135 @@
+"""GPU two-level advance (hm_hierarchy_advance) against the oracle's
+restatement (oracle/amr.py, pinned in test_oracle_amr.py).
+
+- 2D single-species (WCNS6-LD), a fine level of two adjacent patches inside a
+  four-patch coarse level, three coarse steps; and a fine level whose two
+  patches meet across the periodic boundary.
+- 3D five-equation (WCNS5-JS and WCNS6-LD), two fine patches, one coarse step.
+Coarse and fine states: max-norm relative error <= 1e-12 (FP64; the
+restatement keeps the kernels' operation order).  The GPU result keeps the
+composite sums of the conserved variables to rounding.  Errors: misaligned
+fine boxes, a ratio other than 2, and destroying a level that a hierarchy uses.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import amr
+from oracle import oracle as O
+from hamers_amd import workloads as W
+from tests.util import max_rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a visible GPU"
+    return torch
+
+
+def _rec(dim, model="SINGLE_SPECIES", scheme="WCNS6_LD_HLLC_HLL", ns=1, gamma=(1.4,)):
+    from hamers_amd import ConvectiveFluxReconstructor
+
+    return ConvectiveFluxReconstructor("amr", dim, model, scheme, num_species=ns, species_gamma=gamma,
+                                       species_R=(1.0,) * len(gamma))
+
+
+def _levels(rec, n, dx, coarse_boxes, fine_boxes, T):
+    from hamers_amd import Level
+
+    d = len(n)
+    C = Level(rec, coarse_boxes, ((0,) * d, n), (1,) * d, dx)
+    F = Level(rec, fine_boxes, ((0,) * d, T.nf), (1,) * d, T.dxf)
+    return C, F
+
+
+def _case(torch, rec, prm, Qi, n, dx, coarse_boxes, fine_boxes, dt, nsteps, nconserved, label):
+    from hamers_amd import Hierarchy
+
+    _say(f"case: {label}")
+    T = amr.TwoLevel(prm, n, dx, fine_boxes)
+    fine0 = T.refine_patches(Qi)
+    C, F = _levels(rec, n, dx, coarse_boxes, fine_boxes, T)
+    C.write_global(Qi)
+    for i, q in enumerate(fine0):
+        F.write_patch(i, q)
+    H = Hierarchy(C, F)
+    H.advance(dt, nsteps)
+    torch.cuda.synchronize()
+    Qc = C.read_global(Qi.shape)
+    pts = [F.read_patch(i) for i in range(len(fine_boxes))]
+    del H
+    ref_c, ref_p = Qi, fine0
+    for _ in range(nsteps):
+        ref_c, ref_p = T.advance(ref_c, ref_p, dt)
+    err = max([max_rel_err(Qc, ref_c)] + [max_rel_err(a, b) for a, b in zip(pts, ref_p)])
+    same = np.array_equal(Qc, ref_c) and all(np.array_equal(a, b) for a, b in zip(pts, ref_p))
+    _say(f"  {label}: max rel err {err:.3e} (bit-identical: {same})")
+    assert err <= TOL
+    m0 = T.composite_sum(Qi, fine0, nconserved)
+    drift = np.max(np.abs(T.composite_sum(Qc, pts, nconserved) - m0) / np.abs(m0))
+    _say(f"  composite conservation drift {drift:.2e}")
+    assert drift < 1e-13
+
+
+def test_two_level_2d_single_species(torch_dev):
+    Qi, n, dx = W.density_wave_convergence(16, dim=2)
+    coarse = [((i, j), (8, 8)) for j in (0, 8) for i in (0, 8)]
+    _case(torch_dev, _rec(2), O.Params(dim=2, model=O.SINGLE_SPECIES, scheme=1), Qi, n, dx, coarse,
+          [((8, 8), (12, 16)), ((20, 8), (4, 8))], 0.3 * dx[0], 3, 4, "2D WCNS6-LD, two adjacent fine patches")
+
+
+def test_two_level_2d_across_the_periodic_boundary(torch_dev):
+    Qi, n, dx = W.density_wave_convergence(16, dim=2)
+    coarse = [((0, 0), (16, 6)), ((0, 6), (16, 10))]
+    _case(torch_dev, _rec(2, scheme="WCNS5_JS_HLLC_HLL"), O.Params(dim=2, model=O.SINGLE_SPECIES, scheme=0), Qi, n,
+          dx, coarse, [((26, 4), (6, 10)), ((0, 4), (4, 10))], 0.3 * dx[0], 2, 4,
+          "2D WCNS5-JS, fine patches meeting across the periodic boundary")
+
+
+@pytest.mark.parametrize("scheme", [0, 1], ids=["WCNS5_JS", "WCNS6_LD"])
+def test_two_level_3d_five_eqn(torch_dev, scheme):
+    Qi, n, dx = W.interface_advection_convergence(8, dim=3)
+    key = "WCNS6_LD_HLLC_HLL" if scheme == 1 else "WCNS5_JS_HLLC_HLL"
+    rec = _rec(3, "FIVE_EQN_ALLAIRE", key, 2, (1.6, 1.4))
+    prm = O.Params(dim=3, model=O.FIVE_EQN_ALLAIRE, num_species=2, scheme=scheme, gamma=[1.6, 1.4])
+    coarse = [((0, 0, 0), (8, 8, 4)), ((0, 0, 4), (8, 8, 4))]
+    _case(torch_dev, rec, prm, Qi, n, dx, coarse, [((4, 4, 2), (6, 8, 10)), ((10, 4, 2), (4, 4, 10))],
+          0.3 * dx[0], 1, prm.neq - 1, f"3D five-eqn scheme={scheme}, two fine patches")
+
+
+def test_hierarchy_errors(torch_dev):
+    from hamers_amd import Hierarchy, capi
+
+    Qi, n, dx = W.density_wave_convergence(16, dim=2)
+    rec = _rec(2)
+    prm = O.Params(dim=2, model=O.SINGLE_SPECIES, scheme=1)
+    T = amr.TwoLevel(prm, n, dx, [((8, 8), (8, 8))])
+    C, F = _levels(rec, n, dx, [((0, 0), (16, 16))], [((8, 8), (8, 8))], T)
+    h = ctypes.c_void_p()
+    assert capi.lib.hm_hierarchy_create(C._lvl, F._lvl, 3, ctypes.byref(h)) == capi.HM_ERR_UNSUPPORTED
+    # misaligned fine box (odd lower corner)
+    from hamers_amd import Level
+
+    Fb = Level(rec, [((7, 8), (8, 8))], ((0, 0), T.nf), (1, 1), T.dxf)
+    with pytest.raises(capi.HamersError):
+        Hierarchy(C, Fb)
+    # a fine domain that is not the coarse one refined by 2
+    Fw = Level(rec, [((8, 8), (8, 8))], ((0, 0), (30, 32)), (1, 1), T.dxf)
+    with pytest.raises(capi.HamersError):
+        Hierarchy(C, Fw)
+    # a level in use by a hierarchy is not destroyed
+    H = Hierarchy(C, F)
+    assert capi.lib.hm_level_destroy(F._lvl) == capi.HM_ERR_INVALID_ARGUMENT
+    del H

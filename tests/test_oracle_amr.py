@@ -1,0 +1,126 @@
+"""Pins of the oracle's two-level advance (oracle/amr.py), the restatement the
+GPU hierarchy is checked against.
+
+- Conservative linear refinement: the children of a coarse cell average to it,
+  a linear field is refined exactly, and refined values stay within the range
+  of the coarse cell and its neighbours (the MC limiter).
+- A fine level covering the whole domain steps exactly as the single-level
+  oracle RK3 on the refined grid with dt / 2 (no coarse-fine ghosts), and the
+  coarse level is then the average of it.
+- After one coarse step, coarse cells that are neither under the fine level
+  nor next to a coarse-fine face equal the single-level coarse RK3 (the
+  synchronization recomputes them from the same fluxes), while the cells next
+  to the coarse-fine faces take the refluxed correction.
+- The composite sums of the conserved variables stay constant to rounding
+  over several coarse steps on a periodic domain (2D single-species, 3D
+  five-equation), and a planted mistake -- no refluxing -- breaks that.
+"""
+import numpy as np
+import pytest
+
+from oracle import amr
+from oracle import oracle as O
+from hamers_amd import workloads as W
+from tests.util import max_rel_err, oracle_rk3
+
+G = 4
+
+
+def _two_level_2d(boxes, N=16):
+    Qi, n, dx = W.density_wave_convergence(N, dim=2)
+    prm = O.Params(dim=2, model=O.SINGLE_SPECIES, scheme=1)
+    return Qi, n, dx, prm, amr.TwoLevel(prm, n, dx, boxes)
+
+
+def test_refine_conserves_and_is_exact_on_linear_fields():
+    rng = np.random.default_rng(3)
+    n = (10, 8)
+    T = amr.TwoLevel(O.Params(dim=2, model=O.SINGLE_SPECIES, scheme=1), n, (0.1, 0.1), [((0, 0), (20, 16))])
+    # random field: children average to the parent
+    Qc = rng.uniform(0.5, 1.5, size=(4, n[1], n[0]))
+    (Qf,) = T.refine_patches(Qc)
+    mean = 0.25 * (Qf[:, 0::2, 0::2] + Qf[:, 0::2, 1::2] + Qf[:, 1::2, 0::2] + Qf[:, 1::2, 1::2])
+    assert np.max(np.abs(mean - Qc)) < 1e-15
+    # monotone: within the range of the parent and its axis neighbours
+    Cg = np.pad(Qc, [(0, 0), (1, 1), (1, 1)], mode="wrap")
+    lo = np.minimum.reduce([Cg[:, 1:-1, 1:-1], Cg[:, :-2, 1:-1], Cg[:, 2:, 1:-1], Cg[:, 1:-1, :-2], Cg[:, 1:-1, 2:]])
+    hi = np.maximum.reduce([Cg[:, 1:-1, 1:-1], Cg[:, :-2, 1:-1], Cg[:, 2:, 1:-1], Cg[:, 1:-1, :-2], Cg[:, 1:-1, 2:]])
+    for cy in range(2):
+        for cx in range(2):
+            ch = Qf[:, cy::2, cx::2]
+            assert np.all(ch >= lo - 1e-15) and np.all(ch <= hi + 1e-15)
+    # linear field (away from the periodic seam): refined exactly
+    x = np.arange(n[0]) + 0.5
+    y = np.arange(n[1]) + 0.5
+    lin = np.broadcast_to(2.0 + 0.3 * x[None, :] - 0.2 * y[:, None], (4, n[1], n[0])).copy()
+    (Qf,) = T.refine_patches(lin)
+    xf = (np.arange(2 * n[0]) + 0.5) / 2
+    yf = (np.arange(2 * n[1]) + 0.5) / 2
+    exact = 2.0 + 0.3 * xf[None, :] - 0.2 * yf[:, None]
+    inner = (slice(2, -2), slice(2, -2))
+    assert np.max(np.abs(Qf[0][inner] - exact[inner])) < 1e-14
+    # at an extremum the slope is zero: the children copy the parent
+    spike = np.ones((4, n[1], n[0]))
+    spike[:, 4, 5] = 3.0
+    (Qf,) = T.refine_patches(spike)
+    assert np.all(Qf[:, 8:10, 10:12] == 3.0)
+
+
+def test_fully_covered_fine_level_is_the_single_level_rk3():
+    Qi, n, dx, prm, T = _two_level_2d([((0, 0), (16, 32)), ((16, 0), (16, 32))])
+    fine = T.refine_patches(Qi)
+    Qf_glob = np.concatenate(fine, axis=2)
+    dt = 0.3 * dx[0]
+    Qc, pts = T.advance(Qi, fine, dt)
+    ref = oracle_rk3(prm, Qf_glob, T.nf, T.dxf, 0.5 * dt, 2)
+    assert np.array_equal(np.concatenate(pts, axis=2), ref)
+    avg = 0.25 * (ref[:, 0::2, 0::2] + ref[:, 0::2, 1::2] + ref[:, 1::2, 0::2] + ref[:, 1::2, 1::2])
+    assert max_rel_err(Qc, avg) < 1e-15
+
+
+def test_coarse_cells_away_from_the_fine_level_are_the_single_level_step():
+    Qi, n, dx, prm, T = _two_level_2d([((8, 8), (12, 10))])
+    fine = T.refine_patches(Qi)
+    dt = 0.3 * dx[0]
+    Qc, _ = T.advance(Qi, fine, dt)
+    ref = oracle_rk3(prm, Qi, n, dx, dt, 1)
+    cov = T.covered_coarse()
+    near = np.zeros_like(cov)
+    for sh in ((0, 1), (0, -1), (1, 1), (1, -1)):
+        near |= np.roll(cov, sh[1], axis=sh[0])
+    near &= ~cov
+    far = ~cov & ~near
+    scale = np.max(np.abs(ref), axis=(1, 2))[:, None]
+    assert np.max(np.abs(Qc[:, far] - ref[:, far]) / scale) < 1e-14
+    # next to the coarse-fine faces the refluxed fluxes change the update
+    assert np.max(np.abs(Qc[:, near] - ref[:, near]) / scale) > 1e-9
+
+
+@pytest.mark.parametrize("reflux", [True, False], ids=["refluxed", "planted-no-reflux"])
+def test_composite_conservation_2d(reflux):
+    Qi, n, dx, prm, T = _two_level_2d([((8, 8), (12, 16)), ((20, 8), (4, 8))])
+    fine = T.refine_patches(Qi)
+    m0 = T.composite_sum(Qi, fine, 4)
+    Qc, pts = Qi, fine
+    for _ in range(3):
+        Qc, pts = T.advance(Qc, pts, 0.3 * dx[0], reflux=reflux)
+    drift = np.max(np.abs(T.composite_sum(Qc, pts, 4) - m0) / np.abs(m0))
+    if reflux:
+        assert drift < 1e-14
+    else:
+        assert drift > 1e-5
+
+
+def test_composite_conservation_3d_five_eqn():
+    Qi, n, dx = W.interface_advection_convergence(8, dim=3)
+    prm = O.Params(dim=3, model=O.FIVE_EQN_ALLAIRE, num_species=2, scheme=1, gamma=[1.6, 1.4])
+    T = amr.TwoLevel(prm, n, dx, [((4, 4, 2), (6, 8, 10)), ((10, 4, 2), (4, 4, 10))])
+    fine = T.refine_patches(Qi)
+    nc = prm.neq - (prm.num_species - 1)  # partial densities, momentum, energy
+    m0 = T.composite_sum(Qi, fine, nc)
+    Qc, pts = T.advance(Qi, fine, 0.3 * dx[0])
+    assert np.max(np.abs(T.composite_sum(Qc, pts, nc) - m0) / np.abs(m0)) < 1e-14
+    # the last volume fraction is 1 minus the others everywhere
+    assert np.max(np.abs(Qc[-1] - (1.0 - Qc[-2]))) < 1e-15
+    for q in pts:
+        assert np.max(np.abs(q[-1] - (1.0 - q[-2]))) < 1e-15
