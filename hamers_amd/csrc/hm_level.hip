@@ -143,14 +143,16 @@ __global__ __launch_bounds__(kBlock) void k_copy_state(SegTableT<LevelPatch> w, 
 }
 
 // Zero the source of one stage (Euler.cpp:913-926); a kernel, not a memset, so
-// that the step can be captured in a graph.
-__global__ __launch_bounds__(kBlock) void k_zero_source(SegTableT<LevelPatch> w, int stage) {
+// that the step can be captured in a graph.  Only the advective equations'
+// components (from first on) are ever written by the RHS; the others stay at
+// the zero the storage was allocated with.
+__global__ __launch_bounds__(kBlock) void k_zero_source(SegTableT<LevelPatch> w, int stage, int first) {
     const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     const LevelPatch* P;
     int ax;
     int64_t loc;
     if (!w.find(gid, P, ax, loc)) return;
-    P->S[stage][loc] = 0.0;
+    P->S[stage][int64_t(first) * P->n[0] * P->n[1] * P->n[2] + loc] = 0.0;
 }
 
 
@@ -505,7 +507,8 @@ class Level {
         drop_graphs();
         hipLaunchKernelGGL(k_copy_state, dim3(blocks_for(copy_items_)), dim3(kBlock), 0, s, copy_, 0, 1, nq_);
         if (int st = launch_halo(1, s)) return st;
-        hipLaunchKernelGGL(k_zero_source, dim3(blocks_for(zero_items_)), dim3(kBlock), 0, s, zero_, 0);
+        if (zero_items_ > 0)
+            hipLaunchKernelGGL(k_zero_source, dim3(blocks_for(zero_items_)), dim3(kBlock), 0, s, zero_, 0, src_first_);
         const bool single = rec_->config().model == HM_MODEL_SINGLE_SPECIES;
         ConvectiveFluxReconstructorGPU::OwnedPlan trial;
         const Schedule dflt = rec_->defaultSchedule(views_[0].data(), np_);
@@ -1094,9 +1097,13 @@ class Level {
         TableImage<LevelPatch> copy, zero;
         copy.items = patch_;
         zero.items = patch_;
+        // advective equations: the five-equation model's volume fractions
+        const hm_config& cfg = rec_->config();
+        const int nadv = cfg.model == HM_MODEL_FIVE_EQN_ALLAIRE ? cfg.num_species - 1 : 0;
+        src_first_ = cfg.model == HM_MODEL_FIVE_EQN_ALLAIRE ? cfg.num_species + d_ + 1 : 0;
         for (int p = 0; p < np_; p++) {
             copy.add(p, 0, gvol(p));
-            zero.add(p, 0, int64_t(neq_) * ivol(p));
+            if (nadv > 0) zero.add(p, 0, int64_t(nadv) * ivol(p));
         }
         halo_host_ = halo.items;
         halo_items_ = halo.total;
@@ -1248,7 +1255,9 @@ class Level {
                 if (int st = launch_halo(1 + sn, s)) return st;
             if (record_slot >= 0)
                 if (int st = store_op(1 + sn, record_slot + sn, 0, s)) return st;
-            hipLaunchKernelGGL(k_zero_source, dim3(blocks_for(zero_items_)), dim3(kBlock), 0, s, zero_, sn);
+            if (zero_items_ > 0)
+                hipLaunchKernelGGL(k_zero_source, dim3(blocks_for(zero_items_)), dim3(kBlock), 0, s, zero_, sn,
+                                   src_first_);
             if (hipGetLastError() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "level launch failed");
             if (int st = rec_->runOwnedPlan(rhs_plan_[sn], dt, s, face_slot >= 0 ? face_states(face_slot + sn) : nullptr,
                                             dtp))
@@ -1687,6 +1696,7 @@ class Level {
     std::vector<double*> facc_, sacc_;  // per patch and axis / per patch
     RKParams rkp_[kMaxRK] = {};
     int64_t halo_items_ = 0, copy_items_ = 0, zero_items_ = 0, rk_items_ = 0;
+    int src_first_ = 0;  // first source component k_zero_source clears
     size_t rk_off_ = 0;
     hipStream_t cap_stream_ = nullptr;
     hipEvent_t ev_a_ = nullptr, ev_b_ = nullptr;

@@ -17,7 +17,9 @@ namespace hm {
 
 template <int D, bool NS, class W>
 __global__ __launch_bounds__(kBlock) void k_rk_update(W w, RKParams r) {
-    const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    // Blocks numbered per XCD: a cell's y and z face neighbours, read again by
+    // the blocks n0 and n0 n1 cells on, then sit in the same L2.
+    const int64_t gid = int64_t(xcd_block(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
     const RKPatch* P;
     int ax;
     int64_t loc;
