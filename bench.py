@@ -78,6 +78,8 @@ def parse():
     ap.add_argument("--no-per-patch", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch schedule")
     ap.add_argument("--headline-only", action="store_true", help="skip the secondary legs")
+    ap.add_argument("--leg", choices=("2d", "ns", "adjoint"), default=None,
+                    help="run only this secondary leg and print its JSON (for profiling it)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only copy this many MiB device to device 5 times (calibrates HBM byte counters) and exit")
     ap.add_argument("--N", type=int, default=2048, help="2D leg: cells per axis")
@@ -616,6 +618,15 @@ def main():
         return
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.leg is not None:
+        if args.leg == "2d":
+            res = bench_2d(args, torch, np)
+        elif args.leg == "ns":
+            res = bench_ns_3d(args, torch, np, time)
+        else:
+            res = bench_adjoint(args, torch, np, time)
+        print(json.dumps({"leg": args.leg, "result": res}), flush=True)
+        return
     L = UnevenLevel(args, torch, np)
     progress("uneven 3D level built")
     ms, wall, stage, ms_p, sched = bench_headline(args, torch, np, L)
