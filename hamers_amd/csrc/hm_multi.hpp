@@ -26,7 +26,12 @@
 #include "hm_numerics.hpp"
 #include "hm_sensor.hpp"
 
-// Waves per SIMD the multi-species midpoint-flux kernel is compiled for.
+// Waves per SIMD the multi-species midpoint-flux kernel is compiled for, in
+// 3D and in 1D/2D (measured on the uneven 3D five-equation level: 4 waves with
+// 36 B of spill beat 3 waves without by 2.5 %; the 2D kernel would spill 70 B).
+#ifndef HM_MIDFLUX_MULTI_WAVES_3D
+#define HM_MIDFLUX_MULTI_WAVES_3D 4
+#endif
 #ifndef HM_MIDFLUX_MULTI_WAVES
 #define HM_MIDFLUX_MULTI_WAVES 3
 #endif
@@ -287,7 +292,7 @@ __device__ __forceinline__ bool bounded_multi(const double* V, const KParams& k)
 template <int D, int NS, bool FOUR, int A, int SCHEME, int PP, int QQ>
 __device__ __forceinline__ void midflux_multi_at(const DevPatch& P, const Boxes<D>& B, int c0, int c1, int c2,
                                                  const KParams& k, double (&Fo)[Multi<D, NS, FOUR>::NEQ],
-                                                 double& ustar) {
+                                                 double& ustar, double* wl) {
     using M = Multi<D, NS, FOUR>;
     constexpr int NEQ = M::NEQ;
     constexpr int NZ = M::NZ;
@@ -311,8 +316,11 @@ __device__ __forceinline__ void midflux_multi_at(const DevPatch& P, const Boxes<
     // Characteristic fields (FlowModelBasicUtilitiesFiveEqnAllaire.cpp:8232-9178),
     // one at a time: a runtime loop keeps a single field's WENO interpolation
     // live, so the kernel fits the register file with room for more waves.
+    // Each field's two reconstructed values go to this thread's LDS slots wl[f],
+    // wl[NEQ + f] (stride kBlock): the field index is wave-uniform, so the store
+    // is one address add, where storing into a register array by a runtime index
+    // would select over every slot.
     const LDConst ldk{k.p, k.q, k.C, k.alpha_tau};
-    double Wm[NEQ], Wp[NEQ];
 #pragma unroll 1
     for (int f = 0; f < NEQ; f++) {
         Six s;
@@ -341,12 +349,14 @@ __device__ __forceinline__ void midflux_multi_at(const DevPatch& P, const Boxes<
         }
         double wm, wp;
         reconstruct_pair<SCHEME, PP, QQ>(s, ldk, wm, wp);
+        wl[f * kBlock] = wm;
+        wl[(NEQ + f) * kBlock] = wp;
+    }
+    double Wm[NEQ], Wp[NEQ];
 #pragma unroll
-        for (int e = 0; e < NEQ; e++)
-            if (e == f) {
-                Wm[e] = wm;
-                Wp[e] = wp;
-            }
+    for (int e = 0; e < NEQ; e++) {
+        Wm[e] = wl[e * kBlock];
+        Wp[e] = wl[(NEQ + e) * kBlock];
     }
 
     // Back-projection (FlowModelBasicUtilitiesFiveEqnAllaire.cpp:9201-9974).
@@ -543,7 +553,7 @@ __device__ __forceinline__ void midflux_multi_at(const DevPatch& P, const Boxes<
 // Midpoint face `loc` (numbered with x fastest over faces -1..n_A+1 along A)
 // of the unfused pipeline: its flux to fmid[A], u* to ustar[A].
 template <int D, int NS, bool FOUR, int A, int SCHEME, int PP, int QQ>
-__device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t loc, const KParams& k) {
+__device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t loc, const KParams& k, double* wl) {
     constexpr int NEQ = Multi<D, NS, FOUR>::NEQ;
     const Boxes<D> B(P);
     const int m0 = B.mid_ext(A, 0), m1 = B.mid_ext(A, 1);
@@ -557,7 +567,7 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
     if constexpr (A == 2) c2 -= 1;
     double Fo[NEQ];
     double ustar;
-    midflux_multi_at<D, NS, FOUR, A, SCHEME, PP, QQ>(P, B, c0, c1, c2, k, Fo, ustar);
+    midflux_multi_at<D, NS, FOUR, A, SCHEME, PP, QQ>(P, B, c0, c1, c2, k, Fo, ustar, wl);
     const int64_t mv = B.mid_vol(A);
     double* out = P.fmid[A];
     const uint32_t l32 = uint32_t(loc);
@@ -569,20 +579,24 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
 // Midpoint-flux stage: the midpoint faces of one axis per segment, fluxes to
 // memory (face flux in stage 3).
 template <int D, int NS, bool FOUR, int SCHEME, int PP, int QQ, class W>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HM_MIDFLUX_MULTI_WAVES))) void k_midflux_multi(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
+    D == 3 ? HM_MIDFLUX_MULTI_WAVES_3D : HM_MIDFLUX_MULTI_WAVES))) void k_midflux_multi(
     W w, KParams k) {
+    // per thread 2 NEQ reconstructed characteristic values (one face at a time)
+    __shared__ double wlds[2 * Multi<D, NS, FOUR>::NEQ * kBlock];
     const int64_t gid = int64_t(rhs_block(k)) * kBlock + threadIdx.x;
     const DevPatch* P;
     int ax;
     int64_t loc;
     if (!w.find(gid, P, ax, loc)) return;
+    double* wl = wlds + threadIdx.x;
     if (ax == 0) {
-        midflux_multi_face<D, NS, FOUR, 0, SCHEME, PP, QQ>(*P, loc, k);
+        midflux_multi_face<D, NS, FOUR, 0, SCHEME, PP, QQ>(*P, loc, k, wl);
     } else if constexpr (D > 1) {
         if (ax == 1) {
-            midflux_multi_face<D, NS, FOUR, 1, SCHEME, PP, QQ>(*P, loc, k);
+            midflux_multi_face<D, NS, FOUR, 1, SCHEME, PP, QQ>(*P, loc, k, wl);
         } else if constexpr (D > 2) {
-            midflux_multi_face<D, NS, FOUR, 2, SCHEME, PP, QQ>(*P, loc, k);
+            midflux_multi_face<D, NS, FOUR, 2, SCHEME, PP, QQ>(*P, loc, k, wl);
         }
     }
 }
