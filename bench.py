@@ -78,7 +78,7 @@ def parse():
     ap.add_argument("--no-per-patch", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch schedule")
     ap.add_argument("--headline-only", action="store_true", help="skip the secondary legs")
-    ap.add_argument("--leg", choices=("2d", "ns", "adjoint"), default=None,
+    ap.add_argument("--leg", choices=("2d", "ns", "adjoint", "two_level"), default=None,
                     help="run only this secondary leg and print its JSON (for profiling it)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only copy this many MiB device to device 5 times (calibrates HBM byte counters) and exit")
@@ -533,6 +533,78 @@ def bench_ns_3d(args, torch, np, time):
     }
 
 
+def bench_two_level(args, torch, np):
+    """Two-level advance (refinement ratio 2) of the shock / perturbed-interface
+    field: a coarse level of 32^3 patches over the whole periodic domain and a
+    fine level of uneven patches over a slab around the interface (full in y
+    and z, so its coarse-fine boundary is two x-planes).  Times coarse steps of
+    the hierarchy (a coarse SSP-RK3 step, two fine steps of dt / 2, coarse-fine
+    fill, refluxing and coarsening) against the two levels' own steps."""
+    from hamers_amd import ConvectiveFluxReconstructor, Hierarchy, Level
+
+    rng = np.random.default_rng(77)
+    Nc = (224, 192, 192)
+    Nf = tuple(2 * v for v in Nc)
+    dxc = tuple(1.0 / v for v in Nc)
+    dxf = tuple(1.0 / v for v in Nf)
+    modes = [(rng.uniform(0.0, 1.0), rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi)) for _ in range(7)]
+    tot = sum(m[0] for m in modes)
+    modes = [(m[0] / tot, m[1], m[2]) for m in modes]
+    rec = ConvectiveFluxReconstructor("two-level", 3, "FIVE_EQN_ALLAIRE", "WCNS6_LD_HLLC_HLL", num_species=2,
+                                      species_gamma=(1.4, 1.67), species_R=(1.0, 1.0))
+    cboxes = [((i, j, k), (32, 32, 32)) for k in range(0, Nc[2], 32) for j in range(0, Nc[1], 32)
+              for i in range(0, Nc[0], 32)]
+
+    def even_segments(total):
+        out = []
+        while sum(out) < total:
+            out.append(int(min(rng.choice((16, 32, 48, 64)), total - sum(out))))
+        return out
+
+    xs = [224, 64, 64]  # fine x: [224, 352), two patches of 64 (the interface at x = 0.6 is fine cell 269)
+    ys, zs = even_segments(Nf[1]), even_segments(Nf[2])
+    fboxes = []
+    zo = 0
+    for nz in zs:
+        yo = 0
+        for ny in ys:
+            xo = xs[0]
+            for nx in xs[1:]:
+                fboxes.append(((xo, yo, zo), (nx, ny, nz)))
+                xo += nx
+            yo += ny
+        zo += nz
+    C = Level(rec, cboxes, ((0, 0, 0), Nc), (1, 1, 1), dxc)
+    F = Level(rec, fboxes, ((0, 0, 0), Nf), (1, 1, 1), dxf)
+    for lvl, boxes, dx in ((C, cboxes, dxc), (F, fboxes, dxf)):
+        for i, (lo, n) in enumerate(boxes):
+            Q = shock_interface_patch_torch(torch, lo, n, 0, dx, modes)
+            lvl.write_patch(i, Q.cpu().numpy())
+    torch.cuda.synchronize()
+    H = Hierarchy(C, F)
+    dt = 4.0e-5  # 0.4 / 224: the CFL of the headline level's steps on the coarse grid
+    stream = torch.cuda.current_stream()
+    ms_h, _ = timed_events(torch, stream, lambda: H.advance(dt, 1), 2, 1)
+    ms_c, _ = timed_events(torch, stream, lambda: C.advance(dt, 1), 2, 1)
+    ms_f, _ = timed_events(torch, stream, lambda: F.advance(0.5 * dt, 2), 2, 1)
+    finite = all(bool(np.isfinite(F.read_patch(i)).all()) for i in range(0, len(fboxes), 17))
+    del H
+    cc = int(np.prod(Nc))
+    fc = int(sum(n[0] * n[1] * n[2] for _, n in fboxes))
+    return {
+        "workload": "3D five-equation Allaire, Mach-1.5 shock and perturbed interface, WCNS6_LD_HLLC_HLL; coarse "
+                    f"{Nc[0]}x{Nc[1]}x{Nc[2]} in 32^3 patches, fine level (ratio 2) of uneven patches over the slab "
+                    "x in [0.5, 0.786) around the interface, periodic",
+        "coarse_patches": len(cboxes), "coarse_cells": cc, "fine_patches": len(fboxes), "fine_cells": fc,
+        "dt_coarse": dt,
+        "ms_per_coarse_step": ms_h,
+        "ms_coarse_level_step": ms_c, "ms_two_fine_level_steps": ms_f,
+        "ms_coarse_fine_overhead": ms_h - ms_c - ms_f,
+        "cell_updates_per_s": (cc + 2 * fc) / (ms_h * 1e-3),
+        "finite": finite,
+    }
+
+
 def bench_adjoint(args, torch, np, time):
     """Discrete adjoint of SSP-RK3 on a Kelvin-Helmholtz level: 512^2 in 16
     patches of 128^2, WCNS6-LD + hybrid HLLC/HLLC-HLL, 200 steps, checkpoints
@@ -623,6 +695,8 @@ def main():
             res = bench_2d(args, torch, np)
         elif args.leg == "ns":
             res = bench_ns_3d(args, torch, np, time)
+        elif args.leg == "two_level":
+            res = bench_two_level(args, torch, np)
         else:
             res = bench_adjoint(args, torch, np, time)
         print(json.dumps({"leg": args.leg, "result": res}), flush=True)
@@ -640,7 +714,7 @@ def main():
     del L
     torch.cuda.empty_cache()
 
-    rhs2d = ns3d = adjoint = None
+    rhs2d = ns3d = adjoint = two_level = None
     if not args.headline_only:
         rhs2d = bench_2d(args, torch, np)
         progress("2D leg done")
@@ -652,6 +726,9 @@ def main():
         if args.adjoint_steps > 0:
             adjoint = bench_adjoint(args, torch, np, time)
             progress("adjoint leg done")
+            torch.cuda.empty_cache()
+        two_level = bench_two_level(args, torch, np)
+        progress("two-level leg done")
 
     value = config["cells"] / (ms * 1e-3)
     out = {
@@ -680,6 +757,7 @@ def main():
         "rhs_2d": rhs2d,
         "navier_stokes_3d": ns3d,
         "adjoint": adjoint,
+        "two_level": two_level,
     }
     print(json.dumps(out), flush=True)
 
