@@ -47,11 +47,11 @@ FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix)
 HEADLINE_KERNEL_COUNTS = {
     "k_midflux_multi": {
         "cells": 68726784,
-        # 64 x (3 808 445 963 + 5 664 677 342 + 423 476 106 + 2 x 3 067 398 558)
-        "fp64_ops": 64 * (3808445963 + 5664677342 + 423476106 + 2 * 3067398558),
-        # FETCH_SIZE 20 998 064 KiB x 2.0 + WRITE_SIZE 14 210 952 KiB x 1.0
-        "dram_bytes": 20998064 * 1024 * 2 + 14210952 * 1024,
-        "valu_issue_frac": 0.827,
+        # 64 x (3 796 687 379 + 5 664 677 342 + 423 476 106 + 2 x 3 067 398 558)
+        "fp64_ops": 64 * (3796687379 + 5664677342 + 423476106 + 2 * 3067398558),
+        # FETCH_SIZE 24 383 300 KiB x 2.0 + WRITE_SIZE 17 778 258 KiB x 1.0 (includes the 36 B/lane spill)
+        "dram_bytes": 24383300 * 1024 * 2 + 17778258 * 1024,
+        "valu_issue_frac": 0.843,
         "source": "profiles/rhs_uneven3d_five_eqn_counters.md",
     },
 }
