@@ -63,11 +63,15 @@ def _divergence(F, S, dx):
 
 
 class TwoLevel:
-    """prm: oracle Params; coarse domain n (x first), periodic on every axis;
-    fine patch boxes (lo, n) in the refined index space (even lo and n)."""
+    """prm: oracle Params; coarse domain n (x first); fine patch boxes (lo, n)
+    in the refined index space (even lo and n).  bc = (periodic per axis, side
+    conditions, DIRICHLET values) as oracle.fill_boundary takes them; None:
+    periodic on every axis."""
 
-    def __init__(self, prm, n, dx, fine_boxes, g=4):
+    def __init__(self, prm, n, dx, fine_boxes, g=4, bc=None):
         self.prm, self.d, self.g = prm, len(n), g
+        self.bc = bc
+        self.mom0 = 1 if prm.model == O.SINGLE_SPECIES else prm.num_species
         self.n = tuple(n)
         self.dx = tuple(dx)
         self.nf = tuple(2 * v for v in n)
@@ -88,23 +92,30 @@ class TwoLevel:
         """G[(comp,) + wrapped indices] for index lists per axis (x first)."""
         return G[(slice(None),) + np.ix_(*[idx_per_axis[a] for a in reversed(range(self.d))])]
 
-    # ---- the fine level's ghost fill --------------------------------------
-    def fine_ghosted(self, Fglob, patch_q, p, Cold_g, Cnew_g, alpha):
-        """Ghost box of fine patch p: interior from its own data, same-level
-        ghosts (wrapped) from the fine level, coarse-fine ghosts refined from
-        the coarse level (ghosted arrays, ghost width g)."""
+    # ---- ghost fills ------------------------------------------------------
+    def fill(self, Q):
+        """Ghosted array of a whole-level array: periodic images, or the
+        physical boundary conditions (oracle.fill_boundary)."""
         g, d = self.g, self.d
-        lo, m = self.boxes[p]
-        xs = [np.arange(lo[a] - g, lo[a] + m[a] + g) for a in range(d)]
-        xw = [np.mod(xs[a], self.nf[a]) for a in range(d)]
-        Q = self._gather(Fglob, xw)
-        cov = self.cover[np.ix_(*[xw[a] for a in reversed(range(d))])]
-        Q[(slice(None),) + self._sl([g] * d, m)] = patch_q
-        cf = cov < 0
-        if np.any(cf):
-            R = self.refine(Cold_g, Cnew_g, alpha, xw)
-            Q[:, cf] = R[:, cf]
-        return Q
+        if self.bc is None:
+            return np.pad(Q, [(0, 0)] + [(g, g)] * d, mode="wrap")
+        periodic, conds, dirichlet = self.bc
+        return O.fill_boundary(Q, g, periodic, conds, dirichlet, self.mom0)
+
+    def fine_ghosted_all(self, pts, CF):
+        """Ghost boxes of every fine patch: interiors from the patches, cells
+        under no fine patch from CF (the coarse-fine values of this fine step,
+        on the whole fine domain), then the level's periodic images or physical
+        boundary conditions.  A cell inside the domain has one value in every
+        ghost box that holds it (same-level ghosts copy interiors, coarse-fine
+        ghosts depend on the position only), so one whole-domain array gives
+        them all; the boundary fill reads only cells inside the domain."""
+        g, d = self.g, self.d
+        G = CF.copy()
+        for (lo, m), q in zip(self.boxes, pts):
+            G[(slice(None),) + self._sl(lo, m)] = q
+        Gg = self.fill(G)
+        return [Gg[(slice(None),) + self._sl(lo, [v + 2 * g for v in m])].copy() for lo, m in self.boxes]
 
     def refine(self, Cold_g, Cnew_g, alpha, xw):
         """Refined values at the fine cells xw (wrapped fine indices per axis)
@@ -131,7 +142,7 @@ class TwoLevel:
     def refine_patches(self, Qc):
         """Fine patch interiors refined from the coarse state (the fine level's
         initial data in the tests)."""
-        Cg = np.pad(Qc, [(0, 0)] + [(self.g, self.g)] * self.d, mode="wrap")
+        Cg = self.fill(Qc)
         return [self.refine(Cg, Cg, 0.0, [np.arange(lo[a], lo[a] + m[a]) for a in range(self.d)])
                 for lo, m in self.boxes]
 
@@ -195,13 +206,14 @@ class TwoLevel:
         """One coarse step (with two fine steps): Qc (nq, coarse domain),
         Qf_patches: interior arrays of the fine patches.  Returns the new pair."""
         g, d = self.g, self.d
-        fill_c = lambda Q: np.pad(Q, [(0, 0)] + [(g, g)] * d, mode="wrap")  # noqa: E731
+        fill_c = self.fill
         Cold_g = fill_c(Qc)
         facc_c = ([np.zeros(O.face_shape(self.neq, self.n, a)) for a in range(d)],
                   np.zeros((self.neq,) + tuple(reversed(self.n))))
 
         def fill_coarse(sn, pts, prev):
-            # one periodic patch: the same-level fill sets every ghost cell
+            # the whole level as one patch: periodic images or the boundary
+            # conditions set every ghost cell
             return [fill_c(pts[0])]
 
         Qc_new = self._step_level([Qc], dt, self.dx, fill_coarse, [facc_c])[0]
@@ -213,33 +225,12 @@ class TwoLevel:
                        np.zeros((self.neq,) + tuple(reversed(m)))) for _, m in self.boxes]
             alpha = 0.5 * k
 
-            def glob(ps):
-                Gf = np.zeros((self.nq,) + tuple(reversed(self.nf)))
-                for (lo, m), q in zip(self.boxes, ps):
-                    Gf[(slice(None),) + self._sl(lo, m)] = q
-                return Gf
-
-            stage0 = {}
+            # the coarse-fine values of this fine step on the whole fine domain,
+            # kept through its intermediate stages
+            CF = self.refine(Cold_g, Cnew_g, alpha, [np.arange(self.nf[a]) for a in range(d)])
 
             def fill_fine(sn, ps, prev):
-                Gf = glob(ps)
-                out = []
-                for p, q in enumerate(ps):
-                    if sn == 0:
-                        out.append(self.fine_ghosted(Gf, q, p, Cold_g, Cnew_g, alpha))
-                        stage0[p] = out[-1]
-                    else:
-                        # the scratch ghost box (stage-0 fill) with the new interior,
-                        # then the same-level ghosts refilled
-                        Q = stage0[p].copy()
-                        lo, m = self.boxes[p]
-                        xs = [np.mod(np.arange(lo[a] - g, lo[a] + m[a] + g), self.nf[a]) for a in range(d)]
-                        cov = self.cover[np.ix_(*[xs[a] for a in reversed(range(d))])]
-                        G = self._gather(Gf, xs)
-                        Q[:, cov >= 0] = G[:, cov >= 0]
-                        Q[(slice(None),) + self._sl([g] * d, m)] = q
-                        out.append(Q)
-                return out
+                return self.fine_ghosted_all(ps, CF)
 
             pts = self._step_level(pts, 0.5 * dt, self.dxf, fill_fine, facc_f)
             if fsum is None:
@@ -279,6 +270,8 @@ class TwoLevel:
                 R = list(fidx)
                 L = list(fidx)
                 L[a] -= 1
+                if self.bc is not None and not self.bc[0][a] and (L[a] < 0 or R[a] >= self.n[a]):
+                    continue  # a face on the physical boundary
                 Rw = [R[b] % self.n[b] for b in range(d)]
                 Lw = [L[b] % self.n[b] for b in range(d)]
                 cR = covc[tuple(reversed(Rw))]

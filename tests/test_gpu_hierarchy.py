@@ -5,6 +5,9 @@ restatement (oracle/amr.py, pinned in test_oracle_amr.py).
   four-patch coarse level, three coarse steps; and a fine level whose two
   patches meet across the periodic boundary.
 - 3D five-equation (WCNS5-JS and WCNS6-LD), two fine patches, one coarse step.
+- Physical boundaries: 2D single-species with REFLECT and FLOW x sides and a
+  fine patch against the wall, and 3D five-equation with SYMMETRY and DIRICHLET
+  z sides, against the restatement with the same conditions.
 Coarse and fine states: max-norm relative error <= 1e-12 (FP64; the
 restatement keeps the kernels' operation order).  The GPU result keeps the
 composite sums of the conserved variables to rounding.  Errors: misaligned
@@ -45,22 +48,26 @@ def _rec(dim, model="SINGLE_SPECIES", scheme="WCNS6_LD_HLLC_HLL", ns=1, gamma=(1
                                        species_R=(1.0,) * len(gamma))
 
 
-def _levels(rec, n, dx, coarse_boxes, fine_boxes, T):
+def _levels(rec, n, dx, coarse_boxes, fine_boxes, T, bc=None):
     from hamers_amd import Level
 
     d = len(n)
-    C = Level(rec, coarse_boxes, ((0,) * d, n), (1,) * d, dx)
-    F = Level(rec, fine_boxes, ((0,) * d, T.nf), (1,) * d, T.dxf)
+    periodic = (1,) * d if bc is None else tuple(bc[0])
+    C = Level(rec, coarse_boxes, ((0,) * d, n), periodic, dx)
+    F = Level(rec, fine_boxes, ((0,) * d, T.nf), periodic, T.dxf)
+    if bc is not None:
+        for lvl in (C, F):
+            lvl.set_boundary_conditions(bc[1], bc[2])
     return C, F
 
 
-def _case(torch, rec, prm, Qi, n, dx, coarse_boxes, fine_boxes, dt, nsteps, nconserved, label):
+def _case(torch, rec, prm, Qi, n, dx, coarse_boxes, fine_boxes, dt, nsteps, nconserved, label, bc=None):
     from hamers_amd import Hierarchy
 
     _say(f"case: {label}")
-    T = amr.TwoLevel(prm, n, dx, fine_boxes)
+    T = amr.TwoLevel(prm, n, dx, fine_boxes, bc=bc)
     fine0 = T.refine_patches(Qi)
-    C, F = _levels(rec, n, dx, coarse_boxes, fine_boxes, T)
+    C, F = _levels(rec, n, dx, coarse_boxes, fine_boxes, T, bc)
     C.write_global(Qi)
     for i, q in enumerate(fine0):
         F.write_patch(i, q)
@@ -77,6 +84,8 @@ def _case(torch, rec, prm, Qi, n, dx, coarse_boxes, fine_boxes, dt, nsteps, ncon
     same = np.array_equal(Qc, ref_c) and all(np.array_equal(a, b) for a, b in zip(pts, ref_p))
     _say(f"  {label}: max rel err {err:.3e} (bit-identical: {same})")
     assert err <= TOL
+    if nconserved == 0:
+        return
     m0 = T.composite_sum(Qi, fine0, nconserved)
     drift = np.max(np.abs(T.composite_sum(Qc, pts, nconserved) - m0) / np.abs(m0))
     _say(f"  composite conservation drift {drift:.2e}")
@@ -107,6 +116,28 @@ def test_two_level_3d_five_eqn(torch_dev, scheme):
     coarse = [((0, 0, 0), (8, 8, 4)), ((0, 0, 4), (8, 8, 4))]
     _case(torch_dev, rec, prm, Qi, n, dx, coarse, [((4, 4, 2), (6, 8, 10)), ((10, 4, 2), (4, 4, 10))],
           0.3 * dx[0], 1, prm.neq - 1, f"3D five-eqn scheme={scheme}, two fine patches")
+
+
+def test_two_level_2d_walls(torch_dev):
+    from tests.test_oracle_amr import _reflect_x_case
+
+    Qi, n, dx, prm = _reflect_x_case()
+    bc = ((0, 1), [O.BC_REFLECT, O.BC_FLOW, 0, 0], None)
+    coarse = [((0, 0), (8, 16)), ((8, 0), (8, 16))]
+    _case(torch_dev, _rec(2), prm, Qi, n, dx, coarse, [((0, 8), (8, 12)), ((12, 2), (10, 10)), ((26, 0), (6, 32))],
+          0.2 * dx[0], 2, 0, "2D WCNS6-LD, REFLECT / FLOW x sides, fine patches against both", bc=bc)
+
+
+def test_two_level_3d_five_eqn_walls(torch_dev):
+    Qi, n, dx = W.interface_advection_convergence(8, dim=3)
+    rec = _rec(3, "FIVE_EQN_ALLAIRE", "WCNS6_LD_HLLC_HLL", 2, (1.6, 1.4))
+    prm = O.Params(dim=3, model=O.FIVE_EQN_ALLAIRE, num_species=2, scheme=1, gamma=[1.6, 1.4])
+    dirichlet = np.zeros((6, prm.nq))
+    dirichlet[5] = Qi[:, -1, 0, 0]  # z-hi: the state of one boundary cell
+    bc = ((1, 1, 0), [0, 0, 0, 0, O.BC_SYMMETRY, O.BC_DIRICHLET], dirichlet)
+    coarse = [((0, 0, 0), (8, 8, 4)), ((0, 0, 4), (8, 8, 4))]
+    _case(torch_dev, rec, prm, Qi, n, dx, coarse, [((4, 4, 0), (6, 8, 10)), ((10, 4, 8), (4, 4, 8))],
+          0.3 * dx[0], 1, 0, "3D five-eqn, SYMMETRY / DIRICHLET z sides, fine patches against both", bc=bc)
 
 
 def test_hierarchy_errors(torch_dev):
