@@ -85,10 +85,12 @@ class hm_rk_patch_view(ctypes.Structure):
 
 
 class hm_schedule(ctypes.Structure):
-    _fields_ = [("tile_rows", ctypes.c_int), ("xcd_remap", ctypes.c_int), ("order", ctypes.c_int)]
+    _fields_ = [("tile_rows", ctypes.c_int), ("xcd_remap", ctypes.c_int), ("order", ctypes.c_int),
+                ("persistent", ctypes.c_int)]
 
     def as_dict(self):
-        return {"tile_rows": self.tile_rows, "xcd_remap": self.xcd_remap, "order": self.order}
+        return {"tile_rows": self.tile_rows, "xcd_remap": self.xcd_remap, "order": self.order,
+                "persistent": self.persistent}
 
 
 class hm_transport(ctypes.Structure):

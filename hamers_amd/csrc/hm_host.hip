@@ -71,6 +71,7 @@ hm::Schedule to_sched(const hm_schedule& c) {
     s.tile_rows = c.tile_rows;
     s.xcd = c.xcd_remap;
     s.order = c.order;
+    s.persistent = c.persistent;
     return s;
 }
 hm_schedule from_sched(const hm::Schedule& s) {
@@ -78,6 +79,7 @@ hm_schedule from_sched(const hm::Schedule& s) {
     c.tile_rows = s.tile_rows;
     c.xcd_remap = s.xcd;
     c.order = s.order;
+    c.persistent = s.persistent;
     return c;
 }
 }  // namespace

@@ -42,6 +42,9 @@ struct KParams {
     // contiguous range of block numbers (xcd_block), 0 keeps the hardware's
     // round-robin order.
     int xcd;
+    // Work-queue counter of the persistent midpoint-flux kernel (one unsigned
+    // in device memory, cleared in-stream before each launch); NULL otherwise.
+    unsigned* queue;
 };
 
 // Block number a RHS kernel works on.
@@ -58,6 +61,7 @@ struct Schedule {
     int tile_rows = 24;
     int xcd = 0;
     int order = 0;
+    int persistent = 0;  // multi-species midpoint flux as a persistent work-queue kernel
 };
 
 // The time step a kernel scales its fluxes with.
@@ -120,6 +124,8 @@ struct Plan {
     int64_t items[kStages];
     int tile_rows;  // rows of the tiled flux stage (single-species model): 4, 8 or 24
     int xcd;        // KParams::xcd of the plan's launches
+    int persistent; // multi-species midpoint flux from a work queue (KParams::queue)
+    unsigned* queue;
 };
 
 template <class W>

@@ -532,10 +532,7 @@ class Level {
     }
 
     int setSchedule(const Schedule& sc) {
-        if (sc.tile_rows != 4 && sc.tile_rows != 8 && sc.tile_rows != 24)
-            return rec_->fail(HM_ERR_INVALID_ARGUMENT, "tile rows must be 4, 8 or 24");
-        if ((sc.xcd != 0 && sc.xcd != 1) || (sc.order != 0 && sc.order != 1))
-            return rec_->fail(HM_ERR_INVALID_ARGUMENT, "xcd and order are 0 or 1");
+        if (*rec_->scheduleError(sc)) return rec_->fail(HM_ERR_INVALID_ARGUMENT, rec_->scheduleError(sc));
         if (hipDeviceSynchronize() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "device synchronize failed");
         drop_graphs();
         sched_ = sc;
@@ -2509,6 +2506,7 @@ int hm_level_autotune(hm_level* level, double dt, int repetitions, hm_schedule* 
         chosen->tile_rows = best.tile_rows;
         chosen->xcd_remap = best.xcd;
         chosen->order = best.order;
+        chosen->persistent = best.persistent;
     }
     if (ms) {
         ms[0] = best_ms;
@@ -2524,6 +2522,7 @@ int hm_level_set_schedule(hm_level* level, const hm_schedule* schedule) {
     sc.tile_rows = schedule->tile_rows;
     sc.xcd = schedule->xcd_remap;
     sc.order = schedule->order;
+    sc.persistent = schedule->persistent;
     return level->lvl->setSchedule(sc);
 }
 
@@ -2533,6 +2532,7 @@ int hm_level_get_schedule(hm_level* level, hm_schedule* schedule) {
     schedule->tile_rows = sc.tile_rows;
     schedule->xcd_remap = sc.xcd;
     schedule->order = sc.order;
+    schedule->persistent = sc.persistent;
     return HM_OK;
 }
 

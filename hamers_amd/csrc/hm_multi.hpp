@@ -22,6 +22,8 @@
 // k_source_five.
 #pragma once
 
+#include <algorithm>
+
 #include "hm_kernels.hpp"
 #include "hm_numerics.hpp"
 #include "hm_sensor.hpp"
@@ -601,6 +603,49 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
     }
 }
 
+// The midpoint-flux stage as a persistent kernel (Schedule::persistent): a
+// grid of as many blocks as the device holds at once takes the stage's
+// blocks of work in order from a queue (KParams::queue, one atomic fetch per
+// block of work, cleared by k_queue_reset just before the launch), and every
+// block leaves once its fetch is past the last block of work.  Each block of
+// work is the same 256 items as the one-block-per-block launch computes.
+template <int D, int NS, bool FOUR, int SCHEME, int PP, int QQ, class W>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HM_MIDFLUX_MULTI_WAVES))) void
+k_midflux_multi_queue(W w, KParams k, uint32_t nwork) {
+    __shared__ double wlds[2 * Multi<D, NS, FOUR>::NEQ * kBlock];
+    __shared__ uint32_t ticket;
+    double* wl = wlds + threadIdx.x;
+    for (;;) {
+        if (threadIdx.x == 0) ticket = atomicAdd(k.queue, 1u);
+        __syncthreads();
+        const uint32_t b = ticket;
+        __syncthreads();  // every thread has read the ticket before the next fetch
+        if (b >= nwork) return;  // the same for the whole block
+        const int64_t gid = int64_t(b) * kBlock + threadIdx.x;
+        const DevPatch* P;
+        int ax;
+        int64_t loc;
+        if (w.find(gid, P, ax, loc)) {
+            if (ax == 0) {
+                midflux_multi_face<D, NS, FOUR, 0, SCHEME, PP, QQ>(*P, loc, k, wl);
+            } else if constexpr (D > 1) {
+                if (ax == 1) {
+                    midflux_multi_face<D, NS, FOUR, 1, SCHEME, PP, QQ>(*P, loc, k, wl);
+                } else if constexpr (D > 2) {
+                    midflux_multi_face<D, NS, FOUR, 2, SCHEME, PP, QQ>(*P, loc, k, wl);
+                }
+            }
+        }
+    }
+}
+
+namespace {
+// Clears the work-queue counter (a kernel, so that a captured step holds no memset).
+__global__ void k_queue_reset(unsigned* queue) {
+    if (threadIdx.x == 0) *queue = 0u;
+}
+}  // namespace
+
 // ----------------------------------------------------------------------------
 // Midpoint-to-face flux with the model's node flux
 // (FlowModelFiveEqnAllaire.cpp:4877-5901, FlowModelFourEqnConservative.cpp:5499-6338).
@@ -745,9 +790,27 @@ hipError_t launch_multi_dsq(const Plan<W>& plan, const KParams& k, hipStream_t s
         hipLaunchKernelGGL((k_derived_multi<D, NS, FOUR, W>), dim3(nblocks(plan, 0)), dim3(kBlock), 0, s,
                            plan.work[0], k);
     // (stage 1 is empty: the derived-data kernel computes the dilatation and vorticity)
-    if (plan.items[2] > 0)
-        hipLaunchKernelGGL((k_midflux_multi<D, NS, FOUR, SCHEME, PP, QQ, W>), dim3(nblocks(plan, 2)), dim3(kBlock),
-                           0, s, plan.work[2], k);
+    if (plan.items[2] > 0) {
+        if (plan.persistent && k.queue) {
+            // resident blocks of this kernel on the device (queried once per instance)
+            static const unsigned resident = [] {
+                int dev = 0, cus = 0, per_cu = 0;
+                (void)hipGetDevice(&dev);
+                (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+                (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per_cu, reinterpret_cast<const void*>(&k_midflux_multi_queue<D, NS, FOUR, SCHEME, PP, QQ, W>),
+                    kBlock, 0);
+                return unsigned(std::max(1, cus) * std::max(1, per_cu));
+            }();
+            const unsigned nwork = nblocks(plan, 2);
+            hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, s, k.queue);
+            hipLaunchKernelGGL((k_midflux_multi_queue<D, NS, FOUR, SCHEME, PP, QQ, W>),
+                               dim3(std::min(nwork, resident)), dim3(kBlock), 0, s, plan.work[2], k, uint32_t(nwork));
+        } else {
+            hipLaunchKernelGGL((k_midflux_multi<D, NS, FOUR, SCHEME, PP, QQ, W>), dim3(nblocks(plan, 2)),
+                               dim3(kBlock), 0, s, plan.work[2], k);
+        }
+    }
     if (plan.items[3] > 0)
         hipLaunchKernelGGL((k_faceflux_multi<D, NS, FOUR, W>), dim3(nblocks(plan, 3)), dim3(kBlock), 0, s,
                            plan.work[3], k);

@@ -120,6 +120,7 @@ class ConvectiveFluxReconstructorGPU {
         k_.lam_scale = 0.0;
         k_.qoff = 0;
         k_.xcd = 0;
+        k_.queue = nullptr;
         has_advective_ = (cfg.model == HM_MODEL_FIVE_EQN_ALLAIRE && cfg.num_species > 1);
         fused_ = (cfg.model == HM_MODEL_SINGLE_SPECIES);
         plans_.reserve(kMaxPlans);  // plan addresses stay valid while cached
@@ -135,6 +136,7 @@ class ConvectiveFluxReconstructorGPU {
         if (arena_) (void)hipFree(arena_);
         if (stage_buf_) (void)hipFree(stage_buf_);
         if (stage_ev_) (void)hipEventDestroy(stage_ev_);
+        if (queue_) (void)hipFree(queue_);
         if (prev >= 0) (void)hipSetDevice(prev);
     }
 
@@ -254,14 +256,26 @@ class ConvectiveFluxReconstructorGPU {
         const int rows[3] = {4, 8, 24};
         for (int r = 0; r < (fused_ && cfg_.dim > 1 ? 3 : 1); r++)
             for (int x = 0; x < 2; x++)
-                for (int o = 0; o < 2; o++) {
-                    Schedule sc;
-                    sc.tile_rows = (fused_ && cfg_.dim > 1) ? rows[r] : 4;
-                    sc.xcd = x;
-                    sc.order = o;
-                    c.push_back(sc);
-                }
+                for (int o = 0; o < 2; o++)
+                    for (int p = 0; p < (fused_ ? 1 : 2); p++) {
+                        Schedule sc;
+                        sc.tile_rows = (fused_ && cfg_.dim > 1) ? rows[r] : 4;
+                        sc.xcd = x;
+                        sc.order = o;
+                        sc.persistent = p;
+                        c.push_back(sc);
+                    }
         return c;
+    }
+
+    // Whether sc is a schedule of this model ("" if so, else the reason).
+    const char* scheduleError(const Schedule& sc) const {
+        if (sc.tile_rows != 4 && sc.tile_rows != 8 && sc.tile_rows != 24) return "tile rows must be 4, 8 or 24";
+        if ((sc.xcd != 0 && sc.xcd != 1) || (sc.order != 0 && sc.order != 1) ||
+            (sc.persistent != 0 && sc.persistent != 1))
+            return "xcd, order and persistent are 0 or 1";
+        if (fused_ && sc.persistent) return "the persistent midpoint flux is a schedule of the multi-species models";
+        return "";
     }
 
     // Times `run` (one batched RHS of a plan) for each candidate schedule,
@@ -291,7 +305,9 @@ class ConvectiveFluxReconstructorGPU {
             if (st) break;
             std::sort(t.begin(), t.end());
             const double ms = t[t.size() / 2];
-            if (sc.tile_rows == dflt.tile_rows && sc.xcd == dflt.xcd && sc.order == dflt.order) default_ms = ms;
+            if (sc.tile_rows == dflt.tile_rows && sc.xcd == dflt.xcd && sc.order == dflt.order &&
+                sc.persistent == dflt.persistent)
+                default_ms = ms;
             if (best_ms < 0.0 || ms < best_ms) {
                 best_ms = ms;
                 best = sc;
@@ -336,14 +352,22 @@ class ConvectiveFluxReconstructorGPU {
         return setSchedule(views, np, best);
     }
 
+    // The persistent midpoint flux's queue counter, allocated with the first
+    // plan that uses it (calls on a context are ordered, so one serves all).
+    int ensure_queue() {
+        if (queue_) return HM_OK;
+        if (hipMalloc(reinterpret_cast<void**>(&queue_), 256) != hipSuccess) {
+            queue_ = nullptr;
+            return fail(HM_ERR_OUT_OF_MEMORY, "hipMalloc of the work-queue counter failed");
+        }
+        return HM_OK;
+    }
+
     // Pins the schedule of a patch list's batched calls (the autotuner's
     // choice, or a caller's).
     int setSchedule(const hm_patch_view* views, int np, const Schedule& sc) {
         if (np < 1 || !views) return fail(HM_ERR_INVALID_ARGUMENT, "invalid patch list");
-        if (sc.tile_rows != 4 && sc.tile_rows != 8 && sc.tile_rows != 24)
-            return fail(HM_ERR_INVALID_ARGUMENT, "tile rows must be 4, 8 or 24");
-        if ((sc.xcd != 0 && sc.xcd != 1) || (sc.order != 0 && sc.order != 1))
-            return fail(HM_ERR_INVALID_ARGUMENT, "xcd and order are 0 or 1");
+        if (*scheduleError(sc)) return fail(HM_ERR_INVALID_ARGUMENT, scheduleError(sc));
         const uint64_t key = fnv1a(views, sizeof(hm_patch_view) * size_t(np), fnv1a(&np, sizeof np));
         bool found = false;
         for (auto& t : tuned_)
@@ -483,6 +507,8 @@ class ConvectiveFluxReconstructorGPU {
         const int d = cfg_.dim;
         plan.tile_rows = tile_rows_of(&v, 1);
         plan.xcd = fused_ ? 1 : 0;
+        plan.persistent = 0;
+        plan.queue = nullptr;
         const Extents e = extents(v, plan.tile_rows);
         // derived cells
         plan.work[0].p = dp;
@@ -786,6 +812,12 @@ class ConvectiveFluxReconstructorGPU {
         int64_t acc[kStages] = {0, 0, 0, 0, 0};
         plan.tile_rows = fused_ ? sc.tile_rows : 4;
         plan.xcd = sc.xcd;
+        plan.persistent = (!fused_ && sc.persistent) ? 1 : 0;
+        plan.queue = nullptr;
+        if (plan.persistent) {
+            if (int st = ensure_queue()) return st;
+            plan.queue = queue_;
+        }
         // (count, patch * 4 + axis) per stage, in patch order
         std::vector<std::pair<int64_t, int>> segs[kStages];
         for (int i = 0; i < np; i++) {
@@ -885,6 +917,7 @@ class ConvectiveFluxReconstructorGPU {
             if (st < first || st >= last) sub.items[st] = 0;
         KParams k = k0;
         k.xcd = plan.xcd;
+        k.queue = plan.queue;
         hipError_t e = hipSuccess;
         switch (cfg_.model) {
             case HM_MODEL_SINGLE_SPECIES: e = launch_rhs_single<W>(cfg_.dim, cfg_.scheme, sub, k, s); break;
@@ -935,6 +968,7 @@ class ConvectiveFluxReconstructorGPU {
     std::vector<TunedSchedule> tuned_;  // schedules pinned per patch list (autotuner or caller)
     uint64_t use_clock_ = 0;
     void* stage_buf_ = nullptr;
+    unsigned* queue_ = nullptr;  // work-queue counter of the persistent midpoint flux (allocated on first use)
     size_t stage_bytes_ = 0;
     hipEvent_t stage_ev_ = nullptr;
     std::string err_;

@@ -239,6 +239,7 @@ class Level:
     def set_schedule(self, sched: dict):
         sc = capi.hm_schedule()
         sc.tile_rows, sc.xcd_remap, sc.order = int(sched["tile_rows"]), int(sched["xcd_remap"]), int(sched["order"])
+        sc.persistent = int(sched.get("persistent", 0))
         capi.check(self.rec._ctx, capi.lib.hm_level_set_schedule(self._lvl, ctypes.byref(sc)))
 
     def schedule(self) -> dict:

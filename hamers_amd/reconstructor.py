@@ -183,6 +183,7 @@ def _schedule(sched):
         return sched
     c = capi.hm_schedule()
     c.tile_rows, c.xcd_remap, c.order = int(sched["tile_rows"]), int(sched["xcd_remap"]), int(sched["order"])
+    c.persistent = int(sched.get("persistent", 0))
     return c
 
 

@@ -14,8 +14,11 @@ from hamers_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-CANDIDATES_SINGLE = [{"tile_rows": r, "xcd_remap": x, "order": o} for r in (4, 8, 24) for x in (0, 1) for o in (0, 1)]
-CANDIDATES_MULTI = [{"tile_rows": 4, "xcd_remap": x, "order": o} for x in (0, 1) for o in (0, 1)]
+CANDIDATES_SINGLE = [{"tile_rows": r, "xcd_remap": x, "order": o, "persistent": 0}
+                     for r in (4, 8, 24) for x in (0, 1) for o in (0, 1)]
+CANDIDATES_MULTI = [{"tile_rows": 4, "xcd_remap": x, "order": o, "persistent": p}
+                    for x in (0, 1) for o in (0, 1) for p in (0, 1)]
+KEYS = ("tile_rows", "xcd_remap", "order", "persistent")
 
 
 def _say(msg):
@@ -81,9 +84,9 @@ def _check_list(torch, rec, pds, dt, candidates, tag):
     _say(f"{tag}: {len(candidates)} schedules bit-identical")
     best = autotune(rec, plist, dt, repetitions=3)
     _say(f"{tag}: autotuned {best}")
-    assert {k: best[k] for k in ("tile_rows", "xcd_remap", "order")} in candidates
+    assert {k: best[k] for k in KEYS} in candidates
     assert 0.0 < best["ms"] <= best["default_ms"]
-    assert get_schedule(rec, plist) == {k: best[k] for k in ("tile_rows", "xcd_remap", "order")}
+    assert get_schedule(rec, plist) == {k: best[k] for k in KEYS}
     _same(_rhs(torch, rec, plist, pds, dt), ref, f"{tag} after autotune")
 
 
@@ -134,7 +137,7 @@ def test_level_autotune_keeps_results(torch_dev, use_graph):
     best = lvl.autotune(dt, repetitions=3)
     _say(f"level autotuned {best}")
     assert 0.0 < best["ms"] <= best["default_ms"]
-    assert lvl.schedule() == {k: best[k] for k in ("tile_rows", "xcd_remap", "order")}
+    assert lvl.schedule() == {k: best[k] for k in KEYS}
     lvl.write_global(Qi)
     lvl.advance(dt, 2, use_graph=use_graph)
     torch.cuda.synchronize()
@@ -155,3 +158,5 @@ def test_bad_schedule_is_rejected(torch_dev):
     lvl = Level(rec, [((0, 0), (16, 16))], ((0, 0), (16, 16)), (1, 1), (1 / 16, 1 / 16))
     with pytest.raises(capi.HamersError, match="tile rows"):
         lvl.set_schedule({"tile_rows": 5, "xcd_remap": 0, "order": 0})
+    with pytest.raises(capi.HamersError, match="persistent"):
+        lvl.set_schedule({"tile_rows": 4, "xcd_remap": 0, "order": 0, "persistent": 1})
