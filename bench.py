@@ -45,13 +45,13 @@ FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix)
 # + WRITE_SIZE x 1.0, factors from a copy of known size) and the fraction of
 # the VALU issue peak its SQ_INSTS_VALU reaches.
 HEADLINE_KERNEL_COUNTS = {
-    "k_midflux_multi": {
+    "k_flux_multi": {
         "cells": 68726784,
-        # 64 x (3 796 687 379 + 5 664 677 342 + 423 476 106 + 2 x 3 067 398 558)
-        "fp64_ops": 64 * (3796687379 + 5664677342 + 423476106 + 2 * 3067398558),
-        # FETCH_SIZE 24 383 300 KiB x 2.0 + WRITE_SIZE 17 778 258 KiB x 1.0 (includes the 36 B/lane spill)
-        "dram_bytes": 24383300 * 1024 * 2 + 17778258 * 1024,
-        "valu_issue_frac": 0.843,
+        # 64 x (4 089 705 418 + 6 070 537 734 + 443 484 271 + 2 x 3 195 942 072)
+        "fp64_ops": 64 * (4089705418 + 6070537734 + 443484271 + 2 * 3195942072),
+        # FETCH_SIZE 54 003 334 KiB x 2.0 + WRITE_SIZE 24 948 682 KiB x 1.0 (includes the spill)
+        "dram_bytes": 54003334 * 1024 * 2 + 24948682 * 1024,
+        "valu_issue_frac": 0.736,
         "source": "profiles/rhs_uneven3d_five_eqn_counters.md",
     },
 }
@@ -59,7 +59,8 @@ HEADLINE_KERNEL_COUNTS = {
 # patches, WCNS5-JS): DRAM bytes per launch.
 FLUX_TRAFFIC_2048_WCNS5 = {"midflux": 354.92e6 + 135.27e6, "faceflux": 506.47e6 + 135.27e6}
 STAGE_KERNEL_SINGLE = {"midflux": "k_flux_single", "faceflux": "k_flux_tile_single"}
-STAGE_KERNEL_MULTI = {"derived": "k_derived_multi", "midflux": "k_midflux_multi", "faceflux": "k_faceflux_multi",
+# (the multi-species midpoint and face flux run fused in one kernel, timed as the midflux stage)
+STAGE_KERNEL_MULTI = {"derived": "k_derived_multi", "midflux": "k_flux_multi", "faceflux": "k_faceflux_multi",
                       "source": "k_source_five"}
 UNEVEN_CHOICES = (8, 16, 24, 32, 48, 64)
 

@@ -111,6 +111,31 @@ __host__ __device__ inline int64_t fused_items(int axis, const int* n, int dim) 
     return int64_t(64) * ((mid - 3) / kChunk + 1);
 }
 
+// Fused midpoint and face flux of the multi-species models (k_flux_multi): a
+// 256-thread block owns a tile of R consecutive midpoint faces along the axis
+// by W positions across it (R W <= 256), keeps the tile's midpoint fluxes in
+// LDS and outputs the R - 2 inner faces of each of its W columns.  A column of
+// n_A + 3 midpoint faces (n_A + 1 output faces) takes ntA tiles of at most
+// kMTileFaces output faces each, of equal length, which share two midpoint
+// faces; W is as many columns as fit.
+constexpr int kMTileFaces = 62;
+
+__host__ __device__ inline void mtile_shape(int nA, int& R, int& W, int& ntA) {
+    ntA = (nA + 1 + kMTileFaces - 1) / kMTileFaces;
+    R = (nA + 1 + ntA - 1) / ntA + 2;
+    W = 256 / R;
+}
+
+// Work items of the fused axis-a segment of one patch: 256 per tile.
+__host__ __device__ inline int64_t mtile_items(int axis, const int* n, int dim) {
+    int64_t trans = 1;  // columns: positions across the axis
+    for (int b = 0; b < dim; b++)
+        if (b != axis) trans *= n[b];
+    int R, W, ntA;
+    mtile_shape(n[axis], R, W, ntA);
+    return int64_t(256) * ntA * ((trans + W - 1) / W);
+}
+
 // Work of one RHS evaluation: one work source per pipeline stage
 // (0 derived cells, 1 sensor cells -- empty: the derived-data kernels compute
 // the sensor --, 2 midpoint faces, 3 output faces, 4 interior cells for the

@@ -17,9 +17,10 @@
 // then the mixture density rho, over the 4-ghost box.
 //
 // Pipeline: k_derived_multi (with the dilatation and vorticity on the 2-ghost
-// box), k_midflux_multi (five-equation:
-// also the HLLC normal velocity u* of every midpoint face), k_faceflux_multi,
-// k_source_five.
+// box), k_flux_multi (midpoint and face flux fused over tiles; five-equation:
+// also the HLLC normal velocity u* of every midpoint face), k_source_five.
+// The persistent schedule runs the midpoint flux from a work queue
+// (k_midflux_multi_queue, fluxes to memory) and then k_faceflux_multi.
 #pragma once
 
 #include <algorithm>
@@ -28,7 +29,7 @@
 #include "hm_numerics.hpp"
 #include "hm_sensor.hpp"
 
-// Waves per SIMD the multi-species midpoint-flux kernel is compiled for, in
+// Waves per SIMD the multi-species midpoint-flux kernels are compiled for, in
 // 3D and in 1D/2D (measured on the uneven 3D five-equation level: 4 waves with
 // 36 B of spill beat 3 waves without by 2.5 %; the 2D kernel would spill 70 B).
 #ifndef HM_MIDFLUX_MULTI_WAVES_3D
@@ -578,37 +579,12 @@ __device__ __forceinline__ void midflux_multi_face(const DevPatch& P, int64_t lo
     if constexpr (!FOUR) stg(P.ustar[A], l32, ustar);
 }
 
-// Midpoint-flux stage: the midpoint faces of one axis per segment, fluxes to
-// memory (face flux in stage 3).
-template <int D, int NS, bool FOUR, int SCHEME, int PP, int QQ, class W>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
-    D == 3 ? HM_MIDFLUX_MULTI_WAVES_3D : HM_MIDFLUX_MULTI_WAVES))) void k_midflux_multi(
-    W w, KParams k) {
-    // per thread 2 NEQ reconstructed characteristic values (one face at a time)
-    __shared__ double wlds[2 * Multi<D, NS, FOUR>::NEQ * kBlock];
-    const int64_t gid = int64_t(rhs_block(k)) * kBlock + threadIdx.x;
-    const DevPatch* P;
-    int ax;
-    int64_t loc;
-    if (!w.find(gid, P, ax, loc)) return;
-    double* wl = wlds + threadIdx.x;
-    if (ax == 0) {
-        midflux_multi_face<D, NS, FOUR, 0, SCHEME, PP, QQ>(*P, loc, k, wl);
-    } else if constexpr (D > 1) {
-        if (ax == 1) {
-            midflux_multi_face<D, NS, FOUR, 1, SCHEME, PP, QQ>(*P, loc, k, wl);
-        } else if constexpr (D > 2) {
-            midflux_multi_face<D, NS, FOUR, 2, SCHEME, PP, QQ>(*P, loc, k, wl);
-        }
-    }
-}
-
-// The midpoint-flux stage as a persistent kernel (Schedule::persistent): a
-// grid of as many blocks as the device holds at once takes the stage's
-// blocks of work in order from a queue (KParams::queue, one atomic fetch per
-// block of work, cleared by k_queue_reset just before the launch), and every
-// block leaves once its fetch is past the last block of work.  Each block of
-// work is the same 256 items as the one-block-per-block launch computes.
+// The unfused midpoint-flux stage as a persistent kernel (Schedule::persistent):
+// a grid of as many blocks as the device holds at once takes the stage's
+// blocks of 256 midpoint faces in order from a queue (KParams::queue, one
+// atomic fetch per block of work, cleared by k_queue_reset just before the
+// launch), and every block leaves once its fetch is past the last block of
+// work; the midpoint fluxes go to memory for k_faceflux_multi.
 template <int D, int NS, bool FOUR, int SCHEME, int PP, int QQ, class W>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(HM_MIDFLUX_MULTI_WAVES))) void
 k_midflux_multi_queue(W w, KParams k, uint32_t nwork) {
@@ -665,12 +641,30 @@ __device__ __forceinline__ double node_flux_multi(const DevPatch& P, uint32_t q,
 
 // Output flux of face (c0, c1, c2) along A from the midpoint fluxes of faces
 // f - 1, f, f + 1 (m(side, e): side -1, 0, +1) and the node fluxes of cells
-// f - 1 and f: dt (1/30 (m_{f+1} + m_{f-1}) - 3/10 (n_f + n_{f-1}) + 23/15 m_f)
-// (WCNS56-HLLC-HLL.cpp:461-485, 1200-1282, 2327-2490).
+// f - 1 and f (nf(side, e): side -1, 0): dt (1/30 (m_{f+1} + m_{f-1}) -
+// 3/10 (n_f + n_{f-1}) + 23/15 m_f) (WCNS56-HLLC-HLL.cpp:461-485, 1200-1282,
+// 2327-2490).
+template <int D, int NS, bool FOUR, int A, class MF, class NF>
+__device__ __forceinline__ void combine_face_multi(const DevPatch& P, const Boxes<D>& B, int c0, int c1, int c2,
+                                                   double dt, MF&& m, NF&& nf) {
+    constexpr int NEQ = Multi<D, NS, FOUR>::NEQ;
+    const int64_t fv = B.face_vol(A);
+    const uint32_t fi = uint32_t(B.face_idx(A, c0, c1, c2));
+    double* out = P.F[A];
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) {
+        const double nL = nf(-1, e), nR = nf(0, e);
+        const double mR = m(1, e), mL = m(-1, e), mM = m(0, e);
+        stg(out + e * fv, fi,
+            dt * (double(1) / double(30) * (mR + mL) - double(3) / double(10) * (nR + nL) +
+                  double(23) / double(15) * mM));
+    }
+}
+
+// Output flux of face (c0, c1, c2) along A with the node fluxes formed here.
 template <int D, int NS, bool FOUR, int A, class MF>
 __device__ __forceinline__ void store_face_multi(const DevPatch& P, const Boxes<D>& B, int c0, int c1, int c2,
                                                  double dt, MF&& m) {
-    constexpr int NEQ = Multi<D, NS, FOUR>::NEQ;
     constexpr int IP = NS + D;
     const int64_t v4 = B.vol(kGc);
     const uint32_t st4 = (A == 0) ? 1u : (A == 1) ? uint32_t(B.ext(0, kGc)) : uint32_t(B.ext(0, kGc) * B.ext(1, kGc));
@@ -681,18 +675,9 @@ __device__ __forceinline__ void store_face_multi(const DevPatch& P, const Boxes<
     const uint32_t qR = uint32_t(q_idx<D>(P, c0, c1, c2)), qL = qR - qst;
     const double uaL = ldg(P.prim + (NS + A) * v4, pL), uaR = ldg(P.prim + (NS + A) * v4, pR);
     const double prL = ldg(P.prim + IP * v4, pL), prR = ldg(P.prim + IP * v4, pR);
-    const int64_t fv = B.face_vol(A);
-    const uint32_t fi = uint32_t(B.face_idx(A, c0, c1, c2));
-    double* out = P.F[A];
-#pragma unroll
-    for (int e = 0; e < NEQ; e++) {
-        const double nL = node_flux_multi<D, NS, A>(P, qL, uaL, prL, e);
-        const double nR = node_flux_multi<D, NS, A>(P, qR, uaR, prR, e);
-        const double mR = m(1, e), mL = m(-1, e), mM = m(0, e);
-        stg(out + e * fv, fi,
-            dt * (double(1) / double(30) * (mR + mL) - double(3) / double(10) * (nR + nL) +
-                  double(23) / double(15) * mM));
-    }
+    combine_face_multi<D, NS, FOUR, A>(P, B, c0, c1, c2, dt, m, [&](int side, int e) {
+        return side < 0 ? node_flux_multi<D, NS, A>(P, qL, uaL, prL, e) : node_flux_multi<D, NS, A>(P, qR, uaR, prR, e);
+    });
 }
 
 // Output face `loc` of the unfused pipeline, from the midpoint fluxes in fmid[A].
@@ -727,6 +712,93 @@ __global__ __launch_bounds__(kBlock) void k_faceflux_multi(W w, KParams k) {
             faceflux_multi_face<D, NS, FOUR, 1>(*P, loc, k);
         } else if constexpr (D > 2) {
             faceflux_multi_face<D, NS, FOUR, 2>(*P, loc, k);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Fused midpoint and face flux (tiles of mtile_shape): the block's tile `loc >> 8`
+// of the patch's axis-A segment, thread `loc & 255` one of its midpoint faces.
+// Midpoint face j (0 .. n_A + 2, the face between cells j - 2 and j - 1 along
+// A) of column t: tile (tA, tw) = (tile % ntA, tile / ntA) holds faces
+// j = tA (R - 2) + r, r < R, of columns t = tw W + w, w < W, thread
+// r + w R on axis 0 (consecutive threads along x), w + r W on axes 1 and 2.
+// Phase 1 computes the midpoint flux into the thread's LDS slots (those the
+// reconstruction staged its values in); after the barrier, a thread with
+// 1 <= r <= R - 2 and j <= n_A + 1 writes the output face j - 1 from its own
+// and its two neighbours' midpoint fluxes.  Every output face has one such
+// thread; the five-equation u* of a midpoint face is written by one tile (its
+// first R - 2 faces, and the last tile of a column all of its faces).
+static_assert(kBlock == 256, "a fused multi-species tile is one 256-thread block");
+
+template <int D, int NS, bool FOUR, int A, int SCHEME, int PP, int QQ>
+__device__ __forceinline__ void flux_multi_tile(const DevPatch& P, int64_t loc, const KParams& k, double* wlds) {
+    constexpr int NEQ = Multi<D, NS, FOUR>::NEQ;
+    const Boxes<D> B(P);
+    const int nA = B.n[A];
+    int R, W, ntA;
+    mtile_shape(nA, R, W, ntA);
+    const uint32_t tile = uint32_t(loc) >> 8, lane = uint32_t(loc) & 255u;
+    const uint32_t tA = tile % uint32_t(ntA), tw = tile / uint32_t(ntA);
+    uint32_t r, w;
+    if constexpr (A == 0) {
+        w = lane / uint32_t(R);
+        r = lane - w * uint32_t(R);
+    } else {
+        r = lane / uint32_t(W);
+        w = lane - r * uint32_t(W);
+    }
+    const int j = int(tA) * (R - 2) + int(r);
+    // columns: positions across the axis, x fastest over the other axes
+    constexpr int U = (A == 0) ? 1 : 0;  // first axis across
+    constexpr int V = (A == 2) ? 1 : 2;  // second axis across
+    const uint32_t t = tw * uint32_t(W) + w;
+    const uint32_t nU = uint32_t(B.n[U]);
+    const uint32_t tv = t / nU;
+    const bool in = lane < uint32_t(R * W) && j < nA + 3 && tv < uint32_t(B.n[V]);
+    int c[3];
+    c[A] = j - 1;
+    c[U] = int(t - tv * nU);
+    c[V] = int(tv);
+    double* wl = wlds + threadIdx.x;
+    if (in) {
+        double Fo[NEQ];
+        double ustar;
+        midflux_multi_at<D, NS, FOUR, A, SCHEME, PP, QQ>(P, B, c[0], c[1], c[2], k, Fo, ustar, wl);
+#pragma unroll
+        for (int e = 0; e < NEQ; e++) wl[e * kBlock] = Fo[e];
+        if constexpr (!FOUR) {
+            if (int(r) < R - 2 || int(tA) == ntA - 1)
+                stg(P.ustar[A], uint32_t(B.mid_idx(A, c[0], c[1], c[2])), ustar);
+        }
+    }
+    __syncthreads();
+    if (in && r >= 1u && int(r) <= R - 2 && j <= nA + 1) {
+        const int step = (A == 0) ? 1 : W;  // thread distance of the neighbouring midpoint faces
+        store_face_multi<D, NS, FOUR, A>(P, B, c[0], c[1], c[2], kdt(k), [&](int side, int e) {
+            return wlds[e * kBlock + int(threadIdx.x) + side * step];
+        });
+    }
+}
+
+// The fused flux stage: every block is one tile (the stage's segments hold
+// whole tiles), so all threads of a block reach the barrier.
+template <int D, int NS, bool FOUR, int SCHEME, int PP, int QQ, class W>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
+    D == 3 ? HM_MIDFLUX_MULTI_WAVES_3D : HM_MIDFLUX_MULTI_WAVES))) void k_flux_multi(W w, KParams k) {
+    __shared__ double wlds[2 * Multi<D, NS, FOUR>::NEQ * kBlock];
+    const int64_t gid = int64_t(rhs_block(k)) * kBlock + threadIdx.x;
+    const DevPatch* P;
+    int ax;
+    int64_t loc;
+    if (!w.find(gid, P, ax, loc)) return;  // whole blocks only
+    if (ax == 0) {
+        flux_multi_tile<D, NS, FOUR, 0, SCHEME, PP, QQ>(*P, loc, k, wlds);
+    } else if constexpr (D > 1) {
+        if (ax == 1) {
+            flux_multi_tile<D, NS, FOUR, 1, SCHEME, PP, QQ>(*P, loc, k, wlds);
+        } else if constexpr (D > 2) {
+            flux_multi_tile<D, NS, FOUR, 2, SCHEME, PP, QQ>(*P, loc, k, wlds);
         }
     }
 }
@@ -791,7 +863,9 @@ hipError_t launch_multi_dsq(const Plan<W>& plan, const KParams& k, hipStream_t s
                            plan.work[0], k);
     // (stage 1 is empty: the derived-data kernel computes the dilatation and vorticity)
     if (plan.items[2] > 0) {
-        if (plan.persistent && k.queue) {
+        if (plan.persistent) {
+            // unfused: midpoint faces from the work queue here, output faces in stage 3
+            if (!k.queue) return hipErrorInvalidValue;
             // resident blocks of this kernel on the device (queried once per instance)
             static const unsigned resident = [] {
                 int dev = 0, cus = 0, per_cu = 0;
@@ -807,8 +881,9 @@ hipError_t launch_multi_dsq(const Plan<W>& plan, const KParams& k, hipStream_t s
             hipLaunchKernelGGL((k_midflux_multi_queue<D, NS, FOUR, SCHEME, PP, QQ, W>),
                                dim3(std::min(nwork, resident)), dim3(kBlock), 0, s, plan.work[2], k, uint32_t(nwork));
         } else {
-            hipLaunchKernelGGL((k_midflux_multi<D, NS, FOUR, SCHEME, PP, QQ, W>), dim3(nblocks(plan, 2)),
-                               dim3(kBlock), 0, s, plan.work[2], k);
+            // fused midpoint and face flux over tiles (stage 3 is then empty)
+            hipLaunchKernelGGL((k_flux_multi<D, NS, FOUR, SCHEME, PP, QQ, W>), dim3(nblocks(plan, 2)), dim3(kBlock),
+                               0, s, plan.work[2], k);
         }
     }
     if (plan.items[3] > 0)

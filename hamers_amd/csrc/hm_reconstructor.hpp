@@ -509,7 +509,7 @@ class ConvectiveFluxReconstructorGPU {
         plan.xcd = fused_ ? 1 : 0;
         plan.persistent = 0;
         plan.queue = nullptr;
-        const Extents e = extents(v, plan.tile_rows);
+        const Extents e = extents(v, plan.tile_rows, true);
         // derived cells
         plan.work[0].p = dp;
         plan.work[0].nseg = 1;
@@ -593,8 +593,9 @@ class ConvectiveFluxReconstructorGPU {
     }
 
     // rows: the call's tile rows (the work-item counts of the tiled stage
-    // depend on it; the scratch layout does not).
-    Extents extents(const hm_patch_view& v, int rows = 24) const {
+    // depend on it; the scratch layout does not).  mtiles: a multi-species
+    // call with the fused flux (every schedule but the persistent one).
+    Extents extents(const hm_patch_view& v, int rows = 24, bool mtiles = false) const {
         const int d = cfg_.dim;
         Extents e{};
         int64_t n[3] = {1, 1, 1};
@@ -621,7 +622,11 @@ class ConvectiveFluxReconstructorGPU {
             // The single-species model fuses the midpoint and face flux: axis 0
             // in stage 2, axes 1 and 2 (tiles) in stage 3.
             const int nn[3] = {int(n[0]), int(n[1]), int(n[2])};
-            if (!fused_) {
+            if (mtiles && !fused_) {
+                // multi-species: midpoint and face flux fused over tiles in stage 2
+                e.work2[a] = mtile_items(a, nn, d);
+                e.work3[a] = 0;
+            } else if (!fused_) {
                 e.work2[a] = m;
                 e.work3[a] = f;
             } else if (a == 0) {
@@ -821,7 +826,7 @@ class ConvectiveFluxReconstructorGPU {
         // (count, patch * 4 + axis) per stage, in patch order
         std::vector<std::pair<int64_t, int>> segs[kStages];
         for (int i = 0; i < np; i++) {
-            const Extents e = extents(views[i], plan.tile_rows);
+            const Extents e = extents(views[i], plan.tile_rows, !plan.persistent);
             auto add = [&](int stage, int axis, int64_t c) {
                 if (c > 0) segs[stage].push_back({c, i * 4 + axis});
             };
