@@ -47,11 +47,11 @@ FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix)
 HEADLINE_KERNEL_COUNTS = {
     "k_flux_multi": {
         "cells": 68726784,
-        # 64 x (4 089 705 418 + 6 070 537 734 + 443 484 271 + 2 x 3 195 942 072)
-        "fp64_ops": 64 * (4089705418 + 6070537734 + 443484271 + 2 * 3195942072),
-        # FETCH_SIZE 54 003 334 KiB x 2.0 + WRITE_SIZE 24 948 682 KiB x 1.0 (includes the spill)
-        "dram_bytes": 54003334 * 1024 * 2 + 24948682 * 1024,
-        "valu_issue_frac": 0.736,
+        # 64 x (4 082 214 738 + 6 044 320 354 + 443 484 271 + 2 x 3 195 942 072)
+        "fp64_ops": 64 * (4082214738 + 6044320354 + 443484271 + 2 * 3195942072),
+        # FETCH_SIZE 54 973 956 KiB x 2.0 + WRITE_SIZE 26 786 422 KiB x 1.0 (includes the 72 B/lane spill)
+        "dram_bytes": 54973956 * 1024 * 2 + 26786422 * 1024,
+        "valu_issue_frac": 0.755,
         "source": "profiles/rhs_uneven3d_five_eqn_counters.md",
     },
 }

@@ -661,6 +661,20 @@ __device__ __forceinline__ void combine_face_multi(const DevPatch& P, const Boxe
     }
 }
 
+// Node fluxes of the cell (c0, c1, c2) along A, nf[e] for every equation.
+template <int D, int NS, bool FOUR, int A>
+__device__ __forceinline__ void cell_node_flux_multi(const DevPatch& P, const Boxes<D>& B, int c0, int c1, int c2,
+                                                     double* nf, int stride) {
+    constexpr int NEQ = Multi<D, NS, FOUR>::NEQ;
+    constexpr int IP = NS + D;
+    const int64_t v4 = B.vol(kGc);
+    const uint32_t pc = uint32_t(B.idx(kGc, c0, c1, c2));
+    const uint32_t q = uint32_t(q_idx<D>(P, c0, c1, c2));
+    const double ua = ldg(P.prim + (NS + A) * v4, pc), pr = ldg(P.prim + IP * v4, pc);
+#pragma unroll
+    for (int e = 0; e < NEQ; e++) nf[e * stride] = node_flux_multi<D, NS, A>(P, q, ua, pr, e);
+}
+
 // Output flux of face (c0, c1, c2) along A with the node fluxes formed here.
 template <int D, int NS, bool FOUR, int A, class MF>
 __device__ __forceinline__ void store_face_multi(const DevPatch& P, const Boxes<D>& B, int c0, int c1, int c2,
@@ -723,10 +737,11 @@ __global__ __launch_bounds__(kBlock) void k_faceflux_multi(W w, KParams k) {
 // A) of column t: tile (tA, tw) = (tile % ntA, tile / ntA) holds faces
 // j = tA (R - 2) + r, r < R, of columns t = tw W + w, w < W, thread
 // r + w R on axis 0 (consecutive threads along x), w + r W on axes 1 and 2.
-// Phase 1 computes the midpoint flux into the thread's LDS slots (those the
-// reconstruction staged its values in); after the barrier, a thread with
-// 1 <= r <= R - 2 and j <= n_A + 1 writes the output face j - 1 from its own
-// and its two neighbours' midpoint fluxes.  Every output face has one such
+// Phase 1 computes the midpoint flux and the node fluxes of the cell right of
+// the face (cell j - 1) into the thread's LDS slots (those the reconstruction
+// staged its values in); after the barrier, a thread with 1 <= r <= R - 2 and
+// j <= n_A + 1 writes the output face j - 1 from its own and its two
+// neighbours' midpoint fluxes and its own and its lower neighbour's node fluxes.  Every output face has one such
 // thread; the five-equation u* of a midpoint face is written by one tile (its
 // first R - 2 faces, and the last tile of a column all of its faces).
 static_assert(kBlock == 256, "a fused multi-species tile is one 256-thread block");
@@ -767,6 +782,8 @@ __device__ __forceinline__ void flux_multi_tile(const DevPatch& P, int64_t loc, 
         midflux_multi_at<D, NS, FOUR, A, SCHEME, PP, QQ>(P, B, c[0], c[1], c[2], k, Fo, ustar, wl);
 #pragma unroll
         for (int e = 0; e < NEQ; e++) wl[e * kBlock] = Fo[e];
+        // node fluxes of the cell right of the face (the right cell of output face j - 1, the left of face j)
+        cell_node_flux_multi<D, NS, FOUR, A>(P, B, c[0], c[1], c[2], wl + NEQ * kBlock, kBlock);
         if constexpr (!FOUR) {
             if (int(r) < R - 2 || int(tA) == ntA - 1)
                 stg(P.ustar[A], uint32_t(B.mid_idx(A, c[0], c[1], c[2])), ustar);
@@ -775,9 +792,10 @@ __device__ __forceinline__ void flux_multi_tile(const DevPatch& P, int64_t loc, 
     __syncthreads();
     if (in && r >= 1u && int(r) <= R - 2 && j <= nA + 1) {
         const int step = (A == 0) ? 1 : W;  // thread distance of the neighbouring midpoint faces
-        store_face_multi<D, NS, FOUR, A>(P, B, c[0], c[1], c[2], kdt(k), [&](int side, int e) {
-            return wlds[e * kBlock + int(threadIdx.x) + side * step];
-        });
+        const int me = int(threadIdx.x);
+        combine_face_multi<D, NS, FOUR, A>(
+            P, B, c[0], c[1], c[2], kdt(k), [&](int side, int e) { return wlds[e * kBlock + me + side * step]; },
+            [&](int side, int e) { return wlds[(NEQ + e) * kBlock + me + side * step]; });
     }
 }
 
