@@ -543,11 +543,13 @@ class Level {
     // Flux sums for the synchronization of a two-level hierarchy: F_acc of
     // every axis and S_acc accumulate gamma_s F_s and gamma_s S_s over the
     // stages of every step (Euler.cpp:1230-1270, 1360-1420, 1560-1660) until
-    // clearFluxSums.  Drops the captured steps (their tables change).
+    // clearFluxSums; in the Navier-Stokes form also Fd_acc the diffusive
+    // fluxes (a flux variable of its own, NavierStokes.cpp:393-402).  Drops
+    // the captured steps (their tables change).
     int enableFluxSums() {
         if (acc_mem_) return HM_OK;
         size_t total = 0;
-        std::vector<size_t> off(size_t(np_) * 4);
+        std::vector<size_t> off(size_t(np_) * 4), doff(size_t(np_) * 3, 0);
         for (int p = 0; p < np_; p++) {
             for (int a = 0; a < d_; a++) {
                 off[size_t(p) * 4 + a] = total;
@@ -555,6 +557,11 @@ class Level {
             }
             off[size_t(p) * 4 + 3] = total;
             total += up256(sizeof(double) * size_t(neq_) * ivol(p));
+            if (diff_)
+                for (int a = 0; a < d_; a++) {
+                    doff[size_t(p) * 3 + a] = total;
+                    total += up256(sizeof(double) * size_t(neq_) * fvol(p, a));
+                }
         }
         if (hipDeviceSynchronize() != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "device synchronize failed");
         if (hipMalloc(&acc_mem_, total) != hipSuccess) {
@@ -564,10 +571,14 @@ class Level {
         if (hipMemset(acc_mem_, 0, total) != hipSuccess) return rec_->fail(HM_ERR_DEVICE, "flux-sum clear");
         acc_bytes_ = total;
         facc_.assign(size_t(np_) * 3, nullptr);
+        fdacc_.assign(size_t(np_) * 3, nullptr);
         sacc_.assign(size_t(np_), nullptr);
         char* b = static_cast<char*>(acc_mem_);
         for (int p = 0; p < np_; p++) {
-            for (int a = 0; a < d_; a++) facc_[size_t(p) * 3 + a] = reinterpret_cast<double*>(b + off[size_t(p) * 4 + a]);
+            for (int a = 0; a < d_; a++) {
+                facc_[size_t(p) * 3 + a] = reinterpret_cast<double*>(b + off[size_t(p) * 4 + a]);
+                if (diff_) fdacc_[size_t(p) * 3 + a] = reinterpret_cast<double*>(b + doff[size_t(p) * 3 + a]);
+            }
             sacc_[p] = reinterpret_cast<double*>(b + off[size_t(p) * 4 + 3]);
         }
         drop_graphs();
@@ -586,6 +597,7 @@ class Level {
     const LevelPatch* devPatches() const { return patch_dev(); }
     const LevelPatch& hostPatch(int p) const { return patch_[p]; }
     double* fluxSum(int p, int a) const { return facc_[size_t(p) * 3 + a]; }
+    double* diffusiveFluxSum(int p, int a) const { return fdacc_.empty() ? nullptr : fdacc_[size_t(p) * 3 + a]; }
     double* sourceSum(int p) const { return sacc_[p]; }
     int dim() const { return d_; }
     bool diffusive() const { return diff_ != nullptr; }
@@ -1180,7 +1192,10 @@ class Level {
                     R.Si[m] = patch_[p].S[m];
                 }
                 if (acc_mem_) {
-                    for (int a = 0; a < d_; a++) R.Facc[a] = facc_[size_t(p) * 3 + a];
+                    for (int a = 0; a < d_; a++) {
+                        R.Facc[a] = facc_[size_t(p) * 3 + a];
+                        if (diff_) R.Fdacc[a] = fdacc_[size_t(p) * 3 + a];
+                    }
                     R.Sacc = sacc_[p];
                 }
                 t.add(p, 0, ivol(p));
@@ -1691,6 +1706,7 @@ class Level {
     void* acc_mem_ = nullptr;           // flux sums (enableFluxSums)
     size_t acc_bytes_ = 0;
     std::vector<double*> facc_, sacc_;  // per patch and axis / per patch
+    std::vector<double*> fdacc_;        // diffusive flux sums per patch and axis (Navier-Stokes form)
     RKParams rkp_[kMaxRK] = {};
     int64_t halo_items_ = 0, copy_items_ = 0, zero_items_ = 0, rk_items_ = 0;
     int src_first_ = 0;  // first source component k_zero_source clears
@@ -1925,6 +1941,7 @@ struct SyncPatch {   // coarse patch data the synchronization kernels use
     const double* Qold;  // state at the start of the coarse step (nq comps over the ghost box)
     double* Q;           // state (nq comps over the ghost box)
     double* Facc[3];     // flux sums (neq comps over the faces of each axis)
+    double* Fdacc[3];    // diffusive flux sums (Navier-Stokes form), else NULL
     const double* Sacc;  // source sums (neq comps over the interior)
 };
 
@@ -1986,8 +2003,9 @@ __global__ __launch_bounds__(kBlock) void k_fsum_add(const RefluxItem* items, in
 
 // A coarse face on the coarse-fine boundary takes the area average of the
 // fine flux sums over it: their sum times 2^(1-d).
+// kind 0: the convective flux sums (SyncPatch::Facc), 1: the diffusive ones (Fdacc).
 __global__ __launch_bounds__(kBlock) void k_reflux(const RefluxItem* items, int64_t n_items, const SyncPatch* coarse,
-                                                   const double* fsum, int neq) {
+                                                   const double* fsum, int neq, int kind) {
     const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= n_items) return;
     const RefluxItem it = items[t];
@@ -1998,12 +2016,13 @@ __global__ __launch_bounds__(kBlock) void k_reflux(const RefluxItem* items, int6
     for (int e = 0; e < neq; e++) {
         double sum = fsum[(t * 4) * neq + e];
         for (int j = 1; j < it.nf; j++) sum += fsum[(t * 4 + j) * neq + e];
-        C.Facc[it.axis][e * cv + it.cface] = sum * w;
+        (kind ? C.Fdacc : C.Facc)[it.axis][e * cv + it.cface] = sum * w;
     }
 }
 
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_sync_update(SegTableT<SyncPatch> w, int neq, int g, int nz, int zfirst) {
+__global__ __launch_bounds__(kBlock) void k_sync_update(SegTableT<SyncPatch> w, int neq, int g, int nz, int zfirst,
+                                                        int ns) {
     const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     const SyncPatch* P;
     int ax;
@@ -2022,10 +2041,17 @@ __global__ __launch_bounds__(kBlock) void k_sync_update(SegTableT<SyncPatch> w, 
     const int64_t fv0 = int64_t(n0 + 1) * n1 * n2, fv1 = int64_t(n0) * (n1 + 1) * n2, fv2 = int64_t(n0) * n1 * (n2 + 1);
     const int64_t sv = int64_t(n0) * n1 * n2;
     for (int e = 0; e < neq; e++) {
-        // Euler::synchronizeFluxes (Euler.cpp:1682-1943): Q += (-(F_x,R - F_x,L)/dx_0 - ... + S)
-        double t = -(P->Facc[0][e * fv0 + fR0] - P->Facc[0][e * fv0 + fL0]) / P->dx[0];
-        if constexpr (D > 1) t = t - (P->Facc[1][e * fv1 + fR1] - P->Facc[1][e * fv1 + fL1]) / P->dx[1];
-        if constexpr (D > 2) t = t - (P->Facc[2][e * fv2 + fR2] - P->Facc[2][e * fv2 + fL2]) / P->dx[2];
+        // Euler::synchronizeFluxes (Euler.cpp:1682-1943): Q += (-(F_x,R - F_x,L)/dx_0 - ... + S);
+        // NavierStokes::synchronizeFluxes (NavierStokes.cpp:2424-2700):
+        // Q += (-(F_c,x,R - F_c,x,L + F_d,x,R - F_d,x,L)/dx_0 - ... + S)
+        auto diff = [&](int a, int64_t fL, int64_t fR, int64_t fv) {
+            double dF = P->Facc[a][e * fv + fR] - P->Facc[a][e * fv + fL];
+            if (ns) dF = dF + P->Fdacc[a][e * fv + fR] - P->Fdacc[a][e * fv + fL];
+            return dF;
+        };
+        double t = -diff(0, fL0, fR0, fv0) / P->dx[0];
+        if constexpr (D > 1) t = t - diff(1, fL1, fR1, fv1) / P->dx[1];
+        if constexpr (D > 2) t = t - diff(2, fL2, fR2, fv2) / P->dx[2];
         P->Q[e * P->gvol + iq] = P->Qold[e * P->gvol + iq] + (t + P->Sacc[e * sv + loc]);
     }
     // Five-equation model: the last volume fraction is 1 minus the others, in
@@ -2070,9 +2096,9 @@ class Hierarchy {
         if (f_->reconstructor() != rec) return rec->fail(HM_ERR_INVALID_ARGUMENT, "levels of different contexts");
         if (f_->dim() != d || f_->ghost() != c_->ghost())
             return rec->fail(HM_ERR_INVALID_ARGUMENT, "levels of different dimension or ghost width");
-        if (c_->diffusive())
-            return rec->fail(HM_ERR_UNSUPPORTED,
-                             "the two-level advance keeps convective flux sums only (no Navier-Stokes form)");
+        if (c_->diffusive() != f_->diffusive())
+            return rec->fail(HM_ERR_INVALID_ARGUMENT, "one level in the Navier-Stokes form, the other not");
+        ns_ = c_->diffusive();
         for (int a = 0; a < d; a++) {
             if (f_->domainLo()[a] != 2 * c_->domainLo()[a] || f_->domainN()[a] != 2 * c_->domainN()[a])
                 return rec->fail(HM_ERR_INVALID_ARGUMENT, "the fine domain must be the coarse domain refined by 2");
@@ -2115,20 +2141,28 @@ class Hierarchy {
                     }
                 }
                 if (int st = f_->stepOnce(0.5 * dt, s)) return st;
-                if (nrf_ > 0)
+                if (nrf_ > 0) {
                     hipLaunchKernelGGL(k_fsum_add, dim3(blocks_for(nrf_)), dim3(kBlock), 0, s, rf_, nrf_, ffacc_, ffvol_,
                                        c_->numEqn(), fsum_, int(k == 0));
+                    if (ns_)
+                        hipLaunchKernelGGL(k_fsum_add, dim3(blocks_for(nrf_)), dim3(kBlock), 0, s, rf_, nrf_, ffdacc_,
+                                           ffvol_, c_->numEqn(), fsumd_, int(k == 0));
+                }
             }
-            if (nrf_ > 0)
+            if (nrf_ > 0) {
                 hipLaunchKernelGGL(k_reflux, dim3(blocks_for(nrf_)), dim3(kBlock), 0, s, rf_, nrf_, sync_.patches, fsum_,
-                                   c_->numEqn());
+                                   c_->numEqn(), 0);
+                if (ns_)
+                    hipLaunchKernelGGL(k_reflux, dim3(blocks_for(nrf_)), dim3(kBlock), 0, s, rf_, nrf_, sync_.patches,
+                                       fsumd_, c_->numEqn(), 1);
+            }
             const hm_config& cfg = rec->config();
             const bool five = cfg.model == HM_MODEL_FIVE_EQN_ALLAIRE;
             const int nz = five ? cfg.num_species - 1 : 0, zfirst = five ? cfg.num_species + d + 1 : 0;
             switch (d) {
-                case 1: hipLaunchKernelGGL(k_sync_update<1>, dim3(blocks_for(sync_items_)), dim3(kBlock), 0, s, sync_, c_->numEqn(), g_, nz, zfirst); break;
-                case 2: hipLaunchKernelGGL(k_sync_update<2>, dim3(blocks_for(sync_items_)), dim3(kBlock), 0, s, sync_, c_->numEqn(), g_, nz, zfirst); break;
-                default: hipLaunchKernelGGL(k_sync_update<3>, dim3(blocks_for(sync_items_)), dim3(kBlock), 0, s, sync_, c_->numEqn(), g_, nz, zfirst); break;
+                case 1: hipLaunchKernelGGL(k_sync_update<1>, dim3(blocks_for(sync_items_)), dim3(kBlock), 0, s, sync_, c_->numEqn(), g_, nz, zfirst, int(ns_)); break;
+                case 2: hipLaunchKernelGGL(k_sync_update<2>, dim3(blocks_for(sync_items_)), dim3(kBlock), 0, s, sync_, c_->numEqn(), g_, nz, zfirst, int(ns_)); break;
+                default: hipLaunchKernelGGL(k_sync_update<3>, dim3(blocks_for(sync_items_)), dim3(kBlock), 0, s, sync_, c_->numEqn(), g_, nz, zfirst, int(ns_)); break;
             }
             if (ncs_ > 0) {
                 switch (d) {
@@ -2312,17 +2346,21 @@ class Hierarchy {
             P.gvol = c_->ghostVolume(q);
             P.Qold = c_->hostPatch(q).Q[1];
             P.Q = c_->hostPatch(q).Q[0];
-            for (int a = 0; a < d; a++) P.Facc[a] = c_->fluxSum(q, a);
+            for (int a = 0; a < d; a++) {
+                P.Facc[a] = c_->fluxSum(q, a);
+                P.Fdacc[a] = c_->diffusiveFluxSum(q, a);
+            }
             P.Sacc = c_->sourceSum(q);
             int64_t v = 1;
             for (int a = 0; a < d; a++) v *= c_->patchN(q)[a];
             sync.add(q, 0, v);
         }
-        std::vector<double*> ffacc(size_t(nfp) * 3, nullptr);
+        std::vector<double*> ffacc(size_t(nfp) * 3, nullptr), ffdacc(size_t(nfp) * 3, nullptr);
         std::vector<int64_t> ffvol(size_t(nfp) * 3, 0);
         for (int p = 0; p < nfp; p++)
             for (int a = 0; a < d; a++) {
                 ffacc[size_t(p) * 3 + a] = f_->fluxSum(p, a);
+                ffdacc[size_t(p) * 3 + a] = f_->diffusiveFluxSum(p, a);
                 int64_t v = 1;
                 for (int b = 0; b < d; b++) v *= f_->patchN(p)[b] + (b == a);
                 ffvol[size_t(p) * 3 + a] = v;
@@ -2330,8 +2368,8 @@ class Hierarchy {
         const size_t fsum_bytes = up256(sizeof(double) * rf.size() * 4 * size_t(c_->numEqn()));
         const size_t bytes = up256(sizeof(CfItem) * cf.size()) + up256(sizeof(RefluxItem) * rf.size()) +
                              up256(sizeof(CoarsenItem) * cs.size()) + sync.bytes() +
-                             up256(sizeof(double*) * ffacc.size()) + up256(sizeof(int64_t) * ffvol.size()) +
-                             fsum_bytes;
+                             2 * up256(sizeof(double*) * ffacc.size()) + up256(sizeof(int64_t) * ffvol.size()) +
+                             fsum_bytes * (ns_ ? 2 : 1);
         if (hipMalloc(&mem_, bytes) != hipSuccess) {
             mem_ = nullptr;
             return rec->fail(HM_ERR_OUT_OF_MEMORY, "hierarchy tables");
@@ -2352,9 +2390,14 @@ class Hierarchy {
         sync_items_ = sync.total;
         o += sync.bytes();
         ffacc_ = reinterpret_cast<double* const*>(put(ffacc.data(), sizeof(double*) * ffacc.size()));
+        ffdacc_ = reinterpret_cast<double* const*>(put(ffdacc.data(), sizeof(double*) * ffdacc.size()));
         ffvol_ = reinterpret_cast<const int64_t*>(put(ffvol.data(), sizeof(int64_t) * ffvol.size()));
         fsum_ = reinterpret_cast<double*>(dv + o);  // written by k_fsum_add before it is read
         o += fsum_bytes;
+        if (ns_) {
+            fsumd_ = reinterpret_cast<double*>(dv + o);
+            o += fsum_bytes;
+        }
         ncf_ = int64_t(cf.size());
         nrf_ = int64_t(rf.size());
         ncs_ = int64_t(cs.size());
@@ -2372,8 +2415,11 @@ class Hierarchy {
     const CoarsenItem* cs_ = nullptr;
     SegTableT<SyncPatch> sync_{};
     double* const* ffacc_ = nullptr;
+    double* const* ffdacc_ = nullptr;  // the fine diffusive flux sums (Navier-Stokes form)
     const int64_t* ffvol_ = nullptr;
     double* fsum_ = nullptr;
+    double* fsumd_ = nullptr;  // the fine diffusive flux sums over the coarse step
+    bool ns_ = false;          // both levels in the Navier-Stokes form
     int64_t ncf_ = 0, nrf_ = 0, ncs_ = 0, sync_items_ = 0;
 };
 

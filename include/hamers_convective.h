@@ -487,7 +487,10 @@ int hm_level_read_adjoint(hm_level* level, int patch, double* lam);
  * (Euler::synchronizeFluxes, Euler.cpp:1682-1943; five-equation model: the
  * last volume fraction then 1 minus the others) and the coarse cells under the
  * fine level take the average of their fine cells (CONSERVATIVE_COARSEN).
- * Euler form only: a context with diffusive fluxes returns HM_ERR_UNSUPPORTED.
+ * A context with diffusive fluxes (four-equation Navier-Stokes form) refluxes
+ * the diffusive flux the same way, as a flux variable of its own
+ * (NavierStokes.cpp:393-402), and repeats the coarse update in the
+ * Navier-Stokes form (NavierStokes::synchronizeFluxes, NavierStokes.cpp:2424-2700).
  * Both levels must belong to one context; the fine
  * domain is the coarse domain refined by 2, fine patch boxes start and end on
  * coarse cells, the fine level lies inside the coarse level with room for the

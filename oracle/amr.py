@@ -23,6 +23,13 @@ area average written out), the
 coarse update is repeated from the start state with the flux sums
 (Euler::synchronizeFluxes, Euler.cpp:1682-1943) and coarse cells under fine
 patches take the average of their 2^d fine cells (CONSERVATIVE_COARSEN).
+With transport properties (four-equation Navier-Stokes form) both levels add
+the diffusive flux to every stage (NavierStokes::advanceSingleStepOnPatch), the
+diffusive flux is a second flux variable with its own sums, refluxed the same
+way (it is registered as FLUX with CONSERVATIVE_COARSEN, NavierStokes.cpp:
+393-402), and the coarse update of the synchronization is the Navier-Stokes
+form Q += (-(F_c,R - F_c,L + F_d,R - F_d,L)/dx_0 - ... + S)
+(NavierStokes::synchronizeFluxes, NavierStokes.cpp:2424-2700).
 Every arithmetic step keeps the operation order of the GPU kernels it checks.
 """
 from __future__ import annotations
@@ -46,8 +53,10 @@ def mc_slope(vL, v0, vR):
     return np.where(dR * dL > 0.0, np.copysign(np.minimum(np.abs(c), b), c), 0.0)
 
 
-def _divergence(F, S, dx):
-    """-(sum_a (F_a,R - F_a,L)/dx_a) + S, in the stage update's order."""
+def _divergence(F, S, dx, Fd=None):
+    """-(sum_a (F_a,R - F_a,L)/dx_a) + S in the stage update's order; with
+    diffusive fluxes Fd the Navier-Stokes form -(sum_a (F_a,R - F_a,L + Fd_a,R -
+    Fd_a,L)/dx_a) + S."""
     d = len(F)
     acc = None
     for a in range(d):
@@ -57,7 +66,10 @@ def _divergence(F, S, dx):
         L = [slice(None)] * Fa.ndim
         R[ax] = slice(1, None)
         L[ax] = slice(0, -1)
-        t = (Fa[tuple(R)] - Fa[tuple(L)]) / dx[a]
+        dF = Fa[tuple(R)] - Fa[tuple(L)]
+        if Fd is not None:
+            dF = dF + Fd[a][tuple(R)] - Fd[a][tuple(L)]
+        t = dF / dx[a]
         acc = -t if acc is None else acc - t
     return acc + S
 
@@ -66,9 +78,13 @@ class TwoLevel:
     """prm: oracle Params; coarse domain n (x first); fine patch boxes (lo, n)
     in the refined index space (even lo and n).  bc = (periodic per axis, side
     conditions, DIRICHLET values) as oracle.fill_boundary takes them; None:
-    periodic on every axis."""
+    periodic on every axis.  transport: oracle Transport for the four-equation
+    Navier-Stokes form (6 ghost cells), None for the Euler form."""
 
-    def __init__(self, prm, n, dx, fine_boxes, g=4, bc=None):
+    def __init__(self, prm, n, dx, fine_boxes, g=4, bc=None, transport=None):
+        self.tr = transport
+        if transport is not None:
+            g = max(g, 6)
         self.prm, self.d, self.g = prm, len(n), g
         self.bc = bc
         self.mom0 = 1 if prm.model == O.SINGLE_SPECIES else prm.num_species
@@ -158,8 +174,9 @@ class TwoLevel:
     # ---- one level step ----------------------------------------------------
     def _step_level(self, patches, dt, dxl, fill, acc):
         """SSP-RK3 step of a list of patch states; fill(sn, interiors, ghosted
-        stage-0 inputs) -> ghosted inputs of stage sn; acc: flux and source
-        sums per patch (added to)."""
+        stage-0 inputs) -> ghosted inputs of stage sn; acc: per patch the flux,
+        source and diffusive-flux sums (added to; the last is None in the Euler
+        form)."""
         nst = 3
         Qint = []
         R = []
@@ -172,12 +189,16 @@ class TwoLevel:
             for i, q in enumerate(Qg):
                 n = tuple(reversed(q.shape[1:]))
                 n = tuple(v - 2 * self.g for v in n)
-                F, S = O.rhs_patch(self.prm, np.ascontiguousarray(q), n, self.g, dxl, dt)
-                Rs.append(_divergence(F, S[:self.neq], dxl))
-                Facc, Sacc = acc[i]
+                qc = np.ascontiguousarray(q)
+                F, S = O.rhs_patch(self.prm, qc, n, self.g, dxl, dt)
+                Fd = O.diffusive_patch(self.prm, self.tr, qc, n, self.g, dxl, dt) if self.tr is not None else None
+                Rs.append(_divergence(F, S[:self.neq], dxl, Fd))
+                Facc, Sacc, Fdacc = acc[i]
                 for a in range(self.d):
                     if RK_GAMMA[sn][sn] != 0.0:
                         Facc[a] += RK_GAMMA[sn][sn] * F[a]
+                        if Fd is not None:
+                            Fdacc[a] += RK_GAMMA[sn][sn] * Fd[a]
                 if RK_GAMMA[sn][sn] != 0.0:
                     Sacc += RK_GAMMA[sn][sn] * S[:self.neq]
             R.append(Rs)
@@ -202,14 +223,20 @@ class TwoLevel:
         return tuple(slice(self.g, s - self.g) for s in q.shape[1:])
 
     # ---- two-level advance --------------------------------------------------
-    def advance(self, Qc, Qf_patches, dt, reflux=True):
+    def _sums(self, n):
+        """Zero flux, source and (Navier-Stokes form) diffusive-flux sums of a patch of extents n."""
+        d = self.d
+        F = [np.zeros(O.face_shape(self.neq, n, a)) for a in range(d)]
+        Fd = [np.zeros(O.face_shape(self.neq, n, a)) for a in range(d)] if self.tr is not None else None
+        return (F, np.zeros((self.neq,) + tuple(reversed(n))), Fd)
+
+    def advance(self, Qc, Qf_patches, dt, reflux=True, reflux_diffusive=True):
         """One coarse step (with two fine steps): Qc (nq, coarse domain),
         Qf_patches: interior arrays of the fine patches.  Returns the new pair."""
         g, d = self.g, self.d
         fill_c = self.fill
         Cold_g = fill_c(Qc)
-        facc_c = ([np.zeros(O.face_shape(self.neq, self.n, a)) for a in range(d)],
-                  np.zeros((self.neq,) + tuple(reversed(self.n))))
+        facc_c = self._sums(self.n)
 
         def fill_coarse(sn, pts, prev):
             # the whole level as one patch: periodic images or the boundary
@@ -219,10 +246,10 @@ class TwoLevel:
         Qc_new = self._step_level([Qc], dt, self.dx, fill_coarse, [facc_c])[0]
         Cnew_g = fill_c(Qc_new)
         fsum = None  # fine flux sums: (0 + F of the first fine step) + F of the second
+        fsumd = None  # the same of the diffusive flux (Navier-Stokes form)
         pts = [q.copy() for q in Qf_patches]
         for k in range(2):
-            facc_f = [([np.zeros(O.face_shape(self.neq, m, a)) for a in range(d)],
-                       np.zeros((self.neq,) + tuple(reversed(m)))) for _, m in self.boxes]
+            facc_f = [self._sums(m) for _, m in self.boxes]
             alpha = 0.5 * k
 
             # the coarse-fine values of this fine step on the whole fine domain,
@@ -234,15 +261,22 @@ class TwoLevel:
 
             pts = self._step_level(pts, 0.5 * dt, self.dxf, fill_fine, facc_f)
             if fsum is None:
-                fsum = [[0.0 + Fa for Fa in F] for F, _ in facc_f]
+                fsum = [[0.0 + Fa for Fa in F] for F, _, _ in facc_f]
             else:
-                fsum = [[fs + Fa for fs, Fa in zip(fsp, F)] for fsp, (F, _) in zip(fsum, facc_f)]
+                fsum = [[fs + Fa for fs, Fa in zip(fsp, F)] for fsp, (F, _, _) in zip(fsum, facc_f)]
+            if self.tr is not None:
+                if fsumd is None:
+                    fsumd = [[0.0 + Fa for Fa in Fd] for _, _, Fd in facc_f]
+                else:
+                    fsumd = [[fs + Fa for fs, Fa in zip(fsp, Fd)] for fsp, (_, _, Fd) in zip(fsumd, facc_f)]
         # synchronization
-        Fc, Sc = facc_c
+        Fc, Sc, Fdc = facc_c
         if reflux:
             self._reflux(Fc, fsum)
+            if self.tr is not None and reflux_diffusive:
+                self._reflux(Fdc, fsumd)
         Qs = Qc.copy()
-        Qs[:self.neq] = Qc[:self.neq] + _divergence(Fc, Sc, self.dx)
+        Qs[:self.neq] = Qc[:self.neq] + _divergence(Fc, Sc, self.dx, Fdc)
         if self.nq > self.neq:
             # last volume fraction = 1 - the others, in species order
             # (FlowModelFiveEqnAllaire::updateCellDataOfConservativeVariables, Euler.cpp:1937-1939)

@@ -14,6 +14,11 @@ GPU hierarchy is checked against.
 - The composite sums of the conserved variables stay constant to rounding
   over several coarse steps on a periodic domain (2D single-species, 3D
   five-equation), and a planted mistake -- no refluxing -- breaks that.
+- Navier-Stokes form (four-equation model with transport, 6 ghost cells): a
+  fully covered fine level is the single-level Navier-Stokes RK3 bit for bit;
+  the composite sums stay constant to rounding, and a planted mistake --
+  refluxing the convective flux but not the diffusive one -- breaks that; the
+  diffusive flux is in the step (the result differs from the Euler form).
 """
 import numpy as np
 import pytest
@@ -205,3 +210,56 @@ def test_reflect_wall_two_level_equals_mirrored_periodic_domain():
     Tb = amr.TwoLevel(prm, n, dx, boxes, bc=((0, 1), [O.BC_FLOW, O.BC_REFLECT, 0, 0], None))
     Qb, _ = Tb.advance(Qi, fine, dt)
     assert np.max(np.abs(Qb - Qcd[:, :, :n[0]]) / scale) > 1e-6
+
+
+# Four-equation Navier-Stokes form: constant transport properties of two species.
+NS_GAM, NS_R = (1.6, 1.4), (1.0, 2.0)
+NS_TR = O.Transport(mu=[2e-2, 1e-2], mu_v=[5e-3, 0.0], kappa=[3e-2, 2e-2], D=[2e-2, 4e-2], M=[1.0, 4.0])
+
+
+def ns_state(N):
+    """Smooth periodic two-species state on [0,1]^2 in which every primitive
+    variable varies along both axes (so every diffusive term is nonzero)."""
+    n, dx = (N, N), (1.0 / N, 1.0 / N)
+    X = W.cell_centres(n, (0.0, 0.0), dx)
+    ph = lambda k: 2 * np.pi * ((k + 1) * X[0] + (k + 2) * X[1]) + k  # noqa: E731
+    rho = 1.0 + 0.2 * np.sin(ph(0))
+    Y0 = 0.5 + 0.3 * np.sin(ph(1))
+    vel = [0.3 * np.sin(ph(2)), 0.2 * np.cos(ph(3))]
+    p = 1.0 + 0.1 * np.cos(ph(4))
+    Q = W.conservative_from_primitive_four([rho * Y0, rho * (1.0 - Y0)], vel, p, NS_GAM, NS_R)
+    prm = O.Params(dim=2, model=O.FOUR_EQN_CONSERVATIVE, num_species=2, scheme=1, gamma=list(NS_GAM),
+                   R=list(NS_R))
+    return Q, n, dx, prm
+
+
+def test_navier_stokes_fully_covered_fine_level_is_the_single_level_rk3():
+    Qi, n, dx, prm = ns_state(8)
+    T = amr.TwoLevel(prm, n, dx, [((0, 0), (16, 6)), ((0, 6), (16, 10))], transport=NS_TR)
+    assert T.g == 6
+    fine = T.refine_patches(Qi)
+    Qf_glob = np.concatenate(fine, axis=1)
+    dt = 0.2 * dx[0]
+    Qc, pts = T.advance(Qi, fine, dt)
+    ref = oracle_rk3(prm, Qf_glob, T.nf, T.dxf, 0.5 * dt, 2, transport=NS_TR)
+    assert np.array_equal(np.concatenate(pts, axis=1), ref)
+    avg = 0.25 * (ref[:, 0::2, 0::2] + ref[:, 0::2, 1::2] + ref[:, 1::2, 0::2] + ref[:, 1::2, 1::2])
+    assert max_rel_err(Qc, avg) < 1e-15
+    euler = oracle_rk3(prm, Qf_glob, T.nf, T.dxf, 0.5 * dt, 2)
+    assert max_rel_err(euler, ref) > 1e-8  # the diffusive flux is in the step
+
+
+@pytest.mark.parametrize("reflux_diffusive", [True, False], ids=["refluxed", "planted-no-diffusive-reflux"])
+def test_navier_stokes_composite_conservation(reflux_diffusive):
+    Qi, n, dx, prm = ns_state(12)
+    T = amr.TwoLevel(prm, n, dx, [((6, 6), (10, 12)), ((16, 8), (4, 6))], transport=NS_TR)
+    fine = T.refine_patches(Qi)
+    m0 = T.composite_sum(Qi, fine, prm.neq)
+    Qc, pts = Qi, fine
+    for _ in range(2):
+        Qc, pts = T.advance(Qc, pts, 0.2 * dx[0], reflux_diffusive=reflux_diffusive)
+    drift = np.max(np.abs(T.composite_sum(Qc, pts, prm.neq) - m0) / np.maximum(np.abs(m0), 1e-3))
+    if reflux_diffusive:
+        assert drift < 1e-14
+    else:
+        assert drift > 1e-9
