@@ -79,7 +79,7 @@ def parse():
     ap.add_argument("--no-per-patch", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch schedule")
     ap.add_argument("--headline-only", action="store_true", help="skip the secondary legs")
-    ap.add_argument("--leg", choices=("2d", "ns", "adjoint", "two_level"), default=None,
+    ap.add_argument("--leg", choices=("2d", "ns", "adjoint", "two_level", "two_level_ns"), default=None,
                     help="run only this secondary leg and print its JSON (for profiling it)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only copy this many MiB device to device 5 times (calibrates HBM byte counters) and exit")
@@ -93,6 +93,8 @@ def parse():
     ap.add_argument("--adjoint-N", type=int, default=512)
     ap.add_argument("--adjoint-patch", type=int, default=128)
     ap.add_argument("--adjoint-every", type=int, default=20)
+    ap.add_argument("--two-level-ns-n", type=int, default=256,
+                    help="cells per axis of the Navier-Stokes two-level leg's base level (0: skip)")
     return ap.parse_args()
 
 
@@ -606,6 +608,86 @@ def bench_two_level(args, torch, np):
     }
 
 
+def bench_two_level_ns(args, torch, np):
+    """Two-level advance (refinement ratio 2) in the Navier-Stokes form: the
+    four-equation two-species model with constant transport on a base level of
+    n^3 cells in 32^3 patches and a refined level of uneven patches (extents
+    across the interface drawn from UNEVEN_CHOICES) over a slab around the
+    multimode interface, both with 6 ghost cells.  Times coarse steps of the
+    hierarchy (coarse step, two fine steps of dt / 2, coarse-fine fill,
+    convective and diffusive refluxing, coarsening) against the levels' own steps."""
+    from hamers_amd import ConvectiveFluxReconstructor, DiffusiveFluxReconstructor, Hierarchy, Level
+
+    rng = np.random.default_rng(256)
+    N = args.two_level_ns_n
+    Nc = (N, N, N)
+    Nf = tuple(2 * v for v in Nc)
+    dxc = tuple(1.0 / v for v in Nc)
+    dxf = tuple(1.0 / v for v in Nf)
+    ks = np.arange(4, 33)
+    amps = ks ** -2.0
+    amps = 0.02 * amps / amps.sum()
+    modes = [(float(a), rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi)) for a in amps]
+    gamma, R = (1.4, 1.67), (1.0, 0.25)
+    rec = ConvectiveFluxReconstructor("two-level-ns", 3, "FOUR_EQN_CONSERVATIVE", "WCNS6_LD_HLLC_HLL", num_species=2,
+                                      species_gamma=gamma, species_R=R)
+    diff = DiffusiveFluxReconstructor(rec, species_mu=(1e-3, 2e-3), species_mu_v=(0.0, 0.0),
+                                      species_kappa=(1.5e-3, 2.5e-3), species_D=(1e-3, 1e-3), species_M=(1.0, 4.0))
+    cboxes = [((i, j, k), (32, 32, 32)) for k in range(0, N, 32) for j in range(0, N, 32) for i in range(0, N, 32)]
+
+    def segments(total):
+        out = []
+        while sum(out) < total:
+            out.append(int(min(rng.choice(UNEVEN_CHOICES), total - sum(out))))
+        return out
+
+    # fine slab x in [N - 32, N + 32) of the fine grid (the interface at x = 0.5), two patches of 32 cells
+    xs = [N - 32, 32, 32]
+    ys, zs = segments(Nf[1]), segments(Nf[2])
+    fboxes = []
+    zo = 0
+    for nz in zs:
+        yo = 0
+        for ny in ys:
+            xo = xs[0]
+            for nx in xs[1:]:
+                fboxes.append(((xo, yo, zo), (nx, ny, nz)))
+                xo += nx
+            yo += ny
+        zo += nz
+    C = Level(rec, cboxes, ((0, 0, 0), Nc), (1, 1, 1), dxc)
+    F = Level(rec, fboxes, ((0, 0, 0), Nf), (1, 1, 1), dxf)
+    for lvl, boxes, dx in ((C, cboxes, dxc), (F, fboxes, dxf)):
+        for i, (lo, n) in enumerate(boxes):
+            Q = rtm_interface_patch_torch(torch, lo, n, 0, dx, modes, gamma, R)
+            lvl.write_patch(i, Q.cpu().numpy())
+    torch.cuda.synchronize()
+    H = Hierarchy(C, F)
+    dt = 5.0e-4  # CFL ~0.2 of the light gas's sound speed on the base grid; well inside the viscous limit
+    stream = torch.cuda.current_stream()
+    ms_h, _ = timed_events(torch, stream, lambda: H.advance(dt, 1), 2, 1)
+    ms_c, _ = timed_events(torch, stream, lambda: C.advance(dt, 1), 2, 1)
+    ms_f, _ = timed_events(torch, stream, lambda: F.advance(0.5 * dt, 2), 2, 1)
+    finite = all(bool(np.isfinite(F.read_patch(i)).all()) for i in range(0, len(fboxes), 17))
+    del H
+    del diff
+    cc = int(np.prod(Nc))
+    fc = int(sum(n[0] * n[1] * n[2] for _, n in fboxes))
+    return {
+        "workload": "3D four-equation two-species Navier-Stokes (constant transport), multimode interface with a "
+                    f"k^-2 spectrum over k in [4, 32], WCNS6_LD_HLLC_HLL + sixth-order node diffusive flux; base "
+                    f"{N}^3 in 32^3 patches, refined level (ratio 2) of uneven patches (extents across the interface "
+                    "from {8,16,24,32,48,64}) over the slab around the interface, 6-cell halos, periodic",
+        "coarse_patches": len(cboxes), "coarse_cells": cc, "fine_patches": len(fboxes), "fine_cells": fc,
+        "dt_coarse": dt,
+        "ms_per_coarse_step": ms_h,
+        "ms_coarse_level_step": ms_c, "ms_two_fine_level_steps": ms_f,
+        "ms_coarse_fine_overhead": ms_h - ms_c - ms_f,
+        "cell_updates_per_s": (cc + 2 * fc) / (ms_h * 1e-3),
+        "finite": finite,
+    }
+
+
 def bench_adjoint(args, torch, np, time):
     """Discrete adjoint of SSP-RK3 on a Kelvin-Helmholtz level: 512^2 in 16
     patches of 128^2, WCNS6-LD + hybrid HLLC/HLLC-HLL, 200 steps, checkpoints
@@ -698,6 +780,8 @@ def main():
             res = bench_ns_3d(args, torch, np, time)
         elif args.leg == "two_level":
             res = bench_two_level(args, torch, np)
+        elif args.leg == "two_level_ns":
+            res = bench_two_level_ns(args, torch, np)
         else:
             res = bench_adjoint(args, torch, np, time)
         print(json.dumps({"leg": args.leg, "result": res}), flush=True)
@@ -715,7 +799,7 @@ def main():
     del L
     torch.cuda.empty_cache()
 
-    rhs2d = ns3d = adjoint = two_level = None
+    rhs2d = ns3d = adjoint = two_level = two_level_ns = None
     if not args.headline_only:
         rhs2d = bench_2d(args, torch, np)
         progress("2D leg done")
@@ -730,6 +814,10 @@ def main():
             torch.cuda.empty_cache()
         two_level = bench_two_level(args, torch, np)
         progress("two-level leg done")
+        torch.cuda.empty_cache()
+        if args.two_level_ns_n > 0:
+            two_level_ns = bench_two_level_ns(args, torch, np)
+            progress("Navier-Stokes two-level leg done")
 
     value = config["cells"] / (ms * 1e-3)
     out = {
@@ -759,6 +847,7 @@ def main():
         "navier_stokes_3d": ns3d,
         "adjoint": adjoint,
         "two_level": two_level,
+        "two_level_navier_stokes": two_level_ns,
     }
     print(json.dumps(out), flush=True)
 
