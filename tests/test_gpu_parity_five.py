@@ -219,3 +219,38 @@ def test_five_eqn_null_last_volume_fraction_status(torch_dev):
     arr = (capi.hm_patch_view * 1)(v)
     st = capi.lib.hm_compute_convective_flux_and_source_on_patches(rec._ctx, arr, 1, 0.0, 1e-3, 0, None)
     assert st == capi.HM_ERR_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_five_eqn_tile_shapes(torch_dev, dim):
+    """Extents that exercise the fused flux's tiles of whole columns: columns
+    of two and three tiles (n_A >= 62 faces: 62, 63, 64, 65, 125, 130 cells),
+    columns that do not fill a tile's width, single-cell axes; batched and per
+    patch, against the oracle."""
+    torch = torch_dev
+    if dim == 2:
+        sizes = [(62, 5), (63, 3), (64, 7), (125, 4), (3, 130), (1, 70), (65, 1), (9, 13)]
+    else:
+        sizes = [(64, 6, 5), (6, 65, 4), (5, 4, 62), (1, 3, 64), (12, 9, 7)]
+    rec = _rec(dim, 1)
+    pds, refs = [], []
+    for i, n in enumerate(sizes):
+        Q = shock_interface(n, 4, seed=40 + i)
+        dx = tuple(1.0 / max(v, 8) for v in n)
+        pds.append(_patchdata(torch, Q, n, 4, dx, rec.num_eqn))
+        refs.append(O.rhs_patch(_prm(dim, 1), Q, n, 4, dx, 1e-3))
+    _say(f"case five-eqn {dim}D tile shapes, batched")
+    rec.computeConvectiveFluxAndSourceOnPatches(pds, 0.0, 1e-3)
+    torch.cuda.synchronize()
+    for i, pd in enumerate(pds):
+        compare(f"five-eqn {dim}D tile shapes batched {sizes[i]}", [f.cpu().numpy() for f in pd.F], refs[i][0],
+                pd.S.cpu().numpy(), refs[i][1])
+    _say(f"case five-eqn {dim}D tile shapes, per patch")
+    for i, pd in enumerate(pds):
+        for f in pd.F:
+            f.fill_(np.nan)
+        pd.S.zero_()
+        rec.computeConvectiveFluxAndSourceOnPatch(pd, 0.0, 1e-3)
+        torch.cuda.synchronize()
+        compare(f"five-eqn {dim}D tile shapes per patch {sizes[i]}", [f.cpu().numpy() for f in pd.F], refs[i][0],
+                pd.S.cpu().numpy(), refs[i][1])

@@ -187,3 +187,32 @@ def test_four_eqn_level_rk3(torch_dev):
     err = max_rel_err(got, want)
     _say(f"four-eqn level RK3: max rel err {err:.3e}, bit-identical {int(np.sum(got == want))}/{got.size}")
     assert err <= TOL
+
+
+def test_four_eqn_tile_shapes_2d(torch_dev):
+    """Extents that exercise the fused flux's tiles of whole columns (two and
+    three tiles per column, single-cell axes, part-filled tiles), batched and
+    per patch, against the oracle."""
+    torch = torch_dev
+    sizes = [(62, 5), (64, 7), (125, 4), (3, 130), (1, 70), (9, 13)]
+    rec = _rec(2, 1)
+    pds, refs = [], []
+    for i, n in enumerate(sizes):
+        Q = shock_interface_four(n, 4, seed=60 + i)
+        dx = tuple(1.0 / max(v, 8) for v in n)
+        pds.append(_patchdata(torch, Q, n, 4, dx, rec.num_eqn))
+        refs.append(O.rhs_patch(_prm(2, 1), Q, n, 4, dx, 1e-3))
+    _say("case four-eqn 2D tile shapes, batched")
+    rec.computeConvectiveFluxAndSourceOnPatches(pds, 0.0, 1e-3)
+    torch.cuda.synchronize()
+    for i, pd in enumerate(pds):
+        compare(f"four-eqn 2D tile shapes batched {sizes[i]}", [f.cpu().numpy() for f in pd.F], refs[i][0],
+                pd.S.cpu().numpy())
+    _say("case four-eqn 2D tile shapes, per patch")
+    for i, pd in enumerate(pds):
+        for f in pd.F:
+            f.fill_(np.nan)
+        rec.computeConvectiveFluxAndSourceOnPatch(pd, 0.0, 1e-3)
+        torch.cuda.synchronize()
+        compare(f"four-eqn 2D tile shapes per patch {sizes[i]}", [f.cpu().numpy() for f in pd.F], refs[i][0],
+                pd.S.cpu().numpy())
