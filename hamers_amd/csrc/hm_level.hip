@@ -1225,6 +1225,16 @@ class Level {
                 if (m <= s && beta_[s][m] != 0.0) r.last_beta = m;
             }
         }
+        // the flux sums' terms of a whole step: stage by stage, intermediates in order
+        step_terms_ = AccTerms{};
+        step_terms_.neq = neq_;
+        for (int s = 0; s < nst_; s++)
+            for (int m = 0; m <= s; m++)
+                if (gamma_[s][m] != 0.0) {
+                    step_terms_.src[step_terms_.n] = m;
+                    step_terms_.gamma[step_terms_.n] = gamma_[s][m];
+                    step_terms_.n++;
+                }
         if (hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice) != hipSuccess)
             return rec_->fail(HM_ERR_DEVICE, "level table upload");
         return HM_OK;
@@ -1276,10 +1286,14 @@ class Level {
                 return st;
             if (diff_)
                 if (int st = diff_->runOwnedPlan(diff_plan_[sn], dt, s, dtp)) return st;
-            if (launch_rk_update<SegTableT<RKPatch>>(d_, rk_[sn], rk_items_, acc_mem_ ? &acc_[sn] : nullptr,
-                                                     acc_items_, rkp_[sn], s) != hipSuccess)
+            if (launch_rk_update<SegTableT<RKPatch>>(d_, rk_[sn], rk_items_, nullptr, 0, rkp_[sn], s) != hipSuccess)
                 return rec_->fail(HM_ERR_DEVICE, "stage update launch failed");
         }
+        // Flux sums: the whole step's gamma terms in one pass over the stages'
+        // fluxes (the same additions, in the same order, as one pass per stage).
+        if (acc_mem_ &&
+            launch_rk_accumulate<SegTableT<RKPatch>>(d_, acc_[nst_ - 1], acc_items_, step_terms_, s) != hipSuccess)
+            return rec_->fail(HM_ERR_DEVICE, "flux-sum launch failed");
         return HM_OK;
     }
 
@@ -1707,6 +1721,7 @@ class Level {
     size_t acc_bytes_ = 0;
     std::vector<double*> facc_, sacc_;  // per patch and axis / per patch
     std::vector<double*> fdacc_;        // diffusive flux sums per patch and axis (Navier-Stokes form)
+    AccTerms step_terms_{};             // gamma terms of one step's flux sums
     RKParams rkp_[kMaxRK] = {};
     int64_t halo_items_ = 0, copy_items_ = 0, zero_items_ = 0, rk_items_ = 0;
     int src_first_ = 0;  // first source component k_zero_source clears

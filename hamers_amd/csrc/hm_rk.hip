@@ -92,9 +92,9 @@ __global__ __launch_bounds__(kBlock) void k_rk_update(W w, RKParams r) {
     if (nz > 0) P->Q[r.neq][iq] = (r.last_beta >= 0) ? zl : double(0);
 }
 
-// gamma-weighted accumulation of the stage fluxes and sources.
+// gamma-weighted accumulation of stage fluxes and sources: the terms t in order.
 template <int D, class W>
-__global__ __launch_bounds__(kBlock) void k_rk_accumulate(W w, RKParams r) {
+__global__ __launch_bounds__(kBlock) void k_rk_accumulate(W w, AccTerms t) {
     const int64_t gid = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     const RKPatch* P;
     int ax;  // 0..D-1: faces of that axis, D: cells (source)
@@ -106,27 +106,41 @@ __global__ __launch_bounds__(kBlock) void k_rk_accumulate(W w, RKParams r) {
         vol = int64_t(P->n[0]) * n1 * n2;
     else
         vol = int64_t(P->n[0] + (ax == 0)) * (n1 + (ax == 1)) * (n2 + (ax == 2));
-    for (int e = 0; e < r.neq; e++) {
+    for (int e = 0; e < t.neq; e++) {
         double* acc = (ax == D) ? P->Sacc : P->Facc[ax];
         if (acc) {
             double v = acc[e * vol + loc];
-            for (int n = 0; n < r.nc; n++) {
-                if (r.gamma[n] == 0.0) continue;
+            for (int i = 0; i < t.n; i++) {
+                const int n = t.src[i];
                 const double* src = (ax == D) ? P->Si[n] : P->Fi[n][ax];
-                v += r.gamma[n] * src[e * vol + loc];
+                v += t.gamma[i] * src[e * vol + loc];
             }
             acc[e * vol + loc] = v;
         }
         double* dacc = (ax == D) ? nullptr : P->Fdacc[ax];
         if (dacc) {
             double v = dacc[e * vol + loc];
-            for (int n = 0; n < r.nc; n++) {
-                if (r.gamma[n] == 0.0 || !P->Fdi[n][ax]) continue;
-                v += r.gamma[n] * P->Fdi[n][ax][e * vol + loc];
+            for (int i = 0; i < t.n; i++) {
+                const int n = t.src[i];
+                if (!P->Fdi[n][ax]) continue;
+                v += t.gamma[i] * P->Fdi[n][ax][e * vol + loc];
             }
             dacc[e * vol + loc] = v;
         }
     }
+}
+
+template <class W>
+hipError_t launch_rk_accumulate(int dim, const W& acc, int64_t acc_items, const AccTerms& terms, hipStream_t s) {
+    if (acc_items <= 0 || terms.n == 0) return hipSuccess;
+    const unsigned na = unsigned((acc_items + kBlock - 1) / kBlock);
+    switch (dim) {
+        case 1: hipLaunchKernelGGL((k_rk_accumulate<1, W>), dim3(na), dim3(kBlock), 0, s, acc, terms); break;
+        case 2: hipLaunchKernelGGL((k_rk_accumulate<2, W>), dim3(na), dim3(kBlock), 0, s, acc, terms); break;
+        case 3: hipLaunchKernelGGL((k_rk_accumulate<3, W>), dim3(na), dim3(kBlock), 0, s, acc, terms); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 template <class W>
@@ -146,13 +160,17 @@ hipError_t launch_rk_update(int dim, const W& cells, int64_t cell_items, const W
         }
     }
     if (acc && acc_items > 0) {
-        const unsigned na = unsigned((acc_items + kBlock - 1) / kBlock);
-        switch (dim) {
-            case 1: hipLaunchKernelGGL((k_rk_accumulate<1, W>), dim3(na), dim3(kBlock), 0, s, *acc, r); break;
-            case 2: hipLaunchKernelGGL((k_rk_accumulate<2, W>), dim3(na), dim3(kBlock), 0, s, *acc, r); break;
-            case 3: hipLaunchKernelGGL((k_rk_accumulate<3, W>), dim3(na), dim3(kBlock), 0, s, *acc, r); break;
-            default: return hipErrorInvalidValue;
-        }
+        // this stage's terms: its intermediates with nonzero gamma, in order
+        AccTerms t{};
+        t.neq = r.neq;
+        for (int n = 0; n < r.nc; n++)
+            if (r.gamma[n] != 0.0) {
+                t.src[t.n] = n;
+                t.gamma[t.n] = r.gamma[n];
+                t.n++;
+            }
+        if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
+        return launch_rk_accumulate<W>(dim, *acc, acc_items, t, s);
     }
     return hipGetLastError();
 }
@@ -163,5 +181,7 @@ template hipError_t launch_rk_update<SegSingleT<RKPatch>>(int, const SegSingleT<
 template hipError_t launch_rk_update<SegTableT<RKPatch>>(int, const SegTableT<RKPatch>&, int64_t,
                                                          const SegTableT<RKPatch>*, int64_t, const RKParams&,
                                                          hipStream_t);
+template hipError_t launch_rk_accumulate<SegTableT<RKPatch>>(int, const SegTableT<RKPatch>&, int64_t,
+                                                             const AccTerms&, hipStream_t);
 
 }  // namespace hm

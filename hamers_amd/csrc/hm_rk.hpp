@@ -34,10 +34,26 @@ struct RKParams {
     double alpha[kMaxRK], beta[kMaxRK], gamma[kMaxRK];
 };
 
+// Terms of one gamma accumulation, added in order: acc += gamma[t] * (flux,
+// diffusive flux or source of intermediate src[t]).  One stage's terms, or a
+// whole step's (stage by stage, each stage's in intermediate order: the same
+// sequence of additions as one accumulation after every stage).
+constexpr int kMaxAccTerms = kMaxRK * (kMaxRK + 1) / 2;
+struct AccTerms {
+    int n;
+    int neq;
+    int src[kMaxAccTerms];
+    double gamma[kMaxAccTerms];
+};
+
 // Launch the update over `cell_items` interior cells and, when `acc` is given,
 // the gamma accumulation over `acc_items` items (faces per axis, then cells).
 template <class W>
 hipError_t launch_rk_update(int dim, const W& cells, int64_t cell_items, const W* acc, int64_t acc_items,
                             const RKParams& r, hipStream_t s);
+
+// Launch the gamma accumulation of `terms` over `acc_items` items.
+template <class W>
+hipError_t launch_rk_accumulate(int dim, const W& acc, int64_t acc_items, const AccTerms& terms, hipStream_t s);
 
 }  // namespace hm
