@@ -165,10 +165,12 @@ int hm_compute_convective_flux_and_source_on_patches(hm_ctx* ctx, const hm_patch
  *   xcd_remap  1: each of the eight XCDs (own L2) takes a contiguous range of
  *              block numbers; 0: the hardware's round-robin order
  *   order      0: work segments in patch order; 1: largest segments first
- *   persistent four- and five-equation models: 1 runs the midpoint-flux stage
- *              as a persistent kernel (as many blocks as the device holds at
+ *   persistent four- and five-equation models: 1 runs the midpoint flux as
+ *              a persistent kernel (as many blocks as the device holds at
  *              once, taking the stage's blocks of work from a queue until it
- *              is empty); 0: one block per block of work.  Single-species: 0
+ *              is empty) followed by a separate face-flux launch; 0: the
+ *              midpoint and face flux fused, one block per tile of the patch's
+ *              columns.  Single-species: 0
  * Without a pinned schedule a list uses the built-in rule (tile rows by the
  * call's size, xcd_remap 1 for the single-species model, order 0).
  * hm_autotune_on_patches times every candidate schedule on these views
