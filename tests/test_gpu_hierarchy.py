@@ -8,8 +8,9 @@ restatement (oracle/amr.py, pinned in test_oracle_amr.py).
 - Physical boundaries: 2D single-species with REFLECT and FLOW x sides and a
   fine patch against the wall, and 3D five-equation with SYMMETRY and DIRICHLET
   z sides, against the restatement with the same conditions.
-- Navier-Stokes form: 2D four-equation model with transport (6 ghost cells),
-  the diffusive flux refluxed beside the convective one, two coarse steps.
+- Navier-Stokes form: four-equation model with transport (6 ghost cells), the
+  diffusive flux refluxed beside the convective one: 2D, two coarse steps; 3D,
+  two fine patches, one coarse step; 2D with REFLECT / FLOW x sides.
 Coarse and fine states: max-norm relative error <= 1e-12 (FP64; the
 restatement keeps the kernels' operation order).  The GPU result keeps the
 composite sums of the conserved variables to rounding.  Errors: misaligned
@@ -184,4 +185,36 @@ def test_two_level_2d_navier_stokes(torch_dev):
     coarse = [((i, j), (8, 8)) for j in (0, 8) for i in (0, 8)]
     _case(torch_dev, rec, prm, Qi, n, dx, coarse, [((8, 8), (12, 14)), ((20, 10), (6, 8))], 0.2 * dx[0], 2, prm.neq,
           "2D four-equation Navier-Stokes, two fine patches", transport=NS_TR)
+    del diff
+
+
+def _ns_context(prm):
+    from hamers_amd import DiffusiveFluxReconstructor
+    from tests.test_oracle_amr import NS_TR
+
+    rec = _rec(prm.dim, "FOUR_EQN_CONSERVATIVE", "WCNS6_LD_HLLC_HLL", 2, tuple(prm.gamma), tuple(prm.R))
+    diff = DiffusiveFluxReconstructor(rec, NS_TR.mu, NS_TR.mu_v, NS_TR.kappa, NS_TR.D, NS_TR.M)
+    return rec, diff
+
+
+def test_two_level_3d_navier_stokes(torch_dev):
+    from tests.test_oracle_amr import NS_TR, ns_state_3d
+
+    Qi, n, dx, prm = ns_state_3d(8)
+    rec, diff = _ns_context(prm)
+    _case(torch_dev, rec, prm, Qi, n, dx, [((0, 0, 0), (8, 8, 8))], [((4, 4, 2), (6, 8, 10)), ((10, 4, 2), (4, 4, 10))],
+          0.2 * dx[0], 1, prm.neq, "3D four-equation Navier-Stokes, two fine patches", transport=NS_TR)
+    del diff
+
+
+def test_two_level_2d_navier_stokes_walls(torch_dev):
+    from tests.test_oracle_amr import NS_TR, ns_state
+
+    Qi, n, dx, prm = ns_state(16)
+    rec, diff = _ns_context(prm)
+    bc = ((0, 1), [O.BC_REFLECT, O.BC_FLOW, 0, 0], None)
+    coarse = [((0, 0), (8, 16)), ((8, 0), (8, 16))]
+    _case(torch_dev, rec, prm, Qi, n, dx, coarse, [((0, 8), (8, 12)), ((12, 2), (10, 10)), ((26, 0), (6, 32))],
+          0.2 * dx[0], 2, 0, "2D four-equation Navier-Stokes, REFLECT / FLOW x sides, fine patches against both",
+          bc=bc, transport=NS_TR)
     del diff

@@ -17,8 +17,9 @@ GPU hierarchy is checked against.
 - Navier-Stokes form (four-equation model with transport, 6 ghost cells): a
   fully covered fine level is the single-level Navier-Stokes RK3 bit for bit;
   the composite sums stay constant to rounding, and a planted mistake --
-  refluxing the convective flux but not the diffusive one -- breaks that; the
-  diffusive flux is in the step (the result differs from the Euler form).
+  refluxing the convective flux but not the diffusive one -- breaks that (2D;
+  3D conservation too); the diffusive flux is in the step (the result differs
+  from the Euler form).
 """
 import numpy as np
 import pytest
@@ -263,3 +264,28 @@ def test_navier_stokes_composite_conservation(reflux_diffusive):
         assert drift < 1e-14
     else:
         assert drift > 1e-9
+
+
+def ns_state_3d(N):
+    """The 3D counterpart of ns_state: every primitive variable varies along all three axes."""
+    n, dx = (N, N, N), (1.0 / N,) * 3
+    X = W.cell_centres(n, (0.0, 0.0, 0.0), dx)
+    ph = lambda k: 2 * np.pi * ((k + 1) * X[0] + (k + 2) * X[1] + (3 - k % 2) * X[2]) + k  # noqa: E731
+    rho = 1.0 + 0.2 * np.sin(ph(0))
+    Y0 = 0.5 + 0.3 * np.sin(ph(1))
+    vel = [0.3 * np.sin(ph(2)), 0.2 * np.cos(ph(3)), 0.1 * np.sin(ph(4))]
+    p = 1.0 + 0.1 * np.cos(ph(5))
+    Q = W.conservative_from_primitive_four([rho * Y0, rho * (1.0 - Y0)], vel, p, NS_GAM, NS_R)
+    prm = O.Params(dim=3, model=O.FOUR_EQN_CONSERVATIVE, num_species=2, scheme=1, gamma=list(NS_GAM),
+                   R=list(NS_R))
+    return Q, n, dx, prm
+
+
+def test_navier_stokes_3d_composite_conservation():
+    Qi, n, dx, prm = ns_state_3d(8)
+    T = amr.TwoLevel(prm, n, dx, [((4, 4, 2), (6, 8, 10)), ((10, 4, 2), (4, 4, 10))], transport=NS_TR)
+    fine = T.refine_patches(Qi)
+    m0 = T.composite_sum(Qi, fine, prm.neq)
+    Qc, pts = T.advance(Qi, fine, 0.2 * dx[0])
+    drift = np.max(np.abs(T.composite_sum(Qc, pts, prm.neq) - m0) / np.maximum(np.abs(m0), 1e-3))
+    assert drift < 1e-14
